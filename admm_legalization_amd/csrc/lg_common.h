@@ -1,0 +1,52 @@
+// Shared definitions of the device legalizer (host and device side).
+#pragma once
+
+#include <cstdint>
+
+namespace lgdev {
+
+// Device-resident iteration control.  Written only by single-thread kernels
+// (k_scan, k_finish, setup kernels); every other kernel only reads it.
+struct Ctl {
+  int32_t iter;          // index of the iteration being executed
+  int32_t done;          // 1: converged or max_admm_iter reached -> kernels exit
+  int32_t converged;
+  int32_t conv_iter;     // iteration at which overflow reached the threshold
+  int32_t npad;          // pow2(max cells in a valid partition) for this iteration
+  int32_t max_cnt;
+  int32_t exceptional;   // next iteration runs the adaptive search
+  int32_t last_slow;
+  int32_t min_ovf;
+  int32_t error;         // nonzero: a device-side limit was exceeded (see kErr*)
+  int32_t adaptive_calls;
+  int32_t ovf_window_n;  // entries in the stagnation window
+  int32_t ovf_window[64];
+  float rho;
+  float lambda0;
+  float step;
+  float pad0;
+  unsigned long long ovf_acc;     // this iteration's overflow (sum over partitions)
+  unsigned long long last_ovf;
+  unsigned long long demand_sum;  // exact integer demand (rho init)
+  unsigned long long invaug_fx;   // fixed-point (2^-32) sum of the inverse augmented demand
+  unsigned long long disp_sum;
+  unsigned long long disp_max;
+  unsigned long long check_ovf;
+  unsigned long long row_vio;
+};
+
+enum : int32_t {
+  kErrNone = 0,
+  kErrPartitionTooDense = 1,  // more than kMaxNpad cells in one partition
+  kErrUsageRange = 2,         // a pixel usage above 65535 (16-bit LDS packing)
+};
+
+constexpr int kWave = 64;
+constexpr int kNumDirDefault = 245;
+constexpr int kNumDirPadded = 256;
+constexpr int kNumDirWide = 4093;
+constexpr int kNumDirWidePadded = 4096;
+constexpr int kMaxNpad = 4096;
+constexpr int kNpadSmall = 1024;
+
+}  // namespace lgdev

@@ -1,0 +1,444 @@
+// Device layer of the legalizer: C ABI lg_create / lg_solve / lg_postprocess /
+// lg_check / lg_copy_out / lg_destroy (include/skyline_lg.h).  Replaces the
+// reference's CudaDatabase + CudaAdmmLegalizer + CudaPostProcess
+// (database/CudaDatabase.h:16-49, admm_legalizer/CudaAdmmLegalizer.h:19-33,
+// post_process/CudaPostProcess.h:12-20).  The handle owns all device memory.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/skyline_lg.h"
+#include "lg_common.h"
+#include "lg_internal.h"
+
+// The kernels and this orchestration form one translation unit, so the
+// templated partition kernel and the __constant__ direction tables are local.
+#include "lg_kernels.hip"
+
+using namespace lgdev;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      return fail(LG_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+int dalloc(T** p, size_t n) {
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+  if (e != hipSuccess) return fail(LG_ERR_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
+  return LG_OK;
+}
+
+bool g_dirs_uploaded[64] = {false};
+
+int upload_directions() {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 64 && g_dirs_uploaded[dev]) return LG_OK;
+  std::vector<int2> dd, dw;
+  for (int y = -6; y <= 6; y++) {  // Legalizer::makeDirectionVectors (Legalizer.cpp:252-275)
+    static const int xr[7] = {40, 20, 10, 5, 2, 1, 0};
+    for (int x = -xr[std::abs(y)]; x <= xr[std::abs(y)]; x++) dd.push_back(make_int2(x, y));
+  }
+  for (int y = -12; y <= 12; y++) {  // (Legalizer.cpp:323-351)
+    static const int xr[13] = {280, 240, 200, 160, 120, 80, 40, 20, 10, 4, 2, 1, 0};
+    for (int x = -xr[std::abs(y)]; x <= xr[std::abs(y)]; x++) dw.push_back(make_int2(x, y));
+  }
+  if ((int)dd.size() != kNumDirDefault || (int)dw.size() != kNumDirWide)
+    return fail(LG_ERR_STATE, "direction table size");
+  HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_dir_default), dd.data(), dd.size() * sizeof(int2)));
+  HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_dir_wide), dw.data(), dw.size() * sizeof(int2)));
+  if (dev < 64) g_dirs_uploaded[dev] = true;
+  return LG_OK;
+}
+
+}  // namespace
+
+struct LgHandle {
+  LgParams p;
+  Args a;
+  hipStream_t own_stream = nullptr;
+  int solved = 0, posted = 0;
+  size_t P = 0;
+  // device buffers (freed in lg_destroy)
+  std::vector<void*> bufs;
+  Ctl host_ctl;
+  std::vector<int32_t> trace;
+  int trace_n = 0;
+  LgVioStat vio{};
+  int checked = 0;
+  float coeff_init_rho, coeff_init_dual;
+};
+
+namespace {
+
+template <typename T>
+int halloc(LgHandle* h, T** p, size_t n) {
+  int st = dalloc(p, n);
+  if (st == LG_OK) h->bufs.push_back((void*)*p);
+  return st;
+}
+
+template <typename T>
+int upload(LgHandle* h, T** dst, const T* src, size_t n) {
+  int st = halloc(h, dst, n);
+  if (st) return st;
+  if (n) HIP_TRY(hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+  return LG_OK;
+}
+
+hipStream_t pick(LgHandle* h, void* s) { return s ? (hipStream_t)s : h->own_stream; }
+
+size_t partition_lds_bytes(const LgHandle* h, int npad_hi) {
+  size_t pix = (size_t)h->a.x_hint * h->a.y_hint * 8;
+  size_t vdd = (size_t)((h->a.y_hint + 3) & ~3) * 4;
+  return pix + vdd + (size_t)npad_hi * 16 + (size_t)npad_hi * 12;
+}
+
+// Launch one partition-kernel pass (three LDS size classes; the device picks
+// the class from ctl->npad and the others exit at once).
+template <int MODE>
+int launch_partition(LgHandle* h, hipStream_t s) {
+  const int classes[3][2] = {{0, 256}, {257, 1024}, {1025, kMaxNpad}};
+  for (auto& c : classes) {
+    size_t lds = partition_lds_bytes(h, c[1]);
+    hipLaunchKernelGGL(k_partition<MODE>, dim3(h->a.K), dim3(kWave), lds, s, h->a, c[0], c[1]);
+  }
+  HIP_TRY(hipGetLastError());
+  return LG_OK;
+}
+
+int read_ctl(LgHandle* h, hipStream_t s) {
+  HIP_TRY(hipMemcpyAsync(&h->host_ctl, h->a.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return LG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* lg_last_error(void) { return g_err.c_str(); }
+
+int lg_create(const LgGridDesc* g, const LgCellsSoA* c, const LgParams* p, LgHandle** out) {
+  if (!g || !c || !p || !out) return fail(LG_ERR_ARG, "null argument");
+  if (g->x_size <= 0 || g->y_size <= 0 || c->num_cells < 0) return fail(LG_ERR_ARG, "bad sizes");
+  if (g->site_width <= 0 || g->row_height <= 0) return fail(LG_ERR_ARG, "bad site/row size");
+  if (p->x_hint < 3 || p->y_hint < 3 || p->iter_update_partition <= 0 || p->iter_update_dual_step <= 0)
+    return fail(LG_ERR_ARG, "bad partition / schedule parameters");
+  if (g->x_size > 65535 || g->y_size > 65535) return fail(LG_ERR_ARG, "grid larger than 65535 sites/rows");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(LG_ERR_DEVICE, "no HIP device");
+  int st = upload_directions();
+  if (st) return st;
+
+  LgHandle* h = new LgHandle();
+  h->p = *p;
+  h->coeff_init_rho = p->coeff_to_init_rho;
+  h->coeff_init_dual = p->coeff_to_init_dual;
+  Args& a = h->a;
+  std::memset(&a, 0, sizeof(a));
+  a.x_size = g->x_size; a.y_size = g->y_size;
+  a.site_width = g->site_width; a.row_height = g->row_height;
+  a.core_lx = g->core_lx; a.core_ly = g->core_ly;
+  a.x_hint = p->x_hint; a.y_hint = p->y_hint;
+  // CudaAdmmLegalizer ctor (CudaAdmmLegalizer.cu:86-89)
+  a.ncol = g->x_size % p->x_hint == 0 ? g->x_size / p->x_hint : g->x_size / p->x_hint + 1;
+  a.nrow = g->y_size % p->y_hint == 0 ? g->y_size / p->y_hint : g->y_size / p->y_hint + 1;
+  a.K = a.ncol * a.nrow;
+  a.N = c->num_cells;
+  const size_t P = (size_t)g->x_size * g->y_size;
+  h->P = P;
+  const int N = a.N;
+
+  auto bail = [&](int code) { lg_destroy(h); return code; };
+
+  // pixel code: valid ? group + 2 : 0
+  std::vector<int16_t> pcode(P);
+  for (size_t i = 0; i < P; i++) {
+    int grp = g->pixel_group[i];
+    if (g->pixel_valid[i] && (grp + 2 > 0x7fff || grp + 2 < 0)) {
+      lg_destroy(h);
+      return fail(LG_ERR_ARG, "fence group index out of range");
+    }
+    pcode[i] = g->pixel_valid[i] ? (int16_t)(grp + 2) : (int16_t)0;
+  }
+  std::vector<int32_t> lxg(N), lyg(N), code(N);
+  int max_h = 1;
+  for (int i = 0; i < N; i++) {
+    // dbu_to_grid_functor (DatabaseKernels.hpp:7-20)
+    lxg[i] = (c->lx_dbu[i] - g->core_lx) / g->site_width;
+    lyg[i] = (c->ly_dbu[i] - g->core_ly) / g->row_height;
+    code[i] = c->group[i] + 2;
+    int w = c->w_grid[i], hh = c->h_grid[i];
+    if (w <= 0 || hh <= 0 || w > 4095 || hh > 63 || code[i] <= 0 || code[i] > 8191) {
+      lg_destroy(h);
+      return fail(LG_ERR_ARG, "cell size or group outside the supported range");
+    }
+    if ((w > p->x_hint && a.ncol > 1) || (hh > p->y_hint && a.nrow > 1)) {
+      lg_destroy(h);
+      return fail(LG_ERR_ARG, "cell larger than a partition (x_hint/y_hint)");
+    }
+    if (lxg[i] < 0 || lyg[i] < 0 || lxg[i] + w > g->x_size || lyg[i] + hh > g->y_size) {
+      lg_destroy(h);
+      return fail(LG_ERR_ARG, "cell outside the grid (not preplaced?)");
+    }
+    max_h = std::max(max_h, hh);
+  }
+  std::vector<int32_t> khn(max_h, 0);
+  for (int i = 0; i < N; i++) khn[c->h_grid[i] - 1]++;
+  std::vector<int32_t> rprio(N, 0);
+  if (p->sorting_policy == LG_SORT_POLICY_RANDOM) {  // CudaAdmmLegalizer.cu:120-130
+    std::iota(rprio.begin(), rprio.end(), 0);
+    std::mt19937 gen(p->random_sorting_seed);
+    std::shuffle(rprio.begin(), rprio.end(), gen);
+  }
+
+  HIP_TRY(hipStreamCreateWithFlags(&h->own_stream, hipStreamNonBlocking));
+  int16_t* d_pcode;
+  int32_t *d_region, *d_vdd, *d_rx0, *d_ry0, *d_rx1, *d_ry1;
+  if ((st = upload(h, &d_pcode, pcode.data(), P))) return bail(st);
+  std::vector<int32_t> region_tmp;
+  const int32_t* reg = g->pixel_region;
+  if (!reg) { region_tmp.assign(P, -1); reg = region_tmp.data(); }
+  if ((st = upload(h, &d_region, reg, P))) return bail(st);
+  if ((st = upload(h, &d_vdd, g->row_vdd_up, (size_t)g->y_size))) return bail(st);
+  int nr = std::max(g->num_regions, 0);
+  if ((st = upload(h, &d_rx0, g->region_x_min, nr))) return bail(st);
+  if ((st = upload(h, &d_ry0, g->region_y_min, nr))) return bail(st);
+  if ((st = upload(h, &d_rx1, g->region_x_max, nr))) return bail(st);
+  if ((st = upload(h, &d_ry1, g->region_y_max, nr))) return bail(st);
+  a.pcode = d_pcode; a.pregion = d_region; a.vdd_up = d_vdd;
+  a.rxmin = d_rx0; a.rymin = d_ry0; a.rxmax = d_rx1; a.rymax = d_ry1;
+  if ((st = halloc(h, &a.usage, P))) return bail(st);
+  if ((st = halloc(h, &a.dual, P))) return bail(st);
+
+  int32_t *d_w, *d_h, *d_code, *d_gnd, *d_ox, *d_oy, *d_rp, *d_khn;
+  if ((st = upload(h, &a.lx, lxg.data(), N))) return bail(st);
+  if ((st = upload(h, &a.ly, lyg.data(), N))) return bail(st);
+  if ((st = upload(h, &d_w, c->w_grid, N))) return bail(st);
+  if ((st = upload(h, &d_h, c->h_grid, N))) return bail(st);
+  if ((st = upload(h, &d_code, code.data(), N))) return bail(st);
+  if ((st = upload(h, &d_gnd, c->ground_at_bottom, N))) return bail(st);
+  if ((st = upload(h, &d_ox, c->orig_lx_dbu, N))) return bail(st);
+  if ((st = upload(h, &d_oy, c->orig_ly_dbu, N))) return bail(st);
+  if ((st = upload(h, &d_rp, rprio.data(), N))) return bail(st);
+  if ((st = upload(h, &d_khn, khn.data(), khn.size()))) return bail(st);
+  a.w = d_w; a.h = d_h; a.code = d_code; a.gnd = d_gnd; a.ox = d_ox; a.oy = d_oy;
+  a.rand_prio = d_rp; a.k_height_num = d_khn;
+  if ((st = halloc(h, &a.need_rot, N))) return bail(st);
+  HIP_TRY(hipMemset(a.need_rot, 0, std::max(N, 1) * sizeof(int32_t)));
+  if ((st = halloc(h, &a.part_of, N))) return bail(st);
+  if ((st = halloc(h, &a.cnt, a.K + 1))) return bail(st);
+  if ((st = halloc(h, &a.off, a.K + 2))) return bail(st);
+  if ((st = halloc(h, &a.fill, a.K + 1))) return bail(st);
+  if ((st = halloc(h, &a.grouped, N))) return bail(st);
+  if ((st = halloc(h, &a.ovf_flag, N))) return bail(st);
+  if ((st = halloc(h, &a.ovf_list, N))) return bail(st);
+  if ((st = halloc(h, &a.trace, std::max(p->max_admm_iter, 1)))) return bail(st);
+  if ((st = halloc(h, &a.ctl, 1))) return bail(st);
+  HIP_TRY(hipMemset(a.cnt, 0, (a.K + 1) * sizeof(int32_t)));
+
+  a.max_iter = p->max_admm_iter;
+  a.terminate_thr = p->admm_terminate_thr;
+  a.iter_update_partition = p->iter_update_partition;
+  a.iter_threshold_dual_update = p->iter_threshold_dual_update;
+  a.iter_update_dual_step = p->iter_update_dual_step;
+  a.iter_perturbation_on = p->iter_perturbation_on;
+  a.max_disp_in_row_height = p->max_disp_in_row_height;
+  a.flag_iccad17 = p->flag_iccad17;
+  a.standard_admm = p->standard_admm;
+  a.max_disp_coeff = p->max_disp_coeff;
+  a.coeff_update_dual_step = p->coeff_to_update_dual_step;
+  a.sort_policy = p->sorting_policy;
+  a.sort_algo = p->sorting_algorithm;
+  a.use_adaptive = p->use_adaptive_search;
+  a.stagnation_window = std::min(p->stagnation_window, 64);
+  a.min_iter_adaptive = p->min_iter_to_call_adaptive_search;
+  a.cooltime_adaptive = p->cooltime_to_call_adaptive_search;
+  a.ovf_ratio_adaptive = p->ovf_ratio_to_call_adaptive_search;
+  a.coeff_neglect_disp = p->coeff_to_neglect_displace;
+  *out = h;
+  return LG_OK;
+}
+
+int lg_solve(LgHandle* h, void* stream) {
+  if (!h) return fail(LG_ERR_ARG, "null handle");
+  hipStream_t s = pick(h, stream);
+  Args& a = h->a;
+  const int N = a.N;
+  const int nb = (N + 255) / 256;
+  // CudaAdmmLegalizer::solve (CudaAdmmLegalizer.cu:250-281)
+  Ctl init;
+  std::memset(&init, 0, sizeof(init));
+  init.min_ovf = INT_MAX;
+  init.last_slow = -1;
+  init.npad = 1;
+  HIP_TRY(hipMemcpyAsync(a.ctl, &init, sizeof(Ctl), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemsetAsync(a.usage, 0, h->P * sizeof(int32_t), s));
+  if (N > 0) {
+    hipLaunchKernelGGL(k_usage_scatter, dim3(nb), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_demand, dim3(nb), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_rho, dim3(1), dim3(1), 0, s, a, h->coeff_init_rho);
+    hipLaunchKernelGGL(k_inv_aug, dim3(nb), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_lambda0, dim3(1), dim3(1), 0, s, a, h->coeff_init_dual);
+  }
+  hipLaunchKernelGGL(k_fill_dual, dim3(2048), dim3(256), 0, s, a);
+  HIP_TRY(hipGetLastError());
+  int st;
+  if (N > 0 && h->p.max_admm_iter > 0) {
+    int batch = 8;
+    for (;;) {
+      for (int b = 0; b < batch; b++) {
+        hipLaunchKernelGGL(k_assign, dim3(nb), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, a);
+        hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, s, a);
+        if (a.use_adaptive) {
+          hipLaunchKernelGGL(k_extract_congested, dim3(nb), dim3(256), 0, s, a);
+          hipLaunchKernelGGL(k_adaptive, dim3(1), dim3(1024), 0, s, a);
+        }
+        if ((st = launch_partition<MODE_PRIMAL>(h, s))) return st;
+        hipLaunchKernelGGL(k_finish, dim3(1), dim3(1), 0, s, a);
+      }
+      HIP_TRY(hipGetLastError());
+      if ((st = read_ctl(h, s))) return st;
+      if (h->host_ctl.error) break;
+      if (h->host_ctl.done) break;
+      batch = std::min(batch * 2, 64);
+    }
+    if (h->host_ctl.error == kErrPartitionTooDense)
+      return fail(LG_ERR_STATE, "more than 4096 cells in one partition");
+    if (h->host_ctl.error == kErrUsageRange)
+      return fail(LG_ERR_STATE, "pixel usage above 65535");
+    if (h->host_ctl.converged) {  // doFinalRefine (CudaAdmmLegalizer.cu:1011-1115)
+      if ((st = launch_partition<MODE_REFINE>(h, s))) return st;
+    }
+  } else {
+    HIP_TRY(hipMemsetAsync(a.ctl, 0, sizeof(int32_t) * 2, s));
+  }
+  if ((st = read_ctl(h, s))) return st;
+  h->trace_n = std::min(h->host_ctl.iter, h->p.max_admm_iter);
+  h->trace.resize(std::max(h->trace_n, 1));
+  if (h->trace_n > 0)
+    HIP_TRY(hipMemcpy(h->trace.data(), a.trace, h->trace_n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  h->solved = 1;
+  h->checked = 0;
+  return LG_OK;
+}
+
+int lg_postprocess(LgHandle* h, void* stream) {
+  if (!h) return fail(LG_ERR_ARG, "null handle");
+  if (!h->solved) return fail(LG_ERR_STATE, "lg_postprocess before lg_solve");
+  hipStream_t s = pick(h, stream);
+  const int nb = (h->a.N + 255) / 256;
+  if (h->a.N > 0) hipLaunchKernelGGL(k_rotate, dim3(nb), dim3(256), 0, s, h->a);
+  HIP_TRY(hipGetLastError());
+  if (h->p.qp_admm_on) {
+    int st = lg_qp_refine(h, s);
+    if (st) return st;
+  }
+  HIP_TRY(hipStreamSynchronize(s));
+  h->posted = 1;
+  h->checked = 0;
+  return LG_OK;
+}
+
+int lg_check(LgHandle* h, LgVioStat* out) {
+  if (!h) return fail(LG_ERR_ARG, "null handle");
+  hipStream_t s = h->own_stream;
+  Args& a = h->a;
+  const int nb = (a.N + 255) / 256;
+  // CudaDatabase::checkPlace (CudaDatabase.cu:326-353)
+  HIP_TRY(hipMemsetAsync(a.usage, 0, h->P * sizeof(int32_t), s));
+  HIP_TRY(hipMemsetAsync(&a.ctl->disp_sum, 0, 4 * sizeof(unsigned long long), s));
+  if (a.N > 0) hipLaunchKernelGGL(k_usage_scatter, dim3(nb), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_overflow, dim3(2048), dim3(256), 0, s, a);
+  if (a.N > 0) hipLaunchKernelGGL(k_cell_stats, dim3(nb), dim3(256), 0, s, a);
+  HIP_TRY(hipGetLastError());
+  int st = read_ctl(h, s);
+  if (st) return st;
+  h->vio = LgVioStat{(int32_t)h->host_ctl.check_ovf, (int32_t)h->host_ctl.row_vio, 0, 0, 0};
+  h->checked = 1;
+  if (out) *out = h->vio;
+  return LG_OK;
+}
+
+int lg_copy_out(LgHandle* h, int32_t* lx, int32_t* ly, int32_t* rot) {
+  if (!h) return fail(LG_ERR_ARG, "null handle");
+  HIP_TRY(hipStreamSynchronize(h->own_stream));
+  size_t n = (size_t)h->a.N * sizeof(int32_t);
+  if (n == 0) return LG_OK;
+  if (lx) HIP_TRY(hipMemcpy(lx, h->a.lx, n, hipMemcpyDeviceToHost));
+  if (ly) HIP_TRY(hipMemcpy(ly, h->a.ly, n, hipMemcpyDeviceToHost));
+  if (rot) HIP_TRY(hipMemcpy(rot, h->a.need_rot, n, hipMemcpyDeviceToHost));
+  return LG_OK;
+}
+
+int lg_get_stats(LgHandle* h, LgStats* out) {
+  if (!h || !out) return fail(LG_ERR_ARG, "null argument");
+  if (!h->checked) {
+    int st = lg_check(h, nullptr);
+    if (st) return st;
+  }
+  std::memset(out, 0, sizeof(*out));
+  const Ctl& c = h->host_ctl;
+  out->iterations = h->trace_n;
+  out->converged = c.converged;
+  out->final_overflow = (int64_t)c.last_ovf;
+  out->total_disp_dbu = (int64_t)c.disp_sum;
+  out->max_disp_dbu = (int64_t)c.disp_max;
+  out->rho = c.rho;
+  out->initial_dual = c.lambda0;
+  out->num_cells = h->a.N;
+  out->num_adaptive_calls = c.adaptive_calls;
+  out->vio = h->vio;
+  return LG_OK;
+}
+
+int lg_get_pixel_state(LgHandle* h, int32_t* usage, float* dual) {
+  if (!h) return fail(LG_ERR_ARG, "null handle");
+  HIP_TRY(hipStreamSynchronize(h->own_stream));
+  if (usage) HIP_TRY(hipMemcpy(usage, h->a.usage, h->P * sizeof(int32_t), hipMemcpyDeviceToHost));
+  if (dual) HIP_TRY(hipMemcpy(dual, h->a.dual, h->P * sizeof(float), hipMemcpyDeviceToHost));
+  return LG_OK;
+}
+
+int lg_get_ovf_trace(LgHandle* h, int32_t* trace, int32_t cap, int32_t* n) {
+  if (!h) return fail(LG_ERR_ARG, "null handle");
+  int m = std::min(cap, (int32_t)h->trace_n);
+  if (trace && m > 0) std::memcpy(trace, h->trace.data(), m * sizeof(int32_t));
+  if (n) *n = h->trace_n;
+  return LG_OK;
+}
+
+int lg_destroy(LgHandle* h) {
+  if (!h) return LG_OK;
+  for (void* b : h->bufs) (void)hipFree(b);
+  if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
+  delete h;
+  return LG_OK;
+}
+
+}  // extern "C"
+
+// QP refine (post-process), same translation unit
+#include "lg_qp.inc.hip"

@@ -1,0 +1,560 @@
+// Host side of the legalizer: the design-level entry lg_run_design (replaces
+// legalizer::Legalizer::run, Legalizer.cpp:197-228) and the C++ class
+// legalizer::Legalizer with the reference's setters.
+//
+// Import, pixel grid and preplace are host work in the reference too
+// (DatabaseIO.cpp, Grid.cpp, Preplace.cpp).  Here the grid is kept as flat
+// x-major arrays (no per-pixel objects) and preplace runs on all host cores.
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <climits>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <set>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include <omp.h>
+
+#include "../../include/skyline_lg.h"
+#include "../../include/skyline/Legalizer.hpp"
+
+namespace {
+
+thread_local std::string g_host_err;
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+constexpr int kIntMax = INT_MAX;
+constexpr int kGroupDefault = -1;   // Legalizer.h:33
+constexpr int kGroupBlockage = -2;  // Legalizer.h:34
+
+enum RegionKind : uint8_t { kDefaultRegion, kFenceRegion, kBlockageRegion };
+
+struct Rgn {
+  int lx, ly, ux, uy, group;
+  RegionKind kind;
+};
+
+// Pixel grid over the core, flat and x-major (Grid.h:51-105).
+class PixelGrid {
+ public:
+  int lx = 0, ly = 0, sw = 1, rh = 1, xs = 0, ys = 0;
+  std::vector<uint8_t> valid;
+  std::vector<int32_t> group, region;
+  std::vector<int32_t> hov, vov;  // fence overlap bookkeeping (only if fences exist)
+  std::vector<int32_t> vdd;
+
+  size_t at(int x, int y) const { return (size_t)x * ys + y; }
+  int start_x(int d) const { return (d - lx) / sw; }
+  int start_y(int d) const { return (d - ly) / rh; }
+  int end_x(int d) const {
+    int q = (d - lx) / sw, r = (d - lx) % sw;
+    return std::clamp(r == 0 ? q - 1 : q, 0, xs - 1);
+  }
+  int end_y(int d) const {
+    int q = (d - ly) / rh, r = (d - ly) % rh;
+    return std::clamp(r == 0 ? q - 1 : q, 0, ys - 1);
+  }
+  int dbu_x(int g) const { return g * sw + lx; }
+  int dbu_y(int g) const { return g * rh + ly; }
+  std::array<int, 4> expanded(int a, int b, int c, int d) const {
+    return {start_x(a), start_y(b), end_x(c), end_y(d)};
+  }
+  std::array<int, 4> shrunk(int a, int b, int c, int d) const {
+    return {end_x(a + sw), end_y(b + rh), start_x(c - sw), start_y(d - rh)};
+  }
+};
+
+struct MovCell {
+  int lx, ly, w, h, ilx, ily, group, gnd, inst;
+};
+
+struct Session {
+  LgParams p;
+  const LgDesign* d;
+  int core_lx, core_ly, core_ux, core_uy;
+  PixelGrid grid;
+  std::vector<MovCell> cells;
+  std::vector<Rgn> regions;
+  std::unordered_map<int, std::vector<int>> by_group;  // group -> region ids
+  int max_height = 0;
+  std::vector<std::array<int, 2>> dirs;  // default candidate moves
+  int preplace_fail = 0;
+};
+
+void add_region(Session& S, const Rgn& r) {
+  S.by_group[r.group].push_back((int)S.regions.size());
+  S.regions.push_back(r);
+}
+
+// importCells (DatabaseIO.cpp:79-169) + Grid ctor (Grid.cpp:24-67) +
+// region import (DatabaseIO.cpp:210-379) + region marking (Grid.cpp:184-314,
+// DatabaseIO.cpp:661-681) + adjustYHint (Legalizer.cpp:230-247)
+int import_design(Session& S) {
+  const LgDesign* d = S.d;
+  if (!d || d->num_rows <= 0) { g_host_err = "design has no rows"; return LG_ERR_ARG; }
+  S.core_lx = d->core_lx; S.core_ly = d->core_ly; S.core_ux = d->core_ux; S.core_uy = d->core_uy;
+  const int row_h = d->row_dy[d->num_rows - 1];
+  std::vector<std::array<int, 4>> blk_macro, blk_std;
+  std::map<int, int> hdist;
+  for (int i = 0; i < d->num_insts; i++) {
+    if (d->inst_dx[i] == 0 || d->inst_dy[i] < row_h) continue;
+    std::array<int, 4> r = {d->inst_lx[i], d->inst_ly[i], d->inst_lx[i] + d->inst_dx[i],
+                            d->inst_ly[i] + d->inst_dy[i]};
+    if (d->inst_macro[i]) {
+      if (!d->inst_fixed[i]) { g_host_err = "movable macros are not supported"; return LG_ERR_DESIGN; }
+      blk_macro.push_back(r);
+    } else if (d->inst_fixed[i]) {
+      blk_std.push_back(r);
+    } else {
+      MovCell c{r[0], r[1], d->inst_dx[i], d->inst_dy[i], r[0], r[1],
+                d->inst_group ? d->inst_group[i] : -1, d->inst_ground_at_bottom[i] ? 1 : 0, i};
+      S.cells.push_back(c);
+      hdist[c.h]++;
+    }
+  }
+  PixelGrid& g = S.grid;
+  g.lx = d->core_lx; g.ly = d->core_ly;
+  int min_rh = INT_MAX;
+  for (int r = 0; r < d->num_rows; r++) min_rh = std::min(min_rh, d->row_dy[r]);
+  g.rh = min_rh;
+  g.sw = d->row_site_width[d->num_rows - 1];
+  if (g.sw <= 0 || g.rh <= 0) { g_host_err = "bad site width / row height"; return LG_ERR_ARG; }
+  g.xs = (d->core_ux - d->core_lx) / g.sw;
+  g.ys = (d->core_uy - d->core_ly) / g.rh;
+  if (g.xs <= 0 || g.ys <= 0) { g_host_err = "empty core"; return LG_ERR_ARG; }
+  const size_t P = (size_t)g.xs * g.ys;
+  g.valid.assign(P, 0);
+  g.group.assign(P, kGroupDefault);
+  g.region.assign(P, -1);
+  g.vdd.assign(g.ys, 0);
+  for (int r = 0; r < d->num_rows; r++) {  // markDbRow (Grid.cpp:316-342)
+    auto [x1, y1, x2, y2] = g.shrunk(d->row_lx[r], d->row_ly[r], d->row_lx[r] + d->row_dx[r],
+                                     d->row_ly[r] + d->row_dy[r]);
+    if (y1 > y2 || x1 > x2) continue;
+    for (int x = x1; x <= x2; x++) std::fill_n(&g.valid[g.at(x, y1)], y2 - y1 + 1, (uint8_t)1);
+    int o = d->row_orient[r];
+    g.vdd[y1] = (o == LG_ORIENT_N || o == LG_ORIENT_FN) ? 1 : 0;
+  }
+  auto clip = [&](const std::array<int, 4>& r) {
+    return std::array<int, 4>{std::max(S.core_lx, r[0]), std::max(S.core_ly, r[1]),
+                              std::min(S.core_ux, r[2]), std::min(S.core_uy, r[3])};
+  };
+  for (auto& r : blk_macro) { auto c = clip(r); add_region(S, {c[0], c[1], c[2], c[3], kGroupBlockage, kBlockageRegion}); }
+  for (auto& r : blk_std) { auto c = clip(r); add_region(S, {c[0], c[1], c[2], c[3], kGroupBlockage, kBlockageRegion}); }
+  for (int b = 0; b < d->num_region_boxes; b++) {
+    auto c = clip({d->box_lx[b], d->box_ly[b], d->box_ux[b], d->box_uy[b]});
+    add_region(S, {c[0], c[1], c[2], c[3], d->box_group[b], kFenceRegion});
+  }
+  // default regions: sweep the fence/blockage rectangles bottom-up
+  // (importNonGroupRegions, DatabaseIO.cpp:272-379)
+  {
+    std::vector<int> obst;
+    for (int i = 0; i < (int)S.regions.size(); i++) obst.push_back(i);
+    struct Ev { int y, id; bool enter; };
+    std::vector<Ev> ev;
+    for (int k = 0; k < (int)obst.size(); k++) {
+      ev.push_back({S.regions[obst[k]].ly, k, true});
+      ev.push_back({S.regions[obst[k]].uy, k, false});
+    }
+    std::sort(ev.begin(), ev.end(), [](const Ev& a, const Ev& b) {
+      return a.y != b.y ? a.y < b.y : (!a.enter && b.enter);
+    });
+    std::set<int> live;
+    std::unordered_map<int, std::vector<int>> at_lx;
+    auto emit = [&](int lx, int ly, int ux, int uy) {
+      if (ux - lx < g.sw || uy - ly < g.rh) return;
+      for (int id : at_lx[lx]) {
+        Rgn& r = S.regions[id];
+        if (r.ux - r.lx == ux - lx && r.uy == ly) { r.uy = uy; return; }
+      }
+      at_lx[lx].push_back((int)S.regions.size());
+      add_region(S, {lx, ly, ux, uy, kGroupDefault, kDefaultRegion});
+    };
+    int y0 = S.core_ly;
+    for (const Ev& e : ev) {
+      std::vector<std::pair<int, int>> spans;
+      for (int k : live) spans.push_back({S.regions[obst[k]].lx, S.regions[obst[k]].ux});
+      std::sort(spans.begin(), spans.end());
+      int x = S.core_lx;
+      for (auto& [a, b] : spans) { emit(x, y0, a, e.y); x = b; }
+      emit(x, y0, S.core_ux, e.y);
+      if (e.enter) live.insert(e.id); else live.erase(e.id);
+      y0 = e.y;
+    }
+    emit(S.core_lx, y0, S.core_ux, S.core_uy);
+  }
+  bool any_fence = false;
+  for (auto& r : S.regions) any_fence |= (r.kind == kFenceRegion);
+  if (any_fence) { g.hov.assign(P, 0); g.vov.assign(P, 0); }
+  for (int id = 0; id < (int)S.regions.size(); id++) {
+    const Rgn& r = S.regions[id];
+    if (r.kind == kBlockageRegion) {  // Grid::markBlockageRegion (Grid.cpp:286-314)
+      auto [x1, y1, x2, y2] = g.expanded(r.lx, r.ly, r.ux, r.uy);
+      x1 = std::max(0, x1); y1 = std::max(0, y1); x2 = std::min(g.xs - 1, x2); y2 = std::min(g.ys - 1, y2);
+      for (int x = x1; x <= x2; x++)
+        for (int y = y1; y <= y2; y++) {
+          size_t p = g.at(x, y);
+          g.valid[p] = 0; g.group[p] = r.group; g.region[p] = id;
+        }
+    } else if (r.kind == kFenceRegion) {  // Grid::markGroupRegion (Grid.cpp:184-262)
+      auto [ex1, ey1, ex2, ey2] = g.expanded(r.lx, r.ly, r.ux, r.uy);
+      auto [sx1, sy1, sx2, sy2] = g.shrunk(r.lx, r.ly, r.ux, r.uy);
+      for (int x = ex1; x <= ex2; x++)
+        for (int y = ey1; y <= ey2; y++) {
+          size_t p = g.at(x, y);
+          g.group[p] = r.group; g.hov[p] += g.sw; g.vov[p] += g.rh; g.region[p] = id; g.valid[p] = 1;
+        }
+      if (ex1 == sx1 - 1)
+        for (int y = ey1; y <= ey2; y++) g.hov[g.at(ex1, y)] += (sx1 * g.sw - r.lx) - g.sw;
+      if (ex2 == sx2 + 1)
+        for (int y = ey1; y <= ey2; y++) g.hov[g.at(ex2, y)] += (r.ux - ex2 * g.sw) - g.sw;
+      if (ey1 == sy1 - 1)
+        for (int x = ex1; x <= ex2; x++) g.vov[g.at(x, ey1)] += (sy1 * g.rh - r.ly) - g.rh;
+      if (ey2 == sy2 + 1)
+        for (int x = ex1; x <= ex2; x++) g.vov[g.at(x, ey2)] += (r.uy - ey2 * g.rh) - g.rh;
+    } else {  // Grid::markNonGroupRegion (Grid.cpp:264-284)
+      auto [x1, y1, x2, y2] = g.shrunk(r.lx, r.ly, r.ux, r.uy);
+      x1 = std::clamp(x1, 0, g.xs - 1); y1 = std::clamp(y1, 0, g.ys - 1);
+      x2 = std::clamp(x2, 0, g.xs - 1); y2 = std::clamp(y2, 0, g.ys - 1);
+      for (int x = x1; x <= x2; x++)
+        for (int y = y1; y <= y2; y++) g.region[g.at(x, y)] = id;
+    }
+  }
+  if (any_fence)  // handleRegionSegments (DatabaseIO.cpp:661-681)
+    for (size_t p = 0; p < P; p++)
+      if ((g.hov[p] > 0 || g.vov[p] > 0) && (g.hov[p] < g.sw || g.vov[p] < g.rh)) g.valid[p] = 0;
+  for (auto& [hh, n] : hdist) S.max_height = std::max(S.max_height, hh);
+  int mh = S.max_height / g.rh;
+  if (S.p.y_hint < 3 * mh) S.p.y_hint = 3 * mh;
+  for (int y = -6; y <= 6; y++) {  // default directions (Legalizer.cpp:252-275)
+    static const int xr[7] = {40, 20, 10, 5, 2, 1, 0};
+    for (int x = -xr[std::abs(y)]; x <= xr[std::abs(y)]; x++) S.dirs.push_back({x, y});
+  }
+  return LG_OK;
+}
+
+// ---- preplace (Preplace.cpp) ----------------------------------------------
+
+// Grid::checkPixelStatus (Grid.cpp:365-383): true if every covered pixel is
+// valid and belongs to the cell's group.
+bool pixels_ok(const Session& S, const MovCell& c) {
+  const PixelGrid& g = S.grid;
+  auto [x1, y1, x2, y2] = g.expanded(c.lx, c.ly, c.lx + c.w, c.ly + c.h);
+  for (int x = x1; x <= x2; x++)
+    for (int y = y1; y <= y2; y++) {
+      size_t p = g.at(x, y);
+      if (!g.valid[p] || g.group[p] != c.group) return false;
+    }
+  return true;
+}
+
+struct Cand { int lx, ly, disp; };
+
+bool outside(const Rgn& r, const MovCell& c, int lx, int ly) {  // Preplace.cpp:244-265
+  return !(lx >= r.lx && ly >= r.ly && lx + c.w <= r.ux && ly + c.h <= r.uy);
+}
+
+// findProjectionPoint (Preplace.cpp:267-353)
+Cand project(const Session& S, const MovCell& c, const Rgn& r) {
+  const PixelGrid& g = S.grid;
+  const int wg = c.w / g.sw, hg = c.h / g.rh;
+  auto [gx1, gy1, gx2, gy2] = g.shrunk(r.lx, r.ly, r.ux, r.uy);
+  if (gy1 > gy2 || gx1 > gx2) return {-1, -1, kIntMax};
+  const int rlx = g.dbu_x(gx1), rly = g.dbu_y(gy1), rux = g.dbu_x(gx2), ruy = g.dbu_y(gy2);
+  int nx = c.lx, ny = c.ly;
+  if (c.lx + c.w > rux) nx = g.dbu_x(gx2 - wg + 1);
+  else if (c.lx < rlx) nx = g.dbu_x(gx1);
+  if (c.ly + c.h > ruy) ny = g.dbu_y(gy2 - hg + 1);
+  else if (c.ly < rly) ny = g.dbu_y(gy1);
+  const bool even = (hg % 2) == 0;
+  if (even && g.vdd[g.start_y(ny)] != c.gnd) {
+    int best = kIntMax, by = -1;
+    for (int yc : {ny - g.rh, ny + g.rh}) {
+      int dd = std::abs(nx - c.ilx) + std::abs(yc - c.ily);
+      if (!outside(r, c, nx, yc) && dd < best) { best = dd; by = yc; }
+    }
+    ny = by;
+  }
+  const bool out = outside(r, c, nx, ny);
+  bool bad = false;
+  if (even && !out) {
+    int x0 = g.start_x(nx), y0 = g.start_y(ny);
+    for (int x = x0; x < x0 + wg && !bad; x++)
+      for (int y = y0; y < y0 + hg; y++)
+        if (!g.valid[g.at(x, y)]) { bad = true; break; }
+  }
+  if (out || bad) return {-1, -1, kIntMax};
+  return {nx, ny, std::abs(c.ilx - nx) + std::abs(c.ily - ny)};
+}
+
+bool project_group(const Session& S, MovCell& c) {
+  auto it = S.by_group.find(c.group);
+  if (it == S.by_group.end() || it->second.empty()) return false;
+  Cand best = project(S, c, S.regions[it->second[0]]);
+  for (size_t k = 1; k < it->second.size(); k++) {
+    Cand q = project(S, c, S.regions[it->second[k]]);
+    if (q.disp < best.disp) best = q;
+  }
+  if (best.disp >= kIntMax) return false;
+  c.lx = best.lx; c.ly = best.ly;
+  return true;
+}
+
+// roundToNearsetPoint (Preplace.cpp:111-157)
+void snap(const Session& S, MovCell& c) {
+  const PixelGrid& g = S.grid;
+  const int wg = c.w / g.sw, hg = c.h / g.rh;
+  int gx = std::min(g.xs - wg, std::max(0, g.start_x(c.lx)));
+  int ax = g.dbu_x(gx), bx = g.dbu_x(std::min(g.xs - wg, gx + 1));
+  int nx = std::abs(c.lx - ax) < std::abs(c.lx - bx) ? ax : bx;
+  int gy = std::min(g.ys - hg, std::max(0, g.start_y(c.ly)));
+  int ay = g.dbu_y(gy), by = g.dbu_y(std::min(g.ys - hg, gy + 1));
+  if (ay == by) by = g.dbu_y(std::max(0, gy - 1));
+  int ny = std::abs(c.ly - ay) < std::abs(c.ly - by) ? ay : by;
+  if (hg % 2 == 0 && g.vdd[g.start_y(ny)] != c.gnd) {
+    if (ny == ay) ny = (by + c.h <= S.core_uy) ? by : ay - g.rh;
+    else if (ny == by) ny = (ay >= S.core_ly) ? ay : by + g.rh;
+  }
+  c.lx = nx; c.ly = ny;
+}
+
+// checkHopeless (Preplace.cpp:159-207) over the 245 default moves
+bool hopeless(const Session& S, const MovCell& c0) {
+  const PixelGrid& g = S.grid;
+  MovCell c = c0;
+  const int gx = g.start_x(c0.lx), gy = g.start_y(c0.ly);
+  for (auto& d : S.dirs) {
+    int nx = gx + d[0], ny = gy + d[1];
+    c.lx = g.dbu_x(nx); c.ly = g.dbu_y(ny);
+    if (c.lx + c.w > S.core_ux || c.ly + c.h > S.core_uy || c.lx < S.core_lx || c.ly < S.core_ly) continue;
+    if (g.vdd[ny] == c.gnd && pixels_ok(S, c)) return false;
+  }
+  return true;
+}
+
+// initialLegalize (Preplace.cpp:18-72)
+void preplace(Session& S) {
+  const int n = (int)S.cells.size();
+  const int nt = std::max(1, S.p.num_cpu_threads);
+#pragma omp parallel for num_threads(nt) schedule(dynamic, 1024)
+  for (int i = 0; i < n; i++) {
+    MovCell& c = S.cells[i];
+    snap(S, c);
+    if (!pixels_ok(S, c)) project_group(S, c);
+  }
+  int fails = 0;
+#pragma omp parallel for num_threads(nt) schedule(dynamic, 1024) reduction(+ : fails)
+  for (int i = 0; i < n; i++) {
+    MovCell& c = S.cells[i];
+    if (hopeless(S, c) && !project_group(S, c)) fails++;
+  }
+  S.preplace_fail = fails;
+}
+
+}  // namespace
+
+extern "C" {
+
+void lg_default_params(LgParams* p) {  // Legalizer::initParameters (Legalizer.cpp:47-93)
+  std::memset(p, 0, sizeof(*p));
+  p->flag_iccad17 = 0;
+  p->standard_admm = 0;
+  p->log_freq = 100;
+  p->num_cpu_threads = 16;
+  p->max_disp_in_row_height = 3;
+  p->max_admm_iter = 25000;
+  p->admm_terminate_thr = 0;
+  p->x_hint = 250;
+  p->y_hint = 12;
+  p->iter_update_partition = 1;
+  p->max_disp_coeff = 40.0f;
+  p->tech_penalty = 1.0f;
+  p->edge_spacing_penalty = 10000.0f;
+  p->iter_threshold_dual_update = 300;
+  p->iter_update_dual_step = 100;
+  p->iter_perturbation_on = 2;
+  p->coeff_to_init_rho = 3.0f;
+  p->coeff_to_init_dual = 0.5f;
+  p->coeff_to_update_dual_step = 0.2f;
+  p->qp_admm_on = 0;
+  p->qp_admm_dp_site_align = 1;
+  p->qp_admm_max_iter = 10000;
+  p->qp_admm_min_iter = 5;
+  p->qp_admm_rho = 2.0f;
+  p->use_adaptive_search = 0;
+  p->stagnation_window = 10;
+  p->stagnation_threshold = 0.01f;
+  p->min_iter_to_call_adaptive_search = 300;
+  p->cooltime_to_call_adaptive_search = 100;
+  p->ovf_ratio_to_call_adaptive_search = 1.05f;
+  p->coeff_to_neglect_displace = 20.0f;
+  p->random_sorting_seed = 0;
+  p->sorting_policy = LG_SORT_POLICY_OVERFLOW;
+  p->sorting_algorithm = LG_SORT_M_BITONIC;
+  p->verbose = 0;
+}
+
+// Import + preplace only: pixel maps (x-major) and preplaced movable-cell
+// positions (dbu, movable-cell order).  Exposed for parity tests.
+int lg_import_and_preplace(const LgDesign* d, const LgParams* p, int32_t* shape, int32_t* valid,
+                           int32_t* group, int32_t* region, int32_t* vdd_up, int32_t* region_bounds,
+                           int32_t* cell_lx, int32_t* cell_ly) {
+  if (!d || !p || !shape) { g_host_err = "null argument"; return LG_ERR_ARG; }
+  Session S;
+  S.p = *p;
+  S.d = d;
+  int st = import_design(S);
+  if (st) return st;
+  const PixelGrid& g = S.grid;
+  shape[0] = g.xs; shape[1] = g.ys;
+  shape[2] = (int)S.cells.size();
+  shape[3] = (int)S.regions.size();
+  if (!valid) return LG_OK;  // shape query
+  const size_t P = (size_t)g.xs * g.ys;
+  for (size_t i = 0; i < P; i++) { valid[i] = g.valid[i]; group[i] = g.group[i]; region[i] = g.region[i]; }
+  for (int y = 0; y < g.ys; y++) vdd_up[y] = g.vdd[y];
+  for (size_t k = 0; k < S.regions.size(); k++) {
+    const Rgn& rg = S.regions[k];
+    auto [a, b, c, e] = g.shrunk(rg.lx, rg.ly, rg.ux, rg.uy);
+    region_bounds[4 * k] = a; region_bounds[4 * k + 1] = b;
+    region_bounds[4 * k + 2] = c; region_bounds[4 * k + 3] = e;
+  }
+  preplace(S);
+  for (size_t i = 0; i < S.cells.size(); i++) { cell_lx[i] = S.cells[i].lx; cell_ly[i] = S.cells[i].ly; }
+  return LG_OK;
+}
+
+int lg_run_design(const LgDesign* d, const LgParams* p, LgResult* r) {
+  if (!d || !p || !r) { g_host_err = "null argument"; return LG_ERR_ARG; }
+  const double t0 = now_s();
+  Session S;
+  S.p = *p;
+  S.d = d;
+  int st = import_design(S);
+  if (st) return st;
+  const double t1 = now_s();
+  preplace(S);
+  const double t2 = now_s();
+  const PixelGrid& g = S.grid;
+  const int N = (int)S.cells.size();
+  std::vector<int32_t> valid(g.valid.begin(), g.valid.end());
+  std::vector<int32_t> rx0, ry0, rx1, ry1;
+  for (auto& rg : S.regions) {
+    auto [a, b, c, e] = g.shrunk(rg.lx, rg.ly, rg.ux, rg.uy);
+    rx0.push_back(a); ry0.push_back(b); rx1.push_back(c); ry1.push_back(e);
+  }
+  std::vector<int32_t> ox(N), oy(N), lx(N), ly(N), w(N), h(N), grp(N), gnd(N);
+  for (int i = 0; i < N; i++) {
+    const MovCell& c = S.cells[i];
+    ox[i] = c.ilx; oy[i] = c.ily; lx[i] = c.lx; ly[i] = c.ly;
+    w[i] = c.w / g.sw; h[i] = c.h / g.rh; grp[i] = c.group; gnd[i] = c.gnd;
+  }
+  LgGridDesc gd{g.xs, g.ys, g.sw, g.rh, g.lx, g.ly, valid.data(), g.group.data(), g.region.data(),
+                g.vdd.data(), (int)S.regions.size(), rx0.data(), ry0.data(), rx1.data(), ry1.data()};
+  LgCellsSoA cs{N, ox.data(), oy.data(), lx.data(), ly.data(), w.data(), h.data(), grp.data(), gnd.data()};
+  LgHandle* hd = nullptr;
+  if ((st = lg_create(&gd, &cs, &S.p, &hd))) return st;
+  auto done = [&](int code) { lg_destroy(hd); return code; };
+  if ((st = lg_solve(hd, nullptr))) return done(st);
+  const double t3 = now_s();
+  if ((st = lg_postprocess(hd, nullptr))) return done(st);
+  const double t4 = now_s();
+  std::vector<int32_t> gx(N), gy(N), rot(N);
+  if ((st = lg_copy_out(hd, gx.data(), gy.data(), rot.data()))) return done(st);
+  LgStats ds;
+  if ((st = lg_get_stats(hd, &ds))) return done(st);
+  int32_t ntr = 0;
+  if (r->ovf_trace && r->ovf_trace_cap > 0) lg_get_ovf_trace(hd, r->ovf_trace, r->ovf_trace_cap, &ntr);
+  lg_destroy(hd);
+  // copyToHostCells + exportDB + computeStat (CudaDatabase.cu:274-309,
+  // DatabaseIO.cpp:581-596, Legalizer.cpp:439-476)
+  for (int i = 0; i < d->num_insts; i++) { r->out_lx[i] = d->inst_lx[i]; r->out_ly[i] = d->inst_ly[i]; r->out_flipped[i] = 0; }
+  int64_t tot = 0, mx = 0;
+  for (int i = 0; i < N; i++) {
+    const MovCell& c = S.cells[i];
+    int nx = gx[i] * g.sw + g.lx, ny = gy[i] * g.rh + g.ly;
+    r->out_lx[c.inst] = nx;
+    r->out_ly[c.inst] = ny;
+    r->out_flipped[c.inst] = (uint8_t)rot[i];
+    int64_t dd = std::abs(c.ilx - nx) + std::abs(c.ily - ny);
+    tot += dd;
+    mx = std::max(mx, dd);
+  }
+  LgStats& s = r->stats;
+  s = ds;
+  s.total_disp_dbu = tot;
+  s.max_disp_dbu = mx;
+  const double dbu = d->dbu_per_micron > 0 ? d->dbu_per_micron : 1;
+  s.total_disp_um = double(tot) / dbu;
+  s.max_disp_um = double(mx) / dbu;
+  s.avg_disp_um = s.total_disp_um / double(std::max(d->num_insts, 1));
+  s.t_import_s = t1 - t0;
+  s.t_preplace_s = t2 - t1;
+  s.t_admm_s = t3 - t2;
+  s.t_post_s = t4 - t3;
+  s.t_total_s = now_s() - t0;
+  if (p->verbose) {  // Legalizer::printStats (Util.cpp:107-139)
+    printf("------------------------------------------\n");
+    printf("Legalization Time : %12.1f s\n", s.t_total_s);
+    printf("Sum Displacement  : %12.1f um\n", s.total_disp_um);
+    printf("Avg Displacement  : %12.1f um\n", s.avg_disp_um);
+    printf("Max Displacement  : %12.1f um\n", s.max_disp_um);
+    printf("NumOverlap        : %12d\n", s.vio.total_ovf);
+    printf("NumRowOrientVio   : %12d\n", s.vio.num_row_orient_vio);
+    printf("preplace           finished (takes %5.2f s)\n", s.t_preplace_s);
+    printf("admmLegalize       finished (takes %5.2f s)\n", s.t_admm_s);
+    printf("postProcess        finished (takes %5.2f s)\n", s.t_post_s);
+    printf("------------------------------------------\n");
+  }
+  return LG_OK;
+}
+
+}  // extern "C"
+
+// ---- the reference's public C++ class (include/legalizer/Legalizer.h:66-97)
+namespace legalizer {
+
+Legalizer::Legalizer(const LgDesign* design) : design_(design) { lg_default_params(&params_); }
+
+void Legalizer::setTechPenalty(float v) { params_.tech_penalty = v; }
+void Legalizer::setStandardAdmm(bool v) { params_.standard_admm = v ? 1 : 0; }
+void Legalizer::setQpRefine(bool v) { params_.qp_admm_on = v ? 1 : 0; }
+void Legalizer::setCoeffToInitRho(float v) { params_.coeff_to_init_rho = v; }
+void Legalizer::setCoeffToInitDual(float v) { params_.coeff_to_init_dual = v; }
+void Legalizer::setMaxAdmmIter(int v) { params_.max_admm_iter = v; }
+void Legalizer::setFlagICCAD2017(bool v) { params_.flag_iccad17 = v ? 1 : 0; }
+void Legalizer::setAdaptiveSearch(bool v) { params_.use_adaptive_search = v ? 1 : 0; }
+void Legalizer::setMinIterToCallAdaptiveSearch(int v) { params_.min_iter_to_call_adaptive_search = v; }
+void Legalizer::setOvfRatioToCallAdaptiveSearch(float v) { params_.ovf_ratio_to_call_adaptive_search = v; }
+void Legalizer::setCoeffToNeglectDisplace(float v) { params_.coeff_to_neglect_displace = v; }
+void Legalizer::setRandomSorting(bool v) {
+  params_.sorting_policy = v ? LG_SORT_POLICY_RANDOM : LG_SORT_POLICY_OVERFLOW;
+}
+void Legalizer::setRandomSortingSeed(int v) { params_.random_sorting_seed = v; }
+
+// Legalizer::setSortingPolicy / setSortingAlgorithm (Legalizer.cpp:161-189):
+// unknown names fall back to the defaults
+void Legalizer::setSortingPolicy(std::string_view s) {
+  params_.sorting_policy = s == "RANDOM" ? LG_SORT_POLICY_RANDOM
+                           : s == "SIZE" ? LG_SORT_POLICY_SIZE
+                                         : LG_SORT_POLICY_OVERFLOW;
+}
+void Legalizer::setSortingAlgorithm(std::string_view s) {
+  params_.sorting_algorithm = s == "S_BITONIC" ? LG_SORT_S_BITONIC
+                              : s == "S_MERGE" ? LG_SORT_S_MERGE
+                              : s == "M_MERGE" ? LG_SORT_M_MERGE
+                              : s == "CUB"     ? LG_SORT_CUB
+                                               : LG_SORT_M_BITONIC;
+}
+
+int Legalizer::run(LgResult* result) { return lg_run_design(design_, &params_, result); }
+
+}  // namespace legalizer

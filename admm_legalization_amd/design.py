@@ -1,0 +1,272 @@
+"""In-memory design (the legalizer's view of a placed netlist) and its C ABI.
+
+`Design` holds the subset of the reference's ``db::dbDatabase`` that
+``legalizer::Legalizer`` reads (reference: src/skyline/legalizer/src/
+DatabaseIO.cpp:31-379, Grid.cpp:24-67): the core box, the placement rows, the
+instances with their global-placement positions and the fence groups.  All
+coordinates are integer database units (dbu).
+
+The ctypes mirrors of ``include/skyline_lg.h`` live here as well so that every
+caller of the C ABI marshals structures the same way.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+ORIENT_N, ORIENT_S, ORIENT_W, ORIENT_E, ORIENT_FN, ORIENT_FS, ORIENT_FW, ORIENT_FE = range(8)
+
+SORT_POLICY = {"RANDOM": 0, "SIZE": 1, "OVERFLOW": 2}
+SORT_ALGO = {"S_MERGE": 0, "M_MERGE": 1, "S_BITONIC": 2, "M_BITONIC": 3, "CUB": 4}
+
+
+@dataclass
+class Design:
+    """Rows, instances and fences of one design, all int32 dbu arrays."""
+
+    core: tuple  # (lx, ly, ux, uy)
+    row_lx: np.ndarray
+    row_ly: np.ndarray
+    row_dx: np.ndarray
+    row_dy: np.ndarray
+    row_site_width: np.ndarray
+    row_orient: np.ndarray
+    inst_lx: np.ndarray
+    inst_ly: np.ndarray
+    inst_dx: np.ndarray
+    inst_dy: np.ndarray
+    inst_fixed: np.ndarray
+    inst_macro: np.ndarray
+    inst_group: np.ndarray
+    inst_ground_at_bottom: np.ndarray
+    num_groups: int = 0
+    box_group: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    box_lx: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    box_ly: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    box_ux: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    box_uy: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    dbu_per_micron: int = 1000
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def num_insts(self) -> int:
+        return int(self.inst_lx.shape[0])
+
+    @property
+    def site_width(self) -> int:
+        return int(self.row_site_width[-1])
+
+    @property
+    def row_height(self) -> int:
+        return int(self.row_dy.min())
+
+    def movable_mask(self) -> np.ndarray:
+        """Instances the legalizer moves (DatabaseIO.cpp:100-149): non-macro,
+        non-fixed, nonzero width and at least one row tall."""
+        ok = (self.inst_dx != 0) & (self.inst_dy >= int(self.row_dy[-1]))
+        return ok & (self.inst_macro == 0) & (self.inst_fixed == 0)
+
+    def copy_positions(self, lx, ly) -> "Design":
+        d = Design(**{k: getattr(self, k) for k in self.__dataclass_fields__})
+        d.inst_lx = np.ascontiguousarray(lx, dtype=np.int32)
+        d.inst_ly = np.ascontiguousarray(ly, dtype=np.int32)
+        return d
+
+    # -- persistence (golden fixtures) ------------------------------------
+    def to_npz(self, path: str) -> None:
+        arrays = {k: np.asarray(getattr(self, k)) for k in self.__dataclass_fields__
+                  if isinstance(getattr(self, k), np.ndarray)}
+        np.savez_compressed(path, core=np.asarray(self.core, np.int32),
+                            num_groups=np.int32(self.num_groups),
+                            dbu_per_micron=np.int32(self.dbu_per_micron), **arrays)
+
+    @staticmethod
+    def from_npz(path: str) -> "Design":
+        z = np.load(path, allow_pickle=False)
+        kw = {k: z[k] for k in z.files if k not in ("core", "num_groups", "dbu_per_micron")}
+        return Design(core=tuple(int(v) for v in z["core"]), num_groups=int(z["num_groups"]),
+                      dbu_per_micron=int(z["dbu_per_micron"]), **kw)
+
+
+# ---------------------------------------------------------------------------
+# ctypes mirrors of include/skyline_lg.h
+# ---------------------------------------------------------------------------
+_i32p = C.POINTER(C.c_int32)
+_u8p = C.POINTER(C.c_uint8)
+
+
+class LgParams(C.Structure):
+    _fields_ = [
+        ("flag_iccad17", C.c_int32), ("standard_admm", C.c_int32), ("log_freq", C.c_int32),
+        ("num_cpu_threads", C.c_int32), ("max_disp_in_row_height", C.c_int32),
+        ("max_admm_iter", C.c_int32), ("admm_terminate_thr", C.c_int32),
+        ("x_hint", C.c_int32), ("y_hint", C.c_int32), ("iter_update_partition", C.c_int32),
+        ("max_disp_coeff", C.c_float), ("tech_penalty", C.c_float),
+        ("edge_spacing_penalty", C.c_float),
+        ("iter_threshold_dual_update", C.c_int32), ("iter_update_dual_step", C.c_int32),
+        ("iter_perturbation_on", C.c_int32), ("coeff_to_init_rho", C.c_float),
+        ("coeff_to_init_dual", C.c_float), ("coeff_to_update_dual_step", C.c_float),
+        ("qp_admm_on", C.c_int32), ("qp_admm_dp_site_align", C.c_int32),
+        ("qp_admm_max_iter", C.c_int32), ("qp_admm_min_iter", C.c_int32),
+        ("qp_admm_rho", C.c_float),
+        ("use_adaptive_search", C.c_int32), ("stagnation_window", C.c_int32),
+        ("min_iter_to_call_adaptive_search", C.c_int32),
+        ("cooltime_to_call_adaptive_search", C.c_int32), ("stagnation_threshold", C.c_float),
+        ("ovf_ratio_to_call_adaptive_search", C.c_float),
+        ("coeff_to_neglect_displace", C.c_float),
+        ("random_sorting_seed", C.c_int32), ("sorting_policy", C.c_int32),
+        ("sorting_algorithm", C.c_int32), ("verbose", C.c_int32),
+    ]
+
+
+def default_params() -> LgParams:
+    """Defaults of Legalizer::initParameters (Legalizer.cpp:47-93)."""
+    p = LgParams()
+    p.flag_iccad17 = 0
+    p.standard_admm = 0
+    p.log_freq = 100
+    p.num_cpu_threads = 16
+    p.max_disp_in_row_height = 3
+    p.max_admm_iter = 25000
+    p.admm_terminate_thr = 0
+    p.x_hint = 250
+    p.y_hint = 12
+    p.iter_update_partition = 1
+    p.max_disp_coeff = 40.0
+    p.tech_penalty = 1.0
+    p.edge_spacing_penalty = 10000.0
+    p.iter_threshold_dual_update = 300
+    p.iter_update_dual_step = 100
+    p.iter_perturbation_on = 2
+    p.coeff_to_init_rho = 3.0
+    p.coeff_to_init_dual = 0.5
+    p.coeff_to_update_dual_step = 0.2
+    p.qp_admm_on = 0
+    p.qp_admm_dp_site_align = 1
+    p.qp_admm_max_iter = 10000
+    p.qp_admm_min_iter = 5
+    p.qp_admm_rho = 2.0
+    p.use_adaptive_search = 0
+    p.stagnation_window = 10
+    p.stagnation_threshold = 0.01
+    p.min_iter_to_call_adaptive_search = 300
+    p.cooltime_to_call_adaptive_search = 100
+    p.ovf_ratio_to_call_adaptive_search = 1.05
+    p.coeff_to_neglect_displace = 20.0
+    p.random_sorting_seed = 0
+    p.sorting_policy = SORT_POLICY["OVERFLOW"]
+    p.sorting_algorithm = SORT_ALGO["M_BITONIC"]
+    p.verbose = 0
+    return p
+
+
+class LgDesign(C.Structure):
+    _fields_ = [
+        ("dbu_per_micron", C.c_int32),
+        ("core_lx", C.c_int32), ("core_ly", C.c_int32), ("core_ux", C.c_int32), ("core_uy", C.c_int32),
+        ("num_rows", C.c_int32),
+        ("row_lx", _i32p), ("row_ly", _i32p), ("row_dx", _i32p), ("row_dy", _i32p),
+        ("row_site_width", _i32p), ("row_orient", _i32p),
+        ("num_insts", C.c_int32),
+        ("inst_lx", _i32p), ("inst_ly", _i32p), ("inst_dx", _i32p), ("inst_dy", _i32p),
+        ("inst_fixed", _u8p), ("inst_macro", _u8p), ("inst_group", _i32p),
+        ("inst_ground_at_bottom", _u8p),
+        ("num_groups", C.c_int32), ("num_region_boxes", C.c_int32),
+        ("box_group", _i32p), ("box_lx", _i32p), ("box_ly", _i32p), ("box_ux", _i32p),
+        ("box_uy", _i32p),
+    ]
+
+
+class LgVioStat(C.Structure):
+    _fields_ = [("total_ovf", C.c_int32), ("num_row_orient_vio", C.c_int32),
+                ("num_pin_short_vio", C.c_int32), ("num_pin_access_vio", C.c_int32),
+                ("num_edge_spacing_vio", C.c_int32)]
+
+
+class LgStats(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int32), ("converged", C.c_int32),
+        ("final_overflow", C.c_int64), ("total_disp_dbu", C.c_int64), ("max_disp_dbu", C.c_int64),
+        ("total_disp_um", C.c_double), ("avg_disp_um", C.c_double), ("max_disp_um", C.c_double),
+        ("rho", C.c_float), ("initial_dual", C.c_float), ("num_cells", C.c_int32),
+        ("num_adaptive_calls", C.c_int32), ("vio", LgVioStat),
+        ("t_import_s", C.c_double), ("t_preplace_s", C.c_double), ("t_admm_s", C.c_double),
+        ("t_post_s", C.c_double), ("t_total_s", C.c_double),
+    ]
+
+
+class LgResult(C.Structure):
+    _fields_ = [("out_lx", _i32p), ("out_ly", _i32p), ("out_flipped", _u8p),
+                ("ovf_trace", _i32p), ("ovf_trace_cap", C.c_int32), ("stats", LgStats)]
+
+
+class LgGridDesc(C.Structure):
+    _fields_ = [("x_size", C.c_int32), ("y_size", C.c_int32), ("site_width", C.c_int32),
+                ("row_height", C.c_int32), ("core_lx", C.c_int32), ("core_ly", C.c_int32),
+                ("pixel_valid", _i32p), ("pixel_group", _i32p), ("pixel_region", _i32p),
+                ("row_vdd_up", _i32p), ("num_regions", C.c_int32), ("region_x_min", _i32p),
+                ("region_y_min", _i32p), ("region_x_max", _i32p), ("region_y_max", _i32p)]
+
+
+class LgCellsSoA(C.Structure):
+    _fields_ = [("num_cells", C.c_int32), ("orig_lx_dbu", _i32p), ("orig_ly_dbu", _i32p),
+                ("lx_dbu", _i32p), ("ly_dbu", _i32p), ("w_grid", _i32p), ("h_grid", _i32p),
+                ("group", _i32p), ("ground_at_bottom", _i32p)]
+
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+class MarshalledDesign:
+    """Keeps contiguous copies alive while an LgDesign points into them."""
+
+    def __init__(self, d: Design):
+        conv = lambda a, t: np.ascontiguousarray(a, dtype=t)
+        self.keep = {}
+        s = LgDesign()
+        s.dbu_per_micron = int(d.dbu_per_micron)
+        s.core_lx, s.core_ly, s.core_ux, s.core_uy = (int(v) for v in d.core)
+        s.num_rows = int(d.row_lx.shape[0])
+        for name in ("row_lx", "row_ly", "row_dx", "row_dy", "row_site_width", "row_orient",
+                     "inst_lx", "inst_ly", "inst_dx", "inst_dy", "inst_group",
+                     "box_group", "box_lx", "box_ly", "box_ux", "box_uy"):
+            a = conv(getattr(d, name), np.int32)
+            self.keep[name] = a
+            setattr(s, name, _ptr(a, C.c_int32))
+        for name in ("inst_fixed", "inst_macro", "inst_ground_at_bottom"):
+            a = conv(getattr(d, name), np.uint8)
+            self.keep[name] = a
+            setattr(s, name, _ptr(a, C.c_uint8))
+        s.num_insts = d.num_insts
+        s.num_groups = int(d.num_groups)
+        s.num_region_boxes = int(np.asarray(d.box_group).shape[0])
+        self.struct = s
+
+
+class ResultBuffers:
+    """Caller-owned output arrays of lg_run_design."""
+
+    def __init__(self, num_insts: int, trace_cap: int = 0):
+        self.lx = np.zeros(num_insts, np.int32)
+        self.ly = np.zeros(num_insts, np.int32)
+        self.flipped = np.zeros(num_insts, np.uint8)
+        self.trace = np.zeros(max(trace_cap, 1), np.int32)
+        r = LgResult()
+        r.out_lx = _ptr(self.lx, C.c_int32)
+        r.out_ly = _ptr(self.ly, C.c_int32)
+        r.out_flipped = _ptr(self.flipped, C.c_uint8)
+        r.ovf_trace = _ptr(self.trace, C.c_int32)
+        r.ovf_trace_cap = int(trace_cap)
+        self.struct = r
+
+    def stats_dict(self) -> dict:
+        st = self.struct.stats
+        out = {k: getattr(st, k) for k, _ in LgStats._fields_ if k != "vio"}
+        out["vio"] = {k: getattr(st.vio, k) for k, _ in LgVioStat._fields_}
+        n = min(int(st.iterations), int(self.struct.ovf_trace_cap))
+        out["ovf_trace"] = self.trace[:n].copy()
+        return out

@@ -1,0 +1,236 @@
+"""Seeded synthetic placements for the legalization workloads.
+
+There is no benchmark data on the box, so every workload is generated here:
+site width ``sw`` and row height ``rh`` in dbu (default 100 and 800: a row is
+8 sites tall, and the QP refine's 1-dbu termination tolerance is a hundredth
+of a site as on real designs), rows alternating N / FS (so the
+VDD rail alternates, reference Grid.cpp:337-341), fixed rectangular macros,
+optional rectangular fence regions, and movable cells whose global-placement
+positions are a mixture of a uniform spread and Gaussian hotspots.  Each cell
+carries a ``ground_at_bottom`` bit (drawn 50/50) that decides which rows an
+even-height cell may sit on.
+
+Named workloads (sizes as stated in the bench/test configuration):
+
+* ``plain_20k``   - 20k single-row cells, widths U{1..8} sites, 250 rows,
+  75 % utilisation (row length derived from that: 490 sites), 10 macros
+  20-60 sites x 3-10 rows, 4 Gaussian hotspots (sigma = 6 rows) at ~1.4x
+  local density.  20k cells of mean width 4.5 on 250 x 2000 sites would be
+  18 % utilisation, so the row is shortened to keep 75 %.
+* ``mixed_1m``    - 1M cells, heights {1:86 %, 2:10 %, 3:3 %, 4:1 %},
+  1600 rows x 12000 sites, 85 % utilisation (width range calibrated to it),
+  3 fences each holding 5 % of the cells at the same utilisation, 200 macros.
+* ``mixed(n)``    - the same recipe scaled to ``n`` cells (tests, requests).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .design import ORIENT_FS, ORIENT_N, Design
+
+
+def _rows(x_size: int, y_size: int, sw: int, rh: int):
+    ys = np.arange(y_size, dtype=np.int32)
+    return dict(
+        row_lx=np.zeros(y_size, np.int32),
+        row_ly=(ys * rh).astype(np.int32),
+        row_dx=np.full(y_size, x_size * sw, np.int32),
+        row_dy=np.full(y_size, rh, np.int32),
+        row_site_width=np.full(y_size, sw, np.int32),
+        row_orient=np.where(ys % 2 == 0, ORIENT_N, ORIENT_FS).astype(np.int32),
+    )
+
+
+def _place_rects(rng, x_size, y_size, n, wr, hr, occupied, margin=2, tries=200000):
+    """Non-overlapping grid-aligned rectangles (in sites/rows) avoiding `occupied`."""
+    rects = []
+    occ = occupied
+    t = 0
+    while len(rects) < n and t < tries:
+        t += 1
+        w = int(rng.integers(wr[0], wr[1] + 1))
+        h = int(rng.integers(hr[0], hr[1] + 1))
+        if w >= x_size or h >= y_size:
+            continue
+        x = int(rng.integers(0, x_size - w + 1))
+        y = int(rng.integers(0, y_size - h + 1))
+        x0, y0 = max(0, x - margin), max(0, y - margin)
+        if occ[x0:x + w + margin, y0:y + h + margin].any():
+            continue
+        occ[x:x + w, y:y + h] = True
+        rects.append((x, y, w, h))
+    return rects
+
+
+def make_design(num_cells: int, x_size: int, y_size: int, *, sw: int = 100, rh: int = 800,
+                height_mix=None, width_range=(1, 8), num_macros: int = 0,
+                macro_w=(20, 60), macro_h=(3, 10), fence_frac: float = 0.0,
+                num_fences: int = 0, fence_util: float = 0.85, hotspots: int = 0,
+                hotspot_sigma_rows: float = 6.0, hotspot_frac: float = 0.0,
+                seed: int = 0, name: str = "custom") -> Design:
+    rng = np.random.default_rng(seed)
+    height_mix = height_mix or {1: 1.0}
+    hs = np.array(sorted(height_mix), np.int32)
+    hp = np.array([height_mix[h] for h in hs], np.float64)
+    hp /= hp.sum()
+
+    heights = rng.choice(hs, size=num_cells, p=hp).astype(np.int32)
+    widths = rng.integers(width_range[0], width_range[1] + 1, size=num_cells).astype(np.int32)
+    gnd = rng.integers(0, 2, size=num_cells).astype(np.uint8)
+    group = np.full(num_cells, -1, np.int32)
+
+    occupied = np.zeros((x_size, y_size), bool)
+
+    # fences first (big rectangles), sized for their share of cells
+    fences = []
+    if num_fences > 0 and fence_frac > 0:
+        per = int(round(num_cells * fence_frac))
+        idx = rng.permutation(num_cells)
+        for f in range(num_fences):
+            members = idx[f * per:(f + 1) * per]
+            area_sites = float((widths[members] * heights[members]).sum())
+            # same sites:rows aspect as the mixed-height die (12000:1600)
+            side = math.sqrt(area_sites / fence_util / 7.5)
+            fw = max(8, int(math.ceil(side * 7.5)))
+            fh = max(2, int(math.ceil(area_sites / fence_util / fw)))
+            r = _place_rects(rng, x_size, y_size, 1, (fw, fw), (fh, fh), occupied, margin=4)
+            if not r:
+                raise RuntimeError("cannot place fence")
+            fences.append(r[0])
+            group[members] = f
+    macros = _place_rects(rng, x_size, y_size, num_macros, macro_w, macro_h, occupied, margin=1)
+
+    # -- positions: lower-left corners in dbu ------------------------------
+    W, H = x_size * sw, y_size * rh
+    blocked = np.zeros((x_size, y_size), bool)
+    for (x, y, w, h) in fences + macros:
+        blocked[x:x + w, y:y + h] = True
+
+    def_idx = np.nonzero(group < 0)[0]
+    n_def = def_idx.shape[0]
+    centers = [(rng.uniform(0.15, 0.85) * W, rng.uniform(0.15, 0.85) * H) for _ in range(hotspots)]
+    sig = hotspot_sigma_rows * rh
+    is_hot = rng.random(n_def) < (hotspot_frac if hotspots else 0.0)
+    # prefix sums of the blocked map: reject footprints that touch a fence or macro
+    cs = np.zeros((x_size + 1, y_size + 1), np.int64)
+    cs[1:, 1:] = np.cumsum(np.cumsum(blocked, 0), 1)
+    wd, hd = widths[def_idx], heights[def_idx]
+    px = np.empty(n_def, np.int64)
+    py = np.empty(n_def, np.int64)
+    todo = np.arange(n_def)
+    while todo.size:
+        hot = is_hot[todo]
+        w, h = wd[todo], hd[todo]
+        ux = rng.uniform(0, 1, size=todo.size) * (W - w * sw)
+        uy = rng.uniform(0, 1, size=todo.size) * (H - h * rh)
+        if hotspots:
+            k = rng.integers(0, hotspots, size=todo.size)
+            ccx = np.array([c[0] for c in centers])[k] - w * sw / 2.0
+            ccy = np.array([c[1] for c in centers])[k] - h * rh / 2.0
+            ux = np.where(hot, rng.normal(ccx, sig), ux)
+            uy = np.where(hot, rng.normal(ccy, sig), uy)
+        lxs = np.clip(np.rint(ux), 0, W - w * sw).astype(np.int64)
+        lys = np.clip(np.rint(uy), 0, H - h * rh).astype(np.int64)
+        x0, y0 = lxs // sw, lys // rh
+        x1 = np.minimum(x_size, -(-(lxs + w * sw) // sw))
+        y1 = np.minimum(y_size, -(-(lys + h * rh) // rh))
+        hit = cs[x1, y1] - cs[x0, y1] - cs[x1, y0] + cs[x0, y0]
+        ok = hit == 0
+        px[todo[ok]] = lxs[ok]
+        py[todo[ok]] = lys[ok]
+        todo = todo[~ok]
+    lx = np.zeros(num_cells, np.int64)
+    ly = np.zeros(num_cells, np.int64)
+    lx[def_idx] = px
+    ly[def_idx] = py
+    for f, (x, y, w, h) in enumerate(fences):
+        m = np.nonzero(group == f)[0]
+        lx[m] = np.rint(rng.uniform(0, 1, size=m.size) * ((x + w) * sw - widths[m] * sw - x * sw) + x * sw)
+        ly[m] = np.rint(rng.uniform(0, 1, size=m.size) * ((y + h) * rh - heights[m] * rh - y * rh) + y * rh)
+    lx = lx.astype(np.int32)
+    ly = ly.astype(np.int32)
+
+    # -- instances: movable cells first, then fixed macros ------------------
+    nm = len(macros)
+    inst_lx = np.concatenate([lx, np.array([m[0] * sw for m in macros], np.int32)]).astype(np.int32)
+    inst_ly = np.concatenate([ly, np.array([m[1] * rh for m in macros], np.int32)]).astype(np.int32)
+    inst_dx = np.concatenate([widths * sw, np.array([m[2] * sw for m in macros], np.int32)]).astype(np.int32)
+    inst_dy = np.concatenate([heights * rh, np.array([m[3] * rh for m in macros], np.int32)]).astype(np.int32)
+    inst_fixed = np.concatenate([np.zeros(num_cells, np.uint8), np.ones(nm, np.uint8)])
+    inst_macro = np.concatenate([np.zeros(num_cells, np.uint8), np.ones(nm, np.uint8)])
+    inst_group = np.concatenate([group, np.full(nm, -1, np.int32)]).astype(np.int32)
+    inst_gnd = np.concatenate([gnd, np.ones(nm, np.uint8)])
+
+    box = np.array([(f, x * sw, y * rh, (x + w) * sw, (y + h) * rh)
+                    for f, (x, y, w, h) in enumerate(fences)], np.int32).reshape(-1, 5)
+    free_sites = x_size * y_size - sum(m[2] * m[3] for m in macros)
+    util = float((widths * heights).sum()) / max(free_sites, 1)
+    return Design(
+        core=(0, 0, W, H), **_rows(x_size, y_size, sw, rh),
+        inst_lx=inst_lx, inst_ly=inst_ly, inst_dx=inst_dx, inst_dy=inst_dy,
+        inst_fixed=inst_fixed, inst_macro=inst_macro, inst_group=inst_group,
+        inst_ground_at_bottom=inst_gnd, num_groups=len(fences),
+        box_group=box[:, 0].copy(), box_lx=box[:, 1].copy(), box_ly=box[:, 2].copy(),
+        box_ux=box[:, 3].copy(), box_uy=box[:, 4].copy(),
+        meta=dict(name=name, seed=seed, num_cells=num_cells, x_size=x_size, y_size=y_size,
+                  utilization=util, num_macros=nm, num_fences=len(fences)),
+    )
+
+
+MIXED_HEIGHTS = {1: 0.86, 2: 0.10, 3: 0.03, 4: 0.01}
+
+
+def plain_20k(seed: int = 0) -> Design:
+    """20k single-row cells on 250 rows x 490 sites at 75 % utilisation."""
+    return make_design(20000, 490, 250, width_range=(1, 8), num_macros=10,
+                       macro_w=(20, 60), macro_h=(3, 10), hotspots=4, hotspot_sigma_rows=6.0,
+                       hotspot_frac=0.057, seed=seed, name="plain_20k")
+
+
+def mixed(num_cells: int, util: float = 0.85, seed: int = 0, num_fences: int = 3,
+          fence_frac: float = 0.05, macros_per_mcell: float = 200.0,
+          hotspots: int = 0, name: str = "mixed") -> Design:
+    """The mixed-height recipe at `num_cells` cells: the die keeps the
+    1600:12000 (rows:sites) aspect of the 1M-cell case and is sized so that
+    movable area / free area = `util` with widths U{4..23}."""
+    wr = (4, 23)
+    mean_area = (sum(wr) / 2.0) * sum(h * p for h, p in MIXED_HEIGHTS.items())
+    n_mac = int(round(macros_per_mcell * num_cells / 1e6))
+    mac_area = n_mac * 40 * 6.5
+    total = num_cells * mean_area / util + mac_area
+    # sites:rows = 12000:1600 in site units -> x = 7.5 * y
+    y = max(12, int(round(math.sqrt(total / 7.5))))
+    x = max(60, int(math.ceil(total / y)))
+    return make_design(num_cells, x, y, height_mix=MIXED_HEIGHTS, width_range=wr,
+                       num_macros=n_mac, fence_frac=fence_frac if num_fences else 0.0,
+                       num_fences=num_fences, fence_util=util, hotspots=hotspots,
+                       hotspot_frac=0.05 if hotspots else 0.0, seed=seed, name=name)
+
+
+def mixed_1m(seed: int = 0) -> Design:
+    """1M mixed-height cells, 1600 rows x 12000 sites, 85 % utilisation."""
+    wr = _calibrate_width(1_000_000, 12000 * 1600 - 200 * 260, 0.85)
+    return make_design(1_000_000, 12000, 1600, height_mix=MIXED_HEIGHTS,
+                       width_range=wr, num_macros=200, fence_frac=0.05, num_fences=3,
+                       fence_util=0.85, seed=seed, name="mixed_1m")
+
+
+def _calibrate_width(n, free_sites, util):
+    mean_h = sum(h * p for h, p in MIXED_HEIGHTS.items())
+    target_w = util * free_sites / n / mean_h
+    lo = 4
+    hi = max(lo, int(round(2 * target_w - lo)))
+    return (lo, hi)
+
+
+def tiny(num_cells: int = 40, x_size: int = 48, y_size: int = 12, seed: int = 0,
+         heights=None, num_macros: int = 1, fences: int = 0) -> Design:
+    """Small dense case for exhaustive checks."""
+    return make_design(num_cells, x_size, y_size,
+                       height_mix=heights or {1: 0.8, 2: 0.2}, width_range=(1, 4),
+                       num_macros=num_macros, macro_w=(3, 5), macro_h=(2, 3),
+                       num_fences=fences, fence_frac=0.1 if fences else 0.0,
+                       hotspots=1, hotspot_frac=0.3, hotspot_sigma_rows=2.0, seed=seed,
+                       name="tiny")

@@ -1,0 +1,44 @@
+// legalizer::Legalizer -- the reference's public C++ class
+// (src/skyline/legalizer/include/legalizer/Legalizer.h:66-97) over the C ABI
+// of skyline_lg.h.  Same setter names and meanings; the design comes from an
+// in-memory LgDesign instead of db::dbDatabase, and run() reports a status
+// code and fills an LgResult instead of printing and exiting.
+#pragma once
+
+#include <string_view>
+
+#include "../skyline_lg.h"
+
+namespace legalizer {
+
+class Legalizer {
+ public:
+  explicit Legalizer(const LgDesign* design);
+
+  int run(LgResult* result);  // Legalizer::run (Legalizer.cpp:197-228)
+
+  void setTechPenalty(float val);
+  void setStandardAdmm(bool val);
+  void setQpRefine(bool val);
+  void setCoeffToInitRho(float val);
+  void setCoeffToInitDual(float val);
+  void setMaxAdmmIter(int iter);
+  void setFlagICCAD2017(bool val);
+  void setSortingAlgorithm(std::string_view algo);
+  void setSortingPolicy(std::string_view policy);
+  void setAdaptiveSearch(bool val);
+  void setMinIterToCallAdaptiveSearch(int iter);
+  void setOvfRatioToCallAdaptiveSearch(float ratio);
+  void setCoeffToNeglectDisplace(float val);
+  void setRandomSorting(bool val);
+  void setRandomSortingSeed(int seed);
+
+  const LgParams& params() const { return params_; }
+  LgParams& params() { return params_; }
+
+ private:
+  const LgDesign* design_;
+  LgParams params_;
+};
+
+}  // namespace legalizer

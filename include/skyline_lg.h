@@ -1,0 +1,244 @@
+/*
+ * skyline_lg.h -- C ABI of the MI355X (gfx950) ADMM negotiation legalizer.
+ *
+ * Plain C: fixed-width integers, raw pointers and sizes, no torch or HIP types
+ * in any signature (a stream is passed as `void*`, a hipStream_t value or NULL
+ * for the library's own stream).  Every entry point returns an int status
+ * (LG_OK == 0); on failure lg_last_error() returns a message for the calling
+ * thread.  The library never calls exit() or abort().
+ *
+ * Two layers, each replacing one layer of the reference:
+ *
+ *  1. Design level -- lg_run_design() replaces legalizer::Legalizer::run()
+ *     (reference: src/skyline/legalizer/src/Legalizer.cpp:197-228): import of
+ *     cells/regions (DatabaseIO.cpp:31-379), pixel grid (Grid.cpp:24-407),
+ *     preplace (Preplace.cpp:18-207), then the device layer below, then the
+ *     statistics (Legalizer.cpp:439-476) and checkPlace (CudaDatabase.cu:326-353).
+ *
+ *  2. Device level -- lg_create / lg_solve / lg_postprocess / lg_check /
+ *     lg_copy_out / lg_destroy replace the CUDA layer that Legalizer drives:
+ *     CudaDatabase (database/CudaDatabase.h:16-49), CudaAdmmLegalizer
+ *     (admm_legalizer/CudaAdmmLegalizer.h:19-33) and CudaPostProcess
+ *     (post_process/CudaPostProcess.h:12-20).
+ *
+ * Units: "dbu" are design database units; "grid" coordinates count sites (x)
+ * and rows (y) from the core's lower-left corner.  Pixel arrays are x-major:
+ * pixel_id = x * y_size + y (reference AdmmKernels.hpp:74).
+ */
+#ifndef SKYLINE_LG_H
+#define SKYLINE_LG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  LG_OK = 0,
+  LG_ERR_ARG = 1,        /* bad argument / inconsistent sizes            */
+  LG_ERR_DEVICE = 2,     /* HIP runtime error, or no usable gfx950 device */
+  LG_ERR_DESIGN = 3,     /* design cannot be legalized (movable macro, no space) */
+  LG_ERR_MEMORY = 4,     /* allocation failed or over the memory budget   */
+  LG_ERR_STATE = 5       /* call out of order (e.g. copy_out before solve) */
+};
+
+/* Row orientations (DEF).  N and FN rows have VDD on top of row y
+ * (reference Grid.cpp:337-341: is_vdd_up = 1), all others VSS on top. */
+enum { LG_ORIENT_N = 0, LG_ORIENT_S = 1, LG_ORIENT_W = 2, LG_ORIENT_E = 3,
+       LG_ORIENT_FN = 4, LG_ORIENT_FS = 5, LG_ORIENT_FW = 6, LG_ORIENT_FE = 7 };
+
+/* Sorting policy / algorithm (reference LGHyperParameters.h:4-5). */
+enum { LG_SORT_POLICY_RANDOM = 0, LG_SORT_POLICY_SIZE = 1, LG_SORT_POLICY_OVERFLOW = 2 };
+enum { LG_SORT_S_MERGE = 0, LG_SORT_M_MERGE = 1, LG_SORT_S_BITONIC = 2,
+       LG_SORT_M_BITONIC = 3, LG_SORT_CUB = 4 };
+
+/* Hyper-parameters; field for field the reference's LGHyperParameters
+ * (legalizer/src/LGHyperParameters.h:7-52).  lg_default_params() fills the
+ * defaults of Legalizer::initParameters (Legalizer.cpp:47-93). */
+typedef struct LgParams {
+  int32_t flag_iccad17;
+  int32_t standard_admm;
+  int32_t log_freq;
+  int32_t num_cpu_threads;
+  int32_t max_disp_in_row_height;
+  int32_t max_admm_iter;
+  int32_t admm_terminate_thr;
+  int32_t x_hint;
+  int32_t y_hint;
+  int32_t iter_update_partition;
+  float   max_disp_coeff;
+  float   tech_penalty;
+  float   edge_spacing_penalty;
+  int32_t iter_threshold_dual_update;
+  int32_t iter_update_dual_step;
+  int32_t iter_perturbation_on;
+  float   coeff_to_init_rho;
+  float   coeff_to_init_dual;
+  float   coeff_to_update_dual_step;
+  int32_t qp_admm_on;
+  int32_t qp_admm_dp_site_align;
+  int32_t qp_admm_max_iter;
+  int32_t qp_admm_min_iter;
+  float   qp_admm_rho;
+  int32_t use_adaptive_search;
+  int32_t stagnation_window;
+  int32_t min_iter_to_call_adaptive_search;
+  int32_t cooltime_to_call_adaptive_search;
+  float   stagnation_threshold;
+  float   ovf_ratio_to_call_adaptive_search;
+  float   coeff_to_neglect_displace;
+  int32_t random_sorting_seed;
+  int32_t sorting_policy;     /* LG_SORT_POLICY_* */
+  int32_t sorting_algorithm;  /* LG_SORT_*        */
+  int32_t verbose;            /* 0: silent, 1: reference-style progress on stdout */
+} LgParams;
+
+/* In-memory design: the subset of the reference's db::dbDatabase that the
+ * legalizer reads (core box, rows, instances, fence groups).  All in dbu. */
+typedef struct LgDesign {
+  int32_t dbu_per_micron;
+  int32_t core_lx, core_ly, core_ux, core_uy;
+  int32_t num_rows;
+  const int32_t* row_lx;          /* [num_rows] */
+  const int32_t* row_ly;
+  const int32_t* row_dx;          /* row width  */
+  const int32_t* row_dy;          /* row height */
+  const int32_t* row_site_width;
+  const int32_t* row_orient;      /* LG_ORIENT_* */
+  int32_t num_insts;
+  const int32_t* inst_lx;         /* [num_insts] global-placement position */
+  const int32_t* inst_ly;
+  const int32_t* inst_dx;
+  const int32_t* inst_dy;
+  const uint8_t* inst_fixed;      /* 1: FIXED/PLACED-fixed instance          */
+  const uint8_t* inst_macro;      /* 1: master of class BLOCK                */
+  const int32_t* inst_group;      /* -1 or index of the fence group          */
+  const uint8_t* inst_ground_at_bottom; /* master's VSS pin is below its VDD pin */
+  int32_t num_groups;
+  int32_t num_region_boxes;       /* fence boxes, any number per group       */
+  const int32_t* box_group;
+  const int32_t* box_lx;
+  const int32_t* box_ly;
+  const int32_t* box_ux;
+  const int32_t* box_uy;
+} LgDesign;
+
+/* Violation counts of CudaDatabase::checkPlace (objects/LGVioStat.h). */
+typedef struct LgVioStat {
+  int32_t total_ovf;
+  int32_t num_row_orient_vio;
+  int32_t num_pin_short_vio;
+  int32_t num_pin_access_vio;
+  int32_t num_edge_spacing_vio;
+} LgVioStat;
+
+typedef struct LgStats {
+  int32_t iterations;         /* ADMM iterations executed                          */
+  int32_t converged;          /* 1 if overflow reached admm_terminate_thr          */
+  int64_t final_overflow;     /* sum_p max(usage-1,0) after the last iteration     */
+  int64_t total_disp_dbu;     /* sum over movable cells |dx|+|dy| (exact)          */
+  int64_t max_disp_dbu;
+  double  total_disp_um;      /* Legalizer::computeStat (Legalizer.cpp:439-476)    */
+  double  avg_disp_um;        /*   averaged over ALL design instances, as there    */
+  double  max_disp_um;
+  float   rho;                /* initial penalty rho                               */
+  float   initial_dual;       /* initial lambda                                    */
+  int32_t num_cells;          /* movable standard cells legalized                  */
+  int32_t num_adaptive_calls; /* adaptive-search primal updates executed           */
+  LgVioStat vio;
+  double  t_import_s;         /* importDB + grid                                   */
+  double  t_preplace_s;       /* initialLegalize                                   */
+  double  t_admm_s;           /* device upload + ADMM (runAdmmLegalize)            */
+  double  t_post_s;           /* runPostProcess                                    */
+  double  t_total_s;          /* whole lg_run_design                               */
+} LgStats;
+
+/* Outputs of lg_run_design, all caller-allocated.  Per-instance arrays have
+ * num_insts entries; fixed instances are copied through unchanged. */
+typedef struct LgResult {
+  int32_t* out_lx;            /* legalized lower-left x (dbu)                    */
+  int32_t* out_ly;
+  uint8_t* out_flipped;       /* 1 if the cell gets orientation FS (rotation)    */
+  int32_t* ovf_trace;         /* optional [ovf_trace_cap]: overflow per iteration */
+  int32_t  ovf_trace_cap;
+  LgStats  stats;
+} LgResult;
+
+/* ---- device-level inputs (what CudaDatabase copies to the GPU) ---------- */
+
+typedef struct LgGridDesc {
+  int32_t x_size, y_size;          /* sites x rows                                 */
+  int32_t site_width, row_height;  /* dbu                                          */
+  int32_t core_lx, core_ly;        /* dbu                                          */
+  const int32_t* pixel_valid;      /* [x_size*y_size] 0/1                          */
+  const int32_t* pixel_group;      /* [x_size*y_size] -1 default, -2 blockage, >=0 fence */
+  const int32_t* pixel_region;     /* [x_size*y_size] region id or -1              */
+  const int32_t* row_vdd_up;       /* [y_size] 0/1                                 */
+  int32_t num_regions;             /* shrunk region bounds (adaptive search)       */
+  const int32_t* region_x_min;
+  const int32_t* region_y_min;
+  const int32_t* region_x_max;
+  const int32_t* region_y_max;
+} LgGridDesc;
+
+typedef struct LgCellsSoA {
+  int32_t num_cells;
+  const int32_t* orig_lx_dbu;      /* global-placement position                    */
+  const int32_t* orig_ly_dbu;
+  const int32_t* lx_dbu;           /* preplaced, on the site/row grid              */
+  const int32_t* ly_dbu;
+  const int32_t* w_grid;           /* width in sites                               */
+  const int32_t* h_grid;           /* height in rows                               */
+  const int32_t* group;            /* -1 or fence group                            */
+  const int32_t* ground_at_bottom; /* 0/1                                          */
+} LgCellsSoA;
+
+typedef struct LgHandle LgHandle;
+
+/* Fill `p` with the reference defaults (Legalizer.cpp:47-93). */
+void lg_default_params(LgParams* p);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* lg_last_error(void);
+
+/* Whole legalization of `d` (Legalizer::run). */
+int lg_run_design(const LgDesign* d, const LgParams* p, LgResult* r);
+
+/* Import + preplace only (no device work), for parity checks of the host
+ * side.  shape[4] receives x_size, y_size, num_cells, num_regions; call once
+ * with valid == NULL to query them.  Then: pixel maps (x-major, x_size*y_size),
+ * vdd_up[y_size], region_bounds[4*num_regions] (shrunk grid bounds x0 y0 x1 y1)
+ * and the preplaced movable-cell positions (dbu, movable-cell order).
+ * Replaces importDB + initialLegalize (DatabaseIO.cpp:31-77, Preplace.cpp:18-72). */
+int lg_import_and_preplace(const LgDesign* d, const LgParams* p, int32_t* shape, int32_t* valid,
+                           int32_t* group, int32_t* region, int32_t* vdd_up, int32_t* region_bounds,
+                           int32_t* cell_lx, int32_t* cell_ly);
+
+/* Device layer.  lg_create uploads the grid and the preplaced cells
+ * (CudaDatabase ctor + copyFromHostCells, CudaDatabase.cu:24-236). */
+int lg_create(const LgGridDesc* g, const LgCellsSoA* c, const LgParams* p, LgHandle** out);
+/* ADMM negotiation loop incl. final refine (CudaAdmmLegalizer::solve,
+ * CudaAdmmLegalizer.cu:250-319). */
+int lg_solve(LgHandle* h, void* stream);
+/* Rotation and, if qp_admm_on, the QP refine (CudaPostProcess::doPostProcess,
+ * CudaPostProcess.cu:27-66). */
+int lg_postprocess(LgHandle* h, void* stream);
+/* checkPlace (CudaDatabase.cu:326-353). */
+int lg_check(LgHandle* h, LgVioStat* out);
+/* Grid positions and rotation flags (CudaDatabase::copyToHostCells,
+ * CudaDatabase.cu:274-309); any pointer may be NULL. */
+int lg_copy_out(LgHandle* h, int32_t* lx_grid, int32_t* ly_grid, int32_t* need_rotation);
+/* Solve statistics (iterations, overflow, rho, lambda0, displacement). */
+int lg_get_stats(LgHandle* h, LgStats* out);
+/* Pixel state after the last call: usage and dual, x-major; any may be NULL. */
+int lg_get_pixel_state(LgHandle* h, int32_t* usage, float* dual);
+/* Overflow per executed iteration, up to cap entries; returns count in *n. */
+int lg_get_ovf_trace(LgHandle* h, int32_t* trace, int32_t cap, int32_t* n);
+int lg_destroy(LgHandle* h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SKYLINE_LG_H */
