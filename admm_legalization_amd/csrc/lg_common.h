@@ -33,6 +33,7 @@ struct Ctl {
   unsigned long long disp_max;
   unsigned long long check_ovf;
   unsigned long long row_vio;
+  unsigned long long phase_ticks[16];  // per-phase wall-clock ticks (LG_PHASE_TIMERS builds)
 };
 
 enum : int32_t {

@@ -108,20 +108,29 @@ int upload(LgHandle* h, T** dst, const T* src, size_t n) {
 
 hipStream_t pick(LgHandle* h, void* s) { return s ? (hipStream_t)s : h->own_stream; }
 
-size_t partition_lds_bytes(const LgHandle* h, int npad_hi) {
-  size_t pix = (size_t)h->a.x_hint * h->a.y_hint * 8;
-  size_t vdd = (size_t)((h->a.y_hint + 3) & ~3) * 4;
-  return pix + vdd + (size_t)npad_hi * 16 + (size_t)npad_hi * 12;
+size_t sort_bytes_per(const LgHandle* h) {
+  const int algo = h->p.sorting_algorithm;
+  return (algo == LG_SORT_S_MERGE || algo == LG_SORT_M_MERGE) ? 16 : (algo == LG_SORT_CUB ? 12 : 8);
 }
 
-// Launch one partition-kernel pass (three LDS size classes; the device picks
-// the class from ctl->npad and the others exit at once).
+// LDS of one partition workgroup: pixel tile {usage|code, dual} + dual/rho +
+// vdd rows, plus the sort buffers when the sorts run in LDS.
+size_t partition_lds_bytes(const LgHandle* h, int npad_hi, int lds_sort) {
+  size_t npix = (size_t)h->a.x_hint * h->a.y_hint;
+  size_t tile = npix * 8 + ((npix + 3) & ~(size_t)3) * 4 + (size_t)((h->a.y_hint + 3) & ~3) * 4;  // pix, lam, vdd
+  return tile + (lds_sort ? (size_t)npad_hi * sort_bytes_per(h) : 0);
+}
+
+// Launch one partition-kernel pass.  Two LDS size classes; the device picks
+// the class from ctl->npad and the other launch exits at once.  Up to 1024
+// slots the default bitonic order is sorted in registers.
 template <int MODE>
 int launch_partition(LgHandle* h, hipStream_t s) {
-  const int classes[3][2] = {{0, 256}, {257, 1024}, {1025, kMaxNpad}};
+  const int reg_ok = h->p.sorting_algorithm == LG_SORT_M_BITONIC || h->p.sorting_algorithm == LG_SORT_S_BITONIC;
+  const int classes[2][3] = {{0, kNpadSmall, reg_ok ? 0 : 1}, {kNpadSmall + 1, kMaxNpad, 1}};
   for (auto& c : classes) {
-    size_t lds = partition_lds_bytes(h, c[1]);
-    hipLaunchKernelGGL(k_partition<MODE>, dim3(h->a.K), dim3(kWave), lds, s, h->a, c[0], c[1]);
+    size_t lds = partition_lds_bytes(h, c[1], c[2]);
+    hipLaunchKernelGGL(k_partition<MODE>, dim3(h->a.K), dim3(kWave), lds, s, h->a, c[0], c[1], c[2]);
   }
   HIP_TRY(hipGetLastError());
   return LG_OK;
@@ -252,6 +261,11 @@ int lg_create(const LgGridDesc* g, const LgCellsSoA* c, const LgParams* p, LgHan
   if ((st = halloc(h, &a.off, a.K + 2))) return bail(st);
   if ((st = halloc(h, &a.fill, a.K + 1))) return bail(st);
   if ((st = halloc(h, &a.grouped, N))) return bail(st);
+  if ((st = halloc(h, &a.cellrec, 4 * (size_t)N))) return bail(st);
+  if ((st = halloc(h, &a.rec_a, 4 * (size_t)N))) return bail(st);
+  if ((st = halloc(h, &a.rec_b, 4 * (size_t)N))) return bail(st);
+  if ((st = halloc(h, &a.ids_a, N))) return bail(st);
+  if ((st = halloc(h, &a.sorted, a.K))) return bail(st);
   if ((st = halloc(h, &a.ovf_flag, N))) return bail(st);
   if ((st = halloc(h, &a.ovf_list, N))) return bail(st);
   if ((st = halloc(h, &a.trace, std::max(p->max_admm_iter, 1)))) return bail(st);
@@ -301,6 +315,7 @@ int lg_solve(LgHandle* h, void* stream) {
     hipLaunchKernelGGL(k_rho, dim3(1), dim3(1), 0, s, a, h->coeff_init_rho);
     hipLaunchKernelGGL(k_inv_aug, dim3(nb), dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_lambda0, dim3(1), dim3(1), 0, s, a, h->coeff_init_dual);
+    hipLaunchKernelGGL(k_pack_rec, dim3(nb), dim3(256), 0, s, a);
   }
   hipLaunchKernelGGL(k_fill_dual, dim3(2048), dim3(256), 0, s, a);
   HIP_TRY(hipGetLastError());
@@ -332,6 +347,7 @@ int lg_solve(LgHandle* h, void* stream) {
     if (h->host_ctl.converged) {  // doFinalRefine (CudaAdmmLegalizer.cu:1011-1115)
       if ((st = launch_partition<MODE_REFINE>(h, s))) return st;
     }
+    hipLaunchKernelGGL(k_unpack_pos, dim3(nb), dim3(256), 0, s, a);
   } else {
     HIP_TRY(hipMemsetAsync(a.ctl, 0, sizeof(int32_t) * 2, s));
   }
@@ -427,6 +443,15 @@ int lg_get_ovf_trace(LgHandle* h, int32_t* trace, int32_t cap, int32_t* n) {
   int m = std::min(cap, (int32_t)h->trace_n);
   if (trace && m > 0) std::memcpy(trace, h->trace.data(), m * sizeof(int32_t));
   if (n) *n = h->trace_n;
+  return LG_OK;
+}
+
+// Debug counters of a LG_PHASE_TIMERS build (zeros otherwise).
+int lg_debug_phase_ticks(LgHandle* h, uint64_t* out, int n) {
+  if (!h || !out) return fail(LG_ERR_ARG, "null argument");
+  Ctl c;
+  HIP_TRY(hipMemcpy(&c, h->a.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+  for (int i = 0; i < n && i < 16; i++) out[i] = c.phase_ticks[i];
   return LG_OK;
 }
 

@@ -31,6 +31,22 @@ __constant__ int2 c_dir_wide[kNumDirWide];
 
 constexpr float kInf = __builtin_huge_valf();
 
+// Optional per-phase timing of the partition kernel (build with
+// -DLG_PHASE_TIMERS): lane 0 of every workgroup adds the wall-clock ticks of
+// each phase to Ctl::phase_ticks.
+#ifdef LG_PHASE_TIMERS
+#define PHASE_MARK(i)                                                          \
+  do {                                                                         \
+    unsigned long long t_ = wall_clock64();                                    \
+    if (threadIdx.x == 0) atomicAdd(&a.ctl->phase_ticks[i], t_ - t_phase_);    \
+    t_phase_ = t_;                                                             \
+  } while (0)
+#define PHASE_START() unsigned long long t_phase_ = wall_clock64()
+#else
+#define PHASE_MARK(i) do {} while (0)
+#define PHASE_START() do {} while (0)
+#endif
+
 // std::max(a, b) of the reference's host-side semantics: (a < b) ? b : a
 __device__ __forceinline__ float fmax_ref(float a, float b) { return (a < b) ? b : a; }
 
@@ -68,6 +84,11 @@ struct Args {
   int32_t* fill;
   int32_t* grouped;
   int32_t* trace;
+  int32_t* cellrec;    // [4N] cell records (int4), the positions' home during a solve
+  int32_t* rec_a;      // [4N] cell records in ascending-id order (int4)
+  int32_t* rec_b;      // [4N] cell records in processing order (int4)
+  int32_t* ids_a;      // [N]  cell ids in ascending-id order
+  int32_t* sorted;     // [K]  1 if the partition's order was sorted in the last primal step
   int32_t* ovf_flag;   // [N] adaptive search: cell has overflow
   int32_t* ovf_list;   // [N]
   // parameters
@@ -192,8 +213,9 @@ __global__ void k_assign(Args a) {
   if (c >= a.N) return;
   int tf = triple_fold(a, ctl->iter);
   int oxs = a.x_hint / 3 * tf, oys = a.y_hint / 3 * tf;
-  int lxg = a.lx[c] + oxs, lyg = a.ly[c] + oys;
-  int uxg = lxg + a.w[c] - 1, uyg = lyg + a.h[c] - 1;
+  int4 r = reinterpret_cast<const int4*>(a.cellrec)[c];
+  int lxg = (r.x & 0xffff) + oxs, lyg = (int)((unsigned)r.x >> 16) + oys;
+  int uxg = lxg + (r.y & 0xfff) - 1, uyg = lyg + ((r.y >> 12) & 0x3f) - 1;
   int c1 = (lxg / a.x_hint) % a.ncol, c2 = (uxg / a.x_hint) % a.ncol;
   int r1 = (lyg / a.y_hint) % a.nrow, r2 = (uyg / a.y_hint) % a.nrow;
   int p = (c1 == c2 && r1 == r2) ? c1 * a.nrow + r1 : a.K;
@@ -364,55 +386,77 @@ __device__ void lds_stable_desc(int n, int count, int* key, int* val, int* pos) 
   (void)count;
 }
 
-// Partition geometry for the iteration (paintPixelsKernel colour rule,
-// AdmmKernels.hpp:164-191, evaluated arithmetically instead of stored).
+// Partition geometry of one iteration.  The reference paints every pixel with
+// its partition colour (paintPixelsKernel, AdmmKernels.hpp:164-191):
+// colour(x, y) = ((x+ox)/x_hint % ncol) * nrow + (y+oy)/y_hint % nrow.  Here
+// the colour test is two interval checks per axis: partition column `col`
+// covers x in [col*xh - ox, (col+1)*xh - ox) and, through the wrap-around of
+// the shifted grid, [(col+ncol)*xh - ox, (col+ncol+1)*xh - ox).
 struct PartGeo {
-  int oxs, oys, col, row, xh, yh, ncol, nrow;
-  __device__ bool in_x(int x) const { return ((x + oxs) / xh) % ncol == col; }
-  __device__ bool in_y(int y) const { return ((y + oys) / yh) % nrow == row; }
-  __device__ int lx(int x) const { return (x + oxs) % xh; }
-  __device__ int ly(int y) const { return (y + oys) % yh; }
-  // global coordinate of local column la / row lb, or -1 if outside the grid.
-  // Local index la is reached from X = col*xh + la or (col+ncol)*xh + la
-  // (the wrap-around of the shifted grid); at most one lies in the grid.
-  __device__ int gx(int la, int x_size) const {
-    int x = col * xh + la - oxs;
-    if (x >= 0 && x < x_size) return x;
-    x += ncol * xh;
-    return (x >= 0 && x < x_size) ? x : -1;
+  int oxs, oys, col, row, xh, yh, ncol, nrow, xs, ys;
+  int xa0, xa1, xb0, xb1, ya0, ya1, yb0, yb1;
+  __device__ void init(int xs_, int ys_) {
+    xs = xs_; ys = ys_;
+    xa0 = col * xh - oxs; xa1 = xa0 + xh;
+    xb0 = (col + ncol) * xh - oxs; xb1 = xb0 + xh;
+    ya0 = row * yh - oys; ya1 = ya0 + yh;
+    yb0 = (row + nrow) * yh - oys; yb1 = yb0 + yh;
   }
-  __device__ int gy(int lb, int y_size) const {
-    int y = row * yh + lb - oys;
-    if (y >= 0 && y < y_size) return y;
-    y += nrow * yh;
-    return (y >= 0 && y < y_size) ? y : -1;
+  __device__ bool in_x(int x) const { return (x >= xa0 && x < xa1) || (x >= xb0 && x < xb1); }
+  __device__ bool in_y(int y) const { return (y >= ya0 && y < ya1) || (y >= yb0 && y < yb1); }
+  // local index (x + ox) mod xh of an in-partition (or in-grid, ncol == 1) x
+  __device__ int lx(int x) const {
+    int l = x - xa0;
+    if (l >= xh) l -= ncol * xh;
+    return l;
+  }
+  __device__ int ly(int y) const {
+    int l = y - ya0;
+    if (l >= yh) l -= nrow * yh;
+    return l;
+  }
+  // global coordinate of local column la / row lb, or -1 if outside the grid
+  __device__ int gx(int la) const {
+    int x = xa0 + la;
+    if (x >= 0 && x < xs) return x;
+    x = xb0 + la;
+    return (x >= 0 && x < xs) ? x : -1;
+  }
+  __device__ int gy(int lb) const {
+    int y = ya0 + lb;
+    if (y >= 0 && y < ys) return y;
+    y = yb0 + lb;
+    return (y >= 0 && y < ys) ? y : -1;
   }
 };
 
-// Cached cell record in LDS (16 bytes)
-struct CellRec {
-  int pos;    // lx | ly << 16 (grid)
-  int shape;  // w | h << 12 | gnd << 18 | code << 19
-  int ox, oy; // dbu
-};
-
+// Cell record in HBM (int4, 16 bytes): {lx | ly << 16, w | h << 12 | gnd << 18
+// | code << 19, ox, oy}.  One load brings everything a cell move needs.
 __device__ __forceinline__ int rec_w(int s) { return s & 0xfff; }
 __device__ __forceinline__ int rec_h(int s) { return (s >> 12) & 0x3f; }
 __device__ __forceinline__ int rec_gnd(int s) { return (s >> 18) & 1; }
 __device__ __forceinline__ int rec_code(int s) { return (s >> 19) & 0x1fff; }
+__device__ __forceinline__ int rec_x(int p) { return p & 0xffff; }
+__device__ __forceinline__ int rec_y(int p) { return (int)((unsigned)p >> 16); }
+__device__ __forceinline__ int rec_pos(int x, int y) { return (x & 0xffff) | (y << 16); }
 
-// computeDispCost (AdmmKernels.hpp:508-530)
-__device__ __forceinline__ float disp_cost(const Args& a, float disp_coeff, int nx, int ny, int ox, int oy) {
+// computeDispCost (AdmmKernels.hpp:508-530); `thr` is the loop-invariant
+// float(max_disp_in_row_height) * float(row_height) / float(site_width)
+__device__ __forceinline__ float disp_cost(const Args& a, float disp_coeff, float thr, int nx, int ny, int ox,
+                                           int oy) {
   int dx = nx * a.site_width + a.core_lx;
   int dy = ny * a.row_height + a.core_ly;
   int disp = abs(dx - ox) + abs(dy - oy);
   float ds = float(disp) / float(a.site_width);
-  float thr = float(a.max_disp_in_row_height) * float(a.row_height);
-  thr = thr / float(a.site_width);
   float ex = ds - thr;
   float p = disp_coeff * ds;
   float q = a.max_disp_coeff * fmax_ref(ex, 0.0f);
   return p + q;
+}
+
+__device__ __forceinline__ float disp_threshold(const Args& a) {
+  float thr = float(a.max_disp_in_row_height) * float(a.row_height);
+  return thr / float(a.site_width);
 }
 
 // getNewCellPosition (AdmmKernels.hpp:540-560)
@@ -424,79 +468,395 @@ __device__ __forceinline__ void new_pos(const Args& a, int2 d, int w, int h, int
   ny = max(0, min(a.y_size - h, ny));
 }
 
-template <int MODE>
-struct PartitionKernel;
-
 enum { MODE_PRIMAL = 0, MODE_REFINE = 1 };
 
-// computePenaltySum (AdmmKernels.hpp:333-391) with partition_exclusive = true,
-// evaluated on the LDS copy of the partition's pixels.
-template <int MODE>
-__device__ __forceinline__ float penalty_lds(const Args& a, const PartGeo& g, const uint2* pix,
-                                             const int* vdd_loc, float rho, bool standard, int nx, int ny,
-                                             int w, int h, int code, int gnd) {
-  if (nx + w > a.x_size || ny + h > a.y_size || nx < 0 || ny < 0) return kInf;
-  // all footprint pixels must carry this partition's colour
-  if (g.ncol > 1 && !(w <= g.xh && g.in_x(nx) && g.in_x(nx + w - 1))) return kInf;
-  if (g.nrow > 1 && !(h <= g.yh && g.in_y(ny) && g.in_y(ny + h - 1))) return kInf;
-  int ly0 = g.ly(ny);
-  if (!correct_row(h, gnd, vdd_loc[ly0])) return kInf;
-  float s = 0.0f;
-  int lxi = g.lx(nx);
+// LDS tile of the partition: pix[i] = {usage | code << 16, t}, lam[i] = dual,
+// where t is the penalty term of computePenaltySum for the pixel's current
+// usage u (AdmmKernels.hpp:380-383):
+//   t = rho * max(lam * (u - 1) + lam / rho, 0)     (negotiation ADMM)
+//   t = rho * max(u - 1 + lam / rho, 0)             (standard ADMM)
+// t is recomputed, with the same operations, whenever u changes.
+struct Tile {
+  uint2* pix;
+  float* lam;
+  int* vdd;
+};
+
+__device__ __forceinline__ float pen_term(int ui, float l, float rho, bool standard) {
+  float u = float(ui);
+  float m1 = u - 1.0f;
+  float q = l / rho;
+  if (standard) return rho * fmax_ref(m1 + q, 0.0f);
+  float m = l * m1;
+  return rho * fmax_ref(m + q, 0.0f);
+}
+
+// overflow of a footprint from the LDS tile (computeCellOverflow, AdmmKernels.hpp:393-406)
+__device__ __forceinline__ int ovf_lds(const PartGeo& g, const uint2* pix, int cx, int cy, int w, int h) {
+  int s = 0;
+  int lxi = g.lx(cx);
+  const int ly0 = g.ly(cy);
   for (int x = 0; x < w; x++) {
     int lyi = ly0;
     for (int y = 0; y < h; y++) {
-      uint2 e = pix[lxi * g.yh + lyi];
-      if ((int)(e.x >> 16) != code) return kInf;
-      float u = float(e.x & 0xffffu);
-      float l = __uint_as_float(e.y);
-      if (MODE == MODE_REFINE) {
-        if (u > 0.0f) return kInf;
-      } else {
-        float t;
-        float m1 = u - 1.0f;
-        float b = l / rho;
-        if (standard) {
-          t = rho * fmax_ref(m1 + b, 0.0f);
-        } else {
-          float m = l * m1;
-          t = rho * fmax_ref(m + b, 0.0f);
-        }
-        s += t;
-      }
+      s += max((int)(pix[lxi * g.yh + lyi].x & 0xffffu) - 1, 0);
       if (++lyi == g.yh) lyi = 0;
     }
     if (++lxi == g.xh) lxi = 0;
   }
-  return MODE == MODE_REFINE ? 0.0f : s;
+  return s;
 }
 
-__device__ __forceinline__ void footprint_add(const PartGeo& g, uint2* pix, int x0, int y0, int w, int h,
-                                              int delta) {
+// the same from the pixels in HBM (x-major), for a cell inside the grid
+__device__ __forceinline__ int ovf_hbm(const Args& a, int cx, int cy, int w, int h) {
+  int s = 0;
+  for (int x = cx; x < cx + w; x++)
+    for (int y = cy; y < cy + h; y++) s += max(a.usage[(size_t)x * a.y_size + y] - 1, 0);
+  return s;
+}
+
+// rip-up (delta -1) or commit (+1) of a footprint; refreshes the penalty terms
+__device__ __forceinline__ void footprint_add(const PartGeo& g, const Tile& T, int x0, int y0, int w, int h,
+                                              int delta, float rho, bool standard) {
   const int lane = threadIdx.x;
   const int n = w * h;
-  int lx0 = g.lx(x0), ly0 = g.ly(y0);
+  const int lx0 = g.lx(x0), ly0 = g.ly(y0);
   for (int i = lane; i < n; i += kWave) {
-    int lxi = lx0 + i / h, lyi = ly0 + i % h;
+    int dx = i / h;
+    int lxi = lx0 + dx, lyi = ly0 + (i - dx * h);
     if (lxi >= g.xh) lxi -= g.xh;
     if (lyi >= g.yh) lyi -= g.yh;
-    uint2* e = &pix[lxi * g.yh + lyi];
-    e->x = (uint32_t)((int)e->x + delta);
+    const int idx = lxi * g.yh + lyi;
+    uint2 e = T.pix[idx];
+    e.x = (uint32_t)((int)e.x + delta);
+    e.y = __float_as_uint(pen_term((int)(e.x & 0xffffu), T.lam[idx], rho, standard));
+    T.pix[idx] = e;
   }
+}
+
+// The 4 candidates of one lane (moves d = j*64 + lane), evaluated together so
+// that their LDS reads overlap.  cost = computeDispCost + computePenaltySum
+// (partition_exclusive, AdmmKernels.hpp:333-391); pin-access and edge-spacing
+// terms are zero (no pins, no edge rules).  A footprint that leaves the
+// partition, lands on a wrong-rail row, or touches a pixel of another group or
+// an invalid pixel costs +inf; the penalty terms sum in the reference's
+// x-major order.
+template <int MODE>
+__device__ __forceinline__ void eval_candidates(const Args& a, const PartGeo& g, const Tile& T,
+                                                float disp_coeff, float thr, int cx, int cy,
+                                                int w, int h, int code, int gnd, int ox, int oy,
+                                                const int2 dir[4], float cst[4]) {
+  const int lane = threadIdx.x;
+  int lxi[4], lyi0[4];
+  bool bad[4];
+  float s[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    int d = j * kWave + lane;
+    bad[j] = true;
+    lxi[j] = 0;
+    lyi0[j] = 0;
+    s[j] = 0.0f;
+    cst[j] = kInf;
+    if (d < kNumDirDefault) {
+      int nx, ny;
+      new_pos(a, dir[j], w, h, cx, cy, nx, ny);
+      cst[j] = disp_cost(a, disp_coeff, thr, nx, ny, ox, oy);
+      bool ok = !(nx + w > a.x_size || ny + h > a.y_size || nx < 0 || ny < 0);
+      if (ok && g.ncol > 1) ok = w <= g.xh && g.in_x(nx) && g.in_x(nx + w - 1);
+      if (ok && g.nrow > 1) ok = h <= g.yh && g.in_y(ny) && g.in_y(ny + h - 1);
+      if (ok) {
+        lyi0[j] = g.ly(ny);
+        lxi[j] = g.lx(nx);
+        ok = correct_row(h, gnd, T.vdd[lyi0[j]]);
+      }
+      bad[j] = !ok;
+    }
+  }
+  if (g.ncol > 1 && g.nrow > 1) {
+    // footprints inside one interval never wrap: flat x-major walk
+    int col[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) col[j] = lxi[j] * g.yh + lyi0[j];
+    for (int x = 0; x < w; x++) {
+      for (int y = 0; y < h; y++) {
+        uint2 e[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) e[j] = T.pix[bad[j] ? 0 : col[j] + y];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          if ((int)(e[j].x >> 16) != code) bad[j] = true;
+          if (MODE == MODE_REFINE) {
+            if ((e[j].x & 0xffffu) != 0u) bad[j] = true;
+          } else {
+            s[j] += __uint_as_float(e[j].y);
+          }
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 4; j++) col[j] += g.yh;
+    }
+  } else {
+    for (int x = 0; x < w; x++) {
+      int lyi[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) lyi[j] = lyi0[j];
+      for (int y = 0; y < h; y++) {
+        uint2 e[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) e[j] = T.pix[bad[j] ? 0 : lxi[j] * g.yh + lyi[j]];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          if ((int)(e[j].x >> 16) != code) bad[j] = true;
+          if (MODE == MODE_REFINE) {
+            if ((e[j].x & 0xffffu) != 0u) bad[j] = true;
+          } else {
+            s[j] += __uint_as_float(e[j].y);
+          }
+          if (++lyi[j] == g.yh) lyi[j] = 0;
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+        if (++lxi[j] == g.xh) lxi[j] = 0;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    float pc = (MODE == MODE_REFINE) ? 0.0f : s[j];
+    cst[j] = bad[j] ? kInf : cst[j] + pc;
+  }
+}
+
+// ---- register-resident sorting networks -----------------------------------
+// Element i = r*64 + lane lives in register r of lane `lane`.  Every
+// compare-exchange of the network on (i, i^j) is executed as in the LDS
+// version (AdmmKernels.hpp:899-945): partners with j >= 64 are two registers
+// of one lane, partners with j < 64 are one register of two lanes.
+template <int R, int JR, bool DESC_NET>
+__device__ __forceinline__ void reg_step_inlane(int n, int k, int (&key)[R], int (&val)[R]) {
+  const int lane = threadIdx.x;
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    constexpr int dummy = 0;
+    (void)dummy;
+    const int rp = r ^ JR;
+    if (rp > r && rp * kWave < n) {
+      const int lo = r * kWave + lane;
+      const bool desc = DESC_NET ? ((lo & k) == 0) : ((lo & k) != 0);
+      const int kl = key[r], kh = key[rp];
+      const bool sw = desc ? (kl < kh) : (kl > kh);
+      if (sw) {
+        key[r] = kh; key[rp] = kl;
+        int t = val[r]; val[r] = val[rp]; val[rp] = t;
+      }
+    }
+  }
+}
+
+template <int R, bool DESC_NET>
+__device__ __forceinline__ void reg_bitonic(int n, int (&key)[R], int (&val)[R]) {
+  const int lane = threadIdx.x;
+  for (int k = 2; k <= n; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      if (j >= kWave) {
+        const int jr = j >> 6;
+        if (R > 1 && jr == 1) reg_step_inlane<R, 1, DESC_NET>(n, k, key, val);
+        if (R > 2 && jr == 2) reg_step_inlane<R, (R > 2 ? 2 : 1), DESC_NET>(n, k, key, val);
+        if (R > 4 && jr == 4) reg_step_inlane<R, (R > 4 ? 4 : 1), DESC_NET>(n, k, key, val);
+        if (R > 8 && jr == 8) reg_step_inlane<R, (R > 8 ? 8 : 1), DESC_NET>(n, k, key, val);
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          if (r * kWave < n) {
+            const int i = r * kWave + lane;
+            const int pk = __shfl_xor(key[r], j);
+            const int pv = __shfl_xor(val[r], j);
+            const bool i_lo = (lane & j) == 0;
+            const int lo = i_lo ? i : (i ^ j);
+            const bool desc = DESC_NET ? ((lo & k) == 0) : ((lo & k) != 0);
+            const int klo = i_lo ? key[r] : pk;
+            const int khi = i_lo ? pk : key[r];
+            const bool sw = desc ? (klo < khi) : (klo > khi);
+            if (sw) { key[r] = pk; val[r] = pv; }
+          }
+        }
+      }
+    }
+  }
+}
+
+// Staging of the partition's pixel tile, lanes walking the tile in order
+// with incremental (la, lb) = (i / yh, i % yh); four loads in flight per lane.
+// Returns true if a usage does not fit the 16-bit LDS packing.
+__device__ __forceinline__ bool stage_tile(const Args& a, const PartGeo& g, const Tile& T, float rho,
+                                           bool standard) {
+  const int lane = threadIdx.x;
+  const int npix = g.xh * g.yh;
+  const int step_a = kWave / g.yh, step_b = kWave - step_a * g.yh;
+  int la = lane / g.yh, lb = lane - la * g.yh;
+  bool range_err = false;
+  constexpr int U = 4;
+  for (int i0 = 0; i0 < npix; i0 += U * kWave) {
+    int u[U], code[U];
+    float dl[U];
+#pragma unroll
+    for (int q = 0; q < U; q++) {
+      int xx = g.gx(la), yy = g.gy(lb);
+      bool ok = (i0 + q * kWave + lane < npix) && xx >= 0 && yy >= 0;
+      size_t p = ok ? (size_t)xx * a.y_size + yy : 0;
+      u[q] = ok ? a.usage[p] : 0;
+      code[q] = ok ? (int)(uint16_t)a.pcode[p] : 0;  // code 0: not placeable
+      dl[q] = ok ? a.dual[p] : 0.0f;
+      la += step_a; lb += step_b;
+      if (lb >= g.yh) { lb -= g.yh; la++; }
+    }
+#pragma unroll
+    for (int q = 0; q < U; q++) {
+      int i = i0 + q * kWave + lane;
+      if (i < npix) {
+        range_err |= u[q] > 0xffff;
+        T.pix[i] = make_uint2(((uint32_t)code[q] << 16) | (uint32_t)(u[q] & 0xffff),
+                              __float_as_uint(pen_term(u[q], dl[q], rho, standard)));
+        T.lam[i] = dl[q];
+      }
+    }
+  }
+  return range_err;
+}
+
+// key of computeKeyArrayForSortingKernel (SortingKernels.hpp:9-55) from the LDS usage
+__device__ __forceinline__ int sort_key(const Args& a, const PartGeo& g, const Tile& T, int c, int4 rc) {
+  int cr = 0, co = 10, ca = 1;
+  if (a.sort_policy == 0) { cr = 1; co = 0; ca = 0; }
+  int w = rec_w(rc.y), h = rec_h(rc.y);
+  int ovf = co ? ovf_lds(g, T.pix, rec_x(rc.x), rec_y(rc.x), w, h) : 0;
+  return cr * (cr ? a.rand_prio[c] : 0) + co * ovf + ca * w * h;
+}
+
+// Processing order of one partition with both sorts in registers (npad <=
+// 64*R): ascending cell id (thrust::stable_sort_by_key order), keys, then the
+// reference's descending bitonic network on npad slots.  Writes the order to
+// grouped[off..] (ids) and rec_b[off..] (records).
+template <int R>
+__device__ __forceinline__ void order_partition_reg(const Args& a, const PartGeo& g, const Tile& T, int npad, int off,
+                                    int count, const int4* cellrec) {
+  const int lane = threadIdx.x;
+  int* ids = a.grouped + off;
+  int4* reca = reinterpret_cast<int4*>(a.rec_a) + off;
+  int4* recb = reinterpret_cast<int4*>(a.rec_b) + off;
+  int* ida = a.ids_a + off;
+  int key[R], val[R];
+  int na = 1;
+  while (na < count) na <<= 1;
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    int i = r * kWave + lane;
+    key[r] = (i < count) ? ids[i] : INT_MAX;
+    val[r] = 0;
+  }
+  reg_bitonic<R, false>(na, key, val);
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    int i = r * kWave + lane;
+    if (i < count) {
+      int c = key[r];
+      int4 rc = cellrec[c];
+      reca[i] = rc;
+      ida[i] = c;
+      key[r] = sort_key(a, g, T, c, rc);
+      val[r] = i;
+    } else {
+      key[r] = INT_MIN;
+      val[r] = -1;
+    }
+  }
+  reg_bitonic<R, true>(npad, key, val);
+  __syncthreads();  // reca / ida written by other lanes
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    int i = r * kWave + lane;
+    if (i < count) {
+      ids[i] = ida[val[r]];
+      recb[i] = reca[val[r]];
+    }
+  }
+}
+
+// The same with the sorts in LDS (npad > 1024, or the merge / radix options).
+__device__ void order_partition_lds(const Args& a, const PartGeo& g, const Tile& T, int npad, int npad_hi,
+                                    int off, int count, const int4* cellrec, int* keys, int* slots, int* aux) {
+  const int lane = threadIdx.x;
+  int* ids = a.grouped + off;
+  int4* reca = reinterpret_cast<int4*>(a.rec_a) + off;
+  int4* recb = reinterpret_cast<int4*>(a.rec_b) + off;
+  int* ida = a.ids_a + off;
+  int na = 1;
+  while (na < count) na <<= 1;
+  for (int i = lane; i < na; i += kWave) keys[i] = (i < count) ? ids[i] : INT_MAX;
+  __syncthreads();
+  lds_sort_ids(na, keys);
+  for (int i = lane; i < npad; i += kWave) {
+    if (i < count) {
+      int c = keys[i];
+      int4 rc = cellrec[c];
+      reca[i] = rc;
+      ida[i] = c;
+      keys[i] = sort_key(a, g, T, c, rc);
+      slots[i] = i;
+    } else {
+      keys[i] = INT_MIN;
+      slots[i] = -1;
+    }
+  }
+  __syncthreads();
+  if (a.sort_algo == 0 || a.sort_algo == 1) lds_merge_desc(npad, keys, slots, aux, aux + npad_hi);
+  else if (a.sort_algo == 4) lds_stable_desc(npad, count, keys, slots, aux);
+  else lds_bitonic_desc(npad, keys, slots);
+  for (int i = lane; i < count; i += kWave) {
+    int s = slots[i];
+    ids[i] = ida[s];
+    recb[i] = reca[s];
+  }
+}
+
+// dual_update_functor_with_step_size (AdmmKernels.hpp:26-42): the literal 0.5
+// promotes the quadratic term to double before the float store.
+__device__ __forceinline__ float dual_step(int ui, float l, float rho, float step, float neg) {
+  float u = float(ui);
+  float gg = fmax_ref(u - 1.0f, neg);
+  double q = (double)rho * 0.5;
+  q = q * (double)gg;
+  q = q * (double)gg;
+  float delta = (float)((double)gg + q);
+  float m = step * delta;
+  return fmax_ref(l + m, 0.0f);
 }
 
 // doPrimalUpdateKernel (AdmmKernels.hpp:637-897) / doFinalRefineKernel
 // (AdmmKernels.hpp:1176-1416) for one partition, plus -- in primal mode -- the
-// dual update (dual_update_functor_with_step_size, AdmmKernels.hpp:26-42) and
-// the overflow count of the partition's pixels.
+// dual update and the overflow count of the partition's pixels.
+//
+// One 64-lane wavefront owns one partition.  If none of its cells needs a move
+// (no overflow in its footprint, correct rail) the primal step is a no-op and
+// only the dual update streams over its pixels.  Otherwise the pixels are
+// staged in LDS, the processing order is sorted, and the cells are visited in
+// that order; whether a cell is skipped is evaluated for a window of 64
+// upcoming cells at once and re-evaluated after every move that touches its
+// footprint, which is exactly the sequential decision because a skipped cell
+// changes nothing.
+//
+// npad_lo..npad_hi selects the launch (LDS size class); lds_sort launches keep
+// the sort buffers in LDS.
 template <int MODE>
-__global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_hi) {
+__global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_hi, int lds_sort) {
   Ctl* ctl = a.ctl;
   int iter;
+  const bool adaptive_now = (MODE == MODE_PRIMAL) && a.use_adaptive && ctl->exceptional;
   if (MODE == MODE_PRIMAL) {
     if (ctl->done) return;
-    if (a.use_adaptive && ctl->exceptional) {
-      if (npad_lo != 0) return;  // only the small variant runs the dual-only pass
+    if (adaptive_now) {
+      if (npad_lo != 0) return;  // only the smallest class runs the dual-only pass
     } else {
       int np = ctl->npad;
       if (np < npad_lo || np > npad_hi) return;
@@ -507,210 +867,227 @@ __global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_
     if (np < npad_lo || np > npad_hi) return;
     iter = ctl->conv_iter;
   }
-  const bool dual_only = (MODE == MODE_PRIMAL) && a.use_adaptive && ctl->exceptional;
+  PHASE_START();
   const int k = blockIdx.x;
   const int lane = threadIdx.x;
   PartGeo g;
-  int tf = triple_fold(a, iter);
+  const int tf = triple_fold(a, iter);
   g.xh = a.x_hint; g.yh = a.y_hint; g.ncol = a.ncol; g.nrow = a.nrow;
   g.oxs = g.xh / 3 * tf; g.oys = g.yh / 3 * tf;
-  g.col = k / a.nrow; g.row = k % a.nrow;
-  const int npad = dual_only ? 1 : ctl->npad;
-
-  extern __shared__ __align__(16) unsigned char lds_raw[];
-  uint2* pix = reinterpret_cast<uint2*>(lds_raw);                       // [xh*yh]
-  int* vdd_loc = reinterpret_cast<int*>(pix + g.xh * g.yh);             // [yh]
-  CellRec* rec = reinterpret_cast<CellRec*>(vdd_loc + ((g.yh + 3) & ~3));  // [npad]
-  int* keys = reinterpret_cast<int*>(rec + npad_hi);                    // [npad]
-  int* vals = keys + npad_hi;                                           // [npad]
-  int* aux = vals + npad_hi;                                            // [npad] (merge/stable sorts)
-
-  // -- stage the partition's pixels: local (a_, b_) <-> global (x, y) -------
+  g.col = k / a.nrow; g.row = k - g.col * a.nrow;
+  g.init(a.x_size, a.y_size);
+  int4* cellrec = reinterpret_cast<int4*>(a.cellrec);
   const int npix = g.xh * g.yh;
-  for (int i = lane; i < npix; i += kWave) {
-    int la = i / g.yh, lb = i - la * g.yh;
-    int xx = g.gx(la, a.x_size), yy = g.gy(lb, a.y_size);
-    uint2 e;
-    if (xx >= 0 && yy >= 0) {
-      size_t p = (size_t)xx * a.y_size + yy;
-      int u = a.usage[p];
-      if (u > 0xffff) ctl->error = kErrUsageRange;
-      e.x = ((uint32_t)(uint16_t)a.pcode[p] << 16) | (uint32_t)(u & 0xffff);
-      e.y = __float_as_uint(a.dual[p]);
-    } else {
-      e.x = 0;  // code 0: never placeable
-      e.y = 0;
-    }
-    pix[i] = e;
-  }
-  for (int b = lane; b < g.yh; b += kWave) {
-    int yy = g.gy(b, a.y_size);
-    vdd_loc[b] = yy >= 0 ? a.vdd_up[yy] : 0;
-  }
-  __syncthreads();
+  const int step_a = kWave / g.yh, step_b = kWave - step_a * g.yh;
 
   const int off = a.off[k];
-  const int count = dual_only ? 0 : a.off[k + 1] - off;
+  const int count = adaptive_now ? 0 : a.off[k + 1] - off;
   const float rho = ctl->rho;
+  const float step = ctl->step;
+  const float neg = -1.0f / rho;
+  const bool pert = (MODE == MODE_PRIMAL) && iter < a.iter_perturbation_on;
+  const bool standard = (MODE == MODE_PRIMAL) && a.standard_admm;
+  int* ids = a.grouped + off;
 
-  if (!dual_only && count > 0) {
-    // -- processing order -------------------------------------------------
-    if (MODE == MODE_PRIMAL) {
-      int na = 1;
-      while (na < count) na <<= 1;
-      for (int i = lane; i < na; i += kWave) vals[i] = (i < count) ? a.grouped[off + i] : INT_MAX;
-      __syncthreads();
-      lds_sort_ids(na, vals);  // ascending cell id = thrust::stable_sort_by_key order
-      // computeKeyArrayForSortingKernel (SortingKernels.hpp:9-55)
-      int cr = 0, co = 10, ca = 1;
-      if (a.sort_policy == 0) { cr = 1; co = 0; ca = 0; }
-      for (int i = lane; i < npad; i += kWave) {
-        if (i < count) {
-          int c = vals[i];
-          int x0 = a.lx[c], y0 = a.ly[c], w = a.w[c], h = a.h[c];
-          int ovf = 0;
-          int lx0 = g.lx(x0), ly0 = g.ly(y0);
-          for (int x = 0; x < w; x++) {
-            int lxi = lx0 + x;
-            if (lxi >= g.xh) lxi -= g.xh;
-            for (int y = 0; y < h; y++) {
-              int lyi = ly0 + y;
-              if (lyi >= g.yh) lyi -= g.yh;
-              int u = (int)(pix[lxi * g.yh + lyi].x & 0xffffu);
-              ovf += max(u - 1, 0);
-            }
-          }
-          keys[i] = cr * a.rand_prio[c] + co * ovf + ca * w * h;
-        } else {
-          keys[i] = INT_MIN;
-          vals[i] = -1;
-        }
-      }
-      __syncthreads();
-      if (a.sort_algo == 0 || a.sort_algo == 1) lds_merge_desc(npad, keys, vals, aux, reinterpret_cast<int*>(rec));
-      else if (a.sort_algo == 4) lds_stable_desc(npad, count, keys, vals, aux);
-      else lds_bitonic_desc(npad, keys, vals);
-      for (int i = lane; i < count; i += kWave) a.grouped[off + i] = vals[i];
-      __syncthreads();
-    } else {
-      for (int i = lane; i < count; i += kWave) vals[i] = a.grouped[off + i];
-      __syncthreads();
-    }
-    // -- cache the cells' records ----------------------------------------
+  // -- activity test: does any cell of this partition need a move? ---------
+  if (MODE == MODE_PRIMAL && !pert) {
+    bool need = false;
     for (int i = lane; i < count; i += kWave) {
-      int c = vals[i];
-      CellRec r;
-      r.pos = (a.lx[c] & 0xffff) | (a.ly[c] << 16);
-      r.shape = a.w[c] | (a.h[c] << 12) | (a.gnd[c] << 18) | (a.code[c] << 19);
-      r.ox = a.ox[c];
-      r.oy = a.oy[c];
-      rec[i] = r;
+      int4 rc = cellrec[ids[i]];
+      int cx = rec_x(rc.x), cy = rec_y(rc.x), h = rec_h(rc.y);
+      need |= !correct_row(h, rec_gnd(rc.y), a.vdd_up[cy]) || ovf_hbm(a, cx, cy, rec_w(rc.y), h) != 0;
+    }
+    if (!__any(need)) {
+      // primal step is a no-op: stream the dual update over the pixels
+      unsigned long long ovf = 0;
+      int la = lane / g.yh, lb = lane - la * g.yh;
+      for (int i = lane; i < npix; i += kWave) {
+        int xx = g.gx(la), yy = g.gy(lb);
+        la += step_a; lb += step_b;
+        if (lb >= g.yh) { lb -= g.yh; la++; }
+        if (xx < 0 || yy < 0) continue;
+        size_t p = (size_t)xx * a.y_size + yy;
+        int ui = a.usage[p];
+        a.dual[p] = dual_step(ui, a.dual[p], rho, step, neg);
+        ovf += (unsigned long long)max(ui - 1, 0);
+      }
+      for (int o = 32; o > 0; o >>= 1) ovf += __shfl_down(ovf, o);
+      if (lane == 0) {
+        if (ovf) atomicAdd(&ctl->ovf_acc, ovf);
+        a.sorted[k] = 0;
+      }
+      PHASE_MARK(4);
+      return;
+    }
+  }
+
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  Tile T;
+  T.pix = reinterpret_cast<uint2*>(lds_raw);                   // [npix]
+  T.lam = reinterpret_cast<float*>(T.pix + npix);              // [npix]
+  T.vdd = reinterpret_cast<int*>(T.lam + ((npix + 3) & ~3));   // [yh]
+  int* keys = T.vdd + ((g.yh + 3) & ~3);                       // LDS sorts only
+  int* slots = keys + npad_hi;
+  int* aux = slots + npad_hi;
+
+  if (stage_tile(a, g, T, rho, standard)) ctl->error = kErrUsageRange;
+  for (int b = lane; b < g.yh; b += kWave) {
+    int yy = g.gy(b);
+    T.vdd[b] = yy >= 0 ? a.vdd_up[yy] : 0;
+  }
+  __syncthreads();
+  PHASE_MARK(0);
+
+  if (count > 0) {
+    int4* recs = reinterpret_cast<int4*>(a.rec_b) + off;
+    const int npad = ctl->npad;
+    const bool do_order = (MODE == MODE_PRIMAL) || a.sorted[k] == 0;
+    if (do_order) {
+      // an unsorted partition at the final refine moved nothing in the last
+      // iteration, so its usage still is the one the order is defined on
+      if (lds_sort) order_partition_lds(a, g, T, npad, npad_hi, off, count, cellrec, keys, slots, aux);
+      else order_partition_reg<kNpadSmall / kWave>(a, g, T, npad, off, count, cellrec);
+      if (MODE == MODE_PRIMAL && lane == 0) a.sorted[k] = 1;
+    } else {
+      for (int i = lane; i < count; i += kWave) recs[i] = cellrec[ids[i]];
     }
     __syncthreads();
+    PHASE_MARK(1);
 
-    const bool pert = iter < a.iter_perturbation_on;
-    const bool standard = (MODE == MODE_PRIMAL) && a.standard_admm;
-    // -- the sequential chain of cells ------------------------------------
-    for (int t = 0; t < count; t++) {
-      CellRec r = rec[t];
-      const int cx = r.pos & 0xffff, cy = (int)((unsigned)r.pos >> 16);
-      const int w = rec_w(r.shape), h = rec_h(r.shape), gnd = rec_gnd(r.shape), code = rec_code(r.shape);
-      if (MODE == MODE_PRIMAL) {
-        bool ok_row = correct_row(h, gnd, vdd_loc[g.ly(cy)]);
-        // computeCellOverflow (AdmmKernels.hpp:393-406)
-        int ov = 0;
-        int lx0 = g.lx(cx), ly0 = g.ly(cy);
-        for (int i = lane; i < w * h; i += kWave) {
-          int lxi = lx0 + i / h, lyi = ly0 + i % h;
-          if (lxi >= g.xh) lxi -= g.xh;
-          if (lyi >= g.yh) lyi -= g.yh;
-          int u = (int)(pix[lxi * g.yh + lyi].x & 0xffffu);
-          ov += max(u - 1, 0);
-        }
-        ov = wave_sum(ov);
-        if (!pert && ov == 0 && ok_row) continue;
-      } else {
-        if (h <= 1) continue;
-      }
-      float disp_coeff = 1.0f;
-      if (a.flag_iccad17) disp_coeff = float(a.N) / float(a.k_height_num[h - 1]);
-      footprint_add(g, pix, cx, cy, w, h, -1);  // rip-up
-      __syncthreads();
-      float cst[4];
-      int idx[4];
+    const float thr = disp_threshold(a);
+    // this lane's four candidate moves d = j*64 + lane, for the whole kernel
+    int2 dir[4];
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        int d = j * kWave + lane;
-        idx[j] = d;
-        if (d < kNumDirDefault) {
-          int nx, ny;
-          new_pos(a, c_dir_default[d], w, h, cx, cy, nx, ny);
-          float dc = disp_cost(a, disp_coeff, nx, ny, r.ox, r.oy);
-          float pc = penalty_lds<MODE>(a, g, pix, vdd_loc, rho, standard, nx, ny, w, h, code, gnd);
-          cst[j] = dc + pc;
-        } else {
-          cst[j] = kInf;
+    for (int j = 0; j < 4; j++) {
+      int d = j * kWave + lane;
+      dir[j] = d < kNumDirDefault ? c_dir_default[d] : make_int2(0, 0);
+    }
+    // -- the cells in order, 64 at a time ---------------------------------
+    for (int base = 0; base < count; base += kWave) {
+      const int t = base + lane;
+      int4 r = make_int4(0, 0, 0, 0);
+      int id = -1;
+      if (t < count) { r = recs[t]; id = ids[t]; }
+      const int my_x = rec_x(r.x), my_y = rec_y(r.x), my_w = rec_w(r.y), my_h = rec_h(r.y);
+      // skip test of computeCellOverflow + isPlacedInCorrectRow (AdmmKernels.hpp:722-738)
+      auto skip_test = [&]() -> bool {
+        if (t >= count) return false;
+        if (MODE == MODE_PRIMAL) {
+          if (pert) return true;
+          bool ok_row = correct_row(my_h, rec_gnd(r.y), T.vdd[g.ly(my_y)]);
+          return !ok_row || ovf_lds(g, T.pix, my_x, my_y, my_w, my_h) != 0;
+        }
+        return my_h > 1;
+      };
+      bool need = skip_test();
+      int start = 0;
+      for (;;) {
+        const unsigned long long m = __ballot(need && lane >= start);
+        if (m == 0ull) break;
+        const int first = __builtin_amdgcn_readfirstlane(__ffsll((long long)m) - 1);
+        const int px = __builtin_amdgcn_readlane(r.x, first), py = __builtin_amdgcn_readlane(r.y, first);
+        const int pox = __builtin_amdgcn_readlane(r.z, first), poy = __builtin_amdgcn_readlane(r.w, first);
+        const int pid = __builtin_amdgcn_readlane(id, first);
+        const int cx = rec_x(px), cy = rec_y(px);
+        const int w = rec_w(py), h = rec_h(py), gnd = rec_gnd(py), code = rec_code(py);
+        float disp_coeff = 1.0f;
+        if (a.flag_iccad17) disp_coeff = float(a.N) / float(a.k_height_num[h - 1]);
+#ifdef LG_PHASE_TIMERS
+        if (lane == 0) atomicAdd(&a.ctl->phase_ticks[7], 1ull);
+#endif
+        footprint_add(g, T, cx, cy, w, h, -1, rho, standard);  // rip-up
+        __syncthreads();
+        float cst[4];
+        eval_candidates<MODE>(a, g, T, disp_coeff, thr, cx, cy, w, h, code, gnd, pox, poy, dir, cst);
+        // Tree-min of AdmmKernels.hpp:847-858 over slots s = j*64 + lane: a
+        // halving-stride tournament in which a tie keeps the lower slot.  Its
+        // winner is the lexicographic minimum of (cost, bit-reverse8(slot)),
+        // since the last comparison decides on bit 0 of the slot, the one
+        // before on bit 1, ...  Costs are >= 0 or +inf, so their bit patterns
+        // order like the values.
+        unsigned long long best = ~0ull;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          unsigned slot = (unsigned)(j * kWave + lane);
+          unsigned rev = __brev(slot) >> 24;
+          unsigned long long key = ((unsigned long long)__float_as_uint(cst[j]) << 32) | rev;
+          best = key < best ? key : best;
+        }
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+          unsigned lo = (unsigned)best, hi = (unsigned)(best >> 32);
+          unsigned olo = __shfl_xor(lo, o), ohi = __shfl_xor(hi, o);
+          unsigned long long other = ((unsigned long long)ohi << 32) | olo;
+          best = other < best ? other : best;
+        }
+        const unsigned bc_bits = __builtin_amdgcn_readfirstlane((unsigned)(best >> 32));
+        const int bi = (int)(__builtin_amdgcn_readfirstlane((unsigned)best & 0xffu));
+        const int bslot = (int)(__brev((unsigned)bi) >> 24);
+        int nx = cx, ny = cy;
+        if (__uint_as_float(bc_bits) != kInf) {
+          int2 dd = c_dir_default[bslot];
+          new_pos(a, dd, w, h, cx, cy, nx, ny);
+        }
+        __syncthreads();
+        footprint_add(g, T, nx, ny, w, h, +1, rho, standard);  // commit
+        if (lane == 0) cellrec[pid].x = rec_pos(nx, ny);
+        __syncthreads();
+        start = first + 1;
+        if (start >= kWave) break;
+        // a move changes usage only inside the old and new footprints
+        if (lane >= start && t < count && MODE == MODE_PRIMAL && !pert) {
+          bool hit_old = my_x < cx + w && cx < my_x + my_w && my_y < cy + h && cy < my_y + my_h;
+          bool hit_new = my_x < nx + w && nx < my_x + my_w && my_y < ny + h && ny < my_y + my_h;
+          if (hit_old || hit_new) need = skip_test();
         }
       }
-      // tree-min of AdmmKernels.hpp:847-858: slot s = j*64 + lane
-      if (cst[0] > cst[2]) { cst[0] = cst[2]; idx[0] = idx[2]; }
-      if (cst[1] > cst[3]) { cst[1] = cst[3]; idx[1] = idx[3]; }
-      if (cst[0] > cst[1]) { cst[0] = cst[1]; idx[0] = idx[1]; }
-      float bc = cst[0];
-      int bi = idx[0];
-#pragma unroll
-      for (int s = 32; s > 0; s >>= 1) {
-        float oc = __shfl_down(bc, s);
-        int oi = __shfl_down(bi, s);
-        if (lane < s && bc > oc) { bc = oc; bi = oi; }
-      }
-      bc = wave_bcast0(bc);
-      bi = wave_bcast0(bi);
-      int nx = cx, ny = cy;
-      if (bc != kInf) new_pos(a, c_dir_default[bi], w, h, cx, cy, nx, ny);
-      __syncthreads();
-      footprint_add(g, pix, nx, ny, w, h, +1);  // commit
-      if (lane == 0) rec[t].pos = (nx & 0xffff) | (ny << 16);
-      __syncthreads();
     }
-    // -- write back cell positions ----------------------------------------
-    for (int i = lane; i < count; i += kWave) {
-      int c = vals[i];
-      int pos = rec[i].pos;
-      a.lx[c] = pos & 0xffff;
-      a.ly[c] = (int)((unsigned)pos >> 16);
-    }
+    PHASE_MARK(2);
   }
 
   // -- write back pixels; primal mode fuses the dual update and the overflow
   unsigned long long ovf = 0;
-  const float step = ctl->step;
-  const float neg = -1.0f / rho;
-  for (int i = lane; i < npix; i += kWave) {
-    int la = i / g.yh, lb = i - la * g.yh;
-    int xx = g.gx(la, a.x_size), yy = g.gy(lb, a.y_size);
-    if (xx < 0 || yy < 0) continue;
-    size_t p = (size_t)xx * a.y_size + yy;
-    uint2 e = pix[i];
-    int ui = (int)(e.x & 0xffffu);
-    a.usage[p] = ui;
-    if (MODE == MODE_PRIMAL) {
-      float u = float(ui);
-      float gg = fmax_ref(u - 1.0f, neg);
-      double q = (double)rho * 0.5;
-      q = q * (double)gg;
-      q = q * (double)gg;
-      float delta = (float)((double)gg + q);
-      float m = step * delta;
-      float l = __uint_as_float(e.y) + m;
-      a.dual[p] = fmax_ref(l, 0.0f);
-      ovf += (unsigned long long)max(ui - 1, 0);
+  {
+    int la = lane / g.yh, lb = lane - la * g.yh;
+    for (int i = lane; i < npix; i += kWave) {
+      int xx = g.gx(la), yy = g.gy(lb);
+      la += step_a; lb += step_b;
+      if (lb >= g.yh) { lb -= g.yh; la++; }
+      if (xx < 0 || yy < 0) continue;
+      size_t p = (size_t)xx * a.y_size + yy;
+      int ui = (int)(T.pix[i].x & 0xffffu);
+      a.usage[p] = ui;
+      if (MODE == MODE_PRIMAL) {
+        a.dual[p] = dual_step(ui, T.lam[i], rho, step, neg);
+        ovf += (unsigned long long)max(ui - 1, 0);
+      }
     }
   }
   if (MODE == MODE_PRIMAL) {
     for (int o = 32; o > 0; o >>= 1) ovf += __shfl_down(ovf, o);
     if (lane == 0 && ovf) atomicAdd(&ctl->ovf_acc, ovf);
   }
+  PHASE_MARK(3);
+}
+
+// Cell records <-> SoA positions (records hold positions during the solve)
+__global__ void k_pack_rec(Args a) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.N) return;
+  int4 r;
+  r.x = rec_pos(a.lx[c], a.ly[c]);
+  r.y = a.w[c] | (a.h[c] << 12) | (a.gnd[c] << 18) | (a.code[c] << 19);
+  r.z = a.ox[c];
+  r.w = a.oy[c];
+  reinterpret_cast<int4*>(a.cellrec)[c] = r;
+}
+
+__global__ void k_unpack_pos(Args a) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.N) return;
+  int p = a.cellrec[4 * c];
+  a.lx[c] = rec_x(p);
+  a.ly[c] = rec_y(p);
 }
 
 // examineIteration + detectSlowConvergence + convergence test
@@ -767,9 +1144,11 @@ __global__ void k_extract_congested(Args a) {
   if (ctl->done || !a.use_adaptive || !ctl->exceptional) return;
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= a.N) return;
+  int p0 = a.cellrec[4 * c];
+  int cx = p0 & 0xffff, cy = (int)((unsigned)p0 >> 16);
   int s = 0;
-  for (int x = a.lx[c]; x < a.lx[c] + a.w[c]; x++)
-    for (int y = a.ly[c]; y < a.ly[c] + a.h[c]; y++) s += max(a.usage[(size_t)x * a.y_size + y] - 1, 0);
+  for (int x = cx; x < cx + a.w[c]; x++)
+    for (int y = cy; y < cy + a.h[c]; y++) s += max(a.usage[(size_t)x * a.y_size + y] - 1, 0);
   a.ovf_flag[c] = s == 0 ? 0 : 1;
 }
 
@@ -826,7 +1205,8 @@ __global__ __launch_bounds__(1024) void k_adaptive(Args a) {
   const float rho = ctl->rho;
   for (int i = 0; i < n; i++) {
     int c = a.ovf_list[i];
-    int cx = a.lx[c], cy = a.ly[c], w = a.w[c], h = a.h[c];
+    int p0 = a.cellrec[4 * c];
+    int cx = p0 & 0xffff, cy = (int)((unsigned)p0 >> 16), w = a.w[c], h = a.h[c];
     if (t == 0) {
       int s = 0;
       for (int x = cx; x < cx + w; x++)
@@ -845,7 +1225,7 @@ __global__ __launch_bounds__(1024) void k_adaptive(Args a) {
       if (d < kNumDirWide) {
         int nx, ny;
         proj_pos(a, d, w, h, cx, cy, nx, ny);
-        float dc = disp_cost(a, disp_coeff, nx, ny, a.ox[c], a.oy[c]);
+        float dc = disp_cost(a, disp_coeff, disp_threshold(a), nx, ny, a.ox[c], a.oy[c]);
         float pc;
         if (nx + w > a.x_size || ny + h > a.y_size || nx < 0 || ny < 0) pc = kInf;
         else if (!correct_row(h, gnd, a.vdd_up[ny])) pc = kInf;
@@ -879,7 +1259,7 @@ __global__ __launch_bounds__(1024) void k_adaptive(Args a) {
     if (s_cost[0] != kInf) proj_pos(a, s_idx[0], w, h, cx, cy, nx, ny);
     __syncthreads();
     for (int q = t; q < w * h; q += 1024) a.usage[(size_t)(nx + q / h) * a.y_size + ny + q % h] += 1;
-    if (t == 0) { a.lx[c] = nx; a.ly[c] = ny; }
+    if (t == 0) a.cellrec[4 * c] = (nx & 0xffff) | (ny << 16);
     __syncthreads();
   }
   if (t == 0) ctl->adaptive_calls++;
