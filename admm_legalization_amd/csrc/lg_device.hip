@@ -87,6 +87,14 @@ struct LgHandle {
   LgVioStat vio{};
   int checked = 0;
   float coeff_init_rho, coeff_init_dual;
+  int32_t *init_lx = nullptr, *init_ly = nullptr;  // positions given at lg_create
+  // partition-kernel timing (lg_profile_partition): three events per
+  // iteration bracket the two LDS-class launches
+  int prof = 0;
+  std::vector<hipEvent_t> ev;
+  int ev_used = 0;
+  double prof_ms[2] = {0, 0};
+  int64_t prof_n[2] = {0, 0};
 };
 
 namespace {
@@ -127,19 +135,44 @@ size_t partition_lds_bytes(const LgHandle* h, int npad_hi, int lds_sort) {
 template <int MODE>
 int launch_partition(LgHandle* h, hipStream_t s) {
   const int reg_ok = h->p.sorting_algorithm == LG_SORT_M_BITONIC || h->p.sorting_algorithm == LG_SORT_S_BITONIC;
-  const int classes[2][3] = {{0, kNpadSmall, reg_ok ? 0 : 1}, {kNpadSmall + 1, kMaxNpad, 1}};
-  for (auto& c : classes) {
-    size_t lds = partition_lds_bytes(h, c[1], c[2]);
-    hipLaunchKernelGGL(k_partition<MODE>, dim3(h->a.K), dim3(kWave), lds, s, h->a, c[0], c[1], c[2]);
+  const bool prof = h->prof && MODE == MODE_PRIMAL && h->ev_used + 3 <= (int)h->ev.size();
+  hipEvent_t* e = prof ? &h->ev[h->ev_used] : nullptr;
+  if (prof) HIP_TRY(hipEventRecord(e[0], s));
+  // small class: register sorts when the algorithm is a bitonic network
+  size_t lds0 = partition_lds_bytes(h, kNpadSmall, reg_ok ? 0 : 1);
+  if (reg_ok)
+    hipLaunchKernelGGL((k_partition<MODE, 0>), dim3(h->a.K), dim3(kWave), lds0, s, h->a, 0, kNpadSmall);
+  else
+    hipLaunchKernelGGL((k_partition<MODE, 1>), dim3(h->a.K), dim3(kWave), lds0, s, h->a, 0, kNpadSmall);
+  if (prof) HIP_TRY(hipEventRecord(e[1], s));
+  size_t lds1 = partition_lds_bytes(h, kMaxNpad, 1);
+  hipLaunchKernelGGL((k_partition<MODE, 1>), dim3(h->a.K), dim3(kWave), lds1, s, h->a, kNpadSmall + 1, kMaxNpad);
+  if (prof) {
+    HIP_TRY(hipEventRecord(e[2], s));
+    h->ev_used += 3;
   }
   HIP_TRY(hipGetLastError());
+  return LG_OK;
+}
+
+// Fold the recorded partition-launch events into the totals (stream synced).
+int collect_profile(LgHandle* h) {
+  for (int i = 0; i + 3 <= h->ev_used; i += 3) {
+    for (int c = 0; c < 2; c++) {
+      float ms = 0;
+      HIP_TRY(hipEventElapsedTime(&ms, h->ev[i + c], h->ev[i + c + 1]));
+      h->prof_ms[c] += ms;
+      h->prof_n[c]++;
+    }
+  }
+  h->ev_used = 0;
   return LG_OK;
 }
 
 int read_ctl(LgHandle* h, hipStream_t s) {
   HIP_TRY(hipMemcpyAsync(&h->host_ctl, h->a.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
-  return LG_OK;
+  return h->prof ? collect_profile(h) : LG_OK;
 }
 
 }  // namespace
@@ -242,8 +275,10 @@ int lg_create(const LgGridDesc* g, const LgCellsSoA* c, const LgParams* p, LgHan
   if ((st = halloc(h, &a.dual, P))) return bail(st);
 
   int32_t *d_w, *d_h, *d_code, *d_gnd, *d_ox, *d_oy, *d_rp, *d_khn;
-  if ((st = upload(h, &a.lx, lxg.data(), N))) return bail(st);
-  if ((st = upload(h, &a.ly, lyg.data(), N))) return bail(st);
+  if ((st = upload(h, &h->init_lx, lxg.data(), N))) return bail(st);
+  if ((st = upload(h, &h->init_ly, lyg.data(), N))) return bail(st);
+  if ((st = halloc(h, &a.lx, N))) return bail(st);
+  if ((st = halloc(h, &a.ly, N))) return bail(st);
   if ((st = upload(h, &d_w, c->w_grid, N))) return bail(st);
   if ((st = upload(h, &d_h, c->h_grid, N))) return bail(st);
   if ((st = upload(h, &d_code, code.data(), N))) return bail(st);
@@ -308,6 +343,13 @@ int lg_solve(LgHandle* h, void* stream) {
   init.last_slow = -1;
   init.npad = 1;
   HIP_TRY(hipMemcpyAsync(a.ctl, &init, sizeof(Ctl), hipMemcpyHostToDevice, s));
+  // every solve starts from the positions given at lg_create
+  if (N > 0) {
+    HIP_TRY(hipMemcpyAsync(a.lx, h->init_lx, N * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipMemcpyAsync(a.ly, h->init_ly, N * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipMemsetAsync(a.need_rot, 0, N * sizeof(int32_t), s));
+  }
+  h->posted = 0;
   HIP_TRY(hipMemsetAsync(a.usage, 0, h->P * sizeof(int32_t), s));
   if (N > 0) {
     hipLaunchKernelGGL(k_usage_scatter, dim3(nb), dim3(256), 0, s, a);
@@ -455,8 +497,31 @@ int lg_debug_phase_ticks(LgHandle* h, uint64_t* out, int n) {
   return LG_OK;
 }
 
+int lg_profile_partition(LgHandle* h, int enable) {
+  if (!h) return fail(LG_ERR_ARG, "null handle");
+  if (enable && h->ev.empty()) {
+    h->ev.resize(3 * 64);
+    for (auto& e : h->ev) HIP_TRY(hipEventCreate(&e));
+  }
+  h->prof = enable ? 1 : 0;
+  h->ev_used = 0;
+  h->prof_ms[0] = h->prof_ms[1] = 0;
+  h->prof_n[0] = h->prof_n[1] = 0;
+  return LG_OK;
+}
+
+int lg_profile_read(LgHandle* h, double* ms, int64_t* launches) {
+  if (!h) return fail(LG_ERR_ARG, "null handle");
+  for (int c = 0; c < 2; c++) {
+    if (ms) ms[c] = h->prof_ms[c];
+    if (launches) launches[c] = h->prof_n[c];
+  }
+  return LG_OK;
+}
+
 int lg_destroy(LgHandle* h) {
   if (!h) return LG_OK;
+  for (auto& e : h->ev) (void)hipEventDestroy(e);
   for (void* b : h->bufs) (void)hipFree(b);
   if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
   delete h;

@@ -846,10 +846,10 @@ __device__ __forceinline__ float dual_step(int ui, float l, float rho, float ste
 // footprint, which is exactly the sequential decision because a skipped cell
 // changes nothing.
 //
-// npad_lo..npad_hi selects the launch (LDS size class); lds_sort launches keep
-// the sort buffers in LDS.
-template <int MODE>
-__global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_hi, int lds_sort) {
+// npad_lo..npad_hi selects the launch (LDS size class); LDS_SORT launches keep
+// the sort buffers in LDS, the others sort in registers (npad <= 1024).
+template <int MODE, int LDS_SORT>
+__global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_hi) {
   Ctl* ctl = a.ctl;
   int iter;
   const bool adaptive_now = (MODE == MODE_PRIMAL) && a.use_adaptive && ctl->exceptional;
@@ -945,7 +945,7 @@ __global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_
     if (do_order) {
       // an unsorted partition at the final refine moved nothing in the last
       // iteration, so its usage still is the one the order is defined on
-      if (lds_sort) order_partition_lds(a, g, T, npad, npad_hi, off, count, cellrec, keys, slots, aux);
+      if (LDS_SORT) order_partition_lds(a, g, T, npad, npad_hi, off, count, cellrec, keys, slots, aux);
       else order_partition_reg<kNpadSmall / kWave>(a, g, T, npad, off, count, cellrec);
       if (MODE == MODE_PRIMAL && lane == 0) a.sorted[k] = 1;
     } else {

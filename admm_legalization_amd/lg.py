@@ -27,7 +27,8 @@ _i32p = C.POINTER(C.c_int32)
 
 EXPORTED = ["lg_default_params", "lg_last_error", "lg_run_design", "lg_import_and_preplace",
             "lg_create", "lg_solve", "lg_postprocess", "lg_check", "lg_copy_out", "lg_get_stats",
-            "lg_get_pixel_state", "lg_get_ovf_trace", "lg_destroy"]
+            "lg_get_pixel_state", "lg_get_ovf_trace", "lg_destroy", "lg_profile_partition",
+            "lg_profile_read", "lg_debug_phase_ticks"]
 
 
 class LegalizerError(RuntimeError):
@@ -55,6 +56,9 @@ def lib():
         L.lg_get_pixel_state.argtypes = [C.c_void_p, _i32p, C.POINTER(C.c_float)]
         L.lg_get_ovf_trace.argtypes = [C.c_void_p, _i32p, C.c_int32, _i32p]
         L.lg_destroy.argtypes = [C.c_void_p]
+        L.lg_profile_partition.argtypes = [C.c_void_p, C.c_int]
+        L.lg_profile_read.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+        L.lg_debug_phase_ticks.argtypes = [C.c_void_p, C.POINTER(C.c_uint64), C.c_int]
         _lib = L
     return _lib
 
@@ -200,11 +204,24 @@ class DeviceLegalizer:
         _check(lib().lg_create(C.byref(g), C.byref(c), C.byref(params), C.byref(h)))
         self.h = h
 
-    def solve(self):
-        _check(lib().lg_solve(self.h, None))
+    def solve(self, stream: int | None = None):
+        """Runs the ADMM loop from the positions given at construction
+        (repeatable).  `stream` is a raw hipStream_t handle or None for the
+        handle's own stream."""
+        _check(lib().lg_solve(self.h, stream))
 
-    def postprocess(self):
-        _check(lib().lg_postprocess(self.h, None))
+    def postprocess(self, stream: int | None = None):
+        _check(lib().lg_postprocess(self.h, stream))
+
+    def profile_partition(self, enable: bool = True):
+        _check(lib().lg_profile_partition(self.h, int(bool(enable))))
+
+    def profile_read(self):
+        """(ms[2], launches[2]) of the partition kernel per LDS class."""
+        ms = (C.c_double * 2)()
+        n = (C.c_int64 * 2)()
+        _check(lib().lg_profile_read(self.h, ms, n))
+        return list(ms), list(n)
 
     def check(self) -> dict:
         v = LgVioStat()

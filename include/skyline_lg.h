@@ -237,6 +237,17 @@ int lg_get_pixel_state(LgHandle* h, int32_t* usage, float* dual);
 int lg_get_ovf_trace(LgHandle* h, int32_t* trace, int32_t cap, int32_t* n);
 int lg_destroy(LgHandle* h);
 
+/* Instrumentation; no counterpart in the reference.  With enable != 0 every
+ * later lg_solve brackets each ADMM iteration's partition-kernel launches
+ * (small LDS class, large LDS class) with HIP events on the solve's stream;
+ * lg_profile_read returns the summed milliseconds and launch counts per class
+ * since the last enable.  Adds three event records per iteration. */
+int lg_profile_partition(LgHandle* h, int enable);
+int lg_profile_read(LgHandle* h, double* ms /*[2]*/, int64_t* launches /*[2]*/);
+/* Per-phase clock ticks of a build compiled with LG_PHASE_TIMERS (zeros
+ * otherwise); no counterpart in the reference. */
+int lg_debug_phase_ticks(LgHandle* h, uint64_t* ticks, int n);
+
 #ifdef __cplusplus
 }
 #endif

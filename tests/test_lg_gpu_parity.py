@@ -124,3 +124,26 @@ def test_adaptive_search(gpu):
     g, o = _run_both(d, p, do_post=False)
     _assert_same(g, o)
     assert g["stats"]["num_adaptive_calls"] == o["stats"]["num_adaptive_calls"]
+
+
+def test_repeated_solve_is_idempotent(gpu):
+    """lg_solve restarts from the positions given at lg_create: two solves on
+    one handle (with post-process and a check between) give the same result,
+    and both equal the oracle."""
+    d = synth.mixed(3000, seed=4)
+    p = lg.default_params()
+    grid, cells, p2 = lg.device_inputs(d, p)
+    dev = lg.DeviceLegalizer(grid, cells, p2)
+    runs = []
+    for _ in range(2):
+        dev.solve()
+        dev.postprocess()
+        dev.check()
+        runs.append((dev.positions(), dev.stats()))
+    dev.close()
+    o = ref.solve(grid, cells, p2, do_post=True)
+    for (lx, ly, rot), st in runs:
+        np.testing.assert_array_equal(lx, o["lx"])
+        np.testing.assert_array_equal(ly, o["ly"])
+        np.testing.assert_array_equal(rot, o["rot"])
+        assert st["iterations"] == o["stats"]["iterations"]
