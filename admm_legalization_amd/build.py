@@ -29,6 +29,7 @@ UNITS = [
     ("lg_device.hip", "hip"),
     ("lg_host.cpp", "cxx"),
     ("dens_device.hip", "hip"),
+    ("dens_host.cpp", "cxx"),
 ]
 
 

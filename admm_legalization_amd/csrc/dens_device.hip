@@ -1,0 +1,769 @@
+// Electrostatic density path on MI355X: tiled LDS scatter, overflow, Poisson
+// solve as cosine / sine GEMMs on MFMA, gather.  C ABI in skyline_density.h.
+//
+// Data layout in HBM (bins y-major, bin = x + y * nbx):
+//   cgeo[N]   float4 {density width, density height, scatter scale, gather scale}
+//   cfill[N]  int    filler flag
+//   slab[T][2][SW*SW]  per-tile movable / filler partial areas (SW = 32 + halo)
+//   mov, fil, dens, psi, ex, ey, lambda, fixed, scaled  [B] float
+//
+// Scatter (computeDensityContributionOfEachCell, DensityGradient.cu:61-138):
+// cells are bucketed by the 32x32-bin tile holding their lower-left footprint
+// bin; one workgroup per tile accumulates its cells in LDS (tile plus a halo
+// of H bins up and right), then writes the tile's slab with plain stores; a
+// per-bin resolve pass adds a bin's own slab entry and the halo entries of the
+// (up to three) tiles below / left of it, and computes density and overflow in
+// the same pass.  Cells wider than the halo go to a separate list whose bins
+// are added with global atomics into the slabs before the resolve pass.
+//
+// Poisson (PoissonSolver::solvePoisson, PoissonSolver.cu:224-302): with
+// M = nby rows, N = nbx columns and k-contiguous operands (out = A * B^T):
+//   U[N x M]   = CN   * rho^T                                (CN[j,n] = cos(pi j (2n+1)/2N))
+//   A'[M x N]  = 4/(MN) CM * U^T, divided by wy^2 + wx^2,  A'[0,0] = 0
+//   [V;Vx][2N x M] = [DNt; SNw] * A'^T
+//   [psi;Ey][2M x N] = [DMt; SMw] * V^T,   Ex[M x N] = DMt * Vx^T
+// with DNt[n,w] = c_w cos(pi w (2n+1)/2N) (c_0 = 1, else 2), SNw[n,w] =
+// 2 sin(pi w (2n+1)/2N) * 2 pi w / N (w >= 1), and the same along y.  These are
+// the reference's DCT-II (scaled 4/(MN)), DCT-III, and IDXST transforms with
+// multiplyW folded into the sine matrices.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/skyline_density.h"
+
+namespace densdev {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define DENS_TRY(x)                                                              \
+  do {                                                                           \
+    hipError_t e_ = (x);                                                         \
+    if (e_ != hipSuccess)                                                        \
+      return fail(DENS_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int kTile = 32;   // bins per tile side
+constexpr int kMaxHalo = 8;
+constexpr int kBK = 32;     // GEMM k-step
+
+// ---------------------------------------------------------------------------
+// GEMM: C = alpha * A * B^T, A [R x K], B [Cn x K], both k-contiguous.  Rows
+// below `split` go to C0, the others to C1.  Optional epilogue divides by
+// wy2[row] + wx2[col] (zero at (0,0)).  256 threads = 2 x 2 waves, each wave
+// owns a (BM/2) x (BN/2) block of 32x32 MFMA accumulators.
+// ---------------------------------------------------------------------------
+struct Gemm {
+  const float* A;
+  const float* B;
+  float* C0;
+  float* C1;
+  int lda, ldb, ldc, split;
+  int R, Cn, K;
+  float alpha;
+  const float* wy2;
+  const float* wx2;
+};
+
+__device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
+  hi = (__bf16)x;
+  lo = (__bf16)(x - (float)hi);
+}
+
+template <int PREC, int BM, int BN>
+__global__ __launch_bounds__(256) void k_gemm_nt(Gemm g) {
+  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int FM = WM / 32, FN = WN / 32;
+  constexpr int LD16 = kBK + 8;  // bf16 row stride (80 B: conflict-free 16-B reads)
+  constexpr int LDF = kBK + 4;   // f32 row stride
+  constexpr int AF4 = BM * kBK / 4 / 256;
+  constexpr int BF4 = BN * kBK / 4 / 256;
+  constexpr bool F32 = PREC == DENS_PREC_F32;
+  constexpr bool SPLIT = PREC == DENS_PREC_SPLIT_BF16;
+
+  extern __shared__ __align__(16) unsigned char smem[];
+  __bf16* sAh = reinterpret_cast<__bf16*>(smem);
+  __bf16* sAl = sAh + BM * LD16;
+  __bf16* sBh = sAl + BM * LD16;
+  __bf16* sBl = sBh + BN * LD16;
+  float* sAf = reinterpret_cast<float*>(smem);
+  float* sBf = sAf + BM * LDF;
+
+  const int tiles_n = (g.Cn + BN - 1) / BN;
+  const int tm = blockIdx.x / tiles_n, tn = blockIdx.x - tm * tiles_n;
+  const int r0 = tm * BM, c0 = tn * BN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+
+  float4 ra[AF4], rb[BF4];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int i = 0; i < AF4; i++) {
+      int f = tid + i * 256, row = f / (kBK / 4), kq = f - row * (kBK / 4);
+      int gr = r0 + row, gk = k0 + 4 * kq;
+      ra[i] = (gr < g.R && gk < g.K) ? *reinterpret_cast<const float4*>(g.A + (size_t)gr * g.lda + gk)
+                                     : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int i = 0; i < BF4; i++) {
+      int f = tid + i * 256, row = f / (kBK / 4), kq = f - row * (kBK / 4);
+      int gr = c0 + row, gk = k0 + 4 * kq;
+      rb[i] = (gr < g.Cn && gk < g.K) ? *reinterpret_cast<const float4*>(g.B + (size_t)gr * g.ldb + gk)
+                                      : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  };
+  auto put16 = [&](__bf16* hi, __bf16* lo, int row, int kq, float4 v) {
+    bf16x4 h, l;
+    __bf16 a, b;
+    split_bf16(v.x, a, b); h[0] = a; l[0] = b;
+    split_bf16(v.y, a, b); h[1] = a; l[1] = b;
+    split_bf16(v.z, a, b); h[2] = a; l[2] = b;
+    split_bf16(v.w, a, b); h[3] = a; l[3] = b;
+    *reinterpret_cast<bf16x4*>(hi + row * LD16 + 4 * kq) = h;
+    if (SPLIT) *reinterpret_cast<bf16x4*>(lo + row * LD16 + 4 * kq) = l;
+  };
+  auto sstore = [&]() {
+#pragma unroll
+    for (int i = 0; i < AF4; i++) {
+      int f = tid + i * 256, row = f / (kBK / 4), kq = f - row * (kBK / 4);
+      if (F32) *reinterpret_cast<float4*>(sAf + row * LDF + 4 * kq) = ra[i];
+      else put16(sAh, sAl, row, kq, ra[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < BF4; i++) {
+      int f = tid + i * 256, row = f / (kBK / 4), kq = f - row * (kBK / 4);
+      if (F32) *reinterpret_cast<float4*>(sBf + row * LDF + 4 * kq) = rb[i];
+      else put16(sBh, sBl, row, kq, rb[i]);
+    }
+  };
+
+  f32x16 acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; i++)
+#pragma unroll
+    for (int j = 0; j < FN; j++)
+#pragma unroll
+      for (int r = 0; r < 16; r++) acc[i][j][r] = 0.f;
+
+  const int l32 = lane & 31, lhi = lane >> 5;
+  gload(0);
+  for (int k0 = 0; k0 < g.K; k0 += kBK) {
+    __syncthreads();
+    sstore();
+    __syncthreads();
+    if (k0 + kBK < g.K) gload(k0 + kBK);
+    if (F32) {
+#pragma unroll
+      for (int kk = 0; kk < kBK; kk += 2) {
+        float a[FM], b[FN];
+#pragma unroll
+        for (int i = 0; i < FM; i++) a[i] = sAf[(wr * WM + i * 32 + l32) * LDF + kk + lhi];
+#pragma unroll
+        for (int j = 0; j < FN; j++) b[j] = sBf[(wc * WN + j * 32 + l32) * LDF + kk + lhi];
+#pragma unroll
+        for (int i = 0; i < FM; i++)
+#pragma unroll
+          for (int j = 0; j < FN; j++)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i], b[j], acc[i][j], 0, 0, 0);
+      }
+    } else {
+#pragma unroll
+      for (int kk = 0; kk < kBK; kk += 16) {
+        bf16x8 ah[FM], al[FM], bh[FN], bl[FN];
+#pragma unroll
+        for (int i = 0; i < FM; i++) {
+          int off = (wr * WM + i * 32 + l32) * LD16 + kk + 8 * lhi;
+          ah[i] = *reinterpret_cast<const bf16x8*>(sAh + off);
+          if (SPLIT) al[i] = *reinterpret_cast<const bf16x8*>(sAl + off);
+        }
+#pragma unroll
+        for (int j = 0; j < FN; j++) {
+          int off = (wc * WN + j * 32 + l32) * LD16 + kk + 8 * lhi;
+          bh[j] = *reinterpret_cast<const bf16x8*>(sBh + off);
+          if (SPLIT) bl[j] = *reinterpret_cast<const bf16x8*>(sBl + off);
+        }
+#pragma unroll
+        for (int i = 0; i < FM; i++)
+#pragma unroll
+          for (int j = 0; j < FN; j++) {
+            if (SPLIT) {
+              acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+              acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+            }
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+          }
+      }
+    }
+  }
+
+  // accumulator layout of a 32x32 block: lane l, register r ->
+  // row 8*(r/4) + 4*(l/32) + r%4, column l%32
+#pragma unroll
+  for (int i = 0; i < FM; i++)
+#pragma unroll
+    for (int j = 0; j < FN; j++)
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        int row = r0 + wr * WM + i * 32 + 8 * (r >> 2) + 4 * lhi + (r & 3);
+        int col = c0 + wc * WN + j * 32 + l32;
+        if (row < g.R && col < g.Cn) {
+          float v = g.alpha * acc[i][j][r];
+          if (g.wy2) v = (row == 0 && col == 0) ? 0.f : v / (g.wy2[row] + g.wx2[col]);
+          float* dst = row < g.split ? g.C0 + (size_t)row * g.ldc : g.C1 + (size_t)(row - g.split) * g.ldc;
+          dst[col] = v;
+        }
+      }
+}
+
+template <int PREC, int BM, int BN>
+constexpr size_t gemm_lds() {
+  return PREC == DENS_PREC_F32 ? (size_t)(BM + BN) * (kBK + 4) * 4 : (size_t)(BM + BN) * (kBK + 8) * 2 * 2;
+}
+
+template <int PREC>
+int launch_gemm_p(const Gemm& g, hipStream_t s) {
+  const long tiles128 = (long)((g.R + 127) / 128) * ((g.Cn + 127) / 128);
+  if (tiles128 >= 512) {
+    int grid = (int)tiles128;
+    const size_t lds = gemm_lds<PREC, 128, 128>();
+    hipLaunchKernelGGL((k_gemm_nt<PREC, 128, 128>), dim3(grid), dim3(256), lds, s, g);
+  } else {
+    int grid = ((g.R + 63) / 64) * ((g.Cn + 63) / 64);
+    const size_t lds = gemm_lds<PREC, 64, 64>();
+    hipLaunchKernelGGL((k_gemm_nt<PREC, 64, 64>), dim3(grid), dim3(256), lds, s, g);
+  }
+  DENS_TRY(hipGetLastError());
+  return DENS_OK;
+}
+
+int launch_gemm(int prec, const Gemm& g, hipStream_t s) {
+  if (g.K % 4 || g.lda % 4 || g.ldb % 4) return fail(DENS_ERR_ARG, "GEMM operands must be multiples of 4 wide");
+  switch (prec) {
+    case DENS_PREC_BF16: return launch_gemm_p<DENS_PREC_BF16>(g, s);
+    case DENS_PREC_SPLIT_BF16: return launch_gemm_p<DENS_PREC_SPLIT_BF16>(g, s);
+    case DENS_PREC_F32: return launch_gemm_p<DENS_PREC_F32>(g, s);
+  }
+  return fail(DENS_ERR_ARG, "unknown precision");
+}
+
+// ---------------------------------------------------------------------------
+// scatter / resolve / gather
+// ---------------------------------------------------------------------------
+struct Geo {
+  int nbx, nby, tiles_x, T, H, SW, N;
+  float die_lx, die_ly, bin_w, bin_h;
+  const float *bin_lx, *bin_ux, *bin_ly, *bin_uy;
+};
+
+// findBinWithDensitySize (DensityGradient.cu:20-59)
+__device__ __forceinline__ void bin_range(const Geo& G, float cx, float cy, float dw, float dh, int& x0,
+                                          int& x1, int& y0, int& y1) {
+  float lx = cx - dw / 2, ux = cx + dw / 2;
+  float ly = cy - dh / 2, uy = cy + dh / 2;
+  x0 = max((int)floorf((lx - G.die_lx) / G.bin_w), 0);
+  x1 = min((int)ceilf((ux - G.die_lx) / G.bin_w), G.nbx);
+  y0 = max((int)floorf((ly - G.die_ly) / G.bin_h), 0);
+  y1 = min((int)ceilf((uy - G.die_ly) / G.bin_h), G.nby);
+}
+
+__device__ __forceinline__ float overlap(float a0, float a1, float b0, float b1) {
+  return fmaxf(0.f, fminf(a1, b1) - fmaxf(a0, b0));
+}
+
+__global__ void k_dens_reset(int* cnt, int n, double* ovf) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) cnt[i] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *ovf = 0.0;
+}
+
+// bucket of each cell: its tile, T for cells wider than the halo, -1 for
+// cells outside the die; rank within the bucket
+__global__ void k_dens_count(Geo G, const float* cx, const float* cy, const float4* cgeo, int* cnt,
+                             int* bucket, int* rank) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= G.N) return;
+  float4 c = cgeo[i];
+  int x0, x1, y0, y1;
+  bin_range(G, cx[i], cy[i], c.x, c.y, x0, x1, y0, y1);
+  int b = -1;
+  if (x1 > x0 && y1 > y0) {
+    bool fits = (x1 - x0) <= G.H + 1 && (y1 - y0) <= G.H + 1;
+    b = fits ? (x0 / kTile) + (y0 / kTile) * G.tiles_x : G.T;
+  }
+  bucket[i] = b;
+  if (b >= 0) rank[i] = atomicAdd(&cnt[b], 1);
+}
+
+// exclusive scan of cnt[0..n) into off[0..n] (one workgroup of 1024)
+__global__ __launch_bounds__(1024) void k_dens_scan(const int* cnt, int* off, int n) {
+  __shared__ int part[1024];
+  const int t = threadIdx.x;
+  const int per = (n + 1023) / 1024;
+  const int b = t * per, e = min(b + per, n);
+  int s = 0;
+  for (int i = b; i < e; i++) s += cnt[i];
+  part[t] = s;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    int v = t >= o ? part[t - o] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = part[t] - s;
+  for (int i = b; i < e; i++) {
+    off[i] = run;
+    run += cnt[i];
+  }
+  if (t == 1023) off[n] = part[1023];
+}
+
+__global__ void k_dens_fill(Geo G, const float* cx, const float* cy, const float4* cgeo, const int* cfill,
+                            const int* bucket, const int* rank, const int* off, float4* rec_box, float2* rec_s) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= G.N) return;
+  int b = bucket[i];
+  if (b < 0) return;
+  float4 c = cgeo[i];
+  float x = cx[i], y = cy[i];
+  int p = off[b] + rank[i];
+  rec_box[p] = make_float4(x, y, c.x, c.y);
+  rec_s[p] = make_float2(c.z, cfill ? (float)cfill[i] : 0.f);
+}
+
+// one workgroup per tile: LDS accumulation of the tile plus halo, slab write
+__global__ __launch_bounds__(256) void k_dens_accum(Geo G, const int* off, const float4* rec_box,
+                                                    const float2* rec_s, float* slab) {
+  extern __shared__ float sm[];  // [2][SW*SW]
+  const int t = blockIdx.x;
+  const int SW = G.SW, S2 = SW * SW;
+  for (int e = threadIdx.x; e < 2 * S2; e += blockDim.x) sm[e] = 0.f;
+  __syncthreads();
+  const int tx0 = (t % G.tiles_x) * kTile, ty0 = (t / G.tiles_x) * kTile;
+  const int b = off[t], e = off[t + 1];
+  for (int p = b + threadIdx.x; p < e; p += blockDim.x) {
+    float4 c = rec_box[p];
+    float2 s = rec_s[p];
+    int x0, x1, y0, y1;
+    bin_range(G, c.x, c.y, c.z, c.w, x0, x1, y0, y1);
+    float lx = c.x - c.z / 2, ux = c.x + c.z / 2;
+    float ly = c.y - c.w / 2, uy = c.y + c.w / 2;
+    float* dst = sm + (s.y != 0.f ? S2 : 0);
+    for (int bx = x0; bx < x1; bx++) {
+      float ox = overlap(lx, ux, G.bin_lx[bx], G.bin_ux[bx]);
+      for (int by = y0; by < y1; by++) {
+        float oy = overlap(ly, uy, G.bin_ly[by], G.bin_uy[by]);
+        atomicAdd(&dst[(by - ty0) * SW + (bx - tx0)], ox * oy * s.x);
+      }
+    }
+  }
+  __syncthreads();
+  float* out = slab + (size_t)t * 2 * S2;
+  for (int q = threadIdx.x; q < 2 * S2; q += blockDim.x) out[q] = sm[q];
+}
+
+// cells wider than the halo: global atomics into the owning tiles' slabs
+__global__ __launch_bounds__(256) void k_dens_large(Geo G, const int* off, const float4* rec_box,
+                                                    const float2* rec_s, float* slab) {
+  const int b = off[G.T], e = off[G.T + 1];
+  const int SW = G.SW, S2 = SW * SW;
+  for (int p = b + blockIdx.x; p < e; p += gridDim.x) {
+    float4 c = rec_box[p];
+    float2 s = rec_s[p];
+    int x0, x1, y0, y1;
+    bin_range(G, c.x, c.y, c.z, c.w, x0, x1, y0, y1);
+    float lx = c.x - c.z / 2, ux = c.x + c.z / 2;
+    float ly = c.y - c.w / 2, uy = c.y + c.w / 2;
+    const int nx = x1 - x0, cells = nx * (y1 - y0);
+    for (int q = threadIdx.x; q < cells; q += blockDim.x) {
+      int bx = x0 + q % nx, by = y0 + q / nx;
+      float v = overlap(lx, ux, G.bin_lx[bx], G.bin_ux[bx]) * overlap(ly, uy, G.bin_ly[by], G.bin_uy[by]) * s.x;
+      int t = bx / kTile + (by / kTile) * G.tiles_x;
+      float* dst = slab + (size_t)t * 2 * S2 + (s.y != 0.f ? S2 : 0);
+      atomicAdd(&dst[(by % kTile) * SW + (bx % kTile)], v);
+    }
+  }
+}
+
+// per bin: own slab entry plus the halo entries of the tiles left / below /
+// below-left; density and overflow (computeOverflow, DensityGradient.cu:140-156)
+__global__ __launch_bounds__(256) void k_dens_resolve(Geo G, const float* slab, const float* fixed,
+                                                      const float* scaled, float* mov, float* fil, float* dens,
+                                                      double* ovf_acc) {
+  const int B = G.nbx * G.nby;
+  const int SW = G.SW, S2 = SW * SW;
+  double ovf = 0.0;
+  for (int bin = blockIdx.x * blockDim.x + threadIdx.x; bin < B; bin += gridDim.x * blockDim.x) {
+    int x = bin % G.nbx, y = bin / G.nbx;
+    int tx = x / kTile, ty = y / kTile, lx = x - tx * kTile, ly = y - ty * kTile;
+    const float* s0 = slab + (size_t)(tx + ty * G.tiles_x) * 2 * S2;
+    float m = s0[ly * SW + lx], f = s0[S2 + ly * SW + lx];
+    if (lx < G.H && tx > 0) {
+      const float* s1 = s0 - 2 * S2;
+      m += s1[ly * SW + lx + kTile];
+      f += s1[S2 + ly * SW + lx + kTile];
+    }
+    if (ly < G.H && ty > 0) {
+      const float* s2 = s0 - (size_t)G.tiles_x * 2 * S2;
+      m += s2[(ly + kTile) * SW + lx];
+      f += s2[S2 + (ly + kTile) * SW + lx];
+      if (lx < G.H && tx > 0) {
+        const float* s3 = s2 - 2 * S2;
+        m += s3[(ly + kTile) * SW + lx + kTile];
+        f += s3[S2 + (ly + kTile) * SW + lx + kTile];
+      }
+    }
+    mov[bin] = m;
+    fil[bin] = f;
+    float fx = fixed[bin], sc = scaled[bin];
+    dens[bin] = (m + fx + f) / sc;
+    ovf += (double)fmaxf(0.f, m + fx - sc);
+  }
+  __shared__ double red[4];
+  for (int o = 32; o > 0; o >>= 1) ovf += __shfl_down(ovf, o);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ovf;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(ovf_acc, red[0] + red[1] + red[2] + red[3]);
+}
+
+// getDensityGradForEachCell (DensityGradient.cu:173-281): the GPU Poisson
+// field carries a factor 2, hence the 0.5
+__global__ void k_dens_gather(Geo G, const float* cx, const float* cy, const float4* cgeo, const float* lambda,
+                              const float* ex, const float* ey, float* gx, float* gy, float* precond) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= G.N) return;
+  float4 c = cgeo[i];
+  float x = cx[i], y = cy[i];
+  int x0, x1, y0, y1;
+  bin_range(G, x, y, c.x, c.y, x0, x1, y0, y1);
+  float lx = x - c.x / 2, ux = x + c.x / 2;
+  float ly = y - c.y / 2, uy = y + c.y / 2;
+  float fx = 0.f, fy = 0.f, pc = 0.f;
+  for (int bx = x0; bx < x1; bx++) {
+    float ox = overlap(lx, ux, G.bin_lx[bx], G.bin_ux[bx]);
+    for (int by = y0; by < y1; by++) {
+      float a = ox * overlap(ly, uy, G.bin_ly[by], G.bin_uy[by]) * c.w;
+      int b = bx + by * G.nbx;
+      float la = lambda[b] * a;
+      fx += la * ex[b] * 0.5f;
+      fy += la * ey[b] * 0.5f;
+      pc += la;
+    }
+  }
+  gx[i] = fx;
+  gy[i] = fy;
+  precond[i] = pc;
+}
+
+__global__ void k_dens_ratio(const double* ovf_acc, double inv_area, float* out) {
+  *out = (float)(*ovf_acc * inv_area);
+}
+
+template <typename T>
+int dalloc(std::vector<void*>& bufs, T** p, size_t n) {
+  hipError_t e = hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T));
+  if (e != hipSuccess) return fail(DENS_ERR_ALLOC, std::string("hipMalloc: ") + hipGetErrorString(e));
+  bufs.push_back((void*)*p);
+  return DENS_OK;
+}
+
+template <typename T>
+int dupload(std::vector<void*>& bufs, T** p, const T* src, size_t n) {
+  int st = dalloc(bufs, p, n);
+  if (st) return st;
+  if (n) DENS_TRY(hipMemcpy(*p, src, n * sizeof(T), hipMemcpyHostToDevice));
+  return DENS_OK;
+}
+
+}  // namespace densdev
+
+using namespace densdev;
+
+struct PoissonHandle {
+  int M = 0, N = 0;
+  std::vector<void*> bufs;
+  float *CN = nullptr, *CM = nullptr, *A3 = nullptr, *A4 = nullptr, *wx2 = nullptr, *wy2 = nullptr;
+  float *U = nullptr, *Ap = nullptr, *V = nullptr;
+};
+
+struct DensHandle {
+  Geo G;
+  int B = 0;
+  int prec = DENS_PREC_SPLIT_BF16;
+  double inv_sum_area = 0.0;
+  std::vector<void*> bufs;
+  float4* cgeo = nullptr;
+  int* cfill = nullptr;
+  float *fixed = nullptr, *scaled = nullptr, *lambda = nullptr;
+  int *cnt = nullptr, *off = nullptr, *bucket = nullptr, *rank = nullptr;
+  float4* rec_box = nullptr;
+  float2* rec_s = nullptr;
+  float *slab = nullptr, *mov = nullptr, *fil = nullptr, *dens = nullptr;
+  float *psi = nullptr, *ex = nullptr, *ey = nullptr, *precond = nullptr;
+  double* ovf_acc = nullptr;
+  float* ovf_val = nullptr;
+  PoissonHandle* ps = nullptr;
+};
+
+namespace {
+
+// cosine / sine synthesis tables (FP64 on the host, stored FP32)
+void poisson_tables(int M, int N, std::vector<float>& CN, std::vector<float>& CM, std::vector<float>& A3,
+                    std::vector<float>& A4, std::vector<float>& wx2, std::vector<float>& wy2) {
+  const double pi = 3.14159265358979323846;
+  CN.assign((size_t)N * N, 0.f);
+  CM.assign((size_t)M * M, 0.f);
+  A3.assign((size_t)2 * N * N, 0.f);
+  A4.assign((size_t)2 * M * M, 0.f);
+  for (int j = 0; j < N; j++)
+    for (int n = 0; n < N; n++) CN[(size_t)j * N + n] = (float)std::cos(pi * j * (2 * n + 1) / (2.0 * N));
+  for (int u = 0; u < M; u++)
+    for (int m = 0; m < M; m++) CM[(size_t)u * M + m] = (float)std::cos(pi * u * (2 * m + 1) / (2.0 * M));
+  for (int n = 0; n < N; n++)
+    for (int w = 0; w < N; w++) {
+      double ang = pi * w * (2 * n + 1) / (2.0 * N);
+      A3[(size_t)n * N + w] = (float)((w ? 2.0 : 1.0) * std::cos(ang));
+      A3[(size_t)(N + n) * N + w] = w ? (float)(2.0 * std::sin(ang) * (2.0 * pi * w / N)) : 0.f;
+    }
+  for (int m = 0; m < M; m++)
+    for (int h = 0; h < M; h++) {
+      double ang = pi * h * (2 * m + 1) / (2.0 * M);
+      A4[(size_t)m * M + h] = (float)((h ? 2.0 : 1.0) * std::cos(ang));
+      A4[(size_t)(M + m) * M + h] = h ? (float)(2.0 * std::sin(ang) * (2.0 * pi * h / M)) : 0.f;
+    }
+  // divideByWSquare (PoissonSolver.cu:73-95): w = 2 pi idx / numBin, squared in FP32
+  wx2.resize(N);
+  wy2.resize(M);
+  for (int w = 0; w < N; w++) {
+    float d = (float)(2.0 * pi * w / N);
+    wx2[w] = d * d;
+  }
+  for (int h = 0; h < M; h++) {
+    float d = (float)(2.0 * pi * h / M);
+    wy2[h] = d * d;
+  }
+}
+
+int poisson_run(PoissonHandle* h, const float* rho, float* psi, float* ex, float* ey, int prec, hipStream_t s) {
+  const int M = h->M, N = h->N;
+  int st;
+  Gemm g{};
+  // U = CN * rho^T
+  g = Gemm{h->CN, rho, h->U, nullptr, N, N, M, N, N, M, N, 1.f, nullptr, nullptr};
+  if ((st = launch_gemm(prec, g, s))) return st;
+  // A' = 4/(MN) CM * U^T / (wy^2 + wx^2)
+  g = Gemm{h->CM, h->U, h->Ap, nullptr, M, M, N, M, M, N, M, (float)(4.0 / ((double)M * N)), h->wy2, h->wx2};
+  if ((st = launch_gemm(prec, g, s))) return st;
+  // [V; Vx] = [DNt; SNw] * A'^T
+  g = Gemm{h->A3, h->Ap, h->V, nullptr, N, N, M, 2 * N, 2 * N, M, N, 1.f, nullptr, nullptr};
+  if ((st = launch_gemm(prec, g, s))) return st;
+  // [psi; Ey] = [DMt; SMw] * V^T
+  g = Gemm{h->A4, h->V, psi, ey, M, M, N, M, 2 * M, N, M, 1.f, nullptr, nullptr};
+  if ((st = launch_gemm(prec, g, s))) return st;
+  // Ex = DMt * Vx^T
+  g = Gemm{h->A4, h->V + (size_t)N * M, ex, nullptr, M, M, N, M, M, N, M, 1.f, nullptr, nullptr};
+  return launch_gemm(prec, g, s);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* dens_last_error(void) { return g_err.c_str(); }
+
+int poisson_create(int32_t nbx, int32_t nby, PoissonHandle** out) {
+  if (!out) return fail(DENS_ERR_ARG, "null argument");
+  if (nbx < 4 || nby < 4 || nbx % 4 || nby % 4 || nbx > 16384 || nby > 16384)
+    return fail(DENS_ERR_ARG, "bin counts must be multiples of 4 in [4, 16384]");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(DENS_ERR_DEVICE, "no HIP device");
+  PoissonHandle* h = new PoissonHandle();
+  h->M = nby;
+  h->N = nbx;
+  std::vector<float> CN, CM, A3, A4, wx2, wy2;
+  poisson_tables(nby, nbx, CN, CM, A3, A4, wx2, wy2);
+  const size_t B = (size_t)nbx * nby;
+  int st;
+  auto bail = [&](int c) { poisson_destroy(h); return c; };
+  if ((st = dupload(h->bufs, &h->CN, CN.data(), CN.size()))) return bail(st);
+  if ((st = dupload(h->bufs, &h->CM, CM.data(), CM.size()))) return bail(st);
+  if ((st = dupload(h->bufs, &h->A3, A3.data(), A3.size()))) return bail(st);
+  if ((st = dupload(h->bufs, &h->A4, A4.data(), A4.size()))) return bail(st);
+  if ((st = dupload(h->bufs, &h->wx2, wx2.data(), wx2.size()))) return bail(st);
+  if ((st = dupload(h->bufs, &h->wy2, wy2.data(), wy2.size()))) return bail(st);
+  if ((st = dalloc(h->bufs, &h->U, B))) return bail(st);
+  if ((st = dalloc(h->bufs, &h->Ap, B))) return bail(st);
+  if ((st = dalloc(h->bufs, &h->V, 2 * B))) return bail(st);
+  *out = h;
+  return DENS_OK;
+}
+
+int poisson_solve(PoissonHandle* h, const float* rho, float* psi, float* ex, float* ey, int32_t prec,
+                  void* stream) {
+  if (!h || !rho || !psi || !ex || !ey) return fail(DENS_ERR_ARG, "null argument");
+  hipStream_t s = (hipStream_t)stream;
+  return poisson_run(h, rho, psi, ex, ey, prec, s);
+}
+
+int poisson_destroy(PoissonHandle* h) {
+  if (!h) return DENS_OK;
+  for (void* b : h->bufs) (void)hipFree(b);
+  delete h;
+  return DENS_OK;
+}
+
+int dens_create(const DensGeom* g, DensHandle** out) {
+  if (!g || !out) return fail(DENS_ERR_ARG, "null argument");
+  if (g->num_movable < 0 || !g->bin_lx || !g->bin_ux || !g->bin_ly || !g->bin_uy || !g->fixed_area ||
+      !g->scaled_bin_area)
+    return fail(DENS_ERR_ARG, "missing geometry arrays");
+  if (g->num_movable > 0 && (!g->cell_dw || !g->cell_dh || !g->cell_scale))
+    return fail(DENS_ERR_ARG, "missing cell arrays");
+  if (!(g->bin_w > 0.f) || !(g->bin_h > 0.f)) return fail(DENS_ERR_ARG, "bad bin size");
+  if (g->precision < 0 || g->precision > 2) return fail(DENS_ERR_ARG, "unknown precision");
+  PoissonHandle* ps = nullptr;
+  int st = poisson_create(g->num_bin_x, g->num_bin_y, &ps);
+  if (st) return st;
+  DensHandle* h = new DensHandle();
+  h->ps = ps;
+  auto bail = [&](int c) { dens_destroy(h); return c; };
+  const int nbx = g->num_bin_x, nby = g->num_bin_y, N = g->num_movable;
+  const size_t B = (size_t)nbx * nby;
+  h->B = (int)B;
+  h->prec = g->precision;
+  h->inv_sum_area = g->sum_movable_area > 0.f ? 1.0 / (double)g->sum_movable_area : 0.0;
+  // halo: widest density footprint in bins (a footprint of width w spans at
+  // most ceil(w / bin) + 1 bins), capped; wider cells take the atomic path
+  float maxw = 0.f, maxh = 0.f;
+  std::vector<float4> cg(N);
+  std::vector<int> cf(N, 0);
+  for (int i = 0; i < N; i++) {
+    float dw = g->cell_dw[i], dh = g->cell_dh[i], sc = g->cell_scale[i];
+    bool mac = g->is_macro && g->is_macro[i];
+    cg[i] = make_float4(dw, dh, mac ? g->target_density * sc : sc, sc);
+    cf[i] = g->is_filler && g->is_filler[i] ? 1 : 0;
+    maxw = std::max(maxw, dw);
+    maxh = std::max(maxh, dh);
+  }
+  int H = (int)std::ceil(std::max(maxw / g->bin_w, maxh / g->bin_h));
+  H = std::min(std::max(H, 1), kMaxHalo);
+  Geo& G = h->G;
+  G.nbx = nbx; G.nby = nby; G.N = N; G.H = H; G.SW = kTile + H;
+  G.tiles_x = (nbx + kTile - 1) / kTile;
+  G.T = G.tiles_x * ((nby + kTile - 1) / kTile);
+  G.die_lx = g->die_lx; G.die_ly = g->die_ly; G.bin_w = g->bin_w; G.bin_h = g->bin_h;
+  float *blx, *bux, *bly, *buy;
+  if ((st = dupload(h->bufs, &blx, g->bin_lx, nbx))) return bail(st);
+  if ((st = dupload(h->bufs, &bux, g->bin_ux, nbx))) return bail(st);
+  if ((st = dupload(h->bufs, &bly, g->bin_ly, nby))) return bail(st);
+  if ((st = dupload(h->bufs, &buy, g->bin_uy, nby))) return bail(st);
+  G.bin_lx = blx; G.bin_ux = bux; G.bin_ly = bly; G.bin_uy = buy;
+  if ((st = dupload(h->bufs, &h->cgeo, cg.data(), N))) return bail(st);
+  if ((st = dupload(h->bufs, &h->cfill, cf.data(), N))) return bail(st);
+  if ((st = dupload(h->bufs, &h->fixed, g->fixed_area, B))) return bail(st);
+  if ((st = dupload(h->bufs, &h->scaled, g->scaled_bin_area, B))) return bail(st);
+  std::vector<float> ones;
+  const float* lam = g->bin_lambda;
+  if (!lam) { ones.assign(B, 1.f); lam = ones.data(); }
+  if ((st = dupload(h->bufs, &h->lambda, lam, B))) return bail(st);
+  if ((st = dalloc(h->bufs, &h->cnt, G.T + 1))) return bail(st);
+  if ((st = dalloc(h->bufs, &h->off, G.T + 2))) return bail(st);
+  if ((st = dalloc(h->bufs, &h->bucket, N))) return bail(st);
+  if ((st = dalloc(h->bufs, &h->rank, N))) return bail(st);
+  if ((st = dalloc(h->bufs, &h->rec_box, N))) return bail(st);
+  if ((st = dalloc(h->bufs, &h->rec_s, N))) return bail(st);
+  if ((st = dalloc(h->bufs, &h->slab, (size_t)G.T * 2 * G.SW * G.SW))) return bail(st);
+  if ((st = dalloc(h->bufs, &h->mov, B))) return bail(st);
+  if ((st = dalloc(h->bufs, &h->fil, B))) return bail(st);
+  if ((st = dalloc(h->bufs, &h->dens, B))) return bail(st);
+  if ((st = dalloc(h->bufs, &h->psi, B))) return bail(st);
+  if ((st = dalloc(h->bufs, &h->ex, B))) return bail(st);
+  if ((st = dalloc(h->bufs, &h->ey, B))) return bail(st);
+  if ((st = dalloc(h->bufs, &h->precond, N))) return bail(st);
+  if ((st = dalloc(h->bufs, &h->ovf_acc, 1))) return bail(st);
+  if ((st = dalloc(h->bufs, &h->ovf_val, 1))) return bail(st);
+  *out = h;
+  return DENS_OK;
+}
+
+int dens_compute_grad(DensHandle* h, const float* cx, const float* cy, float* gx, float* gy, float* overflow_out,
+                      void* stream) {
+  if (!h) return fail(DENS_ERR_ARG, "null handle");
+  const Geo& G = h->G;
+  if (G.N > 0 && (!cx || !cy || !gx || !gy)) return fail(DENS_ERR_ARG, "null cell array");
+  hipStream_t s = (hipStream_t)stream;
+  const int nb = (G.N + 255) / 256;
+  hipLaunchKernelGGL(k_dens_reset, dim3((G.T + 256) / 256), dim3(256), 0, s, h->cnt, G.T + 1, h->ovf_acc);
+  if (G.N > 0) {
+    hipLaunchKernelGGL(k_dens_count, dim3(nb), dim3(256), 0, s, G, cx, cy, h->cgeo, h->cnt, h->bucket, h->rank);
+  }
+  hipLaunchKernelGGL(k_dens_scan, dim3(1), dim3(1024), 0, s, h->cnt, h->off, G.T + 1);
+  if (G.N > 0) {
+    hipLaunchKernelGGL(k_dens_fill, dim3(nb), dim3(256), 0, s, G, cx, cy, h->cgeo, h->cfill, h->bucket, h->rank,
+                       h->off, h->rec_box, h->rec_s);
+  }
+  size_t lds = (size_t)2 * G.SW * G.SW * sizeof(float);
+  hipLaunchKernelGGL(k_dens_accum, dim3(G.T), dim3(256), lds, s, G, h->off, h->rec_box, h->rec_s, h->slab);
+  hipLaunchKernelGGL(k_dens_large, dim3(256), dim3(256), 0, s, G, h->off, h->rec_box, h->rec_s, h->slab);
+  hipLaunchKernelGGL(k_dens_resolve, dim3(std::min((h->B + 255) / 256, 4096)), dim3(256), 0, s, G, h->slab,
+                     h->fixed, h->scaled, h->mov, h->fil, h->dens, h->ovf_acc);
+  hipLaunchKernelGGL(k_dens_ratio, dim3(1), dim3(1), 0, s, h->ovf_acc, h->inv_sum_area, h->ovf_val);
+  DENS_TRY(hipGetLastError());
+  int st = poisson_run(h->ps, h->dens, h->psi, h->ex, h->ey, h->prec, s);
+  if (st) return st;
+  if (G.N > 0) {
+    hipLaunchKernelGGL(k_dens_gather, dim3(nb), dim3(256), 0, s, G, cx, cy, h->cgeo, h->lambda, h->ex, h->ey, gx,
+                       gy, h->precond);
+  }
+  DENS_TRY(hipGetLastError());
+  if (overflow_out) {
+    DENS_TRY(hipMemcpyAsync(overflow_out, h->ovf_val, sizeof(float), hipMemcpyDeviceToHost, s));
+    DENS_TRY(hipStreamSynchronize(s));
+  }
+  return DENS_OK;
+}
+
+int dens_get_overflow(DensHandle* h, float* out) {
+  if (!h || !out) return fail(DENS_ERR_ARG, "null argument");
+  DENS_TRY(hipDeviceSynchronize());
+  DENS_TRY(hipMemcpy(out, h->ovf_val, sizeof(float), hipMemcpyDeviceToHost));
+  return DENS_OK;
+}
+
+const float* dens_get_bin_density(DensHandle* h) { return h ? h->dens : nullptr; }
+const float* dens_get_potential(DensHandle* h) { return h ? h->psi : nullptr; }
+const float* dens_get_field_x(DensHandle* h) { return h ? h->ex : nullptr; }
+const float* dens_get_field_y(DensHandle* h) { return h ? h->ey : nullptr; }
+const float* dens_get_preconditioner(DensHandle* h) { return h ? h->precond : nullptr; }
+const float* dens_get_movable_area(DensHandle* h) { return h ? h->mov : nullptr; }
+const float* dens_get_filler_area(DensHandle* h) { return h ? h->fil : nullptr; }
+
+int dens_set_precision(DensHandle* h, int32_t prec) {
+  if (!h) return fail(DENS_ERR_ARG, "null handle");
+  if (prec < 0 || prec > 2) return fail(DENS_ERR_ARG, "unknown precision");
+  h->prec = prec;
+  return DENS_OK;
+}
+
+int dens_destroy(DensHandle* h) {
+  if (!h) return DENS_OK;
+  for (void* b : h->bufs) (void)hipFree(b);
+  if (h->ps) poisson_destroy(h->ps);
+  delete h;
+  return DENS_OK;
+}
+
+}  // extern "C"

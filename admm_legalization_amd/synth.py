@@ -234,3 +234,43 @@ def tiny(num_cells: int = 40, x_size: int = 48, y_size: int = 12, seed: int = 0,
                        num_fences=fences, fence_frac=0.1 if fences else 0.0,
                        hotspots=1, hotspot_frac=0.3, hotspot_sigma_rows=2.0, seed=seed,
                        name="tiny")
+
+
+# ---------------------------------------------------------------------------
+# Electrostatic density workloads (global-placement snapshots)
+# ---------------------------------------------------------------------------
+def density_placement(num_cells: int = 1_000_000, *, fill: float = 0.7, clusters: int = 8,
+                      cluster_frac: float = 0.5, cluster_sigma: float = 0.04, num_fixed: int = 0,
+                      macro_frac: float = 0.0, filler_frac: float = 0.0, seed: int = 0) -> dict:
+    """Square cells with area ~ LogNormal(ln 4, 0.5) (site units) on a square
+    die sized for `fill` movable area / die area; centres half uniform, half
+    drawn from `clusters` Gaussian clusters (sigma = cluster_sigma x die side).
+    Optional fixed rectangles, movable macros (x8 side) and filler cells."""
+    rng = np.random.default_rng(seed)
+    area = rng.lognormal(np.log(4.0), 0.5, num_cells)
+    side = np.sqrt(area)
+    is_macro = np.zeros(num_cells, np.int32)
+    if macro_frac > 0:
+        m = rng.random(num_cells) < macro_frac
+        is_macro[m] = 1
+        side[m] *= 8.0
+    is_filler = (rng.random(num_cells) < filler_frac).astype(np.int32)
+    die_side = float(math.sqrt(float((side ** 2).sum()) / fill))
+    n_cl = int(num_cells * cluster_frac)
+    cx = rng.random(num_cells) * die_side
+    cy = rng.random(num_cells) * die_side
+    if clusters > 0 and n_cl > 0:
+        centres = rng.random((clusters, 2)) * die_side * 0.8 + 0.1 * die_side
+        which = rng.integers(0, clusters, n_cl)
+        cx[:n_cl] = centres[which, 0] + rng.normal(0, cluster_sigma * die_side, n_cl)
+        cy[:n_cl] = centres[which, 1] + rng.normal(0, cluster_sigma * die_side, n_cl)
+    cx = np.clip(cx, side / 2, die_side - side / 2)
+    cy = np.clip(cy, side / 2, die_side - side / 2)
+    fixed = np.zeros((num_fixed, 4), np.float32)
+    for f in range(num_fixed):
+        w, h = rng.uniform(0.02, 0.08, 2) * die_side
+        x, y = rng.random(2) * (die_side - np.array([w, h]))
+        fixed[f] = (x, y, x + w, y + h)
+    return dict(die=(0.0, 0.0, die_side, die_side), dx=side.astype(np.float32), dy=side.astype(np.float32),
+                cx=cx.astype(np.float32), cy=cy.astype(np.float32), is_macro=is_macro, is_filler=is_filler,
+                fixed_rects=fixed, name=f"density_{num_cells}")

@@ -1,0 +1,167 @@
+"""GPU parity of the density path (scatter, overflow, Poisson GEMMs, gather)
+against the FP64 oracle.
+
+Tolerances (norm-wise relative error ||x - x64|| / ||x64||):
+  Poisson potential / field:  split-BF16 <= 1e-4, FP32 <= 1e-5, BF16 <= 2e-2
+  bin areas: max |a - a64| <= 1e-5 * max |a64|  (FP32 accumulation of a few terms)
+  overflow ratio: relative 1e-5
+  gradient / preconditioner: norm-wise 1e-4 (split-BF16 field), 1e-5 (precond)
+"""
+import numpy as np
+import pytest
+
+from admm_legalization_amd import density as D
+from admm_legalization_amd import synth
+from oracle import ref_density as R
+
+pytestmark = pytest.mark.gpu
+
+TOL = {D.PREC_SPLIT_BF16: 1e-4, D.PREC_F32: 1e-5, D.PREC_BF16: 2e-2}
+
+
+def _rel(x, ref):
+    x = np.asarray(x, np.float64).reshape(ref.shape)
+    return float(np.linalg.norm(x - ref) / max(np.linalg.norm(ref), 1e-300))
+
+
+def _density_map(M, N, seed):
+    rng = np.random.default_rng(seed)
+    yy, xx = np.meshgrid(np.arange(M), np.arange(N), indexing="ij")
+    rho = rng.random((M, N)) * 0.6
+    for _ in range(4):
+        cy, cx = rng.random(2) * (M, N)
+        rho += 1.5 * np.exp(-((yy - cy) ** 2 + (xx - cx) ** 2) / (2 * (0.08 * max(M, N)) ** 2))
+    return rho
+
+
+@pytest.mark.parametrize("prec", [D.PREC_SPLIT_BF16, D.PREC_F32, D.PREC_BF16])
+@pytest.mark.parametrize("n", [16, 64, 256, 1024])
+def test_poisson_square(gpu, n, prec):
+    import torch
+    rho = _density_map(n, n, seed=n)
+    print(f"[case] poisson {n}x{n} prec={prec}", flush=True)
+    ps = D.PoissonSolver(n, n)
+    t = torch.tensor(rho, dtype=torch.float32, device="cuda")
+    psi, ex, ey = (torch.empty_like(t) for _ in range(3))
+    ps.solvePoisson(t, psi, ex, ey, precision=prec)
+    torch.cuda.synchronize()
+    # the FP64 oracle on the FP32-rounded input
+    p64, x64, y64 = R.poisson(rho.astype(np.float32).astype(np.float64))
+    errs = [_rel(a.cpu().numpy(), b) for a, b in ((psi, p64), (ex, x64), (ey, y64))]
+    print(f"  rel errors psi/ex/ey: {errs}")
+    assert max(errs) <= TOL[prec], errs
+    ps.close()
+
+
+@pytest.mark.parametrize("M,N", [(32, 64), (128, 32)])
+def test_poisson_rectangular_closed_form(gpu, M, N):
+    """Non-square grids: the product implements the closed-form transforms
+    (the reference's FFT chain is only defined for square grids)."""
+    import torch
+    rho = _density_map(M, N, seed=M + N)
+    ps = D.PoissonSolver(N, M)
+    t = torch.tensor(rho, dtype=torch.float32, device="cuda")
+    psi, ex, ey = (torch.empty_like(t) for _ in range(3))
+    ps.solvePoisson(t, psi, ex, ey, precision=D.PREC_SPLIT_BF16)
+    torch.cuda.synchronize()
+    p64, x64, y64 = R.poisson(rho.astype(np.float32).astype(np.float64))
+    errs = [_rel(a.cpu().numpy(), b) for a, b in ((psi, p64), (ex, x64), (ey, y64))]
+    assert max(errs) <= 1e-4, errs
+
+
+def _run_density(pl, prec=D.PREC_SPLIT_BF16, num_bins=None, target=1.0):
+    import torch
+    g = D.prepare(pl["die"], pl["dx"], pl["dy"], is_macro=pl["is_macro"], is_filler=pl["is_filler"],
+                  fixed_rects=pl["fixed_rects"], target_density=target, num_bins=num_bins)
+    print(f"[case] density {pl['name']} bins {g.num_bin_x}x{g.num_bin_y} prec={prec}", flush=True)
+    dg = D.DensityGradient(g, precision=prec)
+    cx = torch.tensor(pl["cx"], device="cuda")
+    cy = torch.tensor(pl["cy"], device="cuda")
+    gx, gy = torch.empty_like(cx), torch.empty_like(cx)
+    dg.computeGrad(gx, gy, cx, cy)
+    torch.cuda.synchronize()
+    out = dict(geom=g, gx=gx.cpu().numpy(), gy=gy.cpu().numpy(), ovf=dg.overflow(),
+               mov=dg.getDeviceMovableArea().cpu().numpy(), fil=dg.getDeviceFillerArea().cpu().numpy(),
+               dens=dg.getDeviceBinDensity().cpu().numpy(), psi=dg.getDevicePotential().cpu().numpy(),
+               ex=dg.getDeviceFieldX().cpu().numpy(), ey=dg.getDeviceFieldY().cpu().numpy(),
+               pc=dg.getDevicePreconditioner().cpu().numpy())
+    dg.close()
+    return out
+
+
+def _oracle(pl, g, target=1.0):
+    geom = dict(num_bin_x=g.num_bin_x, num_bin_y=g.num_bin_y, bin_w=g.bin_w, bin_h=g.bin_h,
+                bin_lx=g.bin_lx, bin_ux=g.bin_ux, bin_ly=g.bin_ly, bin_uy=g.bin_uy, die=g.die)
+    mov, fil = R.scatter(geom, pl["cx"], pl["cy"], g.cell_dw, g.cell_dh, g.cell_scale, pl["is_filler"],
+                         pl["is_macro"], target)
+    fixed = g.fixed_area.astype(np.float64).reshape(mov.shape)
+    scaled = g.scaled_bin_area.astype(np.float64).reshape(mov.shape)
+    dens, ovf = R.overflow(mov, fil, fixed, scaled, g.sum_movable_area)
+    psi, ex, ey = R.poisson(dens)
+    gx, gy, pc = R.gather(geom, pl["cx"], pl["cy"], g.cell_dw, g.cell_dh, g.cell_scale, ex, ey)
+    return dict(mov=mov, fil=fil, dens=dens, ovf=ovf, psi=psi, ex=ex, ey=ey, gx=gx, gy=gy, pc=pc)
+
+
+def _compare(out, o, field_tol=1e-4):
+    for k in ("mov", "fil"):
+        ref = o[k]
+        err = np.abs(out[k].reshape(ref.shape) - ref).max()
+        assert err <= 1e-5 * max(np.abs(ref).max(), 1e-30), (k, err)
+    assert abs(out["ovf"] - o["ovf"]) <= 1e-5 * max(o["ovf"], 1e-6), (out["ovf"], o["ovf"])
+    for k in ("psi", "ex", "ey"):
+        assert _rel(out[k], o[k]) <= field_tol, (k, _rel(out[k], o[k]))
+    for k in ("gx", "gy"):
+        assert _rel(out[k], o[k]) <= field_tol, (k, _rel(out[k], o[k]))
+    assert _rel(out["pc"], o["pc"]) <= 1e-5
+
+
+def test_density_clustered_small(gpu):
+    pl = synth.density_placement(20000, seed=3)
+    out = _run_density(pl)
+    _compare(out, _oracle(pl, out["geom"]))
+
+
+def test_density_macros_fillers_fixed(gpu):
+    """Movable macros (wider than the LDS halo: atomic path), fillers, fixed
+    blockages and a target density below one."""
+    pl = synth.density_placement(20000, seed=5, num_fixed=6, macro_frac=0.002, filler_frac=0.1)
+    out = _run_density(pl, target=0.9)
+    _compare(out, _oracle(pl, out["geom"], target=0.9))
+
+
+def test_density_bin_override_rectangular(gpu):
+    pl = synth.density_placement(8000, seed=7)
+    out = _run_density(pl, num_bins=(64, 32))
+    _compare(out, _oracle(pl, out["geom"]))
+
+
+def test_density_cells_outside_die(gpu):
+    """Cells partly or wholly outside the die contribute only their overlap."""
+    pl = synth.density_placement(4000, seed=9)
+    side = pl["die"][2]
+    pl["cx"][:50] = -5.0
+    pl["cx"][50:100] = side + 0.4
+    pl["cy"][100:150] = side - 0.2
+    out = _run_density(pl)
+    _compare(out, _oracle(pl, out["geom"]))
+
+
+def test_density_repeat_is_stable(gpu):
+    """Two evaluations on one handle, read back without an explicit device
+    synchronisation, agree with each other and with the oracle."""
+    import torch
+    pl = synth.density_placement(10000, seed=11)
+    g = D.prepare(pl["die"], pl["dx"], pl["dy"])
+    dg = D.DensityGradient(g)
+    cx = torch.tensor(pl["cx"], device="cuda")
+    cy = torch.tensor(pl["cy"], device="cuda")
+    res = []
+    o = _oracle(pl, g)
+    for _ in range(2):
+        gx, gy = torch.empty_like(cx), torch.empty_like(cx)
+        dg.computeGrad(gx, gy, cx, cy)
+        res.append((gx.cpu().numpy(), gy.cpu().numpy(), dg.overflow()))
+        assert _rel(res[-1][0], o["gx"]) <= 1e-4 and _rel(res[-1][1], o["gy"]) <= 1e-4
+    dg.close()
+    assert _rel(res[0][0], res[1][0].astype(np.float64)) <= 1e-6
+    assert abs(res[0][2] - res[1][2]) <= 1e-6
