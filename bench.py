@@ -36,7 +36,10 @@ WORKLOADS = {
                 "rail parity), 1600 rows x 12000 sites, 85 % utilisation, 3 fences, 200 macros, seed 0",
     "plain_20k": "20k single-row cells, 250 rows x 490 sites, widths 1-8 sites, 75 % utilisation, "
                  "4 Gaussian hotspots, 10 macros, seed 0",
+    "density_1m": "electrostatic density: 1M cells, area LogNormal(ln 4, 0.5), half uniform / half in 8 "
+                  "Gaussian clusters, square die at 70 % fill, 1024 x 1024 bins, split-BF16 Poisson, seed 0",
 }
+MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16
 
 
 def traffic_from_profiles(workload: str):
@@ -94,6 +97,8 @@ def main():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
     lg.lib()  # raises if the HIP library is not built
+    if args.workload == "density_1m":
+        return bench_density(args)
     if args.copy_calibration:
         src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
@@ -195,6 +200,98 @@ def main():
     if not args.no_cpu_baseline:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.cpu_sample_cells, args.cpu_threads)
+    print(json.dumps(out), flush=True)
+
+
+def density_cpu_baseline(pl, g):
+    """The FP64 oracle evaluates the same 1M-cell, 1024 x 1024 density once
+    (scatter, overflow, closed-form Poisson, gather) with numpy on the host."""
+    from oracle import ref_density as R
+    geom = dict(num_bin_x=g.num_bin_x, num_bin_y=g.num_bin_y, bin_w=g.bin_w, bin_h=g.bin_h,
+                bin_lx=g.bin_lx, bin_ux=g.bin_ux, bin_ly=g.bin_ly, bin_uy=g.bin_uy, die=g.die)
+    t = time.perf_counter()
+    mov, fil = R.scatter(geom, pl["cx"], pl["cy"], g.cell_dw, g.cell_dh, g.cell_scale)
+    fixed = g.fixed_area.astype(np.float64).reshape(mov.shape)
+    scaled = g.scaled_bin_area.astype(np.float64).reshape(mov.shape)
+    dens, _ = R.overflow(mov, fil, fixed, scaled, g.sum_movable_area)
+    _, ex, ey = R.poisson(dens)
+    R.gather(geom, pl["cx"], pl["cy"], g.cell_dw, g.cell_dh, g.cell_scale, ex, ey)
+    el = time.perf_counter() - t
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": 1.0 / el, "unit": "evaluations/s", "cores": threads, "kind": "port",
+            "sample": f"one full evaluation by the FP64 numpy oracle (BLAS matrix products for the "
+                      f"transforms) on the same 1M-cell, 1024x1024-bin input: {el:.2f} s"}
+
+
+def bench_density(args):
+    import torch
+    from admm_legalization_amd import density as D
+    from admm_legalization_amd import synth
+    pl = synth.density_placement(1_000_000, seed=0)
+    g = D.prepare(pl["die"], pl["dx"], pl["dy"])
+    n_cells = len(pl["dx"])
+    B = g.num_bin_x * g.num_bin_y
+    print(f"# density_1m: N={n_cells} bins {g.num_bin_x}x{g.num_bin_y}", file=sys.stderr, flush=True)
+    dg = D.DensityGradient(g, precision=D.PREC_SPLIT_BF16)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        cx = torch.tensor(pl["cx"], device="cuda")
+        cy = torch.tensor(pl["cy"], device="cuda")
+        gx, gy = torch.empty_like(cx), torch.empty_like(cx)
+        for _ in range(args.warmup):
+            dg.computeGrad(gx, gy, cx, cy)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e0.record(s)
+        for _ in range(args.steps):
+            dg.computeGrad(gx, gy, cx, cy)
+        e1.record(s)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        ms_eval = e0.elapsed_time(e1) / args.steps
+        ovf = dg.overflow()
+        # the Poisson solve alone, per precision, on the evaluation's density map
+        rho = dg.getDeviceBinDensity()
+        ps = D.PoissonSolver(g.num_bin_x, g.num_bin_y)
+        psi, ex, ey = (torch.empty_like(rho) for _ in range(3))
+        solve_ms = {}
+        for name, prec in (("bf16", D.PREC_BF16), ("split_bf16", D.PREC_SPLIT_BF16), ("f32", D.PREC_F32)):
+            for _ in range(2):
+                ps.solvePoisson(rho, psi, ex, ey, precision=prec)
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(s)
+            reps = max(args.steps, 5)
+            for _ in range(reps):
+                ps.solvePoisson(rho, psi, ex, ey, precision=prec)
+            b.record(s)
+            torch.cuda.synchronize()
+            solve_ms[name] = a.elapsed_time(b) / reps
+        ps.close()
+    n = g.num_bin_x
+    flops = 16.0 * n ** 3  # four 2-D transforms as two n^3 GEMMs each
+    achieved = flops / (solve_ms["split_bf16"] / 1e3) / 1e12
+    out = {
+        "metric": "density evaluations/s (scatter + overflow + Poisson + gather)",
+        "value": 1e3 / ms_eval, "unit": "evaluations/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "strong", "vs_baseline": None, "dtype": "f32 (split-bf16 MFMA transforms)",
+        "data": "synthetic",
+        "config": {"workload": f"density_1m: {WORKLOADS['density_1m']}", "cells": n_cells,
+                   "bins": [g.num_bin_x, g.num_bin_y], "overflow": ovf, "event_ms_per_eval": ms_eval,
+                   "cells_per_s": n_cells * 1e3 / ms_eval, "poisson_ms": solve_ms,
+                   "parallelism": "1 GPU"},
+        "roofline": {"bound": "mfma", "kernel": "Poisson solve (split-BF16 cosine/sine GEMMs)",
+                     "achieved": achieved, "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / MFMA_BF16_PEAK_TFLOPS, "traffic": None,
+                     "algorithmic_flops_per_solve": flops,
+                     "note": "split-BF16 issues three MFMA products per algorithmic product"},
+        "cpu_baseline": None,
+    }
+    dg.close()
+    if not args.no_cpu_baseline:
+        print("# cpu baseline ...", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = density_cpu_baseline(pl, g)
     print(json.dumps(out), flush=True)
 
 

@@ -1,0 +1,66 @@
+"""The C ABI: the shared library loads and exports every function the public
+headers declare (no device calls; runs without a GPU)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+from admm_legalization_amd import build, density, lg
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("skyline_lg.h", "skyline_density.h")]
+
+
+def declared(path):
+    text = open(path).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return set(re.findall(r"^\s*(?:const\s+)?\w+\s*\**\s*(\w+)\s*\(", text, flags=re.M)) - {"if", "while"}
+
+
+@pytest.fixture(scope="module")
+def library():
+    if not os.path.exists(build.LIB):
+        build.build()
+    return build.LIB
+
+
+def test_headers_declare_the_expected_entry_points():
+    lg_syms, dens_syms = (declared(h) for h in HEADERS)
+    assert {"lg_create", "lg_solve", "lg_postprocess", "lg_check", "lg_copy_out", "lg_destroy"} <= lg_syms
+    assert {"dens_create", "dens_compute_grad", "poisson_create", "poisson_solve"} <= dens_syms
+    assert set(lg.EXPORTED) == lg_syms
+    assert set(density.EXPORTED) == dens_syms
+
+
+def test_library_exports_every_declared_symbol(library):
+    out = subprocess.run(["nm", "-D", "--defined-only", library], stdout=subprocess.PIPE, text=True,
+                         check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    for h in HEADERS:
+        missing = declared(h) - exported
+        assert not missing, f"{os.path.basename(h)}: not exported {sorted(missing)}"
+
+
+def test_library_loads_with_ctypes(library):
+    L = C.CDLL(library)
+    for h in HEADERS:
+        for name in declared(h):
+            assert getattr(L, name) is not None
+
+
+def test_default_params_and_error_text(library):
+    p = lg.default_params()
+    assert p.x_hint == 250 and p.max_admm_iter > 0
+    L = lg.lib()
+    assert isinstance(L.lg_last_error(), bytes)
+
+
+def test_null_arguments_are_rejected_without_a_device(library):
+    L = lg.lib()
+    assert L.lg_create(None, None, None, None) != 0
+    assert b"null" in L.lg_last_error()
+    D = density.lib()
+    assert D.dens_create(None, None) != 0
+    assert D.poisson_solve(None, None, None, None, None, 1, None) != 0

@@ -165,3 +165,17 @@ def test_density_repeat_is_stable(gpu):
     dg.close()
     assert _rel(res[0][0], res[1][0].astype(np.float64)) <= 1e-6
     assert abs(res[0][2] - res[1][2]) <= 1e-6
+
+
+@pytest.mark.parametrize("n", [16, 64])
+def test_poisson_matches_golden(gpu, n):
+    import os
+    import torch
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", f"poisson_{n}x{n}.npz"))
+    ps = D.PoissonSolver(n, n)
+    t = torch.tensor(z["rho"], device="cuda")
+    psi, ex, ey = (torch.empty_like(t) for _ in range(3))
+    ps.solvePoisson(t, psi, ex, ey, precision=D.PREC_SPLIT_BF16)
+    errs = [_rel(a.cpu().numpy(), z[k]) for a, k in ((psi, "psi"), (ex, "ex"), (ey, "ey"))]
+    assert max(errs) <= TOL[D.PREC_SPLIT_BF16], errs
+    ps.close()

@@ -1,0 +1,178 @@
+"""CPU pins of the legalizer oracle (oracle/ref_lg.cpp) and of the product's
+host layer (import + preplace), by mathematics and brute force.
+
+The reference repository holds no golden vectors, tests or fixtures for this
+path, so each oracle piece is pinned here by a property that a plausible
+mistake would break: sorting networks against a sort, the tie-breaking
+tree-min against an independent characterisation, the Thomas solver against
+a dense solve, the rho / lambda0 initialisation against their closed forms
+on a fixture with known usage, and the whole solve against legality
+invariants.
+"""
+import numpy as np
+import pytest
+
+from admm_legalization_amd import lg, synth
+from admm_legalization_amd.design import default_params
+from oracle import ref
+
+
+# -- sorting networks (bitonicSortDescending, AdmmKernels.hpp:899-945) --------
+@pytest.mark.parametrize("n", [1, 2, 4, 8, 64, 256, 1024])
+def test_bitonic_desc_sorts_distinct_keys(n):
+    rng = np.random.default_rng(n)
+    keys = rng.permutation(10 * n)[:n].astype(np.int32)
+    vals = np.arange(n, dtype=np.int32)
+    k, v = ref.bitonic_desc(keys, vals)
+    order = np.argsort(-keys, kind="stable")
+    np.testing.assert_array_equal(k, keys[order])
+    np.testing.assert_array_equal(v, vals[order])
+
+
+@pytest.mark.parametrize("n", [2, 16, 512])
+def test_bitonic_asc_sorts_distinct_keys(n):
+    rng = np.random.default_rng(100 + n)
+    keys = rng.permutation(10 * n)[:n].astype(np.int32)
+    vals = np.arange(n, dtype=np.int32)
+    k, v = ref.bitonic_asc(keys, vals)
+    order = np.argsort(keys, kind="stable")
+    np.testing.assert_array_equal(k, keys[order])
+    np.testing.assert_array_equal(v, vals[order])
+
+
+def test_bitonic_desc_with_int_min_padding():
+    """Padding slots (INT_MIN) sink to the end of a descending sort."""
+    keys = np.array([5, -2147483648, 7, -2147483648, 1, 3, -2147483648, 9], np.int32)
+    k, _ = ref.bitonic_desc(keys, np.arange(8, dtype=np.int32))
+    np.testing.assert_array_equal(k[:5], [9, 7, 5, 3, 1])
+    assert (k[5:] == -2147483648).all()
+
+
+# -- tree-min with halving strides (AdmmKernels.hpp:830-897) ------------------
+def _bitrev(i, bits):
+    return int(format(i, f"0{bits}b")[::-1], 2)
+
+
+@pytest.mark.parametrize("n", [2, 8, 256])
+def test_tree_argmin_is_lexicographic_min_of_cost_and_reversed_slot(n):
+    """A halving-stride tree that keeps the lower slot on ties returns the
+    slot minimising (cost, bit-reversed slot index)."""
+    bits = n.bit_length() - 1
+    rng = np.random.default_rng(n)
+    for trial in range(200):
+        c = rng.integers(0, 4, n).astype(np.float32)  # many ties
+        if trial % 3 == 0:
+            c[rng.random(n) < 0.3] = np.inf
+        best = min(range(n), key=lambda i: (c[i], _bitrev(i, bits)))
+        assert ref.tree_argmin(c) == best
+
+
+def test_tree_argmin_all_infinite_returns_slot_zero():
+    assert ref.tree_argmin(np.full(256, np.inf, np.float32)) == 0
+
+
+# -- Thomas algorithm (QpSolverKernels.hpp) ------------------------------------
+@pytest.mark.parametrize("n,rho", [(1, 0.5), (5, 1.0), (40, 3.0)])
+def test_thomas_matches_dense_solve(n, rho):
+    a, b = -rho, 1 + 2 * rho
+    cp = np.zeros(n)
+    for i in range(n):
+        cp[i] = a / b if i == 0 else a / (b - a * cp[i - 1])
+    d = np.random.default_rng(n).standard_normal(n)
+    T = np.diag(np.full(n, b)) + np.diag(np.full(n - 1, a), 1) + np.diag(np.full(n - 1, a), -1)
+    np.testing.assert_allclose(ref.thomas(a, b, d, cp), np.linalg.solve(T, d), rtol=1e-12, atol=1e-12)
+
+
+# -- rho / lambda0 closed forms (CudaAdmmLegalizer.cu:321-395) ----------------
+def _grid(xs, ys, valid=None):
+    P = xs * ys
+    return dict(x_size=xs, y_size=ys, site_width=100, row_height=800, core_lx=0, core_ly=0,
+                valid=np.ones(P, np.int32) if valid is None else valid, group=np.full(P, -1, np.int32),
+                vdd_up=(np.arange(ys) % 2 == 0).astype(np.int32))
+
+
+def _cells(lx, ly, w, h):
+    n = len(lx)
+    lx, ly = np.asarray(lx, np.int32), np.asarray(ly, np.int32)
+    return dict(orig_lx_dbu=lx * 100, orig_ly_dbu=ly * 800, lx_dbu=lx * 100, ly_dbu=ly * 800,
+                w_grid=np.asarray(w, np.int32), h_grid=np.asarray(h, np.int32), group=np.full(n, -1, np.int32),
+                ground_at_bottom=np.zeros(n, np.int32))
+
+
+def test_rho_and_lambda0_closed_forms():
+    """Non-overlapping cells: usage is 1 on every footprint pixel, so
+    rho = c_rho * mean(w h) and lambda0 = c_dual * mean((rh/sw) / (w h (2 + rho/2)))."""
+    lx, ly, w, h = [0, 10, 20, 30], [0, 0, 2, 4], [3, 5, 2, 4], [1, 2, 1, 3]
+    p = default_params()
+    p.max_admm_iter = 1
+    o = ref.solve(_grid(40, 12), _cells(lx, ly, w, h), p, do_post=False)
+    area = np.array(w) * np.array(h)
+    rho = np.float32(p.coeff_to_init_rho) * np.float32(area.sum()) / np.float32(4)
+    assert o["stats"]["rho"] == pytest.approx(float(rho), rel=1e-7)
+    lam = p.coeff_to_init_dual * np.mean((800 / 100) / (area * (2 + float(rho) / 2)))
+    assert o["stats"]["initial_dual"] == pytest.approx(lam, rel=1e-6)
+    assert o["stats"]["iterations"] == 1 and o["stats"]["final_overflow"] == 0
+
+
+def test_single_overlap_pair_resolves():
+    """Two identical single-row cells on top of each other in an empty row
+    band: the solve separates them and leaves every pixel at usage <= 1."""
+    p = default_params()
+    o = ref.solve(_grid(60, 12), _cells([20, 20], [4, 4], [4, 4], [1, 1]), p, do_post=True)
+    assert o["stats"]["converged"] == 1
+    occ = np.zeros((60, 12), int)
+    for x, y, w in zip(o["lx"], o["ly"], [4, 4]):
+        occ[x:x + w, y] += 1
+    assert occ.max() == 1
+    # the first-moved cell goes to a minimum-displacement free spot
+    assert sorted(np.abs(o["lx"] - 20) + 8 * np.abs(o["ly"] - 4))[0] == 0
+
+
+# -- invariants of a full oracle solve --------------------------------------
+def _legal(design, grid, cells, lx, ly, rot):
+    xs, ys = grid["x_size"], grid["y_size"]
+    occ = np.zeros(xs * ys, np.int64)
+    valid = grid["valid"].astype(bool)
+    group = grid["group"]
+    for i in range(len(lx)):
+        w, h = cells["w_grid"][i], cells["h_grid"][i]
+        assert 0 <= lx[i] and lx[i] + w <= xs and 0 <= ly[i] and ly[i] + h <= ys
+        xx, yy = np.meshgrid(np.arange(lx[i], lx[i] + w), np.arange(ly[i], ly[i] + h), indexing="ij")
+        pid = (xx * ys + yy).ravel()
+        occ[pid] += 1
+        assert valid[pid].all()
+        assert (group[pid] == cells["group"][i]).all()
+        if h % 2 == 0:  # even heights: the bottom rail must match, they are never flipped
+            assert grid["vdd_up"][ly[i]] == cells["ground_at_bottom"][i]
+            assert rot[i] == 0
+    return occ.max()
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_oracle_solve_invariants_and_determinism(seed):
+    d = synth.mixed(1500, seed=seed, util=0.75)
+    grid, cells, p2 = lg.device_inputs(d)
+    o1 = ref.solve(grid, cells, p2, do_post=True)
+    o2 = ref.solve(grid, cells, p2, do_post=True)
+    for k in ("lx", "ly", "rot", "usage"):
+        np.testing.assert_array_equal(o1[k], o2[k])
+    np.testing.assert_array_equal(o1["dual"].view(np.int32), o2["dual"].view(np.int32))
+    assert o1["stats"]["converged"] == 1
+    assert (o1["dual"] >= 0).all()
+    assert _legal(d, grid, cells, o1["lx"], o1["ly"], o1["rot"]) <= 1
+    # overflow trace ends at zero and the final usage is the recount
+    assert o1["trace"][-1] == 0
+
+
+# -- product host layer == oracle host layer --------------------------------
+@pytest.mark.parametrize("maker", [lambda: synth.mixed(3000, seed=2),
+                                   lambda: synth.tiny(num_cells=120, x_size=64, y_size=24, seed=1, fences=1),
+                                   lambda: synth.plain_20k()])
+def test_host_import_and_preplace_matches_oracle(maker):
+    d = maker()
+    a = lg.import_and_preplace(d)
+    b = ref.import_and_preplace(d)
+    for k in ("x_size", "y_size"):
+        assert a[k] == b[k]
+    for k in ("valid", "group", "region", "vdd_up", "region_bounds", "cell_lx", "cell_ly"):
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
