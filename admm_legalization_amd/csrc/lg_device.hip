@@ -214,16 +214,23 @@ int lg_create(const LgGridDesc* g, const LgCellsSoA* c, const LgParams* p, LgHan
 
   auto bail = [&](int code) { lg_destroy(h); return code; };
 
-  // pixel code: valid ? group + 2 : 0
+  // pixel code: valid ? group + 2 : 0.  The ABI's pixel arrays are x-major
+  // (x * y_size + y, the reference's Grid layout); the device keeps them
+  // row-major (y * x_size + x).
   std::vector<int16_t> pcode(P);
-  for (size_t i = 0; i < P; i++) {
-    int grp = g->pixel_group[i];
-    if (g->pixel_valid[i] && (grp + 2 > 0x7fff || grp + 2 < 0)) {
-      lg_destroy(h);
-      return fail(LG_ERR_ARG, "fence group index out of range");
+  std::vector<int32_t> region_rm(P, -1);
+  const int XS = g->x_size, YS = g->y_size;
+  for (int x = 0; x < XS; x++)
+    for (int y = 0; y < YS; y++) {
+      size_t i = (size_t)x * YS + y, j = (size_t)y * XS + x;
+      int grp = g->pixel_group[i];
+      if (g->pixel_valid[i] && (grp + 2 > 0x7fff || grp + 2 < 0)) {
+        lg_destroy(h);
+        return fail(LG_ERR_ARG, "fence group index out of range");
+      }
+      pcode[j] = g->pixel_valid[i] ? (int16_t)(grp + 2) : (int16_t)0;
+      if (g->pixel_region) region_rm[j] = g->pixel_region[i];
     }
-    pcode[i] = g->pixel_valid[i] ? (int16_t)(grp + 2) : (int16_t)0;
-  }
   std::vector<int32_t> lxg(N), lyg(N), code(N);
   int max_h = 1;
   for (int i = 0; i < N; i++) {
@@ -259,10 +266,7 @@ int lg_create(const LgGridDesc* g, const LgCellsSoA* c, const LgParams* p, LgHan
   int16_t* d_pcode;
   int32_t *d_region, *d_vdd, *d_rx0, *d_ry0, *d_rx1, *d_ry1;
   if ((st = upload(h, &d_pcode, pcode.data(), P))) return bail(st);
-  std::vector<int32_t> region_tmp;
-  const int32_t* reg = g->pixel_region;
-  if (!reg) { region_tmp.assign(P, -1); reg = region_tmp.data(); }
-  if ((st = upload(h, &d_region, reg, P))) return bail(st);
+  if ((st = upload(h, &d_region, region_rm.data(), P))) return bail(st);
   if ((st = upload(h, &d_vdd, g->row_vdd_up, (size_t)g->y_size))) return bail(st);
   int nr = std::max(g->num_regions, 0);
   if ((st = upload(h, &d_rx0, g->region_x_min, nr))) return bail(st);
@@ -475,8 +479,19 @@ int lg_get_stats(LgHandle* h, LgStats* out) {
 int lg_get_pixel_state(LgHandle* h, int32_t* usage, float* dual) {
   if (!h) return fail(LG_ERR_ARG, "null handle");
   HIP_TRY(hipStreamSynchronize(h->own_stream));
-  if (usage) HIP_TRY(hipMemcpy(usage, h->a.usage, h->P * sizeof(int32_t), hipMemcpyDeviceToHost));
-  if (dual) HIP_TRY(hipMemcpy(dual, h->a.dual, h->P * sizeof(float), hipMemcpyDeviceToHost));
+  // device arrays are row-major; the ABI returns x-major
+  const int XS = h->a.x_size, YS = h->a.y_size;
+  std::vector<int32_t> tmp(h->P);
+  if (usage) {
+    HIP_TRY(hipMemcpy(tmp.data(), h->a.usage, h->P * sizeof(int32_t), hipMemcpyDeviceToHost));
+    for (int y = 0; y < YS; y++)
+      for (int x = 0; x < XS; x++) usage[(size_t)x * YS + y] = tmp[(size_t)y * XS + x];
+  }
+  if (dual) {
+    HIP_TRY(hipMemcpy(tmp.data(), h->a.dual, h->P * sizeof(float), hipMemcpyDeviceToHost));
+    for (int y = 0; y < YS; y++)
+      for (int x = 0; x < XS; x++) std::memcpy(&dual[(size_t)x * YS + y], &tmp[(size_t)y * XS + x], 4);
+  }
   return LG_OK;
 }
 

@@ -1,7 +1,8 @@
 // HIP kernels of the ADMM negotiation legalizer for gfx950 (MI355X).
 //
 // Data layout in HBM (one legalization handle):
-//   pixels, x-major (pixel_id = x * y_size + y):
+//   pixels, row-major by grid row (pixel_id = y * x_size + x), so a partition
+//   row of x_hint sites is one contiguous run:
 //     usage  int32   cells covering the pixel
 //     pcode  int16   valid ? fence_group + 2 : 0   (0 = not placeable)
 //     dual   float   negotiation price (Lagrange multiplier)
@@ -102,6 +103,9 @@ struct Args {
   Ctl* ctl;
 };
 
+// linear index of pixel (x, y) in HBM
+__device__ __forceinline__ size_t pidx(const Args& a, int x, int y) { return (size_t)y * a.x_size + x; }
+
 __device__ __forceinline__ bool correct_row(int h, int gnd, int vdd) {  // DeviceUtil.hpp:11-22
   return !((h & 1) == 0 && gnd != vdd);
 }
@@ -121,7 +125,7 @@ __global__ void k_usage_scatter(Args a) {
   int x0 = max(0, a.lx[c]), y0 = max(0, a.ly[c]);
   int x1 = min(a.x_size, a.lx[c] + a.w[c]), y1 = min(a.y_size, a.ly[c] + a.h[c]);
   for (int x = x0; x < x1; x++)
-    for (int y = y0; y < y1; y++) atomicAdd(&a.usage[(size_t)x * a.y_size + y], 1);
+    for (int y = y0; y < y1; y++) atomicAdd(&a.usage[pidx(a, x, y)], 1);
 }
 
 // computeDemandKernel (AdmmKernels.hpp:44-81); summed exactly in 64-bit integers
@@ -136,7 +140,7 @@ __global__ void k_demand(Args a) {
       if (x < 0 || x >= a.x_size) { off = true; break; }
       for (int y = y0; y < y0 + a.h[c]; y++) {
         if (y < 0 || y >= a.y_size) { off = true; break; }
-        s += a.usage[(size_t)x * a.y_size + y];
+        s += a.usage[pidx(a, x, y)];
       }
     }
     if (!off) v = (unsigned long long)s;
@@ -167,7 +171,7 @@ __global__ void k_inv_aug(Args a) {
       if (x < 0 || x >= a.x_size) { off = true; break; }
       for (int y = y0; y < y0 + a.h[c]; y++) {
         if (y < 0 || y >= a.y_size) { off = true; break; }
-        float u = float(a.usage[(size_t)x * a.y_size + y]);
+        float u = float(a.usage[pidx(a, x, y)]);
         float t = rho * 0.5f;
         t = t * u;
         t = t * u;
@@ -499,7 +503,7 @@ __device__ __forceinline__ int ovf_lds(const PartGeo& g, const uint2* pix, int c
   for (int x = 0; x < w; x++) {
     int lyi = ly0;
     for (int y = 0; y < h; y++) {
-      s += max((int)(pix[lxi * g.yh + lyi].x & 0xffffu) - 1, 0);
+      s += max((int)(pix[lyi * g.xh + lxi].x & 0xffffu) - 1, 0);
       if (++lyi == g.yh) lyi = 0;
     }
     if (++lxi == g.xh) lxi = 0;
@@ -511,7 +515,7 @@ __device__ __forceinline__ int ovf_lds(const PartGeo& g, const uint2* pix, int c
 __device__ __forceinline__ int ovf_hbm(const Args& a, int cx, int cy, int w, int h) {
   int s = 0;
   for (int x = cx; x < cx + w; x++)
-    for (int y = cy; y < cy + h; y++) s += max(a.usage[(size_t)x * a.y_size + y] - 1, 0);
+    for (int y = cy; y < cy + h; y++) s += max(a.usage[pidx(a, x, y)] - 1, 0);
   return s;
 }
 
@@ -526,7 +530,7 @@ __device__ __forceinline__ void footprint_add(const PartGeo& g, const Tile& T, i
     int lxi = lx0 + dx, lyi = ly0 + (i - dx * h);
     if (lxi >= g.xh) lxi -= g.xh;
     if (lyi >= g.yh) lyi -= g.yh;
-    const int idx = lxi * g.yh + lyi;
+    const int idx = lyi * g.xh + lxi;
     uint2 e = T.pix[idx];
     e.x = (uint32_t)((int)e.x + delta);
     e.y = __float_as_uint(pen_term((int)(e.x & 0xffffu), T.lam[idx], rho, standard));
@@ -574,15 +578,15 @@ __device__ __forceinline__ void eval_candidates(const Args& a, const PartGeo& g,
     }
   }
   if (g.ncol > 1 && g.nrow > 1) {
-    // footprints inside one interval never wrap: flat x-major walk
+    // footprints inside one interval never wrap: flat walk, x outer
     int col[4];
 #pragma unroll
-    for (int j = 0; j < 4; j++) col[j] = lxi[j] * g.yh + lyi0[j];
+    for (int j = 0; j < 4; j++) col[j] = lyi0[j] * g.xh + lxi[j];
     for (int x = 0; x < w; x++) {
       for (int y = 0; y < h; y++) {
         uint2 e[4];
 #pragma unroll
-        for (int j = 0; j < 4; j++) e[j] = T.pix[bad[j] ? 0 : col[j] + y];
+        for (int j = 0; j < 4; j++) e[j] = T.pix[bad[j] ? 0 : col[j] + y * g.xh];
 #pragma unroll
         for (int j = 0; j < 4; j++) {
           if ((int)(e[j].x >> 16) != code) bad[j] = true;
@@ -594,7 +598,7 @@ __device__ __forceinline__ void eval_candidates(const Args& a, const PartGeo& g,
         }
       }
 #pragma unroll
-      for (int j = 0; j < 4; j++) col[j] += g.yh;
+      for (int j = 0; j < 4; j++) col[j] += 1;
     }
   } else {
     for (int x = 0; x < w; x++) {
@@ -604,7 +608,7 @@ __device__ __forceinline__ void eval_candidates(const Args& a, const PartGeo& g,
       for (int y = 0; y < h; y++) {
         uint2 e[4];
 #pragma unroll
-        for (int j = 0; j < 4; j++) e[j] = T.pix[bad[j] ? 0 : lxi[j] * g.yh + lyi[j]];
+        for (int j = 0; j < 4; j++) e[j] = T.pix[bad[j] ? 0 : lyi[j] * g.xh + lxi[j]];
 #pragma unroll
         for (int j = 0; j < 4; j++) {
           if ((int)(e[j].x >> 16) != code) bad[j] = true;
@@ -687,14 +691,15 @@ __device__ __forceinline__ void reg_bitonic(int n, int (&key)[R], int (&val)[R])
 }
 
 // Staging of the partition's pixel tile, lanes walking the tile in order
-// with incremental (la, lb) = (i / yh, i % yh); four loads in flight per lane.
+// with incremental (la, lb) = (i % xh, i / xh): consecutive lanes read
+// consecutive pixels of a grid row; four loads in flight per lane.
 // Returns true if a usage does not fit the 16-bit LDS packing.
 __device__ __forceinline__ bool stage_tile(const Args& a, const PartGeo& g, const Tile& T, float rho,
                                            bool standard) {
   const int lane = threadIdx.x;
   const int npix = g.xh * g.yh;
-  const int step_a = kWave / g.yh, step_b = kWave - step_a * g.yh;
-  int la = lane / g.yh, lb = lane - la * g.yh;
+  const int step_b = kWave / g.xh, step_a = kWave - step_b * g.xh;
+  int lb = lane / g.xh, la = lane - lb * g.xh;
   bool range_err = false;
   constexpr int U = 4;
   for (int i0 = 0; i0 < npix; i0 += U * kWave) {
@@ -704,12 +709,12 @@ __device__ __forceinline__ bool stage_tile(const Args& a, const PartGeo& g, cons
     for (int q = 0; q < U; q++) {
       int xx = g.gx(la), yy = g.gy(lb);
       bool ok = (i0 + q * kWave + lane < npix) && xx >= 0 && yy >= 0;
-      size_t p = ok ? (size_t)xx * a.y_size + yy : 0;
+      size_t p = ok ? pidx(a, xx, yy) : 0;
       u[q] = ok ? a.usage[p] : 0;
       code[q] = ok ? (int)(uint16_t)a.pcode[p] : 0;  // code 0: not placeable
       dl[q] = ok ? a.dual[p] : 0.0f;
       la += step_a; lb += step_b;
-      if (lb >= g.yh) { lb -= g.yh; la++; }
+      if (la >= g.xh) { la -= g.xh; lb++; }
     }
 #pragma unroll
     for (int q = 0; q < U; q++) {
@@ -878,7 +883,7 @@ __global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_
   g.init(a.x_size, a.y_size);
   int4* cellrec = reinterpret_cast<int4*>(a.cellrec);
   const int npix = g.xh * g.yh;
-  const int step_a = kWave / g.yh, step_b = kWave - step_a * g.yh;
+  const int step_b = kWave / g.xh, step_a = kWave - step_b * g.xh;
 
   const int off = a.off[k];
   const int count = adaptive_now ? 0 : a.off[k + 1] - off;
@@ -900,13 +905,13 @@ __global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_
     if (!__any(need)) {
       // primal step is a no-op: stream the dual update over the pixels
       unsigned long long ovf = 0;
-      int la = lane / g.yh, lb = lane - la * g.yh;
+      int lb = lane / g.xh, la = lane - lb * g.xh;
       for (int i = lane; i < npix; i += kWave) {
         int xx = g.gx(la), yy = g.gy(lb);
         la += step_a; lb += step_b;
-        if (lb >= g.yh) { lb -= g.yh; la++; }
+        if (la >= g.xh) { la -= g.xh; lb++; }
         if (xx < 0 || yy < 0) continue;
-        size_t p = (size_t)xx * a.y_size + yy;
+        size_t p = pidx(a, xx, yy);
         int ui = a.usage[p];
         a.dual[p] = dual_step(ui, a.dual[p], rho, step, neg);
         ovf += (unsigned long long)max(ui - 1, 0);
@@ -1048,13 +1053,13 @@ __global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_
   // -- write back pixels; primal mode fuses the dual update and the overflow
   unsigned long long ovf = 0;
   {
-    int la = lane / g.yh, lb = lane - la * g.yh;
+    int lb = lane / g.xh, la = lane - lb * g.xh;
     for (int i = lane; i < npix; i += kWave) {
       int xx = g.gx(la), yy = g.gy(lb);
       la += step_a; lb += step_b;
-      if (lb >= g.yh) { lb -= g.yh; la++; }
+      if (la >= g.xh) { la -= g.xh; lb++; }
       if (xx < 0 || yy < 0) continue;
-      size_t p = (size_t)xx * a.y_size + yy;
+      size_t p = pidx(a, xx, yy);
       int ui = (int)(T.pix[i].x & 0xffffu);
       a.usage[p] = ui;
       if (MODE == MODE_PRIMAL) {
@@ -1148,7 +1153,7 @@ __global__ void k_extract_congested(Args a) {
   int cx = p0 & 0xffff, cy = (int)((unsigned)p0 >> 16);
   int s = 0;
   for (int x = cx; x < cx + a.w[c]; x++)
-    for (int y = cy; y < cy + a.h[c]; y++) s += max(a.usage[(size_t)x * a.y_size + y] - 1, 0);
+    for (int y = cy; y < cy + a.h[c]; y++) s += max(a.usage[pidx(a, x, y)] - 1, 0);
   a.ovf_flag[c] = s == 0 ? 0 : 1;
 }
 
@@ -1162,7 +1167,7 @@ __device__ void proj_pos(const Args& a, int d, int w, int h, int cx, int cy, int
   for (int x = nx; x < nx + w && !inval; x++)
     for (int y = ny; y < ny + h; y++) {
       if (x >= a.x_size || y >= a.y_size || x < 0 || y < 0) { inval = true; break; }
-      size_t p = (size_t)x * a.y_size + y;
+      size_t p = pidx(a, x, y);
       if (a.pcode[p] == 0) { inval = true; rid = a.pregion[p]; break; }
     }
   if (rid != -1) {
@@ -1210,12 +1215,12 @@ __global__ __launch_bounds__(1024) void k_adaptive(Args a) {
     if (t == 0) {
       int s = 0;
       for (int x = cx; x < cx + w; x++)
-        for (int y = cy; y < cy + h; y++) s += max(a.usage[(size_t)x * a.y_size + y] - 1, 0);
+        for (int y = cy; y < cy + h; y++) s += max(a.usage[pidx(a, x, y)] - 1, 0);
       s_ovf = s;
     }
     __syncthreads();
     if (s_ovf == 0) { __syncthreads(); continue; }
-    for (int q = t; q < w * h; q += 1024) a.usage[(size_t)(cx + q / h) * a.y_size + cy + q % h] -= 1;
+    for (int q = t; q < w * h; q += 1024) a.usage[pidx(a, cx + q / h, cy + q % h)] -= 1;
     __syncthreads();
     float disp_coeff = 1.0f;
     if (a.flag_iccad17) disp_coeff = float(a.N) / float(a.k_height_num[h - 1]);
@@ -1233,7 +1238,7 @@ __global__ __launch_bounds__(1024) void k_adaptive(Args a) {
           pc = 0.0f;
           for (int x = nx; x < nx + w && pc != kInf; x++)
             for (int y = ny; y < ny + h; y++) {
-              size_t p = (size_t)x * a.y_size + y;
+              size_t p = pidx(a, x, y);
               if ((int)a.pcode[p] != code) { pc = kInf; break; }
               float u = float(a.usage[p]);
               float l = a.dual[p];
@@ -1258,7 +1263,7 @@ __global__ __launch_bounds__(1024) void k_adaptive(Args a) {
     int nx = cx, ny = cy;
     if (s_cost[0] != kInf) proj_pos(a, s_idx[0], w, h, cx, cy, nx, ny);
     __syncthreads();
-    for (int q = t; q < w * h; q += 1024) a.usage[(size_t)(nx + q / h) * a.y_size + ny + q % h] += 1;
+    for (int q = t; q < w * h; q += 1024) a.usage[pidx(a, nx + q / h, ny + q % h)] += 1;
     if (t == 0) a.cellrec[4 * c] = (nx & 0xffff) | (ny << 16);
     __syncthreads();
   }

@@ -52,11 +52,11 @@ __global__ void k_qp_stamp(QpArgs q) {  // stampCellsKernel (PostProcessKernel.h
   if (c >= q.N) return;
   for (int x = q.lx[c]; x < q.lx[c] + q.w[c]; x++)
     for (int y = q.ly[c]; y < q.ly[c] + q.h[c]; y++)
-      atomicMax((int*)&q.stamp[(size_t)x * q.y_size + y], c);  // overlaps: highest id, as a sequential sweep
+      atomicMax((int*)&q.stamp[(size_t)y * q.x_size + x], c);  // overlaps: highest id, as a sequential sweep
 }
 
 __device__ __forceinline__ void qp_pix(const QpArgs& q, int x, int y, int& grp, int& val, int& hh) {
-  size_t p = (size_t)x * q.y_size + y;
+  size_t p = (size_t)y * q.x_size + x;
   int code = q.pcode[p];
   val = code != 0 ? 1 : 0;
   grp = code - 2;  // only compared while valid
@@ -110,13 +110,13 @@ __global__ void k_qp_gen_rows(QpArgs q) {
 __global__ void k_qp_mark_rows(QpArgs q) {  // markCutRowToGridKernel (PostProcessKernel.hpp:163-181)
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= q.num_rows) return;
-  for (int x = q.cr_xb[r]; x <= q.cr_xe[r]; x++) atomicMax(&q.pix_row[(size_t)x * q.y_size + q.cr_y[r]], r);
+  for (int x = q.cr_xb[r]; x <= q.cr_xe[r]; x++) atomicMax(&q.pix_row[(size_t)q.cr_y[r] * q.x_size + x], r);
 }
 
 __global__ void k_qp_cell_rows(QpArgs q) {  // markCutRowToCellKernel (PostProcessKernel.hpp:183-213)
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= q.N) return;
-  int r = q.pix_row[(size_t)q.lx[c] * q.y_size + q.ly[c]];
+  int r = q.pix_row[(size_t)q.ly[c] * q.x_size + q.lx[c]];
   int rr = (q.h[c] == 1 && r >= 0) ? r : q.num_rows;
   q.cell_row[c] = rr;
   atomicAdd(&q.cr_cnt[rr], 1);
