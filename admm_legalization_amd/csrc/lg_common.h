@@ -19,6 +19,7 @@ struct Ctl {
   int32_t min_ovf;
   int32_t error;         // nonzero: a device-side limit was exceeded (see kErr*)
   int32_t adaptive_calls;
+  int32_t wrong_row;     // cells on a row of the wrong rail parity at solve start
   int32_t ovf_window_n;  // entries in the stagnation window
   int32_t ovf_window[64];
   float rho;

@@ -695,7 +695,7 @@ __device__ __forceinline__ void reg_bitonic(int n, int (&key)[R], int (&val)[R])
 // consecutive pixels of a grid row; four loads in flight per lane.
 // Returns true if a usage does not fit the 16-bit LDS packing.
 __device__ __forceinline__ bool stage_tile(const Args& a, const PartGeo& g, const Tile& T, float rho,
-                                           bool standard) {
+                                           bool standard, bool& over) {
   const int lane = threadIdx.x;
   const int npix = g.xh * g.yh;
   const int step_b = kWave / g.xh, step_a = kWave - step_b * g.xh;
@@ -721,6 +721,7 @@ __device__ __forceinline__ bool stage_tile(const Args& a, const PartGeo& g, cons
       int i = i0 + q * kWave + lane;
       if (i < npix) {
         range_err |= u[q] > 0xffff;
+        over |= u[q] > 1;
         T.pix[i] = make_uint2(((uint32_t)code[q] << 16) | (uint32_t)(u[q] & 0xffff),
                               __float_as_uint(pen_term(u[q], dl[q], rho, standard)));
         T.lam[i] = dl[q];
@@ -894,8 +895,13 @@ __global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_
   const bool standard = (MODE == MODE_PRIMAL) && a.standard_admm;
   int* ids = a.grouped + off;
 
-  // -- activity test: does any cell of this partition need a move? ---------
-  if (MODE == MODE_PRIMAL && !pert) {
+  // -- activity test: does any cell of this partition need a move?  With
+  // every cell on a correct-parity row, a partition whose pixels all have
+  // usage <= 1 has none (its cells' footprints lie inside it); that test is
+  // fused into the tile staging below.  Otherwise each cell is checked.
+  const bool fast_ok = MODE == MODE_PRIMAL && !pert;
+  const bool cell_test = fast_ok && ctl->wrong_row > 0;
+  if (cell_test) {
     bool need = false;
     for (int i = lane; i < count; i += kWave) {
       int4 rc = cellrec[ids[i]];
@@ -935,7 +941,23 @@ __global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_
   int* slots = keys + npad_hi;
   int* aux = slots + npad_hi;
 
-  if (stage_tile(a, g, T, rho, standard)) ctl->error = kErrUsageRange;
+  bool over = false;
+  if (stage_tile(a, g, T, rho, standard, over)) ctl->error = kErrUsageRange;
+  if (fast_ok && !cell_test && !__any(over)) {
+    // no cell can move: dual update from the staged tile (usage unchanged,
+    // overflow zero)
+    int lb = lane / g.xh, la = lane - lb * g.xh;
+    for (int i = lane; i < npix; i += kWave) {
+      int xx = g.gx(la), yy = g.gy(lb);
+      la += step_a; lb += step_b;
+      if (la >= g.xh) { la -= g.xh; lb++; }
+      if (xx < 0 || yy < 0) continue;
+      a.dual[pidx(a, xx, yy)] = dual_step((int)(T.pix[i].x & 0xffffu), T.lam[i], rho, step, neg);
+    }
+    if (lane == 0) a.sorted[k] = 0;
+    PHASE_MARK(4);
+    return;
+  }
   for (int b = lane; b < g.yh; b += kWave) {
     int yy = g.gy(b);
     T.vdd[b] = yy >= 0 ? a.vdd_up[yy] : 0;
@@ -1085,6 +1107,9 @@ __global__ void k_pack_rec(Args a) {
   r.z = a.ox[c];
   r.w = a.oy[c];
   reinterpret_cast<int4*>(a.cellrec)[c] = r;
+  // moves never leave a cell on a wrong-parity row (such candidates cost
+  // +inf), so this count only decreases during the solve
+  if (!correct_row(a.h[c], a.gnd[c], a.vdd_up[a.ly[c]])) atomicAdd(&a.ctl->wrong_row, 1);
 }
 
 __global__ void k_unpack_pos(Args a) {
