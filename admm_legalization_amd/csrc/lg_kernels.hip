@@ -42,8 +42,20 @@ constexpr float kInf = __builtin_huge_valf();
     if (threadIdx.x == 0) atomicAdd(&a.ctl->phase_ticks[i], t_ - t_phase_);    \
     t_phase_ = t_;                                                             \
   } while (0)
-#define PHASE_START() unsigned long long t_phase_ = wall_clock64()
+#define PHASE_START() unsigned long long t_phase_ = wall_clock64(), t_begin_ = t_phase_
+#define PHASE_END()                                                            \
+  do {                                                                         \
+    unsigned long long d_ = wall_clock64() - t_begin_;                         \
+    if (threadIdx.x == 0) {                                                    \
+      atomicAdd(&a.ctl->phase_ticks[9], d_);                                   \
+      atomicMax(&a.ctl->phase_ticks[10], d_);                                  \
+    }                                                                          \
+  } while (0)
+#define PHASE_COUNT(i) \
+  do { if (threadIdx.x == 0) atomicAdd(&a.ctl->phase_ticks[i], 1ull); } while (0)
 #else
+#define PHASE_END() do {} while (0)
+#define PHASE_COUNT(i) do {} while (0)
 #define PHASE_MARK(i) do {} while (0)
 #define PHASE_START() do {} while (0)
 #endif
@@ -549,7 +561,7 @@ template <int MODE>
 __device__ __forceinline__ void eval_candidates(const Args& a, const PartGeo& g, const Tile& T,
                                                 float disp_coeff, float thr, int cx, int cy,
                                                 int w, int h, int code, int gnd, int ox, int oy,
-                                                const int2 dir[4], float cst[4]) {
+                                                const int2 dir[4], float cst[4], int ucode) {
   const int lane = threadIdx.x;
   int lxi[4], lyi0[4];
   bool bad[4];
@@ -577,28 +589,72 @@ __device__ __forceinline__ void eval_candidates(const Args& a, const PartGeo& g,
       bad[j] = !ok;
     }
   }
-  if (g.ncol > 1 && g.nrow > 1) {
-    // footprints inside one interval never wrap: flat walk, x outer
+  if (ucode >= 0 && g.ncol > 1 && g.nrow > 1) {
+    // every in-grid pixel of the tile carries code `ucode`: the code test is
+    // one comparison, and the walk only sums the penalty terms
+    const float* tt = reinterpret_cast<const float*>(T.pix) + 1;  // .y of pix[i] at tt[2 i]
+    int col[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      if (code != ucode) bad[j] = true;
+      col[j] = 2 * (lyi0[j] * g.xh + lxi[j]);
+    }
+    const int n = w * h;
+    int fx = 0, fy = 0;
+    for (int f0 = 0; f0 < n; f0 += 4) {
+      int off[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        off[q] = 2 * (fy * g.xh + fx);
+        if (++fy == h) { fy = 0; fx++; }
+      }
+      float e[4][4];
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) e[q][j] = tt[(bad[j] || f0 + q >= n) ? 0 : col[j] + off[q]];
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        if (f0 + q < n) {
+#pragma unroll
+          for (int j = 0; j < 4; j++) s[j] += e[q][j];
+        }
+    }
+  } else if (g.ncol > 1 && g.nrow > 1) {
+    // footprints inside one interval never wrap: flat walk over the
+    // footprint in x-major order (f = x * h + y), four pixels per step so
+    // that sixteen LDS reads are in flight; the sums keep that order
     int col[4];
 #pragma unroll
     for (int j = 0; j < 4; j++) col[j] = lyi0[j] * g.xh + lxi[j];
-    for (int x = 0; x < w; x++) {
-      for (int y = 0; y < h; y++) {
-        uint2 e[4];
+    const int n = w * h;
+    int fx = 0, fy = 0;
+    for (int f0 = 0; f0 < n; f0 += 4) {
+      int off[4];
 #pragma unroll
-        for (int j = 0; j < 4; j++) e[j] = T.pix[bad[j] ? 0 : col[j] + y * g.xh];
+      for (int q = 0; q < 4; q++) {
+        off[q] = fy * g.xh + fx;
+        if (++fy == h) { fy = 0; fx++; }
+      }
+      uint2 e[4][4];
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-          if ((int)(e[j].x >> 16) != code) bad[j] = true;
-          if (MODE == MODE_REFINE) {
-            if ((e[j].x & 0xffffu) != 0u) bad[j] = true;
-          } else {
-            s[j] += __uint_as_float(e[j].y);
+      for (int q = 0; q < 4; q++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) e[q][j] = T.pix[(bad[j] || f0 + q >= n) ? 0 : col[j] + off[q]];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        if (f0 + q < n) {
+#pragma unroll
+          for (int j = 0; j < 4; j++) {
+            if ((int)(e[q][j].x >> 16) != code) bad[j] = true;
+            if (MODE == MODE_REFINE) {
+              if ((e[q][j].x & 0xffffu) != 0u) bad[j] = true;
+            } else {
+              s[j] += __uint_as_float(e[q][j].y);
+            }
           }
         }
       }
-#pragma unroll
-      for (int j = 0; j < 4; j++) col[j] += 1;
     }
   } else {
     for (int x = 0; x < w; x++) {
@@ -695,21 +751,23 @@ __device__ __forceinline__ void reg_bitonic(int n, int (&key)[R], int (&val)[R])
 // consecutive pixels of a grid row; four loads in flight per lane.
 // Returns true if a usage does not fit the 16-bit LDS packing.
 __device__ __forceinline__ bool stage_tile(const Args& a, const PartGeo& g, const Tile& T, float rho,
-                                           bool standard, bool& over) {
+                                           bool standard, bool& over, int& cmin, int& cmax) {
   const int lane = threadIdx.x;
   const int npix = g.xh * g.yh;
   const int step_b = kWave / g.xh, step_a = kWave - step_b * g.xh;
   int lb = lane / g.xh, la = lane - lb * g.xh;
   bool range_err = false;
-  constexpr int U = 4;
+  constexpr int U = 8;
   for (int i0 = 0; i0 < npix; i0 += U * kWave) {
     int u[U], code[U];
     float dl[U];
+    bool in[U];
 #pragma unroll
     for (int q = 0; q < U; q++) {
       int xx = g.gx(la), yy = g.gy(lb);
       bool ok = (i0 + q * kWave + lane < npix) && xx >= 0 && yy >= 0;
       size_t p = ok ? pidx(a, xx, yy) : 0;
+      in[q] = ok;
       u[q] = ok ? a.usage[p] : 0;
       code[q] = ok ? (int)(uint16_t)a.pcode[p] : 0;  // code 0: not placeable
       dl[q] = ok ? a.dual[p] : 0.0f;
@@ -719,6 +777,10 @@ __device__ __forceinline__ bool stage_tile(const Args& a, const PartGeo& g, cons
 #pragma unroll
     for (int q = 0; q < U; q++) {
       int i = i0 + q * kWave + lane;
+      if (in[q]) {
+        cmin = min(cmin, code[q]);
+        cmax = max(cmax, code[q]);
+      }
       if (i < npix) {
         range_err |= u[q] > 0xffff;
         over |= u[q] > 1;
@@ -928,6 +990,7 @@ __global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_
         a.sorted[k] = 0;
       }
       PHASE_MARK(4);
+      PHASE_END();
       return;
     }
   }
@@ -942,7 +1005,8 @@ __global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_
   int* aux = slots + npad_hi;
 
   bool over = false;
-  if (stage_tile(a, g, T, rho, standard, over)) ctl->error = kErrUsageRange;
+  int cmin = 0x7fffffff, cmax = -1;
+  if (stage_tile(a, g, T, rho, standard, over, cmin, cmax)) ctl->error = kErrUsageRange;
   if (fast_ok && !cell_test && !__any(over)) {
     // no cell can move: dual update from the staged tile (usage unchanged,
     // overflow zero)
@@ -956,14 +1020,23 @@ __global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_
     }
     if (lane == 0) a.sorted[k] = 0;
     PHASE_MARK(4);
+    PHASE_END();
     return;
   }
   for (int b = lane; b < g.yh; b += kWave) {
     int yy = g.gy(b);
     T.vdd[b] = yy >= 0 ? a.vdd_up[yy] : 0;
   }
+  // one placeable code on every in-grid pixel of the tile: candidate
+  // evaluation then needs no per-pixel code test
+  for (int o = 1; o < kWave; o <<= 1) {
+    cmin = min(cmin, __shfl_xor(cmin, o));
+    cmax = max(cmax, __shfl_xor(cmax, o));
+  }
+  const int ucode = (MODE == MODE_PRIMAL && cmin == cmax && cmin > 0) ? cmin : -1;
   __syncthreads();
   PHASE_MARK(0);
+  PHASE_COUNT(8);
 
   if (count > 0) {
     int4* recs = reinterpret_cast<int4*>(a.rec_b) + off;
@@ -1025,7 +1098,7 @@ __global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_
         footprint_add(g, T, cx, cy, w, h, -1, rho, standard);  // rip-up
         __syncthreads();
         float cst[4];
-        eval_candidates<MODE>(a, g, T, disp_coeff, thr, cx, cy, w, h, code, gnd, pox, poy, dir, cst);
+        eval_candidates<MODE>(a, g, T, disp_coeff, thr, cx, cy, w, h, code, gnd, pox, poy, dir, cst, ucode);
         // Tree-min of AdmmKernels.hpp:847-858 over slots s = j*64 + lane: a
         // halving-stride tournament in which a tie keeps the lower slot.  Its
         // winner is the lexicographic minimum of (cost, bit-reverse8(slot)),
@@ -1095,6 +1168,7 @@ __global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_
     if (lane == 0 && ovf) atomicAdd(&ctl->ovf_acc, ovf);
   }
   PHASE_MARK(3);
+  PHASE_END();
 }
 
 // Cell records <-> SoA positions (records hold positions during the solve)
