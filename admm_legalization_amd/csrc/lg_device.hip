@@ -305,6 +305,22 @@ int lg_create(const LgGridDesc* g, const LgCellsSoA* c, const LgParams* p, LgHan
   if ((st = halloc(h, &a.rec_b, 4 * (size_t)N))) return bail(st);
   if ((st = halloc(h, &a.ids_a, N))) return bail(st);
   if ((st = halloc(h, &a.sorted, a.K))) return bail(st);
+  if ((st = halloc(h, &a.act_flags, 2 * (size_t)a.K))) return bail(st);
+  {
+    // partition column / row of every x / y for the three fold offsets
+    // (paintPixelsKernel, AdmmKernels.hpp:164-191)
+    std::vector<int32_t> cm(3 * (size_t)g->x_size), rm(3 * (size_t)g->y_size);
+    for (int tf = 0; tf < 3; tf++) {
+      const int ox = p->x_hint / 3 * tf, oy = p->y_hint / 3 * tf;
+      for (int x = 0; x < g->x_size; x++) cm[tf * (size_t)g->x_size + x] = ((x + ox) / p->x_hint) % a.ncol;
+      for (int y = 0; y < g->y_size; y++) rm[tf * (size_t)g->y_size + y] = ((y + oy) / p->y_hint) % a.nrow;
+    }
+    int32_t *d_cm, *d_rm;
+    if ((st = upload(h, &d_cm, cm.data(), cm.size()))) return bail(st);
+    if ((st = upload(h, &d_rm, rm.data(), rm.size()))) return bail(st);
+    a.colmap = d_cm;
+    a.rowmap = d_rm;
+  }
   if ((st = halloc(h, &a.ovf_flag, N))) return bail(st);
   if ((st = halloc(h, &a.ovf_list, N))) return bail(st);
   if ((st = halloc(h, &a.trace, std::max(p->max_admm_iter, 1)))) return bail(st);
@@ -364,6 +380,9 @@ int lg_solve(LgHandle* h, void* stream) {
     hipLaunchKernelGGL(k_pack_rec, dim3(nb), dim3(256), 0, s, a);
   }
   hipLaunchKernelGGL(k_fill_dual, dim3(2048), dim3(256), 0, s, a);
+  const dim3 pix_grid((a.x_size + 255) / 256, a.y_size);
+  HIP_TRY(hipMemsetAsync(a.act_flags, 0, 2 * (size_t)a.K * sizeof(int32_t), s));
+  hipLaunchKernelGGL(k_dual_classify, pix_grid, dim3(256), 0, s, a, 1);
   HIP_TRY(hipGetLastError());
   int st;
   if (N > 0 && h->p.max_admm_iter > 0) {
@@ -378,6 +397,7 @@ int lg_solve(LgHandle* h, void* stream) {
           hipLaunchKernelGGL(k_adaptive, dim3(1), dim3(1024), 0, s, a);
         }
         if ((st = launch_partition<MODE_PRIMAL>(h, s))) return st;
+        hipLaunchKernelGGL(k_dual_classify, pix_grid, dim3(256), 0, s, a, 0);
         hipLaunchKernelGGL(k_finish, dim3(1), dim3(1), 0, s, a);
       }
       HIP_TRY(hipGetLastError());

@@ -102,6 +102,9 @@ struct Args {
   int32_t* rec_b;      // [4N] cell records in processing order (int4)
   int32_t* ids_a;      // [N]  cell ids in ascending-id order
   int32_t* sorted;     // [K]  1 if the partition's order was sorted in the last primal step
+  int32_t* act_flags;  // [2][K] partition has a pixel with usage > 1 (by iteration parity)
+  const int32_t* colmap;  // [3][x_size] partition column of x for each fold offset
+  const int32_t* rowmap;  // [3][y_size] partition row of y for each fold offset
   int32_t* ovf_flag;   // [N] adaptive search: cell has overflow
   int32_t* ovf_list;   // [N]
   // parameters
@@ -272,6 +275,9 @@ __global__ __launch_bounds__(1024) void k_scan(Args a) {
     a.fill[i] = 0;
     a.cnt[i] = 0;
   }
+  // activity flags of the next iteration, filled by k_dual_classify
+  int32_t* nf = a.act_flags + (size_t)((ctl->iter + 1) & 1) * K;
+  for (int i = t; i < K; i += 1024) nf[i] = 0;
   if (t == 0) {
     int total = s_part[1023];
     a.off[K] = total;
@@ -963,6 +969,12 @@ __global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_
   // fused into the tile staging below.  Otherwise each cell is checked.
   const bool fast_ok = MODE == MODE_PRIMAL && !pert;
   const bool cell_test = fast_ok && ctl->wrong_row > 0;
+  if (fast_ok && !cell_test && !adaptive_now && a.act_flags[(size_t)(iter & 1) * a.K + k] == 0) {
+    // no pixel above usage one: no cell moves, and k_dual_classify updates
+    // this partition's prices
+    if (lane == 0) a.sorted[k] = 0;
+    return;
+  }
   if (cell_test) {
     bool need = false;
     for (int i = lane; i < count; i += kWave) {
@@ -1169,6 +1181,40 @@ __global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_
   }
   PHASE_MARK(3);
   PHASE_END();
+}
+
+// Grid-wide pass after the primal step: the dual update
+// (dual_update_functor_with_step_size, AdmmKernels.hpp:26-42) of every pixel
+// whose partition took no primal step this iteration, and the activity flags
+// (any pixel with usage > 1) of each partition under the next iteration's
+// fold offset.  One thread per pixel, rows in blockIdx.y.  With init != 0
+// only the flags of the current iteration are computed.
+__global__ __launch_bounds__(256) void k_dual_classify(Args a, int init) {
+  const Ctl* ctl = a.ctl;
+  if (ctl->done) return;
+  const int y = blockIdx.y;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= a.x_size) return;
+  const int iter = ctl->iter;
+  const int nxt = init ? iter : iter + 1;
+  const size_t p = pidx(a, x, y);
+  const int u = a.usage[p];
+  if (!init) {
+    const bool adaptive_now = a.use_adaptive && ctl->exceptional;
+    if (iter >= a.iter_perturbation_on && ctl->wrong_row == 0 && !adaptive_now) {
+      const int tf = triple_fold(a, iter);
+      const int kc = a.colmap[tf * a.x_size + x] * a.nrow + a.rowmap[tf * a.y_size + y];
+      if (a.act_flags[(size_t)(iter & 1) * a.K + kc] == 0) {
+        const float rho = ctl->rho;
+        a.dual[p] = dual_step(u, a.dual[p], rho, ctl->step, -1.0f / rho);
+      }
+    }
+  }
+  if (u > 1) {
+    const int tf = triple_fold(a, nxt);
+    const int kn = a.colmap[tf * a.x_size + x] * a.nrow + a.rowmap[tf * a.y_size + y];
+    a.act_flags[(size_t)(nxt & 1) * a.K + kn] = 1;
+  }
 }
 
 // Cell records <-> SoA positions (records hold positions during the solve)
