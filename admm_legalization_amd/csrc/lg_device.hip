@@ -124,8 +124,11 @@ size_t sort_bytes_per(const LgHandle* h) {
 // LDS of one partition workgroup: pixel tile {usage|code, dual} + dual/rho +
 // vdd rows, plus the sort buffers when the sorts run in LDS.
 size_t partition_lds_bytes(const LgHandle* h, int npad_hi, int lds_sort) {
-  size_t npix = (size_t)h->a.x_hint * h->a.y_hint;
-  size_t tile = npix * 8 + ((npix + 3) & ~(size_t)3) * 4 + (size_t)((h->a.y_hint + 3) & ~3) * 4;  // pix, lam, vdd
+  // three tile arrays (usage|code, penalty term, dual) with the kernel's row
+  // stride, plus the vdd rows
+  const size_t rs = h->a.x_hint <= kRowStride ? kRowStride : h->a.x_hint;
+  const size_t tsz = (rs * h->a.y_hint + 3) & ~(size_t)3;
+  size_t tile = 3 * tsz * 4 + (size_t)((h->a.y_hint + 3) & ~3) * 4;
   return tile + (lds_sort ? (size_t)npad_hi * sort_bytes_per(h) : 0);
 }
 

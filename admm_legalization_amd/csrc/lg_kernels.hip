@@ -53,7 +53,25 @@ constexpr float kInf = __builtin_huge_valf();
   } while (0)
 #define PHASE_COUNT(i) \
   do { if (threadIdx.x == 0) atomicAdd(&a.ctl->phase_ticks[i], 1ull); } while (0)
+#define SUB_START() unsigned long long t_sub_ = wall_clock64()
+#define SUB_MARK(i)                                                            \
+  do {                                                                         \
+    unsigned long long t_ = wall_clock64();                                    \
+    sub_acc_[(i) - 11] += t_ - t_sub_;                                         \
+    t_sub_ = t_;                                                               \
+  } while (0)
+#define SUB_DECL() unsigned long long sub_acc_[6] = {0, 0, 0, 0, 0, 0}
+#define SUB_FLUSH()                                                            \
+  do {                                                                         \
+    if (threadIdx.x == 0)                                                      \
+      for (int q_ = 0; q_ < 6; q_++)                                           \
+        if (sub_acc_[q_]) atomicAdd(&a.ctl->phase_ticks[q_ < 5 ? 11 + q_ : 7], sub_acc_[q_]); \
+  } while (0)
 #else
+#define SUB_DECL() do {} while (0)
+#define SUB_FLUSH() do {} while (0)
+#define SUB_START() do {} while (0)
+#define SUB_MARK(i) do {} while (0)
 #define PHASE_END() do {} while (0)
 #define PHASE_COUNT(i) do {} while (0)
 #define PHASE_MARK(i) do {} while (0)
@@ -492,16 +510,22 @@ __device__ __forceinline__ void new_pos(const Args& a, int2 d, int w, int h, int
 
 enum { MODE_PRIMAL = 0, MODE_REFINE = 1 };
 
-// LDS tile of the partition: pix[i] = {usage | code << 16, t}, lam[i] = dual,
-// where t is the penalty term of computePenaltySum for the pixel's current
-// usage u (AdmmKernels.hpp:380-383):
+// LDS tile of the partition: three arrays with row stride rs (kRowStride
+// when x_hint fits, else x_hint), element ly * rs + lx for local column lx and
+// row ly:  uc = usage | code << 16,  t = penalty term,  lam = dual, where t is
+// the term of computePenaltySum for the pixel's current usage u
+// (AdmmKernels.hpp:380-383):
 //   t = rho * max(lam * (u - 1) + lam / rho, 0)     (negotiation ADMM)
 //   t = rho * max(u - 1 + lam / rho, 0)             (standard ADMM)
 // t is recomputed, with the same operations, whenever u changes.
+constexpr int kRowStride = 256;
 struct Tile {
-  uint2* pix;
+  uint32_t* uc;
+  float* t;
   float* lam;
   int* vdd;
+  int rs;
+  __device__ int at(int lx, int ly) const { return ly * rs + lx; }
 };
 
 __device__ __forceinline__ float pen_term(int ui, float l, float rho, bool standard) {
@@ -514,14 +538,14 @@ __device__ __forceinline__ float pen_term(int ui, float l, float rho, bool stand
 }
 
 // overflow of a footprint from the LDS tile (computeCellOverflow, AdmmKernels.hpp:393-406)
-__device__ __forceinline__ int ovf_lds(const PartGeo& g, const uint2* pix, int cx, int cy, int w, int h) {
+__device__ __forceinline__ int ovf_lds(const PartGeo& g, const Tile& T, int cx, int cy, int w, int h) {
   int s = 0;
   int lxi = g.lx(cx);
   const int ly0 = g.ly(cy);
   for (int x = 0; x < w; x++) {
     int lyi = ly0;
     for (int y = 0; y < h; y++) {
-      s += max((int)(pix[lyi * g.xh + lxi].x & 0xffffu) - 1, 0);
+      s += max((int)(T.uc[T.at(lxi, lyi)] & 0xffffu) - 1, 0);
       if (++lyi == g.yh) lyi = 0;
     }
     if (++lxi == g.xh) lxi = 0;
@@ -529,7 +553,7 @@ __device__ __forceinline__ int ovf_lds(const PartGeo& g, const uint2* pix, int c
   return s;
 }
 
-// the same from the pixels in HBM (x-major), for a cell inside the grid
+// the same from the pixels in HBM, for a cell inside the grid
 __device__ __forceinline__ int ovf_hbm(const Args& a, int cx, int cy, int w, int h) {
   int s = 0;
   for (int x = cx; x < cx + w; x++)
@@ -548,11 +572,45 @@ __device__ __forceinline__ void footprint_add(const PartGeo& g, const Tile& T, i
     int lxi = lx0 + dx, lyi = ly0 + (i - dx * h);
     if (lxi >= g.xh) lxi -= g.xh;
     if (lyi >= g.yh) lyi -= g.yh;
-    const int idx = lyi * g.xh + lxi;
-    uint2 e = T.pix[idx];
-    e.x = (uint32_t)((int)e.x + delta);
-    e.y = __float_as_uint(pen_term((int)(e.x & 0xffffu), T.lam[idx], rho, standard));
-    T.pix[idx] = e;
+    const int idx = T.at(lxi, lyi);
+    const uint32_t e = (uint32_t)((int)T.uc[idx] + delta);
+    T.uc[idx] = e;
+    T.t[idx] = pen_term((int)(e & 0xffffu), T.lam[idx], rho, standard);
+  }
+}
+
+// Penalty sums of Q consecutive footprint columns (x .. x+Q-1) of height H
+// for the 4 candidates, row stride kRowStride: every LDS read uses an
+// immediate offset from one address per candidate.  The sums run over x,
+// then y, as computePenaltySum's loops.
+template <int H, int Q>
+__device__ __forceinline__ void sum_cols(const float* t, const int (&base)[4], int x, float (&s)[4]) {
+  float v[4][Q][H];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const float* p = t + base[j] + x;
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+#pragma unroll
+      for (int y = 0; y < H; y++) v[j][q][y] = p[y * kRowStride + q];
+  }
+#pragma unroll
+  for (int j = 0; j < 4; j++)
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+#pragma unroll
+      for (int y = 0; y < H; y++) s[j] += v[j][q][y];
+}
+
+template <int H>
+__device__ __forceinline__ void sum_rect(const float* t, const int (&base)[4], int w, float (&s)[4]) {
+  int x = 0;
+  for (; x + 4 <= w; x += 4) sum_cols<H, 4>(t, base, x, s);
+  switch (w - x) {
+    case 1: sum_cols<H, 1>(t, base, x, s); break;
+    case 2: sum_cols<H, 2>(t, base, x, s); break;
+    case 3: sum_cols<H, 3>(t, base, x, s); break;
+    default: break;
   }
 }
 
@@ -570,12 +628,12 @@ __device__ __forceinline__ void eval_candidates(const Args& a, const PartGeo& g,
                                                 const int2 dir[4], float cst[4], int ucode) {
   const int lane = threadIdx.x;
   int lxi[4], lyi0[4];
-  bool bad[4];
+  int bad[4];  // 0 / 1 (ints: bool arrays get packed into bytes)
   float s[4];
 #pragma unroll
   for (int j = 0; j < 4; j++) {
     int d = j * kWave + lane;
-    bad[j] = true;
+    bad[j] = 1;
     lxi[j] = 0;
     lyi0[j] = 0;
     s[j] = 0.0f;
@@ -592,72 +650,59 @@ __device__ __forceinline__ void eval_candidates(const Args& a, const PartGeo& g,
         lxi[j] = g.lx(nx);
         ok = correct_row(h, gnd, T.vdd[lyi0[j]]);
       }
-      bad[j] = !ok;
+      bad[j] = ok ? 0 : 1;
     }
   }
-  if (ucode >= 0 && g.ncol > 1 && g.nrow > 1) {
+  const bool flat = g.ncol > 1 && g.nrow > 1;  // footprints inside one interval never wrap
+  if (ucode >= 0 && flat && T.rs == kRowStride && h <= 4) {
     // every in-grid pixel of the tile carries code `ucode`: the code test is
-    // one comparison, and the walk only sums the penalty terms
-    const float* tt = reinterpret_cast<const float*>(T.pix) + 1;  // .y of pix[i] at tt[2 i]
-    int col[4];
+    // one comparison, and the walk only sums the penalty terms (a bad
+    // candidate sums junk from row 0, its cost is +inf anyway)
+    int base[4];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-      if (code != ucode) bad[j] = true;
-      col[j] = 2 * (lyi0[j] * g.xh + lxi[j]);
+      if (code != ucode) bad[j] = 1;
+      base[j] = bad[j] ? 0 : lyi0[j] * kRowStride + lxi[j];
     }
-    const int n = w * h;
-    int fx = 0, fy = 0;
-    for (int f0 = 0; f0 < n; f0 += 4) {
-      int off[4];
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        off[q] = 2 * (fy * g.xh + fx);
-        if (++fy == h) { fy = 0; fx++; }
-      }
-      float e[4][4];
-#pragma unroll
-      for (int q = 0; q < 4; q++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) e[q][j] = tt[(bad[j] || f0 + q >= n) ? 0 : col[j] + off[q]];
-#pragma unroll
-      for (int q = 0; q < 4; q++)
-        if (f0 + q < n) {
-#pragma unroll
-          for (int j = 0; j < 4; j++) s[j] += e[q][j];
-        }
+    switch (h) {
+      case 1: sum_rect<1>(T.t, base, w, s); break;
+      case 2: sum_rect<2>(T.t, base, w, s); break;
+      case 3: sum_rect<3>(T.t, base, w, s); break;
+      default: sum_rect<4>(T.t, base, w, s); break;
     }
-  } else if (g.ncol > 1 && g.nrow > 1) {
-    // footprints inside one interval never wrap: flat walk over the
-    // footprint in x-major order (f = x * h + y), four pixels per step so
-    // that sixteen LDS reads are in flight; the sums keep that order
+  } else if (flat) {
+    // flat walk over the footprint in x-major order (f = x * h + y), four
+    // pixels per step so that sixteen LDS reads are in flight
     int col[4];
 #pragma unroll
-    for (int j = 0; j < 4; j++) col[j] = lyi0[j] * g.xh + lxi[j];
+    for (int j = 0; j < 4; j++) col[j] = T.at(lxi[j], lyi0[j]);
     const int n = w * h;
     int fx = 0, fy = 0;
     for (int f0 = 0; f0 < n; f0 += 4) {
       int off[4];
 #pragma unroll
       for (int q = 0; q < 4; q++) {
-        off[q] = fy * g.xh + fx;
+        off[q] = fy * T.rs + fx;
         if (++fy == h) { fy = 0; fx++; }
       }
-      uint2 e[4][4];
+      uint32_t e[4][4];
+      float tv[4][4];
 #pragma unroll
       for (int q = 0; q < 4; q++)
 #pragma unroll
-        for (int j = 0; j < 4; j++) e[q][j] = T.pix[(bad[j] || f0 + q >= n) ? 0 : col[j] + off[q]];
+        for (int j = 0; j < 4; j++) {
+          const int i = (bad[j] || f0 + q >= n) ? 0 : col[j] + off[q];
+          e[q][j] = T.uc[i];
+          tv[q][j] = T.t[i];
+        }
 #pragma unroll
       for (int q = 0; q < 4; q++) {
         if (f0 + q < n) {
 #pragma unroll
           for (int j = 0; j < 4; j++) {
-            if ((int)(e[q][j].x >> 16) != code) bad[j] = true;
-            if (MODE == MODE_REFINE) {
-              if ((e[q][j].x & 0xffffu) != 0u) bad[j] = true;
-            } else {
-              s[j] += __uint_as_float(e[q][j].y);
-            }
+            bad[j] |= (int)(e[q][j] >> 16) != code;
+            if (MODE == MODE_REFINE) bad[j] |= (e[q][j] & 0xffffu) != 0u;
+            else s[j] += tv[q][j];
           }
         }
       }
@@ -668,17 +713,19 @@ __device__ __forceinline__ void eval_candidates(const Args& a, const PartGeo& g,
 #pragma unroll
       for (int j = 0; j < 4; j++) lyi[j] = lyi0[j];
       for (int y = 0; y < h; y++) {
-        uint2 e[4];
-#pragma unroll
-        for (int j = 0; j < 4; j++) e[j] = T.pix[bad[j] ? 0 : lyi[j] * g.xh + lxi[j]];
+        uint32_t e[4];
+        float tv[4];
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-          if ((int)(e[j].x >> 16) != code) bad[j] = true;
-          if (MODE == MODE_REFINE) {
-            if ((e[j].x & 0xffffu) != 0u) bad[j] = true;
-          } else {
-            s[j] += __uint_as_float(e[j].y);
-          }
+          const int i = bad[j] ? 0 : T.at(lxi[j], lyi[j]);
+          e[j] = T.uc[i];
+          tv[j] = T.t[i];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          bad[j] |= (int)(e[j] >> 16) != code;
+          if (MODE == MODE_REFINE) bad[j] |= (e[j] & 0xffffu) != 0u;
+          else s[j] += tv[j];
           if (++lyi[j] == g.yh) lyi[j] = 0;
         }
       }
@@ -765,7 +812,7 @@ __device__ __forceinline__ bool stage_tile(const Args& a, const PartGeo& g, cons
   bool range_err = false;
   constexpr int U = 8;
   for (int i0 = 0; i0 < npix; i0 += U * kWave) {
-    int u[U], code[U];
+    int u[U], code[U], li[U];
     float dl[U];
     bool in[U];
 #pragma unroll
@@ -773,6 +820,7 @@ __device__ __forceinline__ bool stage_tile(const Args& a, const PartGeo& g, cons
       int xx = g.gx(la), yy = g.gy(lb);
       bool ok = (i0 + q * kWave + lane < npix) && xx >= 0 && yy >= 0;
       size_t p = ok ? pidx(a, xx, yy) : 0;
+      li[q] = T.at(la, lb);
       in[q] = ok;
       u[q] = ok ? a.usage[p] : 0;
       code[q] = ok ? (int)(uint16_t)a.pcode[p] : 0;  // code 0: not placeable
@@ -790,9 +838,9 @@ __device__ __forceinline__ bool stage_tile(const Args& a, const PartGeo& g, cons
       if (i < npix) {
         range_err |= u[q] > 0xffff;
         over |= u[q] > 1;
-        T.pix[i] = make_uint2(((uint32_t)code[q] << 16) | (uint32_t)(u[q] & 0xffff),
-                              __float_as_uint(pen_term(u[q], dl[q], rho, standard)));
-        T.lam[i] = dl[q];
+        T.uc[li[q]] = ((uint32_t)code[q] << 16) | (uint32_t)(u[q] & 0xffff);
+        T.t[li[q]] = pen_term(u[q], dl[q], rho, standard);
+        T.lam[li[q]] = dl[q];
       }
     }
   }
@@ -804,7 +852,7 @@ __device__ __forceinline__ int sort_key(const Args& a, const PartGeo& g, const T
   int cr = 0, co = 10, ca = 1;
   if (a.sort_policy == 0) { cr = 1; co = 0; ca = 0; }
   int w = rec_w(rc.y), h = rec_h(rc.y);
-  int ovf = co ? ovf_lds(g, T.pix, rec_x(rc.x), rec_y(rc.x), w, h) : 0;
+  int ovf = co ? ovf_lds(g, T, rec_x(rc.x), rec_y(rc.x), w, h) : 0;
   return cr * (cr ? a.rand_prio[c] : 0) + co * ovf + ca * w * h;
 }
 
@@ -942,6 +990,7 @@ __global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_
     iter = ctl->conv_iter;
   }
   PHASE_START();
+  SUB_DECL();
   const int k = blockIdx.x;
   const int lane = threadIdx.x;
   PartGeo g;
@@ -1009,9 +1058,12 @@ __global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_
 
   extern __shared__ __align__(16) unsigned char lds_raw[];
   Tile T;
-  T.pix = reinterpret_cast<uint2*>(lds_raw);                   // [npix]
-  T.lam = reinterpret_cast<float*>(T.pix + npix);              // [npix]
-  T.vdd = reinterpret_cast<int*>(T.lam + ((npix + 3) & ~3));   // [yh]
+  T.rs = g.xh <= kRowStride ? kRowStride : g.xh;
+  const int tsz = (T.rs * g.yh + 3) & ~3;
+  T.uc = reinterpret_cast<uint32_t*>(lds_raw);                 // [rs * yh]
+  T.t = reinterpret_cast<float*>(T.uc + tsz);                  // [rs * yh]
+  T.lam = T.t + tsz;                                           // [rs * yh]
+  T.vdd = reinterpret_cast<int*>(T.lam + tsz);                 // [yh]
   int* keys = T.vdd + ((g.yh + 3) & ~3);                       // LDS sorts only
   int* slots = keys + npad_hi;
   int* aux = slots + npad_hi;
@@ -1025,10 +1077,11 @@ __global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_
     int lb = lane / g.xh, la = lane - lb * g.xh;
     for (int i = lane; i < npix; i += kWave) {
       int xx = g.gx(la), yy = g.gy(lb);
+      const int li = T.at(la, lb);
       la += step_a; lb += step_b;
       if (la >= g.xh) { la -= g.xh; lb++; }
       if (xx < 0 || yy < 0) continue;
-      a.dual[pidx(a, xx, yy)] = dual_step((int)(T.pix[i].x & 0xffffu), T.lam[i], rho, step, neg);
+      a.dual[pidx(a, xx, yy)] = dual_step((int)(T.uc[li] & 0xffffu), T.lam[li], rho, step, neg);
     }
     if (lane == 0) a.sorted[k] = 0;
     PHASE_MARK(4);
@@ -1087,7 +1140,7 @@ __global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_
         if (MODE == MODE_PRIMAL) {
           if (pert) return true;
           bool ok_row = correct_row(my_h, rec_gnd(r.y), T.vdd[g.ly(my_y)]);
-          return !ok_row || ovf_lds(g, T.pix, my_x, my_y, my_w, my_h) != 0;
+          return !ok_row || ovf_lds(g, T, my_x, my_y, my_w, my_h) != 0;
         }
         return my_h > 1;
       };
@@ -1096,6 +1149,7 @@ __global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_
       for (;;) {
         const unsigned long long m = __ballot(need && lane >= start);
         if (m == 0ull) break;
+        SUB_START();
         const int first = __builtin_amdgcn_readfirstlane(__ffsll((long long)m) - 1);
         const int px = __builtin_amdgcn_readlane(r.x, first), py = __builtin_amdgcn_readlane(r.y, first);
         const int pox = __builtin_amdgcn_readlane(r.z, first), poy = __builtin_amdgcn_readlane(r.w, first);
@@ -1105,12 +1159,14 @@ __global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_
         float disp_coeff = 1.0f;
         if (a.flag_iccad17) disp_coeff = float(a.N) / float(a.k_height_num[h - 1]);
 #ifdef LG_PHASE_TIMERS
-        if (lane == 0) atomicAdd(&a.ctl->phase_ticks[7], 1ull);
+        sub_acc_[5]++;
 #endif
         footprint_add(g, T, cx, cy, w, h, -1, rho, standard);  // rip-up
         __syncthreads();
+        SUB_MARK(11);
         float cst[4];
         eval_candidates<MODE>(a, g, T, disp_coeff, thr, cx, cy, w, h, code, gnd, pox, poy, dir, cst, ucode);
+        SUB_MARK(12);
         // Tree-min of AdmmKernels.hpp:847-858 over slots s = j*64 + lane: a
         // halving-stride tournament in which a tie keeps the lower slot.  Its
         // winner is the lexicographic minimum of (cost, bit-reverse8(slot)),
@@ -1141,9 +1197,11 @@ __global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_
           new_pos(a, dd, w, h, cx, cy, nx, ny);
         }
         __syncthreads();
+        SUB_MARK(13);
         footprint_add(g, T, nx, ny, w, h, +1, rho, standard);  // commit
         if (lane == 0) cellrec[pid].x = rec_pos(nx, ny);
         __syncthreads();
+        SUB_MARK(14);
         start = first + 1;
         if (start >= kWave) break;
         // a move changes usage only inside the old and new footprints
@@ -1152,6 +1210,7 @@ __global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_
           bool hit_new = my_x < nx + w && nx < my_x + my_w && my_y < ny + h && ny < my_y + my_h;
           if (hit_old || hit_new) need = skip_test();
         }
+        SUB_MARK(15);
       }
     }
     PHASE_MARK(2);
@@ -1163,14 +1222,15 @@ __global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_
     int lb = lane / g.xh, la = lane - lb * g.xh;
     for (int i = lane; i < npix; i += kWave) {
       int xx = g.gx(la), yy = g.gy(lb);
+      const int li = T.at(la, lb);
       la += step_a; lb += step_b;
       if (la >= g.xh) { la -= g.xh; lb++; }
       if (xx < 0 || yy < 0) continue;
       size_t p = pidx(a, xx, yy);
-      int ui = (int)(T.pix[i].x & 0xffffu);
+      int ui = (int)(T.uc[li] & 0xffffu);
       a.usage[p] = ui;
       if (MODE == MODE_PRIMAL) {
-        a.dual[p] = dual_step(ui, T.lam[i], rho, step, neg);
+        a.dual[p] = dual_step(ui, T.lam[li], rho, step, neg);
         ovf += (unsigned long long)max(ui - 1, 0);
       }
     }
@@ -1180,6 +1240,7 @@ __global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_
     if (lane == 0 && ovf) atomicAdd(&ctl->ovf_acc, ovf);
   }
   PHASE_MARK(3);
+  SUB_FLUSH();
   PHASE_END();
 }
 
