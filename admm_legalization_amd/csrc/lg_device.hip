@@ -304,9 +304,25 @@ int lg_create(const LgGridDesc* g, const LgCellsSoA* c, const LgParams* p, LgHan
   if ((st = halloc(h, &a.fill, a.K + 1))) return bail(st);
   if ((st = halloc(h, &a.grouped, N))) return bail(st);
   if ((st = halloc(h, &a.cellrec, 4 * (size_t)N))) return bail(st);
-  if ((st = halloc(h, &a.rec_a, 4 * (size_t)N))) return bail(st);
   if ((st = halloc(h, &a.rec_b, 4 * (size_t)N))) return bail(st);
-  if ((st = halloc(h, &a.ids_a, N))) return bail(st);
+  {
+    // radix grouping: passes of 7 bits over partition keys 0..K
+    int bits = 1;
+    while ((1 << bits) <= a.K) bits++;
+    a.rx_passes = (bits + 6) / 7;
+    a.rx_nb = (N + 4095) / 4096;
+    std::vector<int32_t> iota_v(std::max(N, 1));
+    std::iota(iota_v.begin(), iota_v.end(), 0);
+    int32_t* d_ident;
+    if ((st = upload(h, &d_ident, iota_v.data(), iota_v.size()))) return bail(st);
+    a.ident = d_ident;
+    if ((st = halloc(h, &a.rx_k1, N)) || (st = halloc(h, &a.rx_v1, N)) || (st = halloc(h, &a.rx_k2, N)) ||
+        (st = halloc(h, &a.rx_v2, N)) || (st = halloc(h, &a.rx_keys, N)) ||
+        (st = halloc(h, &a.rx_hist, 128 * (size_t)std::max(a.rx_nb, 1))) || (st = halloc(h, &a.pstart, a.K)) ||
+        (st = halloc(h, &a.pend, a.K)))
+      return bail(st);
+    HIP_TRY(hipMemset(a.pstart, 0xff, std::max(a.K, 1) * sizeof(int32_t)));
+  }
   if ((st = halloc(h, &a.sorted, a.K))) return bail(st);
   if ((st = halloc(h, &a.act_flags, 2 * (size_t)a.K))) return bail(st);
   {
@@ -385,6 +401,12 @@ int lg_solve(LgHandle* h, void* stream) {
   hipLaunchKernelGGL(k_fill_dual, dim3(2048), dim3(256), 0, s, a);
   const dim3 pix_grid((a.x_size + 255) / 256, a.y_size);
   HIP_TRY(hipMemsetAsync(a.act_flags, 0, 2 * (size_t)a.K * sizeof(int32_t), s));
+#ifdef LG_PHASE_TIMERS
+  if (a.rx_passes <= 2 && N > 0) {  // per-iteration block-time maxima / sums (timed builds)
+    HIP_TRY(hipMemsetAsync(a.rx_k2, 0, N * sizeof(int32_t), s));
+    HIP_TRY(hipMemsetAsync(a.rx_v2, 0, N * sizeof(int32_t), s));
+  }
+#endif
   hipLaunchKernelGGL(k_dual_classify, pix_grid, dim3(256), 0, s, a, 1);
   HIP_TRY(hipGetLastError());
   int st;
@@ -392,9 +414,23 @@ int lg_solve(LgHandle* h, void* stream) {
     int batch = 8;
     for (;;) {
       for (int b = 0; b < batch; b++) {
-        hipLaunchKernelGGL(k_assign, dim3(nb), dim3(256), 0, s, a);
+        // partition keys (+ first digit histogram), then a stable radix sort
+        // of (partition, id) into grouped
+        hipLaunchKernelGGL(k_assign, dim3(a.rx_nb), dim3(kRxThreads), 0, s, a);
+        const int32_t* kin = a.part_of;
+        const int32_t* vin = a.ident;
+        for (int pass = 0; pass < a.rx_passes; pass++) {
+          const bool last = pass == a.rx_passes - 1;
+          int32_t* kout = last ? a.rx_keys : (pass & 1 ? a.rx_k2 : a.rx_k1);
+          int32_t* vout = last ? a.grouped : (pass & 1 ? a.rx_v2 : a.rx_v1);
+          if (pass > 0) hipLaunchKernelGGL(k_rx_hist, dim3(a.rx_nb), dim3(kRxThreads), 0, s, a, kin, 7 * pass);
+          hipLaunchKernelGGL(k_rx_scan, dim3(1), dim3(1024), 0, s, a);
+          hipLaunchKernelGGL(k_rx_scatter, dim3(a.rx_nb), dim3(kRxThreads), 0, s, a, kin, vin, kout, vout, 7 * pass);
+          kin = kout;
+          vin = vout;
+        }
+        hipLaunchKernelGGL(k_part_bounds, dim3(nb), dim3(256), 0, s, a);
         hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, a);
-        hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, s, a);
         if (a.use_adaptive) {
           hipLaunchKernelGGL(k_extract_congested, dim3(nb), dim3(256), 0, s, a);
           hipLaunchKernelGGL(k_adaptive, dim3(1), dim3(1024), 0, s, a);
@@ -556,6 +592,16 @@ int lg_profile_read(LgHandle* h, double* ms, int64_t* launches) {
   }
   return LG_OK;
 }
+
+#ifdef LG_PHASE_TIMERS
+int lg_debug_iter_times(LgHandle* h, int32_t* maxt, int32_t* sumt, int32_t* slow, int n) {
+  n = std::min(n, h->a.N / 16);
+  HIP_TRY(hipMemcpy(maxt, h->a.rx_k2, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(sumt, h->a.rx_v2, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(slow, h->a.ovf_list, 16 * n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  return LG_OK;
+}
+#endif
 
 int lg_destroy(LgHandle* h) {
   if (!h) return LG_OK;

@@ -9,8 +9,9 @@
 //     region int32   region id (adaptive search projection only)
 //   cells, SoA int32: lx, ly (grid), w, h, code (= group + 2), gnd, ox, oy (dbu)
 //
-// One iteration = k_assign -> k_scan -> k_scatter -> k_partition<PRIMAL>
-// (sort + sequential cell moves + fused dual update / overflow) -> k_finish.
+// One iteration = k_assign -> radix grouping (k_rx_*, k_part_bounds) ->
+// k_scan -> k_partition<PRIMAL> (sort + sequential cell moves + fused dual
+// update / overflow) -> k_dual_classify -> k_finish.
 // Every kernel reads the iteration index and the done flag from Ctl in
 // device memory, so batches of iterations are launched without host syncs.
 //
@@ -40,15 +41,30 @@ constexpr float kInf = __builtin_huge_valf();
   do {                                                                         \
     unsigned long long t_ = wall_clock64();                                    \
     if (threadIdx.x == 0) atomicAdd(&a.ctl->phase_ticks[i], t_ - t_phase_);    \
+    ph_[(i) & 7] = (int)(t_ - t_phase_);                                       \
     t_phase_ = t_;                                                             \
   } while (0)
-#define PHASE_START() unsigned long long t_phase_ = wall_clock64(), t_begin_ = t_phase_
+#define PHASE_START()                                                          \
+  unsigned long long t_phase_ = wall_clock64(), t_begin_ = t_phase_;           \
+  int ph_[8] = {0, 0, 0, 0, 0, 0, 0, 0}
+// per-iteration slowest block: total and phase times in ovf_list[8 * iter ..]
 #define PHASE_END()                                                            \
   do {                                                                         \
     unsigned long long d_ = wall_clock64() - t_begin_;                         \
     if (threadIdx.x == 0) {                                                    \
       atomicAdd(&a.ctl->phase_ticks[9], d_);                                   \
       atomicMax(&a.ctl->phase_ticks[10], d_);                                  \
+      if (MODE == MODE_PRIMAL && a.rx_passes <= 2) {                           \
+        int old_ = atomicMax(&a.rx_k2[iter], (int)d_);                         \
+        atomicAdd(&a.rx_v2[iter], (int)d_);                                    \
+        if ((int)d_ > old_ && 16 * iter + 15 < a.N) {                          \
+          int* o_ = a.ovf_list + 16 * iter;                                    \
+          o_[0] = (int)d_; o_[1] = ph_[0]; o_[2] = ph_[1]; o_[3] = ph_[2];     \
+          o_[4] = ph_[3]; o_[5] = count; o_[6] = (int)sub_acc_[5]; o_[7] = k;  \
+          o_[8] = ph_[5]; o_[9] = ph_[6]; o_[10] = ph_[7];                     \
+          for (int q_ = 0; q_ < 5; q_++) o_[11 + q_] = (int)sub_acc_[q_];      \
+        }                                                                      \
+      }                                                                        \
     }                                                                          \
   } while (0)
 #define PHASE_COUNT(i) \
@@ -116,9 +132,14 @@ struct Args {
   int32_t* grouped;
   int32_t* trace;
   int32_t* cellrec;    // [4N] cell records (int4), the positions' home during a solve
-  int32_t* rec_a;      // [4N] cell records in ascending-id order (int4)
   int32_t* rec_b;      // [4N] cell records in processing order (int4)
-  int32_t* ids_a;      // [N]  cell ids in ascending-id order
+  // stable radix sort of the cells by partition: identity values, ping-pong
+  // key / value buffers, sorted keys, first / one-past-last slot per partition
+  const int32_t* ident;
+  int32_t *rx_k1, *rx_v1, *rx_k2, *rx_v2, *rx_keys;
+  int32_t* rx_hist;    // [128 * rx_nb]
+  int32_t *pstart, *pend;
+  int rx_nb, rx_passes;
   int32_t* sorted;     // [K]  1 if the partition's order was sorted in the last primal step
   int32_t* act_flags;  // [2][K] partition has a pixel with usage > 1 (by iteration parity)
   const int32_t* colmap;  // [3][x_size] partition column of x for each fold offset
@@ -243,21 +264,222 @@ __global__ void k_fill_dual(Args a) {
 // ---------------------------------------------------------------------------
 
 // assignPartitionKernel (AdmmKernels.hpp:122-162) + histogram
-__global__ void k_assign(Args a) {
+// -- grouping of the cells by partition (updatePartition,
+// CudaAdmmLegalizer.cu:166-228): each cell's partition under this
+// iteration's fold offset (assignPartitionKernel, AdmmKernels.hpp:122-162;
+// K for a cell whose footprint spans two partitions), then a stable LSD radix
+// sort of (partition, cell id) in 7-bit digits, so that every partition lists
+// its cells in ascending id order (the order of thrust::stable_sort_by_key).
+// 4096 items per block of 256 threads.
+constexpr int kRxItems = 4096;
+constexpr int kRxThreads = 1024;
+constexpr int kRxPer = kRxItems / kRxThreads;  // items per thread
+constexpr int kRxWaves = kRxThreads / kWave;
+
+__device__ __forceinline__ int cell_partition(const Args& a, const int* cm, const int* rm, int4 r) {
+  const int x = r.x & 0xffff, y = (int)((unsigned)r.x >> 16);
+  const int w = r.y & 0xfff, h = (r.y >> 12) & 0x3f;
+  const int c1 = cm[x], c2 = cm[x + w - 1], r1 = rm[y], r2 = rm[y + h - 1];
+  return (c1 == c2 && r1 == r2) ? c1 * a.nrow + r1 : a.K;
+}
+
+// partition keys plus the histogram of their first digit
+__global__ __launch_bounds__(kRxThreads) void k_assign(Args a) {
   const Ctl* ctl = a.ctl;
   if (ctl->done) return;
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= a.N) return;
-  int tf = triple_fold(a, ctl->iter);
-  int oxs = a.x_hint / 3 * tf, oys = a.y_hint / 3 * tf;
-  int4 r = reinterpret_cast<const int4*>(a.cellrec)[c];
-  int lxg = (r.x & 0xffff) + oxs, lyg = (int)((unsigned)r.x >> 16) + oys;
-  int uxg = lxg + (r.y & 0xfff) - 1, uyg = lyg + ((r.y >> 12) & 0x3f) - 1;
-  int c1 = (lxg / a.x_hint) % a.ncol, c2 = (uxg / a.x_hint) % a.ncol;
-  int r1 = (lyg / a.y_hint) % a.nrow, r2 = (uyg / a.y_hint) % a.nrow;
-  int p = (c1 == c2 && r1 == r2) ? c1 * a.nrow + r1 : a.K;
-  a.part_of[c] = p;
-  if (p < a.K) atomicAdd(&a.cnt[p], 1);
+  __shared__ int cnt[128];
+  if (threadIdx.x < 128) cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const int tf = triple_fold(a, ctl->iter);
+  const int* cm = a.colmap + tf * a.x_size;
+  const int* rm = a.rowmap + tf * a.y_size;
+  const int base = blockIdx.x * kRxItems + threadIdx.x;
+  const int4* rec = reinterpret_cast<const int4*>(a.cellrec);
+  int4 r[kRxPer];
+#pragma unroll
+  for (int q = 0; q < kRxPer; q++) {
+    const int c = base + q * kRxThreads;
+    r[q] = c < a.N ? rec[c] : make_int4(0, 0, 0, 0);
+  }
+#pragma unroll
+  for (int q = 0; q < kRxPer; q++) {
+    const int c = base + q * kRxThreads;
+    if (c < a.N) {
+      const int p = cell_partition(a, cm, rm, r[q]);
+      a.part_of[c] = p;
+      atomicAdd(&cnt[p & 127], 1);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 128) a.rx_hist[threadIdx.x * a.rx_nb + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// digit histogram of one block (passes after the first)
+__global__ __launch_bounds__(kRxThreads) void k_rx_hist(Args a, const int32_t* keys, int shift) {
+  if (a.ctl->done) return;
+  __shared__ int cnt[128];
+  if (threadIdx.x < 128) cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const int base = blockIdx.x * kRxItems + threadIdx.x;
+  int k[kRxPer];
+#pragma unroll
+  for (int q = 0; q < kRxPer; q++) k[q] = base + q * kRxThreads < a.N ? keys[base + q * kRxThreads] : -1;
+#pragma unroll
+  for (int q = 0; q < kRxPer; q++)
+    if (k[q] >= 0) atomicAdd(&cnt[(k[q] >> shift) & 127], 1);
+  __syncthreads();
+  if (threadIdx.x < 128) a.rx_hist[threadIdx.x * a.rx_nb + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// Exclusive scan of n ints in place by one workgroup of 1024, in chunks of
+// 8192 staged through LDS: coalesced loads, eight contiguous values per
+// thread, wave scans, coalesced stores.
+__device__ __forceinline__ void block_scan_inplace(int* v, int n) {
+  constexpr int CHUNK = 8192, PER = CHUNK / 1024;
+  __shared__ int buf[CHUNK];
+  __shared__ int wsum[16];
+  __shared__ int carry;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  if (t == 0) carry = 0;
+  __syncthreads();
+  for (int c0 = 0; c0 < n; c0 += CHUNK) {
+    const int m = min(CHUNK, n - c0);
+    int y[PER];
+#pragma unroll
+    for (int i = 0; i < PER; i++) y[i] = i * 1024 + t < m ? v[c0 + i * 1024 + t] : 0;
+#pragma unroll
+    for (int i = 0; i < PER; i++) buf[i * 1024 + t] = y[i];
+    __syncthreads();
+    int x[PER], s = 0;
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+      x[i] = buf[t * PER + i];
+      s += x[i];
+    }
+    int incl = s;
+    for (int o = 1; o < kWave; o <<= 1) {
+      int z = __shfl_up(incl, o);
+      if (lane >= o) incl += z;
+    }
+    if (lane == kWave - 1) wsum[wv] = incl;
+    __syncthreads();
+    int run = carry + incl - s;
+    for (int q = 0; q < wv; q++) run += wsum[q];
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+      buf[t * PER + i] = run;
+      run += x[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < PER; i++)
+      if (i * 1024 + t < m) v[c0 + i * 1024 + t] = buf[i * 1024 + t];
+    if (t == 1023) carry = run;
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_rx_scan(Args a) {
+  if (a.ctl->done) return;
+  block_scan_inplace(a.rx_hist, 128 * a.rx_nb);
+}
+
+// Stable scatter of one block: wave w owns items [w*256, (w+1)*256), 64 at a
+// time, and a lane's rank among the lanes with the same digit comes from
+// seven ballots.  Items are first placed in digit order in LDS, then written
+// out in runs of consecutive slots.
+__global__ __launch_bounds__(kRxThreads) void k_rx_scatter(Args a, const int32_t* kin, const int32_t* vin,
+                                                           int32_t* kout, int32_t* vout, int shift) {
+  if (a.ctl->done) return;
+  __shared__ int wcnt[kRxWaves][128];
+  __shared__ int wpos[kRxWaves][128];
+  __shared__ int dstart[128], gbase[128], dsum[128];
+  __shared__ int lkey[kRxItems], lval[kRxItems];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int d = threadIdx.x; d < kRxWaves * 128; d += kRxThreads) wcnt[d >> 7][d & 127] = 0;
+  __syncthreads();
+  constexpr int CH = kRxItems / kRxWaves / kWave;  // chunks of 64 per wave
+  const int base = blockIdx.x * kRxItems + w * (kRxItems / kRxWaves);
+  int key[CH], val[CH];
+#pragma unroll
+  for (int c = 0; c < CH; c++) {
+    const int idx = base + c * kWave + lane;
+    key[c] = idx < a.N ? kin[idx] : -1;
+    val[c] = idx < a.N ? vin[idx] : 0;
+  }
+#pragma unroll
+  for (int c = 0; c < CH; c++)
+    if (key[c] >= 0) atomicAdd(&wcnt[w][(key[c] >> shift) & 127], 1);
+  __syncthreads();
+  if (threadIdx.x < 128) {
+    // block-local start of each digit: inclusive scan of the digit totals
+    // within each of the two waves holding the 128 digits
+    const int d = threadIdx.x;
+    int tot = 0;
+    for (int q = 0; q < kRxWaves; q++) tot += wcnt[q][d];
+    int incl = tot;
+    for (int o = 1; o < kWave; o <<= 1) {
+      int y = __shfl_up(incl, o);
+      if (lane >= o) incl += y;
+    }
+    dsum[d] = incl;
+    dstart[d] = incl - tot;
+  }
+  __syncthreads();
+  if (threadIdx.x < 128) {
+    const int d = threadIdx.x;
+    const int st = dstart[d] + (d >= kWave ? dsum[kWave - 1] : 0);
+    int run = st;
+    for (int q = 0; q < kRxWaves; q++) {
+      wpos[q][d] = run;
+      run += wcnt[q][d];
+    }
+    dstart[d] = st;
+    gbase[d] = a.rx_hist[d * a.rx_nb + blockIdx.x];
+  }
+  __syncthreads();
+  const unsigned long long lt = (1ull << lane) - 1ull;
+#pragma unroll
+  for (int c = 0; c < CH; c++) {
+    const bool valid = key[c] >= 0;
+    const int d = (key[c] >> shift) & 127;
+    unsigned long long peers = __ballot(valid);
+#pragma unroll
+    for (int bit = 0; bit < 7; bit++) {
+      const unsigned long long m = __ballot((d >> bit) & 1);
+      peers &= ((d >> bit) & 1) ? m : ~m;
+    }
+    const int rank = __popcll(peers & lt);
+    const int lpos = valid ? wpos[w][d] + rank : 0;
+    if (valid) {
+      lkey[lpos] = key[c];
+      lval[lpos] = val[c];
+    }
+    // the digit group's last lane advances the wave's slot (LDS operations
+    // of one wave complete in order)
+    if (valid && rank == __popcll(peers) - 1) wpos[w][d] = lpos + 1;
+  }
+  __syncthreads();
+  const int nitem = min(kRxItems, a.N - blockIdx.x * kRxItems);
+  for (int j = threadIdx.x; j < nitem; j += kRxThreads) {
+    const int k = lkey[j];
+    const int d = (k >> shift) & 127;
+    const int g = gbase[d] + (j - dstart[d]);
+    kout[g] = k;
+    vout[g] = lval[j];
+  }
+}
+
+// first / one-past-last sorted slot of every non-empty partition
+__global__ void k_part_bounds(Args a) {
+  if (a.ctl->done) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.N) return;
+  const int* k = a.rx_keys;
+  const int p = k[i];
+  if (p >= a.K) return;
+  if (i == 0 || k[i - 1] != p) a.pstart[p] = i;
+  if (i == a.N - 1 || k[i + 1] != p) a.pend[p] = i + 1;
 }
 
 // Exclusive scan of the partition histogram, max count -> npad, dual step
@@ -266,41 +488,33 @@ __global__ void k_assign(Args a) {
 __global__ __launch_bounds__(1024) void k_scan(Args a) {
   Ctl* ctl = a.ctl;
   if (ctl->done) return;
-  __shared__ int s_part[1024];
-  __shared__ int s_max[1024];
+  __shared__ int s_max[16];
   const int K = a.K;
   const int t = threadIdx.x;
-  // each thread scans a contiguous chunk
-  int per = (K + 1023) / 1024;
-  int b = t * per, e = min(K, b + per);
-  int sum = 0, mx = 0;
-  for (int i = b; i < e; i++) { sum += a.cnt[i]; mx = max(mx, a.cnt[i]); }
-  s_part[t] = sum;
-  s_max[t] = mx;
-  __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {
-    int v = (t >= o) ? s_part[t - o] : 0;
-    int m = (t + o < 1024) ? s_max[t + o] : 0;
-    __syncthreads();
-    s_part[t] += v;
-    s_max[t] = max(s_max[t], m);
-    __syncthreads();
+  // cells per partition from the sorted bounds; off <- counts, then scanned
+  int mx = 0;
+  for (int i = t; i < K; i += 1024) {
+    const int ps = a.pstart[i];
+    const int c = ps >= 0 ? a.pend[i] - ps : 0;
+    a.off[i] = c;
+    a.pstart[i] = -1;
+    mx = max(mx, c);
   }
-  int run = s_part[t] - sum;
-  for (int i = b; i < e; i++) {
-    a.off[i] = run;
-    run += a.cnt[i];
-    a.fill[i] = 0;
-    a.cnt[i] = 0;
-  }
+  for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o));
+  if ((t & 63) == 0) s_max[t >> 6] = mx;
   // activity flags of the next iteration, filled by k_dual_classify
   int32_t* nf = a.act_flags + (size_t)((ctl->iter + 1) & 1) * K;
   for (int i = t; i < K; i += 1024) nf[i] = 0;
+  __syncthreads();
+  const int last = K > 0 ? a.off[K - 1] : 0;
+  __syncthreads();
+  block_scan_inplace(a.off, K);
   if (t == 0) {
-    int total = s_part[1023];
+    int m = 0;
+    for (int q = 0; q < 16; q++) m = max(m, s_max[q]);
+    const int total = K > 0 ? a.off[K - 1] + last : 0;
     a.off[K] = total;
     a.off[K + 1] = total;
-    int m = s_max[0];
     int np = 1;
     while (np < m) np <<= 1;
     ctl->max_cnt = m;
@@ -314,15 +528,6 @@ __global__ __launch_bounds__(1024) void k_scan(Args a) {
   }
 }
 
-__global__ void k_scatter(Args a) {
-  if (a.ctl->done) return;
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= a.N) return;
-  int p = a.part_of[c];
-  if (p >= a.K) return;
-  int slot = atomicAdd(&a.fill[p], 1);
-  a.grouped[a.off[p] + slot] = c;
-}
 
 // ---------------------------------------------------------------------------
 // the partition kernel: one 64-lane wavefront owns one partition
@@ -357,23 +562,6 @@ __device__ void lds_bitonic_desc(int n, int* key, int* val) {
     }
 }
 
-// Ascending sort of distinct keys (cell ids) -- restores ascending-id order
-// within a partition after the unordered scatter.
-__device__ void lds_sort_ids(int n, int* v) {
-  const int lane = threadIdx.x;
-  for (int k = 2; k <= n; k <<= 1)
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = lane; i < n; i += kWave) {
-        int ij = i ^ j;
-        if (ij > i) {
-          int a0 = v[i], a1 = v[ij];
-          bool up = (i & k) == 0;
-          if (up ? (a0 > a1) : (a0 < a1)) { v[i] = a1; v[ij] = a0; }
-        }
-      }
-      __syncthreads();
-    }
-}
 
 // mergeSortDescending (SortingKernels.hpp:143-219): bottom-up merges, a tie
 // takes the right run first.  key/val hold the input; bk/bv are buffers.
@@ -810,7 +998,7 @@ __device__ __forceinline__ bool stage_tile(const Args& a, const PartGeo& g, cons
   const int step_b = kWave / g.xh, step_a = kWave - step_b * g.xh;
   int lb = lane / g.xh, la = lane - lb * g.xh;
   bool range_err = false;
-  constexpr int U = 8;
+  constexpr int U = 16;
   for (int i0 = 0; i0 < npix; i0 += U * kWave) {
     int u[U], code[U], li[U];
     float dl[U];
@@ -862,47 +1050,50 @@ __device__ __forceinline__ int sort_key(const Args& a, const PartGeo& g, const T
 // grouped[off..] (ids) and rec_b[off..] (records).
 template <int R>
 __device__ __forceinline__ void order_partition_reg(const Args& a, const PartGeo& g, const Tile& T, int npad, int off,
-                                    int count, const int4* cellrec) {
+                                    int count, const int4* cellrec
+#ifdef LG_PHASE_TIMERS
+                                    , int* ph_
+#endif
+                                    ) {
   const int lane = threadIdx.x;
+#ifdef LG_PHASE_TIMERS
+  unsigned long long t0_ = wall_clock64();
+#endif
   int* ids = a.grouped + off;
-  int4* reca = reinterpret_cast<int4*>(a.rec_a) + off;
   int4* recb = reinterpret_cast<int4*>(a.rec_b) + off;
-  int* ida = a.ids_a + off;
+  // the grouping lists the cells in ascending id order; the sort carries the
+  // cell id as its payload (pads: -1)
   int key[R], val[R];
-  int na = 1;
-  while (na < count) na <<= 1;
 #pragma unroll
   for (int r = 0; r < R; r++) {
-    int i = r * kWave + lane;
-    key[r] = (i < count) ? ids[i] : INT_MAX;
-    val[r] = 0;
+    const int i = r * kWave + lane;
+    val[r] = i < count ? ids[i] : -1;
   }
-  reg_bitonic<R, false>(na, key, val);
+  int4 rc[R];
 #pragma unroll
-  for (int r = 0; r < R; r++) {
-    int i = r * kWave + lane;
-    if (i < count) {
-      int c = key[r];
-      int4 rc = cellrec[c];
-      reca[i] = rc;
-      ida[i] = c;
-      key[r] = sort_key(a, g, T, c, rc);
-      val[r] = i;
-    } else {
-      key[r] = INT_MIN;
-      val[r] = -1;
-    }
-  }
+  for (int r = 0; r < R; r++) rc[r] = val[r] >= 0 ? cellrec[val[r]] : make_int4(0, 0, 0, 0);
+#pragma unroll
+  for (int r = 0; r < R; r++) key[r] = val[r] >= 0 ? sort_key(a, g, T, val[r], rc[r]) : INT_MIN;
+#ifdef LG_PHASE_TIMERS
+  unsigned long long t1_ = wall_clock64();
+  ph_[5] = (int)(t1_ - t0_);
+#endif
   reg_bitonic<R, true>(npad, key, val);
-  __syncthreads();  // reca / ida written by other lanes
+#ifdef LG_PHASE_TIMERS
+  unsigned long long t2_ = wall_clock64();
+  ph_[6] = (int)(t2_ - t1_);
+#endif
 #pragma unroll
   for (int r = 0; r < R; r++) {
-    int i = r * kWave + lane;
+    const int i = r * kWave + lane;
     if (i < count) {
-      ids[i] = ida[val[r]];
-      recb[i] = reca[val[r]];
+      ids[i] = val[r];
+      recb[i] = cellrec[val[r]];
     }
   }
+#ifdef LG_PHASE_TIMERS
+  ph_[7] = (int)(wall_clock64() - t2_);
+#endif
 }
 
 // The same with the sorts in LDS (npad > 1024, or the merge / radix options).
@@ -910,22 +1101,12 @@ __device__ void order_partition_lds(const Args& a, const PartGeo& g, const Tile&
                                     int off, int count, const int4* cellrec, int* keys, int* slots, int* aux) {
   const int lane = threadIdx.x;
   int* ids = a.grouped + off;
-  int4* reca = reinterpret_cast<int4*>(a.rec_a) + off;
   int4* recb = reinterpret_cast<int4*>(a.rec_b) + off;
-  int* ida = a.ids_a + off;
-  int na = 1;
-  while (na < count) na <<= 1;
-  for (int i = lane; i < na; i += kWave) keys[i] = (i < count) ? ids[i] : INT_MAX;
-  __syncthreads();
-  lds_sort_ids(na, keys);
   for (int i = lane; i < npad; i += kWave) {
     if (i < count) {
-      int c = keys[i];
-      int4 rc = cellrec[c];
-      reca[i] = rc;
-      ida[i] = c;
-      keys[i] = sort_key(a, g, T, c, rc);
-      slots[i] = i;
+      int c = ids[i];
+      keys[i] = sort_key(a, g, T, c, cellrec[c]);
+      slots[i] = c;
     } else {
       keys[i] = INT_MIN;
       slots[i] = -1;
@@ -936,9 +1117,9 @@ __device__ void order_partition_lds(const Args& a, const PartGeo& g, const Tile&
   else if (a.sort_algo == 4) lds_stable_desc(npad, count, keys, slots, aux);
   else lds_bitonic_desc(npad, keys, slots);
   for (int i = lane; i < count; i += kWave) {
-    int s = slots[i];
-    ids[i] = ida[s];
-    recb[i] = reca[s];
+    int c = slots[i];
+    ids[i] = c;
+    recb[i] = cellrec[c];
   }
 }
 
@@ -973,25 +1154,23 @@ __device__ __forceinline__ float dual_step(int ui, float l, float rho, float ste
 template <int MODE, int LDS_SORT>
 __global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_hi) {
   Ctl* ctl = a.ctl;
-  int iter;
+  if (MODE == MODE_PRIMAL && ctl->done) return;
+  const int k = blockIdx.x;
   const bool adaptive_now = (MODE == MODE_PRIMAL) && a.use_adaptive && ctl->exceptional;
-  if (MODE == MODE_PRIMAL) {
-    if (ctl->done) return;
-    if (adaptive_now) {
-      if (npad_lo != 0) return;  // only the smallest class runs the dual-only pass
-    } else {
-      int np = ctl->npad;
-      if (np < npad_lo || np > npad_hi) return;
-    }
-    iter = ctl->iter;
-  } else {
-    int np = ctl->npad;
-    if (np < npad_lo || np > npad_hi) return;
-    iter = ctl->conv_iter;
-  }
+  const int iter = MODE == MODE_PRIMAL ? ctl->iter : ctl->conv_iter;
+  const int off = a.off[k];
+  const int count = adaptive_now ? 0 : a.off[k + 1] - off;
+  // Sort size of this partition.  The reference sorts every partition on
+  // npad = pow2(largest partition) slots, padding with INT_MIN; a bitonic
+  // network whose real keys fill the first c slots sorts them exactly as the
+  // network on pow2(c) slots does (the later merge stages only compare
+  // ordered runs with padding, which never swaps), and the merge / radix
+  // options are stable, so each partition sorts pow2(count) slots.
+  int npad = 1;
+  while (npad < count) npad <<= 1;
+  if (npad < npad_lo || npad > npad_hi) return;
   PHASE_START();
   SUB_DECL();
-  const int k = blockIdx.x;
   const int lane = threadIdx.x;
   PartGeo g;
   const int tf = triple_fold(a, iter);
@@ -1003,8 +1182,6 @@ __global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_
   const int npix = g.xh * g.yh;
   const int step_b = kWave / g.xh, step_a = kWave - step_b * g.xh;
 
-  const int off = a.off[k];
-  const int count = adaptive_now ? 0 : a.off[k + 1] - off;
   const float rho = ctl->rho;
   const float step = ctl->step;
   const float neg = -1.0f / rho;
@@ -1105,13 +1282,16 @@ __global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_
 
   if (count > 0) {
     int4* recs = reinterpret_cast<int4*>(a.rec_b) + off;
-    const int npad = ctl->npad;
     const bool do_order = (MODE == MODE_PRIMAL) || a.sorted[k] == 0;
     if (do_order) {
       // an unsorted partition at the final refine moved nothing in the last
       // iteration, so its usage still is the one the order is defined on
       if (LDS_SORT) order_partition_lds(a, g, T, npad, npad_hi, off, count, cellrec, keys, slots, aux);
-      else order_partition_reg<kNpadSmall / kWave>(a, g, T, npad, off, count, cellrec);
+      else order_partition_reg<kNpadSmall / kWave>(a, g, T, npad, off, count, cellrec
+#ifdef LG_PHASE_TIMERS
+                                                     , ph_
+#endif
+                                                     );
       if (MODE == MODE_PRIMAL && lane == 0) a.sorted[k] = 1;
     } else {
       for (int i = lane; i < count; i += kWave) recs[i] = cellrec[ids[i]];
