@@ -64,3 +64,37 @@ def test_null_arguments_are_rejected_without_a_device(library):
     D = density.lib()
     assert D.dens_create(None, None) != 0
     assert D.poisson_solve(None, None, None, None, None, 1, None) != 0
+
+
+def test_cpp_headers_compile_and_link(library, tmp_path):
+    """A C++ caller of the public headers builds against the library and runs
+    its host-only entry points (parameters, density bin setup)."""
+    src = tmp_path / "caller.cpp"
+    src.write_text(r'''
+#include <cstdio>
+#include <vector>
+#include "skyline/Legalizer.hpp"
+#include "skyline/DensityGradient.hpp"
+int main() {
+  LgParams p;
+  lg_default_params(&p);
+  std::vector<float> dx(100, 1.5f), dy(100, 2.0f);
+  DensDesign d{};
+  d.die_ux = d.die_uy = 64.f;
+  d.target_density = 1.f;
+  d.num_movable = 100;
+  d.cell_dx = dx.data();
+  d.cell_dy = dy.data();
+  int32_t shape[2];
+  if (dens_prepare(&d, shape, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                   nullptr) != DENS_OK) return 2;
+  std::printf("%d %d %d\n", p.x_hint, shape[0], shape[1]);
+  return 0;
+}
+''')
+    exe = tmp_path / "caller"
+    libdir = os.path.dirname(library)
+    subprocess.run(["g++", "-std=c++17", str(src), "-I", os.path.join(ROOT, "include"), "-L", libdir,
+                    "-lskyline_amd", f"-Wl,-rpath,{libdir}", "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], stdout=subprocess.PIPE, text=True, check=True).stdout.split()
+    assert out[0] == "250" and int(out[1]) >= 4 and out[1] == out[2]
