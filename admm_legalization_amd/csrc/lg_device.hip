@@ -191,6 +191,7 @@ int lg_create(const LgGridDesc* g, const LgCellsSoA* c, const LgParams* p, LgHan
   if (p->x_hint < 3 || p->y_hint < 3 || p->iter_update_partition <= 0 || p->iter_update_dual_step <= 0)
     return fail(LG_ERR_ARG, "bad partition / schedule parameters");
   if (g->x_size > 65535 || g->y_size > 65535) return fail(LG_ERR_ARG, "grid larger than 65535 sites/rows");
+  if ((long long)g->x_size * g->y_size >= (1ll << 30)) return fail(LG_ERR_ARG, "grid of 2^30 pixels or more");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(LG_ERR_DEVICE, "no HIP device");
   int st = upload_directions();
@@ -399,7 +400,7 @@ int lg_solve(LgHandle* h, void* stream) {
     hipLaunchKernelGGL(k_pack_rec, dim3(nb), dim3(256), 0, s, a);
   }
   hipLaunchKernelGGL(k_fill_dual, dim3(2048), dim3(256), 0, s, a);
-  const dim3 pix_grid((a.x_size + 255) / 256, a.y_size);
+  const dim3 pix_grid((a.x_size + 1023) / 1024, a.y_size);  // four pixels per thread
   HIP_TRY(hipMemsetAsync(a.act_flags, 0, 2 * (size_t)a.K * sizeof(int32_t), s));
 #ifdef LG_PHASE_TIMERS
   if (a.rx_passes <= 2 && N > 0) {  // per-iteration block-time maxima / sums (timed builds)

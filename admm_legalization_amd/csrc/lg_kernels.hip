@@ -157,8 +157,10 @@ struct Args {
   Ctl* ctl;
 };
 
-// linear index of pixel (x, y) in HBM
-__device__ __forceinline__ size_t pidx(const Args& a, int x, int y) { return (size_t)y * a.x_size + x; }
+// linear index of pixel (x, y) in HBM; grids hold fewer than 2^30 pixels
+__device__ __forceinline__ unsigned pidx(const Args& a, int x, int y) {
+  return (unsigned)y * (unsigned)a.x_size + (unsigned)x;
+}
 
 __device__ __forceinline__ bool correct_row(int h, int gnd, int vdd) {  // DeviceUtil.hpp:11-22
   return !((h & 1) == 0 && gnd != vdd);
@@ -1007,12 +1009,16 @@ __device__ __forceinline__ bool stage_tile(const Args& a, const PartGeo& g, cons
     for (int q = 0; q < U; q++) {
       int xx = g.gx(la), yy = g.gy(lb);
       bool ok = (i0 + q * kWave + lane < npix) && xx >= 0 && yy >= 0;
-      size_t p = ok ? pidx(a, xx, yy) : 0;
+      // unconditional loads at a clamped index, then select
+      const unsigned p = ok ? pidx(a, xx, yy) : 0u;
       li[q] = T.at(la, lb);
       in[q] = ok;
-      u[q] = ok ? a.usage[p] : 0;
-      code[q] = ok ? (int)(uint16_t)a.pcode[p] : 0;  // code 0: not placeable
-      dl[q] = ok ? a.dual[p] : 0.0f;
+      const int uu = a.usage[p];
+      const int cc = (int)(uint16_t)a.pcode[p];
+      const float dd = a.dual[p];
+      u[q] = ok ? uu : 0;
+      code[q] = ok ? cc : 0;  // code 0: not placeable
+      dl[q] = ok ? dd : 0.0f;
       la += step_a; lb += step_b;
       if (la >= g.xh) { la -= g.xh; lb++; }
     }
@@ -1155,6 +1161,7 @@ template <int MODE, int LDS_SORT>
 __global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_hi) {
   Ctl* ctl = a.ctl;
   if (MODE == MODE_PRIMAL && ctl->done) return;
+  if (ctl->npad < npad_lo) return;  // no partition of this size class
   const int k = blockIdx.x;
   const bool adaptive_now = (MODE == MODE_PRIMAL) && a.use_adaptive && ctl->exceptional;
   const int iter = MODE == MODE_PRIMAL ? ctl->iter : ctl->conv_iter;
@@ -1217,7 +1224,7 @@ __global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_
         la += step_a; lb += step_b;
         if (la >= g.xh) { la -= g.xh; lb++; }
         if (xx < 0 || yy < 0) continue;
-        size_t p = pidx(a, xx, yy);
+        const unsigned p = pidx(a, xx, yy);
         int ui = a.usage[p];
         a.dual[p] = dual_step(ui, a.dual[p], rho, step, neg);
         ovf += (unsigned long long)max(ui - 1, 0);
@@ -1406,7 +1413,7 @@ __global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_
       la += step_a; lb += step_b;
       if (la >= g.xh) { la -= g.xh; lb++; }
       if (xx < 0 || yy < 0) continue;
-      size_t p = pidx(a, xx, yy);
+      const unsigned p = pidx(a, xx, yy);
       int ui = (int)(T.uc[li] & 0xffffu);
       a.usage[p] = ui;
       if (MODE == MODE_PRIMAL) {
@@ -1428,33 +1435,66 @@ __global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_
 // (dual_update_functor_with_step_size, AdmmKernels.hpp:26-42) of every pixel
 // whose partition took no primal step this iteration, and the activity flags
 // (any pixel with usage > 1) of each partition under the next iteration's
-// fold offset.  One thread per pixel, rows in blockIdx.y.  With init != 0
-// only the flags of the current iteration are computed.
+// fold offset.  Four consecutive pixels of a row per thread, rows in
+// blockIdx.y.  A pixel with usage 1 has a zero dual increment, so its dual
+// is read only for other usages.  With init != 0 only the flags of the
+// current iteration are computed.
 __global__ __launch_bounds__(256) void k_dual_classify(Args a, int init) {
   const Ctl* ctl = a.ctl;
   if (ctl->done) return;
   const int y = blockIdx.y;
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= a.x_size) return;
+  const int x0 = 4 * (blockIdx.x * blockDim.x + threadIdx.x);
+  if (x0 >= a.x_size) return;
   const int iter = ctl->iter;
   const int nxt = init ? iter : iter + 1;
-  const size_t p = pidx(a, x, y);
-  const int u = a.usage[p];
+  const unsigned p0 = pidx(a, x0, y);
+  const bool full = x0 + 4 <= a.x_size && (a.x_size & 3) == 0;
+  int u[4];
+  if (full) {
+    const int4 v = *reinterpret_cast<const int4*>(a.usage + p0);
+    u[0] = v.x; u[1] = v.y; u[2] = v.z; u[3] = v.w;
+  } else {
+#pragma unroll
+    for (int q = 0; q < 4; q++) u[q] = x0 + q < a.x_size ? a.usage[p0 + q] : 1;
+  }
   if (!init) {
     const bool adaptive_now = a.use_adaptive && ctl->exceptional;
     if (iter >= a.iter_perturbation_on && ctl->wrong_row == 0 && !adaptive_now) {
       const int tf = triple_fold(a, iter);
-      const int kc = a.colmap[tf * a.x_size + x] * a.nrow + a.rowmap[tf * a.y_size + y];
-      if (a.act_flags[(size_t)(iter & 1) * a.K + kc] == 0) {
-        const float rho = ctl->rho;
-        a.dual[p] = dual_step(u, a.dual[p], rho, ctl->step, -1.0f / rho);
+      const int rk = a.rowmap[tf * a.y_size + y];
+      const int* cm = a.colmap + tf * a.x_size;
+      const int* fl = a.act_flags + (size_t)(iter & 1) * a.K;
+      bool upd[4];
+      bool any = false;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        upd[q] = x0 + q < a.x_size && u[q] != 1 && fl[cm[x0 + q] * a.nrow + rk] == 0;
+        any |= upd[q];
+      }
+      if (any) {
+        const float rho = ctl->rho, step = ctl->step, neg = -1.0f / rho;
+        if (full) {
+          float4 d = *reinterpret_cast<const float4*>(a.dual + p0);
+          if (upd[0]) d.x = dual_step(u[0], d.x, rho, step, neg);
+          if (upd[1]) d.y = dual_step(u[1], d.y, rho, step, neg);
+          if (upd[2]) d.z = dual_step(u[2], d.z, rho, step, neg);
+          if (upd[3]) d.w = dual_step(u[3], d.w, rho, step, neg);
+          *reinterpret_cast<float4*>(a.dual + p0) = d;
+        } else {
+#pragma unroll
+          for (int q = 0; q < 4; q++)
+            if (upd[q]) a.dual[p0 + q] = dual_step(u[q], a.dual[p0 + q], rho, step, neg);
+        }
       }
     }
   }
-  if (u > 1) {
+  if (u[0] > 1 || u[1] > 1 || u[2] > 1 || u[3] > 1) {
     const int tf = triple_fold(a, nxt);
-    const int kn = a.colmap[tf * a.x_size + x] * a.nrow + a.rowmap[tf * a.y_size + y];
-    a.act_flags[(size_t)(nxt & 1) * a.K + kn] = 1;
+    const int rk = a.rowmap[tf * a.y_size + y];
+    int32_t* fn = a.act_flags + (size_t)(nxt & 1) * a.K;
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      if (u[q] > 1) fn[a.colmap[tf * a.x_size + x0 + q] * a.nrow + rk] = 1;
   }
 }
 
@@ -1553,7 +1593,7 @@ __device__ void proj_pos(const Args& a, int d, int w, int h, int cx, int cy, int
   for (int x = nx; x < nx + w && !inval; x++)
     for (int y = ny; y < ny + h; y++) {
       if (x >= a.x_size || y >= a.y_size || x < 0 || y < 0) { inval = true; break; }
-      size_t p = pidx(a, x, y);
+      const unsigned p = pidx(a, x, y);
       if (a.pcode[p] == 0) { inval = true; rid = a.pregion[p]; break; }
     }
   if (rid != -1) {
@@ -1624,7 +1664,7 @@ __global__ __launch_bounds__(1024) void k_adaptive(Args a) {
           pc = 0.0f;
           for (int x = nx; x < nx + w && pc != kInf; x++)
             for (int y = ny; y < ny + h; y++) {
-              size_t p = pidx(a, x, y);
+              const unsigned p = pidx(a, x, y);
               if ((int)a.pcode[p] != code) { pc = kInf; break; }
               float u = float(a.usage[p]);
               float l = a.dual[p];
