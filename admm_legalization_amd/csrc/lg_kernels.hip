@@ -743,6 +743,32 @@ __device__ __forceinline__ int ovf_lds(const PartGeo& g, const Tile& T, int cx, 
   return s;
 }
 
+// ovf_lds for a footprint that does not wrap, with row stride kRowStride and
+// h <= 4: columns in groups of four, all rows at once, so sixteen LDS reads
+// are in flight (the per-lane footprints of a wave differ in size)
+__device__ __forceinline__ int ovf_lds_flat(const Tile& T, int lx0, int ly0, int w, int h) {
+  const uint32_t* p = T.uc + ly0 * kRowStride + lx0;
+  int s = 0;
+  for (int x = 0; x < w; x += 4) {
+    uint32_t v[4][4];
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+      for (int y = 0; y < 4; y++) v[q][y] = p[y * kRowStride + x + q];
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+      for (int y = 0; y < 4; y++)
+        if (x + q < w && y < h) s += max((int)(v[q][y] & 0xffffu) - 1, 0);
+  }
+  return s;
+}
+
+__device__ __forceinline__ int ovf_tile(const PartGeo& g, const Tile& T, int cx, int cy, int w, int h) {
+  if (g.ncol > 1 && g.nrow > 1 && T.rs == kRowStride && h <= 4) return ovf_lds_flat(T, g.lx(cx), g.ly(cy), w, h);
+  return ovf_lds(g, T, cx, cy, w, h);
+}
+
 // the same from the pixels in HBM, for a cell inside the grid
 __device__ __forceinline__ int ovf_hbm(const Args& a, int cx, int cy, int w, int h) {
   int s = 0;
@@ -1046,7 +1072,7 @@ __device__ __forceinline__ int sort_key(const Args& a, const PartGeo& g, const T
   int cr = 0, co = 10, ca = 1;
   if (a.sort_policy == 0) { cr = 1; co = 0; ca = 0; }
   int w = rec_w(rc.y), h = rec_h(rc.y);
-  int ovf = co ? ovf_lds(g, T, rec_x(rc.x), rec_y(rc.x), w, h) : 0;
+  int ovf = co ? ovf_tile(g, T, rec_x(rc.x), rec_y(rc.x), w, h) : 0;
   return cr * (cr ? a.rand_prio[c] : 0) + co * ovf + ca * w * h;
 }
 
@@ -1070,16 +1096,29 @@ __device__ __forceinline__ void order_partition_reg(const Args& a, const PartGeo
   // the grouping lists the cells in ascending id order; the sort carries the
   // cell id as its payload (pads: -1)
   int key[R], val[R];
+  // registers beyond the partition's count hold padding only (uniform test);
+  // loads use a clamped index and are selected afterwards
 #pragma unroll
   for (int r = 0; r < R; r++) {
-    const int i = r * kWave + lane;
-    val[r] = i < count ? ids[i] : -1;
+    val[r] = -1;
+    if (r * kWave < count) {
+      const int i = r * kWave + lane;
+      const int v = ids[min(i, count - 1)];
+      val[r] = i < count ? v : -1;
+    }
   }
   int4 rc[R];
 #pragma unroll
-  for (int r = 0; r < R; r++) rc[r] = val[r] >= 0 ? cellrec[val[r]] : make_int4(0, 0, 0, 0);
+  for (int r = 0; r < R; r++)
+    if (r * kWave < count) rc[r] = cellrec[val[r] >= 0 ? val[r] : ids[0]];
 #pragma unroll
-  for (int r = 0; r < R; r++) key[r] = val[r] >= 0 ? sort_key(a, g, T, val[r], rc[r]) : INT_MIN;
+  for (int r = 0; r < R; r++) {
+    key[r] = INT_MIN;
+    if (r * kWave < count) {
+      const int kk = sort_key(a, g, T, val[r] >= 0 ? val[r] : 0, rc[r]);
+      key[r] = val[r] >= 0 ? kk : INT_MIN;
+    }
+  }
 #ifdef LG_PHASE_TIMERS
   unsigned long long t1_ = wall_clock64();
   ph_[5] = (int)(t1_ - t0_);
@@ -1091,10 +1130,13 @@ __device__ __forceinline__ void order_partition_reg(const Args& a, const PartGeo
 #endif
 #pragma unroll
   for (int r = 0; r < R; r++) {
-    const int i = r * kWave + lane;
-    if (i < count) {
-      ids[i] = val[r];
-      recb[i] = cellrec[val[r]];
+    if (r * kWave < count) {
+      const int i = r * kWave + lane;
+      const int4 rv = cellrec[val[r] >= 0 ? val[r] : ids[0]];
+      if (i < count) {
+        ids[i] = val[r];
+        recb[i] = rv;
+      }
     }
   }
 #ifdef LG_PHASE_TIMERS
@@ -1327,7 +1369,7 @@ __global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_
         if (MODE == MODE_PRIMAL) {
           if (pert) return true;
           bool ok_row = correct_row(my_h, rec_gnd(r.y), T.vdd[g.ly(my_y)]);
-          return !ok_row || ovf_lds(g, T, my_x, my_y, my_w, my_h) != 0;
+          return !ok_row || ovf_tile(g, T, my_x, my_y, my_w, my_h) != 0;
         }
         return my_h > 1;
       };
