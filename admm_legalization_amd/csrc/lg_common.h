@@ -6,7 +6,8 @@
 namespace lgdev {
 
 // Device-resident iteration control.  Written only by single-thread kernels
-// (k_scan, k_finish, setup kernels); every other kernel only reads it.
+// (k_scan, k_finish, setup kernels) and by atomic accumulations of the
+// partition kernel; every other kernel only reads it.
 struct Ctl {
   int32_t iter;          // index of the iteration being executed
   int32_t done;          // 1: converged or max_admm_iter reached -> kernels exit
@@ -20,6 +21,7 @@ struct Ctl {
   int32_t error;         // nonzero: a device-side limit was exceeded (see kErr*)
   int32_t adaptive_calls;
   int32_t wrong_row;     // cells on a row of the wrong rail parity at solve start
+  int32_t n_work[2];     // partitions listed in Args::work per sort-size class
   int32_t ovf_window_n;  // entries in the stagnation window
   int32_t ovf_window[64];
   float rho;

@@ -325,6 +325,7 @@ int lg_create(const LgGridDesc* g, const LgCellsSoA* c, const LgParams* p, LgHan
     HIP_TRY(hipMemset(a.pstart, 0xff, std::max(a.K, 1) * sizeof(int32_t)));
   }
   if ((st = halloc(h, &a.sorted, a.K))) return bail(st);
+  if ((st = halloc(h, &a.work, 2 * (size_t)a.K))) return bail(st);
   if ((st = halloc(h, &a.act_flags, 2 * (size_t)a.K))) return bail(st);
   {
     // partition column / row of every x / y for the three fold offsets
@@ -408,7 +409,7 @@ int lg_solve(LgHandle* h, void* stream) {
     HIP_TRY(hipMemsetAsync(a.rx_v2, 0, N * sizeof(int32_t), s));
   }
 #endif
-  hipLaunchKernelGGL(k_dual_classify, pix_grid, dim3(256), 0, s, a, 1);
+  hipLaunchKernelGGL(k_classify_init, pix_grid, dim3(256), 0, s, a);
   HIP_TRY(hipGetLastError());
   int st;
   if (N > 0 && h->p.max_admm_iter > 0) {
@@ -437,7 +438,7 @@ int lg_solve(LgHandle* h, void* stream) {
           hipLaunchKernelGGL(k_adaptive, dim3(1), dim3(1024), 0, s, a);
         }
         if ((st = launch_partition<MODE_PRIMAL>(h, s))) return st;
-        hipLaunchKernelGGL(k_dual_classify, pix_grid, dim3(256), 0, s, a, 0);
+        hipLaunchKernelGGL(k_dual_pass, pix_grid, dim3(256), 0, s, a);
         hipLaunchKernelGGL(k_finish, dim3(1), dim3(1), 0, s, a);
       }
       HIP_TRY(hipGetLastError());

@@ -11,7 +11,7 @@
 //
 // One iteration = k_assign -> radix grouping (k_rx_*, k_part_bounds) ->
 // k_scan -> k_partition<PRIMAL> (sort + sequential cell moves + fused dual
-// update / overflow) -> k_dual_classify -> k_finish.
+// update / overflow / next activity flags) -> k_dual_pass -> k_finish.
 // Every kernel reads the iteration index and the done flag from Ctl in
 // device memory, so batches of iterations are launched without host syncs.
 //
@@ -141,6 +141,8 @@ struct Args {
   int32_t *pstart, *pend;
   int rx_nb, rx_passes;
   int32_t* sorted;     // [K]  1 if the partition's order was sorted in the last primal step
+  int32_t* work;       // [2K] partitions the primal step processes: class 0 (at most
+                       //      kNpadSmall cells) from 0, class 1 from K; counts in Ctl::n_work
   int32_t* act_flags;  // [2][K] partition has a pixel with usage > 1 (by iteration parity)
   const int32_t* colmap;  // [3][x_size] partition column of x for each fold offset
   const int32_t* rowmap;  // [3][y_size] partition row of y for each fold offset
@@ -334,11 +336,15 @@ __global__ __launch_bounds__(kRxThreads) void k_rx_hist(Args a, const int32_t* k
 }
 
 // Exclusive scan of n ints in place by one workgroup of 1024, in chunks of
-// 8192 staged through LDS: coalesced loads, eight contiguous values per
-// thread, wave scans, coalesced stores.
+// 32768: all loads of a chunk are issued before the first use (one memory
+// round trip per chunk), staged through LDS with one pad word per 32 so that
+// both the coalesced and the thread-contiguous accesses are conflict-free,
+// 32 contiguous values per thread, wave scans, coalesced stores.
+__device__ __forceinline__ int scan_pad(int j) { return j + (j >> 5); }
+
 __device__ __forceinline__ void block_scan_inplace(int* v, int n) {
-  constexpr int CHUNK = 8192, PER = CHUNK / 1024;
-  __shared__ int buf[CHUNK];
+  constexpr int CHUNK = 32768, PER = CHUNK / 1024;
+  __shared__ int buf[CHUNK + CHUNK / 32];
   __shared__ int wsum[16];
   __shared__ int carry;
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
@@ -350,13 +356,13 @@ __device__ __forceinline__ void block_scan_inplace(int* v, int n) {
 #pragma unroll
     for (int i = 0; i < PER; i++) y[i] = i * 1024 + t < m ? v[c0 + i * 1024 + t] : 0;
 #pragma unroll
-    for (int i = 0; i < PER; i++) buf[i * 1024 + t] = y[i];
+    for (int i = 0; i < PER; i++) buf[scan_pad(i * 1024 + t)] = y[i];
     __syncthreads();
-    int x[PER], s = 0;
+    int s = 0;
 #pragma unroll
     for (int i = 0; i < PER; i++) {
-      x[i] = buf[t * PER + i];
-      s += x[i];
+      y[i] = buf[t * (PER + 1) + i];
+      s += y[i];
     }
     int incl = s;
     for (int o = 1; o < kWave; o <<= 1) {
@@ -369,13 +375,13 @@ __device__ __forceinline__ void block_scan_inplace(int* v, int n) {
     for (int q = 0; q < wv; q++) run += wsum[q];
 #pragma unroll
     for (int i = 0; i < PER; i++) {
-      buf[t * PER + i] = run;
-      run += x[i];
+      buf[t * (PER + 1) + i] = run;
+      run += y[i];
     }
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < PER; i++)
-      if (i * 1024 + t < m) v[c0 + i * 1024 + t] = buf[i * 1024 + t];
+      if (i * 1024 + t < m) v[c0 + i * 1024 + t] = buf[scan_pad(i * 1024 + t)];
     if (t == 1023) carry = run;
     __syncthreads();
   }
@@ -485,27 +491,71 @@ __global__ void k_part_bounds(Args a) {
 }
 
 // Exclusive scan of the partition histogram, max count -> npad, dual step
-// schedule (updateDualStepSize, CudaAdmmLegalizer.cu:293-294, 687-691).
-// One workgroup of 1024 threads.
+// schedule (updateDualStepSize, CudaAdmmLegalizer.cu:293-294, 687-691), and
+// the work lists of the primal step: every partition except those whose
+// activity flag says the step is a no-op, split by sort-size class.  A
+// listed partition's workgroup then starts in the first dispatch round
+// instead of behind thousands of workgroups that only find their partition
+// idle.  One workgroup of 1024 threads.
 __global__ __launch_bounds__(1024) void k_scan(Args a) {
   Ctl* ctl = a.ctl;
   if (ctl->done) return;
   __shared__ int s_max[16];
+  __shared__ int s_n[2];
   const int K = a.K;
-  const int t = threadIdx.x;
-  // cells per partition from the sorted bounds; off <- counts, then scanned
+  const int t = threadIdx.x, lane = t & 63;
+  const int iter = ctl->iter;
+  const bool adaptive_now = a.use_adaptive && ctl->exceptional;
+  // the same test as the partition kernel's early exit
+  const bool skip_ok = iter >= a.iter_perturbation_on && ctl->wrong_row == 0 && !adaptive_now;
+  const int32_t* cf = a.act_flags + (size_t)(iter & 1) * K;
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  if (t < 2) s_n[t] = 0;
+  __syncthreads();
+  // cells per partition from the sorted bounds (off <- counts, scanned
+  // below); eight partitions per thread with every load issued first
   int mx = 0;
-  for (int i = t; i < K; i += 1024) {
-    const int ps = a.pstart[i];
-    const int c = ps >= 0 ? a.pend[i] - ps : 0;
-    a.off[i] = c;
-    a.pstart[i] = -1;
-    mx = max(mx, c);
+  for (int s0 = 0; s0 < K; s0 += 8 * 1024) {
+    int ps[8], pe[8], fl[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const int i = s0 + j * 1024 + t;
+      ps[j] = i < K ? a.pstart[i] : -1;
+      pe[j] = i < K ? a.pend[i] : 0;
+      fl[j] = (i < K && skip_ok) ? cf[i] : 1;
+    }
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const int i = s0 + j * 1024 + t;
+      const bool in = i < K;
+      const int c = ps[j] >= 0 ? pe[j] - ps[j] : 0;
+      if (in) {
+        a.off[i] = c;
+        a.pstart[i] = -1;
+      }
+      mx = max(mx, c);
+      const bool run = in && fl[j] != 0;
+      if (in && !run) a.sorted[i] = 0;
+      const int cls = (!adaptive_now && c > kNpadSmall) ? 1 : 0;
+      // list order is free: partitions of one step are independent
+      const unsigned long long m0 = __ballot(run && cls == 0), m1 = __ballot(run && cls == 1);
+      int b0 = 0, b1 = 0;
+      if (lane == 0) {
+        if (m0) b0 = atomicAdd(&s_n[0], __popcll(m0));
+        if (m1) b1 = atomicAdd(&s_n[1], __popcll(m1));
+      }
+      b0 = __shfl(b0, 0);
+      b1 = __shfl(b1, 0);
+      if (run && cls == 0) a.work[b0 + __popcll(m0 & lt)] = i;
+      if (run && cls == 1) a.work[K + b1 + __popcll(m1 & lt)] = i;
+    }
   }
+  __syncthreads();
+  if (t < 2) ctl->n_work[t] = s_n[t];
   for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o));
   if ((t & 63) == 0) s_max[t >> 6] = mx;
-  // activity flags of the next iteration, filled by k_dual_classify
-  int32_t* nf = a.act_flags + (size_t)((ctl->iter + 1) & 1) * K;
+  // activity flags of the next iteration, set by the primal step
+  int32_t* nf = a.act_flags + (size_t)((iter + 1) & 1) * K;
   for (int i = t; i < K; i += 1024) nf[i] = 0;
   __syncthreads();
   const int last = K > 0 ? a.off[K - 1] : 0;
@@ -1184,6 +1234,14 @@ __device__ __forceinline__ float dual_step(int ui, float l, float rho, float ste
   return fmax_ref(l + m, 0.0f);
 }
 
+// activity flag, under the next iteration's fold offset, of the partition
+// holding pixel (x, y); set for every pixel with usage > 1 that a primal step
+// writes (only partitions that take the step can hold such pixels)
+__device__ __forceinline__ void flag_next(const Args& a, int iter, int x, int y) {
+  const int tn = triple_fold(a, iter + 1);
+  a.act_flags[(size_t)((iter + 1) & 1) * a.K + a.colmap[tn * a.x_size + x] * a.nrow + a.rowmap[tn * a.y_size + y]] = 1;
+}
+
 // doPrimalUpdateKernel (AdmmKernels.hpp:637-897) / doFinalRefineKernel
 // (AdmmKernels.hpp:1176-1416) for one partition, plus -- in primal mode -- the
 // dual update and the overflow count of the partition's pixels.
@@ -1203,8 +1261,12 @@ template <int MODE, int LDS_SORT>
 __global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_hi) {
   Ctl* ctl = a.ctl;
   if (MODE == MODE_PRIMAL && ctl->done) return;
-  if (ctl->npad < npad_lo) return;  // no partition of this size class
-  const int k = blockIdx.x;
+  // the primal step takes its partitions from the work list of its class;
+  // the final refine visits every partition
+  const int cls = npad_lo > 0 ? 1 : 0;
+  if (MODE == MODE_PRIMAL && (int)blockIdx.x >= ctl->n_work[cls]) return;
+  if (MODE != MODE_PRIMAL && ctl->npad < npad_lo) return;  // no partition of this size class
+  const int k = MODE == MODE_PRIMAL ? a.work[cls * a.K + blockIdx.x] : (int)blockIdx.x;
   const bool adaptive_now = (MODE == MODE_PRIMAL) && a.use_adaptive && ctl->exceptional;
   const int iter = MODE == MODE_PRIMAL ? ctl->iter : ctl->conv_iter;
   const int off = a.off[k];
@@ -1245,7 +1307,7 @@ __global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_
   const bool fast_ok = MODE == MODE_PRIMAL && !pert;
   const bool cell_test = fast_ok && ctl->wrong_row > 0;
   if (fast_ok && !cell_test && !adaptive_now && a.act_flags[(size_t)(iter & 1) * a.K + k] == 0) {
-    // no pixel above usage one: no cell moves, and k_dual_classify updates
+    // no pixel above usage one: no cell moves, and k_dual_pass updates
     // this partition's prices
     if (lane == 0) a.sorted[k] = 0;
     return;
@@ -1270,6 +1332,7 @@ __global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_
         int ui = a.usage[p];
         a.dual[p] = dual_step(ui, a.dual[p], rho, step, neg);
         ovf += (unsigned long long)max(ui - 1, 0);
+        if (ui > 1) flag_next(a, iter, xx, yy);
       }
       for (int o = 32; o > 0; o >>= 1) ovf += __shfl_down(ovf, o);
       if (lane == 0) {
@@ -1448,6 +1511,7 @@ __global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_
   // -- write back pixels; primal mode fuses the dual update and the overflow
   unsigned long long ovf = 0;
   {
+    // the primal step also sets the next iteration's activity flags
     int lb = lane / g.xh, la = lane - lb * g.xh;
     for (int i = lane; i < npix; i += kWave) {
       int xx = g.gx(la), yy = g.gy(lb);
@@ -1461,6 +1525,7 @@ __global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_
       if (MODE == MODE_PRIMAL) {
         a.dual[p] = dual_step(ui, T.lam[li], rho, step, neg);
         ovf += (unsigned long long)max(ui - 1, 0);
+        if (ui > 1) flag_next(a, iter, xx, yy);
       }
     }
   }
@@ -1473,22 +1538,21 @@ __global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_
   PHASE_END();
 }
 
-// Grid-wide pass after the primal step: the dual update
-// (dual_update_functor_with_step_size, AdmmKernels.hpp:26-42) of every pixel
-// whose partition took no primal step this iteration, and the activity flags
-// (any pixel with usage > 1) of each partition under the next iteration's
-// fold offset.  Four consecutive pixels of a row per thread, rows in
-// blockIdx.y.  A pixel with usage 1 has a zero dual increment, so its dual
-// is read only for other usages.  With init != 0 only the flags of the
-// current iteration are computed.
-__global__ __launch_bounds__(256) void k_dual_classify(Args a, int init) {
+// Dual update, after the primal step, of the pixels of partitions that took
+// no step (dual_update_functor_with_step_size, AdmmKernels.hpp:26-42).  Runs
+// only while the activity flags are in use.  A pixel of usage 1 has a zero
+// increment, so its dual is read only for other usages.  Four consecutive
+// pixels of a row per thread, rows in blockIdx.y.  The activity flags of the
+// next iteration are set by the primal step itself.
+__global__ __launch_bounds__(256) void k_dual_pass(Args a) {
   const Ctl* ctl = a.ctl;
   if (ctl->done) return;
+  const int iter = ctl->iter;
+  const bool adaptive_now = a.use_adaptive && ctl->exceptional;
+  if (iter < a.iter_perturbation_on || ctl->wrong_row != 0 || adaptive_now) return;
   const int y = blockIdx.y;
   const int x0 = 4 * (blockIdx.x * blockDim.x + threadIdx.x);
   if (x0 >= a.x_size) return;
-  const int iter = ctl->iter;
-  const int nxt = init ? iter : iter + 1;
   const unsigned p0 = pidx(a, x0, y);
   const bool full = x0 + 4 <= a.x_size && (a.x_size & 3) == 0;
   int u[4];
@@ -1499,45 +1563,46 @@ __global__ __launch_bounds__(256) void k_dual_classify(Args a, int init) {
 #pragma unroll
     for (int q = 0; q < 4; q++) u[q] = x0 + q < a.x_size ? a.usage[p0 + q] : 1;
   }
-  if (!init) {
-    const bool adaptive_now = a.use_adaptive && ctl->exceptional;
-    if (iter >= a.iter_perturbation_on && ctl->wrong_row == 0 && !adaptive_now) {
-      const int tf = triple_fold(a, iter);
-      const int rk = a.rowmap[tf * a.y_size + y];
-      const int* cm = a.colmap + tf * a.x_size;
-      const int* fl = a.act_flags + (size_t)(iter & 1) * a.K;
-      bool upd[4];
-      bool any = false;
+  const int tf = triple_fold(a, iter);
+  const int rk = a.rowmap[tf * a.y_size + y];
+  const int* cm = a.colmap + tf * a.x_size;
+  const int* fl = a.act_flags + (size_t)(iter & 1) * a.K;
+  bool upd[4];
+  bool any = false;
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
-        upd[q] = x0 + q < a.x_size && u[q] != 1 && fl[cm[x0 + q] * a.nrow + rk] == 0;
-        any |= upd[q];
-      }
-      if (any) {
-        const float rho = ctl->rho, step = ctl->step, neg = -1.0f / rho;
-        if (full) {
-          float4 d = *reinterpret_cast<const float4*>(a.dual + p0);
-          if (upd[0]) d.x = dual_step(u[0], d.x, rho, step, neg);
-          if (upd[1]) d.y = dual_step(u[1], d.y, rho, step, neg);
-          if (upd[2]) d.z = dual_step(u[2], d.z, rho, step, neg);
-          if (upd[3]) d.w = dual_step(u[3], d.w, rho, step, neg);
-          *reinterpret_cast<float4*>(a.dual + p0) = d;
-        } else {
-#pragma unroll
-          for (int q = 0; q < 4; q++)
-            if (upd[q]) a.dual[p0 + q] = dual_step(u[q], a.dual[p0 + q], rho, step, neg);
-        }
-      }
-    }
+  for (int q = 0; q < 4; q++) {
+    upd[q] = x0 + q < a.x_size && u[q] != 1 && fl[cm[x0 + q] * a.nrow + rk] == 0;
+    any |= upd[q];
   }
-  if (u[0] > 1 || u[1] > 1 || u[2] > 1 || u[3] > 1) {
-    const int tf = triple_fold(a, nxt);
-    const int rk = a.rowmap[tf * a.y_size + y];
-    int32_t* fn = a.act_flags + (size_t)(nxt & 1) * a.K;
+  if (!any) return;
+  const float rho = ctl->rho, step = ctl->step, neg = -1.0f / rho;
+  if (full) {
+    float4 d = *reinterpret_cast<const float4*>(a.dual + p0);
+    if (upd[0]) d.x = dual_step(u[0], d.x, rho, step, neg);
+    if (upd[1]) d.y = dual_step(u[1], d.y, rho, step, neg);
+    if (upd[2]) d.z = dual_step(u[2], d.z, rho, step, neg);
+    if (upd[3]) d.w = dual_step(u[3], d.w, rho, step, neg);
+    *reinterpret_cast<float4*>(a.dual + p0) = d;
+  } else {
 #pragma unroll
     for (int q = 0; q < 4; q++)
-      if (u[q] > 1) fn[a.colmap[tf * a.x_size + x0 + q] * a.nrow + rk] = 1;
+      if (upd[q]) a.dual[p0 + q] = dual_step(u[q], a.dual[p0 + q], rho, step, neg);
   }
+}
+
+// Before the first iteration: the activity flags of iteration 0 (any pixel
+// with usage > 1 under its fold offset).  Four pixels per thread.
+__global__ __launch_bounds__(256) void k_classify_init(Args a) {
+  const Ctl* ctl = a.ctl;
+  const int y = blockIdx.y;
+  const int x0 = 4 * (blockIdx.x * blockDim.x + threadIdx.x);
+  if (x0 >= a.x_size) return;
+  const int iter = ctl->iter;
+  const int tf = triple_fold(a, iter);
+  const int rk = a.rowmap[tf * a.y_size + y];
+  int32_t* fc = a.act_flags + (size_t)(iter & 1) * a.K;
+  for (int q = 0; q < 4 && x0 + q < a.x_size; q++)
+    if (a.usage[pidx(a, x0 + q, y)] > 1) fc[a.colmap[tf * a.x_size + x0 + q] * a.nrow + rk] = 1;
 }
 
 // Cell records <-> SoA positions (records hold positions during the solve)
