@@ -1007,67 +1007,9 @@ __device__ __forceinline__ void eval_candidates(const Args& a, const PartGeo& g,
   }
 }
 
-// ---- register-resident sorting networks -----------------------------------
-// Element i = r*64 + lane lives in register r of lane `lane`.  Every
-// compare-exchange of the network on (i, i^j) is executed as in the LDS
-// version (AdmmKernels.hpp:899-945): partners with j >= 64 are two registers
-// of one lane, partners with j < 64 are one register of two lanes.
-template <int R, int JR, bool DESC_NET>
-__device__ __forceinline__ void reg_step_inlane(int n, int k, int (&key)[R], int (&val)[R]) {
-  const int lane = threadIdx.x;
-#pragma unroll
-  for (int r = 0; r < R; r++) {
-    constexpr int dummy = 0;
-    (void)dummy;
-    const int rp = r ^ JR;
-    if (rp > r && rp * kWave < n) {
-      const int lo = r * kWave + lane;
-      const bool desc = DESC_NET ? ((lo & k) == 0) : ((lo & k) != 0);
-      const int kl = key[r], kh = key[rp];
-      const bool sw = desc ? (kl < kh) : (kl > kh);
-      if (sw) {
-        key[r] = kh; key[rp] = kl;
-        int t = val[r]; val[r] = val[rp]; val[rp] = t;
-      }
-    }
-  }
-}
-
-template <int R, bool DESC_NET>
-__device__ __forceinline__ void reg_bitonic(int n, int (&key)[R], int (&val)[R]) {
-  const int lane = threadIdx.x;
-  for (int k = 2; k <= n; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      if (j >= kWave) {
-        const int jr = j >> 6;
-        if (R > 1 && jr == 1) reg_step_inlane<R, 1, DESC_NET>(n, k, key, val);
-        if (R > 2 && jr == 2) reg_step_inlane<R, (R > 2 ? 2 : 1), DESC_NET>(n, k, key, val);
-        if (R > 4 && jr == 4) reg_step_inlane<R, (R > 4 ? 4 : 1), DESC_NET>(n, k, key, val);
-        if (R > 8 && jr == 8) reg_step_inlane<R, (R > 8 ? 8 : 1), DESC_NET>(n, k, key, val);
-      } else {
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-          if (r * kWave < n) {
-            const int i = r * kWave + lane;
-            const int pk = __shfl_xor(key[r], j);
-            const int pv = __shfl_xor(val[r], j);
-            const bool i_lo = (lane & j) == 0;
-            const int lo = i_lo ? i : (i ^ j);
-            const bool desc = DESC_NET ? ((lo & k) == 0) : ((lo & k) != 0);
-            const int klo = i_lo ? key[r] : pk;
-            const int khi = i_lo ? pk : key[r];
-            const bool sw = desc ? (klo < khi) : (klo > khi);
-            if (sw) { key[r] = pk; val[r] = pv; }
-          }
-        }
-      }
-    }
-  }
-}
-
 // Staging of the partition's pixel tile, lanes walking the tile in order
 // with incremental (la, lb) = (i % xh, i / xh): consecutive lanes read
-// consecutive pixels of a grid row; four loads in flight per lane.
+// consecutive pixels of a grid row; sixteen pixels in flight per lane.
 // Returns true if a usage does not fit the 16-bit LDS packing.
 __device__ __forceinline__ bool stage_tile(const Args& a, const PartGeo& g, const Tile& T, float rho,
                                            bool standard, bool& over, int& cmin, int& cmax) {
@@ -1126,72 +1068,165 @@ __device__ __forceinline__ int sort_key(const Args& a, const PartGeo& g, const T
   return cr * (cr ? a.rand_prio[c] : 0) + co * ovf + ca * w * h;
 }
 
-// Processing order of one partition with both sorts in registers (npad <=
-// 64*R): ascending cell id (thrust::stable_sort_by_key order), keys, then the
-// reference's descending bitonic network on npad slots.  Writes the order to
-// grouped[off..] (ids) and rec_b[off..] (records).
-template <int R>
-__device__ __forceinline__ void order_partition_reg(const Args& a, const PartGeo& g, const Tile& T, int npad, int off,
-                                    int count, const int4* cellrec
+
+// ---- the descending network on slots held lane-major -------------------
+// Slot i = lane*RN + r lives in register r of lane `lane` (n = 64*RN slots, or
+// n < 64 with RN = 1).  A step of distance j < RN pairs two registers of one
+// lane; j >= RN pairs register r of lanes l and l ^ (j / RN).  The small
+// distances, the most frequent ones in a bitonic network, thus need no
+// cross-lane traffic.  PK: a key and its slot packed in one word
+// ((key + 1) << 10 | slot, pads 0 << 10 | slot) compared on the key bits, so a
+// cross-lane step moves one word; otherwise key and slot move separately.
+// Compare-exchange semantics are those of AdmmKernels.hpp:899-945: the pair
+// (lo, lo ^ j) is ordered descending if (lo & k) == 0, and swapped only if
+// strictly out of order.
+template <bool PK>
+__device__ __forceinline__ bool net_lt(int a, int b) {
+  return PK ? ((unsigned)a >> 10) < ((unsigned)b >> 10) : a < b;
+}
+
+template <int RN, int J, bool PK>
+__device__ __forceinline__ void net_step_lane(int k, int (&key)[RN], int (&slot)[RN]) {
+  const int lane = threadIdx.x;
+#pragma unroll
+  for (int r = 0; r < RN; r++) {
+    const int rp = r ^ J;
+    if (rp > r) {
+      const int lo = lane * RN + r;
+      const bool desc = (lo & k) == 0;
+      const int kl = key[r], kh = key[rp];
+      const bool sw = desc ? net_lt<PK>(kl, kh) : net_lt<PK>(kh, kl);
+      if (sw) {
+        key[r] = kh;
+        key[rp] = kl;
+        if (!PK) {
+          const int t = slot[r];
+          slot[r] = slot[rp];
+          slot[rp] = t;
+        }
+      }
+    }
+  }
+}
+
+template <int RN, bool PK>
+__device__ __forceinline__ void net_sort(int n, int (&key)[RN], int (&slot)[RN]) {
+  const int lane = threadIdx.x;
+  for (int k = 2; k <= n; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      if (j < RN) {
+        if (j == 1) net_step_lane<RN, 1, PK>(k, key, slot);
+        if (RN > 2 && j == 2) net_step_lane<RN, (RN > 2 ? 2 : 1), PK>(k, key, slot);
+        if (RN > 4 && j == 4) net_step_lane<RN, (RN > 4 ? 4 : 1), PK>(k, key, slot);
+        if (RN > 8 && j == 8) net_step_lane<RN, (RN > 8 ? 8 : 1), PK>(k, key, slot);
+      } else {
+        const int d = j / RN;
+        const bool i_lo = (lane & d) == 0;
+#pragma unroll
+        for (int r = 0; r < RN; r++) {
+          const int i = lane * RN + r;
+          const int lo = i_lo ? i : (i ^ j);
+          const bool desc = (lo & k) == 0;
+          const int ok = __shfl_xor(key[r], d);
+          const int os = PK ? 0 : __shfl_xor(slot[r], d);
+          const int klo = i_lo ? key[r] : ok, khi = i_lo ? ok : key[r];
+          const bool sw = desc ? net_lt<PK>(klo, khi) : net_lt<PK>(khi, klo);
+          if (sw) {
+            key[r] = ok;
+            if (!PK) slot[r] = os;
+          }
+        }
+      }
+    }
+  }
+}
+
+// Processing order of one partition with the sort in registers (npad <=
+// 64*RN; slots at or beyond npad are pads that only meet pads): keys of the
+// cells in ascending id order (the grouping's order, that of
+// thrust::stable_sort_by_key), the reference's descending bitonic network on
+// npad slots, then the ids read back by slot.  Writes the order to
+// grouped[off..] (ids) and rec_b[off..] (records).  With PK the keys must
+// pack (key + 1 < 2^22); otherwise nothing is written and false returned.
+// The keys are computed in a loop that is not unrolled, so the kernel holds
+// one copy of the overflow sums per instance.
+template <int RN, bool PK>
+__device__ __forceinline__ bool order_partition_reg(const Args& a, const PartGeo& g, const Tile& T, int npad, int off,
+                                                    int count, const int4* cellrec
 #ifdef LG_PHASE_TIMERS
-                                    , int* ph_
+                                                    , int* ph_
 #endif
-                                    ) {
+                                                    ) {
   const int lane = threadIdx.x;
 #ifdef LG_PHASE_TIMERS
   unsigned long long t0_ = wall_clock64();
 #endif
   int* ids = a.grouped + off;
   int4* recb = reinterpret_cast<int4*>(a.rec_b) + off;
-  // the grouping lists the cells in ascending id order; the sort carries the
-  // cell id as its payload (pads: -1)
-  int key[R], val[R];
-  // registers beyond the partition's count hold padding only (uniform test);
-  // loads use a clamped index and are selected afterwards
+  // loads at a clamped slot, selected afterwards (slots >= count are pads)
+  int val[RN];
 #pragma unroll
-  for (int r = 0; r < R; r++) {
-    val[r] = -1;
-    if (r * kWave < count) {
-      const int i = r * kWave + lane;
-      const int v = ids[min(i, count - 1)];
-      val[r] = i < count ? v : -1;
-    }
+  for (int r = 0; r < RN; r++) val[r] = ids[min(lane * RN + r, count - 1)];
+  int4 rc[RN];
+#pragma unroll
+  for (int r = 0; r < RN; r++) rc[r] = cellrec[val[r]];
+  int key[RN], slot[RN];
+  int kmax = INT_MIN;
+#pragma unroll 1
+  for (int r = 0; r < RN; r++) {
+    int v = val[0];
+    int4 rr = rc[0];
+#pragma unroll
+    for (int q = 1; q < RN; q++)
+      if (q == r) { v = val[q]; rr = rc[q]; }
+    const int i = lane * RN + r;
+    const int kk = i < count ? sort_key(a, g, T, v, rr) : INT_MIN;
+    kmax = max(kmax, kk);
+#pragma unroll
+    for (int q = 0; q < RN; q++)
+      if (q == r) key[q] = kk;
   }
-  int4 rc[R];
 #pragma unroll
-  for (int r = 0; r < R; r++)
-    if (r * kWave < count) rc[r] = cellrec[val[r] >= 0 ? val[r] : ids[0]];
-#pragma unroll
-  for (int r = 0; r < R; r++) {
-    key[r] = INT_MIN;
-    if (r * kWave < count) {
-      const int kk = sort_key(a, g, T, val[r] >= 0 ? val[r] : 0, rc[r]);
-      key[r] = val[r] >= 0 ? kk : INT_MIN;
-    }
+  for (int r = 0; r < RN; r++) slot[r] = lane * RN + r;
+  if (PK) {
+    for (int o = 1; o < kWave; o <<= 1) kmax = max(kmax, __shfl_xor(kmax, o));
+    if (kmax >= (1 << 22) - 1) return false;
   }
 #ifdef LG_PHASE_TIMERS
   unsigned long long t1_ = wall_clock64();
   ph_[5] = (int)(t1_ - t0_);
 #endif
-  reg_bitonic<R, true>(npad, key, val);
+  if (PK) {
+#pragma unroll
+    for (int r = 0; r < RN; r++)
+      key[r] = (int)(((unsigned)(key[r] == INT_MIN ? 0 : key[r] + 1) << 10) | (unsigned)slot[r]);
+    net_sort<RN, PK>(npad, key, slot);
+#pragma unroll
+    for (int r = 0; r < RN; r++) slot[r] = key[r] & 1023;
+  } else {
+    net_sort<RN, PK>(npad, key, slot);
+  }
 #ifdef LG_PHASE_TIMERS
   unsigned long long t2_ = wall_clock64();
   ph_[6] = (int)(t2_ - t1_);
 #endif
+  // pads sort behind every cell, so slots < count map to cells
 #pragma unroll
-  for (int r = 0; r < R; r++) {
-    if (r * kWave < count) {
-      const int i = r * kWave + lane;
-      const int4 rv = cellrec[val[r] >= 0 ? val[r] : ids[0]];
-      if (i < count) {
-        ids[i] = val[r];
-        recb[i] = rv;
-      }
+  for (int r = 0; r < RN; r++) val[r] = ids[min(slot[r], count - 1)];
+#pragma unroll
+  for (int r = 0; r < RN; r++) rc[r] = cellrec[val[r]];
+#pragma unroll
+  for (int r = 0; r < RN; r++) {
+    const int i = lane * RN + r;
+    if (i < count) {
+      ids[i] = val[r];
+      recb[i] = rc[r];
     }
   }
 #ifdef LG_PHASE_TIMERS
   ph_[7] = (int)(wall_clock64() - t2_);
 #endif
+  return true;
 }
 
 // The same with the sorts in LDS (npad > 1024, or the merge / radix options).
@@ -1237,9 +1272,12 @@ __device__ __forceinline__ float dual_step(int ui, float l, float rho, float ste
 // activity flag, under the next iteration's fold offset, of the partition
 // holding pixel (x, y); set for every pixel with usage > 1 that a primal step
 // writes (only partitions that take the step can hold such pixels)
+// (computed like the colmap / rowmap tables, without their loads)
 __device__ __forceinline__ void flag_next(const Args& a, int iter, int x, int y) {
   const int tn = triple_fold(a, iter + 1);
-  a.act_flags[(size_t)((iter + 1) & 1) * a.K + a.colmap[tn * a.x_size + x] * a.nrow + a.rowmap[tn * a.y_size + y]] = 1;
+  const int col = ((x + a.x_hint / 3 * tn) / a.x_hint) % a.ncol;
+  const int row = ((y + a.y_hint / 3 * tn) / a.y_hint) % a.nrow;
+  a.act_flags[(size_t)((iter + 1) & 1) * a.K + col * a.nrow + row] = 1;
 }
 
 // doPrimalUpdateKernel (AdmmKernels.hpp:637-897) / doFinalRefineKernel
@@ -1399,11 +1437,25 @@ __global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_
       // an unsorted partition at the final refine moved nothing in the last
       // iteration, so its usage still is the one the order is defined on
       if (LDS_SORT) order_partition_lds(a, g, T, npad, npad_hi, off, count, cellrec, keys, slots, aux);
-      else order_partition_reg<kNpadSmall / kWave>(a, g, T, npad, off, count, cellrec
+      else {
 #ifdef LG_PHASE_TIMERS
-                                                     , ph_
+#define ORDER_REG(RN, PK) order_partition_reg<RN, PK>(a, g, T, npad, off, count, cellrec, ph_)
+#else
+#define ORDER_REG(RN, PK) order_partition_reg<RN, PK>(a, g, T, npad, off, count, cellrec)
 #endif
-                                                     );
+        // packed keys on npad / 64 registers per lane; otherwise the
+        // unpacked network on the layout of 16 registers
+        bool ok = false;
+        if (a.sort_policy != 0) {
+          if (npad <= kWave) ok = ORDER_REG(1, true);
+          else if (npad == 2 * kWave) ok = ORDER_REG(2, true);
+          else if (npad == 4 * kWave) ok = ORDER_REG(4, true);
+          else if (npad == 8 * kWave) ok = ORDER_REG(8, true);
+          else ok = ORDER_REG(16, true);
+        }
+        if (!ok) ORDER_REG(16, false);
+#undef ORDER_REG
+      }
       if (MODE == MODE_PRIMAL && lane == 0) a.sorted[k] = 1;
     } else {
       for (int i = lane; i < count; i += kWave) recs[i] = cellrec[ids[i]];
@@ -1511,21 +1563,40 @@ __global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_
   // -- write back pixels; primal mode fuses the dual update and the overflow
   unsigned long long ovf = 0;
   {
-    // the primal step also sets the next iteration's activity flags
+    // eight pixels per lane at a time, their LDS reads issued together; the
+    // primal step also sets the next iteration's activity flags
+    constexpr int U = 8;
     int lb = lane / g.xh, la = lane - lb * g.xh;
-    for (int i = lane; i < npix; i += kWave) {
-      int xx = g.gx(la), yy = g.gy(lb);
-      const int li = T.at(la, lb);
-      la += step_a; lb += step_b;
-      if (la >= g.xh) { la -= g.xh; lb++; }
-      if (xx < 0 || yy < 0) continue;
-      const unsigned p = pidx(a, xx, yy);
-      int ui = (int)(T.uc[li] & 0xffffu);
-      a.usage[p] = ui;
-      if (MODE == MODE_PRIMAL) {
-        a.dual[p] = dual_step(ui, T.lam[li], rho, step, neg);
-        ovf += (unsigned long long)max(ui - 1, 0);
-        if (ui > 1) flag_next(a, iter, xx, yy);
+    for (int i0 = 0; i0 < npix; i0 += U * kWave) {
+      int px[U], py[U], li[U];
+#pragma unroll
+      for (int q = 0; q < U; q++) {
+        px[q] = g.gx(la);
+        py[q] = g.gy(lb);
+        li[q] = T.at(la, lb);
+        if (i0 + q * kWave + lane >= npix || py[q] < 0) px[q] = -1;
+        la += step_a; lb += step_b;
+        if (la >= g.xh) { la -= g.xh; lb++; }
+      }
+      uint32_t uc[U];
+      float lm[U];
+#pragma unroll
+      for (int q = 0; q < U; q++) {
+        const int l = px[q] >= 0 ? li[q] : 0;
+        uc[q] = T.uc[l];
+        lm[q] = T.lam[l];
+      }
+#pragma unroll
+      for (int q = 0; q < U; q++) {
+        if (px[q] < 0) continue;
+        const unsigned p = pidx(a, px[q], py[q]);
+        const int ui = (int)(uc[q] & 0xffffu);
+        a.usage[p] = ui;
+        if (MODE == MODE_PRIMAL) {
+          a.dual[p] = dual_step(ui, lm[q], rho, step, neg);
+          ovf += (unsigned long long)max(ui - 1, 0);
+          if (ui > 1) flag_next(a, iter, px[q], py[q]);
+        }
       }
     }
   }
