@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <numeric>
+#include <cstdlib>
 #include <random>
 #include <string>
 #include <vector>
@@ -367,6 +368,13 @@ int lg_create(const LgGridDesc* g, const LgCellsSoA* c, const LgParams* p, LgHan
   a.cooltime_adaptive = p->cooltime_to_call_adaptive_search;
   a.ovf_ratio_adaptive = p->ovf_ratio_to_call_adaptive_search;
   a.coeff_neglect_disp = p->coeff_to_neglect_displace;
+  // a lower limit (tests of the ranked sort keys) may come from the
+  // environment; keys of up to 22 bits pack with a 10-bit slot
+  a.pack_limit = (1 << 22) - 1;
+  if (const char* e = std::getenv("SKYLINE_LG_PACK_LIMIT")) {
+    const long v = std::strtol(e, nullptr, 10);
+    if (v >= 0 && v < a.pack_limit) a.pack_limit = (int)v;
+  }
   *out = h;
   return LG_OK;
 }

@@ -156,6 +156,7 @@ struct Args {
   int sort_policy, sort_algo;
   int use_adaptive, stagnation_window, min_iter_adaptive, cooltime_adaptive;
   float ovf_ratio_adaptive, coeff_neglect_disp;
+  int pack_limit;      // sort keys at or above it are ranked before packing ((1 << 22) - 1)
   Ctl* ctl;
 };
 
@@ -1146,11 +1147,14 @@ __device__ __forceinline__ void net_sort(int n, int (&key)[RN], int (&slot)[RN])
 // cells in ascending id order (the grouping's order, that of
 // thrust::stable_sort_by_key), the reference's descending bitonic network on
 // npad slots, then the ids read back by slot.  Writes the order to
-// grouped[off..] (ids) and rec_b[off..] (records).  With PK the keys must
-// pack (key + 1 < 2^22); otherwise nothing is written and false returned.
-// The keys are computed in a loop that is not unrolled, so the kernel holds
-// one copy of the overflow sums per instance.
-template <int RN, bool PK>
+// grouped[off..] (ids) and rec_b[off..] (records).  The network moves packed
+// words (key + 1) << 10 | slot; keys with key + 1 >= 2^22 (or < 0) are first
+// replaced by their rank (the number of smaller keys, ties equal), which the
+// network orders the same way -- in the instance of 16 registers only: the
+// others then return false having written nothing.  The keys are computed in
+// a loop that is not unrolled, so the kernel holds one copy of the overflow
+// sums per instance.
+template <int RN>
 __device__ __forceinline__ bool order_partition_reg(const Args& a, const PartGeo& g, const Tile& T, int npad, int off,
                                                     int count, const int4* cellrec
 #ifdef LG_PHASE_TIMERS
@@ -1170,8 +1174,8 @@ __device__ __forceinline__ bool order_partition_reg(const Args& a, const PartGeo
   int4 rc[RN];
 #pragma unroll
   for (int r = 0; r < RN; r++) rc[r] = cellrec[val[r]];
-  int key[RN], slot[RN];
-  int kmax = INT_MIN;
+  int key[RN];
+  int kmin = INT_MAX, kmax = INT_MIN;
 #pragma unroll 1
   for (int r = 0; r < RN; r++) {
     int v = val[0];
@@ -1181,31 +1185,53 @@ __device__ __forceinline__ bool order_partition_reg(const Args& a, const PartGeo
       if (q == r) { v = val[q]; rr = rc[q]; }
     const int i = lane * RN + r;
     const int kk = i < count ? sort_key(a, g, T, v, rr) : INT_MIN;
-    kmax = max(kmax, kk);
+    if (i < count) {
+      kmin = min(kmin, kk);
+      kmax = max(kmax, kk);
+    }
 #pragma unroll
     for (int q = 0; q < RN; q++)
       if (q == r) key[q] = kk;
   }
+  for (int o = 1; o < kWave; o <<= 1) {
+    kmin = min(kmin, __shfl_xor(kmin, o));
+    kmax = max(kmax, __shfl_xor(kmax, o));
+  }
+  if (kmin < 0 || kmax >= a.pack_limit) {
+    // the instance of 16 registers handles such keys
+    if (RN < 16) return false;
+    // rank of each cell key among the cell keys (pads stay INT_MIN)
+    int rk[RN];
 #pragma unroll
-  for (int r = 0; r < RN; r++) slot[r] = lane * RN + r;
-  if (PK) {
-    for (int o = 1; o < kWave; o <<= 1) kmax = max(kmax, __shfl_xor(kmax, o));
-    if (kmax >= (1 << 22) - 1) return false;
+    for (int r = 0; r < RN; r++) rk[r] = 0;
+#pragma unroll 1
+    for (int j = 0; j < count; j++) {
+      const int src = j / RN, q = j - src * RN;
+      int kq = key[0];
+#pragma unroll
+      for (int u = 1; u < RN; u++)
+        if (u == q) kq = key[u];
+      const int ks = __shfl(kq, src);
+#pragma unroll
+      for (int r = 0; r < RN; r++) rk[r] += ks < key[r] ? 1 : 0;
+    }
+#pragma unroll
+    for (int r = 0; r < RN; r++)
+      if (lane * RN + r < count) key[r] = rk[r];
   }
 #ifdef LG_PHASE_TIMERS
   unsigned long long t1_ = wall_clock64();
   ph_[5] = (int)(t1_ - t0_);
 #endif
-  if (PK) {
+  int slot[RN];
 #pragma unroll
-    for (int r = 0; r < RN; r++)
-      key[r] = (int)(((unsigned)(key[r] == INT_MIN ? 0 : key[r] + 1) << 10) | (unsigned)slot[r]);
-    net_sort<RN, PK>(npad, key, slot);
-#pragma unroll
-    for (int r = 0; r < RN; r++) slot[r] = key[r] & 1023;
-  } else {
-    net_sort<RN, PK>(npad, key, slot);
+  for (int r = 0; r < RN; r++) {
+    slot[r] = lane * RN + r;
+    key[r] = (int)(((unsigned)(key[r] == INT_MIN ? 0 : key[r] + 1) << 10) | (unsigned)slot[r]);
   }
+  net_sort<RN, true>(npad, key, slot);
+#pragma unroll
+  for (int r = 0; r < RN; r++) slot[r] = key[r] & 1023;
 #ifdef LG_PHASE_TIMERS
   unsigned long long t2_ = wall_clock64();
   ph_[6] = (int)(t2_ - t1_);
@@ -1439,21 +1465,16 @@ __global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_
       if (LDS_SORT) order_partition_lds(a, g, T, npad, npad_hi, off, count, cellrec, keys, slots, aux);
       else {
 #ifdef LG_PHASE_TIMERS
-#define ORDER_REG(RN, PK) order_partition_reg<RN, PK>(a, g, T, npad, off, count, cellrec, ph_)
+#define ORDER_REG(RN) order_partition_reg<RN>(a, g, T, npad, off, count, cellrec, ph_)
 #else
-#define ORDER_REG(RN, PK) order_partition_reg<RN, PK>(a, g, T, npad, off, count, cellrec)
+#define ORDER_REG(RN) order_partition_reg<RN>(a, g, T, npad, off, count, cellrec)
 #endif
-        // packed keys on npad / 64 registers per lane; otherwise the
-        // unpacked network on the layout of 16 registers
+        // npad / 64 registers per lane, at least four; keys too wide to
+        // pack go to the instance of 16 registers
         bool ok = false;
-        if (a.sort_policy != 0) {
-          if (npad <= kWave) ok = ORDER_REG(1, true);
-          else if (npad == 2 * kWave) ok = ORDER_REG(2, true);
-          else if (npad == 4 * kWave) ok = ORDER_REG(4, true);
-          else if (npad == 8 * kWave) ok = ORDER_REG(8, true);
-          else ok = ORDER_REG(16, true);
-        }
-        if (!ok) ORDER_REG(16, false);
+        if (npad <= 4 * kWave) ok = ORDER_REG(4);
+        else if (npad == 8 * kWave) ok = ORDER_REG(8);
+        if (!ok) ORDER_REG(16);
 #undef ORDER_REG
       }
       if (MODE == MODE_PRIMAL && lane == 0) a.sorted[k] = 1;

@@ -95,6 +95,20 @@ def test_random_policy_and_standard_admm(gpu):
     _assert_same(g, o)
 
 
+@pytest.mark.parametrize("policy", ["OVERFLOW", "RANDOM"])
+def test_ranked_sort_keys(gpu, monkeypatch, policy):
+    """Sort keys too wide to pack with their slot are ranked first; a low
+    packing limit sends every partition down that path."""
+    monkeypatch.setenv("SKYLINE_LG_PACK_LIMIT", "3")
+    d = synth.mixed(3000, seed=3)
+    p = lg.default_params()
+    p.max_admm_iter = 60
+    p.sorting_policy = SORT_POLICY[policy]
+    p.random_sorting_seed = 11
+    g, o = _run_both(d, p, do_post=False)
+    _assert_same(g, o)
+
+
 def test_qp_refine(gpu):
     d = synth.make_design(2000, 250, 50, width_range=(1, 8), num_macros=2, seed=6, name="qp")
     p = lg.default_params()
