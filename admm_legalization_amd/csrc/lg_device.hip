@@ -14,6 +14,8 @@
 #include <random>
 #include <string>
 #include <vector>
+#include <map>
+#include <unordered_map>
 
 #include "../../include/skyline_lg.h"
 #include "lg_common.h"
@@ -74,10 +76,109 @@ int upload_directions() {
 
 }  // namespace
 
+// Device-memory pool of the request server: chunks of device memory taken
+// with hipMalloc on demand up to a byte budget, first-fit sub-allocation with
+// coalescing of free neighbours.  Requests are admitted only while their
+// buffers fit the budget; nothing is returned to the runtime before the
+// server ends (hipFree would synchronise the device mid-stream).
+struct DevPool {
+  struct Chunk {
+    char* base;
+    size_t cap;
+    std::map<size_t, size_t> free;  // offset -> bytes
+  };
+  size_t budget = 0, chunk_bytes = 256u << 20, reserved = 0, in_use = 0, peak = 0;
+  std::vector<Chunk> chunks;
+  std::unordered_map<void*, std::pair<int, size_t>> live;  // block -> (chunk, bytes)
+
+  static size_t round(size_t n) { return (std::max<size_t>(n, 1) + 255) & ~(size_t)255; }
+
+  void* take(int ci, size_t n) {
+    Chunk& c = chunks[ci];
+    for (auto it = c.free.begin(); it != c.free.end(); ++it) {
+      if (it->second < n) continue;
+      const size_t off = it->first, rest = it->second - n;
+      c.free.erase(it);
+      if (rest) c.free.emplace(off + n, rest);
+      void* p = c.base + off;
+      live.emplace(p, std::make_pair(ci, n));
+      in_use += n;
+      peak = std::max(peak, in_use);
+      return p;
+    }
+    return nullptr;
+  }
+  // nullptr when the budget is exhausted (or the runtime refuses a chunk)
+  void* alloc(size_t n) {
+    n = round(n);
+    for (int ci = 0; ci < (int)chunks.size(); ci++)
+      if (void* p = take(ci, n)) return p;
+    if (reserved + n > budget) return nullptr;
+    const size_t want = std::max(std::min(chunk_bytes, budget - reserved), n);
+    char* base = nullptr;
+    if (hipMalloc((void**)&base, want) != hipSuccess) return nullptr;
+    reserved += want;
+    chunks.push_back(Chunk{base, want, {}});
+    chunks.back().free.emplace(0, want);
+    return take((int)chunks.size() - 1, n);
+  }
+  void release(void* p) {
+    auto it = live.find(p);
+    if (it == live.end()) return;
+    Chunk& c = chunks[it->second.first];
+    size_t off = (char*)p - c.base, n = it->second.second;
+    in_use -= n;
+    live.erase(it);
+    auto nx = c.free.lower_bound(off);
+    if (nx != c.free.end() && off + n == nx->first) {  // merge the right neighbour
+      n += nx->second;
+      nx = c.free.erase(nx);
+    }
+    if (nx != c.free.begin()) {  // merge the left neighbour
+      auto pv = std::prev(nx);
+      if (pv->first + pv->second == off) {
+        pv->second += n;
+        return;
+      }
+    }
+    c.free.emplace(off, n);
+  }
+  size_t available() const {
+    size_t f = budget - reserved;
+    for (const Chunk& c : chunks)
+      for (const auto& kv : c.free) f += kv.second;
+    return f;
+  }
+  void destroy() {
+    for (Chunk& c : chunks) (void)hipFree(c.base);
+    chunks.clear();
+    live.clear();
+    reserved = in_use = 0;
+  }
+};
+
+// Host-side preparation of one legalization (what lg_create computes before
+// any device call): validated sizes, the pixel codes and region ids in the
+// device's row-major order, grid positions, packed cell codes, the height
+// histogram, random sorting priorities and the fold-offset maps.
+struct HostPrep {
+  LgParams p;
+  int x_size = 0, y_size = 0, site_width = 0, row_height = 0, core_lx = 0, core_ly = 0;
+  int ncol = 0, nrow = 0, K = 0, N = 0;
+  size_t P = 0;
+  std::vector<int16_t> pcode;
+  std::vector<int32_t> region_rm, vdd, rx0, ry0, rx1, ry1;
+  std::vector<int32_t> lxg, lyg, w, h, code, gnd, ox, oy, rprio, khn, colmap, rowmap, ident;
+};
+
 struct LgHandle {
   LgParams p;
   Args a;
   hipStream_t own_stream = nullptr;
+  bool owns_stream = true;     // false: the stream belongs to a request server
+  DevPool* pool = nullptr;     // non-null: buffers come from a server's pool,
+  char* blk = nullptr;         //   carved from one block of the pool
+  size_t blk_size = 0, blk_used = 0;
   int solved = 0, posted = 0;
   size_t P = 0;
   // device buffers (freed in lg_destroy)
@@ -100,11 +201,40 @@ struct LgHandle {
 
 namespace {
 
+// device buffer of a handle: carved from its pool block (else from the
+// pool itself), or hipMalloc for a handle outside a server
 template <typename T>
 int halloc(LgHandle* h, T** p, size_t n) {
+  if (h->pool) {
+    const size_t b = DevPool::round(std::max<size_t>(n, 1) * sizeof(T));
+    if (h->blk && h->blk_used + b <= h->blk_size) {
+      *p = (T*)(h->blk + h->blk_used);
+      h->blk_used += b;
+      return LG_OK;
+    }
+    *p = (T*)h->pool->alloc(b);
+    if (!*p) return fail(LG_ERR_MEMORY, "request server: device-memory budget exhausted");
+    h->bufs.push_back((void*)*p);
+    return LG_OK;
+  }
   int st = dalloc(p, n);
   if (st == LG_OK) h->bufs.push_back((void*)*p);
   return st;
+}
+
+// temporary device buffer of a handle (returned with hfree_tmp)
+template <typename T>
+int halloc_tmp(LgHandle* h, T** p, size_t n) {
+  if (h->pool) {
+    *p = (T*)h->pool->alloc(std::max<size_t>(n, 1) * sizeof(T));
+    return *p ? LG_OK : fail(LG_ERR_MEMORY, "request server: device-memory budget exhausted");
+  }
+  return dalloc(p, n);
+}
+
+void hfree_tmp(LgHandle* h, void* p) {
+  if (h->pool) h->pool->release(p);
+  else (void)hipFree(p);
 }
 
 template <typename T>
@@ -179,123 +309,180 @@ int read_ctl(LgHandle* h, hipStream_t s) {
   return h->prof ? collect_profile(h) : LG_OK;
 }
 
-}  // namespace
 
-extern "C" {
-
-const char* lg_last_error(void) { return g_err.c_str(); }
-
-int lg_create(const LgGridDesc* g, const LgCellsSoA* c, const LgParams* p, LgHandle** out) {
-  if (!g || !c || !p || !out) return fail(LG_ERR_ARG, "null argument");
+// Validation and host-side preparation of lg_create (no device calls).
+int prepare_host(const LgGridDesc* g, const LgCellsSoA* c, const LgParams* p, HostPrep& hp) {
+  if (!g || !c || !p) return fail(LG_ERR_ARG, "null argument");
   if (g->x_size <= 0 || g->y_size <= 0 || c->num_cells < 0) return fail(LG_ERR_ARG, "bad sizes");
   if (g->site_width <= 0 || g->row_height <= 0) return fail(LG_ERR_ARG, "bad site/row size");
   if (p->x_hint < 3 || p->y_hint < 3 || p->iter_update_partition <= 0 || p->iter_update_dual_step <= 0)
     return fail(LG_ERR_ARG, "bad partition / schedule parameters");
   if (g->x_size > 65535 || g->y_size > 65535) return fail(LG_ERR_ARG, "grid larger than 65535 sites/rows");
   if ((long long)g->x_size * g->y_size >= (1ll << 30)) return fail(LG_ERR_ARG, "grid of 2^30 pixels or more");
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(LG_ERR_DEVICE, "no HIP device");
-  int st = upload_directions();
-  if (st) return st;
-
-  LgHandle* h = new LgHandle();
-  h->p = *p;
-  h->coeff_init_rho = p->coeff_to_init_rho;
-  h->coeff_init_dual = p->coeff_to_init_dual;
-  Args& a = h->a;
-  std::memset(&a, 0, sizeof(a));
-  a.x_size = g->x_size; a.y_size = g->y_size;
-  a.site_width = g->site_width; a.row_height = g->row_height;
-  a.core_lx = g->core_lx; a.core_ly = g->core_ly;
-  a.x_hint = p->x_hint; a.y_hint = p->y_hint;
+  hp.p = *p;
+  hp.x_size = g->x_size; hp.y_size = g->y_size;
+  hp.site_width = g->site_width; hp.row_height = g->row_height;
+  hp.core_lx = g->core_lx; hp.core_ly = g->core_ly;
   // CudaAdmmLegalizer ctor (CudaAdmmLegalizer.cu:86-89)
-  a.ncol = g->x_size % p->x_hint == 0 ? g->x_size / p->x_hint : g->x_size / p->x_hint + 1;
-  a.nrow = g->y_size % p->y_hint == 0 ? g->y_size / p->y_hint : g->y_size / p->y_hint + 1;
-  a.K = a.ncol * a.nrow;
-  a.N = c->num_cells;
+  hp.ncol = g->x_size % p->x_hint == 0 ? g->x_size / p->x_hint : g->x_size / p->x_hint + 1;
+  hp.nrow = g->y_size % p->y_hint == 0 ? g->y_size / p->y_hint : g->y_size / p->y_hint + 1;
+  hp.K = hp.ncol * hp.nrow;
+  hp.N = c->num_cells;
   const size_t P = (size_t)g->x_size * g->y_size;
-  h->P = P;
-  const int N = a.N;
-
-  auto bail = [&](int code) { lg_destroy(h); return code; };
-
+  hp.P = P;
+  const int N = hp.N;
   // pixel code: valid ? group + 2 : 0.  The ABI's pixel arrays are x-major
   // (x * y_size + y, the reference's Grid layout); the device keeps them
   // row-major (y * x_size + x).
-  std::vector<int16_t> pcode(P);
-  std::vector<int32_t> region_rm(P, -1);
+  hp.pcode.assign(P, 0);
+  hp.region_rm.assign(P, -1);
   const int XS = g->x_size, YS = g->y_size;
   for (int x = 0; x < XS; x++)
     for (int y = 0; y < YS; y++) {
       size_t i = (size_t)x * YS + y, j = (size_t)y * XS + x;
       int grp = g->pixel_group[i];
-      if (g->pixel_valid[i] && (grp + 2 > 0x7fff || grp + 2 < 0)) {
-        lg_destroy(h);
+      if (g->pixel_valid[i] && (grp + 2 > 0x7fff || grp + 2 < 0))
         return fail(LG_ERR_ARG, "fence group index out of range");
-      }
-      pcode[j] = g->pixel_valid[i] ? (int16_t)(grp + 2) : (int16_t)0;
-      if (g->pixel_region) region_rm[j] = g->pixel_region[i];
+      hp.pcode[j] = g->pixel_valid[i] ? (int16_t)(grp + 2) : (int16_t)0;
+      if (g->pixel_region) hp.region_rm[j] = g->pixel_region[i];
     }
-  std::vector<int32_t> lxg(N), lyg(N), code(N);
+  hp.vdd.assign(g->row_vdd_up, g->row_vdd_up + YS);
+  const int nr = std::max(g->num_regions, 0);
+  hp.rx0.assign(g->region_x_min, g->region_x_min + nr);
+  hp.ry0.assign(g->region_y_min, g->region_y_min + nr);
+  hp.rx1.assign(g->region_x_max, g->region_x_max + nr);
+  hp.ry1.assign(g->region_y_max, g->region_y_max + nr);
+  hp.lxg.resize(N); hp.lyg.resize(N); hp.code.resize(N);
   int max_h = 1;
   for (int i = 0; i < N; i++) {
     // dbu_to_grid_functor (DatabaseKernels.hpp:7-20)
-    lxg[i] = (c->lx_dbu[i] - g->core_lx) / g->site_width;
-    lyg[i] = (c->ly_dbu[i] - g->core_ly) / g->row_height;
-    code[i] = c->group[i] + 2;
+    hp.lxg[i] = (c->lx_dbu[i] - g->core_lx) / g->site_width;
+    hp.lyg[i] = (c->ly_dbu[i] - g->core_ly) / g->row_height;
+    hp.code[i] = c->group[i] + 2;
     int w = c->w_grid[i], hh = c->h_grid[i];
-    if (w <= 0 || hh <= 0 || w > 4095 || hh > 63 || code[i] <= 0 || code[i] > 8191) {
-      lg_destroy(h);
+    if (w <= 0 || hh <= 0 || w > 4095 || hh > 63 || hp.code[i] <= 0 || hp.code[i] > 8191)
       return fail(LG_ERR_ARG, "cell size or group outside the supported range");
-    }
-    if ((w > p->x_hint && a.ncol > 1) || (hh > p->y_hint && a.nrow > 1)) {
-      lg_destroy(h);
+    if ((w > p->x_hint && hp.ncol > 1) || (hh > p->y_hint && hp.nrow > 1))
       return fail(LG_ERR_ARG, "cell larger than a partition (x_hint/y_hint)");
-    }
-    if (lxg[i] < 0 || lyg[i] < 0 || lxg[i] + w > g->x_size || lyg[i] + hh > g->y_size) {
-      lg_destroy(h);
+    if (hp.lxg[i] < 0 || hp.lyg[i] < 0 || hp.lxg[i] + w > g->x_size || hp.lyg[i] + hh > g->y_size)
       return fail(LG_ERR_ARG, "cell outside the grid (not preplaced?)");
-    }
     max_h = std::max(max_h, hh);
   }
-  std::vector<int32_t> khn(max_h, 0);
-  for (int i = 0; i < N; i++) khn[c->h_grid[i] - 1]++;
-  std::vector<int32_t> rprio(N, 0);
+  hp.w.assign(c->w_grid, c->w_grid + N);
+  hp.h.assign(c->h_grid, c->h_grid + N);
+  hp.gnd.assign(c->ground_at_bottom, c->ground_at_bottom + N);
+  hp.ox.assign(c->orig_lx_dbu, c->orig_lx_dbu + N);
+  hp.oy.assign(c->orig_ly_dbu, c->orig_ly_dbu + N);
+  hp.khn.assign(max_h, 0);
+  for (int i = 0; i < N; i++) hp.khn[c->h_grid[i] - 1]++;
+  hp.rprio.assign(N, 0);
   if (p->sorting_policy == LG_SORT_POLICY_RANDOM) {  // CudaAdmmLegalizer.cu:120-130
-    std::iota(rprio.begin(), rprio.end(), 0);
+    std::iota(hp.rprio.begin(), hp.rprio.end(), 0);
     std::mt19937 gen(p->random_sorting_seed);
-    std::shuffle(rprio.begin(), rprio.end(), gen);
+    std::shuffle(hp.rprio.begin(), hp.rprio.end(), gen);
   }
+  hp.ident.resize(std::max(N, 1));
+  std::iota(hp.ident.begin(), hp.ident.end(), 0);
+  // partition column / row of every x / y for the three fold offsets
+  // (paintPixelsKernel, AdmmKernels.hpp:164-191)
+  hp.colmap.resize(3 * (size_t)XS);
+  hp.rowmap.resize(3 * (size_t)YS);
+  for (int tf = 0; tf < 3; tf++) {
+    const int ox = p->x_hint / 3 * tf, oy = p->y_hint / 3 * tf;
+    for (int x = 0; x < XS; x++) hp.colmap[tf * (size_t)XS + x] = ((x + ox) / p->x_hint) % hp.ncol;
+    for (int y = 0; y < YS; y++) hp.rowmap[tf * (size_t)YS + y] = ((y + oy) / p->y_hint) % hp.nrow;
+  }
+  return LG_OK;
+}
 
-  HIP_TRY(hipStreamCreateWithFlags(&h->own_stream, hipStreamNonBlocking));
+// Device bytes a handle of `hp` takes (an upper bound of create_from_prep's
+// buffers, each rounded to the pool's 256-byte blocks).
+size_t handle_bytes(const HostPrep& hp) {
+  const size_t P = hp.P, N = std::max(hp.N, 1), K = hp.K, blk = 256;
+  const size_t nb = (N + 4095) / 4096;
+  size_t b = P * (2 + 4 + 4 + 4);                       // pcode, region, usage, dual
+  b += N * 4 * (2 + 2 + 8 + 1 + 1 + 1 + 1 + 1 + 5 + 2);  // cells, records, radix buffers, flags
+  b += N * 4 * 4 * 2;                                   // cellrec, rec_b
+  b += K * 4 * (1 + 2 + 1 + 2 + 1 + 2 + 2) + 128 * nb * 4;
+  b += (3 * (size_t)hp.x_size + 3 * (size_t)hp.y_size + hp.y_size + 4 * hp.rx0.size()) * 4;
+  b += (size_t)std::max(hp.p.max_admm_iter, 1) * 4 + sizeof(Ctl);
+  return b + 64 * blk;
+}
+
+// Device side of lg_create from a prepared design.  With a pool, every
+// buffer comes from it; `ext_ctl` places the iteration control in a server's
+// Ctl array; `stream` (if given) is the handle's stream, not owned.
+int create_from_prep(const HostPrep& hp, DevPool* pool, Ctl* ext_ctl, hipStream_t stream, LgHandle** out) {
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(LG_ERR_DEVICE, "no HIP device");
+  int st = upload_directions();
+  if (st) return st;
+  const LgParams* p = &hp.p;
+  if (pool && !(pool->budget >= handle_bytes(hp)))
+    return fail(LG_ERR_MEMORY, "request larger than the server's device-memory budget");
+  char* blk = nullptr;
+  if (pool && !(blk = (char*)pool->alloc(handle_bytes(hp))))
+    return fail(LG_ERR_MEMORY, "request server: device-memory budget exhausted");
+  LgHandle* h = new LgHandle();
+  h->p = *p;
+  h->pool = pool;
+  h->blk = blk;
+  h->blk_size = blk ? handle_bytes(hp) : 0;
+  h->coeff_init_rho = p->coeff_to_init_rho;
+  h->coeff_init_dual = p->coeff_to_init_dual;
+  Args& a = h->a;
+  std::memset(&a, 0, sizeof(a));
+  a.x_size = hp.x_size; a.y_size = hp.y_size;
+  a.site_width = hp.site_width; a.row_height = hp.row_height;
+  a.core_lx = hp.core_lx; a.core_ly = hp.core_ly;
+  a.x_hint = p->x_hint; a.y_hint = p->y_hint;
+  a.ncol = hp.ncol; a.nrow = hp.nrow; a.K = hp.K;
+  a.N = hp.N;
+  const size_t P = hp.P;
+  h->P = P;
+  const int N = a.N;
+
+  auto bail = [&](int code) {
+    std::string msg = g_err;
+    lg_destroy(h);
+    g_err = msg;
+    return code;
+  };
+
+  if (stream) {
+    h->own_stream = stream;
+    h->owns_stream = false;
+  } else {
+    HIP_TRY(hipStreamCreateWithFlags(&h->own_stream, hipStreamNonBlocking));
+  }
   int16_t* d_pcode;
   int32_t *d_region, *d_vdd, *d_rx0, *d_ry0, *d_rx1, *d_ry1;
-  if ((st = upload(h, &d_pcode, pcode.data(), P))) return bail(st);
-  if ((st = upload(h, &d_region, region_rm.data(), P))) return bail(st);
-  if ((st = upload(h, &d_vdd, g->row_vdd_up, (size_t)g->y_size))) return bail(st);
-  int nr = std::max(g->num_regions, 0);
-  if ((st = upload(h, &d_rx0, g->region_x_min, nr))) return bail(st);
-  if ((st = upload(h, &d_ry0, g->region_y_min, nr))) return bail(st);
-  if ((st = upload(h, &d_rx1, g->region_x_max, nr))) return bail(st);
-  if ((st = upload(h, &d_ry1, g->region_y_max, nr))) return bail(st);
+  const size_t nr = hp.rx0.size();
+  if ((st = upload(h, &d_pcode, hp.pcode.data(), P))) return bail(st);
+  if ((st = upload(h, &d_region, hp.region_rm.data(), P))) return bail(st);
+  if ((st = upload(h, &d_vdd, hp.vdd.data(), (size_t)hp.y_size))) return bail(st);
+  if ((st = upload(h, &d_rx0, hp.rx0.data(), nr))) return bail(st);
+  if ((st = upload(h, &d_ry0, hp.ry0.data(), nr))) return bail(st);
+  if ((st = upload(h, &d_rx1, hp.rx1.data(), nr))) return bail(st);
+  if ((st = upload(h, &d_ry1, hp.ry1.data(), nr))) return bail(st);
   a.pcode = d_pcode; a.pregion = d_region; a.vdd_up = d_vdd;
   a.rxmin = d_rx0; a.rymin = d_ry0; a.rxmax = d_rx1; a.rymax = d_ry1;
   if ((st = halloc(h, &a.usage, P))) return bail(st);
   if ((st = halloc(h, &a.dual, P))) return bail(st);
 
   int32_t *d_w, *d_h, *d_code, *d_gnd, *d_ox, *d_oy, *d_rp, *d_khn;
-  if ((st = upload(h, &h->init_lx, lxg.data(), N))) return bail(st);
-  if ((st = upload(h, &h->init_ly, lyg.data(), N))) return bail(st);
+  if ((st = upload(h, &h->init_lx, hp.lxg.data(), N))) return bail(st);
+  if ((st = upload(h, &h->init_ly, hp.lyg.data(), N))) return bail(st);
   if ((st = halloc(h, &a.lx, N))) return bail(st);
   if ((st = halloc(h, &a.ly, N))) return bail(st);
-  if ((st = upload(h, &d_w, c->w_grid, N))) return bail(st);
-  if ((st = upload(h, &d_h, c->h_grid, N))) return bail(st);
-  if ((st = upload(h, &d_code, code.data(), N))) return bail(st);
-  if ((st = upload(h, &d_gnd, c->ground_at_bottom, N))) return bail(st);
-  if ((st = upload(h, &d_ox, c->orig_lx_dbu, N))) return bail(st);
-  if ((st = upload(h, &d_oy, c->orig_ly_dbu, N))) return bail(st);
-  if ((st = upload(h, &d_rp, rprio.data(), N))) return bail(st);
-  if ((st = upload(h, &d_khn, khn.data(), khn.size()))) return bail(st);
+  if ((st = upload(h, &d_w, hp.w.data(), N))) return bail(st);
+  if ((st = upload(h, &d_h, hp.h.data(), N))) return bail(st);
+  if ((st = upload(h, &d_code, hp.code.data(), N))) return bail(st);
+  if ((st = upload(h, &d_gnd, hp.gnd.data(), N))) return bail(st);
+  if ((st = upload(h, &d_ox, hp.ox.data(), N))) return bail(st);
+  if ((st = upload(h, &d_oy, hp.oy.data(), N))) return bail(st);
+  if ((st = upload(h, &d_rp, hp.rprio.data(), N))) return bail(st);
+  if ((st = upload(h, &d_khn, hp.khn.data(), hp.khn.size()))) return bail(st);
   a.w = d_w; a.h = d_h; a.code = d_code; a.gnd = d_gnd; a.ox = d_ox; a.oy = d_oy;
   a.rand_prio = d_rp; a.k_height_num = d_khn;
   if ((st = halloc(h, &a.need_rot, N))) return bail(st);
@@ -313,10 +500,8 @@ int lg_create(const LgGridDesc* g, const LgCellsSoA* c, const LgParams* p, LgHan
     while ((1 << bits) <= a.K) bits++;
     a.rx_passes = (bits + 6) / 7;
     a.rx_nb = (N + 4095) / 4096;
-    std::vector<int32_t> iota_v(std::max(N, 1));
-    std::iota(iota_v.begin(), iota_v.end(), 0);
     int32_t* d_ident;
-    if ((st = upload(h, &d_ident, iota_v.data(), iota_v.size()))) return bail(st);
+    if ((st = upload(h, &d_ident, hp.ident.data(), hp.ident.size()))) return bail(st);
     a.ident = d_ident;
     if ((st = halloc(h, &a.rx_k1, N)) || (st = halloc(h, &a.rx_v1, N)) || (st = halloc(h, &a.rx_k2, N)) ||
         (st = halloc(h, &a.rx_v2, N)) || (st = halloc(h, &a.rx_keys, N)) ||
@@ -329,24 +514,20 @@ int lg_create(const LgGridDesc* g, const LgCellsSoA* c, const LgParams* p, LgHan
   if ((st = halloc(h, &a.work, 2 * (size_t)a.K))) return bail(st);
   if ((st = halloc(h, &a.act_flags, 2 * (size_t)a.K))) return bail(st);
   {
-    // partition column / row of every x / y for the three fold offsets
-    // (paintPixelsKernel, AdmmKernels.hpp:164-191)
-    std::vector<int32_t> cm(3 * (size_t)g->x_size), rm(3 * (size_t)g->y_size);
-    for (int tf = 0; tf < 3; tf++) {
-      const int ox = p->x_hint / 3 * tf, oy = p->y_hint / 3 * tf;
-      for (int x = 0; x < g->x_size; x++) cm[tf * (size_t)g->x_size + x] = ((x + ox) / p->x_hint) % a.ncol;
-      for (int y = 0; y < g->y_size; y++) rm[tf * (size_t)g->y_size + y] = ((y + oy) / p->y_hint) % a.nrow;
-    }
     int32_t *d_cm, *d_rm;
-    if ((st = upload(h, &d_cm, cm.data(), cm.size()))) return bail(st);
-    if ((st = upload(h, &d_rm, rm.data(), rm.size()))) return bail(st);
+    if ((st = upload(h, &d_cm, hp.colmap.data(), hp.colmap.size()))) return bail(st);
+    if ((st = upload(h, &d_rm, hp.rowmap.data(), hp.rowmap.size()))) return bail(st);
     a.colmap = d_cm;
     a.rowmap = d_rm;
   }
   if ((st = halloc(h, &a.ovf_flag, N))) return bail(st);
   if ((st = halloc(h, &a.ovf_list, N))) return bail(st);
   if ((st = halloc(h, &a.trace, std::max(p->max_admm_iter, 1)))) return bail(st);
-  if ((st = halloc(h, &a.ctl, 1))) return bail(st);
+  if (ext_ctl) {
+    a.ctl = ext_ctl;
+  } else if ((st = halloc(h, &a.ctl, 1))) {
+    return bail(st);
+  }
   HIP_TRY(hipMemset(a.cnt, 0, (a.K + 1) * sizeof(int32_t)));
 
   a.max_iter = p->max_admm_iter;
@@ -379,13 +560,15 @@ int lg_create(const LgGridDesc* g, const LgCellsSoA* c, const LgParams* p, LgHan
   return LG_OK;
 }
 
-int lg_solve(LgHandle* h, void* stream) {
-  if (!h) return fail(LG_ERR_ARG, "null handle");
-  hipStream_t s = pick(h, stream);
+
+// lg_solve up to the first iteration (CudaAdmmLegalizer::solve,
+// CudaAdmmLegalizer.cu:250-281): control block, positions given at
+// lg_create, usage, rho / lambda0, cell records, initial duals and the
+// activity flags of iteration 0.  Asynchronous on `s`.
+int solve_begin(LgHandle* h, hipStream_t s) {
   Args& a = h->a;
   const int N = a.N;
   const int nb = (N + 255) / 256;
-  // CudaAdmmLegalizer::solve (CudaAdmmLegalizer.cu:250-281)
   Ctl init;
   std::memset(&init, 0, sizeof(init));
   init.min_ovf = INT_MAX;
@@ -399,6 +582,7 @@ int lg_solve(LgHandle* h, void* stream) {
     HIP_TRY(hipMemsetAsync(a.need_rot, 0, N * sizeof(int32_t), s));
   }
   h->posted = 0;
+  h->solved = 0;
   HIP_TRY(hipMemsetAsync(a.usage, 0, h->P * sizeof(int32_t), s));
   if (N > 0) {
     hipLaunchKernelGGL(k_usage_scatter, dim3(nb), dim3(256), 0, s, a);
@@ -419,54 +603,63 @@ int lg_solve(LgHandle* h, void* stream) {
 #endif
   hipLaunchKernelGGL(k_classify_init, pix_grid, dim3(256), 0, s, a);
   HIP_TRY(hipGetLastError());
+  return LG_OK;
+}
+
+// One ADMM iteration of one handle (updatePartition, executeSorting, primal
+// and dual updates, examineIteration); kernels read the iteration index and
+// the done flag from the device-resident Ctl.
+int launch_iteration(LgHandle* h, hipStream_t s) {
+  Args& a = h->a;
+  const int nb = (a.N + 255) / 256;
+  const dim3 pix_grid((a.x_size + 1023) / 1024, a.y_size);
+  // partition keys (+ first digit histogram), then a stable radix sort of
+  // (partition, id) into grouped
+  hipLaunchKernelGGL(k_assign, dim3(a.rx_nb), dim3(kRxThreads), 0, s, a);
+  const int32_t* kin = a.part_of;
+  const int32_t* vin = a.ident;
+  for (int pass = 0; pass < a.rx_passes; pass++) {
+    const bool last = pass == a.rx_passes - 1;
+    int32_t* kout = last ? a.rx_keys : (pass & 1 ? a.rx_k2 : a.rx_k1);
+    int32_t* vout = last ? a.grouped : (pass & 1 ? a.rx_v2 : a.rx_v1);
+    if (pass > 0) hipLaunchKernelGGL(k_rx_hist, dim3(a.rx_nb), dim3(kRxThreads), 0, s, a, kin, 7 * pass);
+    hipLaunchKernelGGL(k_rx_scan, dim3(1), dim3(1024), 0, s, a);
+    hipLaunchKernelGGL(k_rx_scatter, dim3(a.rx_nb), dim3(kRxThreads), 0, s, a, kin, vin, kout, vout, 7 * pass);
+    kin = kout;
+    vin = vout;
+  }
+  hipLaunchKernelGGL(k_part_bounds, dim3(nb), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, a);
+  if (a.use_adaptive) {
+    hipLaunchKernelGGL(k_extract_congested, dim3(nb), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_adaptive, dim3(1), dim3(1024), 0, s, a);
+  }
   int st;
-  if (N > 0 && h->p.max_admm_iter > 0) {
-    int batch = 8;
-    for (;;) {
-      for (int b = 0; b < batch; b++) {
-        // partition keys (+ first digit histogram), then a stable radix sort
-        // of (partition, id) into grouped
-        hipLaunchKernelGGL(k_assign, dim3(a.rx_nb), dim3(kRxThreads), 0, s, a);
-        const int32_t* kin = a.part_of;
-        const int32_t* vin = a.ident;
-        for (int pass = 0; pass < a.rx_passes; pass++) {
-          const bool last = pass == a.rx_passes - 1;
-          int32_t* kout = last ? a.rx_keys : (pass & 1 ? a.rx_k2 : a.rx_k1);
-          int32_t* vout = last ? a.grouped : (pass & 1 ? a.rx_v2 : a.rx_v1);
-          if (pass > 0) hipLaunchKernelGGL(k_rx_hist, dim3(a.rx_nb), dim3(kRxThreads), 0, s, a, kin, 7 * pass);
-          hipLaunchKernelGGL(k_rx_scan, dim3(1), dim3(1024), 0, s, a);
-          hipLaunchKernelGGL(k_rx_scatter, dim3(a.rx_nb), dim3(kRxThreads), 0, s, a, kin, vin, kout, vout, 7 * pass);
-          kin = kout;
-          vin = vout;
-        }
-        hipLaunchKernelGGL(k_part_bounds, dim3(nb), dim3(256), 0, s, a);
-        hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, a);
-        if (a.use_adaptive) {
-          hipLaunchKernelGGL(k_extract_congested, dim3(nb), dim3(256), 0, s, a);
-          hipLaunchKernelGGL(k_adaptive, dim3(1), dim3(1024), 0, s, a);
-        }
-        if ((st = launch_partition<MODE_PRIMAL>(h, s))) return st;
-        hipLaunchKernelGGL(k_dual_pass, pix_grid, dim3(256), 0, s, a);
-        hipLaunchKernelGGL(k_finish, dim3(1), dim3(1), 0, s, a);
-      }
-      HIP_TRY(hipGetLastError());
-      if ((st = read_ctl(h, s))) return st;
-      if (h->host_ctl.error) break;
-      if (h->host_ctl.done) break;
-      batch = std::min(batch * 2, 64);
-    }
+  if ((st = launch_partition<MODE_PRIMAL>(h, s))) return st;
+  hipLaunchKernelGGL(k_dual_pass, pix_grid, dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(1), 0, s, a);
+  return LG_OK;
+}
+
+// After the last iteration, with h->host_ctl holding the device Ctl: limit
+// errors, the final refine of a converged solve (doFinalRefine,
+// CudaAdmmLegalizer.cu:1011-1115), positions back to the SoA arrays and the
+// overflow trace.  Synchronises `s`.
+int solve_end(LgHandle* h, hipStream_t s) {
+  Args& a = h->a;
+  const int nb = (a.N + 255) / 256;
+  int st;
+  if (a.N > 0 && h->p.max_admm_iter > 0) {
     if (h->host_ctl.error == kErrPartitionTooDense)
       return fail(LG_ERR_STATE, "more than 4096 cells in one partition");
     if (h->host_ctl.error == kErrUsageRange)
       return fail(LG_ERR_STATE, "pixel usage above 65535");
-    if (h->host_ctl.converged) {  // doFinalRefine (CudaAdmmLegalizer.cu:1011-1115)
+    if (h->host_ctl.converged) {
       if ((st = launch_partition<MODE_REFINE>(h, s))) return st;
     }
     hipLaunchKernelGGL(k_unpack_pos, dim3(nb), dim3(256), 0, s, a);
-  } else {
-    HIP_TRY(hipMemsetAsync(a.ctl, 0, sizeof(int32_t) * 2, s));
+    if ((st = read_ctl(h, s))) return st;
   }
-  if ((st = read_ctl(h, s))) return st;
   h->trace_n = std::min(h->host_ctl.iter, h->p.max_admm_iter);
   h->trace.resize(std::max(h->trace_n, 1));
   if (h->trace_n > 0)
@@ -474,6 +667,44 @@ int lg_solve(LgHandle* h, void* stream) {
   h->solved = 1;
   h->checked = 0;
   return LG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* lg_last_error(void) { return g_err.c_str(); }
+
+int lg_create(const LgGridDesc* g, const LgCellsSoA* c, const LgParams* p, LgHandle** out) {
+  if (!out) return fail(LG_ERR_ARG, "null argument");
+  HostPrep hp;
+  int st = prepare_host(g, c, p, hp);
+  if (st) return st;
+  return create_from_prep(hp, nullptr, nullptr, nullptr, out);
+}
+
+int lg_solve(LgHandle* h, void* stream) {
+  if (!h) return fail(LG_ERR_ARG, "null handle");
+  hipStream_t s = pick(h, stream);
+  Args& a = h->a;
+  int st;
+  if ((st = solve_begin(h, s))) return st;
+  if (a.N > 0 && h->p.max_admm_iter > 0) {
+    int batch = 8;
+    for (;;) {
+      for (int b = 0; b < batch; b++)
+        if ((st = launch_iteration(h, s))) return st;
+      HIP_TRY(hipGetLastError());
+      if ((st = read_ctl(h, s))) return st;
+      if (h->host_ctl.error) break;
+      if (h->host_ctl.done) break;
+      batch = std::min(batch * 2, 64);
+    }
+  } else {
+    HIP_TRY(hipMemsetAsync(a.ctl, 0, sizeof(int32_t) * 2, s));
+    if ((st = read_ctl(h, s))) return st;
+  }
+  return solve_end(h, s);
 }
 
 int lg_postprocess(LgHandle* h, void* stream) {
@@ -616,8 +847,12 @@ int lg_debug_iter_times(LgHandle* h, int32_t* maxt, int32_t* sumt, int32_t* slow
 int lg_destroy(LgHandle* h) {
   if (!h) return LG_OK;
   for (auto& e : h->ev) (void)hipEventDestroy(e);
-  for (void* b : h->bufs) (void)hipFree(b);
-  if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
+  for (void* b : h->bufs) {
+    if (h->pool) h->pool->release(b);
+    else (void)hipFree(b);
+  }
+  if (h->pool && h->blk) h->pool->release(h->blk);
+  if (h->own_stream && h->owns_stream) (void)hipStreamDestroy(h->own_stream);
   delete h;
   return LG_OK;
 }
@@ -626,3 +861,6 @@ int lg_destroy(LgHandle* h) {
 
 // QP refine (post-process), same translation unit
 #include "lg_qp.inc.hip"
+
+// request server, same translation unit
+#include "lg_server.inc.hip"

@@ -289,7 +289,7 @@ __device__ __forceinline__ int cell_partition(const Args& a, const int* cm, cons
 }
 
 // partition keys plus the histogram of their first digit
-__global__ __launch_bounds__(kRxThreads) void k_assign(Args a) {
+__device__ __forceinline__ void assign_body(const Args& a, int bx) {
   const Ctl* ctl = a.ctl;
   if (ctl->done) return;
   __shared__ int cnt[128];
@@ -298,7 +298,7 @@ __global__ __launch_bounds__(kRxThreads) void k_assign(Args a) {
   const int tf = triple_fold(a, ctl->iter);
   const int* cm = a.colmap + tf * a.x_size;
   const int* rm = a.rowmap + tf * a.y_size;
-  const int base = blockIdx.x * kRxItems + threadIdx.x;
+  const int base = bx * kRxItems + threadIdx.x;
   const int4* rec = reinterpret_cast<const int4*>(a.cellrec);
   int4 r[kRxPer];
 #pragma unroll
@@ -316,16 +316,17 @@ __global__ __launch_bounds__(kRxThreads) void k_assign(Args a) {
     }
   }
   __syncthreads();
-  if (threadIdx.x < 128) a.rx_hist[threadIdx.x * a.rx_nb + blockIdx.x] = cnt[threadIdx.x];
+  if (threadIdx.x < 128) a.rx_hist[threadIdx.x * a.rx_nb + bx] = cnt[threadIdx.x];
 }
+__global__ __launch_bounds__(kRxThreads) void k_assign(Args a) { assign_body(a, blockIdx.x); }
 
 // digit histogram of one block (passes after the first)
-__global__ __launch_bounds__(kRxThreads) void k_rx_hist(Args a, const int32_t* keys, int shift) {
+__device__ __forceinline__ void rx_hist_body(const Args& a, int bx, const int32_t* keys, int shift) {
   if (a.ctl->done) return;
   __shared__ int cnt[128];
   if (threadIdx.x < 128) cnt[threadIdx.x] = 0;
   __syncthreads();
-  const int base = blockIdx.x * kRxItems + threadIdx.x;
+  const int base = bx * kRxItems + threadIdx.x;
   int k[kRxPer];
 #pragma unroll
   for (int q = 0; q < kRxPer; q++) k[q] = base + q * kRxThreads < a.N ? keys[base + q * kRxThreads] : -1;
@@ -333,8 +334,12 @@ __global__ __launch_bounds__(kRxThreads) void k_rx_hist(Args a, const int32_t* k
   for (int q = 0; q < kRxPer; q++)
     if (k[q] >= 0) atomicAdd(&cnt[(k[q] >> shift) & 127], 1);
   __syncthreads();
-  if (threadIdx.x < 128) a.rx_hist[threadIdx.x * a.rx_nb + blockIdx.x] = cnt[threadIdx.x];
+  if (threadIdx.x < 128) a.rx_hist[threadIdx.x * a.rx_nb + bx] = cnt[threadIdx.x];
 }
+__global__ __launch_bounds__(kRxThreads) void k_rx_hist(Args a, const int32_t* keys, int shift) {
+  rx_hist_body(a, blockIdx.x, keys, shift);
+}
+
 
 // Exclusive scan of n ints in place by one workgroup of 1024, in chunks of
 // 32768: all loads of a chunk are issued before the first use (one memory
@@ -388,17 +393,18 @@ __device__ __forceinline__ void block_scan_inplace(int* v, int n) {
   }
 }
 
-__global__ __launch_bounds__(1024) void k_rx_scan(Args a) {
+__device__ __forceinline__ void rx_scan_body(const Args& a) {
   if (a.ctl->done) return;
   block_scan_inplace(a.rx_hist, 128 * a.rx_nb);
 }
+__global__ __launch_bounds__(1024) void k_rx_scan(Args a) { rx_scan_body(a); }
 
 // Stable scatter of one block: wave w owns items [w*256, (w+1)*256), 64 at a
 // time, and a lane's rank among the lanes with the same digit comes from
 // seven ballots.  Items are first placed in digit order in LDS, then written
 // out in runs of consecutive slots.
-__global__ __launch_bounds__(kRxThreads) void k_rx_scatter(Args a, const int32_t* kin, const int32_t* vin,
-                                                           int32_t* kout, int32_t* vout, int shift) {
+__device__ __forceinline__ void rx_scatter_body(const Args& a, int bx, const int32_t* kin, const int32_t* vin,
+                                                int32_t* kout, int32_t* vout, int shift) {
   if (a.ctl->done) return;
   __shared__ int wcnt[kRxWaves][128];
   __shared__ int wpos[kRxWaves][128];
@@ -408,7 +414,7 @@ __global__ __launch_bounds__(kRxThreads) void k_rx_scatter(Args a, const int32_t
   for (int d = threadIdx.x; d < kRxWaves * 128; d += kRxThreads) wcnt[d >> 7][d & 127] = 0;
   __syncthreads();
   constexpr int CH = kRxItems / kRxWaves / kWave;  // chunks of 64 per wave
-  const int base = blockIdx.x * kRxItems + w * (kRxItems / kRxWaves);
+  const int base = bx * kRxItems + w * (kRxItems / kRxWaves);
   int key[CH], val[CH];
 #pragma unroll
   for (int c = 0; c < CH; c++) {
@@ -444,7 +450,7 @@ __global__ __launch_bounds__(kRxThreads) void k_rx_scatter(Args a, const int32_t
       run += wcnt[q][d];
     }
     dstart[d] = st;
-    gbase[d] = a.rx_hist[d * a.rx_nb + blockIdx.x];
+    gbase[d] = a.rx_hist[d * a.rx_nb + bx];
   }
   __syncthreads();
   const unsigned long long lt = (1ull << lane) - 1ull;
@@ -469,7 +475,7 @@ __global__ __launch_bounds__(kRxThreads) void k_rx_scatter(Args a, const int32_t
     if (valid && rank == __popcll(peers) - 1) wpos[w][d] = lpos + 1;
   }
   __syncthreads();
-  const int nitem = min(kRxItems, a.N - blockIdx.x * kRxItems);
+  const int nitem = min(kRxItems, a.N - bx * kRxItems);
   for (int j = threadIdx.x; j < nitem; j += kRxThreads) {
     const int k = lkey[j];
     const int d = (k >> shift) & 127;
@@ -478,11 +484,16 @@ __global__ __launch_bounds__(kRxThreads) void k_rx_scatter(Args a, const int32_t
     vout[g] = lval[j];
   }
 }
+__global__ __launch_bounds__(kRxThreads) void k_rx_scatter(Args a, const int32_t* kin, const int32_t* vin,
+                                                           int32_t* kout, int32_t* vout, int shift) {
+  rx_scatter_body(a, blockIdx.x, kin, vin, kout, vout, shift);
+}
+
 
 // first / one-past-last sorted slot of every non-empty partition
-__global__ void k_part_bounds(Args a) {
+__device__ __forceinline__ void part_bounds_body(const Args& a, int bx) {
   if (a.ctl->done) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = bx * blockDim.x + threadIdx.x;
   if (i >= a.N) return;
   const int* k = a.rx_keys;
   const int p = k[i];
@@ -490,6 +501,7 @@ __global__ void k_part_bounds(Args a) {
   if (i == 0 || k[i - 1] != p) a.pstart[p] = i;
   if (i == a.N - 1 || k[i + 1] != p) a.pend[p] = i + 1;
 }
+__global__ void k_part_bounds(Args a) { part_bounds_body(a, blockIdx.x); }
 
 // Exclusive scan of the partition histogram, max count -> npad, dual step
 // schedule (updateDualStepSize, CudaAdmmLegalizer.cu:293-294, 687-691), and
@@ -498,7 +510,7 @@ __global__ void k_part_bounds(Args a) {
 // listed partition's workgroup then starts in the first dispatch round
 // instead of behind thousands of workgroups that only find their partition
 // idle.  One workgroup of 1024 threads.
-__global__ __launch_bounds__(1024) void k_scan(Args a) {
+__device__ __forceinline__ void scan_body(const Args& a) {
   Ctl* ctl = a.ctl;
   if (ctl->done) return;
   __shared__ int s_max[16];
@@ -580,6 +592,7 @@ __global__ __launch_bounds__(1024) void k_scan(Args a) {
     }
   }
 }
+__global__ __launch_bounds__(1024) void k_scan(Args a) { scan_body(a); }
 
 
 // ---------------------------------------------------------------------------
@@ -1322,15 +1335,15 @@ __device__ __forceinline__ void flag_next(const Args& a, int iter, int x, int y)
 // npad_lo..npad_hi selects the launch (LDS size class); LDS_SORT launches keep
 // the sort buffers in LDS, the others sort in registers (npad <= 1024).
 template <int MODE, int LDS_SORT>
-__global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_hi) {
+__device__ __forceinline__ void partition_body(const Args& a, int bx, int npad_lo, int npad_hi) {
   Ctl* ctl = a.ctl;
   if (MODE == MODE_PRIMAL && ctl->done) return;
   // the primal step takes its partitions from the work list of its class;
   // the final refine visits every partition
   const int cls = npad_lo > 0 ? 1 : 0;
-  if (MODE == MODE_PRIMAL && (int)blockIdx.x >= ctl->n_work[cls]) return;
+  if (MODE == MODE_PRIMAL && (int)bx >= ctl->n_work[cls]) return;
   if (MODE != MODE_PRIMAL && ctl->npad < npad_lo) return;  // no partition of this size class
-  const int k = MODE == MODE_PRIMAL ? a.work[cls * a.K + blockIdx.x] : (int)blockIdx.x;
+  const int k = MODE == MODE_PRIMAL ? a.work[cls * a.K + bx] : (int)bx;
   const bool adaptive_now = (MODE == MODE_PRIMAL) && a.use_adaptive && ctl->exceptional;
   const int iter = MODE == MODE_PRIMAL ? ctl->iter : ctl->conv_iter;
   const int off = a.off[k];
@@ -1629,6 +1642,11 @@ __global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_
   SUB_FLUSH();
   PHASE_END();
 }
+template <int MODE, int LDS_SORT>
+__global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_hi) {
+  partition_body<MODE, LDS_SORT>(a, blockIdx.x, npad_lo, npad_hi);
+}
+
 
 // Dual update, after the primal step, of the pixels of partitions that took
 // no step (dual_update_functor_with_step_size, AdmmKernels.hpp:26-42).  Runs
@@ -1636,14 +1654,14 @@ __global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_
 // increment, so its dual is read only for other usages.  Four consecutive
 // pixels of a row per thread, rows in blockIdx.y.  The activity flags of the
 // next iteration are set by the primal step itself.
-__global__ __launch_bounds__(256) void k_dual_pass(Args a) {
+__device__ __forceinline__ void dual_pass_body(const Args& a, int bx, int by) {
   const Ctl* ctl = a.ctl;
   if (ctl->done) return;
   const int iter = ctl->iter;
   const bool adaptive_now = a.use_adaptive && ctl->exceptional;
   if (iter < a.iter_perturbation_on || ctl->wrong_row != 0 || adaptive_now) return;
-  const int y = blockIdx.y;
-  const int x0 = 4 * (blockIdx.x * blockDim.x + threadIdx.x);
+  const int y = by;
+  const int x0 = 4 * (bx * blockDim.x + threadIdx.x);
   if (x0 >= a.x_size) return;
   const unsigned p0 = pidx(a, x0, y);
   const bool full = x0 + 4 <= a.x_size && (a.x_size & 3) == 0;
@@ -1681,6 +1699,7 @@ __global__ __launch_bounds__(256) void k_dual_pass(Args a) {
       if (upd[q]) a.dual[p0 + q] = dual_step(u[q], a.dual[p0 + q], rho, step, neg);
   }
 }
+__global__ __launch_bounds__(256) void k_dual_pass(Args a) { dual_pass_body(a, blockIdx.x, blockIdx.y); }
 
 // Before the first iteration: the activity flags of iteration 0 (any pixel
 // with usage > 1 under its fold offset).  Four pixels per thread.
@@ -1722,7 +1741,7 @@ __global__ void k_unpack_pos(Args a) {
 
 // examineIteration + detectSlowConvergence + convergence test
 // (CudaAdmmLegalizer.cu:305-318, 799-840, 995-1009).  Single thread.
-__global__ void k_finish(Args a) {
+__device__ __forceinline__ void finish_body(const Args& a) {
   Ctl* ctl = a.ctl;
   if (ctl->done) return;
   int iter = ctl->iter;
@@ -1764,15 +1783,16 @@ __global__ void k_finish(Args a) {
   ctl->iter = iter + 1;
   if (ctl->iter >= a.max_iter) ctl->done = 1;
 }
+__global__ void k_finish(Args a) { finish_body(a); }
 
 // ---------------------------------------------------------------------------
 // adaptive search: extractCongestedCellsKernel + projectionAwareSearchKernel
 // (AdmmKernels.hpp:947-1174), a single 1024-thread workgroup on global memory
 // ---------------------------------------------------------------------------
-__global__ void k_extract_congested(Args a) {
+__device__ __forceinline__ void extract_congested_body(const Args& a, int bx) {
   Ctl* ctl = a.ctl;
   if (ctl->done || !a.use_adaptive || !ctl->exceptional) return;
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  int c = bx * blockDim.x + threadIdx.x;
   if (c >= a.N) return;
   int p0 = a.cellrec[4 * c];
   int cx = p0 & 0xffff, cy = (int)((unsigned)p0 >> 16);
@@ -1781,6 +1801,7 @@ __global__ void k_extract_congested(Args a) {
     for (int y = cy; y < cy + a.h[c]; y++) s += max(a.usage[pidx(a, x, y)] - 1, 0);
   a.ovf_flag[c] = s == 0 ? 0 : 1;
 }
+__global__ void k_extract_congested(Args a) { extract_congested_body(a, blockIdx.x); }
 
 __device__ void proj_pos(const Args& a, int d, int w, int h, int cx, int cy, int& nx, int& ny) {
   int2 dd = c_dir_wide[d];
@@ -1803,7 +1824,7 @@ __device__ void proj_pos(const Args& a, int d, int w, int h, int cx, int cy, int
   ny = max(0, min(a.y_size - h, ny));
 }
 
-__global__ __launch_bounds__(1024) void k_adaptive(Args a) {
+__device__ __forceinline__ void adaptive_body(const Args& a) {
   Ctl* ctl = a.ctl;
   if (ctl->done || !a.use_adaptive || !ctl->exceptional) return;
   __shared__ float s_cost[kNumDirWidePadded];
@@ -1894,6 +1915,7 @@ __global__ __launch_bounds__(1024) void k_adaptive(Args a) {
   }
   if (t == 0) ctl->adaptive_calls++;
 }
+__global__ __launch_bounds__(1024) void k_adaptive(Args a) { adaptive_body(a); }
 
 // ---------------------------------------------------------------------------
 // post-process and checks
@@ -1945,5 +1967,114 @@ __global__ void k_cell_stats(Args a) {
     atomicMax(&a.ctl->disp_max, dmax);
   }
 }
+
+// ---------------------------------------------------------------------------
+// Batched launches of the request server.  One launch runs one step of an
+// ADMM iteration for every request slot on the device: `tab` holds the
+// slots' Args and block b of the launch does the work of block map[b].y of
+// slot map[b].x's own (solo) launch, through the same body as the solo
+// kernel, so a request computes exactly what it computes alone.  Every body
+// returns at once for a slot whose Ctl says done, so a slot that finishes in
+// the middle of a batch of iterations is left as it is until the host
+// retires it.
+// ---------------------------------------------------------------------------
+#define LG_BATCH_SLOT()                                                        \
+  const int2 m_ = map[blockIdx.x];                                             \
+  const Args& a = tab[m_.x]
+
+// input / output buffers of radix pass `pass` (the solo loop's ping-pong)
+__device__ __forceinline__ void rx_pass_ptrs(const Args& a, int pass, const int32_t*& kin, const int32_t*& vin,
+                                             int32_t*& kout, int32_t*& vout) {
+  const bool last = pass == a.rx_passes - 1;
+  kout = last ? a.rx_keys : (pass & 1 ? a.rx_k2 : a.rx_k1);
+  vout = last ? a.grouped : (pass & 1 ? a.rx_v2 : a.rx_v1);
+  if (pass == 0) {
+    kin = a.part_of;
+    vin = a.ident;
+  } else {
+    kin = pass & 1 ? a.rx_k1 : a.rx_k2;
+    vin = pass & 1 ? a.rx_v1 : a.rx_v2;
+  }
+}
+
+__global__ __launch_bounds__(kRxThreads) void kb_assign(const Args* __restrict__ tab, const int2* __restrict__ map) {
+  LG_BATCH_SLOT();
+  assign_body(a, m_.y);
+}
+
+__global__ __launch_bounds__(kRxThreads) void kb_rx_hist(const Args* __restrict__ tab, const int2* __restrict__ map,
+                                                         int pass) {
+  LG_BATCH_SLOT();
+  if (pass >= a.rx_passes) return;
+  const int32_t *kin, *vin;
+  int32_t *kout, *vout;
+  rx_pass_ptrs(a, pass, kin, vin, kout, vout);
+  rx_hist_body(a, m_.y, kin, 7 * pass);
+}
+
+__global__ __launch_bounds__(1024) void kb_rx_scan(const Args* __restrict__ tab, const int2* __restrict__ map,
+                                                   int pass) {
+  LG_BATCH_SLOT();
+  if (pass >= a.rx_passes) return;
+  rx_scan_body(a);
+}
+
+__global__ __launch_bounds__(kRxThreads) void kb_rx_scatter(const Args* __restrict__ tab,
+                                                            const int2* __restrict__ map, int pass) {
+  LG_BATCH_SLOT();
+  if (pass >= a.rx_passes) return;
+  const int32_t *kin, *vin;
+  int32_t *kout, *vout;
+  rx_pass_ptrs(a, pass, kin, vin, kout, vout);
+  rx_scatter_body(a, m_.y, kin, vin, kout, vout, 7 * pass);
+}
+
+__global__ void kb_part_bounds(const Args* __restrict__ tab, const int2* __restrict__ map) {
+  LG_BATCH_SLOT();
+  part_bounds_body(a, m_.y);
+}
+
+__global__ __launch_bounds__(1024) void kb_scan(const Args* __restrict__ tab, const int2* __restrict__ map) {
+  LG_BATCH_SLOT();
+  scan_body(a);
+}
+
+__global__ void kb_extract_congested(const Args* __restrict__ tab, const int2* __restrict__ map) {
+  LG_BATCH_SLOT();
+  if (!a.use_adaptive) return;
+  extract_congested_body(a, m_.y);
+}
+
+__global__ __launch_bounds__(1024) void kb_adaptive(const Args* __restrict__ tab, const int2* __restrict__ map) {
+  LG_BATCH_SLOT();
+  if (!a.use_adaptive) return;
+  adaptive_body(a);
+}
+
+// reg_filter: 1 = only slots whose sorting algorithm is a bitonic network
+// (sorted in registers up to kNpadSmall slots), 0 = only the others, -1 = all
+template <int MODE, int LDS_SORT>
+__global__ __launch_bounds__(64) void kb_partition(const Args* __restrict__ tab, const int2* __restrict__ map,
+                                                   int npad_lo, int npad_hi, int reg_filter) {
+  LG_BATCH_SLOT();
+  const int reg = (a.sort_algo == LG_SORT_M_BITONIC || a.sort_algo == LG_SORT_S_BITONIC) ? 1 : 0;
+  if (reg_filter >= 0 && reg != reg_filter) return;
+  partition_body<MODE, LDS_SORT>(a, m_.y, npad_lo, npad_hi);
+}
+
+// local block l of the solo launch's (ceil(x_size / 1024), y_size) grid
+__global__ __launch_bounds__(256) void kb_dual_pass(const Args* __restrict__ tab, const int2* __restrict__ map) {
+  LG_BATCH_SLOT();
+  const int gx = (a.x_size + 1023) / 1024;
+  const int by = m_.y / gx;
+  dual_pass_body(a, m_.y - by * gx, by);
+}
+
+__global__ void kb_finish(const Args* __restrict__ tab, const int2* __restrict__ map) {
+  LG_BATCH_SLOT();
+  finish_body(a);
+}
+
+#undef LG_BATCH_SLOT
 
 }  // namespace lgdev

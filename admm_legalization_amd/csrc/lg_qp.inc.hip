@@ -356,12 +356,12 @@ int lg_qp_refine(LgHandle* h, hipStream_t s) {
   q.min_iter = h->p.qp_admm_min_iter;
   std::vector<void*> tmp;
   auto talloc = [&](auto** p, size_t n) -> int {
-    int st = dalloc(p, n);
+    int st = halloc_tmp(h, p, n);
     if (st == LG_OK) tmp.push_back((void*)*p);
     return st;
   };
   auto cleanup = [&](int code) {
-    for (void* b : tmp) (void)hipFree(b);
+    for (void* b : tmp) hfree_tmp(h, b);
     return code;
   };
   const size_t P = h->P;

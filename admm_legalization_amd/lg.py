@@ -28,7 +28,9 @@ _i32p = C.POINTER(C.c_int32)
 EXPORTED = ["lg_default_params", "lg_last_error", "lg_run_design", "lg_import_and_preplace",
             "lg_create", "lg_solve", "lg_postprocess", "lg_check", "lg_copy_out", "lg_get_stats",
             "lg_get_pixel_state", "lg_get_ovf_trace", "lg_destroy", "lg_profile_partition",
-            "lg_profile_read", "lg_debug_phase_ticks"]
+            "lg_profile_read", "lg_debug_phase_ticks", "lg_srv_default_config", "lg_srv_create",
+            "lg_srv_submit", "lg_srv_poll", "lg_srv_wait", "lg_srv_counters", "lg_srv_profile",
+            "lg_srv_profile_read", "lg_srv_destroy"]
 
 
 class LegalizerError(RuntimeError):
@@ -168,35 +170,43 @@ def device_inputs(design: Design, params: LgParams | None = None):
     return grid, cells, p2
 
 
+def marshal_device_inputs(grid: dict, cells: dict):
+    """LgGridDesc / LgCellsSoA over the arrays of `grid` and `cells` (dicts of
+    numpy arrays with the fields of the two structs); the third value keeps
+    the int32 copies alive for as long as the structs are used."""
+    keep = []
+
+    def arr(a):
+        a = np.ascontiguousarray(a, np.int32)
+        keep.append(a)
+        return _p(a)
+
+    g = LgGridDesc()
+    for k in ("x_size", "y_size", "site_width", "row_height", "core_lx", "core_ly"):
+        setattr(g, k, int(grid[k]))
+    g.pixel_valid = arr(grid["valid"])
+    g.pixel_group = arr(grid["group"])
+    g.pixel_region = arr(grid.get("region", np.full(int(grid["x_size"]) * int(grid["y_size"]), -1)))
+    g.row_vdd_up = arr(grid["vdd_up"])
+    nr = len(grid.get("region_x_min", []))
+    g.num_regions = nr
+    for k in ("region_x_min", "region_y_min", "region_x_max", "region_y_max"):
+        setattr(g, k, arr(grid.get(k, np.zeros(0, np.int32))))
+    c = LgCellsSoA()
+    c.num_cells = int(len(cells["w_grid"]))
+    for k in ("orig_lx_dbu", "orig_ly_dbu", "lx_dbu", "ly_dbu", "w_grid", "h_grid", "group",
+              "ground_at_bottom"):
+        setattr(c, k, arr(cells[k]))
+    return g, c, keep
+
+
 class DeviceLegalizer:
     """The device layer on explicit arrays (CudaDatabase + CudaAdmmLegalizer +
     CudaPostProcess of the reference).  `grid` and `cells` are dicts of numpy
     arrays with the fields of LgGridDesc / LgCellsSoA."""
 
     def __init__(self, grid: dict, cells: dict, params: LgParams):
-        self._keep = []
-
-        def arr(a):
-            a = np.ascontiguousarray(a, np.int32)
-            self._keep.append(a)
-            return _p(a)
-
-        g = LgGridDesc()
-        for k in ("x_size", "y_size", "site_width", "row_height", "core_lx", "core_ly"):
-            setattr(g, k, int(grid[k]))
-        g.pixel_valid = arr(grid["valid"])
-        g.pixel_group = arr(grid["group"])
-        g.pixel_region = arr(grid.get("region", np.full(int(grid["x_size"]) * int(grid["y_size"]), -1)))
-        g.row_vdd_up = arr(grid["vdd_up"])
-        nr = len(grid.get("region_x_min", []))
-        g.num_regions = nr
-        for k in ("region_x_min", "region_y_min", "region_x_max", "region_y_max"):
-            setattr(g, k, arr(grid.get(k, np.zeros(0, np.int32))))
-        c = LgCellsSoA()
-        c.num_cells = int(len(cells["w_grid"]))
-        for k in ("orig_lx_dbu", "orig_ly_dbu", "lx_dbu", "ly_dbu", "w_grid", "h_grid", "group",
-                  "ground_at_bottom"):
-            setattr(c, k, arr(cells[k]))
+        g, c, self._keep = marshal_device_inputs(grid, cells)
         self.num_cells = c.num_cells
         self.P = int(grid["x_size"]) * int(grid["y_size"])
         self.params = params

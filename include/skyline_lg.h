@@ -248,6 +248,80 @@ int lg_profile_read(LgHandle* h, double* ms /*[2]*/, int64_t* launches /*[2]*/);
  * otherwise); no counterpart in the reference. */
 int lg_debug_phase_ticks(LgHandle* h, uint64_t* ticks, int n);
 
+/* ---- request server (no counterpart in the reference) ------------------
+ *
+ * A continuous-batching legalization service on one GPU.  Requests (whole
+ * designs, ECO windows, sub-designs: the device-level inputs of lg_create)
+ * are submitted from any thread and queued in arrival order.  A server
+ * thread admits the head of the queue whenever a slot is free and its device
+ * buffers fit the memory budget, keeps each admitted request's cell arrays,
+ * pixel grid and dual prices resident until it converges, and runs one ADMM
+ * iteration of every resident request per batched kernel launch, so requests
+ * that arrive while others run share the device with them.  Each request
+ * computes exactly what lg_create + lg_solve + lg_postprocess compute for it
+ * alone (same kernels, same order of operations, per-request rho / lambda /
+ * iteration control), so its result does not depend on what it is batched
+ * with.  Finished requests are retired between batches (final refine,
+ * rotation, optional QP refine, checkPlace, positions copied to the host). */
+
+enum { LG_SRV_QUEUED = 0, LG_SRV_RUNNING = 1, LG_SRV_DONE = 2, LG_SRV_FAILED = 3 };
+
+typedef struct LgSrvConfig {
+  int32_t max_slots;          /* requests resident on the device at once (1..1024)   */
+  int32_t iters_per_sync;     /* ADMM iterations launched between host checks (1..1024) */
+  int64_t mem_budget_bytes;   /* device memory the server may hold for requests     */
+  int64_t pool_chunk_bytes;   /* granularity in which that memory is taken          */
+} LgSrvConfig;
+
+typedef struct LgSrvResult {
+  int32_t* out_lx;            /* [num_cells] grid x (sites), or NULL                */
+  int32_t* out_ly;            /* [num_cells] grid y (rows), or NULL                 */
+  int32_t* out_rot;           /* [num_cells] need_rotation, or NULL                 */
+  int32_t  num_cells;         /* cells of the request (written)                     */
+  int32_t  status;            /* LG_OK or the request's error code (written)        */
+  LgStats  stats;             /* as lg_get_stats after lg_check (written)           */
+  double   queued_s;          /* submit -> admitted to a slot                       */
+  double   run_s;             /* admitted -> result on the host                     */
+  double   latency_s;         /* submit -> result on the host                       */
+} LgSrvResult;
+
+typedef struct LgSrvCounters {
+  int64_t submitted, completed, failed;
+  int64_t batches;            /* host checks of the batched loop                    */
+  int64_t iterations;         /* batched ADMM iterations launched                   */
+  int32_t peak_slots;         /* most requests resident at once                     */
+  int32_t active, queued;
+  int64_t pool_reserved_bytes, pool_peak_bytes;
+} LgSrvCounters;
+
+typedef struct LgServer LgServer;
+
+/* max_slots 64, iters_per_sync 16, budget 200 GB, chunks of 256 MiB. */
+void lg_srv_default_config(LgSrvConfig* c);
+/* Starts the server thread on the calling thread's current device. */
+int lg_srv_create(const LgSrvConfig* c, LgServer** out);
+/* Validates and prepares the request on the calling thread (the host work of
+ * lg_create), queues it and returns its ticket; the inputs may be freed on
+ * return.  LG_ERR_MEMORY if the request alone exceeds the budget. */
+int lg_srv_submit(LgServer* s, const LgGridDesc* g, const LgCellsSoA* c, const LgParams* p, int64_t* ticket);
+/* LG_SRV_* state of a ticket that has not been collected by lg_srv_wait. */
+int lg_srv_poll(LgServer* s, int64_t ticket, int32_t* state);
+/* Blocks until the request is done, fills `r` and releases the ticket.
+ * Returns the request's status (its message through lg_last_error). */
+int lg_srv_wait(LgServer* s, int64_t ticket, LgSrvResult* r);
+int lg_srv_counters(LgServer* s, LgSrvCounters* out);
+/* Instrumentation: with enable != 0, every later batched iteration brackets
+ * its partition-kernel launches (the primal step of all resident requests)
+ * with HIP events on the server's stream.  lg_srv_profile_read returns the
+ * summed milliseconds, the number of bracketed iterations and the
+ * algorithmic bytes of the iterations the requests executed in them
+ * (12 per pixel + 24 per cell and iteration) since the last enable. */
+int lg_srv_profile(LgServer* s, int enable);
+int lg_srv_profile_read(LgServer* s, double* ms, int64_t* iterations, double* bytes);
+/* Stops the server: queued and running requests are dropped (their waiters
+ * get LG_ERR_STATE), device memory is released. */
+int lg_srv_destroy(LgServer* s);
+
 #ifdef __cplusplus
 }
 #endif

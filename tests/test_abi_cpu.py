@@ -98,3 +98,21 @@ int main() {
                     "-lskyline_amd", f"-Wl,-rpath,{libdir}", "-o", str(exe)], check=True)
     out = subprocess.run([str(exe)], stdout=subprocess.PIPE, text=True, check=True).stdout.split()
     assert out[0] == "250" and int(out[1]) >= 4 and out[1] == out[2]
+
+
+def test_server_rejects_bad_arguments_without_a_device(library):
+    from admm_legalization_amd import server
+    L = server._bind()
+    c = server.default_config()
+    assert (c.max_slots, c.iters_per_sync) == (64, 16)
+    assert c.mem_budget_bytes == 200 * 10 ** 9 and c.pool_chunk_bytes == 256 << 20
+    h = C.c_void_p()
+    assert L.lg_srv_create(None, C.byref(h)) == 1  # LG_ERR_ARG
+    for bad in (server.LgSrvConfig(0, 16, 1 << 30, 1 << 20), server.LgSrvConfig(8, 0, 1 << 30, 1 << 20),
+                server.LgSrvConfig(8, 16, 0, 1 << 20), server.LgSrvConfig(2000, 16, 1 << 30, 1 << 20)):
+        assert L.lg_srv_create(C.byref(bad), C.byref(h)) == 1
+        assert b"configuration" in L.lg_last_error()
+    assert L.lg_srv_submit(None, None, None, None, None) == 1
+    assert L.lg_srv_wait(None, 1, None) == 1
+    assert L.lg_srv_poll(None, 1, None) == 1
+    assert L.lg_srv_destroy(None) == 0
