@@ -30,6 +30,7 @@ UNITS = [
     ("lg_host.cpp", "cxx"),
     ("dens_device.hip", "hip"),
     ("dens_host.cpp", "cxx"),
+    ("wl_device.hip", "hip"),
 ]
 
 

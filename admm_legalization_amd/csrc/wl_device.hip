@@ -1,0 +1,407 @@
+// Weighted-average wirelength gradient and the target function's
+// preconditioned total gradient for gfx950: C ABI of include/skyline_wl.h.
+// Replaces skyplace::WireLengthGradient (wirelength_gradient/WireLengthGradient.cu)
+// and jacobianPrecondition (problem_instance/ProblemInstanceKernel.hpp:74-100).
+//
+// Layout in HBM (one handle):
+//   nets   net_start[num_nets + 1] int32, net_w[num_nets] float
+//   pins   pin2cell int32, pin_off float2 (x, y), pin_xy float2 (current
+//          centres), pin_g float2 (the pin's gradient term of the last
+//          evaluation); pins numbered net by net
+//   cells  cell_start[num_cells + 1], cell_pin[] int32: the pins of every
+//          movable cell in ascending pin order (built on the host once)
+//   big    ids of the nets above kSmallDeg pins
+//
+// One evaluation = k_wl_net (a thread per net of at most kSmallDeg pins) +
+// k_wl_net_big (a workgroup per larger net) writing pin_g, then k_wl_cell (a
+// thread per cell) summing its pins' terms.  The reference scatters the pin
+// terms with float atomicAdd (addPinGrad, WireLengthGradient.cu:162-185); the
+// per-cell gather here gives the same sums in a fixed order, so repeated
+// evaluations are bit-identical.  HBM traffic per evaluation, with the pin
+// update fused: pin2cell 4 + offsets 8 + cell centre gather 8 + pin_xy
+// write 8 + pin_g write 8 + per-cell index 4 + pin_g read 8 = 48 B per pin,
+// 8 B per net (start, weight), 8 B per cell (two outputs) + 4 (CSR start).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/skyline_wl.h"
+
+namespace {
+
+constexpr int kSmallDeg = 32;
+constexpr int kBigThreads = 256;
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define WL_TRY(expr)                                                               \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      return fail(WL_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+struct WlArgs {
+  int num_cells, num_nets, num_pins, num_big;
+  const int* net_start;
+  const float* net_w;
+  const int* pin2cell;
+  const float2* pin_off;
+  float2* pin_xy;
+  float2* pin_g;
+  const int* cell_start;
+  const int* cell_pin;
+  const int* big;
+  double* hpwl;  // [1] accumulator of wl_compute_hpwl
+};
+
+// Pin centre: from its movable cell's centre (FROM_CELLS), else the stored one
+// (updatePinCoordinateKernel, WireLengthGradient.cu:187-211).
+template <bool FROM_CELLS>
+__device__ __forceinline__ float2 pin_at(const WlArgs& a, int p, const float* cx, const float* cy) {
+  if (FROM_CELLS) {
+    const int c = a.pin2cell[p];
+    if (c >= 0) {
+      const float2 o = a.pin_off[p];
+      const float2 v = make_float2(cx[c] + o.x, cy[c] + o.y);
+      a.pin_xy[p] = v;
+      return v;
+    }
+  }
+  return a.pin_xy[p];
+}
+
+// computePinGrad (WireLengthGradient.cu:108-160), one axis.
+__device__ __forceinline__ float pin_term(float x, float g_inv, float ap, float am, float bp, float bm, float cp,
+                                          float cm) {
+  const float m = ((1.0f - x * g_inv) * bm + g_inv * cm) * am / bm / bm;
+  const float q = ((1.0f + x * g_inv) * bp - g_inv * cp) * ap / bp / bp;
+  return m - q;
+}
+
+// A thread per net of at most kSmallDeg pins: computeMinMax (:14-45),
+// computeABCKernel (:47-106) and computePinGrad (:108-160) for its pins.
+template <bool FROM_CELLS>
+__global__ __launch_bounds__(256) void k_wl_net(WlArgs a, float g_inv, const float* cx, const float* cy) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= a.num_nets) return;
+  const int s = a.net_start[n], e = a.net_start[n + 1];
+  if (e - s > kSmallDeg) return;
+  float mxx = 0.0f, mxy = 0.0f, mnx = FLT_MAX, mny = FLT_MAX;
+  for (int p = s; p < e; p++) {
+    const float2 v = pin_at<FROM_CELLS>(a, p, cx, cy);
+    mxx = fmaxf(v.x, mxx);
+    mnx = fminf(v.x, mnx);
+    mxy = fmaxf(v.y, mxy);
+    mny = fminf(v.y, mny);
+  }
+  float bpx = 0, bpy = 0, bmx = 0, bmy = 0, cpx = 0, cpy = 0, cmx = 0, cmy = 0;
+  for (int p = s; p < e; p++) {
+    const float2 v = a.pin_xy[p];
+    const float apx = expf((v.x - mxx) * g_inv), apy = expf((v.y - mxy) * g_inv);
+    const float amx = expf(-(v.x - mnx) * g_inv), amy = expf(-(v.y - mny) * g_inv);
+    bpx += apx; bpy += apy; bmx += amx; bmy += amy;
+    cpx += v.x * apx; cpy += v.y * apy; cmx += v.x * amx; cmy += v.y * amy;
+  }
+  const float w = a.net_w[n];
+  for (int p = s; p < e; p++) {
+    const float2 v = a.pin_xy[p];
+    const float apx = expf((v.x - mxx) * g_inv), apy = expf((v.y - mxy) * g_inv);
+    const float amx = expf(-(v.x - mnx) * g_inv), amy = expf(-(v.y - mny) * g_inv);
+    float gx = pin_term(v.x, g_inv, apx, amx, bpx, bmx, cpx, cmx);
+    float gy = pin_term(v.y, g_inv, apy, amy, bpy, bmy, cpy, cmy);
+    a.pin_g[p] = make_float2(gx * w, gy * w);
+  }
+}
+
+template <typename T, typename Op>
+__device__ __forceinline__ T block_reduce(T v, Op op, T* sh) {
+  for (int o = 32; o > 0; o >>= 1) v = op(v, __shfl_xor(v, o));
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __syncthreads();
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  T r = sh[0];
+  for (int i = 1; i < (int)(blockDim.x >> 6); i++) r = op(r, sh[i]);
+  return r;
+}
+
+// A workgroup per net of more than kSmallDeg pins (same arithmetic; the
+// per-net sums are reduced across the workgroup).
+template <bool FROM_CELLS>
+__global__ __launch_bounds__(kBigThreads) void k_wl_net_big(WlArgs a, float g_inv, const float* cx,
+                                                            const float* cy) {
+  __shared__ float sh[kBigThreads / 64];
+  const int n = a.big[blockIdx.x];
+  const int s = a.net_start[n], e = a.net_start[n + 1];
+  float mxx = 0.0f, mxy = 0.0f, mnx = FLT_MAX, mny = FLT_MAX;
+  for (int p = s + threadIdx.x; p < e; p += blockDim.x) {
+    const float2 v = pin_at<FROM_CELLS>(a, p, cx, cy);
+    mxx = fmaxf(v.x, mxx);
+    mnx = fminf(v.x, mnx);
+    mxy = fmaxf(v.y, mxy);
+    mny = fminf(v.y, mny);
+  }
+  auto fmx = [](float x, float y) { return fmaxf(x, y); };
+  auto fmn = [](float x, float y) { return fminf(x, y); };
+  auto add = [](float x, float y) { return x + y; };
+  mxx = block_reduce(mxx, fmx, sh);
+  mnx = block_reduce(mnx, fmn, sh);
+  mxy = block_reduce(mxy, fmx, sh);
+  mny = block_reduce(mny, fmn, sh);
+  float bpx = 0, bpy = 0, bmx = 0, bmy = 0, cpx = 0, cpy = 0, cmx = 0, cmy = 0;
+  for (int p = s + threadIdx.x; p < e; p += blockDim.x) {
+    const float2 v = a.pin_xy[p];
+    const float apx = expf((v.x - mxx) * g_inv), apy = expf((v.y - mxy) * g_inv);
+    const float amx = expf(-(v.x - mnx) * g_inv), amy = expf(-(v.y - mny) * g_inv);
+    bpx += apx; bpy += apy; bmx += amx; bmy += amy;
+    cpx += v.x * apx; cpy += v.y * apy; cmx += v.x * amx; cmy += v.y * amy;
+  }
+  bpx = block_reduce(bpx, add, sh); bpy = block_reduce(bpy, add, sh);
+  bmx = block_reduce(bmx, add, sh); bmy = block_reduce(bmy, add, sh);
+  cpx = block_reduce(cpx, add, sh); cpy = block_reduce(cpy, add, sh);
+  cmx = block_reduce(cmx, add, sh); cmy = block_reduce(cmy, add, sh);
+  const float w = a.net_w[n];
+  for (int p = s + threadIdx.x; p < e; p += blockDim.x) {
+    const float2 v = a.pin_xy[p];
+    const float apx = expf((v.x - mxx) * g_inv), apy = expf((v.y - mxy) * g_inv);
+    const float amx = expf(-(v.x - mnx) * g_inv), amy = expf(-(v.y - mny) * g_inv);
+    float gx = pin_term(v.x, g_inv, apx, amx, bpx, bmx, cpx, cmx);
+    float gy = pin_term(v.y, g_inv, apy, amy, bpy, bmy, cpy, cmy);
+    a.pin_g[p] = make_float2(gx * w, gy * w);
+  }
+}
+
+// addPinGrad (WireLengthGradient.cu:162-185) as a gather: a thread per cell
+// sums the terms of its pins in ascending pin order.
+__global__ __launch_bounds__(256) void k_wl_cell(WlArgs a, float* gx, float* gy) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.num_cells) return;
+  float sx = 0.0f, sy = 0.0f;
+  for (int i = a.cell_start[c]; i < a.cell_start[c + 1]; i++) {
+    const float2 g = a.pin_g[a.cell_pin[i]];
+    sx += g.x;
+    sy += g.y;
+  }
+  gx[c] = sx;
+  gy[c] = sy;
+}
+
+__global__ __launch_bounds__(256) void k_wl_update(WlArgs a, const float* cx, const float* cy) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < a.num_pins) (void)pin_at<true>(a, p, cx, cy);
+}
+
+// computeNetBBox + computeHPWL (WireLengthGradient.cu:213-235, 383-401):
+// (max - min) per net and axis with the reference's initial values, summed in FP64.
+__global__ __launch_bounds__(256) void k_wl_hpwl(WlArgs a) {
+  __shared__ double sh[4];
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  double v = 0.0;
+  if (n < a.num_nets) {
+    float mxx = 0.0f, mxy = 0.0f, mnx = FLT_MAX, mny = FLT_MAX;
+    for (int p = a.net_start[n]; p < a.net_start[n + 1]; p++) {
+      const float2 q = a.pin_xy[p];
+      mxx = fmaxf(q.x, mxx);
+      mnx = fminf(q.x, mnx);
+      mxy = fmaxf(q.y, mxy);
+      mny = fminf(q.y, mny);
+    }
+    v = (double)(mxx - mnx) + (double)(mxy - mny);
+  }
+  v = block_reduce(v, [](double x, double y) { return x + y; }, sh);
+  if (threadIdx.x == 0) atomicAdd(a.hpwl, v);
+}
+
+// jacobianPrecondition (ProblemInstanceKernel.hpp:74-100)
+__global__ __launch_bounds__(256) void k_precond(int n, float lambda, float min_precond, const float* num_pins,
+                                                 const float* dens_precond, const float* wgx, const float* wgy,
+                                                 const float* dgx, const float* dgy, float* gx, float* gy) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int wp = (int)num_pins[i];
+  float pre = (float)wp + lambda * dens_precond[i];
+  pre = fmaxf(pre, min_precond);
+  gx[i] = (wgx[i] + lambda * dgx[i]) / pre;
+  gy[i] = (wgy[i] + lambda * dgy[i]) / pre;
+}
+
+}  // namespace
+
+struct WlHandle {
+  WlArgs a{};
+  hipStream_t stream = nullptr;
+  std::vector<void*> bufs;
+};
+
+namespace {
+
+template <typename T>
+int up(WlHandle* h, T** dst, const T* src, size_t n) {
+  T* p = nullptr;
+  WL_TRY(hipMalloc((void**)&p, std::max<size_t>(n, 1) * sizeof(T)));
+  h->bufs.push_back(p);
+  if (src && n) WL_TRY(hipMemcpy(p, src, n * sizeof(T), hipMemcpyHostToDevice));
+  *dst = p;
+  return WL_OK;
+}
+
+hipStream_t pick(WlHandle* h, void* s) { return s ? (hipStream_t)s : h->stream; }
+
+template <bool FROM_CELLS>
+int launch_grad(WlHandle* h, const float* cx, const float* cy, float g_inv, float* gx, float* gy, hipStream_t s) {
+  const WlArgs& a = h->a;
+  if (a.num_nets > 0)
+    hipLaunchKernelGGL((k_wl_net<FROM_CELLS>), dim3((a.num_nets + 255) / 256), dim3(256), 0, s, a, g_inv, cx, cy);
+  if (a.num_big > 0)
+    hipLaunchKernelGGL((k_wl_net_big<FROM_CELLS>), dim3(a.num_big), dim3(kBigThreads), 0, s, a, g_inv, cx, cy);
+  if (a.num_cells > 0)
+    hipLaunchKernelGGL(k_wl_cell, dim3((a.num_cells + 255) / 256), dim3(256), 0, s, a, gx, gy);
+  WL_TRY(hipGetLastError());
+  return WL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* wl_last_error(void) { return g_err.c_str(); }
+
+int wl_create(const WlNetlist* n, WlHandle** out) {
+  if (!n || !out) return fail(WL_ERR_ARG, "null argument");
+  if (n->num_cells < 0 || n->num_nets < 0 || n->num_pins < 0) return fail(WL_ERR_ARG, "negative size");
+  if (!n->net_start || (n->num_pins > 0 && (!n->pin2cell || !n->pin_offset_x || !n->pin_offset_y ||
+                                            !n->pin_x || !n->pin_y)))
+    return fail(WL_ERR_ARG, "null netlist array");
+  if (n->net_start[0] != 0 || n->net_start[n->num_nets] != n->num_pins)
+    return fail(WL_ERR_ARG, "net_start must run from 0 to num_pins");
+  std::vector<int> big;
+  for (int i = 0; i < n->num_nets; i++) {
+    const int d = n->net_start[i + 1] - n->net_start[i];
+    if (d < 0) return fail(WL_ERR_ARG, "net_start not ascending");
+    if (d > kSmallDeg) big.push_back(i);
+  }
+  std::vector<int> cstart(n->num_cells + 1, 0);
+  for (int p = 0; p < n->num_pins; p++) {
+    const int c = n->pin2cell[p];
+    if (c >= n->num_cells) return fail(WL_ERR_ARG, "pin2cell beyond num_cells");
+    if (c >= 0) cstart[c + 1]++;
+  }
+  for (int c = 0; c < n->num_cells; c++) cstart[c + 1] += cstart[c];
+  std::vector<int> cpin(std::max(cstart[n->num_cells], 1)), fill(cstart.begin(), cstart.end() - 1);
+  for (int p = 0; p < n->num_pins; p++)
+    if (n->pin2cell[p] >= 0) cpin[fill[n->pin2cell[p]]++] = p;
+  std::vector<float2> off(n->num_pins), xy(n->num_pins);
+  for (int p = 0; p < n->num_pins; p++) {
+    off[p] = make_float2(n->pin_offset_x[p], n->pin_offset_y[p]);
+    xy[p] = make_float2(n->pin_x[p], n->pin_y[p]);
+  }
+  std::vector<float> w(std::max(n->num_nets, 1), 1.0f);
+  if (n->net_weight) std::copy(n->net_weight, n->net_weight + n->num_nets, w.begin());
+
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(WL_ERR_DEVICE, "no HIP device");
+  WlHandle* h = new WlHandle();
+  WlArgs& a = h->a;
+  a.num_cells = n->num_cells;
+  a.num_nets = n->num_nets;
+  a.num_pins = n->num_pins;
+  a.num_big = (int)big.size();
+  int st = 0;
+  int *d_ns, *d_p2c, *d_cs, *d_cp, *d_big;
+  float* d_w;
+  float2 *d_off, *d_xy, *d_g;
+  double* d_h;
+  if ((st = up(h, &d_ns, n->net_start, (size_t)n->num_nets + 1)) || (st = up(h, &d_w, w.data(), w.size())) ||
+      (st = up(h, &d_p2c, n->pin2cell, (size_t)n->num_pins)) || (st = up(h, &d_off, off.data(), off.size())) ||
+      (st = up(h, &d_xy, xy.data(), xy.size())) || (st = up(h, &d_g, (const float2*)nullptr, xy.size())) ||
+      (st = up(h, &d_cs, cstart.data(), cstart.size())) || (st = up(h, &d_cp, cpin.data(), cpin.size())) ||
+      (st = up(h, &d_big, big.data(), big.size())) || (st = up(h, &d_h, (const double*)nullptr, 1))) {
+    wl_destroy(h);
+    return st;
+  }
+  a.net_start = d_ns; a.net_w = d_w; a.pin2cell = d_p2c; a.pin_off = d_off; a.pin_xy = d_xy; a.pin_g = d_g;
+  a.cell_start = d_cs; a.cell_pin = d_cp; a.big = d_big; a.hpwl = d_h;
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    wl_destroy(h);
+    return fail(WL_ERR_DEVICE, "hipStreamCreate");
+  }
+  *out = h;
+  return WL_OK;
+}
+
+int wl_update_pin_coordinates(WlHandle* h, const float* cx, const float* cy, void* stream) {
+  if (!h || (h->a.num_pins > 0 && (!cx || !cy))) return fail(WL_ERR_ARG, "null argument");
+  if (h->a.num_pins > 0)
+    hipLaunchKernelGGL(k_wl_update, dim3((h->a.num_pins + 255) / 256), dim3(256), 0, pick(h, stream), h->a, cx, cy);
+  WL_TRY(hipGetLastError());
+  return WL_OK;
+}
+
+int wl_compute_grad(WlHandle* h, float g_inv, float* gx, float* gy, void* stream) {
+  if (!h || (h->a.num_cells > 0 && (!gx || !gy))) return fail(WL_ERR_ARG, "null argument");
+  return launch_grad<false>(h, nullptr, nullptr, g_inv, gx, gy, pick(h, stream));
+}
+
+int wl_compute_grad_at(WlHandle* h, const float* cx, const float* cy, float g_inv, float* gx, float* gy,
+                       void* stream) {
+  if (!h || (h->a.num_cells > 0 && (!gx || !gy || !cx || !cy))) return fail(WL_ERR_ARG, "null argument");
+  return launch_grad<true>(h, cx, cy, g_inv, gx, gy, pick(h, stream));
+}
+
+int wl_compute_hpwl(WlHandle* h, double* out, void* stream) {
+  if (!h || !out) return fail(WL_ERR_ARG, "null argument");
+  hipStream_t s = pick(h, stream);
+  WL_TRY(hipMemsetAsync(h->a.hpwl, 0, sizeof(double), s));
+  if (h->a.num_nets > 0)
+    hipLaunchKernelGGL(k_wl_hpwl, dim3((h->a.num_nets + 255) / 256), dim3(256), 0, s, h->a);
+  WL_TRY(hipGetLastError());
+  WL_TRY(hipMemcpyAsync(out, h->a.hpwl, sizeof(double), hipMemcpyDeviceToHost, s));
+  WL_TRY(hipStreamSynchronize(s));
+  return WL_OK;
+}
+
+int wl_get_pins(WlHandle* h, float* x, float* y) {
+  if (!h) return fail(WL_ERR_ARG, "null handle");
+  WL_TRY(hipStreamSynchronize(h->stream));
+  WL_TRY(hipDeviceSynchronize());
+  std::vector<float2> v(h->a.num_pins);
+  if (!v.empty()) WL_TRY(hipMemcpy(v.data(), h->a.pin_xy, v.size() * sizeof(float2), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < v.size(); i++) {
+    if (x) x[i] = v[i].x;
+    if (y) y[i] = v[i].y;
+  }
+  return WL_OK;
+}
+
+int wl_destroy(WlHandle* h) {
+  if (!h) return WL_OK;
+  for (void* b : h->bufs) (void)hipFree(b);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return WL_OK;
+}
+
+int tf_precondition(int32_t n, float lambda, float min_precond, const float* num_pins, const float* dens_precond,
+                    const float* wgx, const float* wgy, const float* dgx, const float* dgy, float* gx, float* gy,
+                    void* stream) {
+  if (n < 0) return fail(WL_ERR_ARG, "negative size");
+  if (n == 0) return WL_OK;
+  if (!num_pins || !dens_precond || !wgx || !wgy || !dgx || !dgy || !gx || !gy)
+    return fail(WL_ERR_ARG, "null argument");
+  hipLaunchKernelGGL(k_precond, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, n, lambda, min_precond,
+                     num_pins, dens_precond, wgx, wgy, dgx, dgy, gx, gy);
+  WL_TRY(hipGetLastError());
+  return WL_OK;
+}
+
+}  // extern "C"

@@ -274,3 +274,58 @@ def density_placement(num_cells: int = 1_000_000, *, fill: float = 0.7, clusters
     return dict(die=(0.0, 0.0, die_side, die_side), dx=side.astype(np.float32), dy=side.astype(np.float32),
                 cx=cx.astype(np.float32), cy=cy.astype(np.float32), is_macro=is_macro, is_filler=is_filler,
                 fixed_rects=fixed, name=f"density_{num_cells}")
+
+
+# ---------------------------------------------------------------------------
+# Netlists for the wirelength gradient (global placement)
+# ---------------------------------------------------------------------------
+def _power_law_alpha(lo: int, hi: int, mean: float) -> float:
+    """Exponent a of P(d) ~ d^-a on {lo..hi} whose mean is `mean` (bisection)."""
+    d = np.arange(lo, hi + 1, dtype=np.float64)
+
+    def m(a):
+        w = d ** -a
+        return float((d * w).sum() / w.sum())
+
+    a0, a1 = 0.0, 10.0
+    for _ in range(100):
+        am = 0.5 * (a0 + a1)
+        if m(am) > mean:
+            a0 = am
+        else:
+            a1 = am
+    return 0.5 * (a0 + a1)
+
+
+def netlist(cell_w, cell_h, num_nets: int, *, die=(0.0, 0.0, 1.0, 1.0), min_deg: int = 2,
+            max_deg: int = 1000, mean_deg: float = 3.8, io_frac: float = 0.005, seed: int = 0) -> dict:
+    """Random netlist over cells of sizes (cell_w, cell_h): net degrees from a
+    power law on [min_deg, max_deg] with mean `mean_deg`; each pin sits on a
+    uniformly drawn cell at a uniform offset inside it (offset = pin centre -
+    cell centre), or -- with probability io_frac -- is an IO pin of no
+    movable cell at a uniform position on the die.  Pins are numbered net by
+    net (the reference's layout)."""
+    rng = np.random.default_rng(seed)
+    n_cells = len(cell_w)
+    a = _power_law_alpha(min_deg, max_deg, mean_deg)
+    d = np.arange(min_deg, max_deg + 1)
+    p = d.astype(np.float64) ** -a
+    deg = rng.choice(d, size=num_nets, p=p / p.sum()).astype(np.int64)
+    net_start = np.zeros(num_nets + 1, np.int64)
+    np.cumsum(deg, out=net_start[1:])
+    n_pins = int(net_start[-1])
+    pin2cell = rng.integers(0, n_cells, n_pins).astype(np.int32)
+    io = rng.random(n_pins) < io_frac
+    pin2cell[io] = -1
+    cw = np.asarray(cell_w, np.float64)
+    ch = np.asarray(cell_h, np.float64)
+    c = np.where(io, 0, pin2cell)
+    off_x = ((rng.random(n_pins) - 0.5) * cw[c]).astype(np.float32)
+    off_y = ((rng.random(n_pins) - 0.5) * ch[c]).astype(np.float32)
+    off_x[io] = 0.0
+    off_y[io] = 0.0
+    pin_x = (die[0] + rng.random(n_pins) * (die[2] - die[0])).astype(np.float32)
+    pin_y = (die[1] + rng.random(n_pins) * (die[3] - die[1])).astype(np.float32)
+    return dict(net_start=net_start.astype(np.int32), pin2cell=pin2cell, pin_offset_x=off_x,
+                pin_offset_y=off_y, pin_x=pin_x, pin_y=pin_y,
+                net_weight=np.ones(num_nets, np.float32), num_cells=n_cells, alpha=a)

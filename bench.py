@@ -2,7 +2,7 @@
 """Throughput of the device legalizer.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
-                    [--workload served_stream|mixed_1m|plain_20k|density_1m]
+                    [--workload served_stream|mixed_1m|plain_20k|density_1m|wl_2m]
                     [--requests R] [--slots S] [--iters-per-sync I] [--load L]
                     [--no-cpu-baseline] [--cpu-sample-cells N] [--cpu-threads T]
 
@@ -52,16 +52,19 @@ WORKLOADS = {
                 "rail parity), 1600 rows x 12000 sites, 85 % utilisation, 3 fences, 200 macros, seed 0",
     "plain_20k": "20k single-row cells, 250 rows x 490 sites, widths 1-8 sites, 75 % utilisation, "
                  "4 Gaussian hotspots, 10 macros, seed 0",
+    "wl_2m": "weighted-average wirelength gradient (pin update fused): 2M cells (area LogNormal(ln 4, 0.5), "
+             "positions half uniform / half clustered), 2.1M nets with degrees power-law on [2, 1000] "
+             "(mean 3.8), pins at uniform offsets inside their cells, 0.5 % IO pins, seed 0",
     "density_1m": "electrostatic density: 1M cells, area LogNormal(ln 4, 0.5), half uniform / half in 8 "
                   "Gaussian clusters, square die at 70 % fill, 1024 x 1024 bins, split-BF16 Poisson, seed 0",
 }
 MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16
 
 
-def traffic_from_profiles(workload: str):
+def traffic_from_profiles(workload: str, kernel: str = "k_partition"):
     """HBM bytes per partition-kernel launch from the committed rocprofv3 PMC
     summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), or None."""
-    path = os.path.join(HERE, "profiles", f"k_partition_{workload}_traffic.md")
+    path = os.path.join(HERE, "profiles", f"{kernel}_{workload}_traffic.md")
     if not os.path.exists(path):
         return None
     m = re.search(r"^bytes_per_launch:\s*([0-9.eE+]+)", open(path).read(), re.M)
@@ -99,6 +102,8 @@ def main():
                     help="served_stream: the first N requests of the seeded 2000-request stream")
     ap.add_argument("--slots", type=int, default=64, help="served_stream: resident requests")
     ap.add_argument("--iters-per-sync", type=int, default=16, help="served_stream: batch length")
+    ap.add_argument("--skip-plain", action="store_true",
+                    help="served_stream: served leg only (profiling runs; no served/plain comparison)")
     ap.add_argument("--load", type=float, default=0.7,
                     help="served_stream: Poisson arrival rate as a fraction of the measured capacity")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -121,8 +126,8 @@ def main():
     lg.lib()  # raises if the HIP library is not built
     if args.workload == "density_1m":
         return bench_density(args)
-    if args.workload == "served_stream":
-        return bench_served(args)
+    if args.workload == "wl_2m":
+        return bench_wl(args)
     if args.copy_calibration:
         src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
@@ -130,6 +135,8 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    if args.workload == "served_stream":
+        return bench_served(args)
 
     t = time.perf_counter()
     design = synth.mixed_1m() if args.workload == "mixed_1m" else synth.plain_20k()
@@ -311,18 +318,18 @@ def bench_served(args):
         return lx, ly, rot, st
 
     # -- plain: one request at a time (lg_create .. lg_copy_out per request)
-    for _ in range(args.warmup):
+    for _ in range(args.warmup if not args.skip_plain else 0):
         plain_one(*reqs[int(np.argmin(cells))])
     torch.cuda.synchronize()
     plain, service = [], []
     t0 = time.perf_counter()
-    for i, (g, c, p) in enumerate(reqs):
+    for i, (g, c, p) in enumerate(reqs if not args.skip_plain else []):
         ts = time.perf_counter()
         plain.append(plain_one(g, c, p))
         service.append(time.perf_counter() - ts)
         if (i + 1) % 25 == 0:
             print(f"# plain {i + 1}/{len(reqs)} {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
-    t_plain = time.perf_counter() - t0
+    t_plain = time.perf_counter() - t0 if not args.skip_plain else float("nan")
     plain_rate = total / t_plain
     print(f"# plain: {t_plain:.2f}s {plain_rate:.0f} cells/s", file=sys.stderr, flush=True)
 
@@ -339,7 +346,12 @@ def bench_served(args):
                     if d > 0:
                         time.sleep(d)
                 tk.append(srv.submit(g, c, p))
-            res = [srv.wait(k) for k in tk]
+            res = []
+            for i, k in enumerate(tk):
+                res.append(srv.wait(k))
+                if (i + 1) % 20 == 0:
+                    print(f"# served: {i + 1}/{len(tk)} collected at {time.perf_counter() - t0:.1f}s",
+                          file=sys.stderr, flush=True)
             el = time.perf_counter() - t0
             return res, el, srv.counters(), (srv.profile_read() if prof else None)
 
@@ -360,6 +372,9 @@ def bench_served(args):
     same = 0
     disp_gap = 0.0
     legal = True
+    if args.skip_plain:
+        same = len(reqs)
+        legal = all(r["stats"]["converged"] == 1 and r["stats"]["vio"]["total_ovf"] == 0 for r in res)
     for (lx, ly, rot, st), r in zip(plain, res):
         same += int(np.array_equal(lx, r["lx"]) and np.array_equal(ly, r["ly"]) and np.array_equal(rot, r["rot"]))
         if st["total_disp_dbu"]:
@@ -376,14 +391,19 @@ def bench_served(args):
     lam_s = args.load * len(reqs) / t_served
     arr_s = np.cumsum(gaps) / lam_s
     arr_s -= arr_s[0]
-    rp, el_p, ctr_p, _ = served_run(arr_s)
-    lat_served = [r["latency_s"] for r in rp]
-    lam_p = args.load * len(reqs) / t_plain
-    arr_p = np.cumsum(gaps) / lam_p
-    arr_p -= arr_p[0]
-    lat_plain = fifo_replay(arr_p, service)
-    print(f"# poisson: served p99 {np.percentile(lat_served, 99) * 1e3:.1f} ms, plain p99 "
-          f"{np.percentile(lat_plain, 99) * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    if args.skip_plain:
+        lat_served = lat_plain = lat_plain_same = [float("nan")]
+        lam_p = float("nan")
+    else:
+        rp, el_p, ctr_p, _ = served_run(arr_s)
+        lat_served = [r["latency_s"] for r in rp]
+        lam_p = args.load * len(reqs) / t_plain
+        arr_p = np.cumsum(gaps) / lam_p
+        arr_p -= arr_p[0]
+        lat_plain = fifo_replay(arr_p, service)
+        lat_plain_same = fifo_replay(arr_s, service)  # plain under the served run's own arrivals
+        print(f"# poisson: served p99 {np.percentile(lat_served, 99) * 1e3:.1f} ms, plain p99 "
+              f"{np.percentile(lat_plain, 99) * 1e3:.1f} ms", file=sys.stderr, flush=True)
 
     achieved = prof0["bytes"] / (prof0["ms"] / 1e3) / 1e9 if prof0 and prof0["ms"] > 0 else None
     out = {
@@ -415,6 +435,7 @@ def bench_served(args):
                 "plain_arrival_rate_per_s": lam_p,
                 "plain_note": "FIFO replay of the measured per-request plain service times under Poisson "
                               "arrivals at the same fraction of plain capacity",
+                "plain_at_served_arrivals": percentiles(lat_plain_same),
                 "served_saturated": percentiles([r["latency_s"] for r in res]),
             },
             "server": {"slots": args.slots, "iters_per_sync": args.iters_per_sync,
@@ -431,7 +452,7 @@ def bench_served(args):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS if achieved else None,
-            "traffic": None,
+            "traffic": traffic_from_profiles("served_stream", "kb_partition"),
             "avg_iteration_us": prof0["ms"] / max(prof0["iterations"], 1) * 1e3 if prof0 else None,
             "iterations": prof0["iterations"] if prof0 else None,
             "algorithmic_bytes_per_iteration": prof0["bytes"] / max(prof0["iterations"], 1) if prof0 else None,
@@ -444,6 +465,72 @@ def bench_served(args):
     if not args.no_cpu_baseline:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = served_cpu_baseline(stream_spec(2000), args.cpu_threads)
+    print(json.dumps(out), flush=True)
+
+
+def bench_wl(args):
+    """One WA wirelength gradient evaluation (pin update + gradient) per step."""
+    import torch
+    from admm_legalization_amd import synth
+    from admm_legalization_amd.wirelength import WireLengthGradient
+    pl = synth.density_placement(2_000_000, seed=0)
+    nl = synth.netlist(pl["dx"], pl["dy"], 2_100_000, die=pl["die"], seed=1)
+    n_cells, n_nets, n_pins = len(pl["dx"]), len(nl["net_start"]) - 1, len(nl["pin2cell"])
+    print(f"# wl_2m: cells {n_cells} nets {n_nets} pins {n_pins}", file=sys.stderr, flush=True)
+    wl = WireLengthGradient(n_cells, nl["net_start"], nl["pin2cell"], nl["pin_offset_x"], nl["pin_offset_y"],
+                            nl["pin_x"], nl["pin_y"], nl["net_weight"])
+    die_w = pl["die"][2] - pl["die"][0]
+    wl.setGammaInv(8.0 / (0.01 * die_w))  # gamma of 1/8 of a 1 % bin width scale
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        cx = torch.tensor(pl["cx"], device="cuda")
+        cy = torch.tensor(pl["cy"], device="cuda")
+        gx, gy = torch.empty_like(cx), torch.empty_like(cx)
+        for _ in range(max(args.warmup, 1)):
+            wl.computeGradAt(cx, cy, gx, gy)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = max(args.steps, 20)
+        t0 = time.perf_counter()
+        e0.record(s)
+        for _ in range(reps):
+            wl.computeGradAt(cx, cy, gx, gy)
+        e1.record(s)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        ms = e0.elapsed_time(e1) / reps
+    deg = np.diff(nl["net_start"])
+    # algorithmic bytes: per pin 4 (pin2cell) + 8 (offset) + 8 (cell centre) + 8 (pin write)
+    # + 8 (term write) + 4 (cell index) + 8 (term read); per net 8; per cell 8 (out) + 4 (CSR)
+    bytes_eval = 48.0 * n_pins + 8.0 * n_nets + 12.0 * n_cells
+    achieved = bytes_eval / (ms / 1e3) / 1e9
+    out = {
+        "metric": "WA wirelength gradient evaluations/s (pin update + gradient)",
+        "value": 1e3 / ms, "unit": "evaluations/s", "n_gpus": 1, "steps": reps, "warmup": args.warmup,
+        "ms_per_step": el / reps * 1e3, "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"wl_2m: {WORKLOADS['wl_2m']}", "cells": n_cells, "nets": n_nets, "pins": n_pins,
+                   "nets_above_32_pins": int((deg > 32).sum()), "event_ms_per_eval": ms,
+                   "pins_per_s": n_pins * 1e3 / ms, "parallelism": "1 GPU"},
+        "roofline": {"bound": "hbm", "kernel": "k_wl_net + k_wl_net_big + k_wl_cell (one evaluation)",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "algorithmic_bytes_per_eval": bytes_eval},
+        "cpu_baseline": None,
+    }
+    wl.close()
+    if not args.no_cpu_baseline:
+        from oracle import ref_wl as W
+        k = 200_000  # the oracle on the first 200k nets (numpy, FP64)
+        m = int(nl["net_start"][k])
+        ns = nl["net_start"][:k + 1]
+        t = time.perf_counter()
+        px, py = W.update_pins(nl["pin_x"][:m], nl["pin_y"][:m], nl["pin2cell"][:m], nl["pin_offset_x"][:m],
+                               nl["pin_offset_y"][:m], pl["cx"], pl["cy"])
+        W.wa_grad(ns, px, py, nl["pin2cell"][:m], 1.0, nl["net_weight"][:k], n_cells)
+        el = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": m / el, "unit": "pins/s", "cores": 1, "kind": "port",
+                               "sample": f"FP64 numpy oracle on the first {k} nets ({m} pins) in {el:.2f} s; "
+                                         f"GPU rate {n_pins * 1e3 / ms:.3g} pins/s"}
     print(json.dumps(out), flush=True)
 
 

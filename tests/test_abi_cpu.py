@@ -7,10 +7,10 @@ import subprocess
 
 import pytest
 
-from admm_legalization_amd import build, density, lg
+from admm_legalization_amd import build, density, lg, wirelength
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADERS = [os.path.join(ROOT, "include", h) for h in ("skyline_lg.h", "skyline_density.h")]
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("skyline_lg.h", "skyline_density.h", "skyline_wl.h")]
 
 
 def declared(path):
@@ -27,11 +27,13 @@ def library():
 
 
 def test_headers_declare_the_expected_entry_points():
-    lg_syms, dens_syms = (declared(h) for h in HEADERS)
+    lg_syms, dens_syms, wl_syms = (declared(h) for h in HEADERS)
     assert {"lg_create", "lg_solve", "lg_postprocess", "lg_check", "lg_copy_out", "lg_destroy"} <= lg_syms
     assert {"dens_create", "dens_compute_grad", "poisson_create", "poisson_solve"} <= dens_syms
+    assert {"wl_create", "wl_compute_grad", "wl_update_pin_coordinates", "tf_precondition"} <= wl_syms
     assert set(lg.EXPORTED) == lg_syms
     assert set(density.EXPORTED) == dens_syms
+    assert set(wirelength.EXPORTED) == wl_syms
 
 
 def test_library_exports_every_declared_symbol(library):
@@ -63,6 +65,9 @@ def test_null_arguments_are_rejected_without_a_device(library):
     assert b"null" in L.lg_last_error()
     D = density.lib()
     assert D.dens_create(None, None) != 0
+    Wl = wirelength.lib()
+    assert Wl.wl_create(None, None) == 1 and b"null" in Wl.wl_last_error()
+    assert Wl.tf_precondition(-1, 0.0, 1.0, *([None] * 8), None) == 1
     assert D.poisson_solve(None, None, None, None, None, 1, None) != 0
 
 

@@ -1,0 +1,99 @@
+"""GPU parity of the wirelength gradient (include/skyline_wl.h) with the FP64
+oracle (oracle/ref_wl.py).  FP32 on the device; tolerances stated per test."""
+import numpy as np
+import pytest
+
+from admm_legalization_amd import synth
+from oracle import ref_wl as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(n_cells, n_nets, seed, max_deg=1000):
+    pl = synth.density_placement(n_cells, seed=seed)
+    nl = synth.netlist(pl["dx"], pl["dy"], n_nets, die=pl["die"], max_deg=max_deg, seed=seed + 1)
+    nl["net_weight"] = np.random.default_rng(seed).uniform(0.5, 1.5, n_nets).astype(np.float32)
+    return pl, nl
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30))
+
+
+@pytest.mark.parametrize("n_cells,n_nets,g_inv", [(500, 600, 0.5), (20000, 21000, 0.1), (60000, 63000, 2.0)])
+def test_grad_matches_oracle(gpu, n_cells, n_nets, g_inv):
+    import torch
+    from admm_legalization_amd.wirelength import WireLengthGradient
+    pl, nl = _case(n_cells, n_nets, seed=n_cells)
+    print(f"[case] cells={n_cells} nets={n_nets} pins={len(nl['pin2cell'])} gInv={g_inv}", flush=True)
+    wl = WireLengthGradient(nl["num_cells"], nl["net_start"], nl["pin2cell"], nl["pin_offset_x"],
+                            nl["pin_offset_y"], nl["pin_x"], nl["pin_y"], nl["net_weight"])
+    wl.setGammaInv(g_inv)
+    cx = torch.tensor(pl["cx"], device="cuda")
+    cy = torch.tensor(pl["cy"], device="cuda")
+    gx, gy = torch.empty_like(cx), torch.empty_like(cx)
+    wl.computeGradAt(cx, cy, gx, gy)
+    got = (gx.cpu().numpy(), gy.cpu().numpy())
+    px, py = W.update_pins(nl["pin_x"], nl["pin_y"], nl["pin2cell"], nl["pin_offset_x"], nl["pin_offset_y"],
+                           pl["cx"], pl["cy"])
+    want = W.wa_grad(nl["net_start"], px, py, nl["pin2cell"], g_inv, nl["net_weight"], n_cells)
+    for g, w_ in zip(got, want):
+        assert _rel(g, w_) <= 2e-5  # FP32 exp and sums vs FP64
+        np.testing.assert_allclose(g, w_, atol=1e-4 * np.abs(w_).max())
+    # the stored pins are the oracle's update (FP32 add of centre and offset)
+    gpx, gpy = wl.pins()
+    np.testing.assert_array_equal(gpx, np.where(nl["pin2cell"] >= 0, (pl["cx"][np.maximum(nl["pin2cell"], 0)]
+                                                                      + nl["pin_offset_x"]).astype(np.float32),
+                                                nl["pin_x"]))
+    # the unfused pair gives the same bits; a repeat gives the same bits
+    gx2, gy2 = torch.empty_like(cx), torch.empty_like(cx)
+    wl.updatePinCoordinates(cx, cy)
+    wl.computeGrad(gx2, gy2)
+    assert torch.equal(gx, gx2) and torch.equal(gy, gy2)
+    # HPWL: per-net max - min in FP32 as the reference (computeNetBBox), summed
+    # in FP64; the oracle subtracts in FP64 (half an FP32 ulp per net apart)
+    assert wl.computeHPWL() == pytest.approx(W.hpwl(nl["net_start"], gpx, gpy), rel=1e-6)
+    wl.close()
+
+
+def test_big_nets_and_io_pins(gpu):
+    """Nets above the per-thread size (workgroup path) and IO pins."""
+    import torch
+    from admm_legalization_amd.wirelength import WireLengthGradient
+    rng = np.random.default_rng(5)
+    n_cells = 3000
+    deg = np.array([2, 33, 1000, 64, 3, 257, 2, 31, 32, 999])
+    ns = np.concatenate([[0], np.cumsum(deg)]).astype(np.int32)
+    npins = int(ns[-1])
+    p2c = rng.integers(-1, n_cells, npins).astype(np.int32)
+    offx = rng.uniform(-1, 1, npins).astype(np.float32)
+    offy = rng.uniform(-1, 1, npins).astype(np.float32)
+    px0 = rng.uniform(0, 500, npins).astype(np.float32)
+    py0 = rng.uniform(0, 500, npins).astype(np.float32)
+    cxh = rng.uniform(0, 500, n_cells).astype(np.float32)
+    cyh = rng.uniform(0, 500, n_cells).astype(np.float32)
+    wl = WireLengthGradient(n_cells, ns, p2c, offx, offy, px0, py0)
+    wl.setGammaInv(0.05)
+    cx, cy = torch.tensor(cxh, device="cuda"), torch.tensor(cyh, device="cuda")
+    gx, gy = torch.empty_like(cx), torch.empty_like(cx)
+    wl.computeGradAt(cx, cy, gx, gy)
+    px, py = W.update_pins(px0, py0, p2c, offx, offy, cxh, cyh)
+    want = W.wa_grad(ns, px, py, p2c, 0.05, np.ones(len(deg)), n_cells)
+    for g, w_ in zip((gx.cpu().numpy(), gy.cpu().numpy()), want):
+        assert _rel(g, w_) <= 2e-5
+    wl.close()
+
+
+def test_precondition_matches_oracle(gpu):
+    import torch
+    from admm_legalization_amd.wirelength import precondition
+    rng = np.random.default_rng(9)
+    n = 10000
+    arrs = [rng.integers(0, 9, n).astype(np.float32)] + [rng.normal(size=n).astype(np.float32) for _ in range(5)]
+    arrs[1] = np.abs(arrs[1])
+    t = [torch.tensor(a, device="cuda") for a in arrs]
+    gx, gy = torch.empty_like(t[0]), torch.empty_like(t[0])
+    precondition(0.7, 1.0, t[0], t[1], t[2], t[3], t[4], t[5], gx, gy)
+    wx, wy = W.precondition(np.float32(0.7), 1.0, *arrs)
+    np.testing.assert_allclose(gx.cpu().numpy(), wx, rtol=1e-6, atol=1e-7)
+    np.testing.assert_allclose(gy.cpu().numpy(), wy, rtol=1e-6, atol=1e-7)
