@@ -168,7 +168,7 @@ struct HostPrep {
   size_t P = 0;
   std::vector<int16_t> pcode;
   std::vector<int32_t> region_rm, vdd, rx0, ry0, rx1, ry1;
-  std::vector<int32_t> lxg, lyg, w, h, code, gnd, ox, oy, rprio, khn, colmap, rowmap, ident;
+  std::vector<int32_t> lxg, lyg, w, h, code, gnd, ox, oy, rprio, khn, colmap, rowmap;
 };
 
 struct LgHandle {
@@ -381,8 +381,6 @@ int prepare_host(const LgGridDesc* g, const LgCellsSoA* c, const LgParams* p, Ho
     std::mt19937 gen(p->random_sorting_seed);
     std::shuffle(hp.rprio.begin(), hp.rprio.end(), gen);
   }
-  hp.ident.resize(std::max(N, 1));
-  std::iota(hp.ident.begin(), hp.ident.end(), 0);
   // partition column / row of every x / y for the three fold offsets
   // (paintPixelsKernel, AdmmKernels.hpp:164-191)
   hp.colmap.resize(3 * (size_t)XS);
@@ -399,11 +397,10 @@ int prepare_host(const LgGridDesc* g, const LgCellsSoA* c, const LgParams* p, Ho
 // buffers, each rounded to the pool's 256-byte blocks).
 size_t handle_bytes(const HostPrep& hp) {
   const size_t P = hp.P, N = std::max(hp.N, 1), K = hp.K, blk = 256;
-  const size_t nb = (N + 4095) / 4096;
-  size_t b = P * (2 + 4 + 4 + 4);                       // pcode, region, usage, dual
-  b += N * 4 * (2 + 2 + 8 + 1 + 1 + 1 + 1 + 1 + 5 + 2);  // cells, records, radix buffers, flags
-  b += N * 4 * 4 * 2;                                   // cellrec, rec_b
-  b += K * 4 * (1 + 2 + 1 + 2 + 1 + 2 + 2) + 128 * nb * 4;
+  size_t b = P * (2 + 4 + 4 + 4);                     // pcode, region, usage, dual
+  b += N * 4 * (2 + 2 + 8 + 1 + 1 + 2 + 2);           // positions, cells, rot, grouped, flags, timing
+  b += N * 4 * 4 * 2;                                 // cellrec, rec_b
+  b += K * 4 * ((size_t)kMaxNpad + 1 + 1 + 2 + 1 + 1 + 2 + 2);  // buckets, counts, offsets, flags, lists
   b += (3 * (size_t)hp.x_size + 3 * (size_t)hp.y_size + hp.y_size + 4 * hp.rx0.size()) * 4;
   b += (size_t)std::max(hp.p.max_admm_iter, 1) * 4 + sizeof(Ctl);
   return b + 64 * blk;
@@ -487,29 +484,16 @@ int create_from_prep(const HostPrep& hp, DevPool* pool, Ctl* ext_ctl, hipStream_
   a.rand_prio = d_rp; a.k_height_num = d_khn;
   if ((st = halloc(h, &a.need_rot, N))) return bail(st);
   HIP_TRY(hipMemset(a.need_rot, 0, std::max(N, 1) * sizeof(int32_t)));
-  if ((st = halloc(h, &a.part_of, N))) return bail(st);
-  if ((st = halloc(h, &a.cnt, a.K + 1))) return bail(st);
+  if ((st = halloc(h, &a.bucket, (size_t)a.K * kMaxNpad))) return bail(st);
+  if ((st = halloc(h, &a.bcnt, a.K))) return bail(st);
+  HIP_TRY(hipMemset(a.bcnt, 0, std::max(a.K, 1) * sizeof(int32_t)));
+  if ((st = halloc(h, &a.pcnt, a.K))) return bail(st);
   if ((st = halloc(h, &a.off, a.K + 2))) return bail(st);
-  if ((st = halloc(h, &a.fill, a.K + 1))) return bail(st);
+  if ((st = halloc(h, &a.scratch, a.K + 1))) return bail(st);
   if ((st = halloc(h, &a.grouped, N))) return bail(st);
   if ((st = halloc(h, &a.cellrec, 4 * (size_t)N))) return bail(st);
   if ((st = halloc(h, &a.rec_b, 4 * (size_t)N))) return bail(st);
-  {
-    // radix grouping: passes of 7 bits over partition keys 0..K
-    int bits = 1;
-    while ((1 << bits) <= a.K) bits++;
-    a.rx_passes = (bits + 6) / 7;
-    a.rx_nb = (N + 4095) / 4096;
-    int32_t* d_ident;
-    if ((st = upload(h, &d_ident, hp.ident.data(), hp.ident.size()))) return bail(st);
-    a.ident = d_ident;
-    if ((st = halloc(h, &a.rx_k1, N)) || (st = halloc(h, &a.rx_v1, N)) || (st = halloc(h, &a.rx_k2, N)) ||
-        (st = halloc(h, &a.rx_v2, N)) || (st = halloc(h, &a.rx_keys, N)) ||
-        (st = halloc(h, &a.rx_hist, 128 * (size_t)std::max(a.rx_nb, 1))) || (st = halloc(h, &a.pstart, a.K)) ||
-        (st = halloc(h, &a.pend, a.K)))
-      return bail(st);
-    HIP_TRY(hipMemset(a.pstart, 0xff, std::max(a.K, 1) * sizeof(int32_t)));
-  }
+  if ((st = halloc(h, &a.dbg_a, N)) || (st = halloc(h, &a.dbg_b, N))) return bail(st);
   if ((st = halloc(h, &a.sorted, a.K))) return bail(st);
   if ((st = halloc(h, &a.work, 2 * (size_t)a.K))) return bail(st);
   if ((st = halloc(h, &a.act_flags, 2 * (size_t)a.K))) return bail(st);
@@ -528,7 +512,6 @@ int create_from_prep(const HostPrep& hp, DevPool* pool, Ctl* ext_ctl, hipStream_
   } else if ((st = halloc(h, &a.ctl, 1))) {
     return bail(st);
   }
-  HIP_TRY(hipMemset(a.cnt, 0, (a.K + 1) * sizeof(int32_t)));
 
   a.max_iter = p->max_admm_iter;
   a.terminate_thr = p->admm_terminate_thr;
@@ -595,10 +578,11 @@ int solve_begin(LgHandle* h, hipStream_t s) {
   hipLaunchKernelGGL(k_fill_dual, dim3(2048), dim3(256), 0, s, a);
   const dim3 pix_grid((a.x_size + 1023) / 1024, a.y_size);  // four pixels per thread
   HIP_TRY(hipMemsetAsync(a.act_flags, 0, 2 * (size_t)a.K * sizeof(int32_t), s));
+  HIP_TRY(hipMemsetAsync(a.bcnt, 0, std::max(a.K, 1) * sizeof(int32_t), s));
 #ifdef LG_PHASE_TIMERS
-  if (a.rx_passes <= 2 && N > 0) {  // per-iteration block-time maxima / sums (timed builds)
-    HIP_TRY(hipMemsetAsync(a.rx_k2, 0, N * sizeof(int32_t), s));
-    HIP_TRY(hipMemsetAsync(a.rx_v2, 0, N * sizeof(int32_t), s));
+  if (N > 0) {  // per-iteration block-time maxima / sums (timed builds)
+    HIP_TRY(hipMemsetAsync(a.dbg_a, 0, N * sizeof(int32_t), s));
+    HIP_TRY(hipMemsetAsync(a.dbg_b, 0, N * sizeof(int32_t), s));
   }
 #endif
   hipLaunchKernelGGL(k_classify_init, pix_grid, dim3(256), 0, s, a);
@@ -613,22 +597,8 @@ int launch_iteration(LgHandle* h, hipStream_t s) {
   Args& a = h->a;
   const int nb = (a.N + 255) / 256;
   const dim3 pix_grid((a.x_size + 1023) / 1024, a.y_size);
-  // partition keys (+ first digit histogram), then a stable radix sort of
-  // (partition, id) into grouped
-  hipLaunchKernelGGL(k_assign, dim3(a.rx_nb), dim3(kRxThreads), 0, s, a);
-  const int32_t* kin = a.part_of;
-  const int32_t* vin = a.ident;
-  for (int pass = 0; pass < a.rx_passes; pass++) {
-    const bool last = pass == a.rx_passes - 1;
-    int32_t* kout = last ? a.rx_keys : (pass & 1 ? a.rx_k2 : a.rx_k1);
-    int32_t* vout = last ? a.grouped : (pass & 1 ? a.rx_v2 : a.rx_v1);
-    if (pass > 0) hipLaunchKernelGGL(k_rx_hist, dim3(a.rx_nb), dim3(kRxThreads), 0, s, a, kin, 7 * pass);
-    hipLaunchKernelGGL(k_rx_scan, dim3(1), dim3(1024), 0, s, a);
-    hipLaunchKernelGGL(k_rx_scatter, dim3(a.rx_nb), dim3(kRxThreads), 0, s, a, kin, vin, kout, vout, 7 * pass);
-    kin = kout;
-    vin = vout;
-  }
-  hipLaunchKernelGGL(k_part_bounds, dim3(nb), dim3(256), 0, s, a);
+  // cells of the listed partitions into their buckets
+  if (a.N > 0) hipLaunchKernelGGL(k_bucket, dim3(nb), dim3(256), 0, s, a);
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, a);
   if (a.use_adaptive) {
     hipLaunchKernelGGL(k_extract_congested, dim3(nb), dim3(256), 0, s, a);
@@ -655,6 +625,8 @@ int solve_end(LgHandle* h, hipStream_t s) {
     if (h->host_ctl.error == kErrUsageRange)
       return fail(LG_ERR_STATE, "pixel usage above 65535");
     if (h->host_ctl.converged) {
+      hipLaunchKernelGGL(k_refine_bucket, dim3(nb), dim3(256), 0, s, a);
+      hipLaunchKernelGGL(k_refine_scan, dim3(1), dim3(1024), 0, s, a);
       if ((st = launch_partition<MODE_REFINE>(h, s))) return st;
     }
     hipLaunchKernelGGL(k_unpack_pos, dim3(nb), dim3(256), 0, s, a);
@@ -837,8 +809,8 @@ int lg_profile_read(LgHandle* h, double* ms, int64_t* launches) {
 #ifdef LG_PHASE_TIMERS
 int lg_debug_iter_times(LgHandle* h, int32_t* maxt, int32_t* sumt, int32_t* slow, int n) {
   n = std::min(n, h->a.N / 16);
-  HIP_TRY(hipMemcpy(maxt, h->a.rx_k2, n * sizeof(int32_t), hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(sumt, h->a.rx_v2, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(maxt, h->a.dbg_a, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(sumt, h->a.dbg_b, n * sizeof(int32_t), hipMemcpyDeviceToHost));
   HIP_TRY(hipMemcpy(slow, h->a.ovf_list, 16 * n * sizeof(int32_t), hipMemcpyDeviceToHost));
   return LG_OK;
 }

@@ -9,8 +9,8 @@
 //     region int32   region id (adaptive search projection only)
 //   cells, SoA int32: lx, ly (grid), w, h, code (= group + 2), gnd, ox, oy (dbu)
 //
-// One iteration = k_assign -> radix grouping (k_rx_*, k_part_bounds) ->
-// k_scan -> k_partition<PRIMAL> (sort + sequential cell moves + fused dual
+// One iteration = k_bucket (cells of the listed partitions into per-partition
+// buckets) -> k_scan -> k_partition<PRIMAL> (sort + sequential cell moves + fused dual
 // update / overflow / next activity flags) -> k_dual_pass -> k_finish.
 // Every kernel reads the iteration index and the done flag from Ctl in
 // device memory, so batches of iterations are launched without host syncs.
@@ -54,9 +54,9 @@ constexpr float kInf = __builtin_huge_valf();
     if (threadIdx.x == 0) {                                                    \
       atomicAdd(&a.ctl->phase_ticks[9], d_);                                   \
       atomicMax(&a.ctl->phase_ticks[10], d_);                                  \
-      if (MODE == MODE_PRIMAL && a.rx_passes <= 2) {                           \
-        int old_ = atomicMax(&a.rx_k2[iter], (int)d_);                         \
-        atomicAdd(&a.rx_v2[iter], (int)d_);                                    \
+      if (MODE == MODE_PRIMAL && iter < a.N) {                                 \
+        int old_ = atomicMax(&a.dbg_a[iter], (int)d_);                         \
+        atomicAdd(&a.dbg_b[iter], (int)d_);                                    \
         if ((int)d_ > old_ && 16 * iter + 15 < a.N) {                          \
           int* o_ = a.ovf_list + 16 * iter;                                    \
           o_[0] = (int)d_; o_[1] = ph_[0]; o_[2] = ph_[1]; o_[3] = ph_[2];     \
@@ -125,21 +125,16 @@ struct Args {
   const int32_t* k_height_num;
   int32_t* need_rot;
   // partitioning
-  int32_t* part_of;
-  int32_t* cnt;
-  int32_t* off;
-  int32_t* fill;
-  int32_t* grouped;
+  int32_t* bucket;     // [K * kMaxNpad] cells of each listed partition, in arrival order
+  int32_t* bcnt;       // [K] cells appended to each bucket this iteration (zeroed by k_scan)
+  int32_t* pcnt;       // [K] cells the partition kernel processes per partition
+  int32_t* off;        // [K + 2] exclusive scan of pcnt: where each partition's order is kept
+  int32_t* scratch;    // [K + 1] scan scratch of the final refine's grouping
+  int32_t* grouped;    // [N] cell ids in processing order, partition by partition
   int32_t* trace;
   int32_t* cellrec;    // [4N] cell records (int4), the positions' home during a solve
   int32_t* rec_b;      // [4N] cell records in processing order (int4)
-  // stable radix sort of the cells by partition: identity values, ping-pong
-  // key / value buffers, sorted keys, first / one-past-last slot per partition
-  const int32_t* ident;
-  int32_t *rx_k1, *rx_v1, *rx_k2, *rx_v2, *rx_keys;
-  int32_t* rx_hist;    // [128 * rx_nb]
-  int32_t *pstart, *pend;
-  int rx_nb, rx_passes;
+  int32_t *dbg_a, *dbg_b;  // [N] per-iteration timing of LG_PHASE_TIMERS builds
   int32_t* sorted;     // [K]  1 if the partition's order was sorted in the last primal step
   int32_t* work;       // [2K] partitions the primal step processes: class 0 (at most
                        //      kNpadSmall cells) from 0, class 1 from K; counts in Ctl::n_work
@@ -268,19 +263,16 @@ __global__ void k_fill_dual(Args a) {
 // per-iteration kernels
 // ---------------------------------------------------------------------------
 
-// assignPartitionKernel (AdmmKernels.hpp:122-162) + histogram
 // -- grouping of the cells by partition (updatePartition,
-// CudaAdmmLegalizer.cu:166-228): each cell's partition under this
-// iteration's fold offset (assignPartitionKernel, AdmmKernels.hpp:122-162;
-// K for a cell whose footprint spans two partitions), then a stable LSD radix
-// sort of (partition, cell id) in 7-bit digits, so that every partition lists
-// its cells in ascending id order (the order of thrust::stable_sort_by_key).
-// 4096 items per block of 256 threads.
-constexpr int kRxItems = 4096;
-constexpr int kRxThreads = 1024;
-constexpr int kRxPer = kRxItems / kRxThreads;  // items per thread
-constexpr int kRxWaves = kRxThreads / kWave;
-
+// CudaAdmmLegalizer.cu:166-228): a cell's partition under this iteration's
+// fold offset, or none if its footprint spans two (assignPartitionKernel,
+// AdmmKernels.hpp:122-162).  Only the partitions the primal step processes
+// need their cells: every partition in the perturbation iterations or while
+// a cell sits on a wrong-parity row, otherwise those whose activity flag is
+// set (a partition without a pixel above usage one moves nothing).  Their
+// cells are appended to a bucket of kMaxNpad slots per partition in arbitrary
+// order; the partition kernel sorts its bucket by cell id, the order of the
+// reference's thrust::stable_sort_by_key.  One thread per cell.
 __device__ __forceinline__ int cell_partition(const Args& a, const int* cm, const int* rm, int4 r) {
   const int x = r.x & 0xffff, y = (int)((unsigned)r.x >> 16);
   const int w = r.y & 0xfff, h = (r.y >> 12) & 0x3f;
@@ -288,58 +280,29 @@ __device__ __forceinline__ int cell_partition(const Args& a, const int* cm, cons
   return (c1 == c2 && r1 == r2) ? c1 * a.nrow + r1 : a.K;
 }
 
-// partition keys plus the histogram of their first digit
-__device__ __forceinline__ void assign_body(const Args& a, int bx) {
+__device__ __forceinline__ void append_to_bucket(const Args& a, int p, int c) {
+  const int s = atomicAdd(&a.bcnt[p], 1);
+  // a count above kMaxNpad makes k_scan stop the solve with an error
+  if (s < kMaxNpad) a.bucket[(size_t)p * kMaxNpad + s] = c;
+}
+
+__device__ __forceinline__ void bucket_body(const Args& a, int bx) {
   const Ctl* ctl = a.ctl;
   if (ctl->done) return;
-  __shared__ int cnt[128];
-  if (threadIdx.x < 128) cnt[threadIdx.x] = 0;
-  __syncthreads();
-  const int tf = triple_fold(a, ctl->iter);
-  const int* cm = a.colmap + tf * a.x_size;
-  const int* rm = a.rowmap + tf * a.y_size;
-  const int base = bx * kRxItems + threadIdx.x;
-  const int4* rec = reinterpret_cast<const int4*>(a.cellrec);
-  int4 r[kRxPer];
-#pragma unroll
-  for (int q = 0; q < kRxPer; q++) {
-    const int c = base + q * kRxThreads;
-    r[q] = c < a.N ? rec[c] : make_int4(0, 0, 0, 0);
-  }
-#pragma unroll
-  for (int q = 0; q < kRxPer; q++) {
-    const int c = base + q * kRxThreads;
-    if (c < a.N) {
-      const int p = cell_partition(a, cm, rm, r[q]);
-      a.part_of[c] = p;
-      atomicAdd(&cnt[p & 127], 1);
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x < 128) a.rx_hist[threadIdx.x * a.rx_nb + bx] = cnt[threadIdx.x];
+  const int c = bx * blockDim.x + threadIdx.x;
+  if (c >= a.N) return;
+  const int iter = ctl->iter;
+  // the adaptive search's iteration orders no partition (count 0)
+  if (a.use_adaptive && ctl->exceptional) return;
+  const bool skip_ok = iter >= a.iter_perturbation_on && ctl->wrong_row == 0;
+  const int tf = triple_fold(a, iter);
+  const int4 r = reinterpret_cast<const int4*>(a.cellrec)[c];
+  const int p = cell_partition(a, a.colmap + tf * a.x_size, a.rowmap + tf * a.y_size, r);
+  if (p >= a.K) return;
+  if (skip_ok && a.act_flags[(size_t)(iter & 1) * a.K + p] == 0) return;
+  append_to_bucket(a, p, c);
 }
-__global__ __launch_bounds__(kRxThreads) void k_assign(Args a) { assign_body(a, blockIdx.x); }
-
-// digit histogram of one block (passes after the first)
-__device__ __forceinline__ void rx_hist_body(const Args& a, int bx, const int32_t* keys, int shift) {
-  if (a.ctl->done) return;
-  __shared__ int cnt[128];
-  if (threadIdx.x < 128) cnt[threadIdx.x] = 0;
-  __syncthreads();
-  const int base = bx * kRxItems + threadIdx.x;
-  int k[kRxPer];
-#pragma unroll
-  for (int q = 0; q < kRxPer; q++) k[q] = base + q * kRxThreads < a.N ? keys[base + q * kRxThreads] : -1;
-#pragma unroll
-  for (int q = 0; q < kRxPer; q++)
-    if (k[q] >= 0) atomicAdd(&cnt[(k[q] >> shift) & 127], 1);
-  __syncthreads();
-  if (threadIdx.x < 128) a.rx_hist[threadIdx.x * a.rx_nb + bx] = cnt[threadIdx.x];
-}
-__global__ __launch_bounds__(kRxThreads) void k_rx_hist(Args a, const int32_t* keys, int shift) {
-  rx_hist_body(a, blockIdx.x, keys, shift);
-}
-
+__global__ __launch_bounds__(256) void k_bucket(Args a) { bucket_body(a, blockIdx.x); }
 
 // Exclusive scan of n ints in place by one workgroup of 1024, in chunks of
 // 32768: all loads of a chunk are issued before the first use (one memory
@@ -393,117 +356,7 @@ __device__ __forceinline__ void block_scan_inplace(int* v, int n) {
   }
 }
 
-__device__ __forceinline__ void rx_scan_body(const Args& a) {
-  if (a.ctl->done) return;
-  block_scan_inplace(a.rx_hist, 128 * a.rx_nb);
-}
-__global__ __launch_bounds__(1024) void k_rx_scan(Args a) { rx_scan_body(a); }
-
-// Stable scatter of one block: wave w owns items [w*256, (w+1)*256), 64 at a
-// time, and a lane's rank among the lanes with the same digit comes from
-// seven ballots.  Items are first placed in digit order in LDS, then written
-// out in runs of consecutive slots.
-__device__ __forceinline__ void rx_scatter_body(const Args& a, int bx, const int32_t* kin, const int32_t* vin,
-                                                int32_t* kout, int32_t* vout, int shift) {
-  if (a.ctl->done) return;
-  __shared__ int wcnt[kRxWaves][128];
-  __shared__ int wpos[kRxWaves][128];
-  __shared__ int dstart[128], gbase[128], dsum[128];
-  __shared__ int lkey[kRxItems], lval[kRxItems];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for (int d = threadIdx.x; d < kRxWaves * 128; d += kRxThreads) wcnt[d >> 7][d & 127] = 0;
-  __syncthreads();
-  constexpr int CH = kRxItems / kRxWaves / kWave;  // chunks of 64 per wave
-  const int base = bx * kRxItems + w * (kRxItems / kRxWaves);
-  int key[CH], val[CH];
-#pragma unroll
-  for (int c = 0; c < CH; c++) {
-    const int idx = base + c * kWave + lane;
-    key[c] = idx < a.N ? kin[idx] : -1;
-    val[c] = idx < a.N ? vin[idx] : 0;
-  }
-#pragma unroll
-  for (int c = 0; c < CH; c++)
-    if (key[c] >= 0) atomicAdd(&wcnt[w][(key[c] >> shift) & 127], 1);
-  __syncthreads();
-  if (threadIdx.x < 128) {
-    // block-local start of each digit: inclusive scan of the digit totals
-    // within each of the two waves holding the 128 digits
-    const int d = threadIdx.x;
-    int tot = 0;
-    for (int q = 0; q < kRxWaves; q++) tot += wcnt[q][d];
-    int incl = tot;
-    for (int o = 1; o < kWave; o <<= 1) {
-      int y = __shfl_up(incl, o);
-      if (lane >= o) incl += y;
-    }
-    dsum[d] = incl;
-    dstart[d] = incl - tot;
-  }
-  __syncthreads();
-  if (threadIdx.x < 128) {
-    const int d = threadIdx.x;
-    const int st = dstart[d] + (d >= kWave ? dsum[kWave - 1] : 0);
-    int run = st;
-    for (int q = 0; q < kRxWaves; q++) {
-      wpos[q][d] = run;
-      run += wcnt[q][d];
-    }
-    dstart[d] = st;
-    gbase[d] = a.rx_hist[d * a.rx_nb + bx];
-  }
-  __syncthreads();
-  const unsigned long long lt = (1ull << lane) - 1ull;
-#pragma unroll
-  for (int c = 0; c < CH; c++) {
-    const bool valid = key[c] >= 0;
-    const int d = (key[c] >> shift) & 127;
-    unsigned long long peers = __ballot(valid);
-#pragma unroll
-    for (int bit = 0; bit < 7; bit++) {
-      const unsigned long long m = __ballot((d >> bit) & 1);
-      peers &= ((d >> bit) & 1) ? m : ~m;
-    }
-    const int rank = __popcll(peers & lt);
-    const int lpos = valid ? wpos[w][d] + rank : 0;
-    if (valid) {
-      lkey[lpos] = key[c];
-      lval[lpos] = val[c];
-    }
-    // the digit group's last lane advances the wave's slot (LDS operations
-    // of one wave complete in order)
-    if (valid && rank == __popcll(peers) - 1) wpos[w][d] = lpos + 1;
-  }
-  __syncthreads();
-  const int nitem = min(kRxItems, a.N - bx * kRxItems);
-  for (int j = threadIdx.x; j < nitem; j += kRxThreads) {
-    const int k = lkey[j];
-    const int d = (k >> shift) & 127;
-    const int g = gbase[d] + (j - dstart[d]);
-    kout[g] = k;
-    vout[g] = lval[j];
-  }
-}
-__global__ __launch_bounds__(kRxThreads) void k_rx_scatter(Args a, const int32_t* kin, const int32_t* vin,
-                                                           int32_t* kout, int32_t* vout, int shift) {
-  rx_scatter_body(a, blockIdx.x, kin, vin, kout, vout, shift);
-}
-
-
-// first / one-past-last sorted slot of every non-empty partition
-__device__ __forceinline__ void part_bounds_body(const Args& a, int bx) {
-  if (a.ctl->done) return;
-  const int i = bx * blockDim.x + threadIdx.x;
-  if (i >= a.N) return;
-  const int* k = a.rx_keys;
-  const int p = k[i];
-  if (p >= a.K) return;
-  if (i == 0 || k[i - 1] != p) a.pstart[p] = i;
-  if (i == a.N - 1 || k[i + 1] != p) a.pend[p] = i + 1;
-}
-__global__ void k_part_bounds(Args a) { part_bounds_body(a, blockIdx.x); }
-
-// Exclusive scan of the partition histogram, max count -> npad, dual step
+// Exclusive scan of the partition cell counts, max count -> npad, dual step
 // schedule (updateDualStepSize, CudaAdmmLegalizer.cu:293-294, 687-691), and
 // the work lists of the primal step: every partition except those whose
 // activity flag says the step is a no-op, split by sort-size class.  A
@@ -525,26 +378,27 @@ __device__ __forceinline__ void scan_body(const Args& a) {
   const unsigned long long lt = (1ull << lane) - 1ull;
   if (t < 2) s_n[t] = 0;
   __syncthreads();
-  // cells per partition from the sorted bounds (off <- counts, scanned
-  // below); eight partitions per thread with every load issued first
+  // cells per partition from the bucket counts (off <- counts, scanned
+  // below; the counters are cleared for the next iteration); eight
+  // partitions per thread with every load issued first
   int mx = 0;
   for (int s0 = 0; s0 < K; s0 += 8 * 1024) {
-    int ps[8], pe[8], fl[8];
+    int cn[8], fl[8];
 #pragma unroll
     for (int j = 0; j < 8; j++) {
       const int i = s0 + j * 1024 + t;
-      ps[j] = i < K ? a.pstart[i] : -1;
-      pe[j] = i < K ? a.pend[i] : 0;
+      cn[j] = i < K ? a.bcnt[i] : 0;
       fl[j] = (i < K && skip_ok) ? cf[i] : 1;
     }
 #pragma unroll
     for (int j = 0; j < 8; j++) {
       const int i = s0 + j * 1024 + t;
       const bool in = i < K;
-      const int c = ps[j] >= 0 ? pe[j] - ps[j] : 0;
+      const int c = cn[j];
       if (in) {
         a.off[i] = c;
-        a.pstart[i] = -1;
+        a.pcnt[i] = c;
+        a.bcnt[i] = 0;
       }
       mx = max(mx, c);
       const bool run = in && fl[j] != 0;
@@ -593,6 +447,54 @@ __device__ __forceinline__ void scan_body(const Args& a) {
   }
 }
 __global__ __launch_bounds__(1024) void k_scan(Args a) { scan_body(a); }
+
+// Grouping for the final refine, which visits every partition under the
+// fold offset of the converged iteration.  A partition ordered by the last
+// primal step keeps that order (grouped, off, pcnt as it left them); the
+// cells of every other partition go to its bucket, and its order is placed
+// behind all kept ones.
+__global__ __launch_bounds__(256) void k_refine_bucket(Args a) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.N) return;
+  const int tf = triple_fold(a, a.ctl->conv_iter);
+  const int4 r = reinterpret_cast<const int4*>(a.cellrec)[c];
+  const int p = cell_partition(a, a.colmap + tf * a.x_size, a.rowmap + tf * a.y_size, r);
+  if (p < a.K && a.sorted[p] == 0) append_to_bucket(a, p, c);
+}
+
+// off / pcnt of the refine's grouping and npad = pow2(largest partition).
+// One workgroup of 1024 threads.
+__global__ __launch_bounds__(1024) void k_refine_scan(Args a) {
+  Ctl* ctl = a.ctl;
+  __shared__ int s_max[16];
+  const int K = a.K, t = threadIdx.x;
+  int mx = 0;
+  for (int i = t; i < K; i += 1024) {
+    const int c = a.sorted[i] ? 0 : a.bcnt[i];
+    a.scratch[i] = c;
+    mx = max(mx, a.sorted[i] ? a.pcnt[i] : c);
+  }
+  for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o));
+  if ((t & 63) == 0) s_max[t >> 6] = mx;
+  __syncthreads();
+  const int base = a.off[K];  // end of the kept orders
+  block_scan_inplace(a.scratch, K);
+  for (int i = t; i < K; i += 1024) {
+    if (!a.sorted[i]) {
+      a.off[i] = base + a.scratch[i];
+      a.pcnt[i] = a.bcnt[i];
+    }
+    a.bcnt[i] = 0;
+  }
+  if (t == 0) {
+    int m = 0;
+    for (int q = 0; q < 16; q++) m = max(m, s_max[q]);
+    int np = 1;
+    while (np < m) np <<= 1;
+    ctl->npad = np;
+    if (np > kMaxNpad) ctl->error = kErrPartitionTooDense;
+  }
+}
 
 
 // ---------------------------------------------------------------------------
@@ -894,6 +796,18 @@ __device__ __forceinline__ void sum_rect(const float* t, const int (&base)[4], i
   }
 }
 
+// One step of a wave minimum of 64-bit keys hi:lo held in two registers:
+// each lane takes the smaller of its key and the key DPP control CTRL brings
+// it (lanes the control does not write keep their own).
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ void key_min_step(unsigned& hi, unsigned& lo) {
+  const unsigned ohi = (unsigned)__builtin_amdgcn_update_dpp((int)hi, (int)hi, CTRL, ROWMASK, 0xf, false);
+  const unsigned olo = (unsigned)__builtin_amdgcn_update_dpp((int)lo, (int)lo, CTRL, ROWMASK, 0xf, false);
+  const bool take = ohi < hi || (ohi == hi && olo < lo);
+  hi = take ? ohi : hi;
+  lo = take ? olo : lo;
+}
+
 // The 4 candidates of one lane (moves d = j*64 + lane), evaluated together so
 // that their LDS reads overlap.  cost = computeDispCost + computePenaltySum
 // (partition_exclusive, AdmmKernels.hpp:333-391); pin-access and edge-spacing
@@ -905,8 +819,50 @@ template <int MODE>
 __device__ __forceinline__ void eval_candidates(const Args& a, const PartGeo& g, const Tile& T,
                                                 float disp_coeff, float thr, int cx, int cy,
                                                 int w, int h, int code, int gnd, int ox, int oy,
-                                                const int2 dir[4], float cst[4], int ucode) {
+                                                const int2 dir[4], float cst[4], int ucode,
+                                                unsigned long long vmask) {
   const int lane = threadIdx.x;
+  const bool flat = g.ncol > 1 && g.nrow > 1;  // footprints inside one interval never wrap
+  if (MODE == MODE_PRIMAL && ucode >= 0 && flat && T.rs == kRowStride && h <= 4 && g.yh <= 64) {
+    // The common case without branches: a tile of one placement code, a
+    // footprint of at most four rows.  The clamp of getNewCellPosition keeps
+    // a candidate on the grid (cells fit the grid), and -- the partition's
+    // two intervals per axis lying at least a partition apart -- a footprint
+    // is inside the partition iff its local corner l (lx() / ly()) has
+    // 0 <= l <= xh - w (resp. yh - h).  The rail of the landing row comes
+    // from the mask.  Same arithmetic as the general walk below.
+    const int eps = (h % 2 == 0) ? 2 : 1;
+    const bool wide = w > g.xh || h > g.yh || code != ucode;
+    int base[4];
+    float s4[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int nx = max(0, min(a.x_size - w, cx + dir[j].x));
+      const int ny = max(0, min(a.y_size - h, cy + eps * dir[j].y));
+      cst[j] = disp_cost(a, disp_coeff, thr, nx, ny, ox, oy);
+      int lx = nx - g.xa0;
+      lx -= lx >= g.xh ? g.ncol * g.xh : 0;
+      int ly = ny - g.ya0;
+      ly -= ly >= g.yh ? g.nrow * g.yh : 0;
+      const bool in = (lx >= 0) & (lx + w <= g.xh) & (ly >= 0) & (ly + h <= g.yh);
+      const int rail = (int)((vmask >> (unsigned)(ly & 63)) & 1ull);
+      const bool ok = in & !wide & (j * kWave + lane < kNumDirDefault) & (((h & 1) != 0) | (rail == gnd));
+      base[j] = ok ? ly * kRowStride + lx : -1;
+      s4[j] = 0.0f;
+    }
+    int bz[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) bz[j] = base[j] < 0 ? 0 : base[j];
+    switch (h) {
+      case 1: sum_rect<1>(T.t, bz, w, s4); break;
+      case 2: sum_rect<2>(T.t, bz, w, s4); break;
+      case 3: sum_rect<3>(T.t, bz, w, s4); break;
+      default: sum_rect<4>(T.t, bz, w, s4); break;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) cst[j] = base[j] < 0 ? kInf : cst[j] + s4[j];
+    return;
+  }
   int lxi[4], lyi0[4];
   int bad[4];  // 0 / 1 (ints: bool arrays get packed into bytes)
   float s[4];
@@ -933,7 +889,6 @@ __device__ __forceinline__ void eval_candidates(const Args& a, const PartGeo& g,
       bad[j] = ok ? 0 : 1;
     }
   }
-  const bool flat = g.ncol > 1 && g.nrow > 1;  // footprints inside one interval never wrap
   if (ucode >= 0 && flat && T.rs == kRowStride && h <= 4) {
     // every in-grid pixel of the tile carries code `ucode`: the code test is
     // one comparison, and the walk only sums the penalty terms (a bad
@@ -1168,8 +1123,8 @@ __device__ __forceinline__ void net_sort(int n, int (&key)[RN], int (&slot)[RN])
 // a loop that is not unrolled, so the kernel holds one copy of the overflow
 // sums per instance.
 template <int RN>
-__device__ __forceinline__ bool order_partition_reg(const Args& a, const PartGeo& g, const Tile& T, int npad, int off,
-                                                    int count, const int4* cellrec
+__device__ __forceinline__ bool order_partition_reg(const Args& a, const PartGeo& g, const Tile& T, int npad,
+                                                    const int* src, int off, int count, const int4* cellrec
 #ifdef LG_PHASE_TIMERS
                                                     , int* ph_
 #endif
@@ -1180,10 +1135,39 @@ __device__ __forceinline__ bool order_partition_reg(const Args& a, const PartGeo
 #endif
   int* ids = a.grouped + off;
   int4* recb = reinterpret_cast<int4*>(a.rec_b) + off;
-  // loads at a clamped slot, selected afterwards (slots >= count are pads)
+  // the cells in ascending id order (the input order of the reference's
+  // sort): the source ids through a descending network on -id (pads
+  // INT_MIN sort last), stored to ids[]; loads at a clamped slot, selected
+  // afterwards (slots >= count are pads)
   int val[RN];
+  {
+    int kk[RN], ss[RN];
 #pragma unroll
-  for (int r = 0; r < RN; r++) val[r] = ids[min(lane * RN + r, count - 1)];
+    for (int r = 0; r < RN; r++) {
+      const int i = lane * RN + r;
+      const int v = src[min(i, count - 1)];
+      kk[r] = i < count ? -v : INT_MIN;
+      ss[r] = 0;
+    }
+    net_sort<RN, false>(npad, kk, ss);
+    // pads take the last cell's id (slot count - 1)
+    const int ls = count - 1, ll = ls / RN, lr = ls - ll * RN;
+    int last = 0;
+#pragma unroll
+    for (int q = 0; q < RN; q++) {
+      const int v = __shfl(-kk[q], ll);
+      if (q == lr) last = v;
+    }
+#pragma unroll
+    for (int r = 0; r < RN; r++) val[r] = lane * RN + r < count ? -kk[r] : last;
+    if (RN > 8) {
+      // the wide instance gathers by slot through memory
+#pragma unroll
+      for (int r = 0; r < RN; r++)
+        if (lane * RN + r < count) ids[lane * RN + r] = val[r];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    }
+  }
   int4 rc[RN];
 #pragma unroll
   for (int r = 0; r < RN; r++) rc[r] = cellrec[val[r]];
@@ -1249,9 +1233,26 @@ __device__ __forceinline__ bool order_partition_reg(const Args& a, const PartGeo
   unsigned long long t2_ = wall_clock64();
   ph_[6] = (int)(t2_ - t1_);
 #endif
-  // pads sort behind every cell, so slots < count map to cells
+  // pads sort behind every cell, so slots < count map to cells; the id of
+  // slot s lives in register s % RN of lane s / RN
+  if (RN <= 8) {
+    int nv[RN];
 #pragma unroll
-  for (int r = 0; r < RN; r++) val[r] = ids[min(slot[r], count - 1)];
+    for (int r = 0; r < RN; r++) {
+      const int sl = min(slot[r], count - 1), sr = sl - (sl / RN) * RN;
+      nv[r] = 0;
+#pragma unroll
+      for (int q = 0; q < RN; q++) {
+        const int v = __shfl(val[q], sl / RN);
+        if (q == sr) nv[r] = v;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < RN; r++) val[r] = nv[r];
+  } else {
+#pragma unroll
+    for (int r = 0; r < RN; r++) val[r] = ids[min(slot[r], count - 1)];
+  }
 #pragma unroll
   for (int r = 0; r < RN; r++) rc[r] = cellrec[val[r]];
 #pragma unroll
@@ -1270,13 +1271,22 @@ __device__ __forceinline__ bool order_partition_reg(const Args& a, const PartGeo
 
 // The same with the sorts in LDS (npad > 1024, or the merge / radix options).
 __device__ void order_partition_lds(const Args& a, const PartGeo& g, const Tile& T, int npad, int npad_hi,
-                                    int off, int count, const int4* cellrec, int* keys, int* slots, int* aux) {
+                                    const int* src, int off, int count, const int4* cellrec, int* keys, int* slots,
+                                    int* aux) {
   const int lane = threadIdx.x;
   int* ids = a.grouped + off;
   int4* recb = reinterpret_cast<int4*>(a.rec_b) + off;
+  // the cells in ascending id order (the input order of the reference's
+  // sort): a descending network on -id, pads INT_MIN last
+  for (int i = lane; i < npad; i += kWave) {
+    keys[i] = i < count ? -src[i] : INT_MIN;
+    slots[i] = 0;
+  }
+  __syncthreads();
+  lds_bitonic_desc(npad, keys, slots);
   for (int i = lane; i < npad; i += kWave) {
     if (i < count) {
-      int c = ids[i];
+      int c = -keys[i];
       keys[i] = sort_key(a, g, T, c, cellrec[c]);
       slots[i] = c;
     } else {
@@ -1347,7 +1357,7 @@ __device__ __forceinline__ void partition_body(const Args& a, int bx, int npad_l
   const bool adaptive_now = (MODE == MODE_PRIMAL) && a.use_adaptive && ctl->exceptional;
   const int iter = MODE == MODE_PRIMAL ? ctl->iter : ctl->conv_iter;
   const int off = a.off[k];
-  const int count = adaptive_now ? 0 : a.off[k + 1] - off;
+  const int count = adaptive_now ? 0 : a.pcnt[k];
   // Sort size of this partition.  The reference sorts every partition on
   // npad = pow2(largest partition) slots, padding with INT_MIN; a bitonic
   // network whose real keys fill the first c slots sorts them exactly as the
@@ -1375,7 +1385,11 @@ __device__ __forceinline__ void partition_body(const Args& a, int bx, int npad_l
   const float neg = -1.0f / rho;
   const bool pert = (MODE == MODE_PRIMAL) && iter < a.iter_perturbation_on;
   const bool standard = (MODE == MODE_PRIMAL) && a.standard_admm;
-  int* ids = a.grouped + off;
+  int* ids = a.grouped + off;  // the processing order, once sorted
+  // the partition's cells: its bucket (arbitrary order), or the order the
+  // last primal step left for the final refine
+  const bool from_bucket = MODE == MODE_PRIMAL || a.sorted[k] == 0;
+  const int* src = from_bucket ? a.bucket + (size_t)k * kMaxNpad : ids;
 
   // -- activity test: does any cell of this partition need a move?  With
   // every cell on a correct-parity row, a partition whose pixels all have
@@ -1392,7 +1406,7 @@ __device__ __forceinline__ void partition_body(const Args& a, int bx, int npad_l
   if (cell_test) {
     bool need = false;
     for (int i = lane; i < count; i += kWave) {
-      int4 rc = cellrec[ids[i]];
+      int4 rc = cellrec[src[i]];
       int cx = rec_x(rc.x), cy = rec_y(rc.x), h = rec_h(rc.y);
       need |= !correct_row(h, rec_gnd(rc.y), a.vdd_up[cy]) || ovf_hbm(a, cx, cy, rec_w(rc.y), h) != 0;
     }
@@ -1466,21 +1480,22 @@ __device__ __forceinline__ void partition_body(const Args& a, int bx, int npad_l
   }
   const int ucode = (MODE == MODE_PRIMAL && cmin == cmax && cmin > 0) ? cmin : -1;
   __syncthreads();
+  // rail of every tile row as one mask (rows < 64), for the candidate test
+  const unsigned long long vmask = __ballot(lane < g.yh && T.vdd[lane < g.yh ? lane : 0] != 0);
   PHASE_MARK(0);
   PHASE_COUNT(8);
 
   if (count > 0) {
     int4* recs = reinterpret_cast<int4*>(a.rec_b) + off;
-    const bool do_order = (MODE == MODE_PRIMAL) || a.sorted[k] == 0;
-    if (do_order) {
+    if (from_bucket) {
       // an unsorted partition at the final refine moved nothing in the last
       // iteration, so its usage still is the one the order is defined on
-      if (LDS_SORT) order_partition_lds(a, g, T, npad, npad_hi, off, count, cellrec, keys, slots, aux);
+      if (LDS_SORT) order_partition_lds(a, g, T, npad, npad_hi, src, off, count, cellrec, keys, slots, aux);
       else {
 #ifdef LG_PHASE_TIMERS
-#define ORDER_REG(RN) order_partition_reg<RN>(a, g, T, npad, off, count, cellrec, ph_)
+#define ORDER_REG(RN) order_partition_reg<RN>(a, g, T, npad, src, off, count, cellrec, ph_)
 #else
-#define ORDER_REG(RN) order_partition_reg<RN>(a, g, T, npad, off, count, cellrec)
+#define ORDER_REG(RN) order_partition_reg<RN>(a, g, T, npad, src, off, count, cellrec)
 #endif
         // npad / 64 registers per lane, at least four; keys too wide to
         // pack go to the instance of 16 registers
@@ -1543,7 +1558,7 @@ __device__ __forceinline__ void partition_body(const Args& a, int bx, int npad_l
         __syncthreads();
         SUB_MARK(11);
         float cst[4];
-        eval_candidates<MODE>(a, g, T, disp_coeff, thr, cx, cy, w, h, code, gnd, pox, poy, dir, cst, ucode);
+        eval_candidates<MODE>(a, g, T, disp_coeff, thr, cx, cy, w, h, code, gnd, pox, poy, dir, cst, ucode, vmask);
         SUB_MARK(12);
         // Tree-min of AdmmKernels.hpp:847-858 over slots s = j*64 + lane: a
         // halving-stride tournament in which a tie keeps the lower slot.  Its
@@ -1559,15 +1574,17 @@ __device__ __forceinline__ void partition_body(const Args& a, int bx, int npad_l
           unsigned long long key = ((unsigned long long)__float_as_uint(cst[j]) << 32) | rev;
           best = key < best ? key : best;
         }
-#pragma unroll
-        for (int o = 1; o < kWave; o <<= 1) {
-          unsigned lo = (unsigned)best, hi = (unsigned)(best >> 32);
-          unsigned olo = __shfl_xor(lo, o), ohi = __shfl_xor(hi, o);
-          unsigned long long other = ((unsigned long long)ohi << 32) | olo;
-          best = other < best ? other : best;
-        }
-        const unsigned bc_bits = __builtin_amdgcn_readfirstlane((unsigned)(best >> 32));
-        const int bi = (int)(__builtin_amdgcn_readfirstlane((unsigned)best & 0xffu));
+        // minimum over the wave by DPP steps (keys are unique, so the order
+        // of the steps does not matter); lane 63 ends with it
+        unsigned bhi = (unsigned)(best >> 32), blo = (unsigned)best;
+        key_min_step<0xB1, 0xf>(bhi, blo);   // quad_perm [1,0,3,2]
+        key_min_step<0x4E, 0xf>(bhi, blo);   // quad_perm [2,3,0,1]
+        key_min_step<0x141, 0xf>(bhi, blo);  // row_half_mirror
+        key_min_step<0x140, 0xf>(bhi, blo);  // row_mirror
+        key_min_step<0x142, 0xa>(bhi, blo);  // row_bcast:15 into rows 1, 3
+        key_min_step<0x143, 0xc>(bhi, blo);  // row_bcast:31 into rows 2, 3
+        const unsigned bc_bits = __builtin_amdgcn_readlane(bhi, 63);
+        const int bi = (int)(__builtin_amdgcn_readlane(blo, 63) & 0xffu);
         const int bslot = (int)(__brev((unsigned)bi) >> 24);
         int nx = cx, ny = cy;
         if (__uint_as_float(bc_bits) != kInf) {
@@ -1982,56 +1999,9 @@ __global__ void k_cell_stats(Args a) {
   const int2 m_ = map[blockIdx.x];                                             \
   const Args& a = tab[m_.x]
 
-// input / output buffers of radix pass `pass` (the solo loop's ping-pong)
-__device__ __forceinline__ void rx_pass_ptrs(const Args& a, int pass, const int32_t*& kin, const int32_t*& vin,
-                                             int32_t*& kout, int32_t*& vout) {
-  const bool last = pass == a.rx_passes - 1;
-  kout = last ? a.rx_keys : (pass & 1 ? a.rx_k2 : a.rx_k1);
-  vout = last ? a.grouped : (pass & 1 ? a.rx_v2 : a.rx_v1);
-  if (pass == 0) {
-    kin = a.part_of;
-    vin = a.ident;
-  } else {
-    kin = pass & 1 ? a.rx_k1 : a.rx_k2;
-    vin = pass & 1 ? a.rx_v1 : a.rx_v2;
-  }
-}
-
-__global__ __launch_bounds__(kRxThreads) void kb_assign(const Args* __restrict__ tab, const int2* __restrict__ map) {
+__global__ __launch_bounds__(256) void kb_bucket(const Args* __restrict__ tab, const int2* __restrict__ map) {
   LG_BATCH_SLOT();
-  assign_body(a, m_.y);
-}
-
-__global__ __launch_bounds__(kRxThreads) void kb_rx_hist(const Args* __restrict__ tab, const int2* __restrict__ map,
-                                                         int pass) {
-  LG_BATCH_SLOT();
-  if (pass >= a.rx_passes) return;
-  const int32_t *kin, *vin;
-  int32_t *kout, *vout;
-  rx_pass_ptrs(a, pass, kin, vin, kout, vout);
-  rx_hist_body(a, m_.y, kin, 7 * pass);
-}
-
-__global__ __launch_bounds__(1024) void kb_rx_scan(const Args* __restrict__ tab, const int2* __restrict__ map,
-                                                   int pass) {
-  LG_BATCH_SLOT();
-  if (pass >= a.rx_passes) return;
-  rx_scan_body(a);
-}
-
-__global__ __launch_bounds__(kRxThreads) void kb_rx_scatter(const Args* __restrict__ tab,
-                                                            const int2* __restrict__ map, int pass) {
-  LG_BATCH_SLOT();
-  if (pass >= a.rx_passes) return;
-  const int32_t *kin, *vin;
-  int32_t *kout, *vout;
-  rx_pass_ptrs(a, pass, kin, vin, kout, vout);
-  rx_scatter_body(a, m_.y, kin, vin, kout, vout, 7 * pass);
-}
-
-__global__ void kb_part_bounds(const Args* __restrict__ tab, const int2* __restrict__ map) {
-  LG_BATCH_SLOT();
-  part_bounds_body(a, m_.y);
+  bucket_body(a, m_.y);
 }
 
 __global__ __launch_bounds__(1024) void kb_scan(const Args* __restrict__ tab, const int2* __restrict__ map) {

@@ -9,7 +9,8 @@
 //      request's device buffers fit the budget (pool of device memory);
 //      upload them and run their solve set-up (rho, lambda0, duals, flags);
 //   2. when the set of resident requests changed, rebuild the slot table
-//      (one Args per slot) and the block maps of the batched launches;
+//      (one Args per slot) and the block maps of the batched launches
+//      (256-cell blocks, one block per slot, per partition, pixel blocks);
 //   3. launch `iters_per_sync` ADMM iterations, each as one batched launch
 //      per step covering every resident request (kb_* kernels);
 //   4. copy every slot's Ctl to the host in one transfer, synchronise, and
@@ -70,8 +71,8 @@ struct LgServer {
   std::vector<int> active;  // resident slots, in admission order
   bool dirty = true;
   // geometry of the current tables
-  MapRange m4k, m256, m1, mk, mpix;
-  int max_passes = 0, any_adaptive = 0, any_reg = 0, any_nonreg = 0, top_slot = 0;
+  MapRange m256, m1, mk, mpix;
+  int any_adaptive = 0, any_reg = 0, any_nonreg = 0, top_slot = 0;
   size_t lds_reg = 0, lds_small = 0, lds_big = 0;
   // queue and request registry (guarded by mu)
   std::mutex mu;
@@ -112,8 +113,7 @@ int srv_grow_map(LgServer* S, size_t need) {
 
 // Slot table and block maps of the resident requests (server stream idle).
 int srv_build_tables(LgServer* S) {
-  std::vector<int2> m[5];
-  S->max_passes = 0;
+  std::vector<int2> m[4];
   S->any_adaptive = S->any_reg = S->any_nonreg = 0;
   S->lds_reg = S->lds_small = S->lds_big = 0;
   S->top_slot = 0;
@@ -122,13 +122,11 @@ int srv_build_tables(LgServer* S) {
     const Args& a = h->a;
     S->h_tab[sl] = a;
     S->top_slot = std::max(S->top_slot, sl + 1);
-    for (int b = 0; b < a.rx_nb; b++) m[0].push_back(make_int2(sl, b));
-    for (int b = 0; b < (a.N + 255) / 256; b++) m[1].push_back(make_int2(sl, b));
-    m[2].push_back(make_int2(sl, 0));
-    for (int b = 0; b < a.K; b++) m[3].push_back(make_int2(sl, b));
+    for (int b = 0; b < (a.N + 255) / 256; b++) m[0].push_back(make_int2(sl, b));
+    m[1].push_back(make_int2(sl, 0));
+    for (int b = 0; b < a.K; b++) m[2].push_back(make_int2(sl, b));
     const int gx = (a.x_size + 1023) / 1024;
-    for (int b = 0; b < gx * a.y_size; b++) m[4].push_back(make_int2(sl, b));
-    S->max_passes = std::max(S->max_passes, a.rx_passes);
+    for (int b = 0; b < gx * a.y_size; b++) m[3].push_back(make_int2(sl, b));
     S->any_adaptive |= a.use_adaptive ? 1 : 0;
     const int reg = h->p.sorting_algorithm == LG_SORT_M_BITONIC || h->p.sorting_algorithm == LG_SORT_S_BITONIC;
     if (reg) {
@@ -141,8 +139,8 @@ int srv_build_tables(LgServer* S) {
     S->lds_big = std::max(S->lds_big, partition_lds_bytes(h, kMaxNpad, 1));
   }
   size_t total = 0;
-  MapRange* r[5] = {&S->m4k, &S->m256, &S->m1, &S->mk, &S->mpix};
-  for (int i = 0; i < 5; i++) {
+  MapRange* r[4] = {&S->m256, &S->m1, &S->mk, &S->mpix};
+  for (int i = 0; i < 4; i++) {
     r[i]->off = (int)total;
     r[i]->n = (int)m[i].size();
     total += m[i].size();
@@ -150,7 +148,7 @@ int srv_build_tables(LgServer* S) {
   if (total > (size_t)INT_MAX) return fail(LG_ERR_MEMORY, "request server: block maps too large");
   int st = srv_grow_map(S, total);
   if (st) return st;
-  for (int i = 0; i < 5; i++)
+  for (int i = 0; i < 4; i++)
     if (!m[i].empty()) std::memcpy(S->h_map + r[i]->off, m[i].data(), m[i].size() * sizeof(int2));
   HIP_TRY(hipMemcpyAsync(S->d_tab, S->h_tab, S->top_slot * sizeof(Args), hipMemcpyHostToDevice, S->stream));
   HIP_TRY(hipMemcpyAsync(S->d_map, S->h_map, total * sizeof(int2), hipMemcpyHostToDevice, S->stream));
@@ -165,15 +163,7 @@ int srv_launch_iteration(LgServer* S, int it) {
   const Args* tab = S->d_tab;
   const int2* mp = S->d_map;
   auto at = [&](const MapRange& r) { return mp + r.off; };
-  if (S->m4k.n > 0) {
-    hipLaunchKernelGGL(kb_assign, dim3(S->m4k.n), dim3(kRxThreads), 0, s, tab, at(S->m4k));
-    for (int pass = 0; pass < S->max_passes; pass++) {
-      if (pass > 0) hipLaunchKernelGGL(kb_rx_hist, dim3(S->m4k.n), dim3(kRxThreads), 0, s, tab, at(S->m4k), pass);
-      hipLaunchKernelGGL(kb_rx_scan, dim3(S->m1.n), dim3(1024), 0, s, tab, at(S->m1), pass);
-      hipLaunchKernelGGL(kb_rx_scatter, dim3(S->m4k.n), dim3(kRxThreads), 0, s, tab, at(S->m4k), pass);
-    }
-  }
-  if (S->m256.n > 0) hipLaunchKernelGGL(kb_part_bounds, dim3(S->m256.n), dim3(256), 0, s, tab, at(S->m256));
+  if (S->m256.n > 0) hipLaunchKernelGGL(kb_bucket, dim3(S->m256.n), dim3(256), 0, s, tab, at(S->m256));
   hipLaunchKernelGGL(kb_scan, dim3(S->m1.n), dim3(1024), 0, s, tab, at(S->m1));
   if (S->any_adaptive) {
     if (S->m256.n > 0)
