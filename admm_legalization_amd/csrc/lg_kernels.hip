@@ -673,7 +673,8 @@ enum { MODE_PRIMAL = 0, MODE_REFINE = 1 };
 // (AdmmKernels.hpp:380-383):
 //   t = rho * max(lam * (u - 1) + lam / rho, 0)     (negotiation ADMM)
 //   t = rho * max(u - 1 + lam / rho, 0)             (standard ADMM)
-// t is recomputed, with the same operations, whenever u changes.
+// or +inf on a pixel of code 0, which no cell may use.  t is recomputed,
+// with the same operations, whenever u changes.
 constexpr int kRowStride = 256;
 struct Tile {
   uint32_t* uc;
@@ -757,7 +758,7 @@ __device__ __forceinline__ void footprint_add(const PartGeo& g, const Tile& T, i
     const int idx = T.at(lxi, lyi);
     const uint32_t e = (uint32_t)((int)T.uc[idx] + delta);
     T.uc[idx] = e;
-    T.t[idx] = pen_term((int)(e & 0xffffu), T.lam[idx], rho, standard);
+    T.t[idx] = (e >> 16) == 0u ? kInf : pen_term((int)(e & 0xffffu), T.lam[idx], rho, standard);
   }
 }
 
@@ -1012,7 +1013,7 @@ __device__ __forceinline__ bool stage_tile(const Args& a, const PartGeo& g, cons
 #pragma unroll
     for (int q = 0; q < U; q++) {
       int i = i0 + q * kWave + lane;
-      if (in[q]) {
+      if (in[q] && code[q] > 0) {
         cmin = min(cmin, code[q]);
         cmax = max(cmax, code[q]);
       }
@@ -1020,7 +1021,9 @@ __device__ __forceinline__ bool stage_tile(const Args& a, const PartGeo& g, cons
         range_err |= u[q] > 0xffff;
         over |= u[q] > 1;
         T.uc[li[q]] = ((uint32_t)code[q] << 16) | (uint32_t)(u[q] & 0xffff);
-        T.t[li[q]] = pen_term(u[q], dl[q], rho, standard);
+        // a pixel no cell may use (code 0) carries +inf: any footprint over
+        // it sums to +inf, the cost of an invalid candidate
+        T.t[li[q]] = code[q] == 0 ? kInf : pen_term(u[q], dl[q], rho, standard);
         T.lam[li[q]] = dl[q];
       }
     }
@@ -1472,8 +1475,9 @@ __device__ __forceinline__ void partition_body(const Args& a, int bx, int npad_l
     int yy = g.gy(b);
     T.vdd[b] = yy >= 0 ? a.vdd_up[yy] : 0;
   }
-  // one placeable code on every in-grid pixel of the tile: candidate
-  // evaluation then needs no per-pixel code test
+  // one placeable code on the tile's placeable pixels (the others carry
+  // +inf penalty terms): candidate evaluation then needs no per-pixel code
+  // test
   for (int o = 1; o < kWave; o <<= 1) {
     cmin = min(cmin, __shfl_xor(cmin, o));
     cmax = max(cmax, __shfl_xor(cmax, o));
