@@ -221,6 +221,8 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic_from_profiles(args.workload),
+            "hbm_measured_GBs": (traffic_from_profiles(args.workload) / avg_s / 1e9
+                                 if traffic_from_profiles(args.workload) else None),
             "avg_launch_us": avg_s * 1e6,
             "launches": int(launches[c]),
             "algorithmic_bytes_per_launch": bytes_per_launch,
@@ -453,12 +455,17 @@ def bench_served(args):
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS if achieved else None,
             "traffic": traffic_from_profiles("served_stream", "kb_partition"),
+            "hbm_measured_GBs": (traffic_from_profiles("served_stream", "kb_partition") /
+                                 (prof0["ms"] / max(prof0["iterations"], 1) / 1e3) / 1e9
+                                 if prof0 and traffic_from_profiles("served_stream", "kb_partition") else None),
             "avg_iteration_us": prof0["ms"] / max(prof0["iterations"], 1) * 1e3 if prof0 else None,
             "iterations": prof0["iterations"] if prof0 else None,
             "algorithmic_bytes_per_iteration": prof0["bytes"] / max(prof0["iterations"], 1) if prof0 else None,
             "note": "algorithmic bytes = 12 per pixel + 24 per cell for every ADMM iteration a request "
                     "executed; time = HIP events around the partition-kernel launches of each batched "
-                    "iteration on the server stream",
+                    "iteration on the server stream; traffic / hbm_measured_GBs = the partition kernel's "
+                    "own HBM bytes (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated) -- it skips "
+                    "partitions without overflow, whose duals the dual pass updates",
         },
         "cpu_baseline": None,
     }
