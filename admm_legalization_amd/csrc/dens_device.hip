@@ -234,28 +234,41 @@ constexpr size_t gemm_lds() {
   return PREC == DENS_PREC_F32 ? (size_t)(BM + BN) * (kBK + 4) * 4 : (size_t)(BM + BN) * (kBK + 8) * 2 * 2;
 }
 
+// Tile shapes of the GEMM kernel (BM x BN): 1 = 64x64, 2 = 64x128,
+// 3 = 128x64, 4 = 128x128; 0 = by problem size (128x128 from 512 tiles).
+// Every shape accumulates each output element over k in the same order, so
+// the shape changes the speed only.
+constexpr int kNumTiles = 4;
+
+template <int PREC, int BM, int BN>
+void launch_gemm_tile(const Gemm& g, hipStream_t s) {
+  const int grid = ((g.R + BM - 1) / BM) * ((g.Cn + BN - 1) / BN);
+  constexpr size_t lds = gemm_lds<PREC, BM, BN>();
+  hipLaunchKernelGGL((k_gemm_nt<PREC, BM, BN>), dim3(grid), dim3(256), lds, s, g);
+}
+
 template <int PREC>
-int launch_gemm_p(const Gemm& g, hipStream_t s) {
-  const long tiles128 = (long)((g.R + 127) / 128) * ((g.Cn + 127) / 128);
-  if (tiles128 >= 512) {
-    int grid = (int)tiles128;
-    const size_t lds = gemm_lds<PREC, 128, 128>();
-    hipLaunchKernelGGL((k_gemm_nt<PREC, 128, 128>), dim3(grid), dim3(256), lds, s, g);
-  } else {
-    int grid = ((g.R + 63) / 64) * ((g.Cn + 63) / 64);
-    const size_t lds = gemm_lds<PREC, 64, 64>();
-    hipLaunchKernelGGL((k_gemm_nt<PREC, 64, 64>), dim3(grid), dim3(256), lds, s, g);
+int launch_gemm_p(const Gemm& g, hipStream_t s, int tile) {
+  if (tile == 0) {
+    const long tiles128 = (long)((g.R + 127) / 128) * ((g.Cn + 127) / 128);
+    tile = tiles128 >= 512 ? 4 : 1;
+  }
+  switch (tile) {
+    case 1: launch_gemm_tile<PREC, 64, 64>(g, s); break;
+    case 2: launch_gemm_tile<PREC, 64, 128>(g, s); break;
+    case 3: launch_gemm_tile<PREC, 128, 64>(g, s); break;
+    default: launch_gemm_tile<PREC, 128, 128>(g, s); break;
   }
   DENS_TRY(hipGetLastError());
   return DENS_OK;
 }
 
-int launch_gemm(int prec, const Gemm& g, hipStream_t s) {
+int launch_gemm(int prec, const Gemm& g, hipStream_t s, int tile) {
   if (g.K % 4 || g.lda % 4 || g.ldb % 4) return fail(DENS_ERR_ARG, "GEMM operands must be multiples of 4 wide");
   switch (prec) {
-    case DENS_PREC_BF16: return launch_gemm_p<DENS_PREC_BF16>(g, s);
-    case DENS_PREC_SPLIT_BF16: return launch_gemm_p<DENS_PREC_SPLIT_BF16>(g, s);
-    case DENS_PREC_F32: return launch_gemm_p<DENS_PREC_F32>(g, s);
+    case DENS_PREC_BF16: return launch_gemm_p<DENS_PREC_BF16>(g, s, tile);
+    case DENS_PREC_SPLIT_BF16: return launch_gemm_p<DENS_PREC_SPLIT_BF16>(g, s, tile);
+    case DENS_PREC_F32: return launch_gemm_p<DENS_PREC_F32>(g, s, tile);
   }
   return fail(DENS_ERR_ARG, "unknown precision");
 }
@@ -494,6 +507,9 @@ using namespace densdev;
 
 struct PoissonHandle {
   int M = 0, N = 0;
+  int tile_mode = 0;                // 0: tuned per precision at first use, 1..4: fixed shape
+  int tile[3] = {0, 0, 0};          // chosen shape per precision (0: not tuned yet)
+  float tune_ms[3][kNumTiles] = {};  // measured chain time per shape
   std::vector<void*> bufs;
   float *CN = nullptr, *CM = nullptr, *A3 = nullptr, *A4 = nullptr, *wx2 = nullptr, *wy2 = nullptr;
   float *U = nullptr, *Ap = nullptr, *V = nullptr;
@@ -557,25 +573,71 @@ void poisson_tables(int M, int N, std::vector<float>& CN, std::vector<float>& CM
   }
 }
 
+int poisson_chain(PoissonHandle* h, const float* rho, float* psi, float* ex, float* ey, int prec, hipStream_t s,
+                  int tile);
+
+// Autotuner: the first solve of a precision times the whole 5-GEMM chain with
+// every tile shape on the caller's own buffers (two runs each, the second
+// timed with HIP events) and keeps the fastest.
+int poisson_tune(PoissonHandle* h, const float* rho, float* psi, float* ex, float* ey, int prec, hipStream_t s) {
+  hipEvent_t e0, e1;
+  DENS_TRY(hipEventCreate(&e0));
+  DENS_TRY(hipEventCreate(&e1));
+  int best = 0, st = DENS_OK;
+  float best_ms = 0.f;
+  for (int t = 1; t <= kNumTiles && !st; t++) {
+    if ((st = poisson_chain(h, rho, psi, ex, ey, prec, s, t))) break;
+    if (hipEventRecord(e0, s) != hipSuccess) st = fail(DENS_ERR_DEVICE, "hipEventRecord");
+    if (!st) st = poisson_chain(h, rho, psi, ex, ey, prec, s, t);
+    if (!st && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
+      st = fail(DENS_ERR_DEVICE, "tuning events");
+    float ms = 0.f;
+    if (!st && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) st = fail(DENS_ERR_DEVICE, "hipEventElapsedTime");
+    h->tune_ms[prec][t - 1] = ms;
+    if (!st && (best == 0 || ms < best_ms)) {
+      best = t;
+      best_ms = ms;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (!st) h->tile[prec] = best;
+  return st;
+}
+
 int poisson_run(PoissonHandle* h, const float* rho, float* psi, float* ex, float* ey, int prec, hipStream_t s) {
+  if (prec < 0 || prec > 2) return fail(DENS_ERR_ARG, "unknown precision");
+  int tile = h->tile_mode;
+  if (tile == 0) {
+    if (h->tile[prec] == 0) {
+      int st = poisson_tune(h, rho, psi, ex, ey, prec, s);
+      if (st) return st;
+    }
+    tile = h->tile[prec];
+  }
+  return poisson_chain(h, rho, psi, ex, ey, prec, s, tile);
+}
+
+int poisson_chain(PoissonHandle* h, const float* rho, float* psi, float* ex, float* ey, int prec, hipStream_t s,
+                  int tile) {
   const int M = h->M, N = h->N;
   int st;
   Gemm g{};
   // U = CN * rho^T
   g = Gemm{h->CN, rho, h->U, nullptr, N, N, M, N, N, M, N, 1.f, nullptr, nullptr};
-  if ((st = launch_gemm(prec, g, s))) return st;
+  if ((st = launch_gemm(prec, g, s, tile))) return st;
   // A' = 4/(MN) CM * U^T / (wy^2 + wx^2)
   g = Gemm{h->CM, h->U, h->Ap, nullptr, M, M, N, M, M, N, M, (float)(4.0 / ((double)M * N)), h->wy2, h->wx2};
-  if ((st = launch_gemm(prec, g, s))) return st;
+  if ((st = launch_gemm(prec, g, s, tile))) return st;
   // [V; Vx] = [DNt; SNw] * A'^T
   g = Gemm{h->A3, h->Ap, h->V, nullptr, N, N, M, 2 * N, 2 * N, M, N, 1.f, nullptr, nullptr};
-  if ((st = launch_gemm(prec, g, s))) return st;
+  if ((st = launch_gemm(prec, g, s, tile))) return st;
   // [psi; Ey] = [DMt; SMw] * V^T
   g = Gemm{h->A4, h->V, psi, ey, M, M, N, M, 2 * M, N, M, 1.f, nullptr, nullptr};
-  if ((st = launch_gemm(prec, g, s))) return st;
+  if ((st = launch_gemm(prec, g, s, tile))) return st;
   // Ex = DMt * Vx^T
   g = Gemm{h->A4, h->V + (size_t)N * M, ex, nullptr, M, M, N, M, M, N, M, 1.f, nullptr, nullptr};
-  return launch_gemm(prec, g, s);
+  return launch_gemm(prec, g, s, tile);
 }
 
 }  // namespace
@@ -616,6 +678,20 @@ int poisson_solve(PoissonHandle* h, const float* rho, float* psi, float* ex, flo
   if (!h || !rho || !psi || !ex || !ey) return fail(DENS_ERR_ARG, "null argument");
   hipStream_t s = (hipStream_t)stream;
   return poisson_run(h, rho, psi, ex, ey, prec, s);
+}
+
+int poisson_set_tile(PoissonHandle* h, int32_t tile) {
+  if (!h || tile < 0 || tile > kNumTiles) return fail(DENS_ERR_ARG, "tile must be 0 (tuned) or 1..4");
+  h->tile_mode = tile;
+  return DENS_OK;
+}
+
+int poisson_get_tile(PoissonHandle* h, int32_t prec, int32_t* tile, float* ms) {
+  if (!h || prec < 0 || prec > 2) return fail(DENS_ERR_ARG, "bad argument");
+  if (tile) *tile = h->tile_mode ? h->tile_mode : h->tile[prec];
+  if (ms)
+    for (int t = 0; t < kNumTiles; t++) ms[t] = h->tune_ms[prec][t];
+  return DENS_OK;
 }
 
 int poisson_destroy(PoissonHandle* h) {
@@ -695,6 +771,14 @@ int dens_create(const DensGeom* g, DensHandle** out) {
   if ((st = dalloc(h->bufs, &h->precond, N))) return bail(st);
   if ((st = dalloc(h->bufs, &h->ovf_acc, 1))) return bail(st);
   if ((st = dalloc(h->bufs, &h->ovf_val, 1))) return bail(st);
+  // tile shapes of the Poisson GEMMs tuned now for every precision, so that
+  // neither dens_compute_grad nor dens_set_precision waits on the host (the
+  // timing does not depend on the values)
+  if (hipMemset(h->dens, 0, (size_t)B * sizeof(float)) != hipSuccess)
+    return bail(fail(DENS_ERR_DEVICE, "hipMemset"));
+  for (int p = 0; p < 3; p++)
+    if ((st = poisson_tune(h->ps, h->dens, h->psi, h->ex, h->ey, p, nullptr))) return bail(st);
+  if (hipDeviceSynchronize() != hipSuccess) return bail(fail(DENS_ERR_DEVICE, "tuning"));
   *out = h;
   return DENS_OK;
 }

@@ -44,7 +44,8 @@ class DensGeom(C.Structure):
 EXPORTED = ["dens_last_error", "dens_prepare", "dens_create", "dens_compute_grad", "dens_get_overflow",
             "dens_get_bin_density", "dens_get_potential", "dens_get_field_x", "dens_get_field_y",
             "dens_get_preconditioner", "dens_get_movable_area", "dens_get_filler_area",
-            "dens_set_precision", "dens_destroy", "poisson_create", "poisson_solve", "poisson_destroy"]
+            "dens_set_precision", "dens_destroy", "poisson_create", "poisson_solve", "poisson_set_tile",
+            "poisson_get_tile", "poisson_destroy"]
 
 _bound = None
 _hip = None
@@ -71,6 +72,8 @@ def lib():
         L.dens_destroy.argtypes = [C.c_void_p]
         L.poisson_create.argtypes = [C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]
         L.poisson_solve.argtypes = [C.c_void_p] + [C.c_void_p] * 4 + [C.c_int32, C.c_void_p]
+        L.poisson_set_tile.argtypes = [C.c_void_p, C.c_int32]
+        L.poisson_get_tile.argtypes = [C.c_void_p, C.c_int32, _i32p, _f32p]
         L.poisson_destroy.argtypes = [C.c_void_p]
         _bound = L
     return _bound
@@ -231,6 +234,17 @@ class PoissonSolver:
         args = (_ptr(binDensity, B), _ptr(potential, B), _ptr(electroForceX, B), _ptr(electroForceY, B))
         with _OnStream() as s:
             _check(lib().poisson_solve(self.h, *args, int(precision), s))
+
+    def setTile(self, tile: int):
+        """GEMM tile shape: 0 = tuned per precision, 1 = 64x64, 2 = 64x128, 3 = 128x64, 4 = 128x128."""
+        _check(lib().poisson_set_tile(self.h, int(tile)))
+
+    def getTile(self, precision=PREC_SPLIT_BF16):
+        """(shape in use, tuner's whole-solve ms per shape) for `precision`."""
+        t = C.c_int32()
+        ms = (C.c_float * 4)()
+        _check(lib().poisson_get_tile(self.h, int(precision), C.byref(t), ms))
+        return t.value, list(ms)
 
     def close(self):
         if getattr(self, "h", None):

@@ -593,7 +593,7 @@ def bench_density(args):
         rho = dg.getDeviceBinDensity()
         ps = D.PoissonSolver(g.num_bin_x, g.num_bin_y)
         psi, ex, ey = (torch.empty_like(rho) for _ in range(3))
-        solve_ms = {}
+        solve_ms, solve_tiles = {}, {}
         for name, prec in (("bf16", D.PREC_BF16), ("split_bf16", D.PREC_SPLIT_BF16), ("f32", D.PREC_F32)):
             for _ in range(2):
                 ps.solvePoisson(rho, psi, ex, ey, precision=prec)
@@ -605,6 +605,9 @@ def bench_density(args):
             b.record(s)
             torch.cuda.synchronize()
             solve_ms[name] = a.elapsed_time(b) / reps
+            tile, tune_ms = ps.getTile(prec)
+            solve_tiles[name] = {"tile": ["", "64x64", "64x128", "128x64", "128x128"][tile],
+                                 "tuner_ms": [round(m, 4) for m in tune_ms]}
         ps.close()
     n = g.num_bin_x
     flops = 16.0 * n ** 3  # four 2-D transforms as two n^3 GEMMs each
@@ -618,6 +621,7 @@ def bench_density(args):
         "config": {"workload": f"density_1m: {WORKLOADS['density_1m']}", "cells": n_cells,
                    "bins": [g.num_bin_x, g.num_bin_y], "overflow": ovf, "event_ms_per_eval": ms_eval,
                    "cells_per_s": n_cells * 1e3 / ms_eval, "poisson_ms": solve_ms,
+                   "poisson_tiles": solve_tiles,
                    "parallelism": "1 GPU"},
         "roofline": {"bound": "mfma", "kernel": "Poisson solve (split-BF16 cosine/sine GEMMs)",
                      "achieved": achieved, "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",

@@ -124,9 +124,19 @@ int dens_destroy(DensHandle* h);
 
 int poisson_create(int32_t num_bin_x, int32_t num_bin_y, PoissonHandle** out);
 /* potential, field x, field y from bin density (all [B] device arrays,
- * y-major). */
+ * y-major).  Unless a tile shape is fixed with poisson_set_tile, the first
+ * solve of each precision is the autotuner's: it times the five GEMMs with
+ * every tile shape on the given buffers, keeps the fastest (results do not
+ * depend on the shape) and waits for the stream once.  dens_create tunes its
+ * own solver for all three precisions up front. */
 int poisson_solve(PoissonHandle* h, const float* d_rho, float* d_psi, float* d_ex, float* d_ey,
                   int32_t precision, void* stream);
+/* GEMM tile shape: 0 = tuned per precision (default), 1 = 64x64, 2 = 64x128,
+ * 3 = 128x64, 4 = 128x128.  No counterpart in the reference (cuFFT there). */
+int poisson_set_tile(PoissonHandle* h, int32_t tile);
+/* The shape in use for `precision` (0 before tuning) and, if ms != NULL, the
+ * tuner's time of the whole solve per shape [4] (zeros if not tuned). */
+int poisson_get_tile(PoissonHandle* h, int32_t precision, int32_t* tile, float* ms);
 int poisson_destroy(PoissonHandle* h);
 
 #ifdef __cplusplus

@@ -179,3 +179,32 @@ def test_poisson_matches_golden(gpu, n):
     errs = [_rel(a.cpu().numpy(), z[k]) for a, k in ((psi, "psi"), (ex, "ex"), (ey, "ey"))]
     assert max(errs) <= TOL[D.PREC_SPLIT_BF16], errs
     ps.close()
+
+
+@pytest.mark.parametrize("prec", [D.PREC_SPLIT_BF16, D.PREC_F32, D.PREC_BF16])
+@pytest.mark.parametrize("M,N", [(64, 64), (1024, 1024), (96, 160)])
+def test_poisson_tile_shapes_agree(gpu, M, N, prec):
+    """Every GEMM tile shape accumulates each output over k in the same order,
+    so the tuned choice and each fixed shape give bit-identical fields."""
+    import torch
+    print(f"[case] poisson tiles {M}x{N} prec={prec}", flush=True)
+    rho = _density_map(M, N, seed=M + N)
+    ps = D.PoissonSolver(N, M)
+    assert ps.getTile(prec) == (0, [0.0] * 4)
+    t = torch.tensor(rho, dtype=torch.float32, device="cuda")
+    outs = []
+    for tile in (0, 1, 2, 3, 4):
+        ps.setTile(tile)
+        psi, ex, ey = (torch.empty_like(t) for _ in range(3))
+        ps.solvePoisson(t, psi, ex, ey, precision=prec)
+        torch.cuda.synchronize()
+        outs.append(torch.stack([psi, ex, ey]).cpu().numpy())
+        if tile == 0:
+            chosen, ms = ps.getTile(prec)
+            print(f"  tuned tile {chosen}, ms per shape {ms}")
+            assert 1 <= chosen <= 4 and all(m > 0 for m in ms) and ms[chosen - 1] == min(ms)
+    for o in outs[1:]:
+        assert np.array_equal(o.view(np.uint32), outs[0].view(np.uint32))
+    with pytest.raises(RuntimeError):
+        ps.setTile(5)
+    ps.close()
