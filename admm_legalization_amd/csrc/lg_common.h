@@ -46,6 +46,10 @@ enum : int32_t {
 };
 
 constexpr int kWave = 64;
+// waves per partition workgroup of the register-sort launches: the candidate
+// moves of a cell are split over them (one per lane); the LDS-sort launches
+// keep one wave
+constexpr int kPartWaves = 4;
 constexpr int kNumDirDefault = 245;
 constexpr int kNumDirPadded = 256;
 constexpr int kNumDirWide = 4093;

@@ -247,21 +247,28 @@ int upload(LgHandle* h, T** dst, const T* src, size_t n) {
 
 hipStream_t pick(LgHandle* h, void* s) { return s ? (hipStream_t)s : h->own_stream; }
 
-size_t sort_bytes_per(const LgHandle* h) {
-  const int algo = h->p.sorting_algorithm;
+size_t sort_bytes_per(int algo) {
   return (algo == LG_SORT_S_MERGE || algo == LG_SORT_M_MERGE) ? 16 : (algo == LG_SORT_CUB ? 12 : 8);
 }
 
 // LDS of one partition workgroup: pixel tile {usage|code, dual} + dual/rho +
 // vdd rows, plus the sort buffers when the sorts run in LDS.
-size_t partition_lds_bytes(const LgHandle* h, int npad_hi, int lds_sort) {
+size_t partition_lds_bytes(int x_hint, int y_hint, int algo, int npad_hi, int lds_sort) {
   // three tile arrays (usage|code, penalty term, dual) with the kernel's row
   // stride, plus the vdd rows
-  const size_t rs = h->a.x_hint <= kRowStride ? kRowStride : h->a.x_hint;
-  const size_t tsz = (rs * h->a.y_hint + 3) & ~(size_t)3;
-  size_t tile = 3 * tsz * 4 + (size_t)((h->a.y_hint + 3) & ~3) * 4;
-  return tile + (lds_sort ? (size_t)npad_hi * sort_bytes_per(h) : 0);
+  const size_t rs = x_hint <= kRowStride ? kRowStride : x_hint;
+  const size_t tsz = (rs * y_hint + 3) & ~(size_t)3;
+  size_t tile = 3 * tsz * 4 + (size_t)((y_hint + 3) & ~3) * 4;
+  // + the waves' exchange slots (24 words) of the four-wave launches
+  return tile + (lds_sort ? (size_t)npad_hi * sort_bytes_per(algo) : 128);
 }
+
+size_t partition_lds_bytes(const LgHandle* h, int npad_hi, int lds_sort) {
+  return partition_lds_bytes(h->a.x_hint, h->a.y_hint, h->p.sorting_algorithm, npad_hi, lds_sort);
+}
+
+// LDS of a workgroup on gfx950
+constexpr size_t kMaxLds = 160 * 1024;
 
 // Launch one partition-kernel pass.  Two LDS size classes; the device picks
 // the class from ctl->npad and the other launch exits at once.  Up to 1024
@@ -274,13 +281,20 @@ int launch_partition(LgHandle* h, hipStream_t s) {
   if (prof) HIP_TRY(hipEventRecord(e[0], s));
   // small class: register sorts when the algorithm is a bitonic network
   size_t lds0 = partition_lds_bytes(h, kNpadSmall, reg_ok ? 0 : 1);
-  if (reg_ok)
-    hipLaunchKernelGGL((k_partition<MODE, 0>), dim3(h->a.K), dim3(kWave), lds0, s, h->a, 0, kNpadSmall);
-  else
-    hipLaunchKernelGGL((k_partition<MODE, 1>), dim3(h->a.K), dim3(kWave), lds0, s, h->a, 0, kNpadSmall);
+  if (reg_ok) {
+    if constexpr (MODE == MODE_PRIMAL)
+      hipLaunchKernelGGL((k_partition<MODE_PRIMAL, 0, kPartWaves>), dim3(h->a.K), dim3(kPartWaves * kWave), lds0, s,
+                         h->a, 0, kNpadSmall);
+    else
+      hipLaunchKernelGGL((k_partition<MODE, 0, 1>), dim3(h->a.K), dim3(kWave), lds0, s, h->a, 0, kNpadSmall);
+  } else
+    hipLaunchKernelGGL((k_partition<MODE, 1, 1>), dim3(h->a.K), dim3(kWave), lds0, s, h->a, 0, kNpadSmall);
   if (prof) HIP_TRY(hipEventRecord(e[1], s));
   size_t lds1 = partition_lds_bytes(h, kMaxNpad, 1);
-  hipLaunchKernelGGL((k_partition<MODE, 1>), dim3(h->a.K), dim3(kWave), lds1, s, h->a, kNpadSmall + 1, kMaxNpad);
+  // the primal step's large class also takes the partitions the four-wave
+  // launch deferred (its list, every size)
+  const int lo1 = MODE == MODE_PRIMAL ? 1 : kNpadSmall + 1;
+  hipLaunchKernelGGL((k_partition<MODE, 1, 1>), dim3(h->a.K), dim3(kWave), lds1, s, h->a, lo1, kMaxNpad);
   if (prof) {
     HIP_TRY(hipEventRecord(e[2], s));
     h->ev_used += 3;
@@ -415,6 +429,14 @@ int create_from_prep(const HostPrep& hp, DevPool* pool, Ctl* ext_ctl, hipStream_
   int st = upload_directions();
   if (st) return st;
   const LgParams* p = &hp.p;
+  {
+    // a partition's pixels are staged in LDS (the large size class also
+    // holds its sort buffers there)
+    const size_t need = partition_lds_bytes(p->x_hint, p->y_hint, p->sorting_algorithm, kMaxNpad, 1);
+    if (need > kMaxLds)
+      return fail(LG_ERR_ARG, "partition of x_hint x y_hint pixels needs " + std::to_string(need) +
+                                  " bytes of LDS, more than the " + std::to_string(kMaxLds) + " of a workgroup");
+  }
   if (pool && !(pool->budget >= handle_bytes(hp)))
     return fail(LG_ERR_MEMORY, "request larger than the server's device-memory budget");
   char* blk = nullptr;

@@ -744,13 +744,14 @@ __device__ __forceinline__ int ovf_hbm(const Args& a, int cx, int cy, int w, int
   return s;
 }
 
-// rip-up (delta -1) or commit (+1) of a footprint; refreshes the penalty terms
+// rip-up (delta -1) or commit (+1) of a footprint by the NW waves of the
+// workgroup; refreshes the penalty terms
+template <int NW>
 __device__ __forceinline__ void footprint_add(const PartGeo& g, const Tile& T, int x0, int y0, int w, int h,
                                               int delta, float rho, bool standard) {
-  const int lane = threadIdx.x;
   const int n = w * h;
   const int lx0 = g.lx(x0), ly0 = g.ly(y0);
-  for (int i = lane; i < n; i += kWave) {
+  for (int i = threadIdx.x; i < n; i += NW * kWave) {
     int dx = i / h;
     int lxi = lx0 + dx, lyi = ly0 + (i - dx * h);
     if (lxi >= g.xh) lxi -= g.xh;
@@ -763,14 +764,14 @@ __device__ __forceinline__ void footprint_add(const PartGeo& g, const Tile& T, i
 }
 
 // Penalty sums of Q consecutive footprint columns (x .. x+Q-1) of height H
-// for the 4 candidates, row stride kRowStride: every LDS read uses an
-// immediate offset from one address per candidate.  The sums run over x,
+// for the NC candidates of a lane, row stride kRowStride: every LDS read uses
+// an immediate offset from one address per candidate.  The sums run over x,
 // then y, as computePenaltySum's loops.
-template <int H, int Q>
-__device__ __forceinline__ void sum_cols(const float* t, const int (&base)[4], int x, float (&s)[4]) {
-  float v[4][Q][H];
+template <int NC, int H, int Q>
+__device__ __forceinline__ void sum_cols(const float* t, const int (&base)[NC], int x, float (&s)[NC]) {
+  float v[NC][Q][H];
 #pragma unroll
-  for (int j = 0; j < 4; j++) {
+  for (int j = 0; j < NC; j++) {
     const float* p = t + base[j] + x;
 #pragma unroll
     for (int q = 0; q < Q; q++)
@@ -778,21 +779,21 @@ __device__ __forceinline__ void sum_cols(const float* t, const int (&base)[4], i
       for (int y = 0; y < H; y++) v[j][q][y] = p[y * kRowStride + q];
   }
 #pragma unroll
-  for (int j = 0; j < 4; j++)
+  for (int j = 0; j < NC; j++)
 #pragma unroll
     for (int q = 0; q < Q; q++)
 #pragma unroll
       for (int y = 0; y < H; y++) s[j] += v[j][q][y];
 }
 
-template <int H>
-__device__ __forceinline__ void sum_rect(const float* t, const int (&base)[4], int w, float (&s)[4]) {
+template <int NC, int H>
+__device__ __forceinline__ void sum_rect(const float* t, const int (&base)[NC], int w, float (&s)[NC]) {
   int x = 0;
-  for (; x + 4 <= w; x += 4) sum_cols<H, 4>(t, base, x, s);
+  for (; x + 4 <= w; x += 4) sum_cols<NC, H, 4>(t, base, x, s);
   switch (w - x) {
-    case 1: sum_cols<H, 1>(t, base, x, s); break;
-    case 2: sum_cols<H, 2>(t, base, x, s); break;
-    case 3: sum_cols<H, 3>(t, base, x, s); break;
+    case 1: sum_cols<NC, H, 1>(t, base, x, s); break;
+    case 2: sum_cols<NC, H, 2>(t, base, x, s); break;
+    case 3: sum_cols<NC, H, 3>(t, base, x, s); break;
     default: break;
   }
 }
@@ -809,20 +810,19 @@ __device__ __forceinline__ void key_min_step(unsigned& hi, unsigned& lo) {
   lo = take ? olo : lo;
 }
 
-// The 4 candidates of one lane (moves d = j*64 + lane), evaluated together so
-// that their LDS reads overlap.  cost = computeDispCost + computePenaltySum
+// The NC candidates of a lane (moves d = d0 + j*64, j < NC), evaluated
+// together so that their LDS reads overlap.  cost = computeDispCost + computePenaltySum
 // (partition_exclusive, AdmmKernels.hpp:333-391); pin-access and edge-spacing
 // terms are zero (no pins, no edge rules).  A footprint that leaves the
 // partition, lands on a wrong-rail row, or touches a pixel of another group or
 // an invalid pixel costs +inf; the penalty terms sum in the reference's
 // x-major order.
-template <int MODE>
+template <int MODE, int NC>
 __device__ __forceinline__ void eval_candidates(const Args& a, const PartGeo& g, const Tile& T,
                                                 float disp_coeff, float thr, int cx, int cy,
-                                                int w, int h, int code, int gnd, int ox, int oy,
-                                                const int2 dir[4], float cst[4], int ucode,
+                                                int w, int h, int code, int gnd, int ox, int oy, int d0,
+                                                const int2 (&dir)[NC], float (&cst)[NC], int ucode,
                                                 unsigned long long vmask) {
-  const int lane = threadIdx.x;
   const bool flat = g.ncol > 1 && g.nrow > 1;  // footprints inside one interval never wrap
   if (MODE == MODE_PRIMAL && ucode >= 0 && flat && T.rs == kRowStride && h <= 4 && g.yh <= 64) {
     // The common case without branches: a tile of one placement code, a
@@ -834,10 +834,10 @@ __device__ __forceinline__ void eval_candidates(const Args& a, const PartGeo& g,
     // from the mask.  Same arithmetic as the general walk below.
     const int eps = (h % 2 == 0) ? 2 : 1;
     const bool wide = w > g.xh || h > g.yh || code != ucode;
-    int base[4];
-    float s4[4];
+    int base[NC];
+    float s4[NC];
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
+    for (int j = 0; j < NC; j++) {
       const int nx = max(0, min(a.x_size - w, cx + dir[j].x));
       const int ny = max(0, min(a.y_size - h, cy + eps * dir[j].y));
       cst[j] = disp_cost(a, disp_coeff, thr, nx, ny, ox, oy);
@@ -847,29 +847,29 @@ __device__ __forceinline__ void eval_candidates(const Args& a, const PartGeo& g,
       ly -= ly >= g.yh ? g.nrow * g.yh : 0;
       const bool in = (lx >= 0) & (lx + w <= g.xh) & (ly >= 0) & (ly + h <= g.yh);
       const int rail = (int)((vmask >> (unsigned)(ly & 63)) & 1ull);
-      const bool ok = in & !wide & (j * kWave + lane < kNumDirDefault) & (((h & 1) != 0) | (rail == gnd));
+      const bool ok = in & !wide & (d0 + j * kWave < kNumDirDefault) & (((h & 1) != 0) | (rail == gnd));
       base[j] = ok ? ly * kRowStride + lx : -1;
       s4[j] = 0.0f;
     }
-    int bz[4];
+    int bz[NC];
 #pragma unroll
-    for (int j = 0; j < 4; j++) bz[j] = base[j] < 0 ? 0 : base[j];
+    for (int j = 0; j < NC; j++) bz[j] = base[j] < 0 ? 0 : base[j];
     switch (h) {
-      case 1: sum_rect<1>(T.t, bz, w, s4); break;
-      case 2: sum_rect<2>(T.t, bz, w, s4); break;
-      case 3: sum_rect<3>(T.t, bz, w, s4); break;
-      default: sum_rect<4>(T.t, bz, w, s4); break;
+      case 1: sum_rect<NC, 1>(T.t, bz, w, s4); break;
+      case 2: sum_rect<NC, 2>(T.t, bz, w, s4); break;
+      case 3: sum_rect<NC, 3>(T.t, bz, w, s4); break;
+      default: sum_rect<NC, 4>(T.t, bz, w, s4); break;
     }
 #pragma unroll
-    for (int j = 0; j < 4; j++) cst[j] = base[j] < 0 ? kInf : cst[j] + s4[j];
+    for (int j = 0; j < NC; j++) cst[j] = base[j] < 0 ? kInf : cst[j] + s4[j];
     return;
   }
-  int lxi[4], lyi0[4];
-  int bad[4];  // 0 / 1 (ints: bool arrays get packed into bytes)
-  float s[4];
+  int lxi[NC], lyi0[NC];
+  int bad[NC];  // 0 / 1 (ints: bool arrays get packed into bytes)
+  float s[NC];
 #pragma unroll
-  for (int j = 0; j < 4; j++) {
-    int d = j * kWave + lane;
+  for (int j = 0; j < NC; j++) {
+    int d = d0 + j * kWave;
     bad[j] = 1;
     lxi[j] = 0;
     lyi0[j] = 0;
@@ -894,24 +894,24 @@ __device__ __forceinline__ void eval_candidates(const Args& a, const PartGeo& g,
     // every in-grid pixel of the tile carries code `ucode`: the code test is
     // one comparison, and the walk only sums the penalty terms (a bad
     // candidate sums junk from row 0, its cost is +inf anyway)
-    int base[4];
+    int base[NC];
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
+    for (int j = 0; j < NC; j++) {
       if (code != ucode) bad[j] = 1;
       base[j] = bad[j] ? 0 : lyi0[j] * kRowStride + lxi[j];
     }
     switch (h) {
-      case 1: sum_rect<1>(T.t, base, w, s); break;
-      case 2: sum_rect<2>(T.t, base, w, s); break;
-      case 3: sum_rect<3>(T.t, base, w, s); break;
-      default: sum_rect<4>(T.t, base, w, s); break;
+      case 1: sum_rect<NC, 1>(T.t, base, w, s); break;
+      case 2: sum_rect<NC, 2>(T.t, base, w, s); break;
+      case 3: sum_rect<NC, 3>(T.t, base, w, s); break;
+      default: sum_rect<NC, 4>(T.t, base, w, s); break;
     }
   } else if (flat) {
     // flat walk over the footprint in x-major order (f = x * h + y), four
     // pixels per step so that sixteen LDS reads are in flight
-    int col[4];
+    int col[NC];
 #pragma unroll
-    for (int j = 0; j < 4; j++) col[j] = T.at(lxi[j], lyi0[j]);
+    for (int j = 0; j < NC; j++) col[j] = T.at(lxi[j], lyi0[j]);
     const int n = w * h;
     int fx = 0, fy = 0;
     for (int f0 = 0; f0 < n; f0 += 4) {
@@ -921,12 +921,12 @@ __device__ __forceinline__ void eval_candidates(const Args& a, const PartGeo& g,
         off[q] = fy * T.rs + fx;
         if (++fy == h) { fy = 0; fx++; }
       }
-      uint32_t e[4][4];
-      float tv[4][4];
+      uint32_t e[4][NC];
+      float tv[4][NC];
 #pragma unroll
       for (int q = 0; q < 4; q++)
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
+        for (int j = 0; j < NC; j++) {
           const int i = (bad[j] || f0 + q >= n) ? 0 : col[j] + off[q];
           e[q][j] = T.uc[i];
           tv[q][j] = T.t[i];
@@ -935,7 +935,7 @@ __device__ __forceinline__ void eval_candidates(const Args& a, const PartGeo& g,
       for (int q = 0; q < 4; q++) {
         if (f0 + q < n) {
 #pragma unroll
-          for (int j = 0; j < 4; j++) {
+          for (int j = 0; j < NC; j++) {
             bad[j] |= (int)(e[q][j] >> 16) != code;
             if (MODE == MODE_REFINE) bad[j] |= (e[q][j] & 0xffffu) != 0u;
             else s[j] += tv[q][j];
@@ -945,20 +945,20 @@ __device__ __forceinline__ void eval_candidates(const Args& a, const PartGeo& g,
     }
   } else {
     for (int x = 0; x < w; x++) {
-      int lyi[4];
+      int lyi[NC];
 #pragma unroll
-      for (int j = 0; j < 4; j++) lyi[j] = lyi0[j];
+      for (int j = 0; j < NC; j++) lyi[j] = lyi0[j];
       for (int y = 0; y < h; y++) {
-        uint32_t e[4];
-        float tv[4];
+        uint32_t e[NC];
+        float tv[NC];
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
+        for (int j = 0; j < NC; j++) {
           const int i = bad[j] ? 0 : T.at(lxi[j], lyi[j]);
           e[j] = T.uc[i];
           tv[j] = T.t[i];
         }
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
+        for (int j = 0; j < NC; j++) {
           bad[j] |= (int)(e[j] >> 16) != code;
           if (MODE == MODE_REFINE) bad[j] |= (e[j] & 0xffffu) != 0u;
           else s[j] += tv[j];
@@ -966,37 +966,40 @@ __device__ __forceinline__ void eval_candidates(const Args& a, const PartGeo& g,
         }
       }
 #pragma unroll
-      for (int j = 0; j < 4; j++)
+      for (int j = 0; j < NC; j++)
         if (++lxi[j] == g.xh) lxi[j] = 0;
     }
   }
 #pragma unroll
-  for (int j = 0; j < 4; j++) {
+  for (int j = 0; j < NC; j++) {
     float pc = (MODE == MODE_REFINE) ? 0.0f : s[j];
     cst[j] = bad[j] ? kInf : cst[j] + pc;
   }
 }
 
-// Staging of the partition's pixel tile, lanes walking the tile in order
-// with incremental (la, lb) = (i % xh, i / xh): consecutive lanes read
-// consecutive pixels of a grid row; sixteen pixels in flight per lane.
-// Returns true if a usage does not fit the 16-bit LDS packing.
+// Staging of the partition's pixel tile by the NW waves of the workgroup,
+// threads walking the tile in order with incremental (la, lb) = (i % xh,
+// i / xh): consecutive threads read consecutive pixels of a grid row;
+// 16 / NW pixels in flight per thread.  Returns true if a usage does not fit
+// the 16-bit LDS packing.
+template <int NW>
 __device__ __forceinline__ bool stage_tile(const Args& a, const PartGeo& g, const Tile& T, float rho,
                                            bool standard, bool& over, int& cmin, int& cmax) {
-  const int lane = threadIdx.x;
+  constexpr int NT = NW * kWave;
+  const int tid = threadIdx.x;
   const int npix = g.xh * g.yh;
-  const int step_b = kWave / g.xh, step_a = kWave - step_b * g.xh;
-  int lb = lane / g.xh, la = lane - lb * g.xh;
+  const int step_b = NT / g.xh, step_a = NT - step_b * g.xh;
+  int lb = tid / g.xh, la = tid - lb * g.xh;
   bool range_err = false;
-  constexpr int U = 16;
-  for (int i0 = 0; i0 < npix; i0 += U * kWave) {
+  constexpr int U = 16 / NW;
+  for (int i0 = 0; i0 < npix; i0 += U * NT) {
     int u[U], code[U], li[U];
     float dl[U];
     bool in[U];
 #pragma unroll
     for (int q = 0; q < U; q++) {
       int xx = g.gx(la), yy = g.gy(lb);
-      bool ok = (i0 + q * kWave + lane < npix) && xx >= 0 && yy >= 0;
+      bool ok = (i0 + q * NT + tid < npix) && xx >= 0 && yy >= 0;
       // unconditional loads at a clamped index, then select
       const unsigned p = ok ? pidx(a, xx, yy) : 0u;
       li[q] = T.at(la, lb);
@@ -1012,7 +1015,7 @@ __device__ __forceinline__ bool stage_tile(const Args& a, const PartGeo& g, cons
     }
 #pragma unroll
     for (int q = 0; q < U; q++) {
-      int i = i0 + q * kWave + lane;
+      int i = i0 + q * NT + tid;
       if (in[q] && code[q] > 0) {
         cmin = min(cmin, code[q]);
         cmax = max(cmax, code[q]);
@@ -1171,9 +1174,12 @@ __device__ __forceinline__ bool order_partition_reg(const Args& a, const PartGeo
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     }
   }
-  int4 rc[RN];
+  // records loaded together, except in the wide instance, which loads each
+  // when its key is computed (sixteen int4 would hold 64 registers)
+  constexpr int RC = RN > 8 ? 1 : RN;
+  int4 rc[RC];
 #pragma unroll
-  for (int r = 0; r < RN; r++) rc[r] = cellrec[val[r]];
+  for (int r = 0; r < RC; r++) rc[r] = cellrec[val[r]];
   int key[RN];
   int kmin = INT_MAX, kmax = INT_MIN;
 #pragma unroll 1
@@ -1182,7 +1188,13 @@ __device__ __forceinline__ bool order_partition_reg(const Args& a, const PartGeo
     int4 rr = rc[0];
 #pragma unroll
     for (int q = 1; q < RN; q++)
-      if (q == r) { v = val[q]; rr = rc[q]; }
+      if (q == r) v = val[q];
+    if (RN > 8) rr = cellrec[v];
+    else {
+#pragma unroll
+      for (int q = 1; q < RC; q++)
+        if (q == r) rr = rc[q];
+    }
     const int i = lane * RN + r;
     const int kk = i < count ? sort_key(a, g, T, v, rr) : INT_MIN;
     if (i < count) {
@@ -1256,14 +1268,25 @@ __device__ __forceinline__ bool order_partition_reg(const Args& a, const PartGeo
 #pragma unroll
     for (int r = 0; r < RN; r++) val[r] = ids[min(slot[r], count - 1)];
   }
+  if (RN > 8) {
 #pragma unroll
-  for (int r = 0; r < RN; r++) rc[r] = cellrec[val[r]];
+    for (int r = 0; r < RN; r++) {
+      const int i = lane * RN + r;
+      if (i < count) {
+        ids[i] = val[r];
+        recb[i] = cellrec[val[r]];
+      }
+    }
+  } else {
 #pragma unroll
-  for (int r = 0; r < RN; r++) {
-    const int i = lane * RN + r;
-    if (i < count) {
-      ids[i] = val[r];
-      recb[i] = rc[r];
+    for (int r = 0; r < RC; r++) rc[r] = cellrec[val[r]];
+#pragma unroll
+    for (int r = 0; r < RC; r++) {
+      const int i = lane * RN + r;
+      if (i < count) {
+        ids[i] = val[r];
+        recb[i] = rc[r];
+      }
     }
   }
 #ifdef LG_PHASE_TIMERS
@@ -1308,6 +1331,27 @@ __device__ void order_partition_lds(const Args& a, const PartGeo& g, const Tile&
   }
 }
 
+// Appends partition k to the large class's list of this iteration; that
+// launch runs after the current one and reads the count then.
+__device__ __forceinline__ void defer_partition(const Args& a, int k) {
+  const int pos = atomicAdd(&a.ctl->n_work[1], 1);
+  a.work[a.K + pos] = k;
+}
+
+// Or of a predicate over the workgroup's NW waves; four-wave workgroups
+// exchange through LDS slots x[0..NW), each call site with slots of its own.
+template <int NW>
+__device__ __forceinline__ bool block_any(bool v, int* x, int lane, int wave) {
+  const bool any = __ballot(v) != 0ull;
+  if (NW == 1) return any;
+  if (lane == 0) x[wave] = any ? 1 : 0;
+  __syncthreads();
+  int r = 0;
+#pragma unroll
+  for (int w2 = 0; w2 < NW; w2++) r |= x[w2];
+  return r != 0;
+}
+
 // dual_update_functor_with_step_size (AdmmKernels.hpp:26-42): the literal 0.5
 // promotes the quadratic term to double before the float store.
 __device__ __forceinline__ float dual_step(int ui, float l, float rho, float step, float neg) {
@@ -1347,8 +1391,14 @@ __device__ __forceinline__ void flag_next(const Args& a, int iter, int x, int y)
 //
 // npad_lo..npad_hi selects the launch (LDS size class); LDS_SORT launches keep
 // the sort buffers in LDS, the others sort in registers (npad <= 1024).
-template <int MODE, int LDS_SORT>
+template <int MODE, int LDS_SORT, int NW>
 __device__ __forceinline__ void partition_body(const Args& a, int bx, int npad_lo, int npad_hi) {
+  static_assert(NW == 1 || (NW == 4 && LDS_SORT == 0 && MODE == MODE_PRIMAL), "four waves: primal, register sorts");
+  constexpr int NT = NW * kWave;
+  // Four-wave launches sort up to 8 * 64 slots in registers.  A larger
+  // partition, or one whose keys are too wide to pack, is appended to the
+  // list of the large class, whose one-wave launch (LDS sorts) comes next.
+  constexpr bool kDefer = NW > 1;
   Ctl* ctl = a.ctl;
   if (MODE == MODE_PRIMAL && ctl->done) return;
   // the primal step takes its partitions from the work list of its class;
@@ -1370,9 +1420,15 @@ __device__ __forceinline__ void partition_body(const Args& a, int bx, int npad_l
   int npad = 1;
   while (npad < count) npad <<= 1;
   if (npad < npad_lo || npad > npad_hi) return;
+  if (kDefer && npad > 8 * kWave) {
+    if (threadIdx.x == 0) defer_partition(a, k);
+    return;
+  }
   PHASE_START();
   SUB_DECL();
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x;
+  const int lane = NW == 1 ? tid : (tid & (kWave - 1));
+  const int wave = NW == 1 ? 0 : (tid >> 6);
   PartGeo g;
   const int tf = triple_fold(a, iter);
   g.xh = a.x_hint; g.yh = a.y_hint; g.ncol = a.ncol; g.nrow = a.nrow;
@@ -1381,7 +1437,7 @@ __device__ __forceinline__ void partition_body(const Args& a, int bx, int npad_l
   g.init(a.x_size, a.y_size);
   int4* cellrec = reinterpret_cast<int4*>(a.cellrec);
   const int npix = g.xh * g.yh;
-  const int step_b = kWave / g.xh, step_a = kWave - step_b * g.xh;
+  const int step_b = NT / g.xh, step_a = NT - step_b * g.xh;
 
   const float rho = ctl->rho;
   const float step = ctl->step;
@@ -1394,51 +1450,6 @@ __device__ __forceinline__ void partition_body(const Args& a, int bx, int npad_l
   const bool from_bucket = MODE == MODE_PRIMAL || a.sorted[k] == 0;
   const int* src = from_bucket ? a.bucket + (size_t)k * kMaxNpad : ids;
 
-  // -- activity test: does any cell of this partition need a move?  With
-  // every cell on a correct-parity row, a partition whose pixels all have
-  // usage <= 1 has none (its cells' footprints lie inside it); that test is
-  // fused into the tile staging below.  Otherwise each cell is checked.
-  const bool fast_ok = MODE == MODE_PRIMAL && !pert;
-  const bool cell_test = fast_ok && ctl->wrong_row > 0;
-  if (fast_ok && !cell_test && !adaptive_now && a.act_flags[(size_t)(iter & 1) * a.K + k] == 0) {
-    // no pixel above usage one: no cell moves, and k_dual_pass updates
-    // this partition's prices
-    if (lane == 0) a.sorted[k] = 0;
-    return;
-  }
-  if (cell_test) {
-    bool need = false;
-    for (int i = lane; i < count; i += kWave) {
-      int4 rc = cellrec[src[i]];
-      int cx = rec_x(rc.x), cy = rec_y(rc.x), h = rec_h(rc.y);
-      need |= !correct_row(h, rec_gnd(rc.y), a.vdd_up[cy]) || ovf_hbm(a, cx, cy, rec_w(rc.y), h) != 0;
-    }
-    if (!__any(need)) {
-      // primal step is a no-op: stream the dual update over the pixels
-      unsigned long long ovf = 0;
-      int lb = lane / g.xh, la = lane - lb * g.xh;
-      for (int i = lane; i < npix; i += kWave) {
-        int xx = g.gx(la), yy = g.gy(lb);
-        la += step_a; lb += step_b;
-        if (la >= g.xh) { la -= g.xh; lb++; }
-        if (xx < 0 || yy < 0) continue;
-        const unsigned p = pidx(a, xx, yy);
-        int ui = a.usage[p];
-        a.dual[p] = dual_step(ui, a.dual[p], rho, step, neg);
-        ovf += (unsigned long long)max(ui - 1, 0);
-        if (ui > 1) flag_next(a, iter, xx, yy);
-      }
-      for (int o = 32; o > 0; o >>= 1) ovf += __shfl_down(ovf, o);
-      if (lane == 0) {
-        if (ovf) atomicAdd(&ctl->ovf_acc, ovf);
-        a.sorted[k] = 0;
-      }
-      PHASE_MARK(4);
-      PHASE_END();
-      return;
-    }
-  }
-
   extern __shared__ __align__(16) unsigned char lds_raw[];
   Tile T;
   T.rs = g.xh <= kRowStride ? kRowStride : g.xh;
@@ -1450,31 +1461,58 @@ __device__ __forceinline__ void partition_body(const Args& a, int bx, int npad_l
   int* keys = T.vdd + ((g.yh + 3) & ~3);                       // LDS sorts only
   int* slots = keys + npad_hi;
   int* aux = slots + npad_hi;
+  // exchange slots of the waves (four-wave launches, which have no LDS
+  // sorts): block-wide or, code range, and the wave minima of the moves
+  int* xch = keys;
+  unsigned* xmin = reinterpret_cast<unsigned*>(xch + 16);      // [2][NW]; xch[24]: deferred
+  (void)aux;
+
+  // -- activity test: does any cell of this partition need a move?  With
+  // every cell on a correct-parity row, a partition whose pixels all have
+  // usage <= 1 has none (its cells' footprints lie inside it); that test is
+  // fused into the tile staging below.  Otherwise each cell is checked.
+  const bool fast_ok = MODE == MODE_PRIMAL && !pert;
+  const bool cell_test = fast_ok && ctl->wrong_row > 0;
+  if (fast_ok && !cell_test && !adaptive_now && a.act_flags[(size_t)(iter & 1) * a.K + k] == 0) {
+    // no pixel above usage one: no cell moves, and k_dual_pass updates
+    // this partition's prices
+    if (tid == 0) a.sorted[k] = 0;
+    return;
+  }
+  if (cell_test) {
+    bool need = false;
+    for (int i = tid; i < count; i += NT) {
+      int4 rc = cellrec[src[i]];
+      int cx = rec_x(rc.x), cy = rec_y(rc.x), h = rec_h(rc.y);
+      need |= !correct_row(h, rec_gnd(rc.y), a.vdd_up[cy]) || ovf_hbm(a, cx, cy, rec_w(rc.y), h) != 0;
+    }
+    if (!block_any<NW>(need, xch, lane, wave)) {
+      // primal step is a no-op: stream the dual update over the pixels
+      unsigned long long ovf = 0;
+      int lb = tid / g.xh, la = tid - lb * g.xh;
+      for (int i = tid; i < npix; i += NT) {
+        int xx = g.gx(la), yy = g.gy(lb);
+        la += step_a; lb += step_b;
+        if (la >= g.xh) { la -= g.xh; lb++; }
+        if (xx < 0 || yy < 0) continue;
+        const unsigned p = pidx(a, xx, yy);
+        int ui = a.usage[p];
+        a.dual[p] = dual_step(ui, a.dual[p], rho, step, neg);
+        ovf += (unsigned long long)max(ui - 1, 0);
+        if (ui > 1) flag_next(a, iter, xx, yy);
+      }
+      for (int o = 32; o > 0; o >>= 1) ovf += __shfl_down(ovf, o);
+      if (lane == 0 && ovf) atomicAdd(&ctl->ovf_acc, ovf);
+      if (tid == 0) a.sorted[k] = 0;
+      PHASE_MARK(4);
+      PHASE_END();
+      return;
+    }
+  }
 
   bool over = false;
   int cmin = 0x7fffffff, cmax = -1;
-  if (stage_tile(a, g, T, rho, standard, over, cmin, cmax)) ctl->error = kErrUsageRange;
-  if (fast_ok && !cell_test && !__any(over)) {
-    // no cell can move: dual update from the staged tile (usage unchanged,
-    // overflow zero)
-    int lb = lane / g.xh, la = lane - lb * g.xh;
-    for (int i = lane; i < npix; i += kWave) {
-      int xx = g.gx(la), yy = g.gy(lb);
-      const int li = T.at(la, lb);
-      la += step_a; lb += step_b;
-      if (la >= g.xh) { la -= g.xh; lb++; }
-      if (xx < 0 || yy < 0) continue;
-      a.dual[pidx(a, xx, yy)] = dual_step((int)(T.uc[li] & 0xffffu), T.lam[li], rho, step, neg);
-    }
-    if (lane == 0) a.sorted[k] = 0;
-    PHASE_MARK(4);
-    PHASE_END();
-    return;
-  }
-  for (int b = lane; b < g.yh; b += kWave) {
-    int yy = g.gy(b);
-    T.vdd[b] = yy >= 0 ? a.vdd_up[yy] : 0;
-  }
+  if (stage_tile<NW>(a, g, T, rho, standard, over, cmin, cmax)) ctl->error = kErrUsageRange;
   // one placeable code on the tile's placeable pixels (the others carry
   // +inf penalty terms): candidate evaluation then needs no per-pixel code
   // test
@@ -1482,8 +1520,40 @@ __device__ __forceinline__ void partition_body(const Args& a, int bx, int npad_l
     cmin = min(cmin, __shfl_xor(cmin, o));
     cmax = max(cmax, __shfl_xor(cmax, o));
   }
-  const int ucode = (MODE == MODE_PRIMAL && cmin == cmax && cmin > 0) ? cmin : -1;
+  if (NW > 1 && lane == 0) {
+    xch[8 + wave] = cmin;
+    xch[12 + wave] = cmax;
+  }
+  if (fast_ok && !cell_test && !block_any<NW>(over, xch + 4, lane, wave)) {
+    // no cell can move: dual update from the staged tile (usage unchanged,
+    // overflow zero)
+    int lb = tid / g.xh, la = tid - lb * g.xh;
+    for (int i = tid; i < npix; i += NT) {
+      int xx = g.gx(la), yy = g.gy(lb);
+      const int li = T.at(la, lb);
+      la += step_a; lb += step_b;
+      if (la >= g.xh) { la -= g.xh; lb++; }
+      if (xx < 0 || yy < 0) continue;
+      a.dual[pidx(a, xx, yy)] = dual_step((int)(T.uc[li] & 0xffffu), T.lam[li], rho, step, neg);
+    }
+    if (tid == 0) a.sorted[k] = 0;
+    PHASE_MARK(4);
+    PHASE_END();
+    return;
+  }
+  for (int b = tid; b < g.yh; b += NT) {
+    int yy = g.gy(b);
+    T.vdd[b] = yy >= 0 ? a.vdd_up[yy] : 0;
+  }
   __syncthreads();
+  if (NW > 1) {
+#pragma unroll
+    for (int w2 = 0; w2 < NW; w2++) {
+      cmin = min(cmin, xch[8 + w2]);
+      cmax = max(cmax, xch[12 + w2]);
+    }
+  }
+  const int ucode = (MODE == MODE_PRIMAL && cmin == cmax && cmin > 0) ? cmin : -1;
   // rail of every tile row as one mask (rows < 64), for the candidate test
   const unsigned long long vmask = __ballot(lane < g.yh && T.vdd[lane < g.yh ? lane : 0] != 0);
   PHASE_MARK(0);
@@ -1495,7 +1565,7 @@ __device__ __forceinline__ void partition_body(const Args& a, int bx, int npad_l
       // an unsorted partition at the final refine moved nothing in the last
       // iteration, so its usage still is the one the order is defined on
       if (LDS_SORT) order_partition_lds(a, g, T, npad, npad_hi, src, off, count, cellrec, keys, slots, aux);
-      else {
+      else if (wave == 0) {  // the register sorts run on the first wave
 #ifdef LG_PHASE_TIMERS
 #define ORDER_REG(RN) order_partition_reg<RN>(a, g, T, npad, src, off, count, cellrec, ph_)
 #else
@@ -1506,22 +1576,33 @@ __device__ __forceinline__ void partition_body(const Args& a, int bx, int npad_l
         bool ok = false;
         if (npad <= 4 * kWave) ok = ORDER_REG(4);
         else if (npad == 8 * kWave) ok = ORDER_REG(8);
-        if (!ok) ORDER_REG(16);
+        if constexpr (kDefer) {
+          if (tid == 0) xch[24] = ok ? 0 : 1;
+        } else {
+          if (!ok) ORDER_REG(16);
+        }
 #undef ORDER_REG
       }
-      if (MODE == MODE_PRIMAL && lane == 0) a.sorted[k] = 1;
+      if (MODE == MODE_PRIMAL && tid == 0) a.sorted[k] = 1;
     } else {
-      for (int i = lane; i < count; i += kWave) recs[i] = cellrec[ids[i]];
+      for (int i = tid; i < count; i += NT) recs[i] = cellrec[ids[i]];
     }
     __syncthreads();
+    if (kDefer && xch[24] != 0) {
+      if (tid == 0) defer_partition(a, k);
+      return;
+    }
     PHASE_MARK(1);
 
     const float thr = disp_threshold(a);
-    // this lane's four candidate moves d = j*64 + lane, for the whole kernel
-    int2 dir[4];
+    // this lane's candidate moves d = d0 + j*64, j < NC, for the whole
+    // kernel: four per lane on one wave, one per lane on four
+    constexpr int NC = 4 / NW;
+    const int d0 = wave * kWave + lane;
+    int2 dir[NC];
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-      int d = j * kWave + lane;
+    for (int j = 0; j < NC; j++) {
+      int d = d0 + j * kWave;
       dir[j] = d < kNumDirDefault ? c_dir_default[d] : make_int2(0, 0);
     }
     // -- the cells in order, 64 at a time ---------------------------------
@@ -1558,11 +1639,14 @@ __device__ __forceinline__ void partition_body(const Args& a, int bx, int npad_l
 #ifdef LG_PHASE_TIMERS
         sub_acc_[5]++;
 #endif
-        footprint_add(g, T, cx, cy, w, h, -1, rho, standard);  // rip-up
+        // every wave has finished its reads of the last commit's pixels
+        if (NW > 1) __syncthreads();
+        footprint_add<NW>(g, T, cx, cy, w, h, -1, rho, standard);  // rip-up
         __syncthreads();
         SUB_MARK(11);
-        float cst[4];
-        eval_candidates<MODE>(a, g, T, disp_coeff, thr, cx, cy, w, h, code, gnd, pox, poy, dir, cst, ucode, vmask);
+        float cst[NC];
+        eval_candidates<MODE, NC>(a, g, T, disp_coeff, thr, cx, cy, w, h, code, gnd, pox, poy, d0, dir, cst, ucode,
+                                  vmask);
         SUB_MARK(12);
         // Tree-min of AdmmKernels.hpp:847-858 over slots s = j*64 + lane: a
         // halving-stride tournament in which a tie keeps the lower slot.  Its
@@ -1572,8 +1656,8 @@ __device__ __forceinline__ void partition_body(const Args& a, int bx, int npad_l
         // order like the values.
         unsigned long long best = ~0ull;
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-          unsigned slot = (unsigned)(j * kWave + lane);
+        for (int j = 0; j < NC; j++) {
+          unsigned slot = (unsigned)(d0 + j * kWave);
           unsigned rev = __brev(slot) >> 24;
           unsigned long long key = ((unsigned long long)__float_as_uint(cst[j]) << 32) | rev;
           best = key < best ? key : best;
@@ -1587,18 +1671,34 @@ __device__ __forceinline__ void partition_body(const Args& a, int bx, int npad_l
         key_min_step<0x140, 0xf>(bhi, blo);  // row_mirror
         key_min_step<0x142, 0xa>(bhi, blo);  // row_bcast:15 into rows 1, 3
         key_min_step<0x143, 0xc>(bhi, blo);  // row_bcast:31 into rows 2, 3
-        const unsigned bc_bits = __builtin_amdgcn_readlane(bhi, 63);
-        const int bi = (int)(__builtin_amdgcn_readlane(blo, 63) & 0xffu);
+        unsigned bc_bits = __builtin_amdgcn_readlane(bhi, 63);
+        unsigned bc_lo = __builtin_amdgcn_readlane(blo, 63);
+        if (NW > 1) {
+          // minimum over the waves' minima
+          if (lane == kWave - 1) {
+            xmin[wave] = bhi;
+            xmin[NW + wave] = blo;
+          }
+          __syncthreads();
+#pragma unroll
+          for (int w2 = 0; w2 < NW; w2++) {
+            const unsigned ohi = xmin[w2], olo = xmin[NW + w2];
+            const bool take = ohi < bc_bits || (ohi == bc_bits && olo < bc_lo);
+            bc_bits = take ? ohi : bc_bits;
+            bc_lo = take ? olo : bc_lo;
+          }
+        }
+        const int bi = (int)(bc_lo & 0xffu);
         const int bslot = (int)(__brev((unsigned)bi) >> 24);
         int nx = cx, ny = cy;
         if (__uint_as_float(bc_bits) != kInf) {
           int2 dd = c_dir_default[bslot];
           new_pos(a, dd, w, h, cx, cy, nx, ny);
         }
-        __syncthreads();
+        if (NW == 1) __syncthreads();
         SUB_MARK(13);
-        footprint_add(g, T, nx, ny, w, h, +1, rho, standard);  // commit
-        if (lane == 0) cellrec[pid].x = rec_pos(nx, ny);
+        footprint_add<NW>(g, T, nx, ny, w, h, +1, rho, standard);  // commit
+        if (tid == 0) cellrec[pid].x = rec_pos(nx, ny);
         __syncthreads();
         SUB_MARK(14);
         start = first + 1;
@@ -1618,18 +1718,18 @@ __device__ __forceinline__ void partition_body(const Args& a, int bx, int npad_l
   // -- write back pixels; primal mode fuses the dual update and the overflow
   unsigned long long ovf = 0;
   {
-    // eight pixels per lane at a time, their LDS reads issued together; the
-    // primal step also sets the next iteration's activity flags
-    constexpr int U = 8;
-    int lb = lane / g.xh, la = lane - lb * g.xh;
-    for (int i0 = 0; i0 < npix; i0 += U * kWave) {
+    // 8 / NW pixels per thread at a time, their LDS reads issued together;
+    // the primal step also sets the next iteration's activity flags
+    constexpr int U = 8 / NW;
+    int lb = tid / g.xh, la = tid - lb * g.xh;
+    for (int i0 = 0; i0 < npix; i0 += U * NT) {
       int px[U], py[U], li[U];
 #pragma unroll
       for (int q = 0; q < U; q++) {
         px[q] = g.gx(la);
         py[q] = g.gy(lb);
         li[q] = T.at(la, lb);
-        if (i0 + q * kWave + lane >= npix || py[q] < 0) px[q] = -1;
+        if (i0 + q * NT + tid >= npix || py[q] < 0) px[q] = -1;
         la += step_a; lb += step_b;
         if (la >= g.xh) { la -= g.xh; lb++; }
       }
@@ -1663,9 +1763,9 @@ __device__ __forceinline__ void partition_body(const Args& a, int bx, int npad_l
   SUB_FLUSH();
   PHASE_END();
 }
-template <int MODE, int LDS_SORT>
-__global__ __launch_bounds__(64) void k_partition(Args a, int npad_lo, int npad_hi) {
-  partition_body<MODE, LDS_SORT>(a, blockIdx.x, npad_lo, npad_hi);
+template <int MODE, int LDS_SORT, int NW>
+__global__ __launch_bounds__(NW * 64) void k_partition(Args a, int npad_lo, int npad_hi) {
+  partition_body<MODE, LDS_SORT, NW>(a, blockIdx.x, npad_lo, npad_hi);
 }
 
 
@@ -2027,13 +2127,13 @@ __global__ __launch_bounds__(1024) void kb_adaptive(const Args* __restrict__ tab
 
 // reg_filter: 1 = only slots whose sorting algorithm is a bitonic network
 // (sorted in registers up to kNpadSmall slots), 0 = only the others, -1 = all
-template <int MODE, int LDS_SORT>
-__global__ __launch_bounds__(64) void kb_partition(const Args* __restrict__ tab, const int2* __restrict__ map,
-                                                   int npad_lo, int npad_hi, int reg_filter) {
+template <int MODE, int LDS_SORT, int NW>
+__global__ __launch_bounds__(NW * 64) void kb_partition(const Args* __restrict__ tab, const int2* __restrict__ map,
+                                                        int npad_lo, int npad_hi, int reg_filter) {
   LG_BATCH_SLOT();
   const int reg = (a.sort_algo == LG_SORT_M_BITONIC || a.sort_algo == LG_SORT_S_BITONIC) ? 1 : 0;
   if (reg_filter >= 0 && reg != reg_filter) return;
-  partition_body<MODE, LDS_SORT>(a, m_.y, npad_lo, npad_hi);
+  partition_body<MODE, LDS_SORT, NW>(a, m_.y, npad_lo, npad_hi);
 }
 
 // local block l of the solo launch's (ceil(x_size / 1024), y_size) grid

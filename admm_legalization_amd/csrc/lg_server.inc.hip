@@ -176,13 +176,15 @@ int srv_launch_iteration(LgServer* S, int it) {
     HIP_TRY(hipEventRecord(e[0], s));
   }
   if (S->any_reg)
-    hipLaunchKernelGGL((kb_partition<MODE_PRIMAL, 0>), dim3(S->mk.n), dim3(kWave), S->lds_reg, s, tab, at(S->mk), 0,
+    hipLaunchKernelGGL((kb_partition<MODE_PRIMAL, 0, kPartWaves>), dim3(S->mk.n), dim3(kPartWaves * kWave), S->lds_reg,
+                       s, tab, at(S->mk), 0,
                        kNpadSmall, 1);
   if (S->any_nonreg)
-    hipLaunchKernelGGL((kb_partition<MODE_PRIMAL, 1>), dim3(S->mk.n), dim3(kWave), S->lds_small, s, tab, at(S->mk),
+    hipLaunchKernelGGL((kb_partition<MODE_PRIMAL, 1, 1>), dim3(S->mk.n), dim3(kWave), S->lds_small, s, tab, at(S->mk),
                        0, kNpadSmall, 0);
-  hipLaunchKernelGGL((kb_partition<MODE_PRIMAL, 1>), dim3(S->mk.n), dim3(kWave), S->lds_big, s, tab, at(S->mk),
-                     kNpadSmall + 1, kMaxNpad, -1);
+  // the large class also takes the partitions the four-wave launch deferred
+  hipLaunchKernelGGL((kb_partition<MODE_PRIMAL, 1, 1>), dim3(S->mk.n), dim3(kWave), S->lds_big, s, tab, at(S->mk), 1,
+                     kMaxNpad, -1);
   if (e) HIP_TRY(hipEventRecord(e[1], s));
   hipLaunchKernelGGL(kb_dual_pass, dim3(S->mpix.n), dim3(256), 0, s, tab, at(S->mpix));
   hipLaunchKernelGGL(kb_finish, dim3(S->m1.n), dim3(1), 0, s, tab, at(S->m1));

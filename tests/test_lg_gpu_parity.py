@@ -109,6 +109,33 @@ def test_ranked_sort_keys(gpu, monkeypatch, policy):
     _assert_same(g, o)
 
 
+@pytest.mark.parametrize("x_hint,y_hint", [(300, 12), (300, 24), (150, 12)])
+def test_dense_partitions(gpu, x_hint, y_hint):
+    """Partitions of 513-1024 cells (sorted in LDS by the one-wave launch
+    after the four-wave launch hands them over) and of more than 1024 cells
+    (the large class), mixed with small ones.  Cells per partition at the
+    start: (300, 12) 182 / 837-933 / 1053-1208; (300, 24) 182 / 1698-2261;
+    (150, 12) 83-511 / 705-718."""
+    d = synth.make_design(6000, 300, 75, height_mix=synth.MIXED_HEIGHTS, width_range=(1, 4), num_macros=2,
+                          hotspots=2, hotspot_frac=0.1, hotspot_sigma_rows=4.0, seed=9, name="dense")
+    p = lg.default_params()
+    p.max_admm_iter = 40
+    p.x_hint, p.y_hint = x_hint, y_hint
+    g, o = _run_both(d, p, do_post=False)
+    _assert_same(g, o)
+
+
+def test_partition_larger_than_lds_is_rejected(gpu):
+    """A partition tile of 660 x 24 pixels (190 KB) does not fit a
+    workgroup's LDS: lg_create fails with a clear error, before any launch."""
+    d = synth.mixed(3000, seed=2)
+    p = lg.default_params()
+    p.x_hint, p.y_hint = 660, 24
+    grid, cells, p2 = lg.device_inputs(d, p)
+    with pytest.raises(lg.LegalizerError, match="LDS"):
+        lg.DeviceLegalizer(grid, cells, p2)
+
+
 def test_qp_refine(gpu):
     d = synth.make_design(2000, 250, 50, width_range=(1, 8), num_macros=2, seed=6, name="qp")
     p = lg.default_params()
