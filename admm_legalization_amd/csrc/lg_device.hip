@@ -197,6 +197,11 @@ struct LgHandle {
   int ev_used = 0;
   double prof_ms[2] = {0, 0};
   int64_t prof_n[2] = {0, 0};
+  // side stream of the dual pass, which runs beside the partition kernels
+  // (it updates only the pixels of partitions they leave alone); created at
+  // the first solve
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 namespace {
@@ -622,14 +627,34 @@ int launch_iteration(LgHandle* h, hipStream_t s) {
   // cells of the listed partitions into their buckets
   if (a.N > 0) hipLaunchKernelGGL(k_bucket, dim3(nb), dim3(256), 0, s, a);
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, a);
+  // the dual pass needs the step k_scan sets; the pixels it updates belong
+  // to partitions the primal step leaves alone, so it runs on the side
+  // stream beside the partition kernels, joined before k_finish
+  const bool fork = h->side != nullptr;
+  if (fork) {
+    HIP_TRY(hipEventRecord(h->ev_fork, s));
+    HIP_TRY(hipStreamWaitEvent(h->side, h->ev_fork, 0));
+    hipLaunchKernelGGL(k_dual_pass, pix_grid, dim3(256), 0, h->side, a);
+    HIP_TRY(hipEventRecord(h->ev_join, h->side));
+  }
   if (a.use_adaptive) {
     hipLaunchKernelGGL(k_extract_congested, dim3(nb), dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_adaptive, dim3(1), dim3(1024), 0, s, a);
   }
   int st;
   if ((st = launch_partition<MODE_PRIMAL>(h, s))) return st;
-  hipLaunchKernelGGL(k_dual_pass, pix_grid, dim3(256), 0, s, a);
+  if (fork) HIP_TRY(hipStreamWaitEvent(s, h->ev_join, 0));
+  else hipLaunchKernelGGL(k_dual_pass, pix_grid, dim3(256), 0, s, a);
   hipLaunchKernelGGL(k_finish, dim3(1), dim3(1), 0, s, a);
+  return LG_OK;
+}
+
+// side stream and fork / join events of the dual pass
+int ensure_side(LgHandle* h) {
+  if (h->side) return LG_OK;
+  HIP_TRY(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
+  HIP_TRY(hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming));
+  HIP_TRY(hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking));
   return LG_OK;
 }
 
@@ -684,6 +709,7 @@ int lg_solve(LgHandle* h, void* stream) {
   int st;
   if ((st = solve_begin(h, s))) return st;
   if (a.N > 0 && h->p.max_admm_iter > 0) {
+    if ((st = ensure_side(h))) return st;
     int batch = 8;
     for (;;) {
       for (int b = 0; b < batch; b++)
@@ -841,6 +867,12 @@ int lg_debug_iter_times(LgHandle* h, int32_t* maxt, int32_t* sumt, int32_t* slow
 int lg_destroy(LgHandle* h) {
   if (!h) return LG_OK;
   for (auto& e : h->ev) (void)hipEventDestroy(e);
+  if (h->side) {
+    (void)hipStreamSynchronize(h->side);
+    (void)hipStreamDestroy(h->side);
+  }
+  if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
+  if (h->ev_join) (void)hipEventDestroy(h->ev_join);
   for (void* b : h->bufs) {
     if (h->pool) h->pool->release(b);
     else (void)hipFree(b);

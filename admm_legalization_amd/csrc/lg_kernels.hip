@@ -1769,9 +1769,10 @@ __global__ __launch_bounds__(NW * 64) void k_partition(Args a, int npad_lo, int 
 }
 
 
-// Dual update, after the primal step, of the pixels of partitions that took
-// no step (dual_update_functor_with_step_size, AdmmKernels.hpp:26-42).  Runs
-// only while the activity flags are in use.  A pixel of usage 1 has a zero
+// Dual update of the pixels of partitions that take no primal step
+// (dual_update_functor_with_step_size, AdmmKernels.hpp:26-42).  Runs only
+// while the activity flags are in use, beside the partition kernels: the two
+// touch disjoint pixels.  A pixel of usage 1 has a zero
 // increment, so its dual is read only for other usages.  Four consecutive
 // pixels of a row per thread, rows in blockIdx.y.  The activity flags of the
 // next iteration are set by the primal step itself.
@@ -1807,7 +1808,10 @@ __device__ __forceinline__ void dual_pass_body(const Args& a, int bx, int by) {
   }
   if (!any) return;
   const float rho = ctl->rho, step = ctl->step, neg = -1.0f / rho;
-  if (full) {
+  // a vector store only when all four pixels are updated here: a pixel of a
+  // partition the primal step processes may share the group, and that step
+  // runs at the same time
+  if (full && upd[0] && upd[1] && upd[2] && upd[3]) {
     float4 d = *reinterpret_cast<const float4*>(a.dual + p0);
     if (upd[0]) d.x = dual_step(u[0], d.x, rho, step, neg);
     if (upd[1]) d.y = dual_step(u[1], d.y, rho, step, neg);

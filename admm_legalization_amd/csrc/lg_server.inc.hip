@@ -59,6 +59,8 @@ struct LgServer {
   LgSrvConfig cfg{};
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;  // the batched dual pass, beside the partition kernels
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   DevPool pool;
   Ctl* d_ctl = nullptr;   // [max_slots] iteration control of every slot
   Ctl* h_ctl = nullptr;   // pinned host copy
@@ -165,6 +167,11 @@ int srv_launch_iteration(LgServer* S, int it) {
   auto at = [&](const MapRange& r) { return mp + r.off; };
   if (S->m256.n > 0) hipLaunchKernelGGL(kb_bucket, dim3(S->m256.n), dim3(256), 0, s, tab, at(S->m256));
   hipLaunchKernelGGL(kb_scan, dim3(S->m1.n), dim3(1024), 0, s, tab, at(S->m1));
+  // the dual pass on the side stream, beside the partition kernels
+  HIP_TRY(hipEventRecord(S->ev_fork, s));
+  HIP_TRY(hipStreamWaitEvent(S->side, S->ev_fork, 0));
+  hipLaunchKernelGGL(kb_dual_pass, dim3(S->mpix.n), dim3(256), 0, S->side, tab, at(S->mpix));
+  HIP_TRY(hipEventRecord(S->ev_join, S->side));
   if (S->any_adaptive) {
     if (S->m256.n > 0)
       hipLaunchKernelGGL(kb_extract_congested, dim3(S->m256.n), dim3(256), 0, s, tab, at(S->m256));
@@ -186,7 +193,7 @@ int srv_launch_iteration(LgServer* S, int it) {
   hipLaunchKernelGGL((kb_partition<MODE_PRIMAL, 1, 1>), dim3(S->mk.n), dim3(kWave), S->lds_big, s, tab, at(S->mk), 1,
                      kMaxNpad, -1);
   if (e) HIP_TRY(hipEventRecord(e[1], s));
-  hipLaunchKernelGGL(kb_dual_pass, dim3(S->mpix.n), dim3(256), 0, s, tab, at(S->mpix));
+  HIP_TRY(hipStreamWaitEvent(s, S->ev_join, 0));
   hipLaunchKernelGGL(kb_finish, dim3(S->m1.n), dim3(1), 0, s, tab, at(S->m1));
   HIP_TRY(hipGetLastError());
   return LG_OK;
@@ -410,6 +417,9 @@ int lg_srv_create(const LgSrvConfig* c, LgServer** out) {
     if (S->h_ctl) (void)hipHostFree(S->h_ctl);
     if (S->h_tab) (void)hipHostFree(S->h_tab);
     if (S->stream) (void)hipStreamDestroy(S->stream);
+    if (S->side) (void)hipStreamDestroy(S->side);
+    if (S->ev_fork) (void)hipEventDestroy(S->ev_fork);
+    if (S->ev_join) (void)hipEventDestroy(S->ev_join);
     delete S;
     g_err = msg;
     return code;
@@ -419,6 +429,9 @@ int lg_srv_create(const LgSrvConfig* c, LgServer** out) {
   S->pool.chunk_bytes = (size_t)c->pool_chunk_bytes;
   const size_t ns = (size_t)c->max_slots;
   if (hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&S->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&S->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&S->ev_join, hipEventDisableTiming) != hipSuccess ||
       hipMalloc((void**)&S->d_ctl, ns * sizeof(Ctl)) != hipSuccess ||
       hipMalloc((void**)&S->d_tab, ns * sizeof(Args)) != hipSuccess ||
       hipHostMalloc((void**)&S->h_ctl, ns * sizeof(Ctl), hipHostMallocDefault) != hipSuccess ||
@@ -561,6 +574,10 @@ int lg_srv_destroy(LgServer* S) {
   (void)hipHostFree(S->h_ctl);
   (void)hipHostFree(S->h_tab);
   for (auto& e : S->ev) (void)hipEventDestroy(e);
+  (void)hipStreamSynchronize(S->side);
+  (void)hipStreamDestroy(S->side);
+  (void)hipEventDestroy(S->ev_fork);
+  (void)hipEventDestroy(S->ev_join);
   (void)hipStreamDestroy(S->stream);
   delete S;
   return LG_OK;

@@ -63,6 +63,18 @@ def test_mixed_5k_200_iterations(gpu, seed):
     _assert_same(g, o)
 
 
+def test_mixed_30k_many_partition_columns(gpu):
+    """Nine partition columns whose boundaries (x_hint 250 under the fold
+    offsets 0 / 83 / 166) cut through groups of four pixels: the dual pass,
+    which runs beside the partition kernels, must leave the other partitions'
+    pixels of such a group alone."""
+    d = synth.mixed(30000, seed=5)
+    p = lg.default_params()
+    p.max_admm_iter = 300
+    g, o = _run_both(d, p, do_post=False)
+    _assert_same(g, o)
+
+
 def test_plain_small_converges_bit_exact(gpu):
     d = synth.make_design(2000, 240, 50, width_range=(1, 8), num_macros=3, hotspots=1,
                           hotspot_frac=0.03, hotspot_sigma_rows=4.0, seed=5, name="plain_2k")
