@@ -264,8 +264,8 @@ size_t partition_lds_bytes(int x_hint, int y_hint, int algo, int npad_hi, int ld
   const size_t rs = x_hint <= kRowStride ? kRowStride : x_hint;
   const size_t tsz = (rs * y_hint + 3) & ~(size_t)3;
   size_t tile = 3 * tsz * 4 + (size_t)((y_hint + 3) & ~3) * 4;
-  // + the waves' exchange slots (24 words) of the four-wave launches
-  return tile + (lds_sort ? (size_t)npad_hi * sort_bytes_per(algo) : 128);
+  // + the four-wave launches' exchange words (64) and sort arrays (2 x 256)
+  return tile + (lds_sort ? (size_t)npad_hi * sort_bytes_per(algo) : (64 + 512) * 4);
 }
 
 size_t partition_lds_bytes(const LgHandle* h, int npad_hi, int lds_sort) {

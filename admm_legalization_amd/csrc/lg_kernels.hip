@@ -1055,11 +1055,41 @@ __device__ __forceinline__ int sort_key(const Args& a, const PartGeo& g, const T
 // Compare-exchange semantics are those of AdmmKernels.hpp:899-945: the pair
 // (lo, lo ^ j) is ordered descending if (lo & k) == 0, and swapped only if
 // strictly out of order.
-template <bool PK>
-__device__ __forceinline__ bool net_lt(int a, int b) {
-  return PK ? ((unsigned)a >> 10) < ((unsigned)b >> 10) : a < b;
+// v of lane (lane ^ d) for d = 1, 2, 4, 8 by DPP within a row of sixteen
+// lanes (row_shl:N reads lane + N, row_shr:N lane - N, row_ror:8 lane ^ 8);
+// other distances through the LDS crossbar
+template <int D>
+__device__ __forceinline__ int xor_lanes(int v) {
+  if constexpr (D == 1) return __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+  else if constexpr (D == 2) return __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, false);  // [2,3,0,1]
+  else if constexpr (D == 4) {
+    const int up = __builtin_amdgcn_update_dpp(v, v, 0x104, 0xF, 0xF, false);  // row_shl:4
+    const int dn = __builtin_amdgcn_update_dpp(v, v, 0x114, 0xF, 0xF, false);  // row_shr:4
+    return (threadIdx.x & 4) ? dn : up;
+  } else if constexpr (D == 8) return __builtin_amdgcn_update_dpp(v, v, 0x128, 0xF, 0xF, false);  // row_ror:8
+  else return __shfl_xor(v, D);
 }
 
+// minimum and maximum over the wave, in every lane
+__device__ __forceinline__ void wave_min_max(int& mn, int& mx) {
+  mn = min(mn, xor_lanes<1>(mn)); mx = max(mx, xor_lanes<1>(mx));
+  mn = min(mn, xor_lanes<2>(mn)); mx = max(mx, xor_lanes<2>(mx));
+  mn = min(mn, xor_lanes<4>(mn)); mx = max(mx, xor_lanes<4>(mx));
+  mn = min(mn, xor_lanes<8>(mn)); mx = max(mx, xor_lanes<8>(mx));
+  mn = min(mn, xor_lanes<16>(mn)); mx = max(mx, xor_lanes<16>(mx));
+  mn = min(mn, xor_lanes<32>(mn)); mx = max(mx, xor_lanes<32>(mx));
+}
+
+// comparison key of a network word: the key bits of a packed word, or the
+// int with its sign flipped so that unsigned order is int order
+template <bool PK>
+__device__ __forceinline__ unsigned net_cmp(int v) {
+  return PK ? ((unsigned)v >> 10) : ((unsigned)v ^ 0x80000000u);
+}
+
+// Pair (lo, lo + J) of one lane's registers.  Descending pairs ((lo & k) ==
+// 0) swap if key(lo) < key(hi), ascending ones if key(hi) < key(lo): with
+// x, y the two keys in the order the pair's direction gives, swap iff x < y.
 template <int RN, int J, bool PK>
 __device__ __forceinline__ void net_step_lane(int k, int (&key)[RN], int (&slot)[RN]) {
   const int lane = threadIdx.x;
@@ -1067,19 +1097,42 @@ __device__ __forceinline__ void net_step_lane(int k, int (&key)[RN], int (&slot)
   for (int r = 0; r < RN; r++) {
     const int rp = r ^ J;
     if (rp > r) {
-      const int lo = lane * RN + r;
-      const bool desc = (lo & k) == 0;
+      const bool desc = ((lane * RN + r) & k) == 0;
       const int kl = key[r], kh = key[rp];
-      const bool sw = desc ? net_lt<PK>(kl, kh) : net_lt<PK>(kh, kl);
-      if (sw) {
-        key[r] = kh;
-        key[rp] = kl;
-        if (!PK) {
-          const int t = slot[r];
-          slot[r] = slot[rp];
-          slot[rp] = t;
-        }
+      const unsigned cl = net_cmp<PK>(kl), ch = net_cmp<PK>(kh);
+      const bool sw = (desc ? cl : ch) < (desc ? ch : cl);
+      key[r] = sw ? kh : kl;
+      key[rp] = sw ? kl : kh;
+      if (!PK) {
+        const int sl = slot[r], sh = slot[rp];
+        slot[r] = sw ? sh : sl;
+        slot[rp] = sw ? sl : sh;
       }
+    }
+  }
+}
+
+// Register r of lanes l and l ^ D (slots i and i ^ j, j = D * RN).  Both
+// slots of a pair have the same bit k (k > j), so the pair is descending iff
+// (i & k) == 0; this lane holds the pair's low slot iff (i & j) == 0.  It
+// takes the partner's word iff the pair swaps: for the low slot of a
+// descending pair iff mine < other, for its high slot iff other < mine, and
+// the reverse for an ascending pair -- x < y with (x, y) = (mine, other) when
+// both tests agree, else (other, mine).
+template <int RN, int D, bool PK>
+__device__ __forceinline__ void net_step_cross(int k, int j, int (&key)[RN], int (&slot)[RN]) {
+  const int lane = threadIdx.x;
+#pragma unroll
+  for (int r = 0; r < RN; r++) {
+    const int i = lane * RN + r;
+    const bool t = ((i & k) == 0) == ((i & j) == 0);
+    const int ok = xor_lanes<D>(key[r]);
+    const unsigned cm = net_cmp<PK>(key[r]), co = net_cmp<PK>(ok);
+    const bool take = (t ? cm : co) < (t ? co : cm);
+    key[r] = take ? ok : key[r];
+    if (!PK) {
+      const int os = xor_lanes<D>(slot[r]);
+      slot[r] = take ? os : slot[r];
     }
   }
 }
@@ -1095,21 +1148,14 @@ __device__ __forceinline__ void net_sort(int n, int (&key)[RN], int (&slot)[RN])
         if (RN > 4 && j == 4) net_step_lane<RN, (RN > 4 ? 4 : 1), PK>(k, key, slot);
         if (RN > 8 && j == 8) net_step_lane<RN, (RN > 8 ? 8 : 1), PK>(k, key, slot);
       } else {
-        const int d = j / RN;
-        const bool i_lo = (lane & d) == 0;
-#pragma unroll
-        for (int r = 0; r < RN; r++) {
-          const int i = lane * RN + r;
-          const int lo = i_lo ? i : (i ^ j);
-          const bool desc = (lo & k) == 0;
-          const int ok = __shfl_xor(key[r], d);
-          const int os = PK ? 0 : __shfl_xor(slot[r], d);
-          const int klo = i_lo ? key[r] : ok, khi = i_lo ? ok : key[r];
-          const bool sw = desc ? net_lt<PK>(klo, khi) : net_lt<PK>(khi, klo);
-          if (sw) {
-            key[r] = ok;
-            if (!PK) slot[r] = os;
-          }
+        // register r of lanes l and l ^ (j / RN), j / RN <= 32
+        switch (j / RN) {
+          case 1: net_step_cross<RN, 1, PK>(k, j, key, slot); break;
+          case 2: net_step_cross<RN, 2, PK>(k, j, key, slot); break;
+          case 4: net_step_cross<RN, 4, PK>(k, j, key, slot); break;
+          case 8: net_step_cross<RN, 8, PK>(k, j, key, slot); break;
+          case 16: net_step_cross<RN, 16, PK>(k, j, key, slot); break;
+          default: net_step_cross<RN, 32, PK>(k, j, key, slot); break;
         }
       }
     }
@@ -1164,8 +1210,9 @@ __device__ __forceinline__ bool order_partition_reg(const Args& a, const PartGeo
       const int v = __shfl(-kk[q], ll);
       if (q == lr) last = v;
     }
+    // (ids clamped into the cell range: the loads below stay in bounds)
 #pragma unroll
-    for (int r = 0; r < RN; r++) val[r] = lane * RN + r < count ? -kk[r] : last;
+    for (int r = 0; r < RN; r++) val[r] = min(max(lane * RN + r < count ? -kk[r] : last, 0), a.N - 1);
     if (RN > 8) {
       // the wide instance gathers by slot through memory
 #pragma unroll
@@ -1205,10 +1252,7 @@ __device__ __forceinline__ bool order_partition_reg(const Args& a, const PartGeo
     for (int q = 0; q < RN; q++)
       if (q == r) key[q] = kk;
   }
-  for (int o = 1; o < kWave; o <<= 1) {
-    kmin = min(kmin, __shfl_xor(kmin, o));
-    kmax = max(kmax, __shfl_xor(kmax, o));
-  }
+  wave_min_max(kmin, kmax);
   if (kmin < 0 || kmax >= a.pack_limit) {
     // the instance of 16 registers handles such keys
     if (RN < 16) return false;
@@ -1292,6 +1336,84 @@ __device__ __forceinline__ bool order_partition_reg(const Args& a, const PartGeo
 #ifdef LG_PHASE_TIMERS
   ph_[7] = (int)(wall_clock64() - t2_);
 #endif
+  return true;
+}
+
+// The network on n <= 256 slots held one per thread of a four-wave
+// workgroup (slot = thread): distances below 64 within a wave (DPP / LDS
+// crossbar), 64 and 128 through the LDS words xw[256].  Same pairs,
+// directions and swap rule as net_step_cross.
+template <bool PK>
+__device__ __forceinline__ void coop_net(int n, int& w, int* xw) {
+  const int tid = threadIdx.x;
+  for (int k = 2; k <= n; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      int o;
+      if (j >= kWave) {
+        xw[tid] = w;
+        __syncthreads();
+        o = xw[tid ^ j];
+        __syncthreads();
+      } else {
+        switch (j) {
+          case 1: o = xor_lanes<1>(w); break;
+          case 2: o = xor_lanes<2>(w); break;
+          case 4: o = xor_lanes<4>(w); break;
+          case 8: o = xor_lanes<8>(w); break;
+          case 16: o = xor_lanes<16>(w); break;
+          default: o = xor_lanes<32>(w); break;
+        }
+      }
+      const bool t = ((tid & k) == 0) == ((tid & j) == 0);
+      const unsigned cm = net_cmp<PK>(w), co = net_cmp<PK>(o);
+      w = ((t ? cm : co) < (t ? co : cm)) ? o : w;
+    }
+  }
+}
+
+// Processing order of a partition of npad <= 256 slots by the four waves,
+// one slot per thread: the same two networks and keys as
+// order_partition_reg.  sids[256] (LDS) holds the ids in ascending order,
+// xw[256] the network's exchange words, xch[32..48) the key range of the
+// waves.  Returns false, having written nothing, if the keys do not pack.
+__device__ __forceinline__ bool order_partition_coop(const Args& a, const PartGeo& g, const Tile& T, int npad,
+                                                     const int* src, int off, int count, const int4* cellrec,
+                                                     int* sids, int* xw, int* xch) {
+  constexpr int NW = 4;
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid >> 6;
+  int* ids = a.grouped + off;
+  int4* recb = reinterpret_cast<int4*>(a.rec_b) + off;
+  // the cells in ascending id order (a descending network on -id, pads
+  // INT_MIN last); ids clamped into the cell range
+  int w = tid < count ? -src[tid] : INT_MIN;
+  coop_net<false>(npad, w, xw);
+  const int id = tid < count ? min(max(-w, 0), a.N - 1) : 0;
+  sids[tid] = id;
+  int key = INT_MIN, kmin = INT_MAX, kmax = INT_MIN;
+  if (tid < count) {
+    key = sort_key(a, g, T, id, cellrec[id]);
+    kmin = kmax = key;
+  }
+  wave_min_max(kmin, kmax);
+  if (lane == 0) {
+    xch[32 + wave] = kmin;
+    xch[40 + wave] = kmax;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int w2 = 0; w2 < NW; w2++) {
+    kmin = min(kmin, xch[32 + w2]);
+    kmax = max(kmax, xch[40 + w2]);
+  }
+  if (kmin < 0 || kmax >= a.pack_limit) return false;
+  int word = (int)(((unsigned)(key == INT_MIN ? 0 : key + 1) << 10) | (unsigned)tid);
+  coop_net<true>(npad, word, xw);
+  // pads sort behind every cell: slots below count hold cells
+  if (tid < count) {
+    const int c = sids[word & 1023];
+    ids[tid] = c;
+    recb[tid] = cellrec[c];
+  }
   return true;
 }
 
@@ -1463,8 +1585,10 @@ __device__ __forceinline__ void partition_body(const Args& a, int bx, int npad_l
   int* aux = slots + npad_hi;
   // exchange slots of the waves (four-wave launches, which have no LDS
   // sorts): block-wide or, code range, and the wave minima of the moves
-  int* xch = keys;
+  int* xch = keys;                                             // [64]
   unsigned* xmin = reinterpret_cast<unsigned*>(xch + 16);      // [2][NW]; xch[24]: deferred
+  int* sids = xch + 64;                                        // [256] four-wave sorts
+  int* xw = sids + 256;                                        // [256]
   (void)aux;
 
   // -- activity test: does any cell of this partition need a move?  With
@@ -1516,10 +1640,7 @@ __device__ __forceinline__ void partition_body(const Args& a, int bx, int npad_l
   // one placeable code on the tile's placeable pixels (the others carry
   // +inf penalty terms): candidate evaluation then needs no per-pixel code
   // test
-  for (int o = 1; o < kWave; o <<= 1) {
-    cmin = min(cmin, __shfl_xor(cmin, o));
-    cmax = max(cmax, __shfl_xor(cmax, o));
-  }
+  wave_min_max(cmin, cmax);
   if (NW > 1 && lane == 0) {
     xch[8 + wave] = cmin;
     xch[12 + wave] = cmax;
@@ -1565,7 +1686,11 @@ __device__ __forceinline__ void partition_body(const Args& a, int bx, int npad_l
       // an unsorted partition at the final refine moved nothing in the last
       // iteration, so its usage still is the one the order is defined on
       if (LDS_SORT) order_partition_lds(a, g, T, npad, npad_hi, src, off, count, cellrec, keys, slots, aux);
-      else if (wave == 0) {  // the register sorts run on the first wave
+      else if (NW > 1 && npad <= NT) {
+        // one slot per thread
+        const bool ok = order_partition_coop(a, g, T, npad, src, off, count, cellrec, sids, xw, xch);
+        if (tid == 0) xch[24] = ok ? 0 : 1;
+      } else if (wave == 0) {  // the register sorts run on the first wave
 #ifdef LG_PHASE_TIMERS
 #define ORDER_REG(RN) order_partition_reg<RN>(a, g, T, npad, src, off, count, cellrec, ph_)
 #else
