@@ -55,6 +55,7 @@ constexpr int kNumDirPadded = 256;
 constexpr int kNumDirWide = 4093;
 constexpr int kNumDirWidePadded = 4096;
 constexpr int kMaxNpad = 4096;
+constexpr int kBucketCells = 1024;  // cells per workgroup of k_bucket
 constexpr int kNpadSmall = 1024;
 
 }  // namespace lgdev

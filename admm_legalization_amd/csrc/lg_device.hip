@@ -625,7 +625,7 @@ int launch_iteration(LgHandle* h, hipStream_t s) {
   const int nb = (a.N + 255) / 256;
   const dim3 pix_grid((a.x_size + 1023) / 1024, a.y_size);
   // cells of the listed partitions into their buckets
-  if (a.N > 0) hipLaunchKernelGGL(k_bucket, dim3(nb), dim3(256), 0, s, a);
+  if (a.N > 0) hipLaunchKernelGGL(k_bucket, dim3((a.N + kBucketCells - 1) / kBucketCells), dim3(256), 0, s, a);
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, a);
   // the dual pass needs the step k_scan sets; the pixels it updates belong
   // to partitions the primal step leaves alone, so it runs on the side

@@ -286,21 +286,42 @@ __device__ __forceinline__ void append_to_bucket(const Args& a, int p, int c) {
   if (s < kMaxNpad) a.bucket[(size_t)p * kMaxNpad + s] = c;
 }
 
+// kBucketCells cells per workgroup of 256 threads, four per thread (cells
+// bx * 1024 + q * 256 + t), each step's loads issued together; a cell's
+// partition from its corners by the arithmetic of the colmap / rowmap tables
 __device__ __forceinline__ void bucket_body(const Args& a, int bx) {
   const Ctl* ctl = a.ctl;
   if (ctl->done) return;
-  const int c = bx * blockDim.x + threadIdx.x;
-  if (c >= a.N) return;
   const int iter = ctl->iter;
   // the adaptive search's iteration orders no partition (count 0)
   if (a.use_adaptive && ctl->exceptional) return;
   const bool skip_ok = iter >= a.iter_perturbation_on && ctl->wrong_row == 0;
   const int tf = triple_fold(a, iter);
-  const int4 r = reinterpret_cast<const int4*>(a.cellrec)[c];
-  const int p = cell_partition(a, a.colmap + tf * a.x_size, a.rowmap + tf * a.y_size, r);
-  if (p >= a.K) return;
-  if (skip_ok && a.act_flags[(size_t)(iter & 1) * a.K + p] == 0) return;
-  append_to_bucket(a, p, c);
+  const int ox = a.x_hint / 3 * tf, oy = a.y_hint / 3 * tf;
+  const int* fl = a.act_flags + (size_t)(iter & 1) * a.K;
+  constexpr int Q = kBucketCells / 256;
+  const int c0 = bx * kBucketCells + (int)threadIdx.x;
+  int2 r[Q];
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const int c = c0 + q * 256;
+    r[q] = c < a.N ? reinterpret_cast<const int2*>(a.cellrec)[2 * c] : make_int2(0, 0);
+  }
+  int p[Q], f[Q];
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const int x = r[q].x & 0xffff, y = (int)((unsigned)r[q].x >> 16);
+    const int w = r[q].y & 0xfff, h = (r[q].y >> 12) & 0x3f;
+    const int c1 = ((x + ox) / a.x_hint) % a.ncol, c2 = ((x + w - 1 + ox) / a.x_hint) % a.ncol;
+    const int r1 = ((y + oy) / a.y_hint) % a.nrow, r2 = ((y + h - 1 + oy) / a.y_hint) % a.nrow;
+    // (assignPartitionKernel, AdmmKernels.hpp:122-162: none if the footprint spans two)
+    p[q] = (c0 + q * 256 < a.N && c1 == c2 && r1 == r2) ? c1 * a.nrow + r1 : a.K;
+  }
+#pragma unroll
+  for (int q = 0; q < Q; q++) f[q] = (p[q] < a.K && skip_ok) ? fl[p[q]] : 1;
+#pragma unroll
+  for (int q = 0; q < Q; q++)
+    if (p[q] < a.K && f[q] != 0) append_to_bucket(a, p[q], c0 + q * 256);
 }
 __global__ __launch_bounds__(256) void k_bucket(Args a) { bucket_body(a, blockIdx.x); }
 
@@ -356,6 +377,116 @@ __device__ __forceinline__ void block_scan_inplace(int* v, int n) {
   }
 }
 
+// Totals of the scan: off[K], off[K + 1], npad from the largest count, and
+// the dual step schedule.  One thread.
+__device__ __forceinline__ void scan_finish(const Args& a, int K, int m, int total) {
+  Ctl* ctl = a.ctl;
+  a.off[K] = total;
+  a.off[K + 1] = total;
+  int np = 1;
+  while (np < m) np <<= 1;
+  ctl->max_cnt = m;
+  ctl->npad = np;
+  if (np > kMaxNpad) { ctl->error = kErrPartitionTooDense; ctl->done = 1; }
+  int it = ctl->iter;
+  if (it % a.iter_update_dual_step == 0 && it > a.iter_threshold_dual_update) {
+    float inc = a.coeff_update_dual_step * ctl->lambda0;
+    ctl->step = ctl->step + inc;
+  }
+}
+
+// scan_body for K <= 8192: eight partitions per thread (i = j * 1024 + t),
+// counts kept in registers and scanned there: the exclusive prefix of i is
+// the sum of the (j', wave') totals before (j, wave) plus the wave's
+// exclusive prefix -- three barriers, no second pass over HBM.
+__device__ __forceinline__ void scan_small(const Args& a, int K, bool skip_ok, bool adaptive_now,
+                                           const int32_t* cf, int* s_n, int* s_max) {
+  __shared__ int tot[128], pre[128], s_total;
+  Ctl* ctl = a.ctl;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  const int iter = ctl->iter;
+  int cn[8], fl[8];
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    const int i = j * 1024 + t;
+    cn[j] = i < K ? a.bcnt[i] : 0;
+    fl[j] = (i < K && skip_ok) ? cf[i] : 1;
+  }
+  int mx = 0;
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    const int i = j * 1024 + t;
+    const bool in = i < K;
+    const int c = cn[j];
+    if (in) {
+      a.pcnt[i] = c;
+      a.bcnt[i] = 0;
+    }
+    mx = max(mx, c);
+    const bool run = in && fl[j] != 0;
+    if (in && !run) a.sorted[i] = 0;
+    const int cls = (!adaptive_now && c > kNpadSmall) ? 1 : 0;
+    // list order is free: partitions of one step are independent
+    const unsigned long long m0 = __ballot(run && cls == 0), m1 = __ballot(run && cls == 1);
+    int b0 = 0, b1 = 0;
+    if (lane == 0) {
+      if (m0) b0 = atomicAdd(&s_n[0], __popcll(m0));
+      if (m1) b1 = atomicAdd(&s_n[1], __popcll(m1));
+    }
+    b0 = __shfl(b0, 0);
+    b1 = __shfl(b1, 0);
+    if (run && cls == 0) a.work[b0 + __popcll(m0 & lt)] = i;
+    if (run && cls == 1) a.work[K + b1 + __popcll(m1 & lt)] = i;
+  }
+  // activity flags of the next iteration, set by the primal step
+  int32_t* nf = a.act_flags + (size_t)((iter + 1) & 1) * K;
+  for (int i = t; i < K; i += 1024) nf[i] = 0;
+  // inclusive wave scans of the eight rows
+  int incl[8];
+#pragma unroll
+  for (int j = 0; j < 8; j++) incl[j] = cn[j];
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const int z = __shfl_up(incl[j], o);
+      if (lane >= o) incl[j] += z;
+    }
+  }
+  if (lane == kWave - 1)
+#pragma unroll
+    for (int j = 0; j < 8; j++) tot[j * 16 + wave] = incl[j];
+  for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o));
+  if (lane == 0) s_max[wave] = mx;
+  __syncthreads();
+  if (t < 2) ctl->n_work[t] = s_n[t];
+  if (t < kWave) {
+    // exclusive scan of the 128 (row, wave) totals, two per lane
+    const int v0 = tot[2 * t], v1 = tot[2 * t + 1];
+    int x = v0 + v1;
+    for (int o = 1; o < kWave; o <<= 1) {
+      const int z = __shfl_up(x, o);
+      if (lane >= o) x += z;
+    }
+    const int ex = x - v0 - v1;
+    pre[2 * t] = ex;
+    pre[2 * t + 1] = ex + v0;
+    if (t == kWave - 1) s_total = x;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    const int i = j * 1024 + t;
+    if (i < K) a.off[i] = pre[j * 16 + wave] + incl[j] - cn[j];
+  }
+  if (t == 0) {
+    int m = 0;
+    for (int q = 0; q < 16; q++) m = max(m, s_max[q]);
+    scan_finish(a, K, m, s_total);
+  }
+}
+
 // Exclusive scan of the partition cell counts, max count -> npad, dual step
 // schedule (updateDualStepSize, CudaAdmmLegalizer.cu:293-294, 687-691), and
 // the work lists of the primal step: every partition except those whose
@@ -378,6 +509,10 @@ __device__ __forceinline__ void scan_body(const Args& a) {
   const unsigned long long lt = (1ull << lane) - 1ull;
   if (t < 2) s_n[t] = 0;
   __syncthreads();
+  if (K <= 8 * 1024) {
+    scan_small(a, K, skip_ok, adaptive_now, cf, s_n, s_max);
+    return;
+  }
   // cells per partition from the bucket counts (off <- counts, scanned
   // below; the counters are cleared for the next iteration); eight
   // partitions per thread with every load issued first
@@ -431,19 +566,7 @@ __device__ __forceinline__ void scan_body(const Args& a) {
   if (t == 0) {
     int m = 0;
     for (int q = 0; q < 16; q++) m = max(m, s_max[q]);
-    const int total = K > 0 ? a.off[K - 1] + last : 0;
-    a.off[K] = total;
-    a.off[K + 1] = total;
-    int np = 1;
-    while (np < m) np <<= 1;
-    ctl->max_cnt = m;
-    ctl->npad = np;
-    if (np > kMaxNpad) { ctl->error = kErrPartitionTooDense; ctl->done = 1; }
-    int it = ctl->iter;
-    if (it % a.iter_update_dual_step == 0 && it > a.iter_threshold_dual_update) {
-      float inc = a.coeff_update_dual_step * ctl->lambda0;
-      ctl->step = ctl->step + inc;
-    }
+    scan_finish(a, K, m, K > 0 ? a.off[K - 1] + last : 0);
   }
 }
 __global__ __launch_bounds__(1024) void k_scan(Args a) { scan_body(a); }

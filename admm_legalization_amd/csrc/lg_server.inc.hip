@@ -73,7 +73,7 @@ struct LgServer {
   std::vector<int> active;  // resident slots, in admission order
   bool dirty = true;
   // geometry of the current tables
-  MapRange m256, m1, mk, mpix;
+  MapRange m256, m1, mk, mpix, mbk;
   int any_adaptive = 0, any_reg = 0, any_nonreg = 0, top_slot = 0;
   size_t lds_reg = 0, lds_small = 0, lds_big = 0;
   // queue and request registry (guarded by mu)
@@ -115,7 +115,7 @@ int srv_grow_map(LgServer* S, size_t need) {
 
 // Slot table and block maps of the resident requests (server stream idle).
 int srv_build_tables(LgServer* S) {
-  std::vector<int2> m[4];
+  std::vector<int2> m[5];
   S->any_adaptive = S->any_reg = S->any_nonreg = 0;
   S->lds_reg = S->lds_small = S->lds_big = 0;
   S->top_slot = 0;
@@ -129,6 +129,7 @@ int srv_build_tables(LgServer* S) {
     for (int b = 0; b < a.K; b++) m[2].push_back(make_int2(sl, b));
     const int gx = (a.x_size + 1023) / 1024;
     for (int b = 0; b < gx * a.y_size; b++) m[3].push_back(make_int2(sl, b));
+    for (int b = 0; b < (a.N + kBucketCells - 1) / kBucketCells; b++) m[4].push_back(make_int2(sl, b));
     S->any_adaptive |= a.use_adaptive ? 1 : 0;
     const int reg = h->p.sorting_algorithm == LG_SORT_M_BITONIC || h->p.sorting_algorithm == LG_SORT_S_BITONIC;
     if (reg) {
@@ -141,8 +142,8 @@ int srv_build_tables(LgServer* S) {
     S->lds_big = std::max(S->lds_big, partition_lds_bytes(h, kMaxNpad, 1));
   }
   size_t total = 0;
-  MapRange* r[4] = {&S->m256, &S->m1, &S->mk, &S->mpix};
-  for (int i = 0; i < 4; i++) {
+  MapRange* r[5] = {&S->m256, &S->m1, &S->mk, &S->mpix, &S->mbk};
+  for (int i = 0; i < 5; i++) {
     r[i]->off = (int)total;
     r[i]->n = (int)m[i].size();
     total += m[i].size();
@@ -150,7 +151,7 @@ int srv_build_tables(LgServer* S) {
   if (total > (size_t)INT_MAX) return fail(LG_ERR_MEMORY, "request server: block maps too large");
   int st = srv_grow_map(S, total);
   if (st) return st;
-  for (int i = 0; i < 4; i++)
+  for (int i = 0; i < 5; i++)
     if (!m[i].empty()) std::memcpy(S->h_map + r[i]->off, m[i].data(), m[i].size() * sizeof(int2));
   HIP_TRY(hipMemcpyAsync(S->d_tab, S->h_tab, S->top_slot * sizeof(Args), hipMemcpyHostToDevice, S->stream));
   HIP_TRY(hipMemcpyAsync(S->d_map, S->h_map, total * sizeof(int2), hipMemcpyHostToDevice, S->stream));
@@ -165,7 +166,7 @@ int srv_launch_iteration(LgServer* S, int it) {
   const Args* tab = S->d_tab;
   const int2* mp = S->d_map;
   auto at = [&](const MapRange& r) { return mp + r.off; };
-  if (S->m256.n > 0) hipLaunchKernelGGL(kb_bucket, dim3(S->m256.n), dim3(256), 0, s, tab, at(S->m256));
+  if (S->mbk.n > 0) hipLaunchKernelGGL(kb_bucket, dim3(S->mbk.n), dim3(256), 0, s, tab, at(S->mbk));
   hipLaunchKernelGGL(kb_scan, dim3(S->m1.n), dim3(1024), 0, s, tab, at(S->m1));
   // the dual pass on the side stream, beside the partition kernels
   HIP_TRY(hipEventRecord(S->ev_fork, s));
