@@ -202,7 +202,14 @@ struct LgHandle {
   // the first solve
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // HIP graph of kGraphIters iterations (lg_use_graphs), captured on `cap`
+  // at the first solve and replayed on the solve's stream
+  int graphs = 0;
+  hipStream_t cap = nullptr;
+  hipGraphExec_t gexec = nullptr;
 };
+
+constexpr int kGraphIters = 8;
 
 namespace {
 
@@ -658,6 +665,30 @@ int ensure_side(LgHandle* h) {
   return LG_OK;
 }
 
+// kGraphIters iterations captured into one graph (once per handle: the
+// launches depend only on the handle's fixed arguments).  The capture only
+// records; nothing runs until the graph is launched.
+int capture_iterations(LgHandle* h) {
+  if (h->gexec) return LG_OK;
+  if (!h->cap) HIP_TRY(hipStreamCreateWithFlags(&h->cap, hipStreamNonBlocking));
+  HIP_TRY(hipStreamBeginCapture(h->cap, hipStreamCaptureModeThreadLocal));
+  int st = LG_OK;
+  for (int i = 0; i < kGraphIters && !st; i++) st = launch_iteration(h, h->cap);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(h->cap, &g);
+  if (st || e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return st ? st : fail(LG_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  }
+  const hipError_t ei = hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ei != hipSuccess) {
+    h->gexec = nullptr;
+    return fail(LG_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+  }
+  return LG_OK;
+}
+
 // After the last iteration, with h->host_ctl holding the device Ctl: limit
 // errors, the final refine of a converged solve (doFinalRefine,
 // CudaAdmmLegalizer.cu:1011-1115), positions back to the SoA arrays and the
@@ -710,10 +741,16 @@ int lg_solve(LgHandle* h, void* stream) {
   if ((st = solve_begin(h, s))) return st;
   if (a.N > 0 && h->p.max_admm_iter > 0) {
     if ((st = ensure_side(h))) return st;
-    int batch = 8;
+    const bool graph = h->graphs && !h->prof;
+    if (graph && (st = capture_iterations(h))) return st;
+    int batch = kGraphIters;  // then 16, 32, 64 iterations between checks
     for (;;) {
-      for (int b = 0; b < batch; b++)
-        if ((st = launch_iteration(h, s))) return st;
+      if (graph) {
+        for (int b = 0; b < batch; b += kGraphIters) HIP_TRY(hipGraphLaunch(h->gexec, s));
+      } else {
+        for (int b = 0; b < batch; b++)
+          if ((st = launch_iteration(h, s))) return st;
+      }
       HIP_TRY(hipGetLastError());
       if ((st = read_ctl(h, s))) return st;
       if (h->host_ctl.error) break;
@@ -845,6 +882,12 @@ int lg_profile_partition(LgHandle* h, int enable) {
   return LG_OK;
 }
 
+int lg_use_graphs(LgHandle* h, int32_t enable) {
+  if (!h) return fail(LG_ERR_ARG, "null handle");
+  h->graphs = enable ? 1 : 0;
+  return LG_OK;
+}
+
 int lg_profile_read(LgHandle* h, double* ms, int64_t* launches) {
   if (!h) return fail(LG_ERR_ARG, "null handle");
   for (int c = 0; c < 2; c++) {
@@ -873,6 +916,8 @@ int lg_destroy(LgHandle* h) {
   }
   if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
   if (h->ev_join) (void)hipEventDestroy(h->ev_join);
+  if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+  if (h->cap) (void)hipStreamDestroy(h->cap);
   for (void* b : h->bufs) {
     if (h->pool) h->pool->release(b);
     else (void)hipFree(b);

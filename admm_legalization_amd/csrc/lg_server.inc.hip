@@ -61,6 +61,11 @@ struct LgServer {
   hipStream_t stream = nullptr;
   hipStream_t side = nullptr;  // the batched dual pass, beside the partition kernels
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // one batched iteration as a HIP graph (cfg.use_graphs), captured on `cap`
+  // whenever the tables change
+  hipStream_t cap = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  bool graph_stale = true;
   DevPool pool;
   Ctl* d_ctl = nullptr;   // [max_slots] iteration control of every slot
   Ctl* h_ctl = nullptr;   // pinned host copy
@@ -156,13 +161,14 @@ int srv_build_tables(LgServer* S) {
   HIP_TRY(hipMemcpyAsync(S->d_tab, S->h_tab, S->top_slot * sizeof(Args), hipMemcpyHostToDevice, S->stream));
   HIP_TRY(hipMemcpyAsync(S->d_map, S->h_map, total * sizeof(int2), hipMemcpyHostToDevice, S->stream));
   S->dirty = false;
+  // the launch geometry changed: the graph of an iteration is captured anew
+  S->graph_stale = true;
   return LG_OK;
 }
 
 // One ADMM iteration of every resident request (the steps of
 // launch_iteration, each as one batched launch).
-int srv_launch_iteration(LgServer* S, int it) {
-  hipStream_t s = S->stream;
+int srv_launch_iteration(LgServer* S, int it, hipStream_t s) {
   const Args* tab = S->d_tab;
   const int2* mp = S->d_map;
   auto at = [&](const MapRange& r) { return mp + r.off; };
@@ -179,7 +185,7 @@ int srv_launch_iteration(LgServer* S, int it) {
     hipLaunchKernelGGL(kb_adaptive, dim3(S->m1.n), dim3(1024), 0, s, tab, at(S->m1));
   }
   hipEvent_t* e = nullptr;
-  if (S->prof_batch) {
+  if (S->prof_batch && s == S->stream) {
     e = &S->ev[2 * it];
     HIP_TRY(hipEventRecord(e[0], s));
   }
@@ -197,6 +203,31 @@ int srv_launch_iteration(LgServer* S, int it) {
   HIP_TRY(hipStreamWaitEvent(s, S->ev_join, 0));
   hipLaunchKernelGGL(kb_finish, dim3(S->m1.n), dim3(1), 0, s, tab, at(S->m1));
   HIP_TRY(hipGetLastError());
+  return LG_OK;
+}
+
+// One batched iteration captured into a graph (tables built, nothing runs)
+// and instantiated.
+int srv_capture(LgServer* S) {
+  HIP_TRY(hipStreamBeginCapture(S->cap, hipStreamCaptureModeThreadLocal));
+  const int st = srv_launch_iteration(S, 0, S->cap);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(S->cap, &g);
+  if (st || e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return st ? st : fail(LG_ERR_DEVICE, std::string("request server: capture: ") + hipGetErrorString(e));
+  }
+  if (S->gexec) (void)hipGraphExecDestroy(S->gexec);
+  S->gexec = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&S->gexec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ei != hipSuccess) {
+    S->gexec = nullptr;
+    return fail(LG_ERR_DEVICE, std::string("request server: hipGraphInstantiate: ") + hipGetErrorString(ei));
+  }
+  S->graph_stale = false;
+  std::lock_guard<std::mutex> lk(S->mu);
+  S->ctr.graph_captures++;
   return LG_OK;
 }
 
@@ -352,7 +383,13 @@ void srv_loop(LgServer* S) {
       S->iter_before.assign(S->top_slot, 0);
       for (int sl : S->active) S->iter_before[sl] = S->h_ctl[sl].iter;
     }
-    for (int it = 0; !st && it < S->cfg.iters_per_sync; it++) st = srv_launch_iteration(S, it);
+    if (!st && S->cfg.use_graphs && !S->prof_batch) {
+      if (!S->gexec || S->graph_stale) st = srv_capture(S);
+      for (int it = 0; !st && it < S->cfg.iters_per_sync; it++)
+        if (hipGraphLaunch(S->gexec, S->stream) != hipSuccess) st = fail(LG_ERR_DEVICE, "request server: graph launch");
+    } else {
+      for (int it = 0; !st && it < S->cfg.iters_per_sync; it++) st = srv_launch_iteration(S, it, S->stream);
+    }
     if (!st && hipMemcpyAsync(S->h_ctl, S->d_ctl, S->top_slot * sizeof(Ctl), hipMemcpyDeviceToHost, S->stream) !=
                    hipSuccess)
       st = fail(LG_ERR_DEVICE, "request server: control copy failed");
@@ -398,6 +435,8 @@ void lg_srv_default_config(LgSrvConfig* c) {
   c->iters_per_sync = 16;
   c->mem_budget_bytes = 200ll * 1000 * 1000 * 1000;
   c->pool_chunk_bytes = 256ll << 20;
+  c->use_graphs = 0;
+  c->reserved = 0;
 }
 
 int lg_srv_create(const LgSrvConfig* c, LgServer** out) {
@@ -419,6 +458,7 @@ int lg_srv_create(const LgSrvConfig* c, LgServer** out) {
     if (S->h_tab) (void)hipHostFree(S->h_tab);
     if (S->stream) (void)hipStreamDestroy(S->stream);
     if (S->side) (void)hipStreamDestroy(S->side);
+    if (S->cap) (void)hipStreamDestroy(S->cap);
     if (S->ev_fork) (void)hipEventDestroy(S->ev_fork);
     if (S->ev_join) (void)hipEventDestroy(S->ev_join);
     delete S;
@@ -431,6 +471,7 @@ int lg_srv_create(const LgSrvConfig* c, LgServer** out) {
   const size_t ns = (size_t)c->max_slots;
   if (hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&S->side, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&S->cap, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&S->ev_fork, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&S->ev_join, hipEventDisableTiming) != hipSuccess ||
       hipMalloc((void**)&S->d_ctl, ns * sizeof(Ctl)) != hipSuccess ||
@@ -577,6 +618,8 @@ int lg_srv_destroy(LgServer* S) {
   for (auto& e : S->ev) (void)hipEventDestroy(e);
   (void)hipStreamSynchronize(S->side);
   (void)hipStreamDestroy(S->side);
+  if (S->gexec) (void)hipGraphExecDestroy(S->gexec);
+  (void)hipStreamDestroy(S->cap);
   (void)hipEventDestroy(S->ev_fork);
   (void)hipEventDestroy(S->ev_join);
   (void)hipStreamDestroy(S->stream);

@@ -28,7 +28,7 @@ _i32p = C.POINTER(C.c_int32)
 EXPORTED = ["lg_default_params", "lg_last_error", "lg_run_design", "lg_import_and_preplace",
             "lg_create", "lg_solve", "lg_postprocess", "lg_check", "lg_copy_out", "lg_get_stats",
             "lg_get_pixel_state", "lg_get_ovf_trace", "lg_destroy", "lg_profile_partition",
-            "lg_profile_read", "lg_debug_phase_ticks", "lg_srv_default_config", "lg_srv_create",
+            "lg_profile_read", "lg_use_graphs", "lg_debug_phase_ticks", "lg_srv_default_config", "lg_srv_create",
             "lg_srv_submit", "lg_srv_poll", "lg_srv_wait", "lg_srv_counters", "lg_srv_profile",
             "lg_srv_profile_read", "lg_srv_destroy"]
 
@@ -59,6 +59,7 @@ def lib():
         L.lg_get_ovf_trace.argtypes = [C.c_void_p, _i32p, C.c_int32, _i32p]
         L.lg_destroy.argtypes = [C.c_void_p]
         L.lg_profile_partition.argtypes = [C.c_void_p, C.c_int]
+        L.lg_use_graphs.argtypes = [C.c_void_p, C.c_int32]
         L.lg_profile_read.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
         L.lg_debug_phase_ticks.argtypes = [C.c_void_p, C.POINTER(C.c_uint64), C.c_int]
         _lib = L
@@ -222,6 +223,10 @@ class DeviceLegalizer:
 
     def postprocess(self, stream: int | None = None):
         _check(lib().lg_postprocess(self.h, stream))
+
+    def use_graphs(self, enable: bool = True):
+        """HIP graphs for the ADMM iterations (off by default; same results)."""
+        _check(lib().lg_use_graphs(self.h, int(bool(enable))))
 
     def profile_partition(self, enable: bool = True):
         _check(lib().lg_profile_partition(self.h, int(bool(enable))))

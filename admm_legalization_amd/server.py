@@ -24,7 +24,8 @@ QUEUED, RUNNING, DONE, FAILED = 0, 1, 2, 3
 
 class LgSrvConfig(C.Structure):
     _fields_ = [("max_slots", C.c_int32), ("iters_per_sync", C.c_int32),
-                ("mem_budget_bytes", C.c_int64), ("pool_chunk_bytes", C.c_int64)]
+                ("mem_budget_bytes", C.c_int64), ("pool_chunk_bytes", C.c_int64),
+                ("use_graphs", C.c_int32), ("reserved", C.c_int32)]
 
 
 class LgSrvResult(C.Structure):
@@ -38,7 +39,7 @@ class LgSrvCounters(C.Structure):
     _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64),
                 ("batches", C.c_int64), ("iterations", C.c_int64), ("peak_slots", C.c_int32),
                 ("active", C.c_int32), ("queued", C.c_int32), ("pool_reserved_bytes", C.c_int64),
-                ("pool_peak_bytes", C.c_int64)]
+                ("pool_peak_bytes", C.c_int64), ("graph_captures", C.c_int64)]
 
 
 def _bind():
@@ -69,8 +70,11 @@ class LegalizationServer:
     """Continuous-batching legalization service on the current GPU."""
 
     def __init__(self, max_slots: int | None = None, iters_per_sync: int | None = None,
-                 mem_budget_bytes: int | None = None, pool_chunk_bytes: int | None = None):
+                 mem_budget_bytes: int | None = None, pool_chunk_bytes: int | None = None,
+                 use_graphs: bool | None = None):
         cfg = default_config()
+        if use_graphs is not None:
+            cfg.use_graphs = 1 if use_graphs else 0
         if max_slots is not None:
             cfg.max_slots = int(max_slots)
         if iters_per_sync is not None:

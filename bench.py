@@ -107,6 +107,8 @@ def main():
     ap.add_argument("--load", type=float, default=0.7,
                     help="served_stream: Poisson arrival rate as a fraction of the measured capacity")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--graphs", action="store_true",
+                    help="replay the ADMM iterations as HIP graphs instead of launching every kernel")
     ap.add_argument("--cpu-sample-cells", type=int, default=10000)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--copy-calibration", action="store_true",
@@ -152,6 +154,7 @@ def main():
     torch.cuda.synchronize()
     t = time.perf_counter()
     dev = lg.DeviceLegalizer(grid, cells, params)
+    dev.use_graphs(args.graphs)
     torch.cuda.synchronize()
     t_upload = time.perf_counter() - t
 
@@ -160,7 +163,6 @@ def main():
         dev.postprocess()
         print(f"# warmup {w} done", file=sys.stderr, flush=True)
 
-    dev.profile_partition(True)
     lat = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -171,6 +173,12 @@ def main():
         lat.append(time.perf_counter() - ts)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
+    # partition-kernel timing from one more solve, outside the timed region
+    # (HIP events around each iteration's partition launches; eager launches)
+    dev.profile_partition(True)
+    dev.solve()
+    dev.postprocess()
+    torch.cuda.synchronize()
     ms, launches = dev.profile_read()
     dev.profile_partition(False)
 
@@ -205,6 +213,7 @@ def main():
         "config": {
             "workload": f"{args.workload}: {WORKLOADS[args.workload]}",
             "mode": "plain (one request at a time)",
+            "hip_graphs": args.graphs,
             "cells": N, "pixels": P, "partitions": int(dev_partitions(grid, params)),
             "admm_iterations": int(st["iterations"]),
             "total_disp_dbu": int(st["total_disp_dbu"]), "max_disp_dbu": int(st["max_disp_dbu"]),
@@ -312,6 +321,7 @@ def bench_served(args):
 
     def plain_one(g, c, p):
         dev = lg.DeviceLegalizer(g, c, p)
+        dev.use_graphs(args.graphs)
         dev.solve()
         dev.postprocess()
         lx, ly, rot = dev.positions()
@@ -337,7 +347,8 @@ def bench_served(args):
 
     # -- served, saturated: every request submitted at once (capacity)
     def served_run(arrivals, prof=False):
-        with LegalizationServer(max_slots=args.slots, iters_per_sync=args.iters_per_sync) as srv:
+        with LegalizationServer(max_slots=args.slots, iters_per_sync=args.iters_per_sync,
+                                use_graphs=args.graphs) as srv:
             if prof:
                 srv.profile(True)
             tk = []
@@ -366,6 +377,7 @@ def bench_served(args):
         times.append(el)
         if k == 0:
             res, ctr0, prof0 = r, ctr, prof
+        ctr_last = ctr
         print(f"# served (saturated) run {k}: {el:.2f}s {total / el:.0f} cells/s, peak slots "
               f"{ctr['peak_slots']}", file=sys.stderr, flush=True)
     t_served = float(np.median(times))
@@ -443,7 +455,8 @@ def bench_served(args):
             "server": {"slots": args.slots, "iters_per_sync": args.iters_per_sync,
                        "peak_slots": ctr0["peak_slots"], "batches": ctr0["batches"],
                        "batched_iterations": ctr0["iterations"],
-                       "pool_reserved_bytes": ctr0["pool_reserved_bytes"]},
+                       "pool_reserved_bytes": ctr0["pool_reserved_bytes"],
+                       "hip_graphs": args.graphs, "graph_captures": ctr_last["graph_captures"]},
             "host_synth_import_preplace_s": t_prep,
             "parallelism": "1 GPU",
         },

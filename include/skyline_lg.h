@@ -244,6 +244,13 @@ int lg_destroy(LgHandle* h);
  * since the last enable.  Adds three event records per iteration. */
 int lg_profile_partition(LgHandle* h, int enable);
 int lg_profile_read(LgHandle* h, double* ms /*[2]*/, int64_t* launches /*[2]*/);
+/* HIP graphs for the ADMM iterations of lg_solve; no counterpart in the
+ * reference.  Off by default (kernel launches).  When enabled, the first
+ * solve captures eight iterations (every kernel of the iteration, with the
+ * dual pass on its side branch) into one graph, which each later batch of
+ * iterations replays; profiled solves (lg_profile_partition) launch
+ * eagerly.  Results are the same either way. */
+int lg_use_graphs(LgHandle* h, int32_t enable);
 /* Per-phase clock ticks of a build compiled with LG_PHASE_TIMERS (zeros
  * otherwise); no counterpart in the reference. */
 int lg_debug_phase_ticks(LgHandle* h, uint64_t* ticks, int n);
@@ -271,6 +278,11 @@ typedef struct LgSrvConfig {
   int32_t iters_per_sync;     /* ADMM iterations launched between host checks (1..1024) */
   int64_t mem_budget_bytes;   /* device memory the server may hold for requests     */
   int64_t pool_chunk_bytes;   /* granularity in which that memory is taken          */
+  int32_t use_graphs;         /* 1: each ADMM iteration of the resident requests is
+                                 one HIP graph launch, recaptured when the set of
+                                 resident requests changes; 0 (default): kernel
+                                 launches.  Results are the same either way.     */
+  int32_t reserved;
 } LgSrvConfig;
 
 typedef struct LgSrvResult {
@@ -292,11 +304,12 @@ typedef struct LgSrvCounters {
   int32_t peak_slots;         /* most requests resident at once                     */
   int32_t active, queued;
   int64_t pool_reserved_bytes, pool_peak_bytes;
+  int64_t graph_captures;     /* iteration graphs captured (use_graphs)             */
 } LgSrvCounters;
 
 typedef struct LgServer LgServer;
 
-/* max_slots 64, iters_per_sync 16, budget 200 GB, chunks of 256 MiB. */
+/* max_slots 64, iters_per_sync 16, budget 200 GB, chunks of 256 MiB, no graphs. */
 void lg_srv_default_config(LgSrvConfig* c);
 /* Starts the server thread on the calling thread's current device. */
 int lg_srv_create(const LgSrvConfig* c, LgServer** out);

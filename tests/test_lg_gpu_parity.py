@@ -179,6 +179,28 @@ def test_adaptive_search(gpu):
     assert g["stats"]["num_adaptive_calls"] == o["stats"]["num_adaptive_calls"]
 
 
+def test_graphs_match_eager(gpu):
+    """lg_solve replaying the captured graph of eight iterations (the
+    default) and launching every kernel itself give the same state bit for
+    bit, over solves that stop mid-graph (max_admm_iter 203)."""
+    d = synth.mixed(5000, seed=1)
+    p = lg.default_params()
+    p.max_admm_iter = 203
+    grid, cells, p2 = lg.device_inputs(d, p)
+    out = []
+    for graphs in (True, False, True):
+        dev = lg.DeviceLegalizer(grid, cells, p2)
+        dev.use_graphs(graphs)
+        dev.solve()
+        usage, dual = dev.pixel_state()
+        lx, ly, rot = dev.positions()
+        out.append((usage, dual.view(np.int32), lx, ly, dev.stats()["iterations"], dev.trace()))
+        dev.close()
+    for o in out[1:]:
+        for x, y in zip(o, out[0]):
+            np.testing.assert_array_equal(x, y)
+
+
 def test_repeated_solve_is_idempotent(gpu):
     """lg_solve restarts from the positions given at lg_create: two solves on
     one handle (with post-process and a check between) give the same result,

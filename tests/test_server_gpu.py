@@ -56,10 +56,13 @@ def cases(gpu):
     return out
 
 
-@pytest.mark.parametrize("slots,iters", [(64, 16), (3, 8), (1, 5)])
-def test_batched_results_equal_solo(cases, slots, iters):
+@pytest.mark.parametrize("slots,iters,graphs", [(64, 16, True), (3, 8, True), (1, 5, True), (64, 16, False),
+                                                 (3, 8, False)])
+def test_batched_results_equal_solo(cases, slots, iters, graphs):
+    """Iterations as graph launches (recaptured whenever the resident set
+    changes) or as kernel launches: the same results."""
     with LegalizationServer(max_slots=slots, iters_per_sync=iters, mem_budget_bytes=4 << 30,
-                            pool_chunk_bytes=64 << 20) as srv:
+                            pool_chunk_bytes=64 << 20, use_graphs=graphs) as srv:
         tickets = [srv.submit(g, c, p) for _, g, c, p, _ in cases]
         res = [srv.wait(t) for t in tickets]
         ctr = srv.counters()
@@ -67,6 +70,7 @@ def test_batched_results_equal_solo(cases, slots, iters):
         _same(got, want, tag)
         assert got["stats"]["vio"]["total_ovf"] == 0 and got["stats"]["converged"] == 1, tag
     assert ctr["completed"] == len(cases) and ctr["failed"] == 0
+    assert (ctr["graph_captures"] > 0) == graphs
     assert 1 <= ctr["peak_slots"] <= min(slots, len(cases))
     if slots >= len(cases):
         assert ctr["peak_slots"] == len(cases)  # all resident together
