@@ -59,7 +59,7 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int kTile = 32;   // bins per tile side
 constexpr int kMaxHalo = 8;
-constexpr int kBK = 32;     // GEMM k-step
+constexpr int kBK = 64;     // GEMM k-step
 
 // ---------------------------------------------------------------------------
 // GEMM: C = alpha * A * B^T, A [R x K], B [Cn x K], both k-contiguous.  Rows

@@ -57,6 +57,9 @@ WORKLOADS = {
              "(mean 3.8), pins at uniform offsets inside their cells, 0.5 % IO pins, seed 0",
     "density_1m": "electrostatic density: 1M cells, area LogNormal(ln 4, 0.5), half uniform / half in 8 "
                   "Gaussian clusters, square die at 70 % fill, 1024 x 1024 bins, split-BF16 Poisson, seed 0",
+    "density_2m_4k": "electrostatic density of the 2M-cell end-to-end design: 2M cells, area LogNormal(ln 4, "
+                     "0.5), half uniform / half in 8 Gaussian clusters, 70 % fill, 4096 x 4096 bins, BF16 "
+                     "Poisson, seed 0",
 }
 MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16
 
@@ -126,7 +129,7 @@ def main():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
     lg.lib()  # raises if the HIP library is not built
-    if args.workload == "density_1m":
+    if args.workload in ("density_1m", "density_2m_4k"):
         return bench_density(args)
     if args.workload == "wl_2m":
         return bench_wl(args)
@@ -224,7 +227,10 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": f"k_partition<0,{c}> (ADMM primal + dual update, one wave per partition)",
+            "kernel": ("k_partition<0,0,4> (ADMM primal + fused write-back/dual update, four waves per partition)"
+                       if c == 0 else "k_partition<0,1,1> (ADMM primal, LDS sorts, one wave per partition)"),
+            "note": "algorithmic bytes = 12 per pixel + 24 per cell per iteration; the dual update of "
+                    "partitions without overflow runs beside this kernel on a second stream",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -555,8 +561,8 @@ def bench_wl(args):
 
 
 def density_cpu_baseline(pl, g):
-    """The FP64 oracle evaluates the same 1M-cell, 1024 x 1024 density once
-    (scatter, overflow, closed-form Poisson, gather) with numpy on the host."""
+    """The FP64 oracle evaluates the same density once (scatter, overflow,
+    closed-form Poisson, gather) with numpy on the host."""
     from oracle import ref_density as R
     geom = dict(num_bin_x=g.num_bin_x, num_bin_y=g.num_bin_y, bin_w=g.bin_w, bin_h=g.bin_h,
                 bin_lx=g.bin_lx, bin_ux=g.bin_ux, bin_ly=g.bin_ly, bin_uy=g.bin_uy, die=g.die)
@@ -571,19 +577,22 @@ def density_cpu_baseline(pl, g):
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": 1.0 / el, "unit": "evaluations/s", "cores": threads, "kind": "port",
             "sample": f"one full evaluation by the FP64 numpy oracle (BLAS matrix products for the "
-                      f"transforms) on the same 1M-cell, 1024x1024-bin input: {el:.2f} s"}
+                      f"transforms) on the same {len(pl['cx'])}-cell, {g.num_bin_x}x{g.num_bin_y}-bin "
+                      f"input: {el:.2f} s"}
 
 
 def bench_density(args):
     import torch
     from admm_legalization_amd import density as D
     from admm_legalization_amd import synth
-    pl = synth.density_placement(1_000_000, seed=0)
-    g = D.prepare(pl["die"], pl["dx"], pl["dy"])
+    big = args.workload == "density_2m_4k"
+    pl = synth.density_placement(2_000_000 if big else 1_000_000, seed=0)
+    g = D.prepare(pl["die"], pl["dx"], pl["dy"], num_bins=(4096, 4096) if big else None)
     n_cells = len(pl["dx"])
     B = g.num_bin_x * g.num_bin_y
-    print(f"# density_1m: N={n_cells} bins {g.num_bin_x}x{g.num_bin_y}", file=sys.stderr, flush=True)
-    dg = D.DensityGradient(g, precision=D.PREC_SPLIT_BF16)
+    prec_name, prec_main = ("bf16", D.PREC_BF16) if big else ("split_bf16", D.PREC_SPLIT_BF16)
+    print(f"# {args.workload}: N={n_cells} bins {g.num_bin_x}x{g.num_bin_y}", file=sys.stderr, flush=True)
+    dg = D.DensityGradient(g, precision=prec_main)
     s = torch.cuda.Stream()
     with torch.cuda.stream(s):
         cx = torch.tensor(pl["cx"], device="cuda")
@@ -624,23 +633,26 @@ def bench_density(args):
         ps.close()
     n = g.num_bin_x
     flops = 16.0 * n ** 3  # four 2-D transforms as two n^3 GEMMs each
-    achieved = flops / (solve_ms["split_bf16"] / 1e3) / 1e12
+    achieved = flops / (solve_ms[prec_name] / 1e3) / 1e12
     out = {
         "metric": "density evaluations/s (scatter + overflow + Poisson + gather)",
         "value": 1e3 / ms_eval, "unit": "evaluations/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3, "higher_is_better": True,
-        "scaling": "strong", "vs_baseline": None, "dtype": "f32 (split-bf16 MFMA transforms)",
+        "scaling": "strong", "vs_baseline": None, "dtype": f"f32 ({prec_name.replace('_', '-')} MFMA transforms)",
         "data": "synthetic",
-        "config": {"workload": f"density_1m: {WORKLOADS['density_1m']}", "cells": n_cells,
+        "config": {"workload": f"{args.workload}: {WORKLOADS[args.workload]}", "cells": n_cells,
                    "bins": [g.num_bin_x, g.num_bin_y], "overflow": ovf, "event_ms_per_eval": ms_eval,
                    "cells_per_s": n_cells * 1e3 / ms_eval, "poisson_ms": solve_ms,
                    "poisson_tiles": solve_tiles,
                    "parallelism": "1 GPU"},
-        "roofline": {"bound": "mfma", "kernel": "Poisson solve (split-BF16 cosine/sine GEMMs)",
+        "roofline": {"bound": "mfma",
+                     "kernel": f"Poisson solve ({prec_name.replace('_', '-')} cosine/sine GEMMs, k_gemm_nt)",
                      "achieved": achieved, "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / MFMA_BF16_PEAK_TFLOPS, "traffic": None,
                      "algorithmic_flops_per_solve": flops,
-                     "note": "split-BF16 issues three MFMA products per algorithmic product"},
+                     "note": ("split-BF16 issues three MFMA products per algorithmic product" if not big else
+                              "16 n^3 flops: eight n^3-MAC products (the kernel issues seven: the potential's "
+                              "and E_y's last products share one stacked GEMM)")},
         "cpu_baseline": None,
     }
     dg.close()
