@@ -303,21 +303,43 @@ __global__ void k_dens_reset(int* cnt, int n, double* ovf) {
 }
 
 // bucket of each cell: its tile, T for cells wider than the halo, -1 for
-// cells outside the die; rank within the bucket
-__global__ void k_dens_count(Geo G, const float* cx, const float* cy, const float4* cgeo, int* cnt,
-                             int* bucket, int* rank) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= G.N) return;
-  float4 c = cgeo[i];
-  int x0, x1, y0, y1;
-  bin_range(G, cx[i], cy[i], c.x, c.y, x0, x1, y0, y1);
-  int b = -1;
-  if (x1 > x0 && y1 > y0) {
-    bool fits = (x1 - x0) <= G.H + 1 && (y1 - y0) <= G.H + 1;
-    b = fits ? (x0 / kTile) + (y0 / kTile) * G.tiles_x : G.T;
+// cells outside the die; rank within the bucket.  With HIST the ranks are
+// taken per workgroup in an LDS histogram of the T + 1 buckets and the
+// workgroup adds each bucket's total to the global count once: clustered
+// placements put tens of thousands of cells in one tile, whose counter would
+// otherwise take one atomic per cell.
+template <bool HIST>
+__global__ __launch_bounds__(256) void k_dens_count(Geo G, const float* cx, const float* cy, const float4* cgeo,
+                                                    int* cnt, int* bucket, int* rank) {
+  extern __shared__ int hist[];  // [T + 1] with HIST
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (HIST) {
+    for (int e = threadIdx.x; e <= G.T; e += blockDim.x) hist[e] = 0;
+    __syncthreads();
+  } else if (i >= G.N) {
+    return;
   }
-  bucket[i] = b;
-  if (b >= 0) rank[i] = atomicAdd(&cnt[b], 1);
+  int b = -1;
+  if (i < G.N) {
+    float4 c = cgeo[i];
+    int x0, x1, y0, y1;
+    bin_range(G, cx[i], cy[i], c.x, c.y, x0, x1, y0, y1);
+    if (x1 > x0 && y1 > y0) {
+      bool fits = (x1 - x0) <= G.H + 1 && (y1 - y0) <= G.H + 1;
+      b = fits ? (x0 / kTile) + (y0 / kTile) * G.tiles_x : G.T;
+    }
+    bucket[i] = b;
+  }
+  if (!HIST) {
+    if (b >= 0) rank[i] = atomicAdd(&cnt[b], 1);
+    return;
+  }
+  const int rl = b >= 0 ? atomicAdd(&hist[b], 1) : 0;
+  __syncthreads();
+  // the first cell of a bucket turns the bucket's total into its base
+  if (b >= 0 && rl == 0) hist[b] = atomicAdd(&cnt[b], hist[b]);
+  __syncthreads();
+  if (b >= 0) rank[i] = hist[b] + rl;
 }
 
 // exclusive scan of cnt[0..n) into off[0..n] (one workgroup of 1024)
@@ -792,7 +814,14 @@ int dens_compute_grad(DensHandle* h, const float* cx, const float* cy, float* gx
   const int nb = (G.N + 255) / 256;
   hipLaunchKernelGGL(k_dens_reset, dim3((G.T + 256) / 256), dim3(256), 0, s, h->cnt, G.T + 1, h->ovf_acc);
   if (G.N > 0) {
-    hipLaunchKernelGGL(k_dens_count, dim3(nb), dim3(256), 0, s, G, cx, cy, h->cgeo, h->cnt, h->bucket, h->rank);
+    // LDS histogram of the buckets when it is small (up to 64 x 64 tiles)
+    const size_t hist = (size_t)(G.T + 1) * sizeof(int);
+    if (hist <= 16 * 1024 + sizeof(int))
+      hipLaunchKernelGGL(k_dens_count<true>, dim3(nb), dim3(256), hist, s, G, cx, cy, h->cgeo, h->cnt, h->bucket,
+                         h->rank);
+    else
+      hipLaunchKernelGGL(k_dens_count<false>, dim3(nb), dim3(256), 0, s, G, cx, cy, h->cgeo, h->cnt, h->bucket,
+                         h->rank);
   }
   hipLaunchKernelGGL(k_dens_scan, dim3(1), dim3(1024), 0, s, h->cnt, h->off, G.T + 1);
   if (G.N > 0) {
