@@ -366,17 +366,27 @@ __global__ __launch_bounds__(1024) void k_dens_scan(const int* cnt, int* off, in
   if (t == 1023) off[n] = part[1023];
 }
 
+// records in bucket order: box, scatter scale and filler flag, and for the
+// gather the cell id and its gather scale; a cell outside the die gets its
+// zero gradient here (it has no record)
 __global__ void k_dens_fill(Geo G, const float* cx, const float* cy, const float4* cgeo, const int* cfill,
-                            const int* bucket, const int* rank, const int* off, float4* rec_box, float2* rec_s) {
+                            const int* bucket, const int* rank, const int* off, float4* rec_box, float2* rec_s,
+                            int2* rec_ig, float* gx, float* gy, float* precond) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= G.N) return;
   int b = bucket[i];
-  if (b < 0) return;
+  if (b < 0) {
+    gx[i] = 0.f;
+    gy[i] = 0.f;
+    precond[i] = 0.f;
+    return;
+  }
   float4 c = cgeo[i];
   float x = cx[i], y = cy[i];
   int p = off[b] + rank[i];
   rec_box[p] = make_float4(x, y, c.x, c.y);
   rec_s[p] = make_float2(c.z, cfill ? (float)cfill[i] : 0.f);
+  rec_ig[p] = make_int2(i, __float_as_int(c.w));
 }
 
 // one workgroup per tile: LDS accumulation of the tile plus halo, slab write
@@ -476,12 +486,17 @@ __global__ __launch_bounds__(256) void k_dens_resolve(Geo G, const float* slab, 
 
 // getDensityGradForEachCell (DensityGradient.cu:173-281): the GPU Poisson
 // field carries a factor 2, hence the 0.5
-__global__ void k_dens_gather(Geo G, const float* cx, const float* cy, const float4* cgeo, const float* lambda,
+// over the records in bucket (tile) order, so that neighbouring threads read
+// neighbouring bins; results go to each record's cell
+__global__ void k_dens_gather(Geo G, const int* off, const float4* rec_box, const int2* rec_ig, const float* lambda,
                               const float* ex, const float* ey, float* gx, float* gy, float* precond) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= G.N) return;
-  float4 c = cgeo[i];
-  float x = cx[i], y = cy[i];
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= off[G.T + 1]) return;
+  const float4 r = rec_box[p];
+  const int2 ig = rec_ig[p];
+  const int i = ig.x;
+  const float4 c = make_float4(r.z, r.w, 0.f, __int_as_float(ig.y));
+  float x = r.x, y = r.y;
   int x0, x1, y0, y1;
   bin_range(G, x, y, c.x, c.y, x0, x1, y0, y1);
   float lx = x - c.x / 2, ux = x + c.x / 2;
@@ -549,6 +564,7 @@ struct DensHandle {
   int *cnt = nullptr, *off = nullptr, *bucket = nullptr, *rank = nullptr;
   float4* rec_box = nullptr;
   float2* rec_s = nullptr;
+  int2* rec_ig = nullptr;
   float *slab = nullptr, *mov = nullptr, *fil = nullptr, *dens = nullptr;
   float *psi = nullptr, *ex = nullptr, *ey = nullptr, *precond = nullptr;
   double* ovf_acc = nullptr;
@@ -783,6 +799,7 @@ int dens_create(const DensGeom* g, DensHandle** out) {
   if ((st = dalloc(h->bufs, &h->rank, N))) return bail(st);
   if ((st = dalloc(h->bufs, &h->rec_box, N))) return bail(st);
   if ((st = dalloc(h->bufs, &h->rec_s, N))) return bail(st);
+  if ((st = dalloc(h->bufs, &h->rec_ig, N))) return bail(st);
   if ((st = dalloc(h->bufs, &h->slab, (size_t)G.T * 2 * G.SW * G.SW))) return bail(st);
   if ((st = dalloc(h->bufs, &h->mov, B))) return bail(st);
   if ((st = dalloc(h->bufs, &h->fil, B))) return bail(st);
@@ -826,7 +843,7 @@ int dens_compute_grad(DensHandle* h, const float* cx, const float* cy, float* gx
   hipLaunchKernelGGL(k_dens_scan, dim3(1), dim3(1024), 0, s, h->cnt, h->off, G.T + 1);
   if (G.N > 0) {
     hipLaunchKernelGGL(k_dens_fill, dim3(nb), dim3(256), 0, s, G, cx, cy, h->cgeo, h->cfill, h->bucket, h->rank,
-                       h->off, h->rec_box, h->rec_s);
+                       h->off, h->rec_box, h->rec_s, h->rec_ig, gx, gy, h->precond);
   }
   size_t lds = (size_t)2 * G.SW * G.SW * sizeof(float);
   hipLaunchKernelGGL(k_dens_accum, dim3(G.T), dim3(256), lds, s, G, h->off, h->rec_box, h->rec_s, h->slab);
@@ -838,7 +855,8 @@ int dens_compute_grad(DensHandle* h, const float* cx, const float* cy, float* gx
   int st = poisson_run(h->ps, h->dens, h->psi, h->ex, h->ey, h->prec, s);
   if (st) return st;
   if (G.N > 0) {
-    hipLaunchKernelGGL(k_dens_gather, dim3(nb), dim3(256), 0, s, G, cx, cy, h->cgeo, h->lambda, h->ex, h->ey, gx,
+    hipLaunchKernelGGL(k_dens_gather, dim3(nb), dim3(256), 0, s, G, h->off, h->rec_box, h->rec_ig, h->lambda, h->ex,
+                       h->ey, gx,
                        gy, h->precond);
   }
   DENS_TRY(hipGetLastError());
