@@ -1854,110 +1854,230 @@ __device__ __forceinline__ void partition_body(const Args& a, int bx, int npad_l
       dir[j] = d < kNumDirDefault ? c_dir_default[d] : make_int2(0, 0);
     }
     // -- the cells in order, 64 at a time ---------------------------------
-    for (int base = 0; base < count; base += kWave) {
-      const int t = base + lane;
-      int4 r = make_int4(0, 0, 0, 0);
-      int id = -1;
-      if (t < count) { r = recs[t]; id = ids[t]; }
-      const int my_x = rec_x(r.x), my_y = rec_y(r.x), my_w = rec_w(r.y), my_h = rec_h(r.y);
-      // skip test of computeCellOverflow + isPlacedInCorrectRow (AdmmKernels.hpp:722-738)
-      auto skip_test = [&]() -> bool {
-        if (t >= count) return false;
-        if (MODE == MODE_PRIMAL) {
+    if constexpr (NW > 1) {
+      // Wave 0 keeps the window, runs the skip tests, rips up and commits;
+      // every wave evaluates its share of the candidates.  Per move wave 0
+      // posts {next cell, its record} (or -1: window done) in one of two
+      // LDS slots by pass parity, then two barriers: B1 (the post and the
+      // rip-up are visible), B2 (the waves' candidate minima are posted).
+      // Tile writes happen only on wave 0 between B2 and the next B1, while
+      // the other waves read nothing of the tile.
+      int pass = 0;
+      for (int base = 0; base < count; base += kWave) {
+        const int t = base + lane;
+        int4 r = make_int4(0, 0, 0, 0);
+        int id = -1;
+        bool need = false;
+        int start = 0;
+        if (wave == 0) {
+          if (t < count) { r = recs[t]; id = ids[t]; }
+        }
+        const int my_x = rec_x(r.x), my_y = rec_y(r.x), my_w = rec_w(r.y), my_h = rec_h(r.y);
+        // skip test of computeCellOverflow + isPlacedInCorrectRow (AdmmKernels.hpp:722-738)
+        auto skip_test = [&]() -> bool {
+          if (t >= count) return false;
           if (pert) return true;
           bool ok_row = correct_row(my_h, rec_gnd(r.y), T.vdd[g.ly(my_y)]);
           return !ok_row || ovf_tile(g, T, my_x, my_y, my_w, my_h) != 0;
-        }
-        return my_h > 1;
-      };
-      bool need = skip_test();
-      int start = 0;
-      for (;;) {
-        const unsigned long long m = __ballot(need && lane >= start);
-        if (m == 0ull) break;
-        SUB_START();
-        const int first = __builtin_amdgcn_readfirstlane(__ffsll((long long)m) - 1);
-        const int px = __builtin_amdgcn_readlane(r.x, first), py = __builtin_amdgcn_readlane(r.y, first);
-        const int pox = __builtin_amdgcn_readlane(r.z, first), poy = __builtin_amdgcn_readlane(r.w, first);
-        const int pid = __builtin_amdgcn_readlane(id, first);
-        const int cx = rec_x(px), cy = rec_y(px);
-        const int w = rec_w(py), h = rec_h(py), gnd = rec_gnd(py), code = rec_code(py);
-        float disp_coeff = 1.0f;
-        if (a.flag_iccad17) disp_coeff = float(a.N) / float(a.k_height_num[h - 1]);
+        };
+        if (wave == 0) need = skip_test();
+        int pid = 0;
+        for (;;) {
+          int* post = xch + 48 + 8 * (pass & 1);
+          SUB_START();
+          if (wave == 0) {
+            const unsigned long long m = __ballot(need && lane >= start);
+            if (m != 0ull) {
+              const int first = __builtin_amdgcn_readfirstlane(__ffsll((long long)m) - 1);
+              const int px = __builtin_amdgcn_readlane(r.x, first), py = __builtin_amdgcn_readlane(r.y, first);
+              pid = __builtin_amdgcn_readlane(id, first);
+              if (lane == 0) {
+                post[0] = first;
+                post[1] = px;
+                post[2] = py;
+                post[3] = __builtin_amdgcn_readlane(r.z, first);
+                post[4] = __builtin_amdgcn_readlane(r.w, first);
+              }
 #ifdef LG_PHASE_TIMERS
-        sub_acc_[5]++;
+              sub_acc_[5]++;
 #endif
-        // every wave has finished its reads of the last commit's pixels
-        if (NW > 1) __syncthreads();
-        footprint_add<NW>(g, T, cx, cy, w, h, -1, rho, standard);  // rip-up
-        __syncthreads();
-        SUB_MARK(11);
-        float cst[NC];
-        eval_candidates<MODE, NC>(a, g, T, disp_coeff, thr, cx, cy, w, h, code, gnd, pox, poy, d0, dir, cst, ucode,
-                                  vmask);
-        SUB_MARK(12);
-        // Tree-min of AdmmKernels.hpp:847-858 over slots s = j*64 + lane: a
-        // halving-stride tournament in which a tie keeps the lower slot.  Its
-        // winner is the lexicographic minimum of (cost, bit-reverse8(slot)),
-        // since the last comparison decides on bit 0 of the slot, the one
-        // before on bit 1, ...  Costs are >= 0 or +inf, so their bit patterns
-        // order like the values.
-        unsigned long long best = ~0ull;
+              footprint_add<1>(g, T, rec_x(px), rec_y(px), rec_w(py), rec_h(py), -1, rho, standard);  // rip-up
+              SUB_MARK(11);
+            } else if (lane == 0) {
+              post[0] = -1;
+            }
+          }
+          __syncthreads();  // B1
+          pass++;
+          const int first = post[0];
+          if (first < 0) break;
+          const int px = post[1], py = post[2], pox = post[3], poy = post[4];
+          const int cx = rec_x(px), cy = rec_y(px);
+          const int w = rec_w(py), h = rec_h(py), gnd = rec_gnd(py), code = rec_code(py);
+          float disp_coeff = 1.0f;
+          if (a.flag_iccad17) disp_coeff = float(a.N) / float(a.k_height_num[h - 1]);
+          float cst[NC];
+          eval_candidates<MODE, NC>(a, g, T, disp_coeff, thr, cx, cy, w, h, code, gnd, pox, poy, d0, dir, cst,
+                                    ucode, vmask);
+          SUB_MARK(12);
+          // the tree-min as below: a DPP butterfly per wave, then the four
+          // wave minima
+          unsigned long long best = ~0ull;
 #pragma unroll
-        for (int j = 0; j < NC; j++) {
-          unsigned slot = (unsigned)(d0 + j * kWave);
-          unsigned rev = __brev(slot) >> 24;
-          unsigned long long key = ((unsigned long long)__float_as_uint(cst[j]) << 32) | rev;
-          best = key < best ? key : best;
-        }
-        // minimum over the wave by DPP steps (keys are unique, so the order
-        // of the steps does not matter); lane 63 ends with it
-        unsigned bhi = (unsigned)(best >> 32), blo = (unsigned)best;
-        key_min_step<0xB1, 0xf>(bhi, blo);   // quad_perm [1,0,3,2]
-        key_min_step<0x4E, 0xf>(bhi, blo);   // quad_perm [2,3,0,1]
-        key_min_step<0x141, 0xf>(bhi, blo);  // row_half_mirror
-        key_min_step<0x140, 0xf>(bhi, blo);  // row_mirror
-        key_min_step<0x142, 0xa>(bhi, blo);  // row_bcast:15 into rows 1, 3
-        key_min_step<0x143, 0xc>(bhi, blo);  // row_bcast:31 into rows 2, 3
-        unsigned bc_bits = __builtin_amdgcn_readlane(bhi, 63);
-        unsigned bc_lo = __builtin_amdgcn_readlane(blo, 63);
-        if (NW > 1) {
-          // minimum over the waves' minima
+          for (int j = 0; j < NC; j++) {
+            unsigned slot = (unsigned)(d0 + j * kWave);
+            unsigned rev = __brev(slot) >> 24;
+            unsigned long long key = ((unsigned long long)__float_as_uint(cst[j]) << 32) | rev;
+            best = key < best ? key : best;
+          }
+          unsigned bhi = (unsigned)(best >> 32), blo = (unsigned)best;
+          key_min_step<0xB1, 0xf>(bhi, blo);
+          key_min_step<0x4E, 0xf>(bhi, blo);
+          key_min_step<0x141, 0xf>(bhi, blo);
+          key_min_step<0x140, 0xf>(bhi, blo);
+          key_min_step<0x142, 0xa>(bhi, blo);
+          key_min_step<0x143, 0xc>(bhi, blo);
           if (lane == kWave - 1) {
             xmin[wave] = bhi;
             xmin[NW + wave] = blo;
           }
-          __syncthreads();
+          __syncthreads();  // B2
+          if (wave == 0) {
+            unsigned bc_bits = xmin[0], bc_lo = xmin[NW];
 #pragma unroll
-          for (int w2 = 0; w2 < NW; w2++) {
-            const unsigned ohi = xmin[w2], olo = xmin[NW + w2];
-            const bool take = ohi < bc_bits || (ohi == bc_bits && olo < bc_lo);
-            bc_bits = take ? ohi : bc_bits;
-            bc_lo = take ? olo : bc_lo;
+            for (int w2 = 1; w2 < NW; w2++) {
+              const unsigned ohi = xmin[w2], olo = xmin[NW + w2];
+              const bool take = ohi < bc_bits || (ohi == bc_bits && olo < bc_lo);
+              bc_bits = take ? ohi : bc_bits;
+              bc_lo = take ? olo : bc_lo;
+            }
+            const int bslot = (int)(__brev(bc_lo & 0xffu) >> 24);
+            int nx = cx, ny = cy;
+            if (__uint_as_float(bc_bits) != kInf) {
+              int2 dd = c_dir_default[bslot];
+              new_pos(a, dd, w, h, cx, cy, nx, ny);
+            }
+            SUB_MARK(13);
+            footprint_add<1>(g, T, nx, ny, w, h, +1, rho, standard);  // commit
+            if (lane == 0) cellrec[pid].x = rec_pos(nx, ny);
+            SUB_MARK(14);
+            // a move changes usage only inside the old and new footprints
+            start = first + 1;
+            if (lane >= start && t < count && !pert) {
+              bool hit_old = my_x < cx + w && cx < my_x + my_w && my_y < cy + h && cy < my_y + my_h;
+              bool hit_new = my_x < nx + w && nx < my_x + my_w && my_y < ny + h && ny < my_y + my_h;
+              if (hit_old || hit_new) need = skip_test();
+            }
+            SUB_MARK(15);
           }
         }
-        const int bi = (int)(bc_lo & 0xffu);
-        const int bslot = (int)(__brev((unsigned)bi) >> 24);
-        int nx = cx, ny = cy;
-        if (__uint_as_float(bc_bits) != kInf) {
-          int2 dd = c_dir_default[bslot];
-          new_pos(a, dd, w, h, cx, cy, nx, ny);
+      }
+    } else {
+      for (int base = 0; base < count; base += kWave) {
+        const int t = base + lane;
+        int4 r = make_int4(0, 0, 0, 0);
+        int id = -1;
+        if (t < count) { r = recs[t]; id = ids[t]; }
+        const int my_x = rec_x(r.x), my_y = rec_y(r.x), my_w = rec_w(r.y), my_h = rec_h(r.y);
+        // skip test of computeCellOverflow + isPlacedInCorrectRow (AdmmKernels.hpp:722-738)
+        auto skip_test = [&]() -> bool {
+          if (t >= count) return false;
+          if (MODE == MODE_PRIMAL) {
+            if (pert) return true;
+            bool ok_row = correct_row(my_h, rec_gnd(r.y), T.vdd[g.ly(my_y)]);
+            return !ok_row || ovf_tile(g, T, my_x, my_y, my_w, my_h) != 0;
+          }
+          return my_h > 1;
+        };
+        bool need = skip_test();
+        int start = 0;
+        for (;;) {
+          const unsigned long long m = __ballot(need && lane >= start);
+          if (m == 0ull) break;
+          SUB_START();
+          const int first = __builtin_amdgcn_readfirstlane(__ffsll((long long)m) - 1);
+          const int px = __builtin_amdgcn_readlane(r.x, first), py = __builtin_amdgcn_readlane(r.y, first);
+          const int pox = __builtin_amdgcn_readlane(r.z, first), poy = __builtin_amdgcn_readlane(r.w, first);
+          const int pid = __builtin_amdgcn_readlane(id, first);
+          const int cx = rec_x(px), cy = rec_y(px);
+          const int w = rec_w(py), h = rec_h(py), gnd = rec_gnd(py), code = rec_code(py);
+          float disp_coeff = 1.0f;
+          if (a.flag_iccad17) disp_coeff = float(a.N) / float(a.k_height_num[h - 1]);
+  #ifdef LG_PHASE_TIMERS
+          sub_acc_[5]++;
+  #endif
+          // every wave has finished its reads of the last commit's pixels
+          if (NW > 1) __syncthreads();
+          footprint_add<NW>(g, T, cx, cy, w, h, -1, rho, standard);  // rip-up
+          __syncthreads();
+          SUB_MARK(11);
+          float cst[NC];
+          eval_candidates<MODE, NC>(a, g, T, disp_coeff, thr, cx, cy, w, h, code, gnd, pox, poy, d0, dir, cst, ucode,
+                                    vmask);
+          SUB_MARK(12);
+          // Tree-min of AdmmKernels.hpp:847-858 over slots s = j*64 + lane: a
+          // halving-stride tournament in which a tie keeps the lower slot.  Its
+          // winner is the lexicographic minimum of (cost, bit-reverse8(slot)),
+          // since the last comparison decides on bit 0 of the slot, the one
+          // before on bit 1, ...  Costs are >= 0 or +inf, so their bit patterns
+          // order like the values.
+          unsigned long long best = ~0ull;
+  #pragma unroll
+          for (int j = 0; j < NC; j++) {
+            unsigned slot = (unsigned)(d0 + j * kWave);
+            unsigned rev = __brev(slot) >> 24;
+            unsigned long long key = ((unsigned long long)__float_as_uint(cst[j]) << 32) | rev;
+            best = key < best ? key : best;
+          }
+          // minimum over the wave by DPP steps (keys are unique, so the order
+          // of the steps does not matter); lane 63 ends with it
+          unsigned bhi = (unsigned)(best >> 32), blo = (unsigned)best;
+          key_min_step<0xB1, 0xf>(bhi, blo);   // quad_perm [1,0,3,2]
+          key_min_step<0x4E, 0xf>(bhi, blo);   // quad_perm [2,3,0,1]
+          key_min_step<0x141, 0xf>(bhi, blo);  // row_half_mirror
+          key_min_step<0x140, 0xf>(bhi, blo);  // row_mirror
+          key_min_step<0x142, 0xa>(bhi, blo);  // row_bcast:15 into rows 1, 3
+          key_min_step<0x143, 0xc>(bhi, blo);  // row_bcast:31 into rows 2, 3
+          unsigned bc_bits = __builtin_amdgcn_readlane(bhi, 63);
+          unsigned bc_lo = __builtin_amdgcn_readlane(blo, 63);
+          if (NW > 1) {
+            // minimum over the waves' minima
+            if (lane == kWave - 1) {
+              xmin[wave] = bhi;
+              xmin[NW + wave] = blo;
+            }
+            __syncthreads();
+  #pragma unroll
+            for (int w2 = 0; w2 < NW; w2++) {
+              const unsigned ohi = xmin[w2], olo = xmin[NW + w2];
+              const bool take = ohi < bc_bits || (ohi == bc_bits && olo < bc_lo);
+              bc_bits = take ? ohi : bc_bits;
+              bc_lo = take ? olo : bc_lo;
+            }
+          }
+          const int bi = (int)(bc_lo & 0xffu);
+          const int bslot = (int)(__brev((unsigned)bi) >> 24);
+          int nx = cx, ny = cy;
+          if (__uint_as_float(bc_bits) != kInf) {
+            int2 dd = c_dir_default[bslot];
+            new_pos(a, dd, w, h, cx, cy, nx, ny);
+          }
+          if (NW == 1) __syncthreads();
+          SUB_MARK(13);
+          footprint_add<NW>(g, T, nx, ny, w, h, +1, rho, standard);  // commit
+          if (tid == 0) cellrec[pid].x = rec_pos(nx, ny);
+          __syncthreads();
+          SUB_MARK(14);
+          start = first + 1;
+          if (start >= kWave) break;
+          // a move changes usage only inside the old and new footprints
+          if (lane >= start && t < count && MODE == MODE_PRIMAL && !pert) {
+            bool hit_old = my_x < cx + w && cx < my_x + my_w && my_y < cy + h && cy < my_y + my_h;
+            bool hit_new = my_x < nx + w && nx < my_x + my_w && my_y < ny + h && ny < my_y + my_h;
+            if (hit_old || hit_new) need = skip_test();
+          }
+          SUB_MARK(15);
         }
-        if (NW == 1) __syncthreads();
-        SUB_MARK(13);
-        footprint_add<NW>(g, T, nx, ny, w, h, +1, rho, standard);  // commit
-        if (tid == 0) cellrec[pid].x = rec_pos(nx, ny);
-        __syncthreads();
-        SUB_MARK(14);
-        start = first + 1;
-        if (start >= kWave) break;
-        // a move changes usage only inside the old and new footprints
-        if (lane >= start && t < count && MODE == MODE_PRIMAL && !pert) {
-          bool hit_old = my_x < cx + w && cx < my_x + my_w && my_y < cy + h && cy < my_y + my_h;
-          bool hit_new = my_x < nx + w && nx < my_x + my_w && my_y < ny + h && ny < my_y + my_h;
-          if (hit_old || hit_new) need = skip_test();
-        }
-        SUB_MARK(15);
       }
     }
     PHASE_MARK(2);
