@@ -1103,8 +1103,8 @@ __device__ __forceinline__ void eval_candidates(const Args& a, const PartGeo& g,
 // Staging of the partition's pixel tile by the NW waves of the workgroup,
 // threads walking the tile in order with incremental (la, lb) = (i % xh,
 // i / xh): consecutive threads read consecutive pixels of a grid row;
-// 16 / NW pixels in flight per thread.  Returns true if a usage does not fit
-// the 16-bit LDS packing.
+// 16 (one wave) or 8 (four waves) pixels in flight per thread.  Returns
+// true if a usage does not fit the 16-bit LDS packing.
 template <int NW>
 __device__ __forceinline__ bool stage_tile(const Args& a, const PartGeo& g, const Tile& T, float rho,
                                            bool standard, bool& over, int& cmin, int& cmax) {
@@ -1114,7 +1114,7 @@ __device__ __forceinline__ bool stage_tile(const Args& a, const PartGeo& g, cons
   const int step_b = NT / g.xh, step_a = NT - step_b * g.xh;
   int lb = tid / g.xh, la = tid - lb * g.xh;
   bool range_err = false;
-  constexpr int U = 16 / NW;
+  constexpr int U = NW > 1 ? 8 : 16;
   for (int i0 = 0; i0 < npix; i0 += U * NT) {
     int u[U], code[U], li[U];
     float dl[U];
@@ -1501,17 +1501,39 @@ __device__ __forceinline__ void coop_net(int n, int& w, int* xw) {
 // waves.  Returns false, having written nothing, if the keys do not pack.
 __device__ __forceinline__ bool order_partition_coop(const Args& a, const PartGeo& g, const Tile& T, int npad,
                                                      const int* src, int off, int count, const int4* cellrec,
-                                                     int* sids, int* xw, int* xch) {
+                                                     int* sids, int* xw, int* xch
+#ifdef LG_PHASE_TIMERS
+                                                     , int* ph_
+#endif
+                                                     ) {
   constexpr int NW = 4;
+#ifdef LG_PHASE_TIMERS
+  unsigned long long t0_ = wall_clock64();
+#endif
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid >> 6;
   int* ids = a.grouped + off;
   int4* recb = reinterpret_cast<int4*>(a.rec_b) + off;
-  // the cells in ascending id order (a descending network on -id, pads
-  // INT_MIN last); ids clamped into the cell range
-  int w = tid < count ? -src[tid] : INT_MIN;
-  coop_net<false>(npad, w, xw);
-  const int id = tid < count ? min(max(-w, 0), a.N - 1) : 0;
-  sids[tid] = id;
+  // the cells in ascending id order: ids are distinct, so an id's slot is
+  // its rank, the number of smaller ids (every thread counts over the
+  // bucket in LDS, broadcast reads; INT_MAX past the count)
+  const int v = tid < count ? src[tid] : INT_MAX;
+  xw[tid] = v;
+  __syncthreads();
+  if (tid < count) {
+    int rank = 0;
+    for (int q = 0; q < count; q += 4) {
+      const int4 u = *reinterpret_cast<const int4*>(xw + q);
+      rank += (u.x < v ? 1 : 0) + (u.y < v ? 1 : 0) + (u.z < v ? 1 : 0) + (u.w < v ? 1 : 0);
+    }
+    sids[rank] = v;
+  }
+  __syncthreads();
+  // ids clamped into the cell range
+  const int id = tid < count ? min(max(sids[tid], 0), a.N - 1) : 0;
+#ifdef LG_PHASE_TIMERS
+  unsigned long long t1_ = wall_clock64();
+  ph_[5] = (int)(t1_ - t0_);
+#endif
   int key = INT_MIN, kmin = INT_MAX, kmax = INT_MIN;
   if (tid < count) {
     key = sort_key(a, g, T, id, cellrec[id]);
@@ -1529,8 +1551,15 @@ __device__ __forceinline__ bool order_partition_coop(const Args& a, const PartGe
     kmax = max(kmax, xch[40 + w2]);
   }
   if (kmin < 0 || kmax >= a.pack_limit) return false;
+#ifdef LG_PHASE_TIMERS
+  unsigned long long t2_ = wall_clock64();
+  ph_[6] = (int)(t2_ - t1_);
+#endif
   int word = (int)(((unsigned)(key == INT_MIN ? 0 : key + 1) << 10) | (unsigned)tid);
   coop_net<true>(npad, word, xw);
+#ifdef LG_PHASE_TIMERS
+  ph_[7] = (int)(wall_clock64() - t2_);
+#endif
   // pads sort behind every cell: slots below count hold cells
   if (tid < count) {
     const int c = sids[word & 1023];
@@ -1811,7 +1840,11 @@ __device__ __forceinline__ void partition_body(const Args& a, int bx, int npad_l
       if (LDS_SORT) order_partition_lds(a, g, T, npad, npad_hi, src, off, count, cellrec, keys, slots, aux);
       else if (NW > 1 && npad <= NT) {
         // one slot per thread
+#ifdef LG_PHASE_TIMERS
+        const bool ok = order_partition_coop(a, g, T, npad, src, off, count, cellrec, sids, xw, xch, ph_);
+#else
         const bool ok = order_partition_coop(a, g, T, npad, src, off, count, cellrec, sids, xw, xch);
+#endif
         if (tid == 0) xch[24] = ok ? 0 : 1;
       } else if (wave == 0) {  // the register sorts run on the first wave
 #ifdef LG_PHASE_TIMERS
@@ -2086,9 +2119,10 @@ __device__ __forceinline__ void partition_body(const Args& a, int bx, int npad_l
   // -- write back pixels; primal mode fuses the dual update and the overflow
   unsigned long long ovf = 0;
   {
-    // 8 / NW pixels per thread at a time, their LDS reads issued together;
-    // the primal step also sets the next iteration's activity flags
-    constexpr int U = 8 / NW;
+    // 8 (one wave) or 4 (four waves) pixels per thread at a time, their LDS
+    // reads issued together; the primal step also sets the next iteration's
+    // activity flags
+    constexpr int U = NW > 1 ? 4 : 8;
     int lb = tid / g.xh, la = tid - lb * g.xh;
     for (int i0 = 0; i0 < npix; i0 += U * NT) {
       int px[U], py[U], li[U];
