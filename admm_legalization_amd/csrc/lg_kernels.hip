@@ -875,7 +875,8 @@ __device__ __forceinline__ void footprint_add(const PartGeo& g, const Tile& T, i
   const int n = w * h;
   const int lx0 = g.lx(x0), ly0 = g.ly(y0);
   for (int i = threadIdx.x; i < n; i += NW * kWave) {
-    int dx = i / h;
+    // (one- and two-row cells, most of them, without the division)
+    const int dx = h == 1 ? i : (h == 2 ? (i >> 1) : i / h);
     int lxi = lx0 + dx, lyi = ly0 + (i - dx * h);
     if (lxi >= g.xh) lxi -= g.xh;
     if (lyi >= g.yh) lyi -= g.yh;
