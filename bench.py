@@ -2,7 +2,7 @@
 """Throughput of the device legalizer.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
-                    [--workload served_stream|mixed_1m|plain_20k|density_1m|wl_2m]
+                    [--workload served_stream|mixed_1m|plain_20k|density_1m|density_2m_4k|wl_2m]
                     [--requests R] [--slots S] [--iters-per-sync I] [--load L]
                     [--no-cpu-baseline] [--cpu-sample-cells N] [--cpu-threads T]
 
