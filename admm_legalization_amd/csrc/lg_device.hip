@@ -363,15 +363,18 @@ int prepare_host(const LgGridDesc* g, const LgCellsSoA* c, const LgParams* p, Ho
   hp.pcode.assign(P, 0);
   hp.region_rm.assign(P, -1);
   const int XS = g->x_size, YS = g->y_size;
-  for (int x = 0; x < XS; x++)
-    for (int y = 0; y < YS; y++) {
-      size_t i = (size_t)x * YS + y, j = (size_t)y * XS + x;
-      int grp = g->pixel_group[i];
-      if (g->pixel_valid[i] && (grp + 2 > 0x7fff || grp + 2 < 0))
-        return fail(LG_ERR_ARG, "fence group index out of range");
-      hp.pcode[j] = g->pixel_valid[i] ? (int16_t)(grp + 2) : (int16_t)0;
-      if (g->pixel_region) hp.region_rm[j] = g->pixel_region[i];
-    }
+  // transposed in 64 x 64 blocks (cache-friendly on both sides)
+  for (int x0 = 0; x0 < XS; x0 += 64)
+    for (int y0 = 0; y0 < YS; y0 += 64)
+      for (int x = x0; x < std::min(x0 + 64, XS); x++)
+        for (int y = y0; y < std::min(y0 + 64, YS); y++) {
+          size_t i = (size_t)x * YS + y, j = (size_t)y * XS + x;
+          int grp = g->pixel_group[i];
+          if (g->pixel_valid[i] && (grp + 2 > 0x7fff || grp + 2 < 0))
+            return fail(LG_ERR_ARG, "fence group index out of range");
+          hp.pcode[j] = g->pixel_valid[i] ? (int16_t)(grp + 2) : (int16_t)0;
+          if (g->pixel_region) hp.region_rm[j] = g->pixel_region[i];
+        }
   hp.vdd.assign(g->row_vdd_up, g->row_vdd_up + YS);
   const int nr = std::max(g->num_regions, 0);
   hp.rx0.assign(g->region_x_min, g->region_x_min + nr);

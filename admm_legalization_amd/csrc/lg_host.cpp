@@ -97,7 +97,9 @@ void add_region(Session& S, const Rgn& r) {
 // importCells (DatabaseIO.cpp:79-169) + Grid ctor (Grid.cpp:24-67) +
 // region import (DatabaseIO.cpp:210-379) + region marking (Grid.cpp:184-314,
 // DatabaseIO.cpp:661-681) + adjustYHint (Legalizer.cpp:230-247)
-int import_design(Session& S) {
+// With pixels == false only the cells, regions and sizes are built (the
+// shape query of lg_import_and_preplace).
+int import_design(Session& S, bool pixels = true) {
   const LgDesign* d = S.d;
   if (!d || d->num_rows <= 0) { g_host_err = "design has no rows"; return LG_ERR_ARG; }
   S.core_lx = d->core_lx; S.core_ly = d->core_ly; S.core_ux = d->core_ux; S.core_uy = d->core_uy;
@@ -131,17 +133,29 @@ int import_design(Session& S) {
   g.ys = (d->core_uy - d->core_ly) / g.rh;
   if (g.xs <= 0 || g.ys <= 0) { g_host_err = "empty core"; return LG_ERR_ARG; }
   const size_t P = (size_t)g.xs * g.ys;
-  g.valid.assign(P, 0);
-  g.group.assign(P, kGroupDefault);
-  g.region.assign(P, -1);
-  g.vdd.assign(g.ys, 0);
-  for (int r = 0; r < d->num_rows; r++) {  // markDbRow (Grid.cpp:316-342)
-    auto [x1, y1, x2, y2] = g.shrunk(d->row_lx[r], d->row_ly[r], d->row_lx[r] + d->row_dx[r],
-                                     d->row_ly[r] + d->row_dy[r]);
-    if (y1 > y2 || x1 > x2) continue;
-    for (int x = x1; x <= x2; x++) std::fill_n(&g.valid[g.at(x, y1)], y2 - y1 + 1, (uint8_t)1);
-    int o = d->row_orient[r];
-    g.vdd[y1] = (o == LG_ORIENT_N || o == LG_ORIENT_FN) ? 1 : 0;
+  if (pixels) {
+    g.valid.assign(P, 0);
+    g.group.assign(P, kGroupDefault);
+    g.region.assign(P, -1);
+    g.vdd.assign(g.ys, 0);
+  }
+  const int nt = std::max(1, S.p.num_cpu_threads);
+  if (pixels) {  // markDbRow (Grid.cpp:316-342)
+    // the rows' x-spans per grid row, then the x-major map column by column
+    std::vector<std::vector<std::pair<int, int>>> spans(g.ys);
+    for (int r = 0; r < d->num_rows; r++) {
+      auto [x1, y1, x2, y2] = g.shrunk(d->row_lx[r], d->row_ly[r], d->row_lx[r] + d->row_dx[r],
+                                       d->row_ly[r] + d->row_dy[r]);
+      if (y1 > y2 || x1 > x2) continue;
+      for (int y = y1; y <= y2; y++) spans[y].push_back({x1, x2});
+      int o = d->row_orient[r];
+      g.vdd[y1] = (o == LG_ORIENT_N || o == LG_ORIENT_FN) ? 1 : 0;
+    }
+#pragma omp parallel for num_threads(nt) schedule(static)
+    for (int x = 0; x < g.xs; x++)
+      for (int y = 0; y < g.ys; y++)
+        for (const auto& sp : spans[y])
+          if (sp.first <= x && x <= sp.second) { g.valid[g.at(x, y)] = 1; break; }
   }
   auto clip = [&](const std::array<int, 4>& r) {
     return std::array<int, 4>{std::max(S.core_lx, r[0]), std::max(S.core_ly, r[1]),
@@ -193,8 +207,8 @@ int import_design(Session& S) {
   }
   bool any_fence = false;
   for (auto& r : S.regions) any_fence |= (r.kind == kFenceRegion);
-  if (any_fence) { g.hov.assign(P, 0); g.vov.assign(P, 0); }
-  for (int id = 0; id < (int)S.regions.size(); id++) {
+  if (pixels && any_fence) { g.hov.assign(P, 0); g.vov.assign(P, 0); }
+  for (int id = 0; id < (int)S.regions.size() && pixels; id++) {
     const Rgn& r = S.regions[id];
     if (r.kind == kBlockageRegion) {  // Grid::markBlockageRegion (Grid.cpp:286-314)
       auto [x1, y1, x2, y2] = g.expanded(r.lx, r.ly, r.ux, r.uy);
@@ -220,17 +234,23 @@ int import_design(Session& S) {
         for (int x = ex1; x <= ex2; x++) g.vov[g.at(x, ey1)] += (sy1 * g.rh - r.ly) - g.rh;
       if (ey2 == sy2 + 1)
         for (int x = ex1; x <= ex2; x++) g.vov[g.at(x, ey2)] += (r.uy - ey2 * g.rh) - g.rh;
-    } else {  // Grid::markNonGroupRegion (Grid.cpp:264-284)
+    } else {  // Grid::markNonGroupRegion (Grid.cpp:264-284); columns in parallel
       auto [x1, y1, x2, y2] = g.shrunk(r.lx, r.ly, r.ux, r.uy);
       x1 = std::clamp(x1, 0, g.xs - 1); y1 = std::clamp(y1, 0, g.ys - 1);
       x2 = std::clamp(x2, 0, g.xs - 1); y2 = std::clamp(y2, 0, g.ys - 1);
+      const long long area = (long long)(x2 - x1 + 1) * (y2 - y1 + 1);
+#pragma omp parallel for num_threads(nt) schedule(static) if (area > (1 << 16))
       for (int x = x1; x <= x2; x++)
         for (int y = y1; y <= y2; y++) g.region[g.at(x, y)] = id;
     }
   }
-  if (any_fence)  // handleRegionSegments (DatabaseIO.cpp:661-681)
-    for (size_t p = 0; p < P; p++)
-      if ((g.hov[p] > 0 || g.vov[p] > 0) && (g.hov[p] < g.sw || g.vov[p] < g.rh)) g.valid[p] = 0;
+  if (pixels) {
+    if (any_fence) {  // handleRegionSegments (DatabaseIO.cpp:661-681)
+#pragma omp parallel for num_threads(nt) schedule(static)
+      for (long long p = 0; p < (long long)P; p++)
+        if ((g.hov[p] > 0 || g.vov[p] > 0) && (g.hov[p] < g.sw || g.vov[p] < g.rh)) g.valid[p] = 0;
+    }
+  }
   for (auto& [hh, n] : hdist) S.max_height = std::max(S.max_height, hh);
   int mh = S.max_height / g.rh;
   if (S.p.y_hint < 3 * mh) S.p.y_hint = 3 * mh;
@@ -411,15 +431,18 @@ int lg_import_and_preplace(const LgDesign* d, const LgParams* p, int32_t* shape,
   Session S;
   S.p = *p;
   S.d = d;
-  int st = import_design(S);
+  const double t0 = now_s();
+  int st = import_design(S, valid != nullptr);  // a shape query builds no pixels
   if (st) return st;
   const PixelGrid& g = S.grid;
   shape[0] = g.xs; shape[1] = g.ys;
   shape[2] = (int)S.cells.size();
   shape[3] = (int)S.regions.size();
   if (!valid) return LG_OK;  // shape query
-  const size_t P = (size_t)g.xs * g.ys;
-  for (size_t i = 0; i < P; i++) { valid[i] = g.valid[i]; group[i] = g.group[i]; region[i] = g.region[i]; }
+  const long long P = (long long)g.xs * g.ys;
+  const int nt = std::max(1, S.p.num_cpu_threads);
+#pragma omp parallel for num_threads(nt) schedule(static)
+  for (long long i = 0; i < P; i++) { valid[i] = g.valid[i]; group[i] = g.group[i]; region[i] = g.region[i]; }
   for (int y = 0; y < g.ys; y++) vdd_up[y] = g.vdd[y];
   for (size_t k = 0; k < S.regions.size(); k++) {
     const Rgn& rg = S.regions[k];
@@ -427,8 +450,11 @@ int lg_import_and_preplace(const LgDesign* d, const LgParams* p, int32_t* shape,
     region_bounds[4 * k] = a; region_bounds[4 * k + 1] = b;
     region_bounds[4 * k + 2] = c; region_bounds[4 * k + 3] = e;
   }
+  const double t1 = now_s();
   preplace(S);
   for (size_t i = 0; i < S.cells.size(); i++) { cell_lx[i] = S.cells[i].lx; cell_ly[i] = S.cells[i].ly; }
+  if (S.p.verbose)
+    std::fprintf(stderr, "[lg] import %.3f s (pixel maps out included), preplace %.3f s\n", t1 - t0, now_s() - t1);
   return LG_OK;
 }
 
