@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <random>
 #include <string>
+#include <thread>
 #include <vector>
 #include <map>
 #include <unordered_map>
@@ -361,20 +362,37 @@ int prepare_host(const LgGridDesc* g, const LgCellsSoA* c, const LgParams* p, Ho
   // (x * y_size + y, the reference's Grid layout); the device keeps them
   // row-major (y * x_size + x).
   hp.pcode.assign(P, 0);
-  hp.region_rm.assign(P, -1);
+  // pixel regions are read by the adaptive search only
+  const bool regions = p->use_adaptive_search && g->pixel_region;
+  if (p->use_adaptive_search) hp.region_rm.assign(P, -1);
   const int XS = g->x_size, YS = g->y_size;
-  // transposed in 64 x 64 blocks (cache-friendly on both sides)
-  for (int x0 = 0; x0 < XS; x0 += 64)
-    for (int y0 = 0; y0 < YS; y0 += 64)
-      for (int x = x0; x < std::min(x0 + 64, XS); x++)
-        for (int y = y0; y < std::min(y0 + 64, YS); y++) {
-          size_t i = (size_t)x * YS + y, j = (size_t)y * XS + x;
-          int grp = g->pixel_group[i];
-          if (g->pixel_valid[i] && (grp + 2 > 0x7fff || grp + 2 < 0))
-            return fail(LG_ERR_ARG, "fence group index out of range");
-          hp.pcode[j] = g->pixel_valid[i] ? (int16_t)(grp + 2) : (int16_t)0;
-          if (g->pixel_region) hp.region_rm[j] = g->pixel_region[i];
-        }
+  // transposed in 64 x 64 blocks (cache-friendly on both sides), column
+  // blocks spread over host threads
+  const int nxb = (XS + 63) / 64;
+  const int nth = std::max(1, std::min({p->num_cpu_threads, nxb, 32}));
+  std::vector<int> bad(nth, 0);
+  auto work = [&](int t) {
+    for (int xb = t; xb < nxb; xb += nth)
+      for (int y0 = 0; y0 < YS; y0 += 64)
+        for (int x = xb * 64; x < std::min(xb * 64 + 64, XS); x++)
+          for (int y = y0; y < std::min(y0 + 64, YS); y++) {
+            size_t i = (size_t)x * YS + y, j = (size_t)y * XS + x;
+            int grp = g->pixel_group[i];
+            if (g->pixel_valid[i] && (grp + 2 > 0x7fff || grp + 2 < 0)) bad[t] = 1;
+            hp.pcode[j] = g->pixel_valid[i] ? (int16_t)(grp + 2) : (int16_t)0;
+            if (regions) hp.region_rm[j] = g->pixel_region[i];
+          }
+  };
+  if (nth == 1 || P < (1u << 20)) {
+    for (int t = 0; t < nth; t++) work(t);
+  } else {
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nth; t++) pool.emplace_back(work, t);
+    work(0);
+    for (auto& th : pool) th.join();
+  }
+  for (int t = 0; t < nth; t++)
+    if (bad[t]) return fail(LG_ERR_ARG, "fence group index out of range");
   hp.vdd.assign(g->row_vdd_up, g->row_vdd_up + YS);
   const int nr = std::max(g->num_regions, 0);
   hp.rx0.assign(g->region_x_min, g->region_x_min + nr);
@@ -426,7 +444,7 @@ int prepare_host(const LgGridDesc* g, const LgCellsSoA* c, const LgParams* p, Ho
 // buffers, each rounded to the pool's 256-byte blocks).
 size_t handle_bytes(const HostPrep& hp) {
   const size_t P = hp.P, N = std::max(hp.N, 1), K = hp.K, blk = 256;
-  size_t b = P * (2 + 4 + 4 + 4);                     // pcode, region, usage, dual
+  size_t b = P * (2 + (hp.p.use_adaptive_search ? 4 : 0) + 4 + 4);  // pcode, region, usage, dual
   b += N * 4 * (2 + 2 + 8 + 1 + 1 + 2 + 2);           // positions, cells, rot, grouped, flags, timing
   b += N * 4 * 4 * 2;                                 // cellrec, rec_b
   b += K * 4 * ((size_t)kMaxNpad + 1 + 1 + 2 + 1 + 1 + 2 + 2);  // buckets, counts, offsets, flags, lists
@@ -490,10 +508,10 @@ int create_from_prep(const HostPrep& hp, DevPool* pool, Ctl* ext_ctl, hipStream_
     HIP_TRY(hipStreamCreateWithFlags(&h->own_stream, hipStreamNonBlocking));
   }
   int16_t* d_pcode;
-  int32_t *d_region, *d_vdd, *d_rx0, *d_ry0, *d_rx1, *d_ry1;
+  int32_t *d_region = nullptr, *d_vdd, *d_rx0, *d_ry0, *d_rx1, *d_ry1;
   const size_t nr = hp.rx0.size();
   if ((st = upload(h, &d_pcode, hp.pcode.data(), P))) return bail(st);
-  if ((st = upload(h, &d_region, hp.region_rm.data(), P))) return bail(st);
+  if (!hp.region_rm.empty() && (st = upload(h, &d_region, hp.region_rm.data(), P))) return bail(st);
   if ((st = upload(h, &d_vdd, hp.vdd.data(), (size_t)hp.y_size))) return bail(st);
   if ((st = upload(h, &d_rx0, hp.rx0.data(), nr))) return bail(st);
   if ((st = upload(h, &d_ry0, hp.ry0.data(), nr))) return bail(st);
