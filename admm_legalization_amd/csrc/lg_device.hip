@@ -12,6 +12,7 @@
 #include <numeric>
 #include <cstdlib>
 #include <random>
+#include <chrono>
 #include <string>
 #include <thread>
 #include <vector>
@@ -749,9 +750,17 @@ const char* lg_last_error(void) { return g_err.c_str(); }
 int lg_create(const LgGridDesc* g, const LgCellsSoA* c, const LgParams* p, LgHandle** out) {
   if (!out) return fail(LG_ERR_ARG, "null argument");
   HostPrep hp;
+  const auto t0 = std::chrono::steady_clock::now();
   int st = prepare_host(g, c, p, hp);
   if (st) return st;
-  return create_from_prep(hp, nullptr, nullptr, nullptr, out);
+  const auto t1 = std::chrono::steady_clock::now();
+  st = create_from_prep(hp, nullptr, nullptr, nullptr, out);
+  if (!st && p->verbose) {
+    const auto t2 = std::chrono::steady_clock::now();
+    std::fprintf(stderr, "[lg] create: host layout %.3f s, device buffers and upload %.3f s\n",
+                 std::chrono::duration<double>(t1 - t0).count(), std::chrono::duration<double>(t2 - t1).count());
+  }
+  return st;
 }
 
 int lg_solve(LgHandle* h, void* stream) {

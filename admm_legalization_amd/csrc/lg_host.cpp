@@ -207,7 +207,23 @@ int import_design(Session& S, bool pixels = true) {
   }
   bool any_fence = false;
   for (auto& r : S.regions) any_fence |= (r.kind == kFenceRegion);
-  if (pixels && any_fence) { g.hov.assign(P, 0); g.vov.assign(P, 0); }
+  // Fences whose pixel boxes do not overlap: a pixel inside one box gets
+  // exactly that fence's overlap counts, full (sw, rh) except on the box's
+  // edge columns / rows, so only the edges need the test of
+  // handleRegionSegments and no per-pixel counts are kept.  Otherwise the
+  // counts are accumulated over the whole grid.
+  bool fences_disjoint = true;
+  {
+    std::vector<std::array<int, 4>> fb;
+    for (auto& r : S.regions)
+      if (r.kind == kFenceRegion) fb.push_back(g.expanded(r.lx, r.ly, r.ux, r.uy));
+    if (fb.size() > 256) fences_disjoint = false;
+    for (size_t i = 0; i < fb.size() && fences_disjoint; i++)
+      for (size_t j = i + 1; j < fb.size() && fences_disjoint; j++)
+        if (fb[i][0] <= fb[j][2] && fb[j][0] <= fb[i][2] && fb[i][1] <= fb[j][3] && fb[j][1] <= fb[i][3])
+          fences_disjoint = false;
+  }
+  if (pixels && any_fence && !fences_disjoint) { g.hov.assign(P, 0); g.vov.assign(P, 0); }
   for (int id = 0; id < (int)S.regions.size() && pixels; id++) {
     const Rgn& r = S.regions[id];
     if (r.kind == kBlockageRegion) {  // Grid::markBlockageRegion (Grid.cpp:286-314)
@@ -224,7 +240,13 @@ int import_design(Session& S, bool pixels = true) {
       for (int x = ex1; x <= ex2; x++)
         for (int y = ey1; y <= ey2; y++) {
           size_t p = g.at(x, y);
-          g.group[p] = r.group; g.hov[p] += g.sw; g.vov[p] += g.rh; g.region[p] = id; g.valid[p] = 1;
+          g.group[p] = r.group; g.region[p] = id; g.valid[p] = 1;
+        }
+      if (fences_disjoint) continue;  // edges tested below
+      for (int x = ex1; x <= ex2; x++)
+        for (int y = ey1; y <= ey2; y++) {
+          size_t p = g.at(x, y);
+          g.hov[p] += g.sw; g.vov[p] += g.rh;
         }
       if (ex1 == sx1 - 1)
         for (int y = ey1; y <= ey2; y++) g.hov[g.at(ex1, y)] += (sx1 * g.sw - r.lx) - g.sw;
@@ -244,11 +266,36 @@ int import_design(Session& S, bool pixels = true) {
         for (int y = y1; y <= y2; y++) g.region[g.at(x, y)] = id;
     }
   }
-  if (pixels) {
-    if (any_fence) {  // handleRegionSegments (DatabaseIO.cpp:661-681)
+  if (pixels && any_fence && !fences_disjoint) {  // handleRegionSegments (DatabaseIO.cpp:661-681)
 #pragma omp parallel for num_threads(nt) schedule(static)
-      for (long long p = 0; p < (long long)P; p++)
-        if ((g.hov[p] > 0 || g.vov[p] > 0) && (g.hov[p] < g.sw || g.vov[p] < g.rh)) g.valid[p] = 0;
+    for (long long p = 0; p < (long long)P; p++)
+      if ((g.hov[p] > 0 || g.vov[p] > 0) && (g.hov[p] < g.sw || g.vov[p] < g.rh)) g.valid[p] = 0;
+  }
+  if (pixels && any_fence && fences_disjoint) {
+    // the same test on the edge columns and rows of each fence's box, with
+    // the counts markGroupRegion gives them (Grid.cpp:184-262)
+    for (const Rgn& r : S.regions) {
+      if (r.kind != kFenceRegion) continue;
+      auto [ex1, ey1, ex2, ey2] = g.expanded(r.lx, r.ly, r.ux, r.uy);
+      auto [sx1, sy1, sx2, sy2] = g.shrunk(r.lx, r.ly, r.ux, r.uy);
+      auto hov = [&](int x) {
+        int v = g.sw;
+        if (x == ex1 && ex1 == sx1 - 1) v += (sx1 * g.sw - r.lx) - g.sw;
+        if (x == ex2 && ex2 == sx2 + 1) v += (r.ux - ex2 * g.sw) - g.sw;
+        return v;
+      };
+      auto vov = [&](int y) {
+        int v = g.rh;
+        if (y == ey1 && ey1 == sy1 - 1) v += (sy1 * g.rh - r.ly) - g.rh;
+        if (y == ey2 && ey2 == sy2 + 1) v += (r.uy - ey2 * g.rh) - g.rh;
+        return v;
+      };
+      auto test = [&](int x, int y) {
+        const int h = hov(x), v = vov(y);
+        if ((h > 0 || v > 0) && (h < g.sw || v < g.rh)) g.valid[g.at(x, y)] = 0;
+      };
+      for (int y = ey1; y <= ey2; y++) { test(ex1, y); test(ex2, y); }
+      for (int x = ex1; x <= ex2; x++) { test(x, ey1); test(x, ey2); }
     }
   }
   for (auto& [hh, n] : hdist) S.max_height = std::max(S.max_height, hh);

@@ -164,10 +164,27 @@ def test_oracle_solve_invariants_and_determinism(seed):
     assert o1["trace"][-1] == 0
 
 
+def _fences_off_grid(overlap):
+    """mixed(3000) with its fence boxes moved off the site/row grid (partial
+    edge pixels, whose validity handleRegionSegments decides), optionally
+    with two boxes overlapping."""
+    d = synth.mixed(3000, seed=2)
+    sw, rh = d.site_width, d.row_height
+    d.box_lx = (d.box_lx + sw // 3).astype(np.int32)
+    d.box_ux = (d.box_ux - sw // 4).astype(np.int32)
+    d.box_ly = (d.box_ly + rh // 2).astype(np.int32)
+    d.box_uy = (d.box_uy + rh // 3).astype(np.int32)
+    if overlap and len(d.box_lx) >= 2:
+        d.box_lx[1], d.box_ly[1] = d.box_lx[0] + sw // 2, d.box_ly[0] + rh // 2
+        d.box_ux[1], d.box_uy[1] = d.box_lx[0] + 7 * sw + sw // 2, d.box_ly[0] + 3 * rh + rh // 3
+    return d
+
+
 # -- product host layer == oracle host layer --------------------------------
 @pytest.mark.parametrize("maker", [lambda: synth.mixed(3000, seed=2),
                                    lambda: synth.tiny(num_cells=120, x_size=64, y_size=24, seed=1, fences=1),
-                                   lambda: synth.plain_20k()])
+                                   lambda: synth.plain_20k(),
+                                   lambda: _fences_off_grid(False), lambda: _fences_off_grid(True)])
 def test_host_import_and_preplace_matches_oracle(maker):
     d = maker()
     a = lg.import_and_preplace(d)
