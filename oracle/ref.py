@@ -46,6 +46,10 @@ def lib():
         L.ref_tree_argmin.argtypes = [C.POINTER(C.c_float), C.c_int]
         L.ref_tree_argmin.restype = C.c_int
         L.ref_thomas.argtypes = [C.c_int, C.c_double, C.c_double] + [C.POINTER(C.c_double)] * 4
+        f64p = C.POINTER(C.c_double)
+        L.ref_qp_admm.argtypes = [C.c_int, i32p, f64p, i32p, i32p, i32p, C.c_double, C.c_int, C.c_int, f64p]
+        L.ref_qp_admm.restype = C.c_int
+        L.ref_site_align_dp_row.argtypes = [C.c_int, f64p, f64p, i32p, C.c_int, C.c_int, C.c_int, C.c_int, i32p]
         _lib = L
     return _lib
 
@@ -162,3 +166,32 @@ def thomas(a, b, d, cprime):
     lib().ref_thomas(n, a, b, d.ctypes.data_as(dd), cp.ctypes.data_as(dd), dp.ctypes.data_as(dd),
                      x.ctypes.data_as(dd))
     return x
+
+
+def qp_admm(ncell, x0, wd, rlx, rux, rho=2.0, min_iter=5, max_iter=10000):
+    """The cut-row QP of the post-process alone (before site alignment): rows of
+    ncell[r] cells in x order, x0/wd per cell in DBU, row extents rlx/rux.
+    Returns (x, iterations)."""
+    ncell = np.ascontiguousarray(ncell, np.int32)
+    x0 = np.ascontiguousarray(x0, np.float64)
+    wd = np.ascontiguousarray(wd, np.int32)
+    rlx = np.ascontiguousarray(rlx, np.int32)
+    rux = np.ascontiguousarray(rux, np.int32)
+    x = np.zeros(len(x0))
+    dd = C.POINTER(C.c_double)
+    it = lib().ref_qp_admm(len(ncell), _p(ncell), x0.ctypes.data_as(dd), _p(wd), _p(rlx), _p(rux), rho, min_iter,
+                           max_iter, x.ctypes.data_as(dd))
+    return x, it
+
+
+def site_align_dp_row(x, x0, wd, rlx, rux, sw, core_lx):
+    """Site alignment of one cut row after the QP: site index per cell."""
+    n = len(x)
+    x = np.ascontiguousarray(x, np.float64)
+    x0 = np.ascontiguousarray(x0, np.float64)
+    wd = np.ascontiguousarray(wd, np.int32)
+    site = np.zeros(n, np.int32)
+    dd = C.POINTER(C.c_double)
+    lib().ref_site_align_dp_row(n, x.ctypes.data_as(dd), x0.ctypes.data_as(dd), _p(wd), rlx, rux, sw, core_lx,
+                                _p(site))
+    return site

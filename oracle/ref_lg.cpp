@@ -16,7 +16,10 @@
 // Parity pins (tests/test_oracle_*.py): brute-force candidate enumeration,
 // invariants after every iteration, closed forms for rho / lambda0, the
 // bitonic network against a reference sort, Thomas against a dense solve,
-// the DP against exhaustive enumeration.
+// the cut-row QP run to convergence against the bounded isotonic-regression
+// closed form, the site-alignment DP against exhaustive enumeration.
+// Partially pinned (device == oracle and legality invariants only): the
+// adaptive candidate search and the ICCAD17 displacement weighting.
 #include "../include/skyline_lg.h"
 
 #include <algorithm>
@@ -1122,47 +1125,45 @@ static void make_cut_rows(const Dev& D, CutRows& R) {
   }
 }
 
-// CudaQpSolver::initializeSolver / solve / siteAlignDP (CudaQpSolver.cu:22-353;
-// QpSolverKernels.hpp:96-475), FP64 throughout
-static int qp_refine(Dev& D) {
-  CutRows R;
-  make_cut_rows(D, R);
-  const int nr = R.num;
-  int nm = 0;
-  for (int r = 0; r < nr; r++) nm += R.ncell[r];
-  const int nc = nm + nr;
-  const double rho = D.p.qp_admm_rho;
-  std::vector<int> l2g(nm), pos_c(nm), neg_c(nm), c_pos(nc), c_neg(nc);
-  std::vector<double> x(nm), x0(nm), bound(nc), pp(nc, 0.0), slack(nc, 0.0), dualv(nc, 0.0);
-  for (int l = 0; l < nm; l++) {  // assignLocalIndexKernel
-    int g = R.cells[l];
-    l2g[l] = g;
-    x[l] = double(D.core_lx + D.sw * D.lx[g]);
-    x0[l] = double(D.ox[g]);
-  }
-  for (int r = 0; r < nr; r++) {  // generateConstraintsKernel
-    int n = R.ncell[r], off = R.offset[r];
-    int rlx = D.sw * R.xb[r] + D.core_lx, rux = D.sw * (R.xe[r] + 1) + D.core_lx;
+// generateConstraintsKernel (QpSolverKernels.hpp): one constraint per cell
+// boundary of every cut row (n + 1 per row, constraint id off + r + i), reading
+// x[pos] - x[neg] >= bound.  wd[l] is cell l's width in DBU; rlx/rux the row's
+// DBU extent.
+static void qp_constraints(int nr, const int* ncell, const int* offset, const int* wd, const int* rlx,
+                           const int* rux, int* pos_c, int* neg_c, int* c_pos, int* c_neg, double* bound) {
+  for (int r = 0; r < nr; r++) {
+    int n = ncell[r], off = offset[r];
     for (int i = 0; i < n + 1; i++) {
       int cid = off + r + i;
       int lid = (i == n) ? -1 : i + off;
       int plid = (i == 0) ? -1 : i + off - 1;
-      int pw = (i == 0) ? 0 : D.sw * D.w[l2g[plid]];
+      int pw = (i == 0) ? 0 : wd[plid];
       if (lid != -1) pos_c[lid] = cid;
       if (plid != -1) neg_c[plid] = cid;
       c_pos[cid] = lid;
       c_neg[cid] = plid;
-      bound[cid] = (n == 0) ? 0.0 : (i == 0) ? double(rlx) : (i == n) ? double(pw - rux) : double(pw);
+      bound[cid] = (n == 0) ? 0.0 : (i == 0) ? double(rlx[r]) : (i == n) ? double(pw - rux[r]) : double(pw);
     }
   }
+}
+
+// CudaQpSolver::solve (CudaQpSolver.cu; QpSolverKernels.hpp:96-300): ADMM on
+// min sum (x - x0)^2 subject to the cut-row constraints; the x-update is one
+// tridiagonal (Thomas) solve per row.  Stops once no constraint is violated by
+// more than 1 DBU and min_iter iterations have run.  Returns the iteration count.
+static int qp_admm(int nr, const int* ncell, const int* offset, int nm, const double* x0, const int* pos_c,
+                   const int* neg_c, const int* c_pos, const int* c_neg, const double* bound, double rho,
+                   int min_iter, int max_iter, double* x) {
+  const int nc = nm + nr;
+  std::vector<double> pp(nc, 0.0), slack(nc, 0.0), dualv(nc, 0.0);
   std::vector<double> cp(nm), din(nm), dp(nm);
   for (int r = 0; r < nr; r++)  // precomputeCPrimeKernel
-    for (int i = 0; i < R.ncell[r]; i++) {
-      int l = R.offset[r] + i;
+    for (int i = 0; i < ncell[r]; i++) {
+      int l = offset[r] + i;
       cp[l] = (i == 0) ? -rho / (1 + 2 * rho) : -rho / (1 + 2 * rho + rho * cp[l - 1]);
     }
   int it_done = 0;
-  for (int it = 0; it < D.p.qp_admm_max_iter; it++) {
+  for (int it = 0; it < max_iter; it++) {
     for (int l = 0; l < nm; l++) {  // updatePrimalXKernel
       double v = x0[l];
       int a = pos_c[l], b = neg_c[l];
@@ -1171,8 +1172,8 @@ static int qp_refine(Dev& D) {
       din[l] = v;
     }
     for (int r = 0; r < nr; r++) {  // runThomasKernel
-      int off = R.offset[r];
-      thomas(R.ncell[r], -rho, 1 + 2 * rho, din.data() + off, cp.data() + off, dp.data() + off, x.data() + off);
+      int off = offset[r];
+      thomas(ncell[r], -rho, 1 + 2 * rho, din.data() + off, cp.data() + off, dp.data() + off, x + off);
     }
     for (int k = 0; k < nc; k++) {  // computeConstraintValueKernel
       int a = c_pos[k], b = c_neg[k];
@@ -1183,52 +1184,92 @@ static int qp_refine(Dev& D) {
     double mn = std::numeric_limits<double>::infinity();
     for (int k = 0; k < nc; k++) mn = std::min(mn, pp[k] - bound[k]);
     it_done = it + 1;
-    if (!(mn < -1 || it < D.p.qp_admm_min_iter)) break;
+    if (!(mn < -1 || it < min_iter)) break;
   }
-  std::vector<int> lx_new(D.lx);
+  return it_done;
+}
+
+// siteAlignDPKernel (QpSolverKernels.hpp:300-475) for one cut row: every cell
+// goes to the site at or just right of its QP position (clamped to the row's
+// left end); a two-state DP over "left site" / "right site" minimises the summed
+// |site - x0| with no overlap and the right site inside the row; ties go left.
+// site[i] receives the chosen site index of cell i.
+static void site_align_dp_row(int n, const double* x, const double* x0, const int* wdv, int rlx, int rux, int sw,
+                              int core_lx, int* bl, int* br, int* site) {
   const double INF = std::numeric_limits<double>::infinity();
+  double cl = 0, cr = 0;
+  int uxl = rlx, uxr = rlx;
+  for (int i = 0; i < n; i++) {
+    int wd = wdv[i];
+    int orig = (int)x0[i];
+    int clamp = std::max(rlx, (int)x[i]);
+    int dist = clamp - core_lx;
+    int nl = (dist / sw) * sw + core_lx, nrt = (dist / sw + 1) * sw + core_lx;
+    double dl = std::abs(nl - orig), dr = std::abs(nrt - orig);
+    double ml, mr;
+    if (i == 0) {
+      ml = dl + ((nl < rlx) ? INF : 0.0);
+      mr = dr + ((nrt < rlx || nrt + wd > rux) ? INF : 0.0);
+      bl[i] = -1; br[i] = -1;
+    } else {
+      double a1 = cl + dl + ((nl < uxl) ? INF : 0.0);
+      double a2 = cr + dl + ((nl < uxr) ? INF : 0.0);
+      if (a1 <= a2) { ml = a1; bl[i] = 0; } else { ml = a2; bl[i] = 1; }
+      double bp = (nrt + wd > rux) ? INF : 0.0;
+      double b1 = cl + dr + (((nrt < uxl) ? INF : 0.0) + bp);
+      double b2 = cr + dr + (((nrt < uxr) ? INF : 0.0) + bp);
+      if (b1 <= b2) { mr = b1; br[i] = 0; } else { mr = b2; br[i] = 1; }
+    }
+    cl = ml; cr = mr;
+    uxl = nl + wd; uxr = nrt + wd;
+  }
+  int st = (cl <= cr) ? 0 : 1;
+  for (int i = n - 1; i >= 0; i--) {
+    int clamp = std::max(rlx, (int)x[i]);
+    int dist = clamp - core_lx;
+    site[i] = (st == 0) ? dist / sw : dist / sw + 1;
+    st = (st == 0) ? bl[i] : br[i];
+  }
+}
+
+// CudaQpSolver::initializeSolver / solve / siteAlignDP (CudaQpSolver.cu:22-353;
+// QpSolverKernels.hpp:96-475), FP64 throughout
+static int qp_refine(Dev& D) {
+  CutRows R;
+  make_cut_rows(D, R);
+  const int nr = R.num;
+  int nm = 0;
+  for (int r = 0; r < nr; r++) nm += R.ncell[r];
+  const int nc = nm + nr;
+  const double rho = D.p.qp_admm_rho;
+  std::vector<int> l2g(nm), pos_c(nm), neg_c(nm), c_pos(nc), c_neg(nc), wd(nm), rlx(nr), rux(nr);
+  std::vector<double> x(nm), x0(nm), bound(nc);
+  for (int l = 0; l < nm; l++) {  // assignLocalIndexKernel
+    int g = R.cells[l];
+    l2g[l] = g;
+    x[l] = double(D.core_lx + D.sw * D.lx[g]);
+    x0[l] = double(D.ox[g]);
+    wd[l] = D.sw * D.w[g];
+  }
+  for (int r = 0; r < nr; r++) {
+    rlx[r] = D.sw * R.xb[r] + D.core_lx;
+    rux[r] = D.sw * (R.xe[r] + 1) + D.core_lx;
+  }
+  qp_constraints(nr, R.ncell.data(), R.offset.data(), wd.data(), rlx.data(), rux.data(), pos_c.data(),
+                 neg_c.data(), c_pos.data(), c_neg.data(), bound.data());
+  const int it_done = qp_admm(nr, R.ncell.data(), R.offset.data(), nm, x0.data(), pos_c.data(), neg_c.data(), c_pos.data(),
+          c_neg.data(), bound.data(), rho, D.p.qp_admm_min_iter, D.p.qp_admm_max_iter, x.data());
+  std::vector<int> lx_new(D.lx);
   if (D.p.qp_admm_dp_site_align) {
-    std::vector<int> bl(nm), br(nm);
+    std::vector<int> bl(nm), br(nm), site(nm);
     for (int r = 0; r < nr; r++) {  // siteAlignDPKernel
       int n = R.ncell[r];
       if (n == 0) continue;
       int rlx = D.sw * R.xb[r] + D.core_lx, rux = D.sw * (R.xe[r] + 1) + D.core_lx;
       int off = R.offset[r];
-      double cl = 0, cr = 0;
-      int uxl = rlx, uxr = rlx;
-      for (int i = 0; i < n; i++) {
-        int l = i + off, g = l2g[l];
-        int wd = D.w[g] * D.sw;
-        int orig = (int)x0[l];
-        int clamp = std::max(rlx, (int)x[l]);
-        int dist = clamp - D.core_lx;
-        int nl = (dist / D.sw) * D.sw + D.core_lx, nrt = (dist / D.sw + 1) * D.sw + D.core_lx;
-        double dl = std::abs(nl - orig), dr = std::abs(nrt - orig);
-        double ml, mr;
-        if (i == 0) {
-          ml = dl + ((nl < rlx) ? INF : 0.0);
-          mr = dr + ((nrt < rlx || nrt + wd > rux) ? INF : 0.0);
-          bl[l] = -1; br[l] = -1;
-        } else {
-          double a1 = cl + dl + ((nl < uxl) ? INF : 0.0);
-          double a2 = cr + dl + ((nl < uxr) ? INF : 0.0);
-          if (a1 <= a2) { ml = a1; bl[l] = 0; } else { ml = a2; bl[l] = 1; }
-          double bp = (nrt + wd > rux) ? INF : 0.0;
-          double b1 = cl + dr + (((nrt < uxl) ? INF : 0.0) + bp);
-          double b2 = cr + dr + (((nrt < uxr) ? INF : 0.0) + bp);
-          if (b1 <= b2) { mr = b1; br[l] = 0; } else { mr = b2; br[l] = 1; }
-        }
-        cl = ml; cr = mr;
-        uxl = nl + wd; uxr = nrt + wd;
-      }
-      int st = (cl <= cr) ? 0 : 1;
-      for (int i = n - 1; i >= 0; i--) {
-        int l = i + off, g = l2g[l];
-        int clamp = std::max(rlx, (int)x[l]);
-        int dist = clamp - D.core_lx;
-        lx_new[g] = (st == 0) ? dist / D.sw : dist / D.sw + 1;
-        st = (st == 0) ? bl[l] : br[l];
-      }
+      site_align_dp_row(n, x.data() + off, x0.data() + off, wd.data() + off, rlx, rux, D.sw, D.core_lx,
+                        bl.data() + off, br.data() + off, site.data() + off);
+      for (int i = n - 1; i >= 0; i--) lx_new[l2g[i + off]] = site[i + off];
     }
   } else {
     for (int r = 0; r < nr; r++) {  // siteAlignNaiveKernel
@@ -1469,6 +1510,29 @@ void ref_bitonic_asc(int n, int* key, int* val) { ref::bitonic_asc(n, key, val);
 int ref_tree_argmin(float* cost, int n) { return ref::tree_argmin(cost, n); }
 void ref_thomas(int n, double a, double b, const double* d, const double* cp, double* dp, double* x) {
   ref::thomas(n, a, b, d, cp, dp, x);
+}
+// One cut row's site alignment alone; site receives site indices.
+void ref_site_align_dp_row(int n, const double* x, const double* x0, const int* wd, int rlx, int rux, int sw,
+                           int core_lx, int* site) {
+  std::vector<int> bl(n), br(n);
+  ref::site_align_dp_row(n, x, x0, wd, rlx, rux, sw, core_lx, bl.data(), br.data(), site);
+}
+// The cut-row QP alone (before site alignment): rows given by ncell; x0, wd in
+// DBU per cell, row extents rlx/rux.  x receives the ADMM solution; returns the
+// iteration count.
+int ref_qp_admm(int nr, const int* ncell, const double* x0, const int* wd, const int* rlx, const int* rux,
+                double rho, int min_iter, int max_iter, double* x) {
+  std::vector<int> offset(nr);
+  int nm = 0;
+  for (int r = 0; r < nr; r++) { offset[r] = nm; nm += ncell[r]; }
+  const int nc = nm + nr;
+  std::vector<int> pos_c(nm), neg_c(nm), c_pos(nc), c_neg(nc);
+  std::vector<double> bound(nc);
+  ref::qp_constraints(nr, ncell, offset.data(), wd, rlx, rux, pos_c.data(), neg_c.data(), c_pos.data(),
+                      c_neg.data(), bound.data());
+  for (int l = 0; l < nm; l++) x[l] = x0[l];
+  return ref::qp_admm(nr, ncell, offset.data(), nm, x0, pos_c.data(), neg_c.data(), c_pos.data(), c_neg.data(),
+                      bound.data(), rho, min_iter, max_iter, x);
 }
 
 }  // extern "C"

@@ -193,3 +193,116 @@ def test_host_import_and_preplace_matches_oracle(maker):
         assert a[k] == b[k]
     for k in ("valid", "group", "region", "vdd_up", "region_bounds", "cell_lx", "cell_ly"):
         np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+
+
+def _pava(t):
+    """L2 isotonic regression (pool adjacent violators), unit weights."""
+    vals, wts = [], []
+    for v in t:
+        vals.append(float(v))
+        wts.append(1)
+        while len(vals) > 1 and vals[-2] > vals[-1]:
+            v2, w2 = vals.pop(), wts.pop()
+            v1, w1 = vals.pop(), wts.pop()
+            vals.append((v1 * w1 + v2 * w2) / (w1 + w2))
+            wts.append(w1 + w2)
+    return np.repeat(vals, wts)
+
+
+def _row_qp_exact(x0, wd, lo, hi):
+    """min sum (x - x0)^2 s.t. x[0] >= lo, x[i+1] - x[i] >= wd[i], x[-1] + wd[-1] <= hi.
+    With y = x - (prefix widths) the constraints read lo <= y[0] <= ... <= y[-1] <= hi - sum(wd):
+    bounded isotonic regression, whose solution is the clipped isotonic fit."""
+    pre = np.concatenate([[0], np.cumsum(wd)[:-1]])
+    return np.clip(_pava(x0 - pre), lo, hi - wd.sum()) + pre
+
+
+def _qp_rows(seed, nrows, sw=100):
+    rng = np.random.default_rng(seed)
+    ncell, x0, wd, rlx, rux = [], [], [], [], []
+    for r in range(nrows):
+        n = int(rng.integers(1, 21))
+        w = rng.integers(1, 6, n) * sw
+        lo = int(rng.integers(0, 50)) * sw
+        slack = 0 if r == 0 else int(w.sum() * rng.uniform(0.0, 0.6)) // sw * sw  # row 0 exactly full
+        hi = lo + int(w.sum()) + slack
+        x = np.sort(rng.uniform(lo - 5 * sw, hi, n)).round() + rng.normal(0, 3 * sw, n).round()
+        ncell.append(n), x0.append(x), wd.append(w), rlx.append(lo), rux.append(hi)
+    return ncell, x0, wd, rlx, rux
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_qp_refine_converges_to_bounded_isotonic_solution(seed):
+    """The post-process QP (CudaQpSolver.cu:22-353) run to convergence equals the
+    exact solution of each cut row's ordered least-squares problem."""
+    ncell, x0, wd, rlx, rux = _qp_rows(seed, 6)
+    x, it = ref.qp_admm(ncell, np.concatenate(x0), np.concatenate(wd), rlx, rux, rho=2.0, min_iter=3000,
+                        max_iter=3000)
+    assert it == 3000
+    off = np.concatenate([[0], np.cumsum(ncell)])
+    for r in range(len(ncell)):
+        xs = _row_qp_exact(x0[r], wd[r], rlx[r], rux[r])
+        np.testing.assert_allclose(x[off[r]:off[r + 1]], xs, rtol=0, atol=1e-6)
+
+
+@pytest.mark.parametrize("seed", [3, 4])
+def test_qp_refine_default_stop_is_near_optimal_and_nearly_feasible(seed):
+    """With the solver's default stop (no constraint violated by more than 1 DBU
+    after min_iter = 5), the objective is within a few percent of the optimum."""
+    ncell, x0, wd, rlx, rux = _qp_rows(seed, 6)
+    x, it = ref.qp_admm(ncell, np.concatenate(x0), np.concatenate(wd), rlx, rux)
+    assert 5 <= it < 10000
+    off = np.concatenate([[0], np.cumsum(ncell)])
+    for r in range(len(ncell)):
+        xr = x[off[r]:off[r + 1]]
+        assert xr[0] >= rlx[r] - 1 and xr[-1] + wd[r][-1] <= rux[r] + 1
+        assert np.all(np.diff(xr) >= wd[r][:-1] - 1)
+        xs = _row_qp_exact(x0[r], wd[r], rlx[r], rux[r])
+        opt = ((xs - x0[r]) ** 2).sum()
+        assert abs(((xr - x0[r]) ** 2).sum() - opt) <= 0.05 * opt + 1e3
+
+
+def _site_align_brute(x, x0, wd, rlx, rux, sw, core_lx):
+    """Exhaustive search over every cell's two candidate sites (the site at or
+    left of its clamped QP position, and the next one): minimum summed
+    |site - x0| over placements in order, without overlap, the first cell not
+    left of the row and a right-site choice inside it."""
+    n = len(x)
+    base = [((max(rlx, int(v)) - core_lx) // sw) * sw + core_lx for v in x]
+    best = np.inf
+    for mask in range(1 << n):
+        s = [base[i] + sw * ((mask >> i) & 1) for i in range(n)]
+        if s[0] < rlx:
+            continue
+        if any(((mask >> i) & 1) and s[i] + wd[i] > rux for i in range(n)):
+            continue
+        if any(s[i] < s[i - 1] + wd[i - 1] for i in range(1, n)):
+            continue
+        best = min(best, sum(abs(s[i] - int(x0[i])) for i in range(n)))
+    return best
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_site_align_dp_matches_exhaustive_search(seed):
+    """siteAlignDPKernel (QpSolverKernels.hpp:300-475): the two-state DP reaches
+    the exhaustive minimum over all 2^n left/right choices, and its sites do not
+    overlap."""
+    rng = np.random.default_rng(100 + seed)
+    sw, core_lx = 50, 200
+    n = int(rng.integers(2, 13))
+    wd = rng.integers(1, 5, n) * sw
+    rlx = core_lx + int(rng.integers(0, 10)) * sw
+    rux = rlx + int(wd.sum()) + int(rng.integers(0, 6)) * sw
+    if seed % 2:  # positions from the QP itself
+        x0 = np.sort(rng.uniform(rlx - 3 * sw, rux, n)).round()
+        x, _ = ref.qp_admm([n], x0, wd, [rlx], [rux], min_iter=500, max_iter=500)
+    else:  # off-site positions with room to spare: both choices matter
+        gaps = rng.uniform(0, (rux - rlx - wd.sum()) / n, n)
+        x = rlx + np.cumsum(gaps) + np.concatenate([[0], np.cumsum(wd)[:-1]])
+        x0 = (x + rng.normal(0, sw, n)).round()
+    site = ref.site_align_dp_row(x, x0, wd, rlx, rux, sw, core_lx)
+    s = site * sw + core_lx
+    best = _site_align_brute(x, x0, wd, rlx, rux, sw, core_lx)
+    assert np.isfinite(best)
+    assert np.abs(s - x0.astype(np.int64)).sum() == best
+    assert s[0] >= rlx and np.all(s[1:] >= s[:-1] + wd[:-1])
