@@ -179,6 +179,18 @@ def test_adaptive_search(gpu):
     assert g["stats"]["num_adaptive_calls"] == o["stats"]["num_adaptive_calls"]
 
 
+def test_iccad17_weighting(gpu):
+    """Displacement weighted by N / (cells of the same height) on a mixed-height
+    design, with the QP refine on."""
+    d = synth.mixed(3000, seed=9, util=0.75, num_fences=1)
+    p = lg.default_params()
+    p.flag_iccad17 = 1
+    p.qp_admm_on = 1
+    g, o = _run_both(d, p)
+    assert o["stats"]["converged"] == 1
+    _assert_same(g, o)
+
+
 def test_graphs_match_eager(gpu):
     """lg_solve replaying the captured graph of eight iterations (the
     default) and launching every kernel itself give the same state bit for

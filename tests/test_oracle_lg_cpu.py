@@ -306,3 +306,35 @@ def test_site_align_dp_matches_exhaustive_search(seed):
     assert np.isfinite(best)
     assert np.abs(s - x0.astype(np.int64)).sum() == best
     assert s[0] >= rlx and np.all(s[1:] >= s[:-1] + wd[:-1])
+
+
+def test_iccad17_weighting_is_identity_on_single_height_designs():
+    """computeDispCost (AdmmKernels.hpp:508-530) with the ICCAD17 flag scales a
+    cell's displacement by N / (number of cells of its height): 1.0 exactly when
+    every cell has height 1, so the whole solve must be bit-identical."""
+    d = synth.make_design(1500, 200, 40, width_range=(1, 8), num_macros=2, hotspots=2,
+                          hotspot_frac=0.1, seed=3, name="iccad_single")
+    grid, cells, p0 = lg.device_inputs(d)
+    assert np.all(np.asarray(cells["h_grid"]) == 1)
+    o0 = ref.solve(grid, cells, p0, do_post=True)
+    p1 = type(p0).from_buffer_copy(p0)
+    p1.flag_iccad17 = 1
+    o1 = ref.solve(grid, cells, p1, do_post=True)
+    for k in ("lx", "ly", "rot", "usage"):
+        np.testing.assert_array_equal(o0[k], o1[k])
+    assert o0["stats"]["iterations"] == o1["stats"]["iterations"]
+
+
+def test_iccad17_weighting_changes_mixed_height_results_and_stays_legal():
+    """With several heights the weights differ from 1 (N / count(h) > 1 for every
+    height), so the trajectory changes; the result is still legal."""
+    d = synth.mixed(1500, seed=4, util=0.75, num_fences=0)
+    grid, cells, p0 = lg.device_inputs(d)
+    assert len(np.unique(cells["h_grid"])) > 1
+    p1 = type(p0).from_buffer_copy(p0)
+    p1.flag_iccad17 = 1
+    o0 = ref.solve(grid, cells, p0, do_post=True)
+    o1 = ref.solve(grid, cells, p1, do_post=True)
+    assert o1["stats"]["converged"] == 1
+    assert not (np.array_equal(o0["lx"], o1["lx"]) and np.array_equal(o0["ly"], o1["ly"]))
+    assert _legal(d, grid, cells, o1["lx"], o1["ly"], o1["rot"]) <= 1
