@@ -338,3 +338,30 @@ def test_iccad17_weighting_changes_mixed_height_results_and_stays_legal():
     assert o1["stats"]["converged"] == 1
     assert not (np.array_equal(o0["lx"], o1["lx"]) and np.array_equal(o0["ly"], o1["ly"]))
     assert _legal(d, grid, cells, o1["lx"], o1["ly"], o1["rot"]) <= 1
+
+
+def test_adaptive_search_only_acts_when_triggered():
+    """detectSlowConvergence (CudaAdmmLegalizer.cu:799-840) gates the adaptive
+    search: enabled but never triggered, the solve is bit-identical to the
+    search disabled; triggered, it runs and the result stays legal."""
+    d = synth.mixed(1500, seed=5, util=0.8, num_fences=0)
+    grid, cells, p0 = lg.device_inputs(d)
+    p0.max_admm_iter = 300
+    o0 = ref.solve(grid, cells, p0, do_post=False)
+    p1 = type(p0).from_buffer_copy(p0)
+    p1.use_adaptive_search = 1
+    p1.min_iter_to_call_adaptive_search = 10**6
+    o1 = ref.solve(grid, cells, p1, do_post=False)
+    assert o1["stats"]["num_adaptive_calls"] == 0
+    for k in ("lx", "ly", "usage"):
+        np.testing.assert_array_equal(o0[k], o1[k])
+    np.testing.assert_array_equal(o0["dual"].view(np.int32), o1["dual"].view(np.int32))
+    p2 = type(p0).from_buffer_copy(p1)
+    p2.max_admm_iter = 5000
+    p2.min_iter_to_call_adaptive_search = 20
+    p2.cooltime_to_call_adaptive_search = 50
+    p2.ovf_ratio_to_call_adaptive_search = 0.0  # every cool-down period
+    o2 = ref.solve(grid, cells, p2, do_post=True)
+    assert o2["stats"]["num_adaptive_calls"] > 0
+    assert o2["stats"]["converged"] == 1
+    assert _legal(d, grid, cells, o2["lx"], o2["ly"], o2["rot"]) <= 1
