@@ -293,8 +293,14 @@ __device__ __forceinline__ void bin_range(const Geo& G, float cx, float cy, floa
   y1 = min((int)ceilf((uy - G.die_ly) / G.bin_h), G.nby);
 }
 
-__device__ __forceinline__ float overlap(float a0, float a1, float b0, float b1) {
-  return fmaxf(0.f, fminf(a1, b1) - fmaxf(a0, b0));
+// Overlap of the cell span [c - hw, c + hw] with the bin [b0, b1], measured
+// from the bin's lower edge: c - b0 and b1 - b0 are exact (Sterbenz) for a
+// cell near the bin, so the result keeps FP32 precision relative to the bin
+// width instead of the absolute coordinate (on a 2500-unit die, the edges
+// c -/+ hw alone would carry rounding errors of 1e-4 units per side).
+__device__ __forceinline__ float overlap(float c, float hw, float b0, float b1) {
+  const float d = c - b0;
+  return fmaxf(0.f, fminf(d + hw, b1 - b0) - fmaxf(d - hw, 0.f));
 }
 
 __global__ void k_dens_reset(int* cnt, int n, double* ovf) {
@@ -404,13 +410,12 @@ __global__ __launch_bounds__(256) void k_dens_accum(Geo G, const int* off, const
     float2 s = rec_s[p];
     int x0, x1, y0, y1;
     bin_range(G, c.x, c.y, c.z, c.w, x0, x1, y0, y1);
-    float lx = c.x - c.z / 2, ux = c.x + c.z / 2;
-    float ly = c.y - c.w / 2, uy = c.y + c.w / 2;
+    const float hw = c.z / 2, hh = c.w / 2;
     float* dst = sm + (s.y != 0.f ? S2 : 0);
     for (int bx = x0; bx < x1; bx++) {
-      float ox = overlap(lx, ux, G.bin_lx[bx], G.bin_ux[bx]);
+      float ox = overlap(c.x, hw, G.bin_lx[bx], G.bin_ux[bx]);
       for (int by = y0; by < y1; by++) {
-        float oy = overlap(ly, uy, G.bin_ly[by], G.bin_uy[by]);
+        float oy = overlap(c.y, hh, G.bin_ly[by], G.bin_uy[by]);
         atomicAdd(&dst[(by - ty0) * SW + (bx - tx0)], ox * oy * s.x);
       }
     }
@@ -430,12 +435,11 @@ __global__ __launch_bounds__(256) void k_dens_large(Geo G, const int* off, const
     float2 s = rec_s[p];
     int x0, x1, y0, y1;
     bin_range(G, c.x, c.y, c.z, c.w, x0, x1, y0, y1);
-    float lx = c.x - c.z / 2, ux = c.x + c.z / 2;
-    float ly = c.y - c.w / 2, uy = c.y + c.w / 2;
+    const float hw = c.z / 2, hh = c.w / 2;
     const int nx = x1 - x0, cells = nx * (y1 - y0);
     for (int q = threadIdx.x; q < cells; q += blockDim.x) {
       int bx = x0 + q % nx, by = y0 + q / nx;
-      float v = overlap(lx, ux, G.bin_lx[bx], G.bin_ux[bx]) * overlap(ly, uy, G.bin_ly[by], G.bin_uy[by]) * s.x;
+      float v = overlap(c.x, hw, G.bin_lx[bx], G.bin_ux[bx]) * overlap(c.y, hh, G.bin_ly[by], G.bin_uy[by]) * s.x;
       int t = bx / kTile + (by / kTile) * G.tiles_x;
       float* dst = slab + (size_t)t * 2 * S2 + (s.y != 0.f ? S2 : 0);
       atomicAdd(&dst[(by % kTile) * SW + (bx % kTile)], v);
@@ -499,13 +503,12 @@ __global__ void k_dens_gather(Geo G, const int* off, const float4* rec_box, cons
   float x = r.x, y = r.y;
   int x0, x1, y0, y1;
   bin_range(G, x, y, c.x, c.y, x0, x1, y0, y1);
-  float lx = x - c.x / 2, ux = x + c.x / 2;
-  float ly = y - c.y / 2, uy = y + c.y / 2;
+  const float hw = c.x / 2, hh = c.y / 2;
   float fx = 0.f, fy = 0.f, pc = 0.f;
   for (int bx = x0; bx < x1; bx++) {
-    float ox = overlap(lx, ux, G.bin_lx[bx], G.bin_ux[bx]);
+    float ox = overlap(x, hw, G.bin_lx[bx], G.bin_ux[bx]);
     for (int by = y0; by < y1; by++) {
-      float a = ox * overlap(ly, uy, G.bin_ly[by], G.bin_uy[by]) * c.w;
+      float a = ox * overlap(y, hh, G.bin_ly[by], G.bin_uy[by]) * c.w;
       int b = bx + by * G.nbx;
       float la = lambda[b] * a;
       fx += la * ex[b] * 0.5f;

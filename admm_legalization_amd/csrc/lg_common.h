@@ -36,6 +36,10 @@ struct Ctl {
   unsigned long long disp_max;
   unsigned long long check_ovf;
   unsigned long long row_vio;
+  unsigned long long site_vio;    // strict legality counts of lg_check (cells)
+  unsigned long long fence_vio;
+  unsigned long long core_vio;
+  unsigned long long rail_vio;
   unsigned long long phase_ticks[16];  // per-phase wall-clock ticks (LG_PHASE_TIMERS builds)
 };
 

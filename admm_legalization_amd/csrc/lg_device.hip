@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <numeric>
@@ -818,14 +819,19 @@ int lg_check(LgHandle* h, LgVioStat* out) {
   const int nb = (a.N + 255) / 256;
   // CudaDatabase::checkPlace (CudaDatabase.cu:326-353)
   HIP_TRY(hipMemsetAsync(a.usage, 0, h->P * sizeof(int32_t), s));
-  HIP_TRY(hipMemsetAsync(&a.ctl->disp_sum, 0, 4 * sizeof(unsigned long long), s));
+  // disp_sum, disp_max, check_ovf, row_vio and the four strict counts
+  static_assert(offsetof(Ctl, rail_vio) - offsetof(Ctl, disp_sum) == 7 * sizeof(unsigned long long),
+                "check counters must be contiguous");
+  HIP_TRY(hipMemsetAsync(&a.ctl->disp_sum, 0, 8 * sizeof(unsigned long long), s));
   if (a.N > 0) hipLaunchKernelGGL(k_usage_scatter, dim3(nb), dim3(256), 0, s, a);
   hipLaunchKernelGGL(k_overflow, dim3(2048), dim3(256), 0, s, a);
   if (a.N > 0) hipLaunchKernelGGL(k_cell_stats, dim3(nb), dim3(256), 0, s, a);
   HIP_TRY(hipGetLastError());
   int st = read_ctl(h, s);
   if (st) return st;
-  h->vio = LgVioStat{(int32_t)h->host_ctl.check_ovf, (int32_t)h->host_ctl.row_vio, 0, 0, 0};
+  const Ctl& k = h->host_ctl;
+  h->vio = LgVioStat{(int32_t)k.check_ovf, (int32_t)k.row_vio, 0, 0, 0, (int32_t)k.site_vio,
+                     (int32_t)k.fence_vio, (int32_t)k.core_vio, (int32_t)k.rail_vio};
   h->checked = 1;
   if (out) *out = h->vio;
   return LG_OK;

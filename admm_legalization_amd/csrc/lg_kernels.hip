@@ -2466,20 +2466,38 @@ __global__ void k_overflow(Args a) {
   if ((threadIdx.x & 63) == 0 && s) atomicAdd(&a.ctl->check_ovf, s);
 }
 
-// countRowOrientVioKernel (DatabaseKernels.hpp:468-504) + displacement sums
+// countRowOrientVioKernel (DatabaseKernels.hpp:468-504) + displacement sums,
+// and the strict legality counts: per cell, whether its footprint leaves the
+// grid, covers a pixel that is no valid site, covers a site of another fence
+// group, and (even heights) whether its bottom row has the wrong rail.
 __global__ void k_cell_stats(Args a) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long vio = 0, disp = 0, dmax = 0;
+  int site = 0, fence = 0, core = 0, rail = 0;
   if (c < a.N) {
-    int vdd = a.vdd_up[a.ly[c]];
-    int hh = a.h[c], gnd = a.gnd[c], rot = a.need_rot[c];
+    const int lx = a.lx[c], ly = a.ly[c], w = a.w[c], hh = a.h[c];
+    core = (lx < 0 || ly < 0 || lx + w > a.x_size || ly + hh > a.y_size) ? 1 : 0;
+    int vdd = (ly >= 0 && ly < a.y_size) ? a.vdd_up[ly] : -1;
+    int gnd = a.gnd[c], rot = a.need_rot[c];
     int v = 0;
     if ((hh & 1) == 1) {
       if (rot == 0 && gnd != vdd) v = 1;
       else if (rot == 1 && gnd == vdd) v = 1;
-    } else if (gnd != vdd) v = 1;
+    } else if (gnd != vdd) {
+      v = 1;
+      rail = 1;
+    }
     vio = v;
-    int dx = a.lx[c] * a.site_width + a.core_lx, dy = a.ly[c] * a.row_height + a.core_ly;
+    const int code = a.code[c];
+    const int x0 = max(lx, 0), x1 = min(lx + w, a.x_size);
+    const int y0 = max(ly, 0), y1 = min(ly + hh, a.y_size);
+    for (int y = y0; y < y1; y++)
+      for (int x = x0; x < x1; x++) {
+        const int pc = a.pcode[pidx(a, x, y)];
+        if (pc == 0) site = 1;
+        else if (pc != code) fence = 1;
+      }
+    int dx = lx * a.site_width + a.core_lx, dy = ly * a.row_height + a.core_ly;
     disp = (unsigned long long)(abs(dx - a.ox[c]) + abs(dy - a.oy[c]));
     dmax = disp;
   }
@@ -2488,11 +2506,19 @@ __global__ void k_cell_stats(Args a) {
     disp += __shfl_down(disp, o);
     unsigned long long m = __shfl_down(dmax, o);
     dmax = m > dmax ? m : dmax;
+    site += __shfl_down(site, o);
+    fence += __shfl_down(fence, o);
+    core += __shfl_down(core, o);
+    rail += __shfl_down(rail, o);
   }
   if ((threadIdx.x & 63) == 0) {
     if (vio) atomicAdd(&a.ctl->row_vio, vio);
     if (disp) atomicAdd(&a.ctl->disp_sum, disp);
     atomicMax(&a.ctl->disp_max, dmax);
+    if (site) atomicAdd(&a.ctl->site_vio, (unsigned long long)site);
+    if (fence) atomicAdd(&a.ctl->fence_vio, (unsigned long long)fence);
+    if (core) atomicAdd(&a.ctl->core_vio, (unsigned long long)core);
+    if (rail) atomicAdd(&a.ctl->rail_vio, (unsigned long long)rail);
   }
 }
 

@@ -182,7 +182,18 @@ class LgDesign(C.Structure):
 class LgVioStat(C.Structure):
     _fields_ = [("total_ovf", C.c_int32), ("num_row_orient_vio", C.c_int32),
                 ("num_pin_short_vio", C.c_int32), ("num_pin_access_vio", C.c_int32),
-                ("num_edge_spacing_vio", C.c_int32)]
+                ("num_edge_spacing_vio", C.c_int32), ("num_site_vio", C.c_int32),
+                ("num_fence_vio", C.c_int32), ("num_core_vio", C.c_int32), ("num_rail_vio", C.c_int32)]
+
+
+STRICT_VIO = ("total_ovf", "num_row_orient_vio", "num_site_vio", "num_fence_vio", "num_core_vio",
+              "num_rail_vio")
+
+
+def strictly_legal(vio: dict) -> bool:
+    """No overlap, no orientation or rail error, every footprint on valid
+    sites of the cell's own fence group and inside the core."""
+    return all(int(vio[k]) == 0 for k in STRICT_VIO)
 
 
 class LgStats(C.Structure):

@@ -217,6 +217,22 @@ def mixed_1m(seed: int = 0) -> Design:
                        fence_util=0.85, seed=seed, name="mixed_1m")
 
 
+def stream_spec(num: int = 2000, seed: int = 0, total: int = 2000):
+    """(cells, utilisation, design seed) of the first `num` requests of the
+    seeded `total`-request legalization stream: cell counts log-uniform on
+    [500, 200k], utilisation uniform on [0.6, 0.9]."""
+    rng = np.random.default_rng(seed)
+    n = np.exp(rng.uniform(np.log(500), np.log(200_000), total)).astype(np.int64)
+    u = rng.uniform(0.6, 0.9, total)
+    return [(int(n[i]), float(u[i]), 1000 + i) for i in range(min(num, total))]
+
+
+def stream_request(n: int, util: float, seed: int) -> Design:
+    """One request of the stream: an ECO window (below 20k cells, no fence)
+    or a sub-design (3 fences) of the mixed-height recipe."""
+    return mixed(n, util=util, seed=seed, num_fences=3 if n >= 20000 else 0, name=f"req{seed}")
+
+
 def _calibrate_width(n, free_sites, util):
     mean_h = sum(h * p for h, p in MIXED_HEIGHTS.items())
     target_w = util * free_sites / n / mean_h

@@ -110,6 +110,8 @@ def main():
     ap.add_argument("--load", type=float, default=0.7,
                     help="served_stream: Poisson arrival rate as a fraction of the measured capacity")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--dump", default=None, help="served_stream: write per-request sizes, plain service "
+                                                  "times and iterations to this JSON file")
     ap.add_argument("--graphs", action="store_true",
                     help="replay the ADMM iterations as HIP graphs instead of launching every kernel")
     ap.add_argument("--cpu-sample-cells", type=int, default=10000)
@@ -187,7 +189,8 @@ def main():
 
     vio = dev.check()
     st = dev.stats()
-    if not st["converged"] or vio["total_ovf"] != 0 or vio["num_row_orient_vio"] != 0:
+    from admm_legalization_amd.design import strictly_legal
+    if not st["converged"] or not strictly_legal(vio):
         print(f"# legalization not legal: {st} {vio}", file=sys.stderr)
         sys.exit(1)
     t_copy = time.perf_counter()
@@ -275,20 +278,9 @@ def main():
         out["cpu_baseline"] = cpu_baseline(args.cpu_sample_cells, args.cpu_threads)
     print(json.dumps(out), flush=True)
 
-def stream_spec(num: int, seed: int = 0, total: int = 2000):
-    """(cells, utilisation, design seed) of the first `num` requests of the
-    seeded `total`-request stream: cell counts log-uniform on [500, 200k],
-    utilisation uniform on [0.6, 0.9]."""
-    rng = np.random.default_rng(seed)
-    n = np.exp(rng.uniform(np.log(500), np.log(200_000), total)).astype(np.int64)
-    u = rng.uniform(0.6, 0.9, total)
-    return [(int(n[i]), float(u[i]), 1000 + i) for i in range(min(num, total))]
-
-
 def make_request(n, util, seed):
     from admm_legalization_amd import lg, synth
-    d = synth.mixed(n, util=util, seed=seed, num_fences=3 if n >= 20000 else 0, name=f"req{seed}")
-    return lg.device_inputs(d)
+    return lg.device_inputs(synth.stream_request(n, util, seed))
 
 
 def percentiles(xs):
@@ -339,9 +331,10 @@ def served_cpu_baseline(spec, threads, budget_cells=12000):
 def bench_served(args):
     """Served (continuous batching) vs plain (one request at a time)."""
     import torch
-    from admm_legalization_amd import lg
+    from admm_legalization_amd import lg, synth
+    from admm_legalization_amd.design import strictly_legal
     from admm_legalization_amd.server import LegalizationServer
-    spec = stream_spec(args.requests)
+    spec = synth.stream_spec(args.requests)
     t = time.perf_counter()
     reqs = [make_request(*r) for r in spec]
     t_prep = time.perf_counter() - t
@@ -373,6 +366,9 @@ def bench_served(args):
         if (i + 1) % 25 == 0:
             print(f"# plain {i + 1}/{len(reqs)} {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
     t_plain = time.perf_counter() - t0 if not args.skip_plain else float("nan")
+    if args.dump and not args.skip_plain:
+        json.dump({"cells": cells.tolist(), "service_s": service,
+                   "iterations": [int(st["iterations"]) for _, _, _, st in plain]}, open(args.dump, "w"))
     plain_rate = total / t_plain
     print(f"# plain: {t_plain:.2f}s {plain_rate:.0f} cells/s", file=sys.stderr, flush=True)
 
@@ -419,13 +415,12 @@ def bench_served(args):
     legal = True
     if args.skip_plain:
         same = len(reqs)
-        legal = all(r["stats"]["converged"] == 1 and r["stats"]["vio"]["total_ovf"] == 0 for r in res)
+        legal = all(r["stats"]["converged"] == 1 and strictly_legal(r["stats"]["vio"]) for r in res)
     for (lx, ly, rot, st), r in zip(plain, res):
         same += int(np.array_equal(lx, r["lx"]) and np.array_equal(ly, r["ly"]) and np.array_equal(rot, r["rot"]))
         if st["total_disp_dbu"]:
             disp_gap = max(disp_gap, abs(r["stats"]["total_disp_dbu"] - st["total_disp_dbu"]) / st["total_disp_dbu"])
-        legal &= r["stats"]["converged"] == 1 and r["stats"]["vio"]["total_ovf"] == 0 and \
-            r["stats"]["vio"]["num_row_orient_vio"] == 0
+        legal &= r["stats"]["converged"] == 1 and strictly_legal(r["stats"]["vio"])
     if same != len(reqs) or not legal:
         print(f"# served results differ from plain ({same}/{len(reqs)} identical) or not legal", file=sys.stderr)
         sys.exit(1)
@@ -515,7 +510,7 @@ def bench_served(args):
     }
     if not args.no_cpu_baseline:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = served_cpu_baseline(stream_spec(2000), args.cpu_threads)
+        out["cpu_baseline"] = served_cpu_baseline(synth.stream_spec(2000), args.cpu_threads)
     print(json.dumps(out), flush=True)
 
 

@@ -124,13 +124,25 @@ typedef struct LgDesign {
   const int32_t* box_uy;
 } LgDesign;
 
-/* Violation counts of CudaDatabase::checkPlace (objects/LGVioStat.h). */
+/* Violation counts of CudaDatabase::checkPlace (objects/LGVioStat.h), then
+ * the strict legality counts, which have no counterpart in the reference's
+ * checkPlace: they make the placement filters of computePenaltySum
+ * (AdmmKernels.hpp:355-373) checkable after the fact.  Each strict count is
+ * a number of movable cells. */
 typedef struct LgVioStat {
-  int32_t total_ovf;
-  int32_t num_row_orient_vio;
+  int32_t total_ovf;            /* sum_p max(usage-1, 0): overlap             */
+  int32_t num_row_orient_vio;   /* countRowOrientVioKernel                    */
   int32_t num_pin_short_vio;
   int32_t num_pin_access_vio;
   int32_t num_edge_spacing_vio;
+  int32_t num_site_vio;         /* footprint covers a pixel that is no valid
+                                   site (blockage, macro, outside every row) */
+  int32_t num_fence_vio;        /* footprint covers a valid site of another
+                                   fence group (or of a fence, for a cell of
+                                   the default region)                        */
+  int32_t num_core_vio;         /* footprint leaves the core (the pixel grid) */
+  int32_t num_rail_vio;         /* even-height cell whose bottom row has the
+                                   wrong power rail (not flippable)           */
 } LgVioStat;
 
 typedef struct LgStats {

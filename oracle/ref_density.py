@@ -136,9 +136,18 @@ def fixed_area(geom, fixed_rects, target_density: float = 1.0):
 # ---------------------------------------------------------------------------
 def _bin_range(c, d, origin, size, nb):
     """findBinWithDensitySize (DensityGradient.cu:20-59): [floor, ceil) of the
-    footprint's bin coordinates, clipped to the grid."""
-    lo = np.floor((c - d / 2 - origin) / size).astype(np.int64)
-    hi = np.ceil((c + d / 2 - origin) / size).astype(np.int64)
+    footprint's bin coordinates, clipped to the grid.  The range is a discrete
+    choice of which bins a cell visits, and the reference makes it in FP32
+    (float positions, sizes and bin size); it is restated in FP32 here.  The
+    bin edges themselves are accumulated in FP32 (createBins,
+    SkyPlaceDB.cpp:504-546) and drift from i * size by up to ~1e-2 units on
+    a 1024-bin side, so an FP64 range would visit a different set of bins
+    for cells whose edge lies within that drift of a bin boundary."""
+    f = np.float32
+    c32 = np.asarray(c).astype(f)
+    h32 = np.asarray(d).astype(f) / f(2)
+    lo = np.floor(((c32 - h32) - f(origin)) / f(size)).astype(np.int64)
+    hi = np.ceil(((c32 + h32) - f(origin)) / f(size)).astype(np.int64)
     return np.maximum(lo, 0), np.minimum(hi, nb)
 
 

@@ -575,7 +575,10 @@ static void update_usage(Dev& D) {
 
 static int64_t compute_overflow(const Dev& D) {  // CudaDatabase.cu:562-572
   int64_t s = 0;
-  for (int u : D.usage) s += std::max(u - 1, 0);
+  const int64_t P = (int64_t)D.usage.size();
+  const int threads = std::max(1, D.p.num_cpu_threads);
+#pragma omp parallel for num_threads(threads) schedule(static) reduction(+ : s)
+  for (int64_t p = 0; p < P; p++) s += std::max(D.usage[p] - 1, 0);
   return s;
 }
 
@@ -654,6 +657,8 @@ static void initialize_dual(Dev& D) {
 static void update_partition(Dev& D, int iter) {
   int tf = (iter / D.p.iter_update_partition) % 3;
   int ox = D.p.x_hint / 3 * tf, oy = D.p.y_hint / 3 * tf;
+  const int threads = std::max(1, D.p.num_cpu_threads);
+#pragma omp parallel for num_threads(threads) schedule(static)
   for (int x = 0; x < D.x_size; x++)
     for (int y = 0; y < D.y_size; y++) {
       int col = ((x + ox) / D.p.x_hint) % D.ncol;
@@ -729,6 +734,8 @@ static void execute_sorting(Dev& D) {
   int cr = 0, co = 10, ca = 1;
   if (D.p.sorting_policy == LG_SORT_POLICY_RANDOM) { cr = 1; co = 0; ca = 0; }
   D.key.resize(D.N);
+  const int threads = std::max(1, D.p.num_cpu_threads);
+#pragma omp parallel for num_threads(threads) schedule(static)
   for (int t = 0; t < D.N; t++) {
     int c = D.grouped[t];
     int ovf = cell_overflow(D, D.lx[c], D.ly[c], D.w[c], D.h[c]);
@@ -736,6 +743,7 @@ static void execute_sorting(Dev& D) {
   }
   int n = pow2_ceil(D.max_in_part);
   int algo = D.p.sorting_algorithm;
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 8)
   for (int k = 0; k < D.K; k++) {
     int off = D.offset[k], m = D.offset[k + 1] - off;
     if (algo == LG_SORT_CUB) {  // segmented radix sort: stable, descending
@@ -898,7 +906,10 @@ static void final_refine_partition(Dev& D, int k, std::vector<float>& buf) {
 static void dual_update(Dev& D) {
   const float rho = D.rho, step = D.dual_step;
   const float neg = -1.0f / rho;
-  for (size_t p = 0; p < D.dual.size(); p++) {
+  const int64_t P = (int64_t)D.dual.size();
+  const int threads = std::max(1, D.p.num_cpu_threads);
+#pragma omp parallel for num_threads(threads) schedule(static)
+  for (int64_t p = 0; p < P; p++) {
     float u = float(D.usage[p]);
     float g = std::max(u - 1.0f, neg);
     double q = (double)rho * 0.5;
@@ -1310,13 +1321,30 @@ static LgVioStat check_place(Dev& D) {
   LgVioStat v{};
   v.total_ovf = (int)compute_overflow(D);
   for (int c = 0; c < D.N; c++) {
-    int vdd = D.vdd_up[D.ly[c]];
+    int vdd = (D.ly[c] >= 0 && D.ly[c] < D.y_size) ? D.vdd_up[D.ly[c]] : -1;
     int viol = 0;
     if ((D.h[c] & 1) == 1) {
       if (D.need_rot[c] == 0 && D.gnd[c] != vdd) viol = 1;
       else if (D.need_rot[c] == 1 && D.gnd[c] == vdd) viol = 1;
     } else if (D.gnd[c] != vdd) viol = 1;
     v.num_row_orient_vio += viol;
+  }
+  // strict counts: every footprint pixel inside the grid, a valid site, of
+  // the cell's own group; even heights on their own rail
+  for (int c = 0; c < D.N; c++) {
+    bool site = false, fence = false, core = false;
+    for (int x = D.lx[c]; x < D.lx[c] + D.w[c]; x++)
+      for (int y = D.ly[c]; y < D.ly[c] + D.h[c]; y++) {
+        if (x < 0 || y < 0 || x >= D.x_size || y >= D.y_size) { core = true; continue; }
+        const size_t p = D.pid(x, y);
+        if (!D.valid[p]) site = true;
+        else if (D.pgroup[p] != D.cgroup[c]) fence = true;
+      }
+    v.num_site_vio += site;
+    v.num_fence_vio += fence;
+    v.num_core_vio += core;
+    const bool row_in = D.ly[c] >= 0 && D.ly[c] < D.y_size;
+    if (D.h[c] % 2 == 0 && (!row_in || D.gnd[c] != D.vdd_up[D.ly[c]])) v.num_rail_vio++;
   }
   return v;
 }

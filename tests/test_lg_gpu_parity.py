@@ -1,7 +1,7 @@
 """GPU parity of the legalizer device layer against the CPU oracle.
 
 Both sides start from the same grid and preplaced cells (the product's host
-import, itself pinned to the oracle in test_lg_host_parity.py).  Integer
+import, itself pinned to the oracle in test_oracle_lg_cpu.py).  Integer
 state (cell positions, pixel usage) and the FP32 dual prices must match
 bit for bit under the shared floating-point contract.
 """
@@ -234,3 +234,27 @@ def test_repeated_solve_is_idempotent(gpu):
         np.testing.assert_array_equal(ly, o["ly"])
         np.testing.assert_array_equal(rot, o["rot"])
         assert st["iterations"] == o["stats"]["iterations"]
+
+
+def test_strict_check_planted_violations(gpu):
+    """lg_check's strict counts on positions planted on macro pixels, in a
+    foreign fence and on the wrong rail equal the oracle's checkPlace and an
+    independent numpy recount."""
+    from strict_check import numpy_strict_counts, plant_violations
+    d = synth.mixed(3000, seed=4)
+    grid, cells, p = lg.device_inputs(d)
+    c2, planted = plant_violations(grid, cells, seed=1)
+    p.max_admm_iter = 0
+    print(f"[case] strict check, planted {planted}", flush=True)
+    dev = lg.DeviceLegalizer(grid, c2, p)
+    dev.solve()
+    vio = dev.check()
+    dev.close()
+    o = ref.solve(grid, c2, p, do_post=False)
+    assert vio == o["stats"]["vio"]
+    sw, rh = int(grid["site_width"]), int(grid["row_height"])
+    lx = (c2["lx_dbu"] - grid["core_lx"]) // sw
+    ly = (c2["ly_dbu"] - grid["core_ly"]) // rh
+    want = numpy_strict_counts(grid, c2, lx, ly, np.zeros_like(lx))
+    assert {k: vio[k] for k in want} == want
+    assert want["num_site_vio"] >= 2 and want["num_fence_vio"] >= 2 and want["num_rail_vio"] >= 2

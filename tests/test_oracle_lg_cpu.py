@@ -365,3 +365,31 @@ def test_adaptive_search_only_acts_when_triggered():
     assert o2["stats"]["num_adaptive_calls"] > 0
     assert o2["stats"]["converged"] == 1
     assert _legal(d, grid, cells, o2["lx"], o2["ly"], o2["rot"]) <= 1
+
+
+# -- strict legality counts of checkPlace -----------------------------------
+def test_oracle_strict_counts_on_planted_violations():
+    """Cells planted on macro/blockage pixels, in a foreign fence and on the
+    wrong rail are each counted once, as an independent numpy recount counts
+    them; the untouched preplaced cells add only overlap."""
+    from strict_check import numpy_strict_counts, plant_violations
+    d = synth.mixed(3000, seed=4)
+    grid, cells, p = lg.device_inputs(d)
+    c2, planted = plant_violations(grid, cells, seed=1)
+    assert all(len(v) >= 2 for v in planted.values()), planted
+    p.max_admm_iter = 0
+    o = ref.solve(grid, c2, p, do_post=False)
+    sw, rh = int(grid["site_width"]), int(grid["row_height"])
+    lx = (c2["lx_dbu"] - grid["core_lx"]) // sw
+    ly = (c2["ly_dbu"] - grid["core_ly"]) // rh
+    want = numpy_strict_counts(grid, c2, lx, ly, np.zeros_like(lx))
+    got = {k: o["stats"]["vio"][k] for k in want}
+    assert got == want
+    assert want["num_site_vio"] >= len(planted["site"])
+    assert want["num_fence_vio"] >= len(planted["fence"])
+    assert want["num_rail_vio"] >= len(planted["rail"])
+    # the solved design is strictly legal by the same recount
+    o2 = ref.solve(grid, cells, lg.default_params(), do_post=True)
+    want2 = numpy_strict_counts(grid, cells, o2["lx"], o2["ly"], o2["rot"])
+    assert {k: o2["stats"]["vio"][k] for k in want2} == want2
+    assert all(v == 0 for v in want2.values()), want2
