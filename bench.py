@@ -3,28 +3,34 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
                     [--workload served_stream|mixed_1m|plain_20k|density_1m|density_2m_4k|wl_2m]
-                    [--requests R] [--slots S] [--iters-per-sync I] [--load L]
-                    [--no-cpu-baseline] [--cpu-sample-cells N] [--cpu-threads T]
+                    [--requests R] [--plain-stride S] [--slots S] [--iters-per-sync I] [--load L]
+                    [--no-e2e] [--no-cpu-baseline] [--cpu-threads T]
 
-served_stream (default): the first R requests of a seeded 2000-request
-stream of legalizations (ECO windows and sub-designs, 500-200k cells, mixed
-heights, 60-90 % utilisation).  Plain = each request alone through
-lg_create .. lg_copy_out, one after another; served = the continuous-batching
-request server with every request queued at once (capacity, the reported
-value; one step = the whole stream), then again with Poisson arrivals at
---load of the measured capacity for p50 / p99 latency.  Every served result is
-checked to be identical to the plain one.
+served_stream (default): the seeded 2000-request stream of legalizations
+(ECO windows and sub-designs, 500-200k cells, mixed heights, 60-90 %
+utilisation, 66.9M cells in all), prepared on the host first.
+  * served (the reported value): every request queued at once to one
+    continuous-batching request server; cells/s over the whole stream.  A
+    step is one slice of R / K requests in arrival order; all slices are in
+    flight in the same server, so ms_per_step = time / K.
+  * plain: every S-th request (default 4: 500 requests, 16.8M cells) alone
+    through lg_create .. lg_copy_out, one after another; each result must be
+    bit-identical to its served result.
+  * latency: the whole stream again with Poisson arrivals at --load of the
+    served capacity (served p50 / p99), and a FIFO replay of the plain
+    service times at the same fraction of plain capacity.
+  * the 1M-cell mixed-height design end to end through lg_run_design.
+  * cpu_baseline: the oracle on every 400th request of the stream.
+Every served result must be strictly legal, else the bench exits with an
+error.
 
-mixed_1m / plain_20k: one step = one whole legalization of the design on the GPU: lg_solve (usage
-scatter, rho / lambda0 init, ADMM negotiation loop with partitioning, ordering,
-primal and dual updates, final refine) followed by lg_postprocess (rotation),
-starting from the preplaced positions that lg_create left in HBM.  Host import
-and preplace run once before the timed region and are reported beside it.
+mixed_1m / plain_20k: one step = one whole legalization of the design on the
+GPU (lg_solve + lg_postprocess from the preplaced positions lg_create left in
+HBM); host import and preplace run once before the timed region.
 
-Prints one JSON line: cells legalized per second over the K timed steps, the
-roofline of the dominant kernel (the partition kernel, timed per launch with
-HIP events on the solve's stream) and a CPU baseline from the oracle on a
-bounded sample of the same recipe.
+Prints one JSON line: cells legalized per second, the roofline of the
+dominant kernel (the partition kernel, timed with HIP events on its stream)
+and a CPU baseline from the oracle.
 """
 from __future__ import annotations
 
@@ -101,12 +107,14 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="served_stream", choices=sorted(WORKLOADS))
-    ap.add_argument("--requests", type=int, default=200,
+    ap.add_argument("--requests", type=int, default=2000,
                     help="served_stream: the first N requests of the seeded 2000-request stream")
+    ap.add_argument("--plain-stride", type=int, default=4,
+                    help="served_stream: plain leg over every Nth request of the stream")
+    ap.add_argument("--no-e2e", action="store_true",
+                    help="served_stream: skip the 1M-cell end-to-end legalization")
     ap.add_argument("--slots", type=int, default=64, help="served_stream: resident requests")
     ap.add_argument("--iters-per-sync", type=int, default=16, help="served_stream: batch length")
-    ap.add_argument("--skip-plain", action="store_true",
-                    help="served_stream: served leg only (profiling runs; no served/plain comparison)")
     ap.add_argument("--load", type=float, default=0.7,
                     help="served_stream: Poisson arrival rate as a fraction of the measured capacity")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -299,20 +307,32 @@ def fifo_replay(arrivals, service):
     return out
 
 
-def served_cpu_baseline(spec, threads, budget_cells=12000):
-    """The oracle legalizes requests of the stream serially (import,
-    preplace, ADMM, post), in stream order, skipping requests above 20k cells,
-    until `budget_cells` cells are done."""
+def host_cpu():
+    """Model name and logical CPU count of this host."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def served_cpu_baseline(spec, threads, stride=400):
+    """The oracle legalizes a stratified sample of the stream (every
+    `stride`-th request in arrival order: import, preplace, ADMM, post)
+    serially, on `threads` OpenMP threads."""
     from admm_legalization_amd import synth
     from admm_legalization_amd.design import default_params
     from oracle import ref
     p = default_params()
     p.num_cpu_threads = threads
     cells, el, used = 0, 0.0, []
-    for n, u, sd in spec:
-        if n > 20000:
-            continue
-        d = synth.mixed(n, util=u, seed=sd, num_fences=0, name=f"req{sd}")
+    for i in range(0, len(spec), stride):
+        n, u, sd = spec[i]
+        d = synth.stream_request(n, u, sd)
         t = time.perf_counter()
         _, _, _, st = ref.run_design(d, p)
         el += time.perf_counter() - t
@@ -320,29 +340,109 @@ def served_cpu_baseline(spec, threads, budget_cells=12000):
             raise RuntimeError("oracle sample did not converge")
         cells += n
         used.append(n)
-        if cells >= budget_cells:
-            break
+    model, ncpu = host_cpu()
     return {"value": cells / el, "unit": "cells/s", "cores": threads, "kind": "port",
-            "sample": f"oracle (C++ restatement, OpenMP over partitions) legalizing {len(used)} requests "
-                      f"of the stream below 20k cells ({cells} cells: {used}) serially end to end in "
-                      f"{el:.2f} s"}
+            "host_cpu": f"{model}, {ncpu} logical CPUs visible",
+            "sample": f"oracle (C++ restatement, OpenMP over partitions and pixels) legalizing every "
+                      f"{stride}th request of the stream ({len(used)} requests, {cells} cells: {used}) "
+                      f"serially end to end in {el:.2f} s"}
+
+
+def mixed_1m_end_to_end(repeats=3):
+    """The 1M-cell mixed-height design legalized end to end through the
+    drop-in entry point lg_run_design, as the reference's scripts time a
+    legalization (run_ispd24.py:41-48: preplace + ADMM + post-process):
+    import, pixel grid, preplace, upload, solve, post-process, positions to
+    the host.  Median of `repeats` runs after one untimed run."""
+    from admm_legalization_amd import lg, synth
+    from admm_legalization_amd.design import strictly_legal
+    d = synth.mixed_1m(0)
+    lg.run_design(d)  # untimed: first-use costs of the runtime
+    runs = []
+    for _ in range(repeats):
+        t = time.perf_counter()
+        lx, ly, fl, st = lg.run_design(d)
+        runs.append((time.perf_counter() - t, st))
+    if not all(st["converged"] == 1 and strictly_legal(st["vio"]) for _, st in runs):
+        print("# mixed_1m end to end: not legal", file=sys.stderr)
+        sys.exit(1)
+    runs.sort(key=lambda r: r[0])
+    el, st = runs[len(runs) // 2]
+    n = int(st["num_cells"])
+    return {"cells": n, "wall_s": el, "cells_per_s": n / el, "runs_s": [r[0] for r in runs],
+            "iterations": int(st["iterations"]),
+            "parts_s": {"import": st["t_import_s"], "preplace": st["t_preplace_s"],
+                        "upload_admm": st["t_admm_s"], "post": st["t_post_s"], "total": st["t_total_s"]},
+            "under_1s": el < 1.0,
+            "note": "lg_run_design wall time (median of 3 after one untimed run); inputs are the host "
+                    "design (the reference's entry point takes its design from host memory), so the "
+                    "PCIe upload is inside"}
 
 
 def bench_served(args):
-    """Served (continuous batching) vs plain (one request at a time)."""
+    """Served (continuous batching) vs plain (one request at a time) on the
+    2000-request stream."""
     import torch
+    from concurrent.futures import ThreadPoolExecutor
     from admm_legalization_amd import lg, synth
     from admm_legalization_amd.design import strictly_legal
     from admm_legalization_amd.server import LegalizationServer
     spec = synth.stream_spec(args.requests)
+    R = len(spec)
     t = time.perf_counter()
-    reqs = [make_request(*r) for r in spec]
+    with ThreadPoolExecutor(max_workers=8) as ex:
+        reqs = list(ex.map(lambda r: make_request(*r), spec))
     t_prep = time.perf_counter() - t
     cells = np.array([len(c["w_grid"]) for _, c, _ in reqs], np.int64)
     total = int(cells.sum())
-    print(f"# served_stream: {len(reqs)} requests, {total} cells (min {cells.min()}, max {cells.max()}); "
-          f"synth + host import/preplace {t_prep:.1f}s", file=sys.stderr, flush=True)
+    K = max(args.steps, 1)
+    print(f"# served_stream: {R} requests, {total} cells (min {cells.min()}, max {cells.max()}); "
+          f"synth + host import/preplace {t_prep:.1f}s; {K} steps of {R / K:.0f} requests",
+          file=sys.stderr, flush=True)
 
+    def served_run(idx, arrivals=None, prof=False, progress=True):
+        with LegalizationServer(max_slots=args.slots, iters_per_sync=args.iters_per_sync,
+                                use_graphs=args.graphs) as srv:
+            if prof:
+                srv.profile(True)
+            tk = []
+            t0 = time.perf_counter()
+            for j, i in enumerate(idx):
+                if arrivals is not None:
+                    d = t0 + arrivals[j] - time.perf_counter()
+                    if d > 0:
+                        time.sleep(d)
+                g, c, p = reqs[i]
+                tk.append(srv.submit(g, c, p))
+            res = []
+            for j, k in enumerate(tk):
+                res.append(srv.wait(k))
+                if progress and (j + 1) % 250 == 0:
+                    print(f"# served: {j + 1}/{len(tk)} collected at {time.perf_counter() - t0:.1f}s",
+                          file=sys.stderr, flush=True)
+            el = time.perf_counter() - t0
+            return res, el, srv.counters(), (srv.profile_read() if prof else None)
+
+    # -- warmup: W of the smallest requests through a server (untimed)
+    small = [int(i) for i in np.argsort(cells)[:max(args.warmup, 0)]]
+    if small:
+        served_run(small, progress=False)
+    torch.cuda.synchronize()
+
+    # -- timed: the whole stream queued at t = 0 (capacity); step k is the
+    # k-th slice of R / K requests in arrival order, all slices in flight in
+    # one server
+    res, t_served, ctr0, prof0 = served_run(list(range(R)), prof=True)
+    torch.cuda.synchronize()
+    served_rate = total / t_served
+    print(f"# served (saturated): {t_served:.2f}s {served_rate:.0f} cells/s, peak slots "
+          f"{ctr0['peak_slots']}", file=sys.stderr, flush=True)
+    legal = [r["stats"]["converged"] == 1 and strictly_legal(r["stats"]["vio"]) for r in res]
+    if not all(legal):
+        print(f"# served results not legal: {R - sum(legal)} of {R}", file=sys.stderr)
+        sys.exit(1)
+
+    # -- plain: every plain_stride-th request alone (lg_create .. lg_copy_out)
     def plain_one(g, c, p):
         dev = lg.DeviceLegalizer(g, c, p)
         dev.use_graphs(args.graphs)
@@ -353,158 +453,130 @@ def bench_served(args):
         dev.close()
         return lx, ly, rot, st
 
-    # -- plain: one request at a time (lg_create .. lg_copy_out per request)
-    for _ in range(args.warmup if not args.skip_plain else 0):
-        plain_one(*reqs[int(np.argmin(cells))])
-    torch.cuda.synchronize()
+    pidx = list(range(0, R, args.plain_stride))
     plain, service = [], []
     t0 = time.perf_counter()
-    for i, (g, c, p) in enumerate(reqs if not args.skip_plain else []):
+    for j, i in enumerate(pidx):
         ts = time.perf_counter()
-        plain.append(plain_one(g, c, p))
+        plain.append(plain_one(*reqs[i]))
         service.append(time.perf_counter() - ts)
-        if (i + 1) % 25 == 0:
-            print(f"# plain {i + 1}/{len(reqs)} {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
-    t_plain = time.perf_counter() - t0 if not args.skip_plain else float("nan")
-    if args.dump and not args.skip_plain:
-        json.dump({"cells": cells.tolist(), "service_s": service,
+        if (j + 1) % 100 == 0:
+            print(f"# plain {j + 1}/{len(pidx)} {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+    t_plain = time.perf_counter() - t0
+    plain_cells = int(cells[pidx].sum())
+    plain_rate = plain_cells / t_plain
+    print(f"# plain: {len(pidx)} requests {t_plain:.2f}s {plain_rate:.0f} cells/s", file=sys.stderr, flush=True)
+    if args.dump:
+        json.dump({"cells": cells[pidx].tolist(), "service_s": service,
                    "iterations": [int(st["iterations"]) for _, _, _, st in plain]}, open(args.dump, "w"))
-    plain_rate = total / t_plain
-    print(f"# plain: {t_plain:.2f}s {plain_rate:.0f} cells/s", file=sys.stderr, flush=True)
-
-    # -- served, saturated: every request submitted at once (capacity)
-    def served_run(arrivals, prof=False):
-        with LegalizationServer(max_slots=args.slots, iters_per_sync=args.iters_per_sync,
-                                use_graphs=args.graphs) as srv:
-            if prof:
-                srv.profile(True)
-            tk = []
-            t0 = time.perf_counter()
-            for i, (g, c, p) in enumerate(reqs):
-                if arrivals is not None:
-                    d = t0 + arrivals[i] - time.perf_counter()
-                    if d > 0:
-                        time.sleep(d)
-                tk.append(srv.submit(g, c, p))
-            res = []
-            for i, k in enumerate(tk):
-                res.append(srv.wait(k))
-                if (i + 1) % 20 == 0:
-                    print(f"# served: {i + 1}/{len(tk)} collected at {time.perf_counter() - t0:.1f}s",
-                          file=sys.stderr, flush=True)
-            el = time.perf_counter() - t0
-            return res, el, srv.counters(), (srv.profile_read() if prof else None)
-
-    for _ in range(args.warmup):
-        served_run(None)
-    times = []
-    res = None
-    for k in range(max(args.steps, 1)):
-        r, el, ctr, prof = served_run(None, prof=(k == 0))
-        times.append(el)
-        if k == 0:
-            res, ctr0, prof0 = r, ctr, prof
-        ctr_last = ctr
-        print(f"# served (saturated) run {k}: {el:.2f}s {total / el:.0f} cells/s, peak slots "
-              f"{ctr['peak_slots']}", file=sys.stderr, flush=True)
-    t_served = float(np.median(times))
-    served_rate = total / t_served
-    # results: identical to the plain ones (same kernels, per-request state)
-    same = 0
-    disp_gap = 0.0
-    legal = True
-    if args.skip_plain:
-        same = len(reqs)
-        legal = all(r["stats"]["converged"] == 1 and strictly_legal(r["stats"]["vio"]) for r in res)
-    for (lx, ly, rot, st), r in zip(plain, res):
+    same, disp_gap = 0, 0.0
+    for (lx, ly, rot, st), i in zip(plain, pidx):
+        r = res[i]
         same += int(np.array_equal(lx, r["lx"]) and np.array_equal(ly, r["ly"]) and np.array_equal(rot, r["rot"]))
         if st["total_disp_dbu"]:
             disp_gap = max(disp_gap, abs(r["stats"]["total_disp_dbu"] - st["total_disp_dbu"]) / st["total_disp_dbu"])
-        legal &= r["stats"]["converged"] == 1 and strictly_legal(r["stats"]["vio"])
-    if same != len(reqs) or not legal:
-        print(f"# served results differ from plain ({same}/{len(reqs)} identical) or not legal", file=sys.stderr)
+    if same != len(pidx):
+        print(f"# served results differ from plain ({same}/{len(pidx)} identical)", file=sys.stderr)
         sys.exit(1)
 
     # -- latency under Poisson arrivals at `load` of each system's capacity
     rng = np.random.default_rng(7)
-    gaps = rng.exponential(1.0, len(reqs))
-    lam_s = args.load * len(reqs) / t_served
+    gaps = rng.exponential(1.0, R)
+    lam_s = args.load * R / t_served
     arr_s = np.cumsum(gaps) / lam_s
     arr_s -= arr_s[0]
-    if args.skip_plain:
-        lat_served = lat_plain = lat_plain_same = [float("nan")]
-        lam_p = float("nan")
-    else:
-        rp, el_p, ctr_p, _ = served_run(arr_s)
-        lat_served = [r["latency_s"] for r in rp]
-        lam_p = args.load * len(reqs) / t_plain
-        arr_p = np.cumsum(gaps) / lam_p
-        arr_p -= arr_p[0]
-        lat_plain = fifo_replay(arr_p, service)
-        lat_plain_same = fifo_replay(arr_s, service)  # plain under the served run's own arrivals
-        print(f"# poisson: served p99 {np.percentile(lat_served, 99) * 1e3:.1f} ms, plain p99 "
-              f"{np.percentile(lat_plain, 99) * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    rp, el_p, ctr_p, _ = served_run(list(range(R)), arrivals=arr_s)
+    lat_served = [r["latency_s"] for r in rp]
+    lam_p = args.load * len(pidx) / t_plain
+    gp = gaps[:len(pidx)]
+    arr_p = np.cumsum(gp) / lam_p
+    arr_p -= arr_p[0]
+    lat_plain = fifo_replay(arr_p, service)
+    print(f"# poisson: served p99 {np.percentile(lat_served, 99) * 1e3:.1f} ms, plain p99 "
+          f"{np.percentile(lat_plain, 99) * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    by_size = {}
+    for lo, hi in ((0, 5000), (5000, 20000), (20000, 80000), (80000, 10**9)):
+        m = [(cells[i] >= lo and cells[i] < hi) for i in range(R)]
+        xs = [lat_served[i] for i in range(R) if m[i]]
+        if xs:
+            by_size[f"{lo}-{hi if hi < 10**9 else 'max'}"] = percentiles(xs)
+
+    e2e = None
+    if not args.no_e2e:
+        print("# mixed_1m end to end ...", file=sys.stderr, flush=True)
+        e2e = mixed_1m_end_to_end()
+        print(f"# mixed_1m end to end: {e2e['wall_s']:.3f}s", file=sys.stderr, flush=True)
 
     achieved = prof0["bytes"] / (prof0["ms"] / 1e3) / 1e9 if prof0 and prof0["ms"] > 0 else None
+    traffic = traffic_from_profiles("served_stream", "kb_partition")
     out = {
         "metric": "cells legalized/s + p99 request latency, served vs plain, at ≤0.5% disp gap",
         "value": served_rate,
         "unit": "cells/s",
         "n_gpus": 1,
-        "steps": max(args.steps, 1),
+        "steps": K,
         "warmup": args.warmup,
-        "ms_per_step": t_served * 1e3,
+        "ms_per_step": t_served / K * 1e3,
         "higher_is_better": True,
         "scaling": "strong",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"served_stream: first {len(reqs)} of 2000 requests; {WORKLOADS['served_stream']}",
-            "mode": "served (continuous batching); value = capacity with every request queued at t=0",
-            "requests": len(reqs), "cells": total,
-            "served_cells_per_s": served_rate, "plain_cells_per_s": plain_rate,
+            "workload": f"served_stream: {R} of 2000 requests; {WORKLOADS['served_stream']}",
+            "mode": "served (continuous batching); value = capacity with the whole stream queued at t=0; "
+                    f"a step is one slice of {R / K:.0f} requests in arrival order (all slices in flight "
+                    "in one server)",
+            "requests": R, "cells": total,
+            "served_cells_per_s": served_rate, "served_s": t_served,
+            "strictly_legal": f"{sum(legal)}/{R}",
+            "plain": {"requests": len(pidx), "stride": args.plain_stride, "cells": plain_cells,
+                      "cells_per_s": plain_rate, "seconds": t_plain,
+                      "note": f"every {args.plain_stride}th request of the stream (arrival order, so "
+                              "unbiased in size) legalized alone through lg_create .. lg_copy_out"},
             "served_vs_plain": served_rate / plain_rate,
-            "served_s": t_served, "plain_s": t_plain,
-            "identical_results": f"{same}/{len(reqs)}", "disp_gap": disp_gap,
+            "identical_results": f"{same}/{len(pidx)} plain-solved requests bit-identical to their served result",
+            "disp_gap": disp_gap,
             "latency_ms": {
                 "load": args.load,
                 "served": percentiles(lat_served),
+                "served_by_cells": by_size,
                 "served_arrival_rate_per_s": lam_s,
                 "plain": percentiles(lat_plain),
                 "plain_arrival_rate_per_s": lam_p,
                 "plain_note": "FIFO replay of the measured per-request plain service times under Poisson "
                               "arrivals at the same fraction of plain capacity",
-                "plain_at_served_arrivals": percentiles(lat_plain_same),
                 "served_saturated": percentiles([r["latency_s"] for r in res]),
             },
             "server": {"slots": args.slots, "iters_per_sync": args.iters_per_sync,
                        "peak_slots": ctr0["peak_slots"], "batches": ctr0["batches"],
                        "batched_iterations": ctr0["iterations"],
                        "pool_reserved_bytes": ctr0["pool_reserved_bytes"],
-                       "hip_graphs": args.graphs, "graph_captures": ctr_last["graph_captures"]},
+                       "hip_graphs": args.graphs},
             "host_synth_import_preplace_s": t_prep,
+            "mixed_1m_end_to_end": e2e,
             "parallelism": "1 GPU",
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "kb_partition (batched ADMM primal + fused dual update over all resident requests)",
+            "kernel": "kb_partition<0,0,4> + <0,1,1> (batched ADMM primal + fused dual update over all "
+                      "resident requests)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS if achieved else None,
-            "traffic": traffic_from_profiles("served_stream", "kb_partition"),
-            "hbm_measured_GBs": (traffic_from_profiles("served_stream", "kb_partition") /
-                                 (prof0["ms"] / max(prof0["iterations"], 1) / 1e3) / 1e9
-                                 if prof0 and traffic_from_profiles("served_stream", "kb_partition") else None),
+            "traffic": traffic,
+            "hbm_measured_GBs": (traffic / (prof0["ms"] / max(prof0["iterations"], 1) / 1e3) / 1e9
+                                 if prof0 and traffic else None),
             "avg_iteration_us": prof0["ms"] / max(prof0["iterations"], 1) * 1e3 if prof0 else None,
             "iterations": prof0["iterations"] if prof0 else None,
             "algorithmic_bytes_per_iteration": prof0["bytes"] / max(prof0["iterations"], 1) if prof0 else None,
             "note": "algorithmic bytes = 12 per pixel + 24 per cell for every ADMM iteration a request "
                     "executed; time = HIP events around the partition-kernel launches of each batched "
-                    "iteration on the server stream; traffic / hbm_measured_GBs = the partition kernel's "
-                    "own HBM bytes (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated) -- it skips "
-                    "partitions without overflow, whose duals the dual pass updates",
+                    "iteration on the server stream, over the timed run; traffic / hbm_measured_GBs = "
+                    "the partition kernel's own HBM bytes (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                    "copy-calibrated) -- it skips partitions without overflow, whose duals the dual "
+                    "pass updates",
         },
         "cpu_baseline": None,
     }

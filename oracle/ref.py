@@ -50,6 +50,14 @@ def lib():
         L.ref_qp_admm.argtypes = [C.c_int, i32p, f64p, i32p, i32p, i32p, C.c_double, C.c_int, C.c_int, f64p]
         L.ref_qp_admm.restype = C.c_int
         L.ref_site_align_dp_row.argtypes = [C.c_int, f64p, f64p, i32p, C.c_int, C.c_int, C.c_int, C.c_int, i32p]
+        f32p = C.POINTER(C.c_float)
+        L.ref_set_mutation.argtypes = [C.c_int]
+        L.ref_move_one.argtypes = [C.POINTER(LgGridDesc), C.POINTER(LgCellsSoA), C.POINTER(LgParams), i32p, f32p,
+                                   i32p, C.c_float, C.c_int32, C.c_int32, C.c_int32, i32p, f32p]
+        L.ref_dual_update.argtypes = [C.c_int64, i32p, f32p, C.c_float, C.c_float]
+        L.ref_partition_order.argtypes = [C.POINTER(LgGridDesc), C.POINTER(LgCellsSoA), C.POINTER(LgParams),
+                                          C.c_int32, i32p, i32p, i32p, i32p, i32p, i32p]
+        L.ref_direction_tables.argtypes = [i32p, i32p]
         _lib = L
     return _lib
 
@@ -135,6 +143,97 @@ def solve(grid: dict, cells: dict, params: LgParams, do_post: bool = True, trace
     it = min(int(st.iterations), trace_cap)
     return dict(lx=lx[:n], ly=ly[:n], rot=rot[:n], usage=usage, dual=dual, stats=stats,
                 trace=tr[:it].copy())
+
+
+def _descs(grid: dict, cells: dict, keep: list):
+    def arr(a):
+        a = np.ascontiguousarray(a, np.int32)
+        keep.append(a)
+        return _p(a)
+
+    g = LgGridDesc()
+    for k in ("x_size", "y_size", "site_width", "row_height", "core_lx", "core_ly"):
+        setattr(g, k, int(grid[k]))
+    g.pixel_valid, g.pixel_group = arr(grid["valid"]), arr(grid["group"])
+    g.pixel_region = arr(grid.get("region", np.full(int(grid["x_size"]) * int(grid["y_size"]), -1)))
+    g.row_vdd_up = arr(grid["vdd_up"])
+    g.num_regions = len(grid.get("region_x_min", []))
+    for k in ("region_x_min", "region_y_min", "region_x_max", "region_y_max"):
+        setattr(g, k, arr(grid.get(k, np.zeros(0, np.int32))))
+    c = LgCellsSoA()
+    c.num_cells = int(len(cells["w_grid"]))
+    for k in ("orig_lx_dbu", "orig_ly_dbu", "lx_dbu", "ly_dbu", "w_grid", "h_grid", "group",
+              "ground_at_bottom"):
+        setattr(c, k, arr(cells[k]))
+    return g, c
+
+
+class mutation:
+    """Context manager planting mistake `m` in the oracle (pin tests only)."""
+
+    def __init__(self, m: int):
+        self.m = m
+
+    def __enter__(self):
+        lib().ref_set_mutation(self.m)
+        return self
+
+    def __exit__(self, *exc):
+        lib().ref_set_mutation(0)
+
+
+def move_one(grid, cells, params, usage, dual, color, rho, cell, part, mode):
+    """One move of `cell` on an explicit state (mode 0 primal, 1 final
+    refine, 2 adaptive).  Returns (x, y), the padded candidate costs and the
+    usage after the commit."""
+    keep = []
+    g, c = _descs(grid, cells, keep)
+    u = np.ascontiguousarray(usage, np.int32).copy()
+    d = np.ascontiguousarray(dual, np.float32)
+    col = np.ascontiguousarray(color, np.int32)
+    xy = np.zeros(2, np.int32)
+    costs = np.zeros(4096 if mode == 2 else 256, np.float32)
+    _check(lib().ref_move_one(C.byref(g), C.byref(c), C.byref(params), _p(u), d.ctypes.data_as(C.POINTER(C.c_float)),
+                              _p(col), float(rho), int(cell), int(part), int(mode), _p(xy),
+                              costs.ctypes.data_as(C.POINTER(C.c_float))))
+    return (int(xy[0]), int(xy[1])), costs, u
+
+
+def dual_update(usage, dual, rho, step):
+    u = np.ascontiguousarray(usage, np.int32)
+    d = np.ascontiguousarray(dual, np.float32).copy()
+    lib().ref_dual_update(len(u), _p(u), d.ctypes.data_as(C.POINTER(C.c_float)), float(rho), float(step))
+    return d
+
+
+def partition_order(grid, cells, params, iteration, usage=None):
+    """Colour map (x-major), partition of every cell (K = excluded) and, with
+    a usage map, the processing order and sort keys of executeSorting."""
+    keep = []
+    g, c = _descs(grid, cells, keep)
+    P = int(grid["x_size"]) * int(grid["y_size"])
+    n = int(len(cells["w_grid"]))
+    xh, yh = int(params.x_hint), int(params.y_hint)
+    K = -(-int(grid["x_size"]) // xh) * -(-int(grid["y_size"]) // yh)
+    color = np.zeros(P, np.int32)
+    part = np.zeros(max(n, 1), np.int32)
+    grouped, key = np.zeros(max(n, 1), np.int32), np.zeros(max(n, 1), np.int32)
+    offset = np.zeros(K + 2, np.int32)
+    up = None
+    if usage is not None:
+        ua = np.ascontiguousarray(usage, np.int32)
+        keep.append(ua)
+        up = _p(ua)
+    _check(lib().ref_partition_order(C.byref(g), C.byref(c), C.byref(params), int(iteration), up, _p(color),
+                                     _p(part), _p(grouped), _p(offset), _p(key)))
+    return dict(color=color, part=part[:n], grouped=grouped[:n], offset=offset, key=key[:n], K=K)
+
+
+def direction_tables():
+    d = np.zeros(2 * 245, np.int32)
+    w = np.zeros(2 * 4093, np.int32)
+    lib().ref_direction_tables(_p(d), _p(w))
+    return d.reshape(-1, 2), w.reshape(-1, 2)
 
 
 def bitonic_desc(keys, vals):

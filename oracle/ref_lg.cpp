@@ -54,6 +54,26 @@ constexpr int kBlockageGroup = -2;  // Legalizer.h:34
 
 static thread_local std::string g_err;
 
+// Planted mistakes for the pin tests (0: none).  A pin test sets one with
+// ref_set_mutation() and must then see the oracle disagree with the
+// independent check it is pinned by; the solve paths never set it.
+enum Mutation {
+  kMutNone = 0,
+  kMutPenaltySwapped = 1,     // standard and negotiation penalty exchanged
+  kMutNoExcessTerm = 2,       // displacement beyond the threshold not charged
+  kMutThresholdInSites = 3,   // threshold max_disp rows taken as sites
+  kMutDualNoHalf = 4,         // dual step rho * g^2 instead of rho/2 * g^2
+  kMutEvenNoDoubleStep = 5,   // even heights move by one row per dy
+  kMutTieKeepsUpper = 6,      // tree-min replaces on >= (ties keep the upper slot)
+  kMutNoFoldOffset = 7,       // partition colouring without the fold offset
+  kMutKeyOverflowWeight1 = 8, // sort key overflow weight 1 instead of 10
+  kMutNoStraddleCheck = 9,    // a cell takes its lower-left pixel's partition
+  kMutNoProjection = 10,      // adaptive candidates never jump past a region
+  kMutAdaptiveDispFull = 11,  // adaptive cost without the displacement divisor
+  kMutIccadWeightInverted = 12,  // ICCAD17 weight count(h) / N
+};
+static int g_mut = kMutNone;
+
 static double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -657,6 +677,7 @@ static void initialize_dual(Dev& D) {
 static void update_partition(Dev& D, int iter) {
   int tf = (iter / D.p.iter_update_partition) % 3;
   int ox = D.p.x_hint / 3 * tf, oy = D.p.y_hint / 3 * tf;
+  if (g_mut == kMutNoFoldOffset) ox = oy = 0;
   const int threads = std::max(1, D.p.num_cpu_threads);
 #pragma omp parallel for num_threads(threads) schedule(static)
   for (int x = 0; x < D.x_size; x++)
@@ -671,7 +692,7 @@ static void update_partition(Dev& D, int iter) {
     int uxg = lxg + D.w[i] - 1, uyg = lyg + D.h[i] - 1;
     int c1 = (lxg / D.p.x_hint) % D.ncol, c2 = (uxg / D.p.x_hint) % D.ncol;
     int r1 = (lyg / D.p.y_hint) % D.nrow, r2 = (uyg / D.p.y_hint) % D.nrow;
-    D.part_of[i] = (c1 == c2 && r1 == r2) ? c1 * D.nrow + r1 : D.K;
+    D.part_of[i] = (c1 == c2 && r1 == r2) || g_mut == kMutNoStraddleCheck ? c1 * D.nrow + r1 : D.K;
   }
   // stable grouping by partition id (thrust::stable_sort_by_key on a sequence)
   std::vector<int> cnt(D.K + 1, 0);
@@ -731,7 +752,7 @@ static int cell_overflow(const Dev& D, int x0, int y0, int w, int h) {  // AdmmK
 
 // executeSorting (CudaAdmmLegalizer.cu:397-556, SortingKernels.hpp:9-55, 221-331)
 static void execute_sorting(Dev& D) {
-  int cr = 0, co = 10, ca = 1;
+  int cr = 0, co = g_mut == kMutKeyOverflowWeight1 ? 1 : 10, ca = 1;
   if (D.p.sorting_policy == LG_SORT_POLICY_RANDOM) { cr = 1; co = 0; ca = 0; }
   D.key.resize(D.N);
   const int threads = std::max(1, D.p.num_cpu_threads);
@@ -765,14 +786,17 @@ static void execute_sorting(Dev& D) {
 static inline float disp_cost(const Dev& D, int nx, int ny, int c) {
   float coeff = D.p.flag_iccad17 == 0 ? 1.0f
                 : float(D.N) / float(D.k_height_to_num[D.h[c] - 1]);
+  if (g_mut == kMutIccadWeightInverted && D.p.flag_iccad17)
+    coeff = float(D.k_height_to_num[D.h[c] - 1]) / float(D.N);
   int dx = nx * D.sw + D.core_lx, dy = ny * D.rh + D.core_ly;
   int disp = std::abs(dx - D.ox[c]) + std::abs(dy - D.oy[c]);
   float ds = float(disp) / float(D.sw);
-  float thr = float(D.p.max_disp_in_row_height) * float(D.rh);
+  float thr = float(D.p.max_disp_in_row_height) * float(g_mut == kMutThresholdInSites ? D.sw : D.rh);
   thr = thr / float(D.sw);
   float ex = ds - thr;
   float a = coeff * ds;
   float b = D.p.max_disp_coeff * std::max(ex, 0.0f);
+  if (g_mut == kMutNoExcessTerm) b = 0.0f;
   return a + b;
 }
 
@@ -792,7 +816,7 @@ static inline float penalty_sum(const Dev& D, bool standard, bool exclusive, boo
       float l = D.dual[p];
       if (final_refine && u > 0.0f) return kInf;
       float t;
-      if (standard) {
+      if (standard != (g_mut == kMutPenaltySwapped)) {
         float a = u - 1.0f;
         float b = l / D.rho;
         t = D.rho * std::max(a + b, 0.0f);
@@ -810,7 +834,7 @@ static inline float penalty_sum(const Dev& D, bool standard, bool exclusive, boo
 // getNewCellPosition (AdmmKernels.hpp:540-560)
 static inline void new_position(const Dev& D, const std::vector<int>& dir, int d, int c, int cx, int cy,
                                 int& nx, int& ny) {
-  int eps = (D.h[c] % 2 == 0) ? 2 : 1;
+  int eps = (D.h[c] % 2 == 0 && g_mut != kMutEvenNoDoubleStep) ? 2 : 1;
   nx = cx + dir[2 * d];
   ny = cy + eps * dir[2 * d + 1];
   nx = std::max(0, std::min(D.x_size - D.w[c], nx));
@@ -830,7 +854,7 @@ static inline void new_position_projected(const Dev& D, int d, int c, int cx, in
       if (x >= D.x_size || y >= D.y_size || x < 0 || y < 0) { inval = true; break; }
       if (D.valid[D.pid(x, y)] == 0) { inval = true; rid = D.pregion[D.pid(x, y)]; break; }
     }
-  if (rid != -1) {
+  if (rid != -1 && g_mut != kMutNoProjection) {
     if (std::abs(dx) >= std::abs(dy)) nx = dx > 0 ? D.rxmax[rid] + 1 : D.rxmin[rid] - 1;
     else ny = dy > 0 ? D.rymax[rid] + 1 : D.rymin[rid] - 1;
   }
@@ -845,7 +869,10 @@ int tree_argmin(float* cost, int n_padded) {
   std::iota(idx.begin(), idx.end(), 0);
   for (int s = n_padded / 2; s > 0; s >>= 1)
     for (int t = 0; t < s; t++)
-      if (cost[t] > cost[t + s]) { cost[t] = cost[t + s]; idx[t] = idx[t + s]; }
+      if (cost[t] > cost[t + s] || (g_mut == kMutTieKeepsUpper && cost[t] == cost[t + s])) {
+        cost[t] = cost[t + s];
+        idx[t] = idx[t + s];
+      }
   return idx[0];
 }
 
@@ -855,7 +882,8 @@ static void change_usage(Dev& D, int x0, int y0, int w, int h, int delta) {
 
 // One cell of doPrimalUpdateKernel / doFinalRefineKernel (AdmmKernels.hpp:710-896,
 // 1246-1415).  Returns after committing the cell (possibly in place).
-static void move_cell(Dev& D, int part, int c, bool final_refine, std::vector<float>& buf) {
+static void move_cell(Dev& D, int part, int c, bool final_refine, std::vector<float>& buf,
+                      float* costs_out = nullptr) {
   const int nd = (int)D.dir_default.size() / 2;
   const int np = pow2_ceil(nd);
   int cx = D.lx[c], cy = D.ly[c];
@@ -872,6 +900,7 @@ static void move_cell(Dev& D, int part, int c, bool final_refine, std::vector<fl
     float edge = (pc == kInf) ? kInf : 0.0f;
     buf[d] = dc + pc + pin + edge;
   }
+  if (costs_out) std::copy(buf.begin(), buf.begin() + np, costs_out);
   int best = tree_argmin(buf.data(), np);
   int nx = cx, ny = cy;
   if (buf[0] != kInf) new_position(D, D.dir_default, best, c, cx, cy, nx, ny);
@@ -912,7 +941,7 @@ static void dual_update(Dev& D) {
   for (int64_t p = 0; p < P; p++) {
     float u = float(D.usage[p]);
     float g = std::max(u - 1.0f, neg);
-    double q = (double)rho * 0.5;
+    double q = (double)rho * (g_mut == kMutDualNoHalf ? 1.0 : 0.5);
     q = q * (double)g;
     q = q * (double)g;
     float delta = (float)((double)g + q);
@@ -922,35 +951,43 @@ static void dual_update(Dev& D) {
   }
 }
 
+// One cell of projectionAwareSearchKernel (AdmmKernels.hpp:1029-1172): skip
+// without overflow, rip up, 4093 wide candidates with projection, cost =
+// displacement / coeff_to_neglect_displace + negotiation penalty (no
+// partition restriction), tree-min, commit.
+static void adaptive_move(Dev& D, int c, std::vector<float>& buf, float* costs_out = nullptr) {
+  const int nd = (int)D.dir_wide.size() / 2;
+  const int np = pow2_ceil(nd);
+  int cx = D.lx[c], cy = D.ly[c];
+  if (cell_overflow(D, cx, cy, D.w[c], D.h[c]) == 0) return;
+  change_usage(D, cx, cy, D.w[c], D.h[c], -1);
+  for (int d = 0; d < np; d++) {
+    if (d >= nd) { buf[d] = kInf; continue; }
+    int nx, ny;
+    new_position_projected(D, d, c, cx, cy, nx, ny);
+    float dc = disp_cost(D, nx, ny, c);
+    float pc = penalty_sum(D, false, false, false, -1, nx, ny, c);
+    float q = g_mut == kMutAdaptiveDispFull ? dc : dc / D.p.coeff_to_neglect_displace;
+    buf[d] = q + pc;
+  }
+  if (costs_out) std::copy(buf.begin(), buf.begin() + np, costs_out);
+  int best = tree_argmin(buf.data(), np);
+  int nx = cx, ny = cy;
+  if (buf[0] != kInf) new_position_projected(D, best, c, cx, cy, nx, ny);
+  change_usage(D, nx, ny, D.w[c], D.h[c], +1);
+  D.lx[c] = nx;
+  D.ly[c] = ny;
+}
+
 // projectionAwareSearchKernel (AdmmKernels.hpp:974-1174) after
-// extractCongestedCellsKernel + streamCompaction (CudaAdmmLegalizer.cu:878-993)
+// extractCongestedCellsKernel + streamCompaction (CudaAdmmLegalizer.cu:878-993):
+// the overflowing cells in id order, one after another
 static void adaptive_search(Dev& D) {
   std::vector<int> ovf_cells;
   for (int c = 0; c < D.N; c++)
     if (cell_overflow(D, D.lx[c], D.ly[c], D.w[c], D.h[c]) != 0) ovf_cells.push_back(c);
-  const int nd = (int)D.dir_wide.size() / 2;
-  const int np = pow2_ceil(nd);
-  std::vector<float> buf(np);
-  for (int c : ovf_cells) {
-    int cx = D.lx[c], cy = D.ly[c];
-    if (cell_overflow(D, cx, cy, D.w[c], D.h[c]) == 0) continue;
-    change_usage(D, cx, cy, D.w[c], D.h[c], -1);
-    for (int d = 0; d < np; d++) {
-      if (d >= nd) { buf[d] = kInf; continue; }
-      int nx, ny;
-      new_position_projected(D, d, c, cx, cy, nx, ny);
-      float dc = disp_cost(D, nx, ny, c);
-      float pc = penalty_sum(D, false, false, false, -1, nx, ny, c);
-      float q = dc / D.p.coeff_to_neglect_displace;
-      buf[d] = q + pc;
-    }
-    int best = tree_argmin(buf.data(), np);
-    int nx = cx, ny = cy;
-    if (buf[0] != kInf) new_position_projected(D, best, c, cx, cy, nx, ny);
-    change_usage(D, nx, ny, D.w[c], D.h[c], +1);
-    D.lx[c] = nx;
-    D.ly[c] = ny;
-  }
+  std::vector<float> buf(pow2_ceil((int)D.dir_wide.size() / 2));
+  for (int c : ovf_cells) adaptive_move(D, c, buf);
 }
 
 // detectSlowConvergence (CudaAdmmLegalizer.cu:799-840)
@@ -1532,10 +1569,78 @@ int ref_run_design(const LgDesign* d, const LgParams* p, LgResult* r) {
   return LG_OK;
 }
 
+// ---- single steps for the parity pins (tests/test_oracle_pins_cpu.py) ----
+void ref_set_mutation(int m) { ref::g_mut = m; }
+
+// One move of one cell on an explicit state (x-major usage, dual and
+// partition colour; positions from c->lx_dbu/ly_dbu).  mode 0: primal step of
+// partition `part` (doPrimalUpdateKernel's move), 1: final refine, 2: adaptive
+// search (projected wide candidates).  out_xy receives the new grid position,
+// costs the padded candidate costs before the tree-min (256 or 4096), usage
+// the state after the commit.
+int ref_move_one(const LgGridDesc* g, const LgCellsSoA* c, const LgParams* p, int32_t* usage,
+                 const float* dual, const int32_t* color, float rho, int32_t cell, int32_t part, int32_t mode,
+                 int32_t* out_xy, float* costs) {
+  ref::Dev D;
+  ref::init_dev(D, g, c, p);
+  const size_t P = (size_t)D.x_size * D.y_size;
+  std::copy(usage, usage + P, D.usage.begin());
+  std::copy(dual, dual + P, D.dual.begin());
+  std::copy(color, color + P, D.color.begin());
+  D.rho = rho;
+  if (cell < 0 || cell >= D.N) return LG_ERR_ARG;
+  std::vector<float> buf(mode == 2 ? 4096 : 256);
+  if (mode == 2) ref::adaptive_move(D, cell, buf, costs);
+  else ref::move_cell(D, part, cell, mode == 1, buf, costs);
+  out_xy[0] = D.lx[cell];
+  out_xy[1] = D.ly[cell];
+  std::copy(D.usage.begin(), D.usage.end(), usage);
+  return LG_OK;
+}
+
+// dual_update_functor_with_step_size over n pixels, in place.
+void ref_dual_update(int64_t n, const int32_t* usage, float* dual, float rho, float step) {
+  ref::Dev D;
+  D.p.num_cpu_threads = 1;
+  D.usage.assign(usage, usage + n);
+  D.dual.assign(dual, dual + n);
+  D.rho = rho;
+  D.dual_step = step;
+  ref::dual_update(D);
+  std::copy(D.dual.begin(), D.dual.end(), dual);
+}
+
+// updatePartition of iteration `iter` (colour map x-major, partition of every
+// cell, K meaning excluded) and, with usage given, executeSorting (processing
+// order per partition: grouped[offset[k] .. offset[k+1]) and the sort keys).
+int ref_partition_order(const LgGridDesc* g, const LgCellsSoA* c, const LgParams* p, int32_t iter,
+                        const int32_t* usage, int32_t* color, int32_t* part_of, int32_t* grouped,
+                        int32_t* offset, int32_t* key) {
+  ref::Dev D;
+  ref::init_dev(D, g, c, p);
+  ref::update_partition(D, iter);
+  std::copy(D.color.begin(), D.color.end(), color);
+  std::copy(D.part_of.begin(), D.part_of.end(), part_of);
+  if (usage) {
+    std::copy(usage, usage + D.usage.size(), D.usage.begin());
+    ref::execute_sorting(D);
+    std::copy(D.grouped.begin(), D.grouped.end(), grouped);
+    std::copy(D.offset.begin(), D.offset.end(), offset);
+    std::copy(D.key.begin(), D.key.end(), key);
+  }
+  return LG_OK;
+}
+
 // Small exported pieces for the parity pins
 void ref_bitonic_desc(int n, int* key, int* val) { ref::bitonic_desc(n, key, val); }
 void ref_bitonic_asc(int n, int* key, int* val) { ref::bitonic_asc(n, key, val); }
 int ref_tree_argmin(float* cost, int n) { return ref::tree_argmin(cost, n); }
+void ref_direction_tables(int32_t* dflt /*[2*245]*/, int32_t* wide /*[2*4093]*/) {
+  ref::Host H;
+  ref::make_directions(H);
+  std::copy(H.dir_default.begin(), H.dir_default.end(), dflt);
+  std::copy(H.dir_wide.begin(), H.dir_wide.end(), wide);
+}
 void ref_thomas(int n, double a, double b, const double* d, const double* cp, double* dp, double* x) {
   ref::thomas(n, a, b, d, cp, dp, x);
 }

@@ -1,0 +1,376 @@
+"""Brute-force pins of the oracle's ADMM steps (CPU only).
+
+Each oracle step is compared with an independent evaluation written here
+from the reference kernels, on random small states, and each comparison is
+shown to have power: with a plausible mistake planted in the oracle
+(ref.mutation), the same comparison fails.
+
+* move (doPrimalUpdateKernel's per-cell step, AdmmKernels.hpp:710-896, and
+  the final refine's, 1246-1415): all 245 candidates enumerated in numpy
+  float32 (clamp, two rows per dy for even heights, rail, fence group,
+  partition exclusivity, displacement with the 40x excess term, both penalty
+  variants, the ICCAD17 weight N / count(h)); the padded cost vector must be
+  bit-identical and the winner must be the lexicographic minimum of (cost,
+  bit-reversed slot), which is what the halving-stride tree-min with
+  strict replacement keeps.
+* dual step (AdmmKernels.hpp:26-42): closed forms for usage 0..3.
+* colouring and exclusion (AdmmKernels.hpp:122-191): the colour map from
+  the fold-offset closed form; a cell's partition from the set of colours
+  under its footprint.
+* sort keys (SortingKernels.hpp:9-55): recount of 10 * overflow + w * h,
+  order non-increasing in the key within each partition.
+* adaptive move (AdmmKernels.hpp:562-635, 974-1174): all 4093 wide
+  candidates with the projection past blocking regions, cost =
+  displacement / coeff + penalty without partition restriction.
+"""
+import numpy as np
+import pytest
+
+from admm_legalization_amd.design import default_params
+from oracle import ref
+
+F = np.float32
+INF = F(np.inf)
+
+# planted mistakes (oracle/ref_lg.cpp, enum Mutation)
+M_PENALTY_SWAPPED, M_NO_EXCESS, M_THR_IN_SITES, M_DUAL_NO_HALF, M_EVEN_SINGLE_STEP, M_TIE_UPPER = 1, 2, 3, 4, 5, 6
+M_NO_FOLD, M_KEY_W1, M_NO_STRADDLE, M_NO_PROJECTION, M_ADAPTIVE_DISP_FULL, M_ICCAD_INVERTED = 7, 8, 9, 10, 11, 12
+
+
+# -- direction tables (Legalizer::makeDirectionVectors, Legalizer.cpp:252-351) --
+def _dirs(ranges):
+    out = []
+    r = len(ranges) - 1
+    for dy in range(-r, r + 1):
+        xr = ranges[abs(dy)]
+        out += [(dx, dy) for dx in range(-xr, xr + 1)]
+    return np.array(out, np.int32)
+
+
+DIRS = _dirs((40, 20, 10, 5, 2, 1, 0))
+WIDE = _dirs((280, 240, 200, 160, 120, 80, 40, 20, 10, 4, 2, 1, 0))
+
+
+def test_direction_tables():
+    d, w = ref.direction_tables()
+    assert len(DIRS) == 245 and len(WIDE) == 4093
+    np.testing.assert_array_equal(d, DIRS)
+    np.testing.assert_array_equal(w, WIDE)
+
+
+# -- random states -----------------------------------------------------------
+def _state(seed, xs=44, ys=18, n=36, x_hint=16, y_hint=6, heights=(1, 1, 1, 2, 2, 3), iccad=False,
+           standard=False):
+    rng = np.random.default_rng(seed)
+    valid = np.ones((xs, ys), np.int32)
+    group = np.full((xs, ys), -1, np.int32)
+    region = np.full((xs, ys), -1, np.int32)
+    rb = []
+    for r in range(2):  # blocking rectangles (macros): invalid, region r
+        w, h = int(rng.integers(3, 7)), int(rng.integers(2, 5))
+        x, y = int(rng.integers(4, xs - w - 4)), int(rng.integers(2, ys - h - 2))
+        valid[x:x + w, y:y + h] = 0
+        group[x:x + w, y:y + h] = -2
+        region[x:x + w, y:y + h] = r
+        rb.append((x, y, x + w - 1, y + h - 1))
+    fx, fy = int(rng.integers(0, xs - 10)), int(rng.integers(0, ys - 6))
+    fence = (valid[fx:fx + 10, fy:fy + 6] == 1)
+    group[fx:fx + 10, fy:fy + 6][fence] = 0
+    vdd = (np.arange(ys) % 2).astype(np.int32)
+    sw, rh, clx, cly = 100, 800, 300, 1600
+    w = rng.integers(1, 5, n).astype(np.int32)
+    h = rng.choice(heights, n).astype(np.int32)
+    lx = np.array([rng.integers(0, xs - wi + 1) for wi in w], np.int32)
+    ly = np.array([rng.integers(0, ys - hi + 1) for hi in h], np.int32)
+    cg = np.where(rng.random(n) < 0.2, 0, -1).astype(np.int32)
+    gnd = rng.integers(0, 2, n).astype(np.int32)
+    # global-placement positions: near, and some far beyond the 3-row threshold
+    far = rng.random(n) < 0.3
+    ox = clx + lx * sw + rng.integers(-300, 300, n) + np.where(far, rng.integers(-6000, 6000, n), 0)
+    oy = cly + ly * rh + rng.integers(-900, 900, n) + np.where(far, rng.integers(-8000, 8000, n), 0)
+    usage = np.zeros((xs, ys), np.int32)
+    for i in range(n):
+        usage[lx[i]:lx[i] + w[i], ly[i]:ly[i] + h[i]] += 1
+    dual = (rng.random((xs, ys)) * 3).astype(np.float32)
+    dual[rng.random((xs, ys)) < 0.2] = 0
+    rho = F(rng.uniform(0.3, 4.0))
+    p = default_params()
+    p.x_hint, p.y_hint = x_hint, y_hint
+    p.flag_iccad17 = int(iccad)
+    p.standard_admm = int(standard)
+    ncol, nrow = -(-xs // x_hint), -(-ys // y_hint)
+    xx, yy = np.meshgrid(np.arange(xs), np.arange(ys), indexing="ij")
+    color = ((xx // x_hint) % ncol) * nrow + (yy // y_hint) % nrow
+    grid = dict(x_size=xs, y_size=ys, site_width=sw, row_height=rh, core_lx=clx, core_ly=cly,
+                valid=valid.ravel(), group=group.ravel(), region=region.ravel(), vdd_up=vdd,
+                region_x_min=np.array([r[0] for r in rb], np.int32), region_y_min=np.array([r[1] for r in rb], np.int32),
+                region_x_max=np.array([r[2] for r in rb], np.int32), region_y_max=np.array([r[3] for r in rb], np.int32))
+    cells = dict(orig_lx_dbu=ox.astype(np.int32), orig_ly_dbu=oy.astype(np.int32),
+                 lx_dbu=(clx + lx * sw).astype(np.int32), ly_dbu=(cly + ly * rh).astype(np.int32),
+                 w_grid=w, h_grid=h, group=cg, ground_at_bottom=gnd)
+    return dict(grid=grid, cells=cells, p=p, usage=usage, dual=dual, rho=rho, color=color.astype(np.int32),
+                lx=lx, ly=ly, w=w, h=h, cg=cg, gnd=gnd, ox=ox, oy=oy, valid=valid, group=group, region=region,
+                rb=rb, vdd=vdd, xs=xs, ys=ys, sw=sw, rh=rh, clx=clx, cly=cly, n=n)
+
+
+# -- independent numpy evaluation --------------------------------------------
+def _disp_cost(S, c, nx, ny):
+    """computeDispCost (AdmmKernels.hpp:508-530) in float32."""
+    p = S["p"]
+    if p.flag_iccad17:
+        cnt = int((S["h"] == S["h"][c]).sum())
+        coeff = F(S["n"]) / F(cnt)
+    else:
+        coeff = F(1)
+    disp = abs(nx * S["sw"] + S["clx"] - int(S["ox"][c])) + abs(ny * S["rh"] + S["cly"] - int(S["oy"][c]))
+    ds = F(disp) / F(S["sw"])
+    thr = F(F(p.max_disp_in_row_height) * F(S["rh"])) / F(S["sw"])
+    return F(coeff * ds) + F(F(p.max_disp_coeff) * max(F(ds - thr), F(0)))
+
+
+def _penalty(S, usage, c, nx, ny, standard, exclusive, refine, part):
+    """computePenaltySum (AdmmKernels.hpp:333-391) in float32, x-major order."""
+    w, h = int(S["w"][c]), int(S["h"][c])
+    if nx < 0 or ny < 0 or nx + w > S["xs"] or ny + h > S["ys"]:
+        return INF
+    if h % 2 == 0 and int(S["gnd"][c]) != int(S["vdd"][ny]):
+        return INF
+    rho = S["rho"]
+    s = F(0)
+    for x in range(nx, nx + w):
+        for y in range(ny, ny + h):
+            if S["valid"][x, y] == 0 or S["group"][x, y] != S["cg"][c]:
+                return INF
+            if exclusive and S["color"][x, y] != part:
+                return INF
+            u = F(usage[x, y])
+            lam = S["dual"][x, y]
+            if refine and u > F(0):
+                return INF
+            a = F(u - F(1))
+            b = F(lam / rho)
+            t = rho * max(F(a + b), F(0)) if standard else rho * max(F(F(lam * a) + b), F(0))
+            s = F(s + F(t))
+    return F(0) if refine else s
+
+
+def _candidate(S, c, dx, dy):
+    """getNewCellPosition (AdmmKernels.hpp:540-560)."""
+    w, h = int(S["w"][c]), int(S["h"][c])
+    eps = 2 if h % 2 == 0 else 1
+    nx = min(max(int(S["lx"][c]) + dx, 0), S["xs"] - w)
+    ny = min(max(int(S["ly"][c]) + eps * dy, 0), S["ys"] - h)
+    return nx, ny
+
+
+def _projected(S, c, dx, dy):
+    """getNewCellPositionWithProjection (AdmmKernels.hpp:562-635)."""
+    w, h = int(S["w"][c]), int(S["h"][c])
+    eps = 2 if h % 2 == 0 else 1
+    nx, ny = int(S["lx"][c]) + dx, int(S["ly"][c]) + eps * dy
+    rid = -1
+    done = False
+    for x in range(nx, nx + w):
+        for y in range(ny, ny + h):
+            if x >= S["xs"] or y >= S["ys"] or x < 0 or y < 0:
+                done = True
+                break
+            if S["valid"][x, y] == 0:
+                rid = int(S["region"][x, y])
+                done = True
+                break
+        if done:
+            break
+    if rid != -1:
+        x0, y0, x1, y1 = S["rb"][rid]
+        if abs(dx) >= abs(dy):
+            nx = x1 + 1 if dx > 0 else x0 - 1
+        else:
+            ny = y1 + 1 if dy > 0 else y0 - 1
+    return min(max(nx, 0), S["xs"] - w), min(max(ny, 0), S["ys"] - h)
+
+
+def _bitrev(i, bits):
+    return int(format(i, f"0{bits}b")[::-1], 2)
+
+
+def _winner(costs):
+    """Lexicographic minimum of (cost, bit-reversed slot)."""
+    bits = len(costs).bit_length() - 1
+    return min(range(len(costs)), key=lambda i: (float(costs[i]), _bitrev(i, bits)))
+
+
+def np_move(S, c, part, mode):
+    """One cell's move: rip-up, candidates, winner, commit (positions only)."""
+    w, h = int(S["w"][c]), int(S["h"][c])
+    cx, cy = int(S["lx"][c]), int(S["ly"][c])
+    usage = S["usage"].copy()
+    if mode == 2 and np.maximum(usage[cx:cx + w, cy:cy + h] - 1, 0).sum() == 0:
+        return (cx, cy), None
+    usage[cx:cx + w, cy:cy + h] -= 1
+    dirs = WIDE if mode == 2 else DIRS
+    costs = np.full(4096 if mode == 2 else 256, INF, F)
+    for d, (dx, dy) in enumerate(dirs):
+        if mode == 2:
+            nx, ny = _projected(S, c, int(dx), int(dy))
+            pc = _penalty(S, usage, c, nx, ny, False, False, False, -1)
+            costs[d] = F(_disp_cost(S, c, nx, ny) / F(S["p"].coeff_to_neglect_displace)) + pc
+        else:
+            nx, ny = _candidate(S, c, int(dx), int(dy))
+            standard = mode == 0 and bool(S["p"].standard_admm)
+            pc = _penalty(S, usage, c, nx, ny, standard, True, mode == 1, part)
+            edge = INF if pc == INF else F(0)
+            costs[d] = F(F(F(_disp_cost(S, c, nx, ny) + pc) + F(0)) + edge)
+    best = _winner(costs)
+    if costs[best] == INF:
+        return (cx, cy), costs
+    dx, dy = (int(v) for v in dirs[best])
+    return (_projected(S, c, dx, dy) if mode == 2 else _candidate(S, c, dx, dy)), costs
+
+
+def _compare_moves(states, mode, cells_per_state=6):
+    """(mismatching moves, moves checked, winners that were ties)."""
+    bad, checked, ties = 0, 0, 0
+    for S in states:
+        rng = np.random.default_rng(int(S["n"]) + len(states))
+        for c in rng.choice(S["n"], cells_per_state, replace=False):
+            c = int(c)
+            part = int(S["color"][S["lx"][c], S["ly"][c]])
+            want_xy, want_costs = np_move(S, c, part, mode)
+            got_xy, got_costs, _ = ref.move_one(S["grid"], S["cells"], S["p"], S["usage"].ravel(),
+                                                S["dual"].ravel(), S["color"].ravel(), S["rho"], c, part, mode)
+            checked += 1
+            same = got_xy == want_xy
+            if want_costs is not None:
+                same &= np.array_equal(got_costs.view(np.uint32), want_costs.view(np.uint32))
+                fin = want_costs[np.isfinite(want_costs)]
+                ties += int(len(fin) > 0 and (fin == fin.min()).sum() > 1)
+            bad += int(not same)
+    return bad, checked, ties
+
+
+def _tie_state(seed):
+    """No prices (every penalty term zero) and every cell's global-placement
+    x half a site right of its current site: the candidates dx = 0 and
+    dx = +1 tie on displacement with different positions."""
+    S = _state(seed, n=20)
+    S["dual"][:] = 0
+    S["ox"] = (S["clx"] + S["lx"] * S["sw"] + S["sw"] // 2).astype(np.int64)
+    S["oy"] = (S["cly"] + S["ly"] * S["rh"]).astype(np.int64)
+    S["cells"]["orig_lx_dbu"] = S["ox"].astype(np.int32)
+    S["cells"]["orig_ly_dbu"] = S["oy"].astype(np.int32)
+    return S
+
+
+STATES = [_state(s, standard=(s % 3 == 0), iccad=(s % 4 == 1)) for s in range(16)] + \
+    [_tie_state(50 + s) for s in range(4)]
+
+
+def test_move_matches_brute_force():
+    bad, checked, ties = _compare_moves(STATES, 0)
+    assert checked == 120 and bad == 0
+    assert ties > 0  # tie-breaks were exercised
+
+
+def test_final_refine_move_matches_brute_force():
+    bad, checked, _ = _compare_moves(STATES[:8], 1)
+    assert checked == 48 and bad == 0
+
+
+@pytest.mark.parametrize("m", [M_PENALTY_SWAPPED, M_NO_EXCESS, M_THR_IN_SITES, M_EVEN_SINGLE_STEP, M_TIE_UPPER,
+                               M_ICCAD_INVERTED])
+def test_move_pin_detects_planted_mistake(m):
+    with ref.mutation(m):
+        bad, checked, _ = _compare_moves(STATES, 0)
+    assert bad > 0, f"mutation {m} went unnoticed in {checked} moves"
+
+
+def test_adaptive_move_matches_brute_force():
+    # a little congestion so that every checked cell has overflow
+    states = [_state(100 + s, n=60, xs=36, ys=14) for s in range(3)]
+    bad, checked, _ = _compare_moves(states, 2, cells_per_state=3)
+    assert checked == 9 and bad == 0
+    for m in (M_NO_PROJECTION, M_ADAPTIVE_DISP_FULL):
+        with ref.mutation(m):
+            b, _, _ = _compare_moves(states, 2, cells_per_state=3)
+        assert b > 0, f"mutation {m} went unnoticed"
+
+
+# -- dual step ---------------------------------------------------------------
+def test_dual_step_closed_form():
+    """g = max(u - 1, -1/rho); delta = g + rho/2 g^2; lambda <- max(lambda +
+    step * delta, 0).  rho = 2, step = 1/4, lambda = 1 keeps every value
+    dyadic, so the closed forms are exact: u=0: g=-1/2, delta=-1/4;
+    u=1: 0; u=2: g=1, delta=2; u=3: g=2, delta=6."""
+    u = np.array([0, 1, 2, 3, 0], np.int32)
+    lam = np.array([1, 1, 1, 1, 0.01], np.float32)
+    got = ref.dual_update(u, lam, 2.0, 0.25)
+    np.testing.assert_array_equal(got, np.array([0.9375, 1.0, 1.5, 2.5, 0.0], np.float32))
+    with ref.mutation(M_DUAL_NO_HALF):
+        assert not np.array_equal(ref.dual_update(u, lam, 2.0, 0.25), got)
+
+
+def test_dual_step_general_rho_within_two_ulps():
+    """For any rho the result is the closed form (FP64 on the FP32 inputs,
+    -1/rho rounded to FP32 as the reference computes it) up to the FP32
+    roundings of delta, step * delta and the sum (half an ulp each): within
+    two ulps of the largest of lambda, step * delta and their sum."""
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        rho, step = F(rng.uniform(0.05, 20)), F(rng.uniform(0.01, 3))
+        u = rng.integers(0, 5, 64).astype(np.int32)
+        lam = (rng.random(64) * 5).astype(np.float32)
+        got = ref.dual_update(u, lam, rho, step).astype(np.float64)
+        g = np.maximum(u.astype(np.float64) - 1, np.float64(F(-1) / rho))
+        m = np.float64(step) * (g + np.float64(rho) / 2 * g * g)
+        want = np.maximum(lam + m, 0)
+        big = np.maximum(np.maximum(np.abs(lam), np.abs(m)), np.abs(lam + m)).astype(np.float32)
+        tol = 2 * np.spacing(big).astype(np.float64)
+        assert (np.abs(got - want) <= tol).all()
+
+
+# -- colouring, exclusion, sort keys -------------------------------------------
+def _colour_closed_form(xs, ys, x_hint, y_hint, iteration, iter_update):
+    """paintPixelsKernel (AdmmKernels.hpp:164-191): offsets (x_hint/3, y_hint/3)
+    times the fold index (iteration / iter_update_partition) mod 3."""
+    tf = (iteration // iter_update) % 3
+    ox, oy = x_hint // 3 * tf, y_hint // 3 * tf
+    ncol, nrow = -(-xs // x_hint), -(-ys // y_hint)
+    xx, yy = np.meshgrid(np.arange(xs), np.arange(ys), indexing="ij")
+    return ((((xx + ox) // x_hint) % ncol) * nrow + ((yy + oy) // y_hint) % nrow).ravel(), ncol * nrow
+
+
+def _check_partitions(S, iteration):
+    p = S["p"]
+    o = ref.partition_order(S["grid"], S["cells"], p, iteration, S["usage"].ravel())
+    want_col, K = _colour_closed_form(S["xs"], S["ys"], p.x_hint, p.y_hint, iteration, p.iter_update_partition)
+    ok = np.array_equal(o["color"], want_col) and o["K"] == K
+    col = want_col.reshape(S["xs"], S["ys"])
+    for i in range(S["n"]):  # a cell belongs to a partition only if its whole footprint has one colour
+        under = np.unique(col[S["lx"][i]:S["lx"][i] + S["w"][i], S["ly"][i]:S["ly"][i] + S["h"][i]])
+        ok &= o["part"][i] == (under[0] if len(under) == 1 else K)
+    # keys: 10 * overflow of the footprint + area, recounted; within each
+    # partition the processing order is non-increasing in the key and holds
+    # exactly the partition's cells
+    ovf = np.array([np.maximum(S["usage"][S["lx"][i]:S["lx"][i] + S["w"][i],
+                                          S["ly"][i]:S["ly"][i] + S["h"][i]] - 1, 0).sum() for i in range(S["n"])])
+    want_key = 10 * ovf + S["w"] * S["h"]
+    off = o["offset"]
+    for k in range(K):
+        ids = o["grouped"][off[k]:off[k + 1]]
+        keys = o["key"][off[k]:off[k + 1]]
+        ok &= sorted(ids.tolist()) == np.flatnonzero(o["part"] == k).tolist()
+        ok &= np.array_equal(keys, want_key[ids]) and bool((np.diff(keys) <= 0).all())
+    return bool(ok)
+
+
+def test_colouring_exclusion_and_keys():
+    for S in STATES[:6]:
+        for it in (0, 4, 5, 10, 11, 17):
+            assert _check_partitions(S, it), it
+
+
+@pytest.mark.parametrize("m", [M_NO_FOLD, M_KEY_W1, M_NO_STRADDLE])
+def test_partition_pin_detects_planted_mistake(m):
+    with ref.mutation(m):
+        assert not all(_check_partitions(S, it) for S in STATES[:6] for it in (0, 5, 10))
