@@ -23,7 +23,7 @@ struct Ctl {
   int32_t wrong_row;     // cells on a row of the wrong rail parity at solve start
   int32_t n_work[2];     // partitions listed in Args::work per sort-size class
   int32_t ovf_window_n;  // entries in the stagnation window
-  int32_t ovf_window[64];
+  int32_t ovf_window[64];  // kMaxStagnationWindow
   float rho;
   float lambda0;
   float step;
@@ -59,6 +59,7 @@ constexpr int kNumDirPadded = 256;
 constexpr int kNumDirWide = 4093;
 constexpr int kNumDirWidePadded = 4096;
 constexpr int kMaxNpad = 4096;
+constexpr int kMaxStagnationWindow = 64;  // Ctl::ovf_window entries
 constexpr int kBucketCells = 1024;  // cells per workgroup of k_bucket
 constexpr int kNpadSmall = 1024;
 

@@ -349,6 +349,8 @@ int prepare_host(const LgGridDesc* g, const LgCellsSoA* c, const LgParams* p, Ho
     return fail(LG_ERR_ARG, "bad partition / schedule parameters");
   if (g->x_size > 65535 || g->y_size > 65535) return fail(LG_ERR_ARG, "grid larger than 65535 sites/rows");
   if ((long long)g->x_size * g->y_size >= (1ll << 30)) return fail(LG_ERR_ARG, "grid of 2^30 pixels or more");
+  if (p->use_adaptive_search && (p->stagnation_window < 1 || p->stagnation_window > kMaxStagnationWindow))
+    return fail(LG_ERR_ARG, "stagnation_window outside [1, 64]");
   hp.p = *p;
   hp.x_size = g->x_size; hp.y_size = g->y_size;
   hp.site_width = g->site_width; hp.row_height = g->row_height;
@@ -373,7 +375,8 @@ int prepare_host(const LgGridDesc* g, const LgCellsSoA* c, const LgParams* p, Ho
   // blocks spread over host threads
   const int nxb = (XS + 63) / 64;
   const int nth = std::max(1, std::min({p->num_cpu_threads, nxb, 32}));
-  std::vector<int> bad(nth, 0);
+  std::vector<int> bad(nth, 0), bad_region(nth, 0);
+  const int nreg = std::max(g->num_regions, 0);
   auto work = [&](int t) {
     for (int xb = t; xb < nxb; xb += nth)
       for (int y0 = 0; y0 < YS; y0 += 64)
@@ -383,7 +386,11 @@ int prepare_host(const LgGridDesc* g, const LgCellsSoA* c, const LgParams* p, Ho
             int grp = g->pixel_group[i];
             if (g->pixel_valid[i] && (grp + 2 > 0x7fff || grp + 2 < 0)) bad[t] = 1;
             hp.pcode[j] = g->pixel_valid[i] ? (int16_t)(grp + 2) : (int16_t)0;
-            if (regions) hp.region_rm[j] = g->pixel_region[i];
+            if (regions) {
+              const int rid = g->pixel_region[i];
+              if (rid < -1 || rid >= nreg) bad_region[t] = 1;
+              hp.region_rm[j] = rid;
+            }
           }
   };
   if (nth == 1 || P < (1u << 20)) {
@@ -394,8 +401,10 @@ int prepare_host(const LgGridDesc* g, const LgCellsSoA* c, const LgParams* p, Ho
     work(0);
     for (auto& th : pool) th.join();
   }
-  for (int t = 0; t < nth; t++)
+  for (int t = 0; t < nth; t++) {
     if (bad[t]) return fail(LG_ERR_ARG, "fence group index out of range");
+    if (bad_region[t]) return fail(LG_ERR_ARG, "pixel region id outside [-1, num_regions)");
+  }
   hp.vdd.assign(g->row_vdd_up, g->row_vdd_up + YS);
   const int nr = std::max(g->num_regions, 0);
   hp.rx0.assign(g->region_x_min, g->region_x_min + nr);
@@ -585,7 +594,7 @@ int create_from_prep(const HostPrep& hp, DevPool* pool, Ctl* ext_ctl, hipStream_
   a.sort_policy = p->sorting_policy;
   a.sort_algo = p->sorting_algorithm;
   a.use_adaptive = p->use_adaptive_search;
-  a.stagnation_window = std::min(p->stagnation_window, 64);
+  a.stagnation_window = p->stagnation_window;
   a.min_iter_adaptive = p->min_iter_to_call_adaptive_search;
   a.cooltime_adaptive = p->cooltime_to_call_adaptive_search;
   a.ovf_ratio_adaptive = p->ovf_ratio_to_call_adaptive_search;

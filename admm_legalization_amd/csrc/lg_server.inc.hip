@@ -37,6 +37,7 @@ struct SrvRequest {
   std::vector<int32_t> lx, ly, rot;
   LgStats stats{};
   int num_cells = 0;
+  bool waited = false;  // a thread is inside lg_srv_wait for this ticket
 };
 
 struct SrvSlot {
@@ -86,6 +87,7 @@ struct LgServer {
   std::condition_variable cv_work, cv_done;
   std::deque<SrvRequest*> queue;
   std::unordered_map<int64_t, SrvRequest*> reqs;
+  int waiters = 0;  // threads inside lg_srv_wait (lg_srv_destroy waits for them)
   int64_t next_ticket = 1;
   bool stop = false;
   LgSrvCounters ctr{};
@@ -535,8 +537,16 @@ int lg_srv_wait(LgServer* S, int64_t ticket, LgSrvResult* res) {
     auto it = S->reqs.find(ticket);
     if (it == S->reqs.end()) return fail(LG_ERR_ARG, "unknown ticket");
     r = it->second;
+    if (r->waited) return fail(LG_ERR_STATE, "another thread is waiting for this ticket");
+    r->waited = true;
+    S->waiters++;
     S->cv_done.wait(lk, [&] { return r->state == LG_SRV_DONE || r->state == LG_SRV_FAILED; });
-    S->reqs.erase(it);
+    // by key: a submit during the wait may have rehashed the map
+    S->reqs.erase(ticket);
+    // the record is this thread's from here on; the server is not touched
+    // again after the lock is released (lg_srv_destroy may free it then)
+    S->waiters--;
+    S->cv_done.notify_all();
   }
   res->num_cells = r->num_cells;
   res->status = r->status;
@@ -602,12 +612,15 @@ int lg_srv_destroy(LgServer* S) {
   S->cv_work.notify_all();
   if (S->worker.joinable()) S->worker.join();
   {
-    // wake waiters, then drop the records nobody collected
-    std::lock_guard<std::mutex> lk(S->mu);
+    // every request is done or failed now (the loop's srv_fail_all); wait
+    // until the threads blocked in lg_srv_wait have taken their records,
+    // then drop the records nobody collected
+    std::unique_lock<std::mutex> lk(S->mu);
     S->cv_done.notify_all();
+    S->cv_done.wait(lk, [&] { return S->waiters == 0; });
+    for (auto& kv : S->reqs) delete kv.second;
+    S->reqs.clear();
   }
-  for (auto& kv : S->reqs) delete kv.second;
-  S->reqs.clear();
   S->pool.destroy();
   if (S->d_map) (void)hipFree(S->d_map);
   if (S->h_map) (void)hipHostFree(S->h_map);

@@ -82,7 +82,7 @@ typedef struct LgParams {
   int32_t qp_admm_min_iter;
   float   qp_admm_rho;
   int32_t use_adaptive_search;
-  int32_t stagnation_window;
+  int32_t stagnation_window;  /* 1..64 with use_adaptive_search (else LG_ERR_ARG) */
   int32_t min_iter_to_call_adaptive_search;
   int32_t cooltime_to_call_adaptive_search;
   float   stagnation_threshold;
@@ -332,7 +332,8 @@ int lg_srv_submit(LgServer* s, const LgGridDesc* g, const LgCellsSoA* c, const L
 /* LG_SRV_* state of a ticket that has not been collected by lg_srv_wait. */
 int lg_srv_poll(LgServer* s, int64_t ticket, int32_t* state);
 /* Blocks until the request is done, fills `r` and releases the ticket.
- * Returns the request's status (its message through lg_last_error). */
+ * Returns the request's status (its message through lg_last_error).  One
+ * thread per ticket: a second concurrent wait returns LG_ERR_STATE. */
 int lg_srv_wait(LgServer* s, int64_t ticket, LgSrvResult* r);
 int lg_srv_counters(LgServer* s, LgSrvCounters* out);
 /* Instrumentation: with enable != 0, every later batched iteration brackets
@@ -343,8 +344,11 @@ int lg_srv_counters(LgServer* s, LgSrvCounters* out);
  * (12 per pixel + 24 per cell and iteration) since the last enable. */
 int lg_srv_profile(LgServer* s, int enable);
 int lg_srv_profile_read(LgServer* s, double* ms, int64_t* iterations, double* bytes);
-/* Stops the server: queued and running requests are dropped (their waiters
- * get LG_ERR_STATE), device memory is released. */
+/* Stops the server: queued and running requests are dropped; threads
+ * blocked in lg_srv_wait return LG_ERR_STATE (or their result, for a request
+ * that finished first) and lg_srv_destroy returns only after every such
+ * thread has left lg_srv_wait.  Device memory is released.  Calls on the
+ * server that start after lg_srv_destroy began are undefined. */
 int lg_srv_destroy(LgServer* s);
 
 #ifdef __cplusplus

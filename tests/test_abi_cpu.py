@@ -121,3 +121,25 @@ def test_server_rejects_bad_arguments_without_a_device(library):
     assert L.lg_srv_wait(None, 1, None) == 1
     assert L.lg_srv_poll(None, 1, None) == 1
     assert L.lg_srv_destroy(None) == 0
+
+
+def test_create_rejects_bad_regions_and_window_without_a_device(library):
+    """With the adaptive search on, pixel region ids index the region
+    bounds: ids outside [-1, num_regions) and a stagnation window above the
+    device's 64 entries are argument errors, found before any device call."""
+    from admm_legalization_amd import synth
+    d = synth.tiny(num_cells=60, x_size=48, y_size=12, seed=2)
+    grid, cells, p = lg.device_inputs(d)
+    p.use_adaptive_search = 1
+    L = lg.lib()
+    bad = dict(grid)
+    bad["region"] = grid["region"].copy()
+    bad["region"][5] = len(grid["region_x_min"]) + 3
+    g, c, keep = lg.marshal_device_inputs(bad, cells)
+    h = C.c_void_p()
+    assert L.lg_create(C.byref(g), C.byref(c), C.byref(p), C.byref(h)) == 1
+    assert b"region id" in L.lg_last_error()
+    g, c, keep = lg.marshal_device_inputs(grid, cells)
+    p.stagnation_window = 65
+    assert L.lg_create(C.byref(g), C.byref(c), C.byref(p), C.byref(h)) == 1
+    assert b"stagnation_window" in L.lg_last_error()

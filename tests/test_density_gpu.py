@@ -163,7 +163,9 @@ def test_density_repeat_is_stable(gpu):
         res.append((gx.cpu().numpy(), gy.cpu().numpy(), dg.overflow()))
         assert _rel(res[-1][0], o["gx"]) <= 1e-4 and _rel(res[-1][1], o["gy"]) <= 1e-4
     dg.close()
-    assert _rel(res[0][0], res[1][0].astype(np.float64)) <= 1e-6
+    # the bin sums are FP32 atomics in LDS, added in a different order on
+    # every run (a bin of k terms moves by up to ~k ulps): equal to 1e-5
+    assert _rel(res[0][0], res[1][0].astype(np.float64)) <= 1e-5
     assert abs(res[0][2] - res[1][2]) <= 1e-6
 
 

@@ -144,3 +144,33 @@ def test_zero_iteration_request(cases):
     with LegalizationServer(max_slots=2) as srv:
         got = srv.wait(srv.submit(g, c, p0))
     _same(got, want, "max_admm_iter=0")
+
+
+def test_destroy_while_a_thread_waits(gpu):
+    """lg_srv_destroy with a thread blocked in lg_srv_wait: the waiter gets
+    LG_ERR_STATE (or its result, had it finished first) and destroy returns
+    only after it has left the call."""
+    import threading
+    from admm_legalization_amd.server import LegalizationServer, LegalizerError
+    d = synth.plain_20k(0)  # thousands of iterations: still running at destroy
+    g, c, p = lg.device_inputs(d)
+    srv = LegalizationServer(max_slots=4, iters_per_sync=8)
+    t1 = srv.submit(g, c, p)
+    t2 = srv.submit(g, c, p)
+    out = {}
+
+    def waiter():
+        try:
+            out["result"] = srv.wait(t1)
+        except LegalizerError as e:
+            out["error"] = str(e)
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    import time
+    time.sleep(0.3)
+    srv.close()
+    th.join(timeout=60)
+    assert not th.is_alive()
+    assert "error" in out and "stopped" in out["error"], out
+    print(f"[case] destroy while waiting: {out}", flush=True)
