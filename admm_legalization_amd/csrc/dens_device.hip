@@ -28,6 +28,8 @@
 // multiplyW folded into the sine matrices.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -614,8 +616,15 @@ void poisson_tables(int M, int N, std::vector<float>& CN, std::vector<float>& CM
   }
 }
 
+// workspace of one solve: U [B], A' [B], [V; Vx] [2B]
+struct PoissonWork {
+  float *U, *Ap, *V;
+};
+
 int poisson_chain(PoissonHandle* h, const float* rho, float* psi, float* ex, float* ey, int prec, hipStream_t s,
-                  int tile);
+                  int tile, PoissonWork w);
+
+PoissonWork own_work(PoissonHandle* h) { return PoissonWork{h->U, h->Ap, h->V}; }
 
 // Autotuner: the first solve of a precision times the whole 5-GEMM chain with
 // every tile shape on the caller's own buffers (two runs each, the second
@@ -627,9 +636,9 @@ int poisson_tune(PoissonHandle* h, const float* rho, float* psi, float* ex, floa
   int best = 0, st = DENS_OK;
   float best_ms = 0.f;
   for (int t = 1; t <= kNumTiles && !st; t++) {
-    if ((st = poisson_chain(h, rho, psi, ex, ey, prec, s, t))) break;
+    if ((st = poisson_chain(h, rho, psi, ex, ey, prec, s, t, own_work(h)))) break;
     if (hipEventRecord(e0, s) != hipSuccess) st = fail(DENS_ERR_DEVICE, "hipEventRecord");
-    if (!st) st = poisson_chain(h, rho, psi, ex, ey, prec, s, t);
+    if (!st) st = poisson_chain(h, rho, psi, ex, ey, prec, s, t, own_work(h));
     if (!st && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
       st = fail(DENS_ERR_DEVICE, "tuning events");
     float ms = 0.f;
@@ -656,30 +665,60 @@ int poisson_run(PoissonHandle* h, const float* rho, float* psi, float* ex, float
     }
     tile = h->tile[prec];
   }
-  return poisson_chain(h, rho, psi, ex, ey, prec, s, tile);
+  return poisson_chain(h, rho, psi, ex, ey, prec, s, tile, own_work(h));
 }
 
+// ---- the allocation-free solve's workspace -------------------------------
+// One static array per GPU the library runs on (held until the process
+// ends), shared by every poisson_solve_noalloc call on that GPU.  The host
+// name `scratch` is only a stand-in: kernels get the device address that
+// hipGetSymbolAddress returns for the current device.
+constexpr size_t kNoAllocFloats = (size_t)4 << 20;  // 16 MiB: 4 * M * N <= 4M, e.g. 1024 x 1024 bins
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) float scratch[kNoAllocFloats];
+float* g_scratch_dev[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+int scratch_for_current_device(float** out) {
+  int dev = 0;
+  DENS_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(DENS_ERR_DEVICE, "device index beyond the library's table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(DENS_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_dev[dev] = (float*)p;
+  }
+  *out = g_scratch_dev[dev];
+  return DENS_OK;
+}
+
+// Every element of U, A' and [V; Vx] is written before it is read, so the
+// workspace needs no clearing between solves.
 int poisson_chain(PoissonHandle* h, const float* rho, float* psi, float* ex, float* ey, int prec, hipStream_t s,
-                  int tile) {
+                  int tile, PoissonWork w) {
   const int M = h->M, N = h->N;
   int st;
   Gemm g{};
   // U = CN * rho^T
-  g = Gemm{h->CN, rho, h->U, nullptr, N, N, M, N, N, M, N, 1.f, nullptr, nullptr};
+  g = Gemm{h->CN, rho, w.U, nullptr, N, N, M, N, N, M, N, 1.f, nullptr, nullptr};
   if ((st = launch_gemm(prec, g, s, tile))) return st;
   // A' = 4/(MN) CM * U^T / (wy^2 + wx^2)
-  g = Gemm{h->CM, h->U, h->Ap, nullptr, M, M, N, M, M, N, M, (float)(4.0 / ((double)M * N)), h->wy2, h->wx2};
+  g = Gemm{h->CM, w.U, w.Ap, nullptr, M, M, N, M, M, N, M, (float)(4.0 / ((double)M * N)), h->wy2, h->wx2};
   if ((st = launch_gemm(prec, g, s, tile))) return st;
   // [V; Vx] = [DNt; SNw] * A'^T
-  g = Gemm{h->A3, h->Ap, h->V, nullptr, N, N, M, 2 * N, 2 * N, M, N, 1.f, nullptr, nullptr};
+  g = Gemm{h->A3, w.Ap, w.V, nullptr, N, N, M, 2 * N, 2 * N, M, N, 1.f, nullptr, nullptr};
   if ((st = launch_gemm(prec, g, s, tile))) return st;
   // [psi; Ey] = [DMt; SMw] * V^T
-  g = Gemm{h->A4, h->V, psi, ey, M, M, N, M, 2 * M, N, M, 1.f, nullptr, nullptr};
+  g = Gemm{h->A4, w.V, psi, ey, M, M, N, M, 2 * M, N, M, 1.f, nullptr, nullptr};
   if ((st = launch_gemm(prec, g, s, tile))) return st;
   // Ex = DMt * Vx^T
-  g = Gemm{h->A4, h->V + (size_t)N * M, ex, nullptr, M, M, N, M, M, N, M, 1.f, nullptr, nullptr};
+  g = Gemm{h->A4, w.V + (size_t)N * M, ex, nullptr, M, M, N, M, M, N, M, 1.f, nullptr, nullptr};
   return launch_gemm(prec, g, s, tile);
 }
+
 
 }  // namespace
 
@@ -719,6 +758,24 @@ int poisson_solve(PoissonHandle* h, const float* rho, float* psi, float* ex, flo
   if (!h || !rho || !psi || !ex || !ey) return fail(DENS_ERR_ARG, "null argument");
   hipStream_t s = (hipStream_t)stream;
   return poisson_run(h, rho, psi, ex, ey, prec, s);
+}
+
+int poisson_solve_noalloc(PoissonHandle* h, const float* rho, float* psi, float* ex, float* ey, int32_t prec,
+                          void* stream) {
+  if (!h || !rho || !psi || !ex || !ey) return fail(DENS_ERR_ARG, "null argument");
+  if (prec < 0 || prec > 2) return fail(DENS_ERR_ARG, "unknown precision");
+  const size_t B = (size_t)h->M * h->N;
+  if (4 * B > kNoAllocFloats)
+    return fail(DENS_ERR_ARG, "poisson_solve_noalloc: " + std::to_string(h->N) + " x " + std::to_string(h->M) +
+                                  " bins need " + std::to_string(16 * B) + " bytes of workspace, more than its " +
+                                  std::to_string(4 * kNoAllocFloats) + "-byte static array");
+  float* w = nullptr;
+  int st = scratch_for_current_device(&w);
+  if (st) return st;
+  // the tile of poisson_solve without tuning: fixed, or tuned before, or
+  // 64 x 64; every shape gives the same bits
+  int tile = h->tile_mode ? h->tile_mode : (h->tile[prec] ? h->tile[prec] : 1);
+  return poisson_chain(h, rho, psi, ex, ey, prec, (hipStream_t)stream, tile, PoissonWork{w, w + B, w + 2 * B});
 }
 
 int poisson_set_tile(PoissonHandle* h, int32_t tile) {

@@ -44,7 +44,8 @@ class DensGeom(C.Structure):
 EXPORTED = ["dens_last_error", "dens_prepare", "dens_create", "dens_compute_grad", "dens_get_overflow",
             "dens_get_bin_density", "dens_get_potential", "dens_get_field_x", "dens_get_field_y",
             "dens_get_preconditioner", "dens_get_movable_area", "dens_get_filler_area",
-            "dens_set_precision", "dens_destroy", "poisson_create", "poisson_solve", "poisson_set_tile",
+            "dens_set_precision", "dens_destroy", "poisson_create", "poisson_solve", "poisson_solve_noalloc",
+            "poisson_set_tile",
             "poisson_get_tile", "poisson_destroy"]
 
 _bound = None
@@ -72,6 +73,7 @@ def lib():
         L.dens_destroy.argtypes = [C.c_void_p]
         L.poisson_create.argtypes = [C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]
         L.poisson_solve.argtypes = [C.c_void_p] + [C.c_void_p] * 4 + [C.c_int32, C.c_void_p]
+        L.poisson_solve_noalloc.argtypes = [C.c_void_p] + [C.c_void_p] * 4 + [C.c_int32, C.c_void_p]
         L.poisson_set_tile.argtypes = [C.c_void_p, C.c_int32]
         L.poisson_get_tile.argtypes = [C.c_void_p, C.c_int32, _i32p, _f32p]
         L.poisson_destroy.argtypes = [C.c_void_p]

@@ -131,6 +131,18 @@ int poisson_create(int32_t num_bin_x, int32_t num_bin_y, PoissonHandle** out);
  * own solver for all three precisions up front. */
 int poisson_solve(PoissonHandle* h, const float* d_rho, float* d_psi, float* d_ex, float* d_ey,
                   int32_t precision, void* stream);
+/* poisson_solve without any device-memory allocation, for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * solve's workspace (4 * num_bin_x * num_bin_y floats) is a static array of
+ * 16 MiB per GPU, held until the process ends: grids of more than 1024 x 1024
+ * bins (4M floats) are rejected with DENS_ERR_ARG.  The tuner never runs
+ * here: the tile shape is the fixed or already tuned one, else 64 x 64, so
+ * the results are poisson_solve's bit for bit.  Every call on one GPU shares
+ * that GPU's array: calls that overlap on one GPU race (serialise them, e.g.
+ * on one stream).  Nothing in the array needs clearing between calls. */
+int poisson_solve_noalloc(PoissonHandle* h, const float* d_rho, float* d_psi, float* d_ex, float* d_ey,
+                          int32_t precision, void* stream);
 /* GEMM tile shape: 0 = tuned per precision (default), 1 = 64x64, 2 = 64x128,
  * 3 = 128x64, 4 = 128x128.  No counterpart in the reference (cuFFT there). */
 int poisson_set_tile(PoissonHandle* h, int32_t tile);
