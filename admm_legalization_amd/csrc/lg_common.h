@@ -27,7 +27,7 @@ struct Ctl {
   float rho;
   float lambda0;
   float step;
-  float pad0;
+  int32_t n_huge;        // partitions above kMaxNpadLds slots sorting in Args::huge this step
   unsigned long long ovf_acc;     // this iteration's overflow (sum over partitions)
   unsigned long long last_ovf;
   unsigned long long demand_sum;  // exact integer demand (rho init)
@@ -45,7 +45,7 @@ struct Ctl {
 
 enum : int32_t {
   kErrNone = 0,
-  kErrPartitionTooDense = 1,  // more than kMaxNpad cells in one partition
+  kErrPartitionTooDense = 1,  // a partition needs more than kMaxNpad sort slots
   kErrUsageRange = 2,         // a pixel usage above 65535 (16-bit LDS packing)
 };
 
@@ -58,7 +58,12 @@ constexpr int kNumDirDefault = 245;
 constexpr int kNumDirPadded = 256;
 constexpr int kNumDirWide = 4093;
 constexpr int kNumDirWidePadded = 4096;
-constexpr int kMaxNpad = 4096;
+// sort slots of a partition: pow2(cells) up to the reference's limit
+// (assert(array_size_padded < 12288), CudaAdmmLegalizer.cu:709-710, lets a
+// power of two reach 8192); up to kMaxNpadLds the sort buffers live in LDS,
+// above it in a global-memory buffer of the partition's own
+constexpr int kMaxNpad = 8192;
+constexpr int kMaxNpadLds = 4096;
 constexpr int kMaxStagnationWindow = 64;  // Ctl::ovf_window entries
 constexpr int kBucketCells = 1024;  // cells per workgroup of k_bucket
 constexpr int kNpadSmall = 1024;

@@ -306,7 +306,9 @@ int launch_partition(LgHandle* h, hipStream_t s) {
   } else
     hipLaunchKernelGGL((k_partition<MODE, 1, 1>), dim3(h->a.K), dim3(kWave), lds0, s, h->a, 0, kNpadSmall);
   if (prof) HIP_TRY(hipEventRecord(e[1], s));
-  size_t lds1 = partition_lds_bytes(h, kMaxNpad, 1);
+  // sort buffers of up to kMaxNpadLds slots in LDS; larger partitions (up
+  // to kMaxNpad) sort in Args::huge
+  size_t lds1 = partition_lds_bytes(h, kMaxNpadLds, 1);
   // the primal step's large class also takes the partitions the four-wave
   // launch deferred (its list, every size)
   const int lo1 = MODE == MODE_PRIMAL ? 1 : kNpadSmall + 1;
@@ -421,8 +423,15 @@ int prepare_host(const LgGridDesc* g, const LgCellsSoA* c, const LgParams* p, Ho
     int w = c->w_grid[i], hh = c->h_grid[i];
     if (w <= 0 || hh <= 0 || w > 4095 || hh > 63 || hp.code[i] <= 0 || hp.code[i] > 8191)
       return fail(LG_ERR_ARG, "cell size or group outside the supported range");
-    if ((w > p->x_hint && hp.ncol > 1) || (hh > p->y_hint && hp.nrow > 1))
-      return fail(LG_ERR_ARG, "cell larger than a partition (x_hint/y_hint)");
+    // A cell wider (taller) than a partition spans partitions of different
+    // colours and, as in the reference (assignPartitionKernel,
+    // AdmmKernels.hpp:150-161), belongs to none: the primal step and the
+    // final refine never move it.  Only when its corner columns can fall on
+    // the same colour through the wrap-around of the colouring (a footprint
+    // across all ncol columns) would the reference hand it to a partition
+    // whose tile does not hold it; such cells are refused.
+    if ((hp.ncol > 1 && w >= (hp.ncol - 1) * p->x_hint + 2) || (hp.nrow > 1 && hh >= (hp.nrow - 1) * p->y_hint + 2))
+      return fail(LG_ERR_DESIGN, "cell spanning every partition column (row): wider than (ncol-1)*x_hint+1 sites");
     if (hp.lxg[i] < 0 || hp.lyg[i] < 0 || hp.lxg[i] + w > g->x_size || hp.lyg[i] + hh > g->y_size)
       return fail(LG_ERR_ARG, "cell outside the grid (not preplaced?)");
     max_h = std::max(max_h, hh);
@@ -452,6 +461,10 @@ int prepare_host(const LgGridDesc* g, const LgCellsSoA* c, const LgParams* p, Ho
   return LG_OK;
 }
 
+// Partitions of one step that can sort above kMaxNpadLds slots at once:
+// each holds more than kMaxNpadLds of the N cells.
+int huge_slots(int N, int K) { return std::max(1, std::min(K, N / (kMaxNpadLds + 1))); }
+
 // Device bytes a handle of `hp` takes (an upper bound of create_from_prep's
 // buffers, each rounded to the pool's 256-byte blocks).
 size_t handle_bytes(const HostPrep& hp) {
@@ -462,6 +475,7 @@ size_t handle_bytes(const HostPrep& hp) {
   b += K * 4 * ((size_t)kMaxNpad + 1 + 1 + 2 + 1 + 1 + 2 + 2);  // buckets, counts, offsets, flags, lists
   b += (3 * (size_t)hp.x_size + 3 * (size_t)hp.y_size + hp.y_size + 4 * hp.rx0.size()) * 4;
   b += (size_t)std::max(hp.p.max_admm_iter, 1) * 4 + sizeof(Ctl);
+  b += (size_t)huge_slots(hp.N, hp.K) * 4 * kMaxNpad * 4;  // sort buffers above kMaxNpadLds slots
   return b + 64 * blk;
 }
 
@@ -477,7 +491,7 @@ int create_from_prep(const HostPrep& hp, DevPool* pool, Ctl* ext_ctl, hipStream_
   {
     // a partition's pixels are staged in LDS (the large size class also
     // holds its sort buffers there)
-    const size_t need = partition_lds_bytes(p->x_hint, p->y_hint, p->sorting_algorithm, kMaxNpad, 1);
+    const size_t need = partition_lds_bytes(p->x_hint, p->y_hint, p->sorting_algorithm, kMaxNpadLds, 1);
     if (need > kMaxLds)
       return fail(LG_ERR_ARG, "partition of x_hint x y_hint pixels needs " + std::to_string(need) +
                                   " bytes of LDS, more than the " + std::to_string(kMaxLds) + " of a workgroup");
@@ -564,6 +578,8 @@ int create_from_prep(const HostPrep& hp, DevPool* pool, Ctl* ext_ctl, hipStream_
   if ((st = halloc(h, &a.sorted, a.K))) return bail(st);
   if ((st = halloc(h, &a.work, 2 * (size_t)a.K))) return bail(st);
   if ((st = halloc(h, &a.act_flags, 2 * (size_t)a.K))) return bail(st);
+  a.huge_slots = huge_slots(N, a.K);
+  if ((st = halloc(h, &a.huge, (size_t)a.huge_slots * 4 * kMaxNpad))) return bail(st);
   {
     int32_t *d_cm, *d_rm;
     if ((st = upload(h, &d_cm, hp.colmap.data(), hp.colmap.size()))) return bail(st);
@@ -732,7 +748,7 @@ int solve_end(LgHandle* h, hipStream_t s) {
   int st;
   if (a.N > 0 && h->p.max_admm_iter > 0) {
     if (h->host_ctl.error == kErrPartitionTooDense)
-      return fail(LG_ERR_STATE, "more than 4096 cells in one partition");
+      return fail(LG_ERR_STATE, "more than 8192 cells in one partition");
     if (h->host_ctl.error == kErrUsageRange)
       return fail(LG_ERR_STATE, "pixel usage above 65535");
     if (h->host_ctl.converged) {

@@ -139,6 +139,8 @@ struct Args {
   int32_t* work;       // [2K] partitions the primal step processes: class 0 (at most
                        //      kNpadSmall cells) from 0, class 1 from K; counts in Ctl::n_work
   int32_t* act_flags;  // [2][K] partition has a pixel with usage > 1 (by iteration parity)
+  int32_t* huge;       // [huge_slots][4 * kMaxNpad] sort buffers of partitions above kMaxNpadLds slots
+  int huge_slots;
   const int32_t* colmap;  // [3][x_size] partition column of x for each fold offset
   const int32_t* rowmap;  // [3][y_size] partition row of y for each fold offset
   int32_t* ovf_flag;   // [N] adaptive search: cell has overflow
@@ -387,6 +389,7 @@ __device__ __forceinline__ void scan_finish(const Args& a, int K, int m, int tot
   while (np < m) np <<= 1;
   ctl->max_cnt = m;
   ctl->npad = np;
+  ctl->n_huge = 0;
   if (np > kMaxNpad) { ctl->error = kErrPartitionTooDense; ctl->done = 1; }
   int it = ctl->iter;
   if (it % a.iter_update_dual_step == 0 && it > a.iter_threshold_dual_update) {
@@ -615,6 +618,7 @@ __global__ __launch_bounds__(1024) void k_refine_scan(Args a) {
     int np = 1;
     while (np < m) np <<= 1;
     ctl->npad = np;
+    ctl->n_huge = 0;
     if (np > kMaxNpad) ctl->error = kErrPartitionTooDense;
   }
 }
@@ -1733,9 +1737,28 @@ __device__ __forceinline__ void partition_body(const Args& a, int bx, int npad_l
   T.t = reinterpret_cast<float*>(T.uc + tsz);                  // [rs * yh]
   T.lam = T.t + tsz;                                           // [rs * yh]
   T.vdd = reinterpret_cast<int*>(T.lam + tsz);                 // [yh]
+  // LDS sorts: key / slot / buffer arrays of lds_hi slots in LDS, or, for a
+  // partition above kMaxNpadLds slots, in its own global-memory buffer
+  const int lds_hi = npad_hi < kMaxNpadLds ? npad_hi : kMaxNpadLds;
   int* keys = T.vdd + ((g.yh + 3) & ~3);                       // LDS sorts only
-  int* slots = keys + npad_hi;
-  int* aux = slots + npad_hi;
+  int* slots = keys + lds_hi;
+  int* aux = slots + lds_hi;
+  int sort_hi = lds_hi;
+  if (LDS_SORT && npad > kMaxNpadLds) {
+    int hs = 0;
+    if (tid == 0) hs = atomicAdd(&ctl->n_huge, 1);
+    hs = __shfl(hs, 0);
+    // every such partition holds more than kMaxNpadLds distinct cells, so
+    // at most N / (kMaxNpadLds + 1) of them run at once: huge_slots
+    if (hs >= a.huge_slots) {
+      if (tid == 0) ctl->error = kErrPartitionTooDense;
+      return;
+    }
+    keys = a.huge + (size_t)hs * 4 * kMaxNpad;
+    slots = keys + kMaxNpad;
+    aux = slots + kMaxNpad;
+    sort_hi = kMaxNpad;
+  }
   // exchange slots of the waves (four-wave launches, which have no LDS
   // sorts): block-wide or, code range, and the wave minima of the moves
   int* xch = keys;                                             // [64]
@@ -1838,7 +1861,7 @@ __device__ __forceinline__ void partition_body(const Args& a, int bx, int npad_l
     if (from_bucket) {
       // an unsorted partition at the final refine moved nothing in the last
       // iteration, so its usage still is the one the order is defined on
-      if (LDS_SORT) order_partition_lds(a, g, T, npad, npad_hi, src, off, count, cellrec, keys, slots, aux);
+      if (LDS_SORT) order_partition_lds(a, g, T, npad, sort_hi, src, off, count, cellrec, keys, slots, aux);
       else if (NW > 1 && npad <= NT) {
         // one slot per thread
 #ifdef LG_PHASE_TIMERS

@@ -146,7 +146,7 @@ int srv_build_tables(LgServer* S) {
       S->any_nonreg = 1;
       S->lds_small = std::max(S->lds_small, partition_lds_bytes(h, kNpadSmall, 1));
     }
-    S->lds_big = std::max(S->lds_big, partition_lds_bytes(h, kMaxNpad, 1));
+    S->lds_big = std::max(S->lds_big, partition_lds_bytes(h, kMaxNpadLds, 1));
   }
   size_t total = 0;
   MapRange* r[5] = {&S->m256, &S->m1, &S->mk, &S->mpix, &S->mbk};

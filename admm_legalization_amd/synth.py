@@ -233,6 +233,24 @@ def stream_request(n: int, util: float, seed: int) -> Design:
     return mixed(n, util=util, seed=seed, num_fences=3 if n >= 20000 else 0, name=f"req{seed}")
 
 
+def with_wide_cells(d: Design, widths=(260, 270, 280, 290), seed: int = 1) -> Design:
+    """A copy of `d` in which a few single-row movable cells are wider than a
+    partition (x_hint = 250 sites): such cells belong to no partition, so the
+    ADMM steps never move them."""
+    rng = np.random.default_rng(seed)
+    mv = np.flatnonzero(d.movable_mask() & (d.inst_dy == d.row_height) & (d.inst_group < 0))
+    pick = rng.choice(mv, len(widths), replace=False)
+    sw = d.site_width
+    e = d.copy_positions(d.inst_lx, d.inst_ly)
+    e.inst_dx = d.inst_dx.copy()
+    e.inst_lx = d.inst_lx.copy()
+    for i, w in zip(pick, widths):
+        e.inst_dx[i] = w * sw
+        e.inst_lx[i] = min(int(e.inst_lx[i]), int(d.core[2]) - w * sw)
+    e.meta = dict(d.meta, wide_cells=[int(i) for i in pick], name=d.meta.get("name", "design") + "_wide")
+    return e
+
+
 def _calibrate_width(n, free_sites, util):
     mean_h = sum(h * p for h, p in MIXED_HEIGHTS.items())
     target_w = util * free_sites / n / mean_h

@@ -143,3 +143,22 @@ def test_create_rejects_bad_regions_and_window_without_a_device(library):
     p.stagnation_window = 65
     assert L.lg_create(C.byref(g), C.byref(c), C.byref(p), C.byref(h)) == 1
     assert b"stagnation_window" in L.lg_last_error()
+
+
+def test_create_refuses_cells_across_every_partition_column(library):
+    """A cell whose footprint can span every partition column would, through
+    the colouring's wrap-around, be handed to a partition whose tile does not
+    hold it: lg_create refuses it (LG_ERR_DESIGN) before any device call.
+    Narrower cells wider than x_hint are accepted (they never move)."""
+    from admm_legalization_amd import synth
+    d = synth.tiny(num_cells=60, x_size=48, y_size=12, seed=2)
+    grid, cells, p = lg.device_inputs(d)
+    p.x_hint = 20  # three partition columns on 48 sites
+    c2 = {k: v.copy() for k, v in cells.items()}
+    c2["w_grid"][0] = 42
+    c2["lx_dbu"][0] = grid["core_lx"]
+    g, c, keep = lg.marshal_device_inputs(grid, c2)
+    h = C.c_void_p()
+    L = lg.lib()
+    assert L.lg_create(C.byref(g), C.byref(c), C.byref(p), C.byref(h)) == 3  # LG_ERR_DESIGN
+    assert b"every partition column" in L.lg_last_error()

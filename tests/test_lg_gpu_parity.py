@@ -258,3 +258,27 @@ def test_strict_check_planted_violations(gpu):
     want = numpy_strict_counts(grid, c2, lx, ly, np.zeros_like(lx))
     assert {k: vio[k] for k in want} == want
     assert want["num_site_vio"] >= 2 and want["num_fence_vio"] >= 2 and want["num_rail_vio"] >= 2
+
+
+def test_cells_wider_than_a_partition(gpu):
+    """Four cells of 260-290 sites (x_hint 250) on a three-column grid: they
+    belong to no partition and stay put; everything else equals the oracle."""
+    d = synth.with_wide_cells(synth.mixed(3000, seed=8, util=0.7))
+    p = lg.default_params()
+    p.max_admm_iter = 200
+    g, o = _run_both(d, p, do_post=False)
+    _assert_same(g, o)
+
+
+@pytest.mark.parametrize("algo", ["M_BITONIC", "S_MERGE", "CUB"])
+def test_partition_above_lds_sort_size(gpu, algo):
+    """A pile of 5254 cells in one partition: 8192 sort slots, beyond the
+    4096 an LDS sort holds, up to the reference's limit; the partition sorts
+    in a global-memory buffer of its own and everything equals the oracle."""
+    d = synth.make_design(9000, 300, 48, width_range=(1, 1), num_macros=0, hotspots=1, hotspot_frac=0.7,
+                          hotspot_sigma_rows=0.6, seed=3, name="dense_pile")
+    p = lg.default_params()
+    p.max_admm_iter = 40
+    p.sorting_algorithm = SORT_ALGO[algo]
+    g, o = _run_both(d, p, do_post=False)
+    _assert_same(g, o)

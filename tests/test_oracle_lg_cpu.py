@@ -393,3 +393,20 @@ def test_oracle_strict_counts_on_planted_violations():
     want2 = numpy_strict_counts(grid, cells, o2["lx"], o2["ly"], o2["rot"])
     assert {k: o2["stats"]["vio"][k] for k in want2} == want2
     assert all(v == 0 for v in want2.values()), want2
+
+
+def test_cells_wider_than_a_partition_never_move():
+    """Cells wider than x_hint span partitions of different colours, belong to
+    none (assignPartitionKernel, AdmmKernels.hpp:150-161) and keep their
+    preplaced position through the ADMM steps."""
+    d = synth.with_wide_cells(synth.mixed(3000, seed=8, util=0.7))
+    grid, cells, p = lg.device_inputs(d)
+    p.max_admm_iter = 200
+    o = ref.solve(grid, cells, p, do_post=False)
+    wide = np.flatnonzero(cells["w_grid"] > p.x_hint)
+    assert len(wide) == 4 and grid["x_size"] > 2 * p.x_hint
+    sw, rh = grid["site_width"], grid["row_height"]
+    np.testing.assert_array_equal(o["lx"][wide], (cells["lx_dbu"][wide] - grid["core_lx"]) // sw)
+    np.testing.assert_array_equal(o["ly"][wide], (cells["ly_dbu"][wide] - grid["core_ly"]) // rh)
+    moved = (o["lx"] != (cells["lx_dbu"] - grid["core_lx"]) // sw) | (o["ly"] != (cells["ly_dbu"] - grid["core_ly"]) // rh)
+    assert moved.sum() > 100  # the others do move
