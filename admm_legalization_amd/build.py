@@ -15,6 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIBDIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIBDIR, "libskyline_amd.so")
+CMD = os.path.join(LIBDIR, "skyline_cmd")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 
@@ -63,6 +64,10 @@ def build(verbose: bool = False) -> str:
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fopenmp", "-o", tmp, *objs,
           f"-Wl,-rpath,{ROCM}/lib", f"-Wl,-rpath,{ROCM}/lib/llvm/lib"], verbose)
     os.replace(tmp, LIB)
+    # the command driver (skyline_cmd: read_design / legalize / write_result)
+    _run(["g++", "-O2", "-std=c++17", os.path.join(CSRC, "skyline_cmd.cpp"), "-L", LIBDIR, "-lskyline_amd",
+          f"-Wl,-rpath,{LIBDIR}", "-o", CMD + ".tmp"], verbose)
+    os.replace(CMD + ".tmp", CMD)
     return LIB
 
 

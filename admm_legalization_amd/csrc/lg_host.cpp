@@ -7,11 +7,13 @@
 // x-major arrays (no per-pixel objects) and preplace runs on all host cores.
 #include <algorithm>
 #include <array>
+#include <cctype>
 #include <chrono>
 #include <climits>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
 #include <map>
 #include <set>
 #include <string>
@@ -87,7 +89,104 @@ struct Session {
   int max_height = 0;
   std::vector<std::array<int, 2>> dirs;  // default candidate moves
   int preplace_fail = 0;
+  std::map<int, int> hdist;  // cell height (dbu) -> movable cells (height_distribution_)
+  float max_util = 0.f, max_disp = 0.f;  // constraint file: percent, dbu
 };
+
+// tokenize (Util.cpp:22-47): split at the delimiters and at white space
+std::vector<std::string> tokens_of(const std::string& line, const char* dels) {
+  std::vector<std::string> out;
+  std::string t;
+  for (char ch : line) {
+    if (std::strchr(dels, ch) || std::isspace((unsigned char)ch)) {
+      if (!t.empty()) out.push_back(std::move(t));
+      t.clear();
+    } else {
+      t.push_back(ch);
+    }
+  }
+  if (!t.empty()) out.push_back(std::move(t));
+  return out;
+}
+
+// parseSizeFile (Util.cpp:220-234): "name=width_in_sites=height_in_rows"
+int read_size_file(const char* path, std::unordered_map<std::string, std::pair<int, int>>& out) {
+  std::ifstream in(path);
+  if (!in) { g_host_err = std::string("cannot open size file ") + path; return LG_ERR_ARG; }
+  std::string line;
+  while (std::getline(in, line)) {
+    const auto t = tokens_of(line, "=");
+    if (t.empty()) continue;
+    if (t.size() < 3) { g_host_err = "size file: bad line '" + line + "'"; return LG_ERR_ARG; }
+    char* e1 = nullptr;
+    char* e2 = nullptr;
+    const long w = std::strtol(t[1].c_str(), &e1, 10), h = std::strtol(t[2].c_str(), &e2, 10);
+    if (*e1 || *e2 || w <= 0 || h <= 0) { g_host_err = "size file: bad size in '" + line + "'"; return LG_ERR_ARG; }
+    out[t[0]] = {(int)w, (int)h};
+  }
+  return LG_OK;
+}
+
+// parseICCAD17Constraint (Util.cpp:190-217): maximum_utilization=U% and
+// maximum_movement=Mrows (rows -> dbu through the row height)
+int read_constraints(const char* path, int row_height, float& util, float& disp) {
+  std::ifstream in(path);
+  if (!in) { g_host_err = std::string("cannot open constraint file ") + path; return LG_ERR_ARG; }
+  std::string line;
+  while (std::getline(in, line)) {
+    const auto t = tokens_of(line, "=");
+    if (t.size() < 2) continue;
+    const std::string& v = t.back();
+    if (t.front() == "maximum_utilization" && v.size() > 1) util = std::strtof(v.substr(0, v.size() - 1).c_str(), nullptr);
+    if (t.front() == "maximum_movement" && v.size() > 4)
+      disp = std::strtof(v.substr(0, v.size() - 4).c_str(), nullptr) * (float)row_height;
+  }
+  return LG_OK;
+}
+
+// computeHpwlFromDB (Legalizer.cpp:402-437) on the given instance positions
+// and flip flags (FS mirrors a pin's y, dbITerm::getAvgXY); nets without
+// pins add nothing.
+int64_t design_hpwl(const LgDesign* d, const int32_t* lx, const int32_t* ly, const uint8_t* flipped) {
+  int64_t sum = 0;
+  for (int k = 0; k < d->num_nets; k++) {
+    int x0 = INT_MAX, y0 = INT_MAX, x1 = INT_MIN, y1 = INT_MIN;
+    for (int p = d->net_start[k]; p < d->net_start[k + 1]; p++) {
+      const int i = d->pin_inst[p];
+      int x, y;
+      if (i < 0) {
+        x = d->pin_dx[p];
+        y = d->pin_dy[p];
+      } else {
+        x = lx[i] + d->pin_dx[p];
+        y = (flipped && flipped[i]) ? ly[i] + d->inst_dy[i] - d->pin_dy[p] : ly[i] + d->pin_dy[p];
+      }
+      x0 = std::min(x0, x); x1 = std::max(x1, x);
+      y0 = std::min(y0, y); y1 = std::max(y1, y);
+    }
+    if (x0 <= x1) sum += (int64_t)(x1 - x0) + (int64_t)(y1 - y0);
+  }
+  return sum;
+}
+
+int check_netlist(const LgDesign* d) {
+  if (d->num_nets < 0 || (d->num_nets > 0 && (!d->net_start || !d->pin_inst || !d->pin_dx || !d->pin_dy))) {
+    g_host_err = "netlist arrays missing";
+    return LG_ERR_ARG;
+  }
+  for (int k = 0; k < d->num_nets; k++)
+    if (d->net_start[k] > d->net_start[k + 1] || d->net_start[k] < 0) {
+      g_host_err = "net_start is not a non-decreasing offset array";
+      return LG_ERR_ARG;
+    }
+  const int np = d->num_nets > 0 ? d->net_start[d->num_nets] : 0;
+  for (int p = 0; p < np; p++)
+    if (d->pin_inst[p] < -1 || d->pin_inst[p] >= d->num_insts) {
+      g_host_err = "pin instance index out of range";
+      return LG_ERR_ARG;
+    }
+  return LG_OK;
+}
 
 void add_region(Session& S, const Rgn& r) {
   S.by_group[r.group].push_back((int)S.regions.size());
@@ -105,7 +204,17 @@ int import_design(Session& S, bool pixels = true) {
   S.core_lx = d->core_lx; S.core_ly = d->core_ly; S.core_ux = d->core_ux; S.core_uy = d->core_uy;
   const int row_h = d->row_dy[d->num_rows - 1];
   std::vector<std::array<int, 4>> blk_macro, blk_std;
-  std::map<int, int> hdist;
+  std::map<int, int>& hdist = S.hdist;
+  hdist.clear();
+  // modified-ISPD2015 cell sizes (setSizeByFile)
+  std::unordered_map<std::string, std::pair<int, int>> sizes;
+  const bool resize = d->size_file && d->size_file[0];
+  if (resize) {
+    if (!d->inst_name) { g_host_err = "size_file needs inst_name"; return LG_ERR_ARG; }
+    int st = read_size_file(d->size_file, sizes);
+    if (st) return st;
+  }
+  const int site_w = d->row_site_width[d->num_rows - 1];
   for (int i = 0; i < d->num_insts; i++) {
     if (d->inst_dx[i] == 0 || d->inst_dy[i] < row_h) continue;
     std::array<int, 4> r = {d->inst_lx[i], d->inst_ly[i], d->inst_lx[i] + d->inst_dx[i],
@@ -118,6 +227,13 @@ int import_design(Session& S, bool pixels = true) {
     } else {
       MovCell c{r[0], r[1], d->inst_dx[i], d->inst_dy[i], r[0], r[1],
                 d->inst_group ? d->inst_group[i] : -1, d->inst_ground_at_bottom[i] ? 1 : 0, i};
+      if (resize && d->inst_name[i]) {
+        auto it = sizes.find(d->inst_name[i]);
+        if (it != sizes.end()) {
+          c.w = it->second.first * site_w;
+          c.h = it->second.second * row_h;
+        }
+      }
       S.cells.push_back(c);
       hdist[c.h]++;
     }
@@ -511,8 +627,15 @@ int lg_run_design(const LgDesign* d, const LgParams* p, LgResult* r) {
   Session S;
   S.p = *p;
   S.d = d;
-  int st = import_design(S);
+  int st = check_netlist(d);
   if (st) return st;
+  st = import_design(S);
+  if (st) return st;
+  const int row_h = d->row_dy[d->num_rows - 1];
+  const bool constrained = d->constraint_file && d->constraint_file[0];
+  if (constrained && (st = read_constraints(d->constraint_file, row_h, S.max_util, S.max_disp))) return st;
+  // original_hpwl_ = computeHpwlFromDB() at import (DatabaseIO.cpp:68)
+  const int64_t hpwl0 = design_hpwl(d, d->inst_lx, d->inst_ly, nullptr);
   const double t1 = now_s();
   preplace(S);
   const double t2 = now_s();
@@ -573,6 +696,33 @@ int lg_run_design(const LgDesign* d, const LgParams* p, LgResult* r) {
   s.t_preplace_s = t2 - t1;
   s.t_admm_s = t3 - t2;
   s.t_post_s = t4 - t3;
+  // computeStat's HPWL (Legalizer.cpp:465-472) after exportDB set the
+  // positions and the FS flips
+  const int64_t hpwl1 = design_hpwl(d, r->out_lx, r->out_ly, r->out_flipped);
+  s.hpwl_before_um = double(hpwl0) / dbu;
+  s.hpwl_after_um = double(hpwl1) / dbu;
+  s.delta_hpwl_pct = hpwl0 > 0 ? s.hpwl_after_um / s.hpwl_before_um * 100.0 - 100.0 : 0.0;
+  // printICCAD17Metric (Util.cpp:141-188)
+  {
+    std::map<int, double> sum_sw;
+    int64_t mx_dbu = 0;
+    for (int i = 0; i < N; i++) {
+      const MovCell& c = S.cells[i];
+      const int64_t dd = std::abs(c.ilx - r->out_lx[c.inst]) + std::abs(c.ily - r->out_ly[c.inst]);
+      mx_dbu = std::max(mx_dbu, dd);
+      sum_sw[c.h] += double(dd) / double(g.sw);
+    }
+    double tot = 0.0;
+    for (auto& [hh, v] : sum_sw) tot += v / double(std::max(1, S.hdist[hh]));
+    const double k = double(std::max<size_t>(S.hdist.size(), 1));
+    s.iccad17_s_am = tot / k / (double(row_h) / double(g.sw));
+    s.iccad17_m_max = double(mx_dbu) / double(row_h);
+    const int np = s.vio.num_pin_short_vio + s.vio.num_pin_access_vio, ne = s.vio.num_edge_spacing_vio;
+    const double drv = double(np + ne) / double(std::max(N, 1));
+    s.iccad17_score = (1.0 + s.delta_hpwl_pct / 100.0 + drv) * (1.0 + s.iccad17_m_max / 100.0) * s.iccad17_s_am;
+    s.max_util_constraint = S.max_util;
+    s.max_disp_constraint = S.max_disp;
+  }
   s.t_total_s = now_s() - t0;
   if (p->verbose) {  // Legalizer::printStats (Util.cpp:107-139)
     printf("------------------------------------------\n");
@@ -580,8 +730,28 @@ int lg_run_design(const LgDesign* d, const LgParams* p, LgResult* r) {
     printf("Sum Displacement  : %12.1f um\n", s.total_disp_um);
     printf("Avg Displacement  : %12.1f um\n", s.avg_disp_um);
     printf("Max Displacement  : %12.1f um\n", s.max_disp_um);
+    printf("\n");
+    printf("HPWL Before       : %12.1f um\n", s.hpwl_before_um);
+    printf("HPWL After        : %12.1f um\n", s.hpwl_after_um);
+    printf("Delta HPWL        : %12.2f %%\n", s.delta_hpwl_pct);
+    printf("\n");
+    const double dbu_d = dbu;
+    printf("Avg Displacement  : %12.2f (rows) \n", s.avg_disp_um * dbu_d / row_h);
+    printf("Avg Displacement  : %12.2f (sites)\n", s.avg_disp_um * dbu_d / g.sw);
+    printf("Max Displacement  : %12.2f (rows) \n", s.max_disp_um * dbu_d / row_h);
+    printf("Max Displacement  : %12.2f (sites)\n", s.max_disp_um * dbu_d / g.sw);
+    printf("Sum Displacement  : %12.2f (sites)\n", s.total_disp_um * dbu_d / g.sw);
+    printf("\n");
     printf("NumOverlap        : %12d\n", s.vio.total_ovf);
     printf("NumRowOrientVio   : %12d\n", s.vio.num_row_orient_vio);
+    if (constrained) {
+      printf("\nICCAD 17 Contest Metric\n");
+      printf("N_p   : %6d  \n", s.vio.num_pin_short_vio + s.vio.num_pin_access_vio);
+      printf("N_e   : %6d  \n", s.vio.num_edge_spacing_vio);
+      printf("M_max : %6.3f\n", s.iccad17_m_max);
+      printf("S_am  : %6.3f\n", s.iccad17_s_am);
+      printf("S     : %6.3f\n", s.iccad17_score);
+    }
     printf("preplace           finished (takes %5.2f s)\n", s.t_preplace_s);
     printf("admmLegalize       finished (takes %5.2f s)\n", s.t_admm_s);
     printf("postProcess        finished (takes %5.2f s)\n", s.t_post_s);
@@ -628,6 +798,15 @@ void Legalizer::setSortingAlgorithm(std::string_view s) {
                                                : LG_SORT_M_BITONIC;
 }
 
-int Legalizer::run(LgResult* result) { return lg_run_design(design_, &params_, result); }
+void Legalizer::setICCAD17Constraint(const std::string& file) { constraint_ = file; }
+void Legalizer::setSizeByFile(const std::string& file) { size_file_ = file; }
+
+int Legalizer::run(LgResult* result) {
+  if (!design_) return lg_run_design(nullptr, &params_, result);
+  LgDesign d = *design_;
+  if (!constraint_.empty()) d.constraint_file = constraint_.c_str();
+  if (!size_file_.empty()) d.size_file = size_file_.c_str();
+  return lg_run_design(&d, &params_, result);
+}
 
 }  // namespace legalizer

@@ -1,0 +1,210 @@
+// skyline_cmd: a plain C++ command driver with the reference's Tcl command
+// surface for the legalization flow, for hosts without Tcl.  It reads a
+// script of commands (one per line or separated by ';', '#' starts a
+// comment) and runs them in order:
+//
+//   read_design <file.lgd>         design in the LGD1 format (lgd_file.hpp)
+//   legalize [options]             CmdLegalize (tclCmd/CmdLegalize.hpp:14-152)
+//   write_result <file.lgr>        legalized positions and flips (LGR1)
+//
+// legalize takes the reference's options verbatim: -constraint <file>
+// -tech_penalty <f> -max_iter <i> -std_admm -coeff_to_init_rho <f>
+// -coeff_to_init_dual <f> -qp_refine -size_file <file> -use_adaptive_search
+// -min_iter_to_call_adaptive_search <i> -ovf_ratio_to_call_adaptive_search <f>
+// -coeff_to_neglect_displace <f> -random_sorting_seed <i> -sorting_algorithm
+// <S_MERGE|M_MERGE|S_BITONIC|M_BITONIC|CUB> -sorting_policy
+// <RANDOM|SIZE|OVERFLOW>.  The option grammar is CmdCore's (CmdCore.cpp:
+// 35-84): an option begins with '-', takes at most one argument, and an
+// unknown option or an option where an argument is due is an error.
+//
+// Usage: skyline_cmd <script> | skyline_cmd -c "<commands>"
+// Exit status 0 when every command succeeded, else 1.
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <map>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/skyline/Legalizer.hpp"
+#include "lgd_file.hpp"
+
+namespace {
+
+struct Option {
+  int nargs = 0;
+  bool valid = false;
+  std::string arg;
+};
+
+// A command's option table and the CmdCore parsing rules.
+struct Command {
+  std::map<std::string, Option> opts;
+  std::vector<std::string> args;
+  void add(const char* name, int nargs) { opts[name] = Option{nargs, false, {}}; }
+  bool parse(const std::vector<std::string>& tok, std::string& err) {
+    Option* cur = nullptr;
+    bool want_arg = false;
+    for (size_t i = 1; i < tok.size(); i++) {
+      const std::string& t = tok[i];
+      if (!t.empty() && t[0] == '-') {
+        auto it = opts.find(t);
+        if (it == opts.end()) { err = "Unknown Option : " + t; return false; }
+        if (want_arg) { err = "Syntax error in option argument"; return false; }
+        cur = &it->second;
+        cur->valid = true;
+        want_arg = cur->nargs > 0;
+      } else if (want_arg) {
+        cur->arg = t;
+        cur = nullptr;
+        want_arg = false;
+      } else {
+        args.push_back(t);
+      }
+    }
+    if (want_arg) { err = "Syntax error in command " + tok[0]; return false; }
+    return true;
+  }
+  const Option* get(const char* name) const {
+    auto it = opts.find(name);
+    return it != opts.end() && it->second.valid ? &it->second : nullptr;
+  }
+};
+
+struct Driver {
+  std::unique_ptr<lgd::DesignFile> design;
+  std::vector<int32_t> lx, ly;
+  std::vector<uint8_t> fl;
+  bool legalized = false;
+
+  bool read_design(const std::vector<std::string>& tok) {
+    if (tok.size() != 2) { std::printf("read_design <file.lgd>\n"); return false; }
+    auto f = std::make_unique<lgd::DesignFile>();
+    std::string err;
+    if (!lgd::read(tok[1].c_str(), *f, err)) { std::printf("read_design: %s\n", err.c_str()); return false; }
+    std::printf("read_design: %d instances, %d rows, %d nets\n", f->d.num_insts, f->d.num_rows, f->d.num_nets);
+    design = std::move(f);
+    legalized = false;
+    return true;
+  }
+
+  // CmdLegalize::execute (CmdLegalize.hpp:36-152)
+  bool legalize(const std::vector<std::string>& tok) {
+    Command c;
+    for (const char* o : {"-constraint", "-tech_penalty", "-max_iter", "-coeff_to_init_rho", "-coeff_to_init_dual",
+                          "-size_file", "-min_iter_to_call_adaptive_search", "-ovf_ratio_to_call_adaptive_search",
+                          "-coeff_to_neglect_displace", "-random_sorting_seed", "-sorting_algorithm",
+                          "-sorting_policy"})
+      c.add(o, 1);
+    for (const char* o : {"-std_admm", "-qp_refine", "-use_adaptive_search"}) c.add(o, 0);
+    std::string err;
+    if (!c.parse(tok, err)) { std::printf("%s\n", err.c_str()); return false; }
+    if (!design) { std::printf("legalize: no design (read_design first)\n"); return false; }
+    legalizer::Legalizer lg(&design->d);
+    if (auto o = c.get("-constraint")) {
+      lg.setICCAD17Constraint(o->arg);
+      lg.setFlagICCAD2017(true);
+    }
+    if (auto o = c.get("-tech_penalty")) lg.setTechPenalty(std::stof(o->arg));
+    if (auto o = c.get("-max_iter")) lg.setMaxAdmmIter(std::stoi(o->arg));
+    if (c.get("-std_admm")) lg.setStandardAdmm(true);
+    if (auto o = c.get("-coeff_to_init_rho")) lg.setCoeffToInitRho(std::stof(o->arg));
+    if (auto o = c.get("-coeff_to_init_dual")) lg.setCoeffToInitDual(std::stof(o->arg));
+    if (c.get("-use_adaptive_search")) lg.setAdaptiveSearch(true);
+    if (auto o = c.get("-min_iter_to_call_adaptive_search")) lg.setMinIterToCallAdaptiveSearch(std::stoi(o->arg));
+    if (auto o = c.get("-ovf_ratio_to_call_adaptive_search")) lg.setOvfRatioToCallAdaptiveSearch(std::stof(o->arg));
+    if (auto o = c.get("-coeff_to_neglect_displace")) lg.setCoeffToNeglectDisplace(std::stof(o->arg));
+    if (c.get("-qp_refine")) lg.setQpRefine(true);
+    if (auto o = c.get("-sorting_algorithm")) lg.setSortingAlgorithm(o->arg);
+    if (auto o = c.get("-sorting_policy")) lg.setSortingPolicy(o->arg);
+    if (auto o = c.get("-random_sorting_seed")) lg.setRandomSortingSeed(std::stoi(o->arg));
+    if (auto o = c.get("-size_file")) lg.setSizeByFile(o->arg);
+    lg.params().verbose = 1;
+    const int n = design->d.num_insts;
+    lx.assign(n, 0);
+    ly.assign(n, 0);
+    fl.assign(n, 0);
+    LgResult r{};
+    r.out_lx = lx.data();
+    r.out_ly = ly.data();
+    r.out_flipped = fl.data();
+    const int st = lg.run(&r);
+    if (st) { std::printf("legalize: error %d: %s\n", st, lg_last_error()); return false; }
+    const LgStats& s = r.stats;
+    // one machine-readable summary line after the reference-style report
+    std::printf("[summary] converged=%d iterations=%d total_disp_dbu=%lld max_disp_dbu=%lld overlap=%d "
+                "row_orient_vio=%d site_vio=%d fence_vio=%d core_vio=%d rail_vio=%d hpwl_before_um=%.6f "
+                "hpwl_after_um=%.6f iccad17_score=%.9f\n",
+                s.converged, s.iterations, (long long)s.total_disp_dbu, (long long)s.max_disp_dbu, s.vio.total_ovf,
+                s.vio.num_row_orient_vio, s.vio.num_site_vio, s.vio.num_fence_vio, s.vio.num_core_vio,
+                s.vio.num_rail_vio, s.hpwl_before_um, s.hpwl_after_um, s.iccad17_score);
+    legalized = true;
+    return true;
+  }
+
+  bool write_result(const std::vector<std::string>& tok) {
+    if (tok.size() != 2) { std::printf("write_result <file.lgr>\n"); return false; }
+    if (!legalized) { std::printf("write_result: nothing legalized yet\n"); return false; }
+    if (!lgd::write_result(tok[1].c_str(), (int)lx.size(), lx.data(), ly.data(), fl.data())) {
+      std::printf("write_result: cannot write %s\n", tok[1].c_str());
+      return false;
+    }
+    return true;
+  }
+
+  bool run(const std::vector<std::string>& tok) {
+    if (tok[0] == "read_design") return read_design(tok);
+    if (tok[0] == "legalize") return legalize(tok);
+    if (tok[0] == "write_result") return write_result(tok);
+    std::printf("invalid command name \"%s\"\n", tok[0].c_str());
+    return false;
+  }
+};
+
+std::vector<std::vector<std::string>> commands_of(const std::string& text) {
+  std::vector<std::vector<std::string>> out;
+  std::string line;
+  std::istringstream in(text);
+  while (std::getline(in, line)) {
+    line = line.substr(0, line.find('#'));
+    std::stringstream parts(line);
+    std::string cmd;
+    while (std::getline(parts, cmd, ';')) {
+      std::istringstream ws(cmd);
+      std::vector<std::string> tok;
+      for (std::string t; ws >> t;) tok.push_back(t);
+      if (!tok.empty()) out.push_back(tok);
+    }
+  }
+  return out;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  std::string text;
+  if (argc == 3 && std::string(argv[1]) == "-c") {
+    text = argv[2];
+  } else if (argc == 2) {
+    std::ifstream f(argv[1]);
+    if (!f) { std::printf("cannot open %s\n", argv[1]); return 1; }
+    std::stringstream ss;
+    ss << f.rdbuf();
+    text = ss.str();
+  } else {
+    std::printf("usage: %s <script> | %s -c \"<commands>\"\n", argv[0], argv[0]);
+    return 1;
+  }
+  Driver drv;
+  for (const auto& tok : commands_of(text)) {
+    std::printf("skyline> ");
+    for (const auto& t : tok) std::printf("%s ", t.c_str());
+    std::printf("\n");
+    std::fflush(stdout);
+    if (!drv.run(tok)) return 1;
+    std::fflush(stdout);
+  }
+  return 0;
+}

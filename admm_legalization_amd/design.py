@@ -47,6 +47,14 @@ class Design:
     box_ly: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
     box_ux: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
     box_uy: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    # optional netlist (HPWL): pins of net k are net_start[k] .. net_start[k+1]-1,
+    # at (pin_dx, pin_dy) from instance pin_inst's lower-left (orientation N),
+    # or absolute for pin_inst = -1 (IO pins)
+    net_start: np.ndarray = field(default_factory=lambda: np.zeros(1, np.int32))
+    pin_inst: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    pin_dx: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    pin_dy: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    inst_name: list = None  # optional instance names (size files)
     dbu_per_micron: int = 1000
     meta: dict = field(default_factory=dict)
 
@@ -81,6 +89,41 @@ class Design:
         np.savez_compressed(path, core=np.asarray(self.core, np.int32),
                             num_groups=np.int32(self.num_groups),
                             dbu_per_micron=np.int32(self.dbu_per_micron), **arrays)
+
+    def save_lgd(self, path: str) -> None:
+        """The design in the LGD1 binary format of the command driver
+        (admm_legalization_amd/csrc/lgd_file.hpp)."""
+        i32 = lambda a: np.ascontiguousarray(a, np.int32).tobytes()
+        u8 = lambda a: np.ascontiguousarray(a, np.uint8).tobytes()
+        n, nr = self.num_insts, int(self.row_lx.shape[0])
+        nb = int(np.asarray(self.box_group).shape[0])
+        ns = np.asarray(self.net_start, np.int32)
+        with open(path, "wb") as f:
+            f.write(b"LGD1")
+            f.write(i32([self.dbu_per_micron, *self.core, nr]))
+            for a in (self.row_lx, self.row_ly, self.row_dx, self.row_dy, self.row_site_width, self.row_orient):
+                f.write(i32(a))
+            f.write(i32([n]))
+            for a in (self.inst_lx, self.inst_ly, self.inst_dx, self.inst_dy):
+                f.write(i32(a))
+            f.write(u8(self.inst_fixed))
+            f.write(u8(self.inst_macro))
+            f.write(i32(self.inst_group))
+            f.write(u8(self.inst_ground_at_bottom))
+            f.write(i32([self.num_groups, nb]))
+            for a in (self.box_group, self.box_lx, self.box_ly, self.box_ux, self.box_uy):
+                f.write(i32(a))
+            f.write(i32([max(len(ns) - 1, 0)]))
+            f.write(i32(ns if len(ns) else [0]))
+            f.write(i32([len(self.pin_inst)]))
+            for a in (self.pin_inst, self.pin_dx, self.pin_dy):
+                f.write(i32(a))
+            f.write(i32([1 if self.inst_name is not None else 0]))
+            if self.inst_name is not None:
+                for name in self.inst_name:
+                    b = name.encode()
+                    f.write(i32([len(b)]))
+                    f.write(b)
 
     @staticmethod
     def from_npz(path: str) -> "Design":
@@ -176,6 +219,9 @@ class LgDesign(C.Structure):
         ("num_groups", C.c_int32), ("num_region_boxes", C.c_int32),
         ("box_group", _i32p), ("box_lx", _i32p), ("box_ly", _i32p), ("box_ux", _i32p),
         ("box_uy", _i32p),
+        ("num_nets", C.c_int32), ("net_start", _i32p), ("pin_inst", _i32p), ("pin_dx", _i32p),
+        ("pin_dy", _i32p), ("inst_name", C.POINTER(C.c_char_p)), ("size_file", C.c_char_p),
+        ("constraint_file", C.c_char_p),
     ]
 
 
@@ -205,6 +251,9 @@ class LgStats(C.Structure):
         ("num_adaptive_calls", C.c_int32), ("vio", LgVioStat),
         ("t_import_s", C.c_double), ("t_preplace_s", C.c_double), ("t_admm_s", C.c_double),
         ("t_post_s", C.c_double), ("t_total_s", C.c_double),
+        ("hpwl_before_um", C.c_double), ("hpwl_after_um", C.c_double), ("delta_hpwl_pct", C.c_double),
+        ("max_util_constraint", C.c_double), ("max_disp_constraint", C.c_double),
+        ("iccad17_s_am", C.c_double), ("iccad17_m_max", C.c_double), ("iccad17_score", C.c_double),
     ]
 
 
@@ -235,7 +284,7 @@ def _ptr(a, ctype):
 class MarshalledDesign:
     """Keeps contiguous copies alive while an LgDesign points into them."""
 
-    def __init__(self, d: Design):
+    def __init__(self, d: Design, size_file: str | None = None, constraint_file: str | None = None):
         conv = lambda a, t: np.ascontiguousarray(a, dtype=t)
         self.keep = {}
         s = LgDesign()
@@ -255,6 +304,19 @@ class MarshalledDesign:
         s.num_insts = d.num_insts
         s.num_groups = int(d.num_groups)
         s.num_region_boxes = int(np.asarray(d.box_group).shape[0])
+        ns = np.asarray(d.net_start)
+        s.num_nets = max(int(ns.shape[0]) - 1, 0)
+        for name in ("net_start", "pin_inst", "pin_dx", "pin_dy"):
+            a = conv(getattr(d, name), np.int32)
+            self.keep[name] = a
+            setattr(s, name, _ptr(a, C.c_int32))
+        if d.inst_name is not None:
+            names = (C.c_char_p * d.num_insts)(*[n.encode() for n in d.inst_name])
+            self.keep["inst_name"] = names
+            s.inst_name = C.cast(names, C.POINTER(C.c_char_p))
+        self.keep["files"] = (size_file.encode() if size_file else None,
+                              constraint_file.encode() if constraint_file else None)
+        s.size_file, s.constraint_file = self.keep["files"]
         self.struct = s
 
 

@@ -80,11 +80,15 @@ class Legalizer:
 
     def __init__(self, design: Design):
         self.design = design
+        self.constraint = None
+        self.size_file = None
         self.params = LgParams()
         lib().lg_default_params(C.byref(self.params))
         self.result = None
 
     # -- setters of the reference (Legalizer.cpp:95-195) ---------------------
+    def setICCAD17Constraint(self, path: str): self.constraint = path or None  # Legalizer.h:74
+    def setSizeByFile(self, path: str): self.size_file = path or None  # Legalizer.h:75
     def setTechPenalty(self, v: float): self.params.tech_penalty = float(v)
     def setStandardAdmm(self, v: bool): self.params.standard_admm = int(bool(v))
     def setQpRefine(self, v: bool): self.params.qp_admm_on = int(bool(v))
@@ -109,25 +113,28 @@ class Legalizer:
 
     def run(self, trace_cap: int = 0):
         """Legalizes; returns (lx, ly, flipped, stats) per instance (dbu)."""
-        md = MarshalledDesign(self.design)
+        md = MarshalledDesign(self.design, self.size_file, self.constraint)
         rb = ResultBuffers(self.design.num_insts, trace_cap)
         _check(lib().lg_run_design(C.byref(md.struct), C.byref(self.params), C.byref(rb.struct)))
         self.result = (rb.lx, rb.ly, rb.flipped, rb.stats_dict())
         return self.result
 
 
-def run_design(design: Design, params: LgParams | None = None, trace_cap: int = 0):
+def run_design(design: Design, params: LgParams | None = None, trace_cap: int = 0, size_file: str | None = None,
+               constraint_file: str | None = None):
     lg = Legalizer(design)
     if params is not None:
         lg.params = params
+    lg.setSizeByFile(size_file)
+    lg.setICCAD17Constraint(constraint_file)
     return lg.run(trace_cap)
 
 
-def import_and_preplace(design: Design, params: LgParams | None = None) -> dict:
+def import_and_preplace(design: Design, params: LgParams | None = None, size_file: str | None = None) -> dict:
     """Host import + preplace: pixel maps (x-major), region bounds and the
     preplaced movable-cell positions (dbu).  Pure host code (no device)."""
     params = params or default_params()
-    md = MarshalledDesign(design)
+    md = MarshalledDesign(design, size_file)
     shape = np.zeros(4, np.int32)
     null = C.cast(None, _i32p)
     _check(lib().lg_import_and_preplace(C.byref(md.struct), C.byref(params), _p(shape), null, null,

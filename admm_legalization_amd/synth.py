@@ -251,6 +251,36 @@ def with_wide_cells(d: Design, widths=(260, 270, 280, 290), seed: int = 1) -> De
     return e
 
 
+def with_netlist(d: Design, nets_per_cell: float = 1.2, io_frac: float = 0.02, seed: int = 0,
+                 names: bool = True) -> Design:
+    """A copy of `d` with a random netlist: nets of 2-8 pins (geometric) on
+    the movable cells and fixed macros, pins at integer offsets inside their
+    instances, a few IO pins on the core boundary, and instance names
+    "o<index>"."""
+    rng = np.random.default_rng(seed)
+    e = d.copy_positions(d.inst_lx, d.inst_ly)
+    cand = np.flatnonzero(d.movable_mask() | (d.inst_macro == 1))
+    nn = max(1, int(nets_per_cell * int(d.movable_mask().sum())))
+    deg = np.minimum(2 + rng.geometric(0.45, nn) - 1, 8)
+    start = np.zeros(nn + 1, np.int64)
+    start[1:] = np.cumsum(deg)
+    npins = int(start[-1])
+    inst = rng.choice(cand, npins).astype(np.int32)
+    io = rng.random(npins) < io_frac
+    dx = (rng.random(npins) * d.inst_dx[inst]).astype(np.int32)
+    dy = (rng.random(npins) * d.inst_dy[inst]).astype(np.int32)
+    lx, ly, ux, uy = d.core
+    side = rng.integers(0, 2, npins)
+    dx = np.where(io, np.where(side == 0, lx, ux), dx).astype(np.int32)
+    dy = np.where(io, rng.integers(ly, uy, npins), dy).astype(np.int32)
+    e.net_start = start.astype(np.int32)
+    e.pin_inst = np.where(io, -1, inst).astype(np.int32)
+    e.pin_dx, e.pin_dy = dx, dy
+    if names:
+        e.inst_name = [f"o{i}" for i in range(d.num_insts)]
+    return e
+
+
 def _calibrate_width(n, free_sites, util):
     mean_h = sum(h * p for h, p in MIXED_HEIGHTS.items())
     target_w = util * free_sites / n / mean_h

@@ -5,6 +5,7 @@
 // code and fills an LgResult instead of printing and exiting.
 #pragma once
 
+#include <string>
 #include <string_view>
 
 #include "../skyline_lg.h"
@@ -17,6 +18,11 @@ class Legalizer {
 
   int run(LgResult* result);  // Legalizer::run (Legalizer.cpp:197-228)
 
+  // Legalizer.h:74-75: the ICCAD 2017 placement.constraints file (turns on
+  // the contest metric in the statistics) and the modified-ISPD2015 cell
+  // size file (needs LgDesign::inst_name); an empty string clears them
+  void setICCAD17Constraint(const std::string& file);
+  void setSizeByFile(const std::string& file);
   void setTechPenalty(float val);
   void setStandardAdmm(bool val);
   void setQpRefine(bool val);
@@ -39,6 +45,7 @@ class Legalizer {
  private:
   const LgDesign* design_;
   LgParams params_;
+  std::string constraint_, size_file_;
 };
 
 }  // namespace legalizer

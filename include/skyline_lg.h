@@ -122,6 +122,28 @@ typedef struct LgDesign {
   const int32_t* box_ly;
   const int32_t* box_ux;
   const int32_t* box_uy;
+  /* Optional netlist (num_nets = 0: none) for the HPWL before and after
+   * legalization (Legalizer::computeHpwlFromDB, Legalizer.cpp:402-437).  The
+   * pins of net k are net_start[k] .. net_start[k+1]-1; pin p sits at
+   * (pin_dx, pin_dy) from the lower-left corner of instance pin_inst[p] in
+   * orientation N (the master pin's average position, dbITerm::getAvgXY), or,
+   * with pin_inst[p] = -1, is an IO pin at the absolute position (pin_dx,
+   * pin_dy).  A cell the legalizer flips (FS) mirrors its pins' y. */
+  int32_t num_nets;
+  const int32_t* net_start;       /* [num_nets + 1] */
+  const int32_t* pin_inst;
+  const int32_t* pin_dx;
+  const int32_t* pin_dy;
+  /* Optional inputs of Legalizer::setSizeByFile / setICCAD17Constraint
+   * (Legalizer.h:74-75), NULL for none: instance names [num_insts]; a size
+   * file of lines "name=width_in_sites=height_in_rows" that resizes the named
+   * movable cells (DatabaseIO.cpp:88-131, Util.cpp:220-234); an ICCAD 2017
+   * placement.constraints file (maximum_utilization=U%,
+   * maximum_movement=Mrows; Util.cpp:190-217), which turns on the contest
+   * metric in the statistics. */
+  const char* const* inst_name;
+  const char* size_file;
+  const char* constraint_file;
 } LgDesign;
 
 /* Violation counts of CudaDatabase::checkPlace (objects/LGVioStat.h), then
@@ -164,6 +186,23 @@ typedef struct LgStats {
   double  t_admm_s;           /* device upload + ADMM (runAdmmLegalize)            */
   double  t_post_s;           /* runPostProcess                                    */
   double  t_total_s;          /* whole lg_run_design                               */
+  /* lg_run_design with a netlist: HPWL of the global placement and of the
+   * result (dbu / dbu_per_micron) and hpwl_after / hpwl_before * 100 - 100
+   * (Util.cpp:113); zeros without nets or at the device level */
+  double  hpwl_before_um;
+  double  hpwl_after_um;
+  double  delta_hpwl_pct;
+  /* lg_run_design with a constraint file: its limits and the ICCAD 2017
+   * contest metric (Legalizer::printICCAD17Metric, Util.cpp:141-188):
+   * S_am = sum over heights h of (sum of displacements in sites of the
+   * h-row cells / their count) / #heights / (row height / site width),
+   * M_max = max displacement / row height, S = (1 + dHPWL/100 + (N_p + N_e) /
+   * #cells) * (1 + M_max / 100) * S_am */
+  double  max_util_constraint;   /* percent */
+  double  max_disp_constraint;   /* dbu */
+  double  iccad17_s_am;
+  double  iccad17_m_max;
+  double  iccad17_score;
 } LgStats;
 
 /* Outputs of lg_run_design, all caller-allocated.  Per-instance arrays have

@@ -71,19 +71,20 @@ def _check(st):
         raise RuntimeError(f"oracle error {st}: {lib().ref_last_error().decode()}")
 
 
-def run_design(d: Design, params: LgParams | None = None, trace_cap: int = 30000):
+def run_design(d: Design, params: LgParams | None = None, trace_cap: int = 30000, size_file: str | None = None,
+               constraint_file: str | None = None):
     params = params or default_params()
-    md = MarshalledDesign(d)
+    md = MarshalledDesign(d, size_file, constraint_file)
     rb = ResultBuffers(d.num_insts, trace_cap)
     _check(lib().ref_run_design(C.byref(md.struct), C.byref(params), C.byref(rb.struct)))
     return rb.lx, rb.ly, rb.flipped, rb.stats_dict()
 
 
-def import_and_preplace(d: Design, params: LgParams | None = None):
+def import_and_preplace(d: Design, params: LgParams | None = None, size_file: str | None = None):
     """Pixel maps (x-major), shrunk region bounds and preplaced movable-cell
     positions (dbu)."""
     params = params or default_params()
-    md = MarshalledDesign(d)
+    md = MarshalledDesign(d, size_file)
     shape = np.zeros(4, np.int32)
     null = C.cast(None, C.POINTER(C.c_int32))
     _check(lib().ref_import_and_preplace(C.byref(md.struct), C.byref(params), _p(shape), null, null,

@@ -30,7 +30,9 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <cctype>
 #include <deque>
+#include <fstream>
 #include <limits>
 #include <map>
 #include <numeric>
@@ -202,8 +204,36 @@ static void add_region(Host& H, int lx, int ly, int ux, int uy, int group, int t
 }
 
 // Legalizer::importCells (DatabaseIO.cpp:79-169)
+// tokenize (Util.cpp:22-47): delimiters and white space separate tokens
+static std::vector<std::string> tokenize(const std::string& line, const std::string& dels) {
+  std::vector<std::string> out;
+  std::string tok;
+  for (char ch : line) {
+    if (dels.find(ch) != std::string::npos || std::isspace((unsigned char)ch)) {
+      if (!tok.empty()) { out.push_back(tok); tok.clear(); }
+    } else {
+      tok.push_back(ch);
+    }
+  }
+  if (!tok.empty()) out.push_back(tok);
+  return out;
+}
+
+// importCells (DatabaseIO.cpp:79-169) with parseSizeFile (Util.cpp:220-234)
 static int import_cells(Host& H, const LgDesign* d) {
   int row_h = d->row_dy[d->num_rows - 1];
+  int site_w = d->row_site_width[d->num_rows - 1];
+  std::map<std::string, std::pair<int, int>> sizes;
+  const bool modified = d->size_file != nullptr && d->size_file[0] != 0;
+  if (modified) {
+    std::ifstream in(d->size_file);
+    if (!in) { g_err = "cannot open size file"; return LG_ERR_ARG; }
+    std::string line;
+    while (std::getline(in, line)) {
+      auto t = tokenize(line, "=");
+      if (t.size() >= 3) sizes[t[0]] = {std::stoi(t[1]), std::stoi(t[2])};
+    }
+  }
   for (int i = 0; i < d->num_insts; i++) {
     if (d->inst_dx[i] == 0 || d->inst_dy[i] < row_h) continue;
     int lx = d->inst_lx[i], ly = d->inst_ly[i];
@@ -215,6 +245,10 @@ static int import_cells(Host& H, const LgDesign* d) {
       Cell c;
       c.lx = c.init_lx = lx; c.ly = c.init_ly = ly;
       c.w = d->inst_dx[i]; c.h = d->inst_dy[i];
+      if (modified && d->inst_name && d->inst_name[i]) {
+        auto it = sizes.find(d->inst_name[i]);
+        if (it != sizes.end()) { c.w = it->second.first * site_w; c.h = it->second.second * row_h; }
+      }
       c.group = d->inst_group ? d->inst_group[i] : -1;  // assignGroupMembers (DatabaseIO.cpp:171-185)
       c.gnd = d->inst_ground_at_bottom[i] ? 1 : 0;
       c.inst = i;
@@ -1386,6 +1420,42 @@ static LgVioStat check_place(Dev& D) {
   return v;
 }
 
+// computeHpwlFromDB (Legalizer.cpp:402-437): per net, the bounding box of
+// its pins' average positions (dbITerm::getAvgXY, dbITerm.cpp:94-151, for
+// orientation N and, on flipped cells, FS: y measured down from the top).
+static int64_t hpwl_from_db(const LgDesign* d, const int32_t* lx, const int32_t* ly, const uint8_t* fs) {
+  int64_t sum = 0;
+  for (int k = 0; k < d->num_nets; k++) {
+    int lo_x = kIntMax, lo_y = kIntMax, hi_x = kIntMin, hi_y = kIntMin;
+    bool any = false;
+    for (int q = d->net_start[k]; q < d->net_start[k + 1]; q++) {
+      const int i = d->pin_inst[q];
+      int ax = d->pin_dx[q], ay = d->pin_dy[q];
+      if (i >= 0) {
+        ax = lx[i] + d->pin_dx[q];
+        ay = (fs && fs[i]) ? (ly[i] + d->inst_dy[i]) - d->pin_dy[q] : ly[i] + d->pin_dy[q];
+      }
+      lo_x = std::min(lo_x, ax); lo_y = std::min(lo_y, ay);
+      hi_x = std::max(hi_x, ax); hi_y = std::max(hi_y, ay);
+      any = true;
+    }
+    if (any) sum += int64_t(hi_x - lo_x) + int64_t(hi_y - lo_y);
+  }
+  return sum;
+}
+
+// parseICCAD17Constraint (Util.cpp:190-217)
+static void parse_constraints(const char* path, int rh, float& util, float& disp) {
+  std::ifstream in(path);
+  std::string line;
+  while (std::getline(in, line)) {
+    auto t = tokenize(line, "=");
+    if (t.empty()) continue;
+    if (t.front() == "maximum_utilization") util = std::stof(t.back().substr(0, t.back().size() - 1));
+    if (t.front() == "maximum_movement") disp = std::stof(t.back().substr(0, t.back().size() - 4)) * float(rh);
+  }
+}
+
 // Export of the host grid into the device-level descriptors
 // (CudaDatabase::copyGridInfo / copyFromHostCells, CudaDatabase.cu:43-230)
 struct Export {
@@ -1563,6 +1633,38 @@ int ref_run_design(const LgDesign* d, const LgParams* p, LgResult* r) {
   s.t_preplace_s = t2 - t1;
   s.t_admm_s = t3 - t2;
   s.t_post_s = t4 - t3;
+  // computeStat (Legalizer.cpp:439-476): HPWL before and after, in microns
+  const int64_t h0 = ref::hpwl_from_db(d, d->inst_lx, d->inst_ly, nullptr);
+  const int64_t h1 = ref::hpwl_from_db(d, r->out_lx, r->out_ly, r->out_flipped);
+  s.hpwl_before_um = double(h0) / dbu;
+  s.hpwl_after_um = double(h1) / dbu;
+  s.delta_hpwl_pct = h0 > 0 ? s.hpwl_after_um / s.hpwl_before_um * 100.0 - 100.0 : 0.0;
+  // printICCAD17Metric (Util.cpp:141-188)
+  if (d->constraint_file && d->constraint_file[0]) {
+    float util = 0.f, disp = 0.f;
+    ref::parse_constraints(d->constraint_file, d->row_dy[d->num_rows - 1], util, disp);
+    s.max_util_constraint = util;
+    s.max_disp_constraint = disp;
+  }
+  {
+    const double swd = H.sw, rhd = d->row_dy[d->num_rows - 1];
+    std::map<int, double> per_h;
+    int max_dbu = 0;
+    for (int k = 0; k < D.N; k++) {
+      const int inst = H.cells[k].inst;
+      const int dd = std::abs(r->out_lx[inst] - H.cells[k].init_lx) + std::abs(r->out_ly[inst] - H.cells[k].init_ly);
+      max_dbu = std::max(max_dbu, dd);
+      per_h[H.cells[k].h] += double(dd) / swd;
+    }
+    double total_sw = 0.0;
+    for (auto& kv : per_h) total_sw += kv.second / double(std::max(1, H.height_distribution.at(kv.first)));
+    const double large_k = double(H.height_distribution.size());
+    s.iccad17_s_am = large_k > 0 ? total_sw / large_k / (rhd / swd) : 0.0;
+    s.iccad17_m_max = max_dbu / rhd;
+    const int np = v.num_pin_short_vio + v.num_pin_access_vio, ne = v.num_edge_spacing_vio;
+    const double pen = double(np + ne) / double(std::max<size_t>(H.cells.size(), 1));
+    s.iccad17_score = (1.0 + s.delta_hpwl_pct / 100.0 + pen) * (1.0 + s.iccad17_m_max / 100.0) * s.iccad17_s_am;
+  }
   s.t_total_s = ref::now_s() - t0;
   if (r->ovf_trace)
     for (int i = 0; i < std::min<int>(r->ovf_trace_cap, (int)D.trace.size()); i++) r->ovf_trace[i] = D.trace[i];
