@@ -172,7 +172,33 @@ struct HostPrep {
   std::vector<int16_t> pcode;
   std::vector<int32_t> region_rm, vdd, rx0, ry0, rx1, ry1;
   std::vector<int32_t> lxg, lyg, w, h, code, gnd, ox, oy, rprio, khn, colmap, rowmap;
+  // dual segments: maximal runs of columns (rows) inside one partition
+  // column (row) under all three fold offsets; segment -> first x (y)
+  std::vector<int32_t> xseg, yseg, xseg0, yseg0;
 };
+
+// Segments of [0, n) cut wherever the partition column (row) changes under
+// any fold offset (x + hint / 3 * tf a multiple of hint): a segment lies in
+// one partition in every iteration.
+// The ids count three segments per period of `hint` (an empty one where two
+// cuts coincide), as the kernels' seg_axis computes them.
+void dual_segments(int n, int hint, std::vector<int32_t>& seg, std::vector<int32_t>& first) {
+  seg.resize(n);
+  const int t = hint / 3;
+  for (int x = 0; x < n; x++) {
+    const int q = x / hint, r = x % hint;
+    seg[x] = 3 * q + (r >= hint - 2 * t) + (r >= hint - t);
+  }
+  const int ns = n > 0 ? seg[n - 1] + 1 : 0;
+  first.assign(ns, -1);
+  for (int x = n - 1; x >= 0; x--) first[seg[x]] = x;
+  // an empty id gets the next segment's first column (its stamp is unused)
+  int next = n > 0 ? n - 1 : 0;
+  for (int s = ns - 1; s >= 0; s--) {
+    if (first[s] < 0) first[s] = next;
+    next = first[s];
+  }
+}
 
 struct LgHandle {
   LgParams p;
@@ -214,6 +240,9 @@ struct LgHandle {
 
 constexpr int kGraphIters = 8;
 constexpr size_t kSideMinPixels = 8u << 20;
+// lazy duals: a catch-up pass every kCatchupPeriod iterations bounds how many
+// dual updates a staged price may have to catch up on
+constexpr int kCatchupPeriod = 32;
 
 namespace {
 
@@ -458,6 +487,8 @@ int prepare_host(const LgGridDesc* g, const LgCellsSoA* c, const LgParams* p, Ho
     for (int x = 0; x < XS; x++) hp.colmap[tf * (size_t)XS + x] = ((x + ox) / p->x_hint) % hp.ncol;
     for (int y = 0; y < YS; y++) hp.rowmap[tf * (size_t)YS + y] = ((y + oy) / p->y_hint) % hp.nrow;
   }
+  dual_segments(XS, p->x_hint, hp.xseg, hp.xseg0);
+  dual_segments(YS, p->y_hint, hp.yseg, hp.yseg0);
   return LG_OK;
 }
 
@@ -476,6 +507,9 @@ size_t handle_bytes(const HostPrep& hp) {
   b += (3 * (size_t)hp.x_size + 3 * (size_t)hp.y_size + hp.y_size + 4 * hp.rx0.size()) * 4;
   b += (size_t)std::max(hp.p.max_admm_iter, 1) * 4 + sizeof(Ctl);
   b += (size_t)huge_slots(hp.N, hp.K) * 4 * kMaxNpad * 4;  // sort buffers above kMaxNpadLds slots
+  // dual segments: tables, stamps, step history
+  b += (2 * (size_t)hp.x_size + 2 * (size_t)hp.y_size + hp.xseg0.size() * hp.yseg0.size()) * 4 + 256 * 4;
+  b += (size_t)std::max(hp.p.max_admm_iter, 1) * 4;
   return b + 64 * blk;
 }
 
@@ -586,6 +620,24 @@ int create_from_prep(const HostPrep& hp, DevPool* pool, Ctl* ext_ctl, hipStream_
     if ((st = upload(h, &d_rm, hp.rowmap.data(), hp.rowmap.size()))) return bail(st);
     a.colmap = d_cm;
     a.rowmap = d_rm;
+    int32_t *d_xs, *d_ys, *d_xs0, *d_ys0;
+    if ((st = upload(h, &d_xs, hp.xseg.data(), hp.xseg.size()))) return bail(st);
+    if ((st = upload(h, &d_ys, hp.yseg.data(), hp.yseg.size()))) return bail(st);
+    if ((st = upload(h, &d_xs0, hp.xseg0.data(), hp.xseg0.size()))) return bail(st);
+    if ((st = upload(h, &d_ys0, hp.yseg0.data(), hp.yseg0.size()))) return bail(st);
+    a.xseg = d_xs; a.yseg = d_ys; a.xseg0 = d_xs0; a.yseg0 = d_ys0;
+    a.nxs = (int)hp.xseg0.size();
+    a.nys = (int)hp.yseg0.size();
+    if ((st = halloc(h, &a.seg_stamp, (size_t)a.nxs * a.nys))) return bail(st);
+    if ((st = halloc(h, &a.m0_hist, std::max(p->max_admm_iter, 1)))) return bail(st);
+    // Lazy duals where the dual pass would sit on the iteration's critical
+    // path: a server's batched iterations (their pass competes with the
+    // partition kernels of every resident request) and grids too small for
+    // the pass's side stream.  On a large solo grid the pass hides behind the
+    // partition kernels, whose staging would only get longer (mixed_1m: 149
+    // against 165 us per launch).  The adaptive search reads prices anywhere
+    // in the grid, so it keeps every price current.
+    a.lazy = !p->use_adaptive_search && (pool != nullptr || P < kSideMinPixels) ? 1 : 0;
   }
   if ((st = halloc(h, &a.ovf_flag, N))) return bail(st);
   if ((st = halloc(h, &a.ovf_list, N))) return bail(st);
@@ -649,6 +701,7 @@ int solve_begin(LgHandle* h, hipStream_t s) {
   }
   h->posted = 0;
   h->solved = 0;
+  h->host_ctl.iter = 0;
   HIP_TRY(hipMemsetAsync(a.usage, 0, h->P * sizeof(int32_t), s));
   if (N > 0) {
     hipLaunchKernelGGL(k_usage_scatter, dim3(nb), dim3(256), 0, s, a);
@@ -661,6 +714,8 @@ int solve_begin(LgHandle* h, hipStream_t s) {
   hipLaunchKernelGGL(k_fill_dual, dim3(2048), dim3(256), 0, s, a);
   const dim3 pix_grid((a.x_size + 1023) / 1024, a.y_size);  // four pixels per thread
   HIP_TRY(hipMemsetAsync(a.act_flags, 0, 2 * (size_t)a.K * sizeof(int32_t), s));
+  // every price holds the state before iteration 0
+  HIP_TRY(hipMemsetAsync(a.seg_stamp, 0xff, (size_t)a.nxs * a.nys * sizeof(int32_t), s));
   HIP_TRY(hipMemsetAsync(a.bcnt, 0, std::max(a.K, 1) * sizeof(int32_t), s));
 #ifdef LG_PHASE_TIMERS
   if (N > 0) {  // per-iteration block-time maxima / sums (timed builds)
@@ -676,21 +731,27 @@ int solve_begin(LgHandle* h, hipStream_t s) {
 // One ADMM iteration of one handle (updatePartition, executeSorting, primal
 // and dual updates, examineIteration); kernels read the iteration index and
 // the done flag from the device-resident Ctl.
-int launch_iteration(LgHandle* h, hipStream_t s) {
+// With lazy duals the dual pass (the catch-up pass then) runs only when
+// `catchup` is set; otherwise the prices of partitions without a primal step
+// wait for their next staging.
+int launch_iteration(LgHandle* h, hipStream_t s, bool catchup) {
   Args& a = h->a;
   const int nb = (a.N + 255) / 256;
   const dim3 pix_grid((a.x_size + 1023) / 1024, a.y_size);
+  const bool pass = !a.lazy || catchup;
+  const int nseg_blocks = (a.nxs * a.nys + 255) / 256;
   // cells of the listed partitions into their buckets
   if (a.N > 0) hipLaunchKernelGGL(k_bucket, dim3((a.N + kBucketCells - 1) / kBucketCells), dim3(256), 0, s, a);
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, a);
   // the dual pass needs the step k_scan sets; the pixels it updates belong
   // to partitions the primal step leaves alone, so it runs on the side
   // stream beside the partition kernels, joined before k_finish
-  const bool fork = h->side != nullptr;
+  const bool fork = h->side != nullptr && pass;
   if (fork) {
     HIP_TRY(hipEventRecord(h->ev_fork, s));
     HIP_TRY(hipStreamWaitEvent(h->side, h->ev_fork, 0));
     hipLaunchKernelGGL(k_dual_pass, pix_grid, dim3(256), 0, h->side, a);
+    if (a.lazy) hipLaunchKernelGGL(k_stamp_inactive, dim3(nseg_blocks), dim3(256), 0, h->side, a);
     HIP_TRY(hipEventRecord(h->ev_join, h->side));
   }
   if (a.use_adaptive) {
@@ -699,8 +760,12 @@ int launch_iteration(LgHandle* h, hipStream_t s) {
   }
   int st;
   if ((st = launch_partition<MODE_PRIMAL>(h, s))) return st;
-  if (fork) HIP_TRY(hipStreamWaitEvent(s, h->ev_join, 0));
-  else hipLaunchKernelGGL(k_dual_pass, pix_grid, dim3(256), 0, s, a);
+  if (fork) {
+    HIP_TRY(hipStreamWaitEvent(s, h->ev_join, 0));
+  } else if (pass) {
+    hipLaunchKernelGGL(k_dual_pass, pix_grid, dim3(256), 0, s, a);
+    if (a.lazy) hipLaunchKernelGGL(k_stamp_inactive, dim3(nseg_blocks), dim3(256), 0, s, a);
+  }
   hipLaunchKernelGGL(k_finish, dim3(1), dim3(1), 0, s, a);
   return LG_OK;
 }
@@ -722,7 +787,8 @@ int capture_iterations(LgHandle* h) {
   if (!h->cap) HIP_TRY(hipStreamCreateWithFlags(&h->cap, hipStreamNonBlocking));
   HIP_TRY(hipStreamBeginCapture(h->cap, hipStreamCaptureModeThreadLocal));
   int st = LG_OK;
-  for (int i = 0; i < kGraphIters && !st; i++) st = launch_iteration(h, h->cap);
+  // with lazy duals the graph's last iteration carries the catch-up pass
+  for (int i = 0; i < kGraphIters && !st; i++) st = launch_iteration(h, h->cap, i == kGraphIters - 1);
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(h->cap, &g);
   if (st || e != hipSuccess) {
@@ -751,6 +817,10 @@ int solve_end(LgHandle* h, hipStream_t s) {
       return fail(LG_ERR_STATE, "more than 8192 cells in one partition");
     if (h->host_ctl.error == kErrUsageRange)
       return fail(LG_ERR_STATE, "pixel usage above 65535");
+    if (a.lazy) {  // every price up to the last iteration, before the refine moves cells
+      const dim3 g1((a.x_size + 255) / 256, a.y_size);
+      hipLaunchKernelGGL(k_dual_final, g1, dim3(256), 0, s, a);
+    }
     if (h->host_ctl.converged) {
       hipLaunchKernelGGL(k_refine_bucket, dim3(nb), dim3(256), 0, s, a);
       hipLaunchKernelGGL(k_refine_scan, dim3(1), dim3(1024), 0, s, a);
@@ -809,8 +879,10 @@ int lg_solve(LgHandle* h, void* stream) {
       if (graph) {
         for (int b = 0; b < batch; b += kGraphIters) HIP_TRY(hipGraphLaunch(h->gexec, s));
       } else {
-        for (int b = 0; b < batch; b++)
-          if ((st = launch_iteration(h, s))) return st;
+        for (int b = 0; b < batch; b++) {
+          const int it = h->host_ctl.iter + b;  // the device's iteration, unless it finishes first
+          if ((st = launch_iteration(h, s, it % kCatchupPeriod == kCatchupPeriod - 1))) return st;
+        }
       }
       HIP_TRY(hipGetLastError());
       if ((st = read_ctl(h, s))) return st;

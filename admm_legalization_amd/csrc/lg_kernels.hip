@@ -97,6 +97,17 @@ constexpr float kInf = __builtin_huge_valf();
 // std::max(a, b) of the reference's host-side semantics: (a < b) ? b : a
 __device__ __forceinline__ float fmax_ref(float a, float b) { return (a < b) ? b : a; }
 
+// delta of dual_update_functor_with_step_size for usage ui: the literal 0.5
+// promotes the quadratic term to double before the float rounding
+__device__ __forceinline__ float dual_delta(int ui, float rho, float neg) {
+  float u = float(ui);
+  float gg = fmax_ref(u - 1.0f, neg);
+  double q = (double)rho * 0.5;
+  q = q * (double)gg;
+  q = q * (double)gg;
+  return (float)((double)gg + q);
+}
+
 struct Args {
   // grid
   int x_size, y_size, site_width, row_height, core_lx, core_ly;
@@ -141,6 +152,17 @@ struct Args {
   int32_t* act_flags;  // [2][K] partition has a pixel with usage > 1 (by iteration parity)
   int32_t* huge;       // [huge_slots][4 * kMaxNpad] sort buffers of partitions above kMaxNpadLds slots
   int huge_slots;
+  // lazy dual updates: the prices of a partition that takes no primal step
+  // are brought up to date when it is next staged (or by the periodic
+  // catch-up pass), from the last iteration that updated its segment
+  const int32_t* xseg;  // [x_size] dual segment column of x
+  const int32_t* yseg;  // [y_size] dual segment row of y
+  const int32_t* xseg0; // [nxs] first x of a segment column
+  const int32_t* yseg0; // [nys] first y of a segment row
+  int nxs, nys;
+  int32_t* seg_stamp;   // [nxs * nys] last iteration whose dual update the segment's prices hold
+  float* m0_hist;       // [max_iter] what iteration k's dual update adds to a price of usage 0
+  int lazy;             // 1: lazy dual updates (0 with the adaptive search)
   const int32_t* colmap;  // [3][x_size] partition column of x for each fold offset
   const int32_t* rowmap;  // [3][y_size] partition row of y for each fold offset
   int32_t* ovf_flag;   // [N] adaptive search: cell has overflow
@@ -395,6 +417,10 @@ __device__ __forceinline__ void scan_finish(const Args& a, int K, int m, int tot
   if (it % a.iter_update_dual_step == 0 && it > a.iter_threshold_dual_update) {
     float inc = a.coeff_update_dual_step * ctl->lambda0;
     ctl->step = ctl->step + inc;
+  }
+  if (it < a.max_iter) {
+    const float rho = ctl->rho;
+    a.m0_hist[it] = ctl->step * dual_delta(0, rho, -1.0f / rho);
   }
 }
 
@@ -1105,14 +1131,71 @@ __device__ __forceinline__ void eval_candidates(const Args& a, const PartGeo& g,
   }
 }
 
+// dual_update_functor_with_step_size (AdmmKernels.hpp:26-42): the literal 0.5
+// promotes the quadratic term to double before the float store.
+__device__ __forceinline__ float dual_step(int ui, float l, float rho, float step, float neg) {
+  float m = step * dual_delta(ui, rho, neg);
+  return fmax_ref(l + m, 0.0f);
+}
+
+// Dual segment column of x: the cuts lie where x + hint/3 * tf (tf = 0, 1, 2)
+// is a multiple of hint, i.e. at x mod hint in {0, hint - 2 (hint/3),
+// hint - hint/3} (the host's tables xseg / yseg hold the same numbers).
+__device__ __forceinline__ int seg_axis(int x, int hint) {
+  const int q = x / hint, r = x - q * hint, t = hint / 3;
+  return 3 * q + (r >= hint - 2 * t ? 1 : 0) + (r >= hint - t ? 1 : 0);
+}
+// Dual segment of pixel (x, y).
+__device__ __forceinline__ int seg_of(const Args& a, int x, int y) {
+  return seg_axis(y, a.y_hint) * a.nxs + seg_axis(x, a.x_hint);
+}
+
+// The dual updates iterations from..to (inclusive) would have made to a
+// pixel whose partition took no primal step in them: its usage u is then 0
+// or 1 and unchanged; u = 1 adds zero, u = 0 lowers the price towards 0,
+// where it stays.  The same dual_step, iteration by iteration, so the bits
+// are those of the eager update.
+// For usage 0 a step is l <- max(l + m0[k], 0) with m0[k] = step_k * delta(0)
+// <= 0 (Args::m0_hist); the increments of iterations win0 .. win0 + kCatchWin
+// - 1 may come from an LDS copy `win`.
+constexpr int kCatchWin = 64;
+__device__ __forceinline__ float dual_catch_up(const Args& a, int u, float l, int from, int to,
+                                               const float* win = nullptr, int win0 = 0) {
+  if (u != 0 || !(l > 0.0f)) return l;
+  int k = from;
+  if (win) {
+    for (; k <= to && k < win0; k++) l = fmax_ref(l + a.m0_hist[k], 0.0f);
+    const int end = min(to, win0 + kCatchWin - 1);
+#pragma unroll 4
+    for (; k <= end; k++) l = fmax_ref(l + win[k - win0], 0.0f);
+  }
+  for (; k <= to; k++) l = fmax_ref(l + a.m0_hist[k], 0.0f);
+  return l;
+}
+
+// LDS copy of the increments m0[w0 .. w0 + kCatchWin) (entries before
+// iteration 0 or from `last` on are not needed and left 0); every thread of
+// the workgroup passes a barrier after it.
+__device__ __forceinline__ void load_catch_window(const Args& a, float* win, int w0, int last, int tid, int nt) {
+  for (int j = tid; j < kCatchWin; j += nt) {
+    const int k = w0 + j;
+    win[j] = (k >= 0 && k <= last) ? a.m0_hist[k] : 0.0f;
+  }
+  __syncthreads();
+}
+
 // Staging of the partition's pixel tile by the NW waves of the workgroup,
 // threads walking the tile in order with incremental (la, lb) = (i % xh,
 // i / xh): consecutive threads read consecutive pixels of a grid row;
 // 16 (one wave) or 8 (four waves) pixels in flight per thread.  Returns
 // true if a usage does not fit the 16-bit LDS packing.
+// With catch_up (lazy duals, primal step of iteration `iter`), prices that
+// missed dual updates are brought up to the end of iteration iter - 1.
 template <int NW>
 __device__ __forceinline__ bool stage_tile(const Args& a, const PartGeo& g, const Tile& T, float rho,
-                                           bool standard, bool& over, int& cmin, int& cmax) {
+                                           bool standard, bool& over, int& cmin, int& cmax, int iter,
+                                           bool catch_up, const float* win) {
+  const float neg = -1.0f / rho;
   constexpr int NT = NW * kWave;
   const int tid = threadIdx.x;
   const int npix = g.xh * g.yh;
@@ -1121,7 +1204,7 @@ __device__ __forceinline__ bool stage_tile(const Args& a, const PartGeo& g, cons
   bool range_err = false;
   constexpr int U = NW > 1 ? 8 : 16;
   for (int i0 = 0; i0 < npix; i0 += U * NT) {
-    int u[U], code[U], li[U];
+    int u[U], code[U], li[U], stamp[U];
     float dl[U];
     bool in[U];
 #pragma unroll
@@ -1135,6 +1218,7 @@ __device__ __forceinline__ bool stage_tile(const Args& a, const PartGeo& g, cons
       const int uu = a.usage[p];
       const int cc = (int)(uint16_t)a.pcode[p];
       const float dd = a.dual[p];
+      stamp[q] = catch_up && ok ? a.seg_stamp[seg_of(a, xx, yy)] : iter - 1;
       u[q] = ok ? uu : 0;
       code[q] = ok ? cc : 0;  // code 0: not placeable
       dl[q] = ok ? dd : 0.0f;
@@ -1149,6 +1233,8 @@ __device__ __forceinline__ bool stage_tile(const Args& a, const PartGeo& g, cons
         cmax = max(cmax, code[q]);
       }
       if (i < npix) {
+        if (stamp[q] < iter - 1)
+          dl[q] = dual_catch_up(a, u[q], dl[q], stamp[q] + 1, iter - 1, win, iter - kCatchWin);
         range_err |= u[q] > 0xffff;
         over |= u[q] > 1;
         T.uc[li[q]] = ((uint32_t)code[q] << 16) | (uint32_t)(u[q] & 0xffff);
@@ -1631,19 +1717,6 @@ __device__ __forceinline__ bool block_any(bool v, int* x, int lane, int wave) {
   return r != 0;
 }
 
-// dual_update_functor_with_step_size (AdmmKernels.hpp:26-42): the literal 0.5
-// promotes the quadratic term to double before the float store.
-__device__ __forceinline__ float dual_step(int ui, float l, float rho, float step, float neg) {
-  float u = float(ui);
-  float gg = fmax_ref(u - 1.0f, neg);
-  double q = (double)rho * 0.5;
-  q = q * (double)gg;
-  q = q * (double)gg;
-  float delta = (float)((double)gg + q);
-  float m = step * delta;
-  return fmax_ref(l + m, 0.0f);
-}
-
 // activity flag, under the next iteration's fold offset, of the partition
 // holding pixel (x, y); set for every pixel with usage > 1 that a primal step
 // writes (only partitions that take the step can hold such pixels)
@@ -1653,6 +1726,24 @@ __device__ __forceinline__ void flag_next(const Args& a, int iter, int x, int y)
   const int col = ((x + a.x_hint / 3 * tn) / a.x_hint) % a.ncol;
   const int row = ((y + a.y_hint / 3 * tn) / a.y_hint) % a.nrow;
   a.act_flags[(size_t)((iter + 1) & 1) * a.K + col * a.nrow + row] = 1;
+}
+
+// Lazy duals: the partition's prices hold iteration `iter`'s update now.
+// Writes the stamp of every dual segment of the tile once (at the segment's
+// first column and row inside the tile); called after a barrier that
+// follows every stamp read of the workgroup.
+__device__ __forceinline__ void stamp_partition(const Args& a, const PartGeo& g, int iter, int tid, int nt) {
+  const int npix = g.xh * g.yh;
+  for (int i = tid; i < npix; i += nt) {
+    const int lb = i / g.xh, la = i - lb * g.xh;
+    const int x = g.gx(la), y = g.gy(lb);
+    if (x < 0 || y < 0) continue;
+    // (x = 0 / y = 0 start a segment; through the wrap-around of the
+    // colouring they can sit inside the tile, after columns outside the grid)
+    const bool fx = la == 0 || x == 0 || g.gx(la - 1) != x - 1 || seg_axis(x - 1, a.x_hint) != seg_axis(x, a.x_hint);
+    const bool fy = lb == 0 || y == 0 || g.gy(lb - 1) != y - 1 || seg_axis(y - 1, a.y_hint) != seg_axis(y, a.y_hint);
+    if (fx && fy) a.seg_stamp[seg_of(a, x, y)] = iter;
+  }
 }
 
 // doPrimalUpdateKernel (AdmmKernels.hpp:637-897) / doFinalRefineKernel
@@ -1779,6 +1870,11 @@ __device__ __forceinline__ void partition_body(const Args& a, int bx, int npad_l
     if (tid == 0) a.sorted[k] = 0;
     return;
   }
+  // lazy duals: the increments of the last kCatchWin iterations in LDS for
+  // the prices this step brings up to date
+  const bool lazy = MODE == MODE_PRIMAL && a.lazy;
+  __shared__ float s_catch[kCatchWin];
+  if (lazy) load_catch_window(a, s_catch, iter - kCatchWin, iter - 1, tid, NT);
   if (cell_test) {
     bool need = false;
     for (int i = tid; i < count; i += NT) {
@@ -1797,13 +1893,23 @@ __device__ __forceinline__ void partition_body(const Args& a, int bx, int npad_l
         if (xx < 0 || yy < 0) continue;
         const unsigned p = pidx(a, xx, yy);
         int ui = a.usage[p];
-        a.dual[p] = dual_step(ui, a.dual[p], rho, step, neg);
+        float l = a.dual[p];
+        if (a.lazy) {
+          const int sg = seg_of(a, xx, yy);
+          const int s = a.seg_stamp[sg];
+          if (s < iter - 1) l = dual_catch_up(a, ui, l, s + 1, iter - 1, s_catch, iter - kCatchWin);
+        }
+        a.dual[p] = dual_step(ui, l, rho, step, neg);
         ovf += (unsigned long long)max(ui - 1, 0);
         if (ui > 1) flag_next(a, iter, xx, yy);
       }
       for (int o = 32; o > 0; o >>= 1) ovf += __shfl_down(ovf, o);
       if (lane == 0 && ovf) atomicAdd(&ctl->ovf_acc, ovf);
       if (tid == 0) a.sorted[k] = 0;
+      if (a.lazy) {
+        __syncthreads();
+        stamp_partition(a, g, iter, tid, NT);
+      }
       PHASE_MARK(4);
       PHASE_END();
       return;
@@ -1812,7 +1918,7 @@ __device__ __forceinline__ void partition_body(const Args& a, int bx, int npad_l
 
   bool over = false;
   int cmin = 0x7fffffff, cmax = -1;
-  if (stage_tile<NW>(a, g, T, rho, standard, over, cmin, cmax)) ctl->error = kErrUsageRange;
+  if (stage_tile<NW>(a, g, T, rho, standard, over, cmin, cmax, iter, lazy, s_catch)) ctl->error = kErrUsageRange;
   // one placeable code on the tile's placeable pixels (the others carry
   // +inf penalty terms): candidate evaluation then needs no per-pixel code
   // test
@@ -1834,6 +1940,10 @@ __device__ __forceinline__ void partition_body(const Args& a, int bx, int npad_l
       a.dual[pidx(a, xx, yy)] = dual_step((int)(T.uc[li] & 0xffffu), T.lam[li], rho, step, neg);
     }
     if (tid == 0) a.sorted[k] = 0;
+    if (lazy) {
+      if (NW > 1) __syncthreads();
+      stamp_partition(a, g, iter, tid, NT);
+    }
     PHASE_MARK(4);
     PHASE_END();
     return;
@@ -2184,6 +2294,7 @@ __device__ __forceinline__ void partition_body(const Args& a, int bx, int npad_l
   if (MODE == MODE_PRIMAL) {
     for (int o = 32; o > 0; o >>= 1) ovf += __shfl_down(ovf, o);
     if (lane == 0 && ovf) atomicAdd(&ctl->ovf_acc, ovf);
+    if (lazy) stamp_partition(a, g, iter, tid, NT);
   }
   PHASE_MARK(3);
   SUB_FLUSH();
@@ -2208,6 +2319,9 @@ __device__ __forceinline__ void dual_pass_body(const Args& a, int bx, int by) {
   const int iter = ctl->iter;
   const bool adaptive_now = a.use_adaptive && ctl->exceptional;
   if (iter < a.iter_perturbation_on || ctl->wrong_row != 0 || adaptive_now) return;
+  // lazy duals: the increments of the last kCatchWin iterations in LDS
+  __shared__ float s_catch[kCatchWin];
+  if (a.lazy) load_catch_window(a, s_catch, iter + 1 - kCatchWin, iter, threadIdx.x, blockDim.x);
   const int y = by;
   const int x0 = 4 * (bx * blockDim.x + threadIdx.x);
   if (x0 >= a.x_size) return;
@@ -2234,21 +2348,67 @@ __device__ __forceinline__ void dual_pass_body(const Args& a, int bx, int by) {
   }
   if (!any) return;
   const float rho = ctl->rho, step = ctl->step, neg = -1.0f / rho;
+  // lazy duals (the periodic catch-up pass): every update since the
+  // segment's stamp, this iteration's included; the stamps are set by
+  // k_stamp_inactive after the pass
+  int from[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) from[q] = (a.lazy && upd[q]) ? a.seg_stamp[seg_of(a, x0 + q, y)] + 1 : iter;
+  auto upd1 = [&](int q, float l) {
+    return a.lazy ? dual_catch_up(a, u[q], l, from[q], iter, s_catch, iter + 1 - kCatchWin)
+                  : dual_step(u[q], l, rho, step, neg);
+  };
   // a vector store only when all four pixels are updated here: a pixel of a
   // partition the primal step processes may share the group, and that step
   // runs at the same time
   if (full && upd[0] && upd[1] && upd[2] && upd[3]) {
     float4 d = *reinterpret_cast<const float4*>(a.dual + p0);
-    if (upd[0]) d.x = dual_step(u[0], d.x, rho, step, neg);
-    if (upd[1]) d.y = dual_step(u[1], d.y, rho, step, neg);
-    if (upd[2]) d.z = dual_step(u[2], d.z, rho, step, neg);
-    if (upd[3]) d.w = dual_step(u[3], d.w, rho, step, neg);
+    d.x = upd1(0, d.x);
+    d.y = upd1(1, d.y);
+    d.z = upd1(2, d.z);
+    d.w = upd1(3, d.w);
     *reinterpret_cast<float4*>(a.dual + p0) = d;
   } else {
 #pragma unroll
     for (int q = 0; q < 4; q++)
-      if (upd[q]) a.dual[p0 + q] = dual_step(u[q], a.dual[p0 + q], rho, step, neg);
+      if (upd[q]) a.dual[p0 + q] = upd1(q, a.dual[p0 + q]);
   }
+}
+
+// Lazy duals, after the catch-up pass: the segments of partitions that took
+// no primal step hold this iteration's update now.  One thread per segment
+// (a segment lies in one partition).
+__device__ __forceinline__ void stamp_inactive_body(const Args& a, int i0, int stride) {
+  const Ctl* ctl = a.ctl;
+  if (ctl->done || !a.lazy) return;
+  const int iter = ctl->iter;
+  if (iter < a.iter_perturbation_on || ctl->wrong_row != 0) return;
+  const int tf = triple_fold(a, iter);
+  const int* fl = a.act_flags + (size_t)(iter & 1) * a.K;
+  const int ns = a.nxs * a.nys;
+  for (int sg = i0; sg < ns; sg += stride) {
+    const int sy = sg / a.nxs, sx = sg - sy * a.nxs;
+    const int k = a.colmap[tf * a.x_size + a.xseg0[sx]] * a.nrow + a.rowmap[tf * a.y_size + a.yseg0[sy]];
+    if (fl[k] == 0) a.seg_stamp[sg] = iter;
+  }
+}
+__global__ __launch_bounds__(256) void k_stamp_inactive(Args a) {
+  stamp_inactive_body(a, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+
+// Lazy duals, end of the solve (before the final refine changes usage):
+// every price brought up to the last executed iteration.  One thread per
+// pixel, rows in blockIdx.y.
+__global__ __launch_bounds__(256) void k_dual_final(Args a) {
+  const Ctl* ctl = a.ctl;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (!a.lazy || x >= a.x_size) return;
+  const int last = ctl->iter - 1;
+  const int s = a.seg_stamp[seg_of(a, x, y)];
+  if (s >= last) return;
+  const unsigned p = pidx(a, x, y);
+  const float rho = ctl->rho;
+  a.dual[p] = dual_catch_up(a, a.usage[p], a.dual[p], s + 1, last);
 }
 __global__ __launch_bounds__(256) void k_dual_pass(Args a) { dual_pass_body(a, blockIdx.x, blockIdx.y); }
 
@@ -2598,6 +2758,12 @@ __global__ __launch_bounds__(256) void kb_dual_pass(const Args* __restrict__ tab
   const int gx = (a.x_size + 1023) / 1024;
   const int by = m_.y / gx;
   dual_pass_body(a, m_.y - by * gx, by);
+}
+
+// one block per slot, looping over the slot's segments
+__global__ __launch_bounds__(256) void kb_stamp_inactive(const Args* __restrict__ tab, const int2* __restrict__ map) {
+  LG_BATCH_SLOT();
+  stamp_inactive_body(a, threadIdx.x, blockDim.x);
 }
 
 __global__ void kb_finish(const Args* __restrict__ tab, const int2* __restrict__ map) {

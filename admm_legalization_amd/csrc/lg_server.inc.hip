@@ -169,18 +169,22 @@ int srv_build_tables(LgServer* S) {
 }
 
 // One ADMM iteration of every resident request (the steps of
-// launch_iteration, each as one batched launch).
-int srv_launch_iteration(LgServer* S, int it, hipStream_t s) {
+// launch_iteration, each as one batched launch).  The dual pass (with lazy
+// duals the catch-up pass, followed by the stamps) runs when `pass` is set.
+int srv_launch_iteration(LgServer* S, int it, hipStream_t s, bool pass) {
   const Args* tab = S->d_tab;
   const int2* mp = S->d_map;
   auto at = [&](const MapRange& r) { return mp + r.off; };
   if (S->mbk.n > 0) hipLaunchKernelGGL(kb_bucket, dim3(S->mbk.n), dim3(256), 0, s, tab, at(S->mbk));
   hipLaunchKernelGGL(kb_scan, dim3(S->m1.n), dim3(1024), 0, s, tab, at(S->m1));
   // the dual pass on the side stream, beside the partition kernels
-  HIP_TRY(hipEventRecord(S->ev_fork, s));
-  HIP_TRY(hipStreamWaitEvent(S->side, S->ev_fork, 0));
-  hipLaunchKernelGGL(kb_dual_pass, dim3(S->mpix.n), dim3(256), 0, S->side, tab, at(S->mpix));
-  HIP_TRY(hipEventRecord(S->ev_join, S->side));
+  if (pass) {
+    HIP_TRY(hipEventRecord(S->ev_fork, s));
+    HIP_TRY(hipStreamWaitEvent(S->side, S->ev_fork, 0));
+    hipLaunchKernelGGL(kb_dual_pass, dim3(S->mpix.n), dim3(256), 0, S->side, tab, at(S->mpix));
+    hipLaunchKernelGGL(kb_stamp_inactive, dim3(S->m1.n), dim3(256), 0, S->side, tab, at(S->m1));
+    HIP_TRY(hipEventRecord(S->ev_join, S->side));
+  }
   if (S->any_adaptive) {
     if (S->m256.n > 0)
       hipLaunchKernelGGL(kb_extract_congested, dim3(S->m256.n), dim3(256), 0, s, tab, at(S->m256));
@@ -202,7 +206,7 @@ int srv_launch_iteration(LgServer* S, int it, hipStream_t s) {
   hipLaunchKernelGGL((kb_partition<MODE_PRIMAL, 1, 1>), dim3(S->mk.n), dim3(kWave), S->lds_big, s, tab, at(S->mk), 1,
                      kMaxNpad, -1);
   if (e) HIP_TRY(hipEventRecord(e[1], s));
-  HIP_TRY(hipStreamWaitEvent(s, S->ev_join, 0));
+  if (pass) HIP_TRY(hipStreamWaitEvent(s, S->ev_join, 0));
   hipLaunchKernelGGL(kb_finish, dim3(S->m1.n), dim3(1), 0, s, tab, at(S->m1));
   HIP_TRY(hipGetLastError());
   return LG_OK;
@@ -212,7 +216,8 @@ int srv_launch_iteration(LgServer* S, int it, hipStream_t s) {
 // and instantiated.
 int srv_capture(LgServer* S) {
   HIP_TRY(hipStreamBeginCapture(S->cap, hipStreamCaptureModeThreadLocal));
-  const int st = srv_launch_iteration(S, 0, S->cap);
+  // every captured iteration carries the dual / catch-up pass
+  const int st = srv_launch_iteration(S, 0, S->cap, true);
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(S->cap, &g);
   if (st || e != hipSuccess) {
@@ -390,7 +395,14 @@ void srv_loop(LgServer* S) {
       for (int it = 0; !st && it < S->cfg.iters_per_sync; it++)
         if (hipGraphLaunch(S->gexec, S->stream) != hipSuccess) st = fail(LG_ERR_DEVICE, "request server: graph launch");
     } else {
-      for (int it = 0; !st && it < S->cfg.iters_per_sync; it++) st = srv_launch_iteration(S, it, S->stream);
+      for (int it = 0; !st && it < S->cfg.iters_per_sync; it++) {
+        // lazy duals: the catch-up pass every kCatchupPeriod batched
+        // iterations (every iteration while a slot runs the adaptive search,
+        // whose prices must stay current)
+        const int64_t n = S->ctr.iterations + it;
+        const bool pass = S->any_adaptive || n % kCatchupPeriod == kCatchupPeriod - 1;
+        st = srv_launch_iteration(S, it, S->stream, pass);
+      }
     }
     if (!st && hipMemcpyAsync(S->h_ctl, S->d_ctl, S->top_slot * sizeof(Ctl), hipMemcpyDeviceToHost, S->stream) !=
                    hipSuccess)
