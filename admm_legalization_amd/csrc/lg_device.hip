@@ -630,14 +630,15 @@ int create_from_prep(const HostPrep& hp, DevPool* pool, Ctl* ext_ctl, hipStream_
     a.nys = (int)hp.yseg0.size();
     if ((st = halloc(h, &a.seg_stamp, (size_t)a.nxs * a.nys))) return bail(st);
     if ((st = halloc(h, &a.m0_hist, std::max(p->max_admm_iter, 1)))) return bail(st);
-    // Lazy duals where the dual pass would sit on the iteration's critical
-    // path: a server's batched iterations (their pass competes with the
-    // partition kernels of every resident request) and grids too small for
-    // the pass's side stream.  On a large solo grid the pass hides behind the
-    // partition kernels, whose staging would only get longer (mixed_1m: 149
-    // against 165 us per launch).  The adaptive search reads prices anywhere
-    // in the grid, so it keeps every price current.
-    a.lazy = !p->use_adaptive_search && (pool != nullptr || P < kSideMinPixels) ? 1 : 0;
+    // Lazy duals for a server's batched iterations, whose dual pass competes
+    // with the partition kernels of every resident request (served slice of
+    // 200 requests: 5.17 -> 4.71 s).  A solo handle keeps the eager pass:
+    // inline on small grids it costs less than the lazy staging adds (92
+    // against 100 us per iteration under 5k cells), and on large grids it
+    // hides behind the partition kernels on the side stream (mixed_1m: 149
+    // against 165 us per partition launch).  The adaptive search reads
+    // prices anywhere in the grid, so it keeps every price current.
+    a.lazy = !p->use_adaptive_search && pool != nullptr ? 1 : 0;
   }
   if ((st = halloc(h, &a.ovf_flag, N))) return bail(st);
   if ((st = halloc(h, &a.ovf_list, N))) return bail(st);
