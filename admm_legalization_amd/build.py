@@ -32,6 +32,7 @@ UNITS = [
     ("dens_device.hip", "hip"),
     ("dens_host.cpp", "cxx"),
     ("wl_device.hip", "hip"),
+    ("gp_device.hip", "hip"),
 ]
 
 

@@ -5,6 +5,7 @@
 //   cgeo[N]   float4 {density width, density height, scatter scale, gather scale}
 //   cfill[N]  int    filler flag
 //   slab[T][2][SW*SW]  per-tile movable / filler partial areas (SW = 32 + halo)
+//   big[2][B] uint64   fixed-point sums of the cells wider than the halo
 //   mov, fil, dens, psi, ex, ey, lambda, fixed, scaled  [B] float
 //
 // Scatter (computeDensityContributionOfEachCell, DensityGradient.cu:61-138):
@@ -14,7 +15,11 @@
 // per-bin resolve pass adds a bin's own slab entry and the halo entries of the
 // (up to three) tiles below / left of it, and computes density and overflow in
 // the same pass.  Cells wider than the halo go to a separate list whose bins
-// are added with global atomics into the slabs before the resolve pass.
+// are added with global atomics into a per-bin buffer that the resolve pass
+// adds and clears.  All sums are 64-bit fixed point (overlap areas scaled by a
+// power of two chosen from the bin area), and the overflow is reduced in
+// block order: an evaluation's results do not depend on the order in which
+// the hardware runs the atomics, so equal inputs give equal bits.
 //
 // Poisson (PoissonSolver::solvePoisson, PoissonSolver.cu:224-302): with
 // M = nby rows, N = nbx columns and k-contiguous operands (out = A * B^T):
@@ -60,6 +65,7 @@ typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int kTile = 32;   // bins per tile side
+constexpr int kResolveBlocks = 4096;
 constexpr int kMaxHalo = 8;
 constexpr int kBK = 64;     // GEMM k-step
 
@@ -282,7 +288,16 @@ struct Geo {
   int nbx, nby, tiles_x, T, H, SW, N;
   float die_lx, die_ly, bin_w, bin_h;
   const float *bin_lx, *bin_ux, *bin_ly, *bin_uy;
+  // fixed-point scale of the bin sums: an overlap area a is added as the
+  // integer rint(a * fx_scale) (order-independent, so every evaluation gives
+  // the same bits), read back as sum * fx_inv
+  float fx_scale;
+  double fx_inv;
 };
+
+__device__ __forceinline__ unsigned long long to_fixed(const Geo& G, float a) {
+  return (unsigned long long)__float2ll_rn(a * G.fx_scale);
+}
 
 // findBinWithDensitySize (DensityGradient.cu:20-59)
 __device__ __forceinline__ void bin_range(const Geo& G, float cx, float cy, float dw, float dh, int& x0,
@@ -305,9 +320,8 @@ __device__ __forceinline__ float overlap(float c, float hw, float b0, float b1) 
   return fmaxf(0.f, fminf(d + hw, b1 - b0) - fmaxf(d - hw, 0.f));
 }
 
-__global__ void k_dens_reset(int* cnt, int n, double* ovf) {
+__global__ void k_dens_reset(int* cnt, int n) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) cnt[i] = 0;
-  if (blockIdx.x == 0 && threadIdx.x == 0) *ovf = 0.0;
 }
 
 // bucket of each cell: its tile, T for cells wider than the halo, -1 for
@@ -397,13 +411,15 @@ __global__ void k_dens_fill(Geo G, const float* cx, const float* cy, const float
   rec_ig[p] = make_int2(i, __float_as_int(c.w));
 }
 
-// one workgroup per tile: LDS accumulation of the tile plus halo, slab write
+// one workgroup per tile: LDS accumulation of the tile plus halo in fixed
+// point (64-bit integer adds, whose order does not change the sum), slab
+// write in FP32
 __global__ __launch_bounds__(256) void k_dens_accum(Geo G, const int* off, const float4* rec_box,
                                                     const float2* rec_s, float* slab) {
-  extern __shared__ float sm[];  // [2][SW*SW]
+  extern __shared__ unsigned long long sm[];  // [2][SW*SW]
   const int t = blockIdx.x;
   const int SW = G.SW, S2 = SW * SW;
-  for (int e = threadIdx.x; e < 2 * S2; e += blockDim.x) sm[e] = 0.f;
+  for (int e = threadIdx.x; e < 2 * S2; e += blockDim.x) sm[e] = 0ull;
   __syncthreads();
   const int tx0 = (t % G.tiles_x) * kTile, ty0 = (t / G.tiles_x) * kTile;
   const int b = off[t], e = off[t + 1];
@@ -413,25 +429,26 @@ __global__ __launch_bounds__(256) void k_dens_accum(Geo G, const int* off, const
     int x0, x1, y0, y1;
     bin_range(G, c.x, c.y, c.z, c.w, x0, x1, y0, y1);
     const float hw = c.z / 2, hh = c.w / 2;
-    float* dst = sm + (s.y != 0.f ? S2 : 0);
+    unsigned long long* dst = sm + (s.y != 0.f ? S2 : 0);
     for (int bx = x0; bx < x1; bx++) {
       float ox = overlap(c.x, hw, G.bin_lx[bx], G.bin_ux[bx]);
       for (int by = y0; by < y1; by++) {
         float oy = overlap(c.y, hh, G.bin_ly[by], G.bin_uy[by]);
-        atomicAdd(&dst[(by - ty0) * SW + (bx - tx0)], ox * oy * s.x);
+        atomicAdd(&dst[(by - ty0) * SW + (bx - tx0)], to_fixed(G, ox * oy * s.x));
       }
     }
   }
   __syncthreads();
   float* out = slab + (size_t)t * 2 * S2;
-  for (int q = threadIdx.x; q < 2 * S2; q += blockDim.x) out[q] = sm[q];
+  for (int q = threadIdx.x; q < 2 * S2; q += blockDim.x) out[q] = (float)((double)(long long)sm[q] * G.fx_inv);
 }
 
-// cells wider than the halo: global atomics into the owning tiles' slabs
+// cells wider than the halo: global 64-bit fixed-point atomics into big[2][B]
+// (movable, filler), which the resolve pass adds and clears
 __global__ __launch_bounds__(256) void k_dens_large(Geo G, const int* off, const float4* rec_box,
-                                                    const float2* rec_s, float* slab) {
+                                                    const float2* rec_s, unsigned long long* big) {
   const int b = off[G.T], e = off[G.T + 1];
-  const int SW = G.SW, S2 = SW * SW;
+  const size_t B = (size_t)G.nbx * G.nby;
   for (int p = b + blockIdx.x; p < e; p += gridDim.x) {
     float4 c = rec_box[p];
     float2 s = rec_s[p];
@@ -442,19 +459,19 @@ __global__ __launch_bounds__(256) void k_dens_large(Geo G, const int* off, const
     for (int q = threadIdx.x; q < cells; q += blockDim.x) {
       int bx = x0 + q % nx, by = y0 + q / nx;
       float v = overlap(c.x, hw, G.bin_lx[bx], G.bin_ux[bx]) * overlap(c.y, hh, G.bin_ly[by], G.bin_uy[by]) * s.x;
-      int t = bx / kTile + (by / kTile) * G.tiles_x;
-      float* dst = slab + (size_t)t * 2 * S2 + (s.y != 0.f ? S2 : 0);
-      atomicAdd(&dst[(by % kTile) * SW + (bx % kTile)], v);
+      atomicAdd(&big[(s.y != 0.f ? B : 0) + (size_t)bx + (size_t)by * G.nbx], to_fixed(G, v));
     }
   }
 }
 
 // per bin: own slab entry plus the halo entries of the tiles left / below /
 // below-left; density and overflow (computeOverflow, DensityGradient.cu:140-156)
-__global__ __launch_bounds__(256) void k_dens_resolve(Geo G, const float* slab, const float* fixed,
+__global__ __launch_bounds__(256) void k_dens_resolve(Geo G, const int* off, const float* slab,
+                                                      unsigned long long* big, const float* fixed,
                                                       const float* scaled, float* mov, float* fil, float* dens,
-                                                      double* ovf_acc) {
+                                                      double* ovf_part) {
   const int B = G.nbx * G.nby;
+  const bool large = off[G.T + 1] > off[G.T];
   const int SW = G.SW, S2 = SW * SW;
   double ovf = 0.0;
   for (int bin = blockIdx.x * blockDim.x + threadIdx.x; bin < B; bin += gridDim.x * blockDim.x) {
@@ -477,6 +494,12 @@ __global__ __launch_bounds__(256) void k_dens_resolve(Geo G, const float* slab, 
         f += s3[S2 + (ly + kTile) * SW + lx + kTile];
       }
     }
+    if (large) {
+      m += (float)((double)(long long)big[bin] * G.fx_inv);
+      f += (float)((double)(long long)big[(size_t)B + bin] * G.fx_inv);
+      big[bin] = 0ull;
+      big[(size_t)B + bin] = 0ull;
+    }
     mov[bin] = m;
     fil[bin] = f;
     float fx = fixed[bin], sc = scaled[bin];
@@ -487,7 +510,7 @@ __global__ __launch_bounds__(256) void k_dens_resolve(Geo G, const float* slab, 
   for (int o = 32; o > 0; o >>= 1) ovf += __shfl_down(ovf, o);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ovf;
   __syncthreads();
-  if (threadIdx.x == 0) atomicAdd(ovf_acc, red[0] + red[1] + red[2] + red[3]);
+  if (threadIdx.x == 0) ovf_part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
 // getDensityGradForEachCell (DensityGradient.cu:173-281): the GPU Poisson
@@ -523,8 +546,19 @@ __global__ void k_dens_gather(Geo G, const int* off, const float4* rec_box, cons
   precond[i] = pc;
 }
 
-__global__ void k_dens_ratio(const double* ovf_acc, double inv_area, float* out) {
-  *out = (float)(*ovf_acc * inv_area);
+// the overflow ratio from the resolve pass's per-block partials, summed in
+// block order
+__global__ __launch_bounds__(64) void k_dens_ratio(const double* ovf_part, int nparts, double inv_area, float* out) {
+  __shared__ double sh[64];
+  double s = 0.0;
+  for (int k = threadIdx.x; k < nparts; k += 64) s += ovf_part[k];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int k = 0; k < 64; k++) t += sh[k];
+    *out = (float)(t * inv_area);
+  }
 }
 
 template <typename T>
@@ -572,7 +606,8 @@ struct DensHandle {
   int2* rec_ig = nullptr;
   float *slab = nullptr, *mov = nullptr, *fil = nullptr, *dens = nullptr;
   float *psi = nullptr, *ex = nullptr, *ey = nullptr, *precond = nullptr;
-  double* ovf_acc = nullptr;
+  unsigned long long* big = nullptr;  // [2][B] fixed-point sums of the cells wider than the halo
+  double* ovf_part = nullptr;         // [kResolveBlocks]
   float* ovf_val = nullptr;
   PoissonHandle* ps = nullptr;
 };
@@ -845,6 +880,14 @@ int dens_create(const DensGeom* g, DensHandle** out) {
   if ((st = dupload(h->bufs, &bly, g->bin_ly, nby))) return bail(st);
   if ((st = dupload(h->bufs, &buy, g->bin_uy, nby))) return bail(st);
   G.bin_lx = blx; G.bin_ux = bux; G.bin_ly = bly; G.bin_uy = buy;
+  // fixed-point scale: sums of up to 2^20 bin areas fit in 62 bits
+  float max_bw = 0.f, max_bh = 0.f;
+  for (int x = 0; x < nbx; x++) max_bw = std::max(max_bw, g->bin_ux[x] - g->bin_lx[x]);
+  for (int y = 0; y < nby; y++) max_bh = std::max(max_bh, g->bin_uy[y] - g->bin_ly[y]);
+  const double amax = std::max((double)max_bw * (double)max_bh * std::max(1.0f, g->target_density), 1e-30);
+  const int fx_exp = std::min(100, (int)std::floor(62.0 - 20.0 - std::log2(amax)));
+  G.fx_scale = std::ldexp(1.0f, fx_exp);
+  G.fx_inv = std::ldexp(1.0, -fx_exp);
   if ((st = dupload(h->bufs, &h->cgeo, cg.data(), N))) return bail(st);
   if ((st = dupload(h->bufs, &h->cfill, cf.data(), N))) return bail(st);
   if ((st = dupload(h->bufs, &h->fixed, g->fixed_area, B))) return bail(st);
@@ -868,15 +911,17 @@ int dens_create(const DensGeom* g, DensHandle** out) {
   if ((st = dalloc(h->bufs, &h->ex, B))) return bail(st);
   if ((st = dalloc(h->bufs, &h->ey, B))) return bail(st);
   if ((st = dalloc(h->bufs, &h->precond, N))) return bail(st);
-  if ((st = dalloc(h->bufs, &h->ovf_acc, 1))) return bail(st);
+  if ((st = dalloc(h->bufs, &h->big, 2 * B))) return bail(st);
+  if (hipMemset(h->big, 0, 2 * B * sizeof(unsigned long long)) != hipSuccess)
+    return bail(fail(DENS_ERR_DEVICE, "hipMemset"));
+  if ((st = dalloc(h->bufs, &h->ovf_part, kResolveBlocks))) return bail(st);
   if ((st = dalloc(h->bufs, &h->ovf_val, 1))) return bail(st);
-  // tile shapes of the Poisson GEMMs tuned now for every precision, so that
-  // neither dens_compute_grad nor dens_set_precision waits on the host (the
-  // timing does not depend on the values)
+  // tile shape of the Poisson GEMMs tuned now for the handle's precision, so
+  // that dens_compute_grad does not wait on the host (the timing does not
+  // depend on the values); another precision is tuned at its first use
   if (hipMemset(h->dens, 0, (size_t)B * sizeof(float)) != hipSuccess)
     return bail(fail(DENS_ERR_DEVICE, "hipMemset"));
-  for (int p = 0; p < 3; p++)
-    if ((st = poisson_tune(h->ps, h->dens, h->psi, h->ex, h->ey, p, nullptr))) return bail(st);
+  if ((st = poisson_tune(h->ps, h->dens, h->psi, h->ex, h->ey, h->prec, nullptr))) return bail(st);
   if (hipDeviceSynchronize() != hipSuccess) return bail(fail(DENS_ERR_DEVICE, "tuning"));
   *out = h;
   return DENS_OK;
@@ -889,7 +934,7 @@ int dens_compute_grad(DensHandle* h, const float* cx, const float* cy, float* gx
   if (G.N > 0 && (!cx || !cy || !gx || !gy)) return fail(DENS_ERR_ARG, "null cell array");
   hipStream_t s = (hipStream_t)stream;
   const int nb = (G.N + 255) / 256;
-  hipLaunchKernelGGL(k_dens_reset, dim3((G.T + 256) / 256), dim3(256), 0, s, h->cnt, G.T + 1, h->ovf_acc);
+  hipLaunchKernelGGL(k_dens_reset, dim3((G.T + 256) / 256), dim3(256), 0, s, h->cnt, G.T + 1);
   if (G.N > 0) {
     // LDS histogram of the buckets when it is small (up to 64 x 64 tiles)
     const size_t hist = (size_t)(G.T + 1) * sizeof(int);
@@ -905,12 +950,13 @@ int dens_compute_grad(DensHandle* h, const float* cx, const float* cy, float* gx
     hipLaunchKernelGGL(k_dens_fill, dim3(nb), dim3(256), 0, s, G, cx, cy, h->cgeo, h->cfill, h->bucket, h->rank,
                        h->off, h->rec_box, h->rec_s, h->rec_ig, gx, gy, h->precond);
   }
-  size_t lds = (size_t)2 * G.SW * G.SW * sizeof(float);
+  size_t lds = (size_t)2 * G.SW * G.SW * sizeof(unsigned long long);
   hipLaunchKernelGGL(k_dens_accum, dim3(G.T), dim3(256), lds, s, G, h->off, h->rec_box, h->rec_s, h->slab);
-  hipLaunchKernelGGL(k_dens_large, dim3(256), dim3(256), 0, s, G, h->off, h->rec_box, h->rec_s, h->slab);
-  hipLaunchKernelGGL(k_dens_resolve, dim3(std::min((h->B + 255) / 256, 4096)), dim3(256), 0, s, G, h->slab,
-                     h->fixed, h->scaled, h->mov, h->fil, h->dens, h->ovf_acc);
-  hipLaunchKernelGGL(k_dens_ratio, dim3(1), dim3(1), 0, s, h->ovf_acc, h->inv_sum_area, h->ovf_val);
+  hipLaunchKernelGGL(k_dens_large, dim3(256), dim3(256), 0, s, G, h->off, h->rec_box, h->rec_s, h->big);
+  const int nres = std::min((h->B + 255) / 256, kResolveBlocks);
+  hipLaunchKernelGGL(k_dens_resolve, dim3(nres), dim3(256), 0, s, G, h->off, h->slab, h->big, h->fixed, h->scaled,
+                     h->mov, h->fil, h->dens, h->ovf_part);
+  hipLaunchKernelGGL(k_dens_ratio, dim3(1), dim3(64), 0, s, h->ovf_part, nres, h->inv_sum_area, h->ovf_val);
   DENS_TRY(hipGetLastError());
   int st = poisson_run(h->ps, h->dens, h->psi, h->ex, h->ey, h->prec, s);
   if (st) return st;
@@ -934,6 +980,7 @@ int dens_get_overflow(DensHandle* h, float* out) {
   return DENS_OK;
 }
 
+const float* dens_get_overflow_device(DensHandle* h) { return h ? h->ovf_val : nullptr; }
 const float* dens_get_bin_density(DensHandle* h) { return h ? h->dens : nullptr; }
 const float* dens_get_potential(DensHandle* h) { return h ? h->psi : nullptr; }
 const float* dens_get_field_x(DensHandle* h) { return h ? h->ex : nullptr; }

@@ -34,9 +34,9 @@ void walk(float lo, float hi, int n, float size, std::vector<float>& l, std::vec
 
 // createBins (SkyPlaceDB.cpp:446-499): power-of-two bin count per side from the
 // average movable area, at most 1024 (x the aspect ratio 1, 2 or 4)
-void auto_counts(const DensDesign* d, float sum_movable_area, int& nbx, int& nby) {
+void auto_counts(const DensDesign* d, float sum_movable_area, int num_movable, int& nbx, int& nby) {
   const float die_area = (d->die_ux - d->die_lx) * (d->die_uy - d->die_ly);
-  const float avg = sum_movable_area / (float)d->num_movable;
+  const float avg = sum_movable_area / (float)num_movable;
   const float ideal_area = avg / d->target_density;
   const float ideal_count = die_area / ideal_area;
   const int ideal = (int)std::ceil(std::sqrt(ideal_count));
@@ -70,14 +70,22 @@ extern "C" int dens_prepare(const DensDesign* d, int32_t* shape, float* bin_lx, 
   if (!(d->die_ux > d->die_lx) || !(d->die_uy > d->die_ly)) return prep_fail("empty die");
   if (!(d->target_density > 0.f)) return prep_fail("target density must be positive");
   const int N = d->num_movable;
-  float sum_area = 0.f;  // sumMovableArea_ (SkyPlaceDB.cpp:197), FP32 accumulation
-  for (int i = 0; i < N; i++) sum_area += d->cell_dx[i] * d->cell_dy[i];
+  // sumMovableArea_ (SkyPlaceDB.cpp:197), FP32 accumulation, over the cells
+  // that are not fillers (importDB runs before createFillers)
+  float sum_area = 0.f;
+  int n_real = 0;
+  for (int i = 0; i < N; i++) {
+    if (d->is_filler && d->is_filler[i]) continue;
+    sum_area += d->cell_dx[i] * d->cell_dy[i];
+    n_real++;
+  }
+  if (n_real == 0) return prep_fail("no movable cells besides fillers");
   int nbx, nby;
   if (d->num_bin_x > 0 && d->num_bin_y > 0) {
     nbx = d->num_bin_x;
     nby = d->num_bin_y;
   } else {
-    auto_counts(d, sum_area, nbx, nby);
+    auto_counts(d, sum_area, n_real, nbx, nby);
   }
   shape[0] = nbx;
   shape[1] = nby;

@@ -61,7 +61,8 @@ struct WlArgs {
   const int* cell_start;
   const int* cell_pin;
   const int* big;
-  double* hpwl;  // [1] accumulator of wl_compute_hpwl
+  double* hpwl;       // [1] result of wl_compute_hpwl
+  double* hpwl_part;  // [kHpwlBlocks] per-block partial sums
 };
 
 // Pin centre: from its movable cell's centre (FROM_CELLS), else the stored one
@@ -202,12 +203,15 @@ __global__ __launch_bounds__(256) void k_wl_update(WlArgs a, const float* cx, co
 }
 
 // computeNetBBox + computeHPWL (WireLengthGradient.cu:213-235, 383-401):
-// (max - min) per net and axis with the reference's initial values, summed in FP64.
-__global__ __launch_bounds__(256) void k_wl_hpwl(WlArgs a) {
+// (max - min) per net and axis with the reference's initial values, summed in
+// FP64: per-block partials over a fixed grid, then k_wl_hpwl_sum adds them in
+// block order (the same bits on every call).
+constexpr int kHpwlBlocks = 1024;
+
+__global__ __launch_bounds__(256) void k_wl_hpwl(WlArgs a, double* part) {
   __shared__ double sh[4];
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
   double v = 0.0;
-  if (n < a.num_nets) {
+  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < a.num_nets; n += gridDim.x * blockDim.x) {
     float mxx = 0.0f, mxy = 0.0f, mnx = FLT_MAX, mny = FLT_MAX;
     for (int p = a.net_start[n]; p < a.net_start[n + 1]; p++) {
       const float2 q = a.pin_xy[p];
@@ -216,10 +220,23 @@ __global__ __launch_bounds__(256) void k_wl_hpwl(WlArgs a) {
       mxy = fmaxf(q.y, mxy);
       mny = fminf(q.y, mny);
     }
-    v = (double)(mxx - mnx) + (double)(mxy - mny);
+    v += (double)(mxx - mnx) + (double)(mxy - mny);
   }
   v = block_reduce(v, [](double x, double y) { return x + y; }, sh);
-  if (threadIdx.x == 0) atomicAdd(a.hpwl, v);
+  if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
+__global__ __launch_bounds__(64) void k_wl_hpwl_sum(const double* part, int nparts, double* out) {
+  __shared__ double sh[64];
+  double s = 0.0;
+  for (int k = threadIdx.x; k < nparts; k += 64) s += part[k];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int k = 0; k < 64; k++) t += sh[k];
+    *out = t;
+  }
 }
 
 // jacobianPrecondition (ProblemInstanceKernel.hpp:74-100)
@@ -320,17 +337,18 @@ int wl_create(const WlNetlist* n, WlHandle** out) {
   int *d_ns, *d_p2c, *d_cs, *d_cp, *d_big;
   float* d_w;
   float2 *d_off, *d_xy, *d_g;
-  double* d_h;
+  double *d_h, *d_hp;
   if ((st = up(h, &d_ns, n->net_start, (size_t)n->num_nets + 1)) || (st = up(h, &d_w, w.data(), w.size())) ||
       (st = up(h, &d_p2c, n->pin2cell, (size_t)n->num_pins)) || (st = up(h, &d_off, off.data(), off.size())) ||
       (st = up(h, &d_xy, xy.data(), xy.size())) || (st = up(h, &d_g, (const float2*)nullptr, xy.size())) ||
       (st = up(h, &d_cs, cstart.data(), cstart.size())) || (st = up(h, &d_cp, cpin.data(), cpin.size())) ||
-      (st = up(h, &d_big, big.data(), big.size())) || (st = up(h, &d_h, (const double*)nullptr, 1))) {
+      (st = up(h, &d_big, big.data(), big.size())) || (st = up(h, &d_h, (const double*)nullptr, 1)) ||
+      (st = up(h, &d_hp, (const double*)nullptr, (size_t)kHpwlBlocks))) {
     wl_destroy(h);
     return st;
   }
   a.net_start = d_ns; a.net_w = d_w; a.pin2cell = d_p2c; a.pin_off = d_off; a.pin_xy = d_xy; a.pin_g = d_g;
-  a.cell_start = d_cs; a.cell_pin = d_cp; a.big = d_big; a.hpwl = d_h;
+  a.cell_start = d_cs; a.cell_pin = d_cp; a.big = d_big; a.hpwl = d_h; a.hpwl_part = d_hp;
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
     wl_destroy(h);
     return fail(WL_ERR_DEVICE, "hipStreamCreate");
@@ -358,13 +376,21 @@ int wl_compute_grad_at(WlHandle* h, const float* cx, const float* cy, float g_in
   return launch_grad<true>(h, cx, cy, g_inv, gx, gy, pick(h, stream));
 }
 
+int wl_compute_hpwl_async(WlHandle* h, double* d_out, void* stream) {
+  if (!h || !d_out) return fail(WL_ERR_ARG, "null argument");
+  hipStream_t s = pick(h, stream);
+  const int nb = std::max(1, std::min(kHpwlBlocks, (h->a.num_nets + 255) / 256));
+  hipLaunchKernelGGL(k_wl_hpwl, dim3(nb), dim3(256), 0, s, h->a, h->a.hpwl_part);
+  hipLaunchKernelGGL(k_wl_hpwl_sum, dim3(1), dim3(64), 0, s, h->a.hpwl_part, nb, d_out);
+  WL_TRY(hipGetLastError());
+  return WL_OK;
+}
+
 int wl_compute_hpwl(WlHandle* h, double* out, void* stream) {
   if (!h || !out) return fail(WL_ERR_ARG, "null argument");
   hipStream_t s = pick(h, stream);
-  WL_TRY(hipMemsetAsync(h->a.hpwl, 0, sizeof(double), s));
-  if (h->a.num_nets > 0)
-    hipLaunchKernelGGL(k_wl_hpwl, dim3((h->a.num_nets + 255) / 256), dim3(256), 0, s, h->a);
-  WL_TRY(hipGetLastError());
+  int st = wl_compute_hpwl_async(h, h->a.hpwl, stream);
+  if (st) return st;
   WL_TRY(hipMemcpyAsync(out, h->a.hpwl, sizeof(double), hipMemcpyDeviceToHost, s));
   WL_TRY(hipStreamSynchronize(s));
   return WL_OK;

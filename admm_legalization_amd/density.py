@@ -42,6 +42,7 @@ class DensGeom(C.Structure):
 
 
 EXPORTED = ["dens_last_error", "dens_prepare", "dens_create", "dens_compute_grad", "dens_get_overflow",
+            "dens_get_overflow_device",
             "dens_get_bin_density", "dens_get_potential", "dens_get_field_x", "dens_get_field_y",
             "dens_get_preconditioner", "dens_get_movable_area", "dens_get_filler_area",
             "dens_set_precision", "dens_destroy", "poisson_create", "poisson_solve", "poisson_solve_noalloc",
@@ -65,7 +66,7 @@ def lib():
         L.dens_create.argtypes = [C.POINTER(DensGeom), C.POINTER(C.c_void_p)]
         L.dens_compute_grad.argtypes = [C.c_void_p] + [C.c_void_p] * 4 + [_f32p, C.c_void_p]
         L.dens_get_overflow.argtypes = [C.c_void_p, _f32p]
-        for name in ("dens_get_bin_density", "dens_get_potential", "dens_get_field_x", "dens_get_field_y",
+        for name in ("dens_get_overflow_device", "dens_get_bin_density", "dens_get_potential", "dens_get_field_x", "dens_get_field_y",
                      "dens_get_preconditioner", "dens_get_movable_area", "dens_get_filler_area"):
             getattr(L, name).argtypes = [C.c_void_p]
             getattr(L, name).restype = C.c_void_p

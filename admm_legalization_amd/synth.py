@@ -281,6 +281,75 @@ def with_netlist(d: Design, nets_per_cell: float = 1.2, io_frac: float = 0.02, s
     return e
 
 
+def _morton(x, y):
+    """Interleaved bits of 16-bit x and y (Z-order key)."""
+    def spread(v):
+        v = v.astype(np.uint64) & np.uint64(0xFFFF)
+        v = (v | (v << np.uint64(8))) & np.uint64(0x00FF00FF)
+        v = (v | (v << np.uint64(4))) & np.uint64(0x0F0F0F0F)
+        v = (v | (v << np.uint64(2))) & np.uint64(0x33333333)
+        return (v | (v << np.uint64(1))) & np.uint64(0x55555555)
+    return spread(x) | (spread(y) << np.uint64(1))
+
+
+def with_power_law_netlist(d: Design, nets_per_cell: float = 1.05, min_deg: int = 2, max_deg: int = 1000,
+                           mean_deg: float = 3.8, io_frac: float = 0.005, local: bool = True,
+                           seed: int = 0) -> Design:
+    """A copy of `d` with a random netlist whose net degrees follow a power
+    law on [min_deg, max_deg] with mean `mean_deg`: pins on movable cells and
+    fixed macros at integer offsets inside them, a fraction `io_frac` of IO
+    pins on the core boundary.  With `local` a net's cells are drawn from a
+    window of 8 x degree (at least 64) consecutive instances in Z order of
+    the design's positions, so that nets are short in a good placement (as
+    in real netlists); otherwise from all instances uniformly."""
+    rng = np.random.default_rng(seed)
+    e = d.copy_positions(d.inst_lx, d.inst_ly)
+    cand = np.flatnonzero(d.movable_mask() | (d.inst_macro == 1))
+    nn = max(1, int(round(nets_per_cell * int(d.movable_mask().sum()))))
+    a = _power_law_alpha(min_deg, max_deg, mean_deg)
+    dd = np.arange(min_deg, max_deg + 1)
+    p = dd.astype(np.float64) ** -a
+    deg = rng.choice(dd, size=nn, p=p / p.sum()).astype(np.int64)
+    start = np.zeros(nn + 1, np.int64)
+    start[1:] = np.cumsum(deg)
+    npins = int(start[-1])
+    if local:
+        lx, ly, ux, uy = d.core
+        cxs = (d.inst_lx[cand] + d.inst_dx[cand] / 2.0 - lx) / max(ux - lx, 1) * 65535
+        cys = (d.inst_ly[cand] + d.inst_dy[cand] / 2.0 - ly) / max(uy - ly, 1) * 65535
+        order = cand[np.argsort(_morton(np.clip(cxs, 0, 65535), np.clip(cys, 0, 65535)), kind="stable")]
+        win = np.maximum(8 * deg, 64)
+        anchor = (rng.random(nn) * np.maximum(cand.size - win, 1)).astype(np.int64)
+        net_of_pin = np.repeat(np.arange(nn), deg)
+        pos = anchor[net_of_pin] + (rng.random(npins) * win[net_of_pin]).astype(np.int64)
+        inst = order[np.minimum(pos, cand.size - 1)].astype(np.int32)
+    else:
+        inst = cand[rng.integers(0, cand.size, npins)].astype(np.int32)
+    io = rng.random(npins) < io_frac
+    dx = (rng.random(npins) * d.inst_dx[inst]).astype(np.int32)
+    dy = (rng.random(npins) * d.inst_dy[inst]).astype(np.int32)
+    lx, ly, ux, uy = d.core
+    side = rng.integers(0, 2, npins)
+    dx = np.where(io, np.where(side == 0, lx, ux), dx).astype(np.int32)
+    dy = np.where(io, rng.integers(ly, uy, npins), dy).astype(np.int32)
+    e.net_start = start.astype(np.int32)
+    e.pin_inst = np.where(io, -1, inst).astype(np.int32)
+    e.pin_dx, e.pin_dy = dx, dy
+    e.meta = dict(d.meta, nets=nn, pins=npins, degree_alpha=a)
+    return e
+
+
+def e2e(num_cells: int, util: float = 0.9, seed: int = 0, macros_per_mcell: float = 150.0,
+        nets_per_cell: float = 1.05) -> Design:
+    """The end-to-end placement recipe at `num_cells` cells: the mixed-height
+    cells at `util` with fixed macros (150 per million cells: 300 at 2M) and
+    no fences, and a power-law netlist (degrees 2-1000, mean 3.8, 1.05 nets
+    per cell: 2.1M nets at 2M cells)."""
+    d = mixed(num_cells, util=util, seed=seed, num_fences=0, macros_per_mcell=macros_per_mcell,
+              name=f"e2e_{num_cells}")
+    return with_power_law_netlist(d, nets_per_cell=nets_per_cell, seed=seed + 1)
+
+
 def _calibrate_width(n, free_sites, util):
     mean_h = sum(h * p for h, p in MIXED_HEIGHTS.items())
     target_w = util * free_sites / n / mean_h

@@ -21,7 +21,7 @@ _f32p = C.POINTER(C.c_float)
 _i32p = C.POINTER(C.c_int32)
 
 EXPORTED = ["wl_last_error", "wl_create", "wl_update_pin_coordinates", "wl_compute_grad",
-            "wl_compute_grad_at", "wl_compute_hpwl", "wl_get_pins", "wl_destroy", "tf_precondition"]
+            "wl_compute_grad_at", "wl_compute_hpwl", "wl_compute_hpwl_async", "wl_get_pins", "wl_destroy", "tf_precondition"]
 
 
 class WlNetlist(C.Structure):
@@ -48,6 +48,7 @@ def lib():
         L.wl_compute_grad_at.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_void_p, C.c_void_p,
                                          C.c_void_p]
         L.wl_compute_hpwl.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_void_p]
+        L.wl_compute_hpwl_async.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.wl_get_pins.argtypes = [C.c_void_p, _f32p, _f32p]
         L.wl_destroy.argtypes = [C.c_void_p]
         L.tf_precondition.argtypes = [C.c_int32, C.c_float, C.c_float] + [C.c_void_p] * 8 + [C.c_void_p]

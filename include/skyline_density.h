@@ -91,7 +91,10 @@ typedef struct PoissonHandle PoissonHandle;
 
 const char* dens_last_error(void);
 
-/* SkyPlaceDB bin / density-size / fixed-area setup.  Call with
+/* SkyPlaceDB bin / density-size / fixed-area setup.  Cells flagged in
+ * is_filler are sized like the others but left out of sum_movable_area and
+ * of the automatic bin count, which the reference derives before
+ * createFillers (SkyPlaceDB.cpp:320-355).  Call with
  * bin_lx == NULL to receive shape[0..1] = num_bin_x, num_bin_y; then with
  * arrays of that size: bin bounds, fixed_area and scaled_bin_area [B],
  * density sizes and scales [num_movable]; scalars[0..2] = bin_w, bin_h,
@@ -100,6 +103,10 @@ int dens_prepare(const DensDesign* d, int32_t* shape, float* bin_lx, float* bin_
                  float* bin_uy, float* fixed_area, float* scaled_bin_area, float* cell_dw,
                  float* cell_dh, float* cell_scale, float* scalars);
 
+/* Tunes the Poisson solver's tile shape for g->precision up front (so that
+ * dens_compute_grad never waits on the host); another precision chosen with
+ * dens_set_precision is tuned at its first evaluation, which then waits for
+ * the stream once. */
 int dens_create(const DensGeom* g, DensHandle** out);
 /* One density evaluation (DensityGradient::computeGrad): scatter, overflow,
  * Poisson solve, gradient.  d_cx / d_cy are cell centres, d_gx / d_gy
@@ -109,6 +116,10 @@ int dens_compute_grad(DensHandle* h, const float* d_cx, const float* d_cy, float
                       float* overflow_out, void* stream);
 /* Overflow of the last dens_compute_grad (synchronises). */
 int dens_get_overflow(DensHandle* h, float* out);
+/* Device address of the overflow ratio of the last dens_compute_grad (one
+ * float, valid once the stream reaches it): lets a caller read it with other
+ * values in one copy.  No counterpart in the reference (a host float there). */
+const float* dens_get_overflow_device(DensHandle* h);
 /* Device arrays owned by the handle: bin density, potential, field x / y
  * ([B] each) and the per-cell preconditioner ([N]). */
 const float* dens_get_bin_density(DensHandle* h);

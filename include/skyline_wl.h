@@ -57,6 +57,9 @@ int wl_compute_grad_at(WlHandle* h, const float* d_cell_cx, const float* d_cell_
                        float* d_grad_x, float* d_grad_y, void* stream);
 /* computeHPWL (WireLengthGradient.cu:383-401), summed in FP64; synchronises. */
 int wl_compute_hpwl(WlHandle* h, double* hpwl, void* stream);
+/* computeHPWL into the device double *d_out, enqueued on the stream without
+ * waiting (no counterpart in the reference, which returns it to the host). */
+int wl_compute_hpwl_async(WlHandle* h, double* d_out, void* stream);
 /* Host copy of the stored pin centres (synchronises); either may be NULL. */
 int wl_get_pins(WlHandle* h, float* pin_x, float* pin_y);
 int wl_destroy(WlHandle* h);

@@ -7,10 +7,11 @@ import subprocess
 
 import pytest
 
-from admm_legalization_amd import build, density, lg, wirelength
+from admm_legalization_amd import build, density, gp, lg, wirelength
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADERS = [os.path.join(ROOT, "include", h) for h in ("skyline_lg.h", "skyline_density.h", "skyline_wl.h")]
+HEADERS = [os.path.join(ROOT, "include", h)
+           for h in ("skyline_lg.h", "skyline_density.h", "skyline_wl.h", "skyline_gp.h")]
 
 
 def declared(path):
@@ -27,13 +28,14 @@ def library():
 
 
 def test_headers_declare_the_expected_entry_points():
-    lg_syms, dens_syms, wl_syms = (declared(h) for h in HEADERS)
+    lg_syms, dens_syms, wl_syms, gp_syms = (declared(h) for h in HEADERS)
     assert {"lg_create", "lg_solve", "lg_postprocess", "lg_check", "lg_copy_out", "lg_destroy"} <= lg_syms
     assert {"dens_create", "dens_compute_grad", "poisson_create", "poisson_solve"} <= dens_syms
     assert {"wl_create", "wl_compute_grad", "wl_update_pin_coordinates", "tf_precondition"} <= wl_syms
     assert set(lg.EXPORTED) == lg_syms
     assert set(density.EXPORTED) == dens_syms
     assert set(wirelength.EXPORTED) == wl_syms
+    assert set(gp.EXPORTED) == gp_syms
 
 
 def test_library_exports_every_declared_symbol(library):
@@ -69,6 +71,22 @@ def test_null_arguments_are_rejected_without_a_device(library):
     assert Wl.wl_create(None, None) == 1 and b"null" in Wl.wl_last_error()
     assert Wl.tf_precondition(-1, 0.0, 1.0, *([None] * 8), None) == 1
     assert D.poisson_solve(None, None, None, None, None, 1, None) != 0
+    P = gp.lib()
+    h = C.c_void_p()
+    assert P.gp_create(None, None, C.byref(h)) == 1 and b"null" in P.gp_last_error()
+    assert P.gp_iterate(None, 1, None) == 1
+
+
+def test_placer_default_params_are_the_reference_hyper_parameters(library):
+    """HyperParam (HyperParam.h:23-48) and SkyPlaceDB's target density 1.0."""
+    p = gp.default_params()
+    assert (p.max_opt_iter, p.max_backtrack_iter, p.log_freq) == (2500, 10, 50)
+    for k, v in dict(init_lambda=1e-4, init_gamma_inv_coef=0.01, target_overflow=0.07, min_phi_coef=0.95,
+                     max_phi_coef=1.05, min_precond=1.0, init_opt_coef=100.0, reference_hpwl=200000.0,
+                     min_step_length=1.0, adam_alpha=100.0, adam_beta1=0.9, adam_beta2=0.999,
+                     target_density=1.0).items():
+        assert getattr(p, k) == pytest.approx(v, rel=1e-7), k
+    assert p.solver == gp.SOLVER_NESTEROV and p.num_bin_x == 0 and p.precision == density.PREC_BF16
 
 
 def test_cpp_headers_compile_and_link(library, tmp_path):
