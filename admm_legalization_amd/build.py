@@ -33,6 +33,7 @@ UNITS = [
     ("dens_host.cpp", "cxx"),
     ("wl_device.hip", "hip"),
     ("gp_device.hip", "hip"),
+    ("gp_host.cpp", "cxx"),
 ]
 
 

@@ -4,8 +4,18 @@
 // comment) and runs them in order:
 //
 //   read_design <file.lgd>         design in the LGD1 format (lgd_file.hpp)
+//   global_place [options]         CmdGlobalPlace (tclCmd/CmdGlobalPlace.hpp:11-93):
+//                                  the placed positions replace the design's
 //   legalize [options]             CmdLegalize (tclCmd/CmdLegalize.hpp:14-152)
 //   write_result <file.lgr>        legalized positions and flips (LGR1)
+//
+// global_place takes the reference's options -target_density <f>
+// -target_overflow <f> -initial_place <random> -initial_place_only
+// -initial_lambda <f> -initial_gamma_inv <f> -solver <nesterov|adam> -plot
+// <dir> (accepted, nothing drawn), and these, which have no counterpart in
+// the reference's command: -bins <n> (n x n bins), -scaling_factor <f>
+// (coordinates divided by f), -ref_hpwl <f>, -ref_hpwl_ratio <f>,
+// -precision <BF16|SPLIT_BF16|F32>, -max_iter <n>, -seed <n>.
 //
 // legalize takes the reference's options verbatim: -constraint <file>
 // -tech_penalty <f> -max_iter <i> -std_admm -coeff_to_init_rho <f>
@@ -29,6 +39,7 @@
 #include <vector>
 
 #include "../../include/skyline/Legalizer.hpp"
+#include "../../include/skyline/SkyPlace.hpp"
 #include "lgd_file.hpp"
 
 namespace {
@@ -87,6 +98,54 @@ struct Driver {
     std::printf("read_design: %d instances, %d rows, %d nets\n", f->d.num_insts, f->d.num_rows, f->d.num_nets);
     design = std::move(f);
     legalized = false;
+    return true;
+  }
+
+  // CmdGlobalPlace::execute (CmdGlobalPlace.hpp:27-93)
+  bool global_place(const std::vector<std::string>& tok) {
+    Command c;
+    for (const char* o : {"-target_density", "-target_overflow", "-initial_place", "-initial_lambda",
+                          "-initial_gamma_inv", "-solver", "-plot", "-bins", "-scaling_factor", "-ref_hpwl",
+                          "-ref_hpwl_ratio", "-precision", "-max_iter", "-seed"})
+      c.add(o, 1);
+    c.add("-initial_place_only", 0);
+    std::string err;
+    if (!c.parse(tok, err)) { std::printf("%s\n", err.c_str()); return false; }
+    if (!design) { std::printf("global_place: no design (read_design first)\n"); return false; }
+    skyplace::SkyPlace sp(&design->d);
+    if (auto o = c.get("-target_overflow")) sp.setTargetOverflow(std::stof(o->arg));
+    if (auto o = c.get("-target_density")) sp.setTargetDensity(std::stof(o->arg));
+    if (auto o = c.get("-initial_place")) sp.setInitialPlaceMethod(o->arg);
+    if (c.get("-initial_place_only")) sp.setInitialPlaceOnly();
+    if (auto o = c.get("-initial_lambda")) sp.setInitLambda(std::stof(o->arg));
+    if (auto o = c.get("-initial_gamma_inv")) sp.setInitGammaInv(std::stof(o->arg));
+    if (auto o = c.get("-solver")) sp.setSolver(o->arg);
+    if (auto o = c.get("-plot")) sp.setPlotMode(o->arg);
+    GpParams& p = sp.params();
+    if (auto o = c.get("-bins")) p.num_bin_x = p.num_bin_y = std::stoi(o->arg);
+    if (auto o = c.get("-scaling_factor")) p.scaling_factor = std::stof(o->arg);
+    if (auto o = c.get("-ref_hpwl")) sp.setRefHpwl(std::stof(o->arg));
+    if (auto o = c.get("-ref_hpwl_ratio")) p.reference_hpwl_ratio = std::stof(o->arg);
+    if (auto o = c.get("-max_iter")) p.max_opt_iter = std::stoi(o->arg);
+    if (auto o = c.get("-seed")) p.seed = std::stoull(o->arg);
+    if (auto o = c.get("-precision")) {
+      if (o->arg == "BF16") p.precision = 0;
+      else if (o->arg == "SPLIT_BF16") p.precision = 1;
+      else if (o->arg == "F32") p.precision = 2;
+      else { std::printf("global_place: unknown precision %s\n", o->arg.c_str()); return false; }
+    }
+    p.verbose = 1;
+    const int n = design->d.num_insts;
+    std::vector<int32_t> lx(n), ly(n);
+    const int st = sp.run(lx.data(), ly.data());
+    if (st) { std::printf("global_place: error %d: %s\n", st, gp_last_error()); return false; }
+    // exportDB: the design now holds the global placement
+    std::copy(lx.begin(), lx.end(), design->lx.begin());
+    std::copy(ly.begin(), ly.end(), design->ly.begin());
+    legalized = false;
+    const GpStats& s = sp.stats();
+    std::printf("[gp_summary] converged=%d iterations=%d overflow=%.6f hpwl=%.6f fillers=%d bins=%dx%d\n",
+                s.converged, s.iterations, s.overflow, s.hpwl_db, s.num_fillers, s.num_bin_x, s.num_bin_y);
     return true;
   }
 
@@ -157,6 +216,7 @@ struct Driver {
   bool run(const std::vector<std::string>& tok) {
     if (tok[0] == "read_design") return read_design(tok);
     if (tok[0] == "legalize") return legalize(tok);
+    if (tok[0] == "global_place") return global_place(tok);
     if (tok[0] == "write_result") return write_result(tok);
     std::printf("invalid command name \"%s\"\n", tok[0].c_str());
     return false;

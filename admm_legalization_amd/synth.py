@@ -350,6 +350,26 @@ def e2e(num_cells: int, util: float = 0.9, seed: int = 0, macros_per_mcell: floa
     return with_power_law_netlist(d, nets_per_cell=nets_per_cell, seed=seed + 1)
 
 
+def e2e_placer_params(d: Design, bins: int = 4096):
+    """Global-placer settings of the end-to-end recipe: BF16 transforms on
+    bins x bins, coordinates in sites (scaling factor = site width), the
+    lambda schedule's reference HPWL at 1 % of the design's own HPWL, target
+    density 0.95 (5 points above the utilisation) and an overflow target for
+    the bin size: 0.37 at 4096 x 4096 bins, where a bin is a quarter of a
+    cell (the loop then stops at the placement that 1024 x 1024 bins measure
+    at overflow 0.17), else 0.2."""
+    from . import density as D
+    from . import gp
+    p = gp.default_params()
+    p.precision = D.PREC_BF16
+    p.num_bin_x = p.num_bin_y = int(bins)
+    p.scaling_factor = float(d.site_width)
+    p.reference_hpwl_ratio = 0.01
+    p.target_density = 0.95
+    p.target_overflow = 0.37 if bins >= 4096 else 0.2
+    return p
+
+
 def _calibrate_width(n, free_sites, util):
     mean_h = sum(h * p for h, p in MIXED_HEIGHTS.items())
     target_w = util * free_sites / n / mean_h

@@ -386,14 +386,13 @@ def mixed_1m_end_to_end(repeats=3):
 
 
 def e2e_2m_end_to_end(num_cells=2_000_000, bins=4096, trace=False):
-    """BASELINE's end-to-end recipe on one GPU: the 2M-cell mixed-height
+    """The end-to-end recipe on one GPU: the 2M-cell mixed-height
     design (90 % utilisation, 300 fixed macros, 2.1M nets of power-law degree
     2-1000, mean 3.8) placed from scratch by the global placer
     (gp_place_design: random initial placement, fillers, Nesterov over WA
     wirelength + density on 4096 x 4096 bins with BF16 transforms), then
     legalized by lg_run_design from the exported positions.  Wall time of
     both calls; the design starts in host memory."""
-    from admm_legalization_amd import density as D
     from admm_legalization_amd import gp, lg, synth
     from admm_legalization_amd.design import strictly_legal
     t = time.perf_counter()
@@ -401,19 +400,7 @@ def e2e_2m_end_to_end(num_cells=2_000_000, bins=4096, trace=False):
     t_gen = time.perf_counter() - t
     print(f"# e2e_2m: {d.num_insts} instances, {len(d.net_start) - 1} nets, {int(d.net_start[-1])} pins, "
           f"generated in {t_gen:.1f}s", file=sys.stderr, flush=True)
-    p = gp.default_params()
-    p.precision = D.PREC_BF16
-    p.num_bin_x = p.num_bin_y = bins
-    # coordinates in sites (the reference's scaling factor), the lambda
-    # schedule's reference HPWL at 1 % of the design's own HPWL, a target
-    # density 5 points above the utilisation, and the overflow target for
-    # 4096 x 4096 bins (a bin is a quarter of a cell here: the loop stops at
-    # the placement that 1024 x 1024 bins measure at overflow 0.17, with the
-    # same HPWL; the legalizer takes it from there)
-    p.scaling_factor = float(d.site_width)
-    p.reference_hpwl_ratio = 0.01
-    p.target_density = 0.95
-    p.target_overflow = 0.37 if bins >= 4096 else 0.2
+    p = synth.e2e_placer_params(d, bins)
     t = time.perf_counter()
     lx, ly, gst = gp.place_design(d, p, trace_cap=4000 if trace else 0)
     t_gp = time.perf_counter() - t

@@ -98,9 +98,18 @@ def test_cpp_headers_compile_and_link(library, tmp_path):
 #include <vector>
 #include "skyline/Legalizer.hpp"
 #include "skyline/DensityGradient.hpp"
+#include "skyline/SkyPlace.hpp"
 int main() {
   LgParams p;
   lg_default_params(&p);
+  LgDesign empty{};
+  skyplace::SkyPlace sp(&empty);
+  sp.setTargetDensity(1.5f);  // out of range: 1.0
+  sp.setSolver("adam");
+  if (sp.params().target_density != 1.0f || sp.params().solver != GP_SOLVER_ADAM) return 3;
+  if (sp.params().max_opt_iter != 2500) return 4;
+  int32_t lx[1], ly[1];
+  if (sp.run(lx, ly) != GP_ERR_ARG) return 5;  // no instances: refused before any device call
   std::vector<float> dx(100, 1.5f), dy(100, 2.0f);
   DensDesign d{};
   d.die_ux = d.die_uy = 64.f;
@@ -119,7 +128,9 @@ int main() {
     libdir = os.path.dirname(library)
     subprocess.run(["g++", "-std=c++17", str(src), "-I", os.path.join(ROOT, "include"), "-L", libdir,
                     "-lskyline_amd", f"-Wl,-rpath,{libdir}", "-o", str(exe)], check=True)
-    out = subprocess.run([str(exe)], stdout=subprocess.PIPE, text=True, check=True).stdout.split()
+    res = subprocess.run([str(exe)], stdout=subprocess.PIPE, text=True, check=True).stdout
+    assert "Invalid target density (150.00%). 100.00% will be used." in res  # SkyPlace.cpp:105-108
+    out = res.splitlines()[-1].split()
     assert out[0] == "250" and int(out[1]) >= 4 and out[1] == out[2]
 
 

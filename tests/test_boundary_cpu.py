@@ -120,3 +120,11 @@ def test_command_driver_reads_designs_and_rejects_bad_options(tmp_path):
         assert r.returncode == 1 and msg in r.stdout, (bad, r.stdout)
     r = _cmd(f"read_design {tmp_path}/missing.lgd")
     assert r.returncode == 1 and "cannot open" in r.stdout
+    # global_place: CmdGlobalPlace's options and the same grammar
+    for bad, msg in (("global_place -bogus", "Unknown Option : -bogus"),
+                     ("global_place -target_overflow", "Syntax error in command global_place"),
+                     ("global_place", "no design")):
+        r = _cmd(bad)
+        assert r.returncode == 1 and msg in r.stdout, (bad, r.stdout)
+    r = _cmd(f"read_design {path}", "global_place -precision FP8")
+    assert r.returncode == 1 and "unknown precision FP8" in r.stdout

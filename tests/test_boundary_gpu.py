@@ -5,7 +5,9 @@
   instances passed through, displacement averaged over all instances, HPWL
   before / after and the contest metric, all equal to ref.run_design;
 * the command driver skyline_cmd (read_design, legalize with the Tcl
-  options, write_result), which drives the C++ class legalizer::Legalizer;
+  options, write_result), which drives the C++ class legalizer::Legalizer,
+  and its global_place (the C++ class skyplace::SkyPlace) followed by
+  legalize, equal to the Python path through the same C ABI;
 * a C++ caller of the device layer (tests/cpp/device_caller.cpp:
   lg_import_and_preplace, lg_create .. lg_destroy) against ref.solve.
 """
@@ -15,7 +17,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from admm_legalization_amd import build, lg, synth
+from admm_legalization_amd import build, gp, lg, synth
 from admm_legalization_amd.design import strictly_legal
 from oracle import ref
 
@@ -113,3 +115,35 @@ def test_cpp_device_layer_caller(gpu, tmp_path):
     np.testing.assert_array_equal(raw[1 + n:1 + 2 * n], o["ly"])
     np.testing.assert_array_equal(raw[1 + 2 * n:1 + 3 * n], o["rot"])
     assert f"total_disp_dbu={o['stats']['total_disp_dbu']}" in r.stdout
+
+
+def test_command_driver_global_place_then_legalize(gpu, tmp_path):
+    d = synth.e2e(8000, seed=3)
+    dpath, rpath = str(tmp_path / "d.lgd"), str(tmp_path / "r.lgr")
+    d.save_lgd(dpath)
+    script = tmp_path / "flow.tcl"
+    script.write_text(f"read_design {dpath}\n"
+                      "global_place -target_density 0.95 -bins 64 -scaling_factor 100 -ref_hpwl_ratio 0.01 "
+                      "-precision F32\n"
+                      "legalize\n"
+                      f"write_result {rpath}\n")
+    print("[case] skyline_cmd global_place + legalize", flush=True)
+    r = subprocess.run([build.CMD, str(script)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                       timeout=300)
+    print(r.stdout[-2000:], flush=True)
+    assert r.returncode == 0
+    assert "[gp_summary] converged=1" in r.stdout and "[summary] converged=1" in r.stdout
+    assert "Nesterov Solve Start" in r.stdout
+    raw = open(rpath, "rb").read()
+    n = int(np.frombuffer(raw[4:8], np.int32)[0])
+    lx = np.frombuffer(raw[8:8 + 4 * n], np.int32)
+    ly = np.frombuffer(raw[8 + 4 * n:8 + 8 * n], np.int32)
+    # the same flow through the Python mirrors of the C ABI
+    p = gp.default_params()
+    p.target_density, p.num_bin_x, p.num_bin_y = 0.95, 64, 64
+    p.scaling_factor, p.reference_hpwl_ratio, p.precision = 100.0, 0.01, 2
+    glx, gly, gst = gp.place_design(d, p)
+    olx, oly, ofl, ost = lg.run_design(d.copy_positions(glx, gly))
+    np.testing.assert_array_equal(lx, olx)
+    np.testing.assert_array_equal(ly, oly)
+    assert f"iterations={gst['iterations']}" in r.stdout and ost["converged"] == 1 and strictly_legal(ost["vio"])
