@@ -66,6 +66,7 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int kTile = 32;   // bins per tile side
 constexpr int kResolveBlocks = 4096;
+constexpr int kChunk = 2048;  // records per accumulation workgroup
 constexpr int kMaxHalo = 8;
 constexpr int kBK = 64;     // GEMM k-step
 
@@ -324,12 +325,32 @@ __global__ void k_dens_reset(int* cnt, int n) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) cnt[i] = 0;
 }
 
+// One rank per active lane from a counter per bucket, the lanes of a wave
+// that share a bucket taking one atomic together (a clustered placement puts
+// tens of thousands of cells in one tile, whose counter would otherwise take
+// one atomic per cell).  Called by every lane of the wave.
+template <typename Ctr>
+__device__ __forceinline__ int wave_rank(Ctr* ctr, int b, bool active) {
+  const int lane = threadIdx.x & 63;
+  unsigned long long pending = __ballot(active);
+  int mine = 0;
+  while (pending) {
+    const int leader = __ffsll((unsigned long long)pending) - 1;
+    const int lb = __shfl(b, leader);
+    const unsigned long long grp = __ballot(active && b == lb && ((pending >> lane) & 1ull));
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&ctr[lb], (int)__popcll(grp));
+    base = __shfl(base, leader);
+    if ((grp >> lane) & 1ull) mine = base + (int)__popcll(grp & ((1ull << lane) - 1ull));
+    pending &= ~grp;
+  }
+  return mine;
+}
+
 // bucket of each cell: its tile, T for cells wider than the halo, -1 for
 // cells outside the die; rank within the bucket.  With HIST the ranks are
 // taken per workgroup in an LDS histogram of the T + 1 buckets and the
-// workgroup adds each bucket's total to the global count once: clustered
-// placements put tens of thousands of cells in one tile, whose counter would
-// otherwise take one atomic per cell.
+// workgroup adds each bucket's total to the global count once.
 template <bool HIST>
 __global__ __launch_bounds__(256) void k_dens_count(Geo G, const float* cx, const float* cy, const float4* cgeo,
                                                     int* cnt, int* bucket, int* rank) {
@@ -338,8 +359,6 @@ __global__ __launch_bounds__(256) void k_dens_count(Geo G, const float* cx, cons
   if (HIST) {
     for (int e = threadIdx.x; e <= G.T; e += blockDim.x) hist[e] = 0;
     __syncthreads();
-  } else if (i >= G.N) {
-    return;
   }
   int b = -1;
   if (i < G.N) {
@@ -353,10 +372,11 @@ __global__ __launch_bounds__(256) void k_dens_count(Geo G, const float* cx, cons
     bucket[i] = b;
   }
   if (!HIST) {
-    if (b >= 0) rank[i] = atomicAdd(&cnt[b], 1);
+    const int r = wave_rank(cnt, b, b >= 0);
+    if (b >= 0) rank[i] = r;
     return;
   }
-  const int rl = b >= 0 ? atomicAdd(&hist[b], 1) : 0;
+  const int rl = wave_rank(hist, b, b >= 0);
   __syncthreads();
   // the first cell of a bucket turns the bucket's total into its base
   if (b >= 0 && rl == 0) hist[b] = atomicAdd(&cnt[b], hist[b]);
@@ -364,14 +384,14 @@ __global__ __launch_bounds__(256) void k_dens_count(Geo G, const float* cx, cons
   if (b >= 0) rank[i] = hist[b] + rl;
 }
 
-// exclusive scan of cnt[0..n) into off[0..n] (one workgroup of 1024)
-__global__ __launch_bounds__(1024) void k_dens_scan(const int* cnt, int* off, int n) {
-  __shared__ int part[1024];
+// exclusive scan of one 1024-thread workgroup over n values given by f
+template <typename F>
+__device__ void block_scan(int n, F f, int* out, int* part) {
   const int t = threadIdx.x;
   const int per = (n + 1023) / 1024;
   const int b = t * per, e = min(b + per, n);
   int s = 0;
-  for (int i = b; i < e; i++) s += cnt[i];
+  for (int i = b; i < e; i++) s += f(i);
   part[t] = s;
   __syncthreads();
   for (int o = 1; o < 1024; o <<= 1) {
@@ -382,10 +402,32 @@ __global__ __launch_bounds__(1024) void k_dens_scan(const int* cnt, int* off, in
   }
   int run = part[t] - s;
   for (int i = b; i < e; i++) {
-    off[i] = run;
-    run += cnt[i];
+    out[i] = run;
+    run += f(i);
   }
-  if (t == 1023) off[n] = part[1023];
+  if (t == 1023) out[n] = part[1023];
+  __syncthreads();
+}
+
+// chunks of a tile's records (one workgroup each): at least one per tile, so
+// that every tile's slab is written
+__device__ __forceinline__ int tile_chunks(int c) { return c > kChunk ? (c + kChunk - 1) / kChunk : 1; }
+
+// off[0..T+1] = exclusive scan of the bucket counts; cstart[0..T] = first
+// chunk of each tile; mstart[0..T] = first partial slot of each tile that
+// has more than one chunk
+__global__ __launch_bounds__(1024) void k_dens_scan(const int* cnt, int* off, int T, int* cstart, int* mstart) {
+  __shared__ int part[1024];
+  block_scan(T + 1, [&](int i) { return cnt[i]; }, off, part);
+  block_scan(T, [&](int i) { return tile_chunks(cnt[i]); }, cstart, part);
+  block_scan(T, [&](int i) { int k = tile_chunks(cnt[i]); return k > 1 ? k : 0; }, mstart, part);
+}
+
+// chunk -> tile map
+__global__ __launch_bounds__(256) void k_dens_chunkmap(int T, const int* cstart, int* cmap) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= T) return;
+  for (int c = cstart[t]; c < cstart[t + 1]; c++) cmap[c] = t;
 }
 
 // records in bucket order: box, scatter scale and filler flag, and for the
@@ -411,18 +453,25 @@ __global__ void k_dens_fill(Geo G, const float* cx, const float* cy, const float
   rec_ig[p] = make_int2(i, __float_as_int(c.w));
 }
 
-// one workgroup per tile: LDS accumulation of the tile plus halo in fixed
-// point (64-bit integer adds, whose order does not change the sum), slab
-// write in FP32
-__global__ __launch_bounds__(256) void k_dens_accum(Geo G, const int* off, const float4* rec_box,
-                                                    const float2* rec_s, float* slab) {
+// one workgroup per chunk of at most kChunk records of one tile: LDS
+// accumulation of the tile plus halo in fixed point (64-bit integer adds,
+// whose order does not change the sum).  A tile of one chunk writes its slab
+// in FP32; the chunks of a larger tile write their integer partials, which
+// k_dens_merge adds (in any grouping of the tile's cells into chunks the
+// integer total is the same).
+__global__ __launch_bounds__(256) void k_dens_accum(Geo G, const int* off, const int* cstart, const int* mstart,
+                                                    const int* cmap, const float4* rec_box, const float2* rec_s,
+                                                    float* slab, unsigned long long* partial) {
   extern __shared__ unsigned long long sm[];  // [2][SW*SW]
-  const int t = blockIdx.x;
+  const int chunk = blockIdx.x;
+  if (chunk >= cstart[G.T]) return;
+  const int t = cmap[chunk];
   const int SW = G.SW, S2 = SW * SW;
   for (int e = threadIdx.x; e < 2 * S2; e += blockDim.x) sm[e] = 0ull;
   __syncthreads();
   const int tx0 = (t % G.tiles_x) * kTile, ty0 = (t / G.tiles_x) * kTile;
-  const int b = off[t], e = off[t + 1];
+  const int k = chunk - cstart[t];
+  const int b = off[t] + k * kChunk, e = min(off[t + 1], b + kChunk);
   for (int p = b + threadIdx.x; p < e; p += blockDim.x) {
     float4 c = rec_box[p];
     float2 s = rec_s[p];
@@ -439,8 +488,30 @@ __global__ __launch_bounds__(256) void k_dens_accum(Geo G, const int* off, const
     }
   }
   __syncthreads();
+  if (cstart[t + 1] - cstart[t] == 1) {
+    float* out = slab + (size_t)t * 2 * S2;
+    for (int q = threadIdx.x; q < 2 * S2; q += blockDim.x) out[q] = (float)((double)(long long)sm[q] * G.fx_inv);
+  } else {
+    unsigned long long* out = partial + (size_t)(mstart[t] + k) * 2 * S2;
+    for (int q = threadIdx.x; q < 2 * S2; q += blockDim.x) out[q] = sm[q];
+  }
+}
+
+// the tiles of more than one chunk: integer sum of the chunks' partials, slab
+// write in FP32
+__global__ __launch_bounds__(256) void k_dens_merge(Geo G, const int* cstart, const int* mstart,
+                                                    const unsigned long long* partial, float* slab) {
+  const int t = blockIdx.x;
+  const int nch = cstart[t + 1] - cstart[t];
+  if (nch < 2) return;
+  const int S2 = G.SW * G.SW;
+  const unsigned long long* src = partial + (size_t)mstart[t] * 2 * S2;
   float* out = slab + (size_t)t * 2 * S2;
-  for (int q = threadIdx.x; q < 2 * S2; q += blockDim.x) out[q] = (float)((double)(long long)sm[q] * G.fx_inv);
+  for (int q = threadIdx.x; q < 2 * S2; q += blockDim.x) {
+    unsigned long long v = 0ull;
+    for (int c = 0; c < nch; c++) v += src[(size_t)c * 2 * S2 + q];
+    out[q] = (float)((double)(long long)v * G.fx_inv);
+  }
 }
 
 // cells wider than the halo: global 64-bit fixed-point atomics into big[2][B]
@@ -601,6 +672,9 @@ struct DensHandle {
   int* cfill = nullptr;
   float *fixed = nullptr, *scaled = nullptr, *lambda = nullptr;
   int *cnt = nullptr, *off = nullptr, *bucket = nullptr, *rank = nullptr;
+  int *cstart = nullptr, *mstart = nullptr, *cmap = nullptr;  // chunks of the tiles' records
+  unsigned long long* partial = nullptr;                      // [max multi-chunk chunks][2][SW*SW]
+  int max_chunks = 0;
   float4* rec_box = nullptr;
   float2* rec_s = nullptr;
   int2* rec_ig = nullptr;
@@ -898,6 +972,14 @@ int dens_create(const DensGeom* g, DensHandle** out) {
   if ((st = dupload(h->bufs, &h->lambda, lam, B))) return bail(st);
   if ((st = dalloc(h->bufs, &h->cnt, G.T + 1))) return bail(st);
   if ((st = dalloc(h->bufs, &h->off, G.T + 2))) return bail(st);
+  // chunks: at most one per tile plus N / kChunk; the partials of tiles of
+  // more than one chunk: at most 2 N / kChunk chunks
+  h->max_chunks = G.T + (N + kChunk - 1) / kChunk;
+  const size_t max_multi = 2 * (size_t)((N + kChunk - 1) / kChunk);
+  if ((st = dalloc(h->bufs, &h->cstart, G.T + 1))) return bail(st);
+  if ((st = dalloc(h->bufs, &h->mstart, G.T + 1))) return bail(st);
+  if ((st = dalloc(h->bufs, &h->cmap, h->max_chunks))) return bail(st);
+  if ((st = dalloc(h->bufs, &h->partial, std::max<size_t>(max_multi, 1) * 2 * (size_t)G.SW * G.SW))) return bail(st);
   if ((st = dalloc(h->bufs, &h->bucket, N))) return bail(st);
   if ((st = dalloc(h->bufs, &h->rank, N))) return bail(st);
   if ((st = dalloc(h->bufs, &h->rec_box, N))) return bail(st);
@@ -945,13 +1027,16 @@ int dens_compute_grad(DensHandle* h, const float* cx, const float* cy, float* gx
       hipLaunchKernelGGL(k_dens_count<false>, dim3(nb), dim3(256), 0, s, G, cx, cy, h->cgeo, h->cnt, h->bucket,
                          h->rank);
   }
-  hipLaunchKernelGGL(k_dens_scan, dim3(1), dim3(1024), 0, s, h->cnt, h->off, G.T + 1);
+  hipLaunchKernelGGL(k_dens_scan, dim3(1), dim3(1024), 0, s, h->cnt, h->off, G.T, h->cstart, h->mstart);
+  hipLaunchKernelGGL(k_dens_chunkmap, dim3((G.T + 255) / 256), dim3(256), 0, s, G.T, h->cstart, h->cmap);
   if (G.N > 0) {
     hipLaunchKernelGGL(k_dens_fill, dim3(nb), dim3(256), 0, s, G, cx, cy, h->cgeo, h->cfill, h->bucket, h->rank,
                        h->off, h->rec_box, h->rec_s, h->rec_ig, gx, gy, h->precond);
   }
   size_t lds = (size_t)2 * G.SW * G.SW * sizeof(unsigned long long);
-  hipLaunchKernelGGL(k_dens_accum, dim3(G.T), dim3(256), lds, s, G, h->off, h->rec_box, h->rec_s, h->slab);
+  hipLaunchKernelGGL(k_dens_accum, dim3(h->max_chunks), dim3(256), lds, s, G, h->off, h->cstart, h->mstart, h->cmap,
+                     h->rec_box, h->rec_s, h->slab, h->partial);
+  hipLaunchKernelGGL(k_dens_merge, dim3(G.T), dim3(256), 0, s, G, h->cstart, h->mstart, h->partial, h->slab);
   hipLaunchKernelGGL(k_dens_large, dim3(256), dim3(256), 0, s, G, h->off, h->rec_box, h->rec_s, h->big);
   const int nres = std::min((h->B + 255) / 256, kResolveBlocks);
   hipLaunchKernelGGL(k_dens_resolve, dim3(nres), dim3(256), 0, s, G, h->off, h->slab, h->big, h->fixed, h->scaled,

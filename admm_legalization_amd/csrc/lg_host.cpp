@@ -44,12 +44,33 @@ struct Rgn {
   RegionKind kind;
 };
 
+// Allocator whose resize() leaves new elements uninitialised, so that a
+// pixel map of tens of millions of entries is filled once, in parallel.
+template <typename T>
+struct NoInit : std::allocator<T> {
+  template <typename U>
+  struct rebind {
+    using other = NoInit<U>;
+  };
+  NoInit() = default;
+  template <typename U>
+  NoInit(const NoInit<U>&) noexcept {}
+  template <typename U>
+  void construct(U* p) noexcept {
+    ::new ((void*)p) U;
+  }
+  template <typename U, typename... A>
+  void construct(U* p, A&&... a) {
+    ::new ((void*)p) U(std::forward<A>(a)...);
+  }
+};
+using PixelMap = std::vector<int32_t, NoInit<int32_t>>;
+
 // Pixel grid over the core, flat and x-major (Grid.h:51-105).
 class PixelGrid {
  public:
   int lx = 0, ly = 0, sw = 1, rh = 1, xs = 0, ys = 0;
-  std::vector<uint8_t> valid;
-  std::vector<int32_t> group, region;
+  PixelMap valid, group, region;
   std::vector<int32_t> hov, vov;  // fence overlap bookkeeping (only if fences exist)
   std::vector<int32_t> vdd;
 
@@ -215,6 +236,11 @@ int import_design(Session& S, bool pixels = true) {
     if (st) return st;
   }
   const int site_w = d->row_site_width[d->num_rows - 1];
+  S.cells.reserve((size_t)d->num_insts);
+  // height_distribution_ counted in a short list of (height, count) pairs
+  // (a handful of distinct heights), moved into the map afterwards
+  std::vector<std::pair<int, int>> hcount;
+  int last_h = -1, last_k = -1;
   for (int i = 0; i < d->num_insts; i++) {
     if (d->inst_dx[i] == 0 || d->inst_dy[i] < row_h) continue;
     std::array<int, 4> r = {d->inst_lx[i], d->inst_ly[i], d->inst_lx[i] + d->inst_dx[i],
@@ -235,9 +261,17 @@ int import_design(Session& S, bool pixels = true) {
         }
       }
       S.cells.push_back(c);
-      hdist[c.h]++;
+      if (c.h != last_h) {
+        last_k = -1;
+        for (int k = 0; k < (int)hcount.size(); k++)
+          if (hcount[k].first == c.h) { last_k = k; break; }
+        if (last_k < 0) { hcount.push_back({c.h, 0}); last_k = (int)hcount.size() - 1; }
+        last_h = c.h;
+      }
+      hcount[last_k].second++;
     }
   }
+  for (auto& [hh, k] : hcount) hdist[hh] += k;
   PixelGrid& g = S.grid;
   g.lx = d->core_lx; g.ly = d->core_ly;
   int min_rh = INT_MAX;
@@ -249,13 +283,19 @@ int import_design(Session& S, bool pixels = true) {
   g.ys = (d->core_uy - d->core_ly) / g.rh;
   if (g.xs <= 0 || g.ys <= 0) { g_host_err = "empty core"; return LG_ERR_ARG; }
   const size_t P = (size_t)g.xs * g.ys;
-  if (pixels) {
-    g.valid.assign(P, 0);
-    g.group.assign(P, kGroupDefault);
-    g.region.assign(P, -1);
+  const int nt = std::max(1, S.p.num_cpu_threads);
+  if (pixels) {  // sized, then filled by all threads (the page faults too)
+    g.valid.resize(P);
+    g.group.resize(P);
+    g.region.resize(P);
+#pragma omp parallel for num_threads(nt) schedule(static)
+    for (long long q = 0; q < (long long)P; q++) {
+      g.valid[q] = 0;
+      g.group[q] = kGroupDefault;
+      g.region[q] = -1;
+    }
     g.vdd.assign(g.ys, 0);
   }
-  const int nt = std::max(1, S.p.num_cpu_threads);
   if (pixels) {  // markDbRow (Grid.cpp:316-342)
     // the rows' x-spans per grid row, then the x-major map column by column
     std::vector<std::vector<std::pair<int, int>>> spans(g.ys);
@@ -641,7 +681,6 @@ int lg_run_design(const LgDesign* d, const LgParams* p, LgResult* r) {
   const double t2 = now_s();
   const PixelGrid& g = S.grid;
   const int N = (int)S.cells.size();
-  std::vector<int32_t> valid(g.valid.begin(), g.valid.end());
   std::vector<int32_t> rx0, ry0, rx1, ry1;
   for (auto& rg : S.regions) {
     auto [a, b, c, e] = g.shrunk(rg.lx, rg.ly, rg.ux, rg.uy);
@@ -653,7 +692,7 @@ int lg_run_design(const LgDesign* d, const LgParams* p, LgResult* r) {
     ox[i] = c.ilx; oy[i] = c.ily; lx[i] = c.lx; ly[i] = c.ly;
     w[i] = c.w / g.sw; h[i] = c.h / g.rh; grp[i] = c.group; gnd[i] = c.gnd;
   }
-  LgGridDesc gd{g.xs, g.ys, g.sw, g.rh, g.lx, g.ly, valid.data(), g.group.data(), g.region.data(),
+  LgGridDesc gd{g.xs, g.ys, g.sw, g.rh, g.lx, g.ly, g.valid.data(), g.group.data(), g.region.data(),
                 g.vdd.data(), (int)S.regions.size(), rx0.data(), ry0.data(), rx1.data(), ry1.data()};
   LgCellsSoA cs{N, ox.data(), oy.data(), lx.data(), ly.data(), w.data(), h.data(), grp.data(), gnd.data()};
   LgHandle* hd = nullptr;

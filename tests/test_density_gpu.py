@@ -211,3 +211,21 @@ def test_poisson_tile_shapes_agree(gpu, M, N, prec):
     with pytest.raises(RuntimeError):
         ps.setTile(5)
     ps.close()
+
+
+def test_density_dense_cluster_spans_many_chunks(gpu):
+    """20000 cells in one tight cluster (a global placer's first iterations):
+    tiles of more than 2048 cells are accumulated by several workgroups whose
+    integer partials are merged; equal to the oracle, and bit-identical on a
+    second evaluation."""
+    rng = np.random.default_rng(8)
+    pl = synth.density_placement(20000, seed=8, clusters=0)
+    side = pl["die"][2]
+    pl["cx"] = np.clip(rng.normal(side / 2, side * 0.01, 20000), pl["dx"] / 2, side - pl["dx"] / 2).astype(np.float32)
+    pl["cy"] = np.clip(rng.normal(side / 2, side * 0.01, 20000), pl["dy"] / 2, side - pl["dy"] / 2).astype(np.float32)
+    out = _run_density(pl, num_bins=(256, 256))
+    o = _oracle(pl, out["geom"])
+    _compare(out, o)
+    out2 = _run_density(pl, num_bins=(256, 256))
+    np.testing.assert_array_equal(out["gx"], out2["gx"])
+    assert out["ovf"] == out2["ovf"]
