@@ -28,6 +28,7 @@ CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off", "-Wa
 # (source, compiler) pairs; header-only .inc.hip files are pulled in by includes
 UNITS = [
     ("lg_device.hip", "hip"),
+    ("lg_grid.hip", "hip"),
     ("lg_host.cpp", "cxx"),
     ("dens_device.hip", "hip"),
     ("dens_host.cpp", "cxx"),

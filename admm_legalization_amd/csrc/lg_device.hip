@@ -23,6 +23,7 @@
 #include "../../include/skyline_lg.h"
 #include "lg_common.h"
 #include "lg_internal.h"
+#include "lg_grid.h"
 
 // The kernels and this orchestration form one translation unit, so the
 // templated partition kernel and the __constant__ direction tables are local.
@@ -170,6 +171,7 @@ struct HostPrep {
   int ncol = 0, nrow = 0, K = 0, N = 0;
   size_t P = 0;
   std::vector<int16_t> pcode;
+  void* dev_pcode = nullptr;  // pixel codes built on the device (lg_create_device_grid)
   std::vector<int32_t> region_rm, vdd, rx0, ry0, rx1, ry1;
   std::vector<int32_t> lxg, lyg, w, h, code, gnd, ox, oy, rprio, khn, colmap, rowmap;
   // dual segments: maximal runs of columns (rows) inside one partition
@@ -372,7 +374,8 @@ int read_ctl(LgHandle* h, hipStream_t s) {
 
 
 // Validation and host-side preparation of lg_create (no device calls).
-int prepare_host(const LgGridDesc* g, const LgCellsSoA* c, const LgParams* p, HostPrep& hp) {
+int prepare_host(const LgGridDesc* g, const LgCellsSoA* c, const LgParams* p, HostPrep& hp,
+                 bool pixels_on_device = false) {
   if (!g || !c || !p) return fail(LG_ERR_ARG, "null argument");
   if (g->x_size <= 0 || g->y_size <= 0 || c->num_cells < 0) return fail(LG_ERR_ARG, "bad sizes");
   if (g->site_width <= 0 || g->row_height <= 0) return fail(LG_ERR_ARG, "bad site/row size");
@@ -397,7 +400,7 @@ int prepare_host(const LgGridDesc* g, const LgCellsSoA* c, const LgParams* p, Ho
   // pixel code: valid ? group + 2 : 0.  The ABI's pixel arrays are x-major
   // (x * y_size + y, the reference's Grid layout); the device keeps them
   // row-major (y * x_size + x).
-  hp.pcode.assign(P, 0);
+  if (!pixels_on_device) hp.pcode.assign(P, 0);
   // pixel regions are read by the adaptive search only
   const bool regions = p->use_adaptive_search && g->pixel_region;
   if (p->use_adaptive_search) hp.region_rm.assign(P, -1);
@@ -424,7 +427,9 @@ int prepare_host(const LgGridDesc* g, const LgCellsSoA* c, const LgParams* p, Ho
             }
           }
   };
-  if (nth == 1 || P < (1u << 20)) {
+  if (pixels_on_device) {
+    if (p->use_adaptive_search) return fail(LG_ERR_ARG, "the adaptive search reads host pixel regions");
+  } else if (nth == 1 || P < (1u << 20)) {
     for (int t = 0; t < nth; t++) work(t);
   } else {
     std::vector<std::thread> pool;
@@ -517,25 +522,31 @@ size_t handle_bytes(const HostPrep& hp) {
 // buffer comes from it; `ext_ctl` places the iteration control in a server's
 // Ctl array; `stream` (if given) is the handle's stream, not owned.
 int create_from_prep(const HostPrep& hp, DevPool* pool, Ctl* ext_ctl, hipStream_t stream, LgHandle** out) {
+  // until the handle exists, a failure frees the device-built pixel codes
+  auto early = [&](int code) {
+    if (hp.dev_pcode) (void)hipFree(hp.dev_pcode);
+    return code;
+  };
   int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(LG_ERR_DEVICE, "no HIP device");
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return early(fail(LG_ERR_DEVICE, "no HIP device"));
   int st = upload_directions();
-  if (st) return st;
+  if (st) return early(st);
   const LgParams* p = &hp.p;
   {
     // a partition's pixels are staged in LDS (the large size class also
     // holds its sort buffers there)
     const size_t need = partition_lds_bytes(p->x_hint, p->y_hint, p->sorting_algorithm, kMaxNpadLds, 1);
     if (need > kMaxLds)
-      return fail(LG_ERR_ARG, "partition of x_hint x y_hint pixels needs " + std::to_string(need) +
-                                  " bytes of LDS, more than the " + std::to_string(kMaxLds) + " of a workgroup");
+      return early(fail(LG_ERR_ARG, "partition of x_hint x y_hint pixels needs " + std::to_string(need) +
+                                  " bytes of LDS, more than the " + std::to_string(kMaxLds) + " of a workgroup"));
   }
   if (pool && !(pool->budget >= handle_bytes(hp)))
-    return fail(LG_ERR_MEMORY, "request larger than the server's device-memory budget");
+    return early(fail(LG_ERR_MEMORY, "request larger than the server's device-memory budget"));
   char* blk = nullptr;
   if (pool && !(blk = (char*)pool->alloc(handle_bytes(hp))))
-    return fail(LG_ERR_MEMORY, "request server: device-memory budget exhausted");
+    return early(fail(LG_ERR_MEMORY, "request server: device-memory budget exhausted"));
   LgHandle* h = new LgHandle();
+  if (hp.dev_pcode) h->bufs.push_back(hp.dev_pcode);  // freed by lg_destroy from here on
   h->p = *p;
   h->pool = pool;
   h->blk = blk;
@@ -570,7 +581,11 @@ int create_from_prep(const HostPrep& hp, DevPool* pool, Ctl* ext_ctl, hipStream_
   int16_t* d_pcode;
   int32_t *d_region = nullptr, *d_vdd, *d_rx0, *d_ry0, *d_rx1, *d_ry1;
   const size_t nr = hp.rx0.size();
-  if ((st = upload(h, &d_pcode, hp.pcode.data(), P))) return bail(st);
+  if (hp.dev_pcode) {  // built on the device (owned by the handle, above)
+    d_pcode = (int16_t*)hp.dev_pcode;
+  } else if ((st = upload(h, &d_pcode, hp.pcode.data(), P))) {
+    return bail(st);
+  }
   if (!hp.region_rm.empty() && (st = upload(h, &d_region, hp.region_rm.data(), P))) return bail(st);
   if ((st = upload(h, &d_vdd, hp.vdd.data(), (size_t)hp.y_size))) return bail(st);
   if ((st = upload(h, &d_rx0, hp.rx0.data(), nr))) return bail(st);
@@ -860,6 +875,28 @@ int lg_create(const LgGridDesc* g, const LgCellsSoA* c, const LgParams* p, LgHan
   }
   return st;
 }
+
+}  // extern "C"
+
+int lg_create_device_grid(const LgGridDesc* g, const LgCellsSoA* c, const LgParams* p, void* d_pcode,
+                          LgHandle** out) {
+  if (!out || !d_pcode) {
+    (void)hipFree(d_pcode);
+    return fail(LG_ERR_ARG, "null argument");
+  }
+  HostPrep hp;
+  int st = prepare_host(g, c, p, hp, true);
+  if (st) {
+    (void)hipFree(d_pcode);
+    return st;
+  }
+  hp.dev_pcode = d_pcode;
+  // create_from_prep owns the buffer from its first allocation on; before it
+  // a failure frees it here
+  return create_from_prep(hp, nullptr, nullptr, nullptr, out);
+}
+
+extern "C" {
 
 int lg_solve(LgHandle* h, void* stream) {
   if (!h) return fail(LG_ERR_ARG, "null handle");

@@ -23,6 +23,7 @@
 #include <omp.h>
 
 #include "../../include/skyline_lg.h"
+#include "lg_grid.h"
 #include "../../include/skyline/Legalizer.hpp"
 
 namespace {
@@ -111,6 +112,8 @@ struct Session {
   std::vector<std::array<int, 2>> dirs;  // default candidate moves
   int preplace_fail = 0;
   std::map<int, int> hdist;  // cell height (dbu) -> movable cells (height_distribution_)
+  std::vector<std::vector<std::pair<int, int>>> spans;  // per grid row: the rows' x-spans (markDbRow)
+  bool any_fence = false, fences_disjoint = true;
   float max_util = 0.f, max_disp = 0.f;  // constraint file: percent, dbu
 };
 
@@ -214,171 +217,31 @@ void add_region(Session& S, const Rgn& r) {
   S.regions.push_back(r);
 }
 
-// importCells (DatabaseIO.cpp:79-169) + Grid ctor (Grid.cpp:24-67) +
-// region import (DatabaseIO.cpp:210-379) + region marking (Grid.cpp:184-314,
-// DatabaseIO.cpp:661-681) + adjustYHint (Legalizer.cpp:230-247)
-// With pixels == false only the cells, regions and sizes are built (the
-// shape query of lg_import_and_preplace).
-int import_design(Session& S, bool pixels = true) {
-  const LgDesign* d = S.d;
-  if (!d || d->num_rows <= 0) { g_host_err = "design has no rows"; return LG_ERR_ARG; }
-  S.core_lx = d->core_lx; S.core_ly = d->core_ly; S.core_ux = d->core_ux; S.core_uy = d->core_uy;
-  const int row_h = d->row_dy[d->num_rows - 1];
-  std::vector<std::array<int, 4>> blk_macro, blk_std;
-  std::map<int, int>& hdist = S.hdist;
-  hdist.clear();
-  // modified-ISPD2015 cell sizes (setSizeByFile)
-  std::unordered_map<std::string, std::pair<int, int>> sizes;
-  const bool resize = d->size_file && d->size_file[0];
-  if (resize) {
-    if (!d->inst_name) { g_host_err = "size_file needs inst_name"; return LG_ERR_ARG; }
-    int st = read_size_file(d->size_file, sizes);
-    if (st) return st;
-  }
-  const int site_w = d->row_site_width[d->num_rows - 1];
-  S.cells.reserve((size_t)d->num_insts);
-  // height_distribution_ counted in a short list of (height, count) pairs
-  // (a handful of distinct heights), moved into the map afterwards
-  std::vector<std::pair<int, int>> hcount;
-  int last_h = -1, last_k = -1;
-  for (int i = 0; i < d->num_insts; i++) {
-    if (d->inst_dx[i] == 0 || d->inst_dy[i] < row_h) continue;
-    std::array<int, 4> r = {d->inst_lx[i], d->inst_ly[i], d->inst_lx[i] + d->inst_dx[i],
-                            d->inst_ly[i] + d->inst_dy[i]};
-    if (d->inst_macro[i]) {
-      if (!d->inst_fixed[i]) { g_host_err = "movable macros are not supported"; return LG_ERR_DESIGN; }
-      blk_macro.push_back(r);
-    } else if (d->inst_fixed[i]) {
-      blk_std.push_back(r);
-    } else {
-      MovCell c{r[0], r[1], d->inst_dx[i], d->inst_dy[i], r[0], r[1],
-                d->inst_group ? d->inst_group[i] : -1, d->inst_ground_at_bottom[i] ? 1 : 0, i};
-      if (resize && d->inst_name[i]) {
-        auto it = sizes.find(d->inst_name[i]);
-        if (it != sizes.end()) {
-          c.w = it->second.first * site_w;
-          c.h = it->second.second * row_h;
-        }
-      }
-      S.cells.push_back(c);
-      if (c.h != last_h) {
-        last_k = -1;
-        for (int k = 0; k < (int)hcount.size(); k++)
-          if (hcount[k].first == c.h) { last_k = k; break; }
-        if (last_k < 0) { hcount.push_back({c.h, 0}); last_k = (int)hcount.size() - 1; }
-        last_h = c.h;
-      }
-      hcount[last_k].second++;
-    }
-  }
-  for (auto& [hh, k] : hcount) hdist[hh] += k;
+// The pixel maps (x-major): markDbRow's valid pixels from the row spans
+// (Grid.cpp:316-342), then the regions in order (markBlockageRegion,
+// markGroupRegion, markNonGroupRegion: Grid.cpp:184-314) and the fence edge
+// tests (handleRegionSegments, DatabaseIO.cpp:661-681).
+void paint_pixels(Session& S) {
   PixelGrid& g = S.grid;
-  g.lx = d->core_lx; g.ly = d->core_ly;
-  int min_rh = INT_MAX;
-  for (int r = 0; r < d->num_rows; r++) min_rh = std::min(min_rh, d->row_dy[r]);
-  g.rh = min_rh;
-  g.sw = d->row_site_width[d->num_rows - 1];
-  if (g.sw <= 0 || g.rh <= 0) { g_host_err = "bad site width / row height"; return LG_ERR_ARG; }
-  g.xs = (d->core_ux - d->core_lx) / g.sw;
-  g.ys = (d->core_uy - d->core_ly) / g.rh;
-  if (g.xs <= 0 || g.ys <= 0) { g_host_err = "empty core"; return LG_ERR_ARG; }
   const size_t P = (size_t)g.xs * g.ys;
   const int nt = std::max(1, S.p.num_cpu_threads);
-  if (pixels) {  // sized, then filled by all threads (the page faults too)
-    g.valid.resize(P);
-    g.group.resize(P);
-    g.region.resize(P);
+  const bool pixels = true;
+  const bool any_fence = S.any_fence, fences_disjoint = S.fences_disjoint;
+  g.valid.resize(P);
+  g.group.resize(P);
+  g.region.resize(P);
+  const auto& spans = S.spans;
 #pragma omp parallel for num_threads(nt) schedule(static)
-    for (long long q = 0; q < (long long)P; q++) {
-      g.valid[q] = 0;
+  for (int x = 0; x < g.xs; x++)
+    for (int y = 0; y < g.ys; y++) {
+      const size_t q = g.at(x, y);
+      int v = 0;
+      for (const auto& sp : spans[y])
+        if (sp.first <= x && x <= sp.second) { v = 1; break; }
+      g.valid[q] = v;
       g.group[q] = kGroupDefault;
       g.region[q] = -1;
     }
-    g.vdd.assign(g.ys, 0);
-  }
-  if (pixels) {  // markDbRow (Grid.cpp:316-342)
-    // the rows' x-spans per grid row, then the x-major map column by column
-    std::vector<std::vector<std::pair<int, int>>> spans(g.ys);
-    for (int r = 0; r < d->num_rows; r++) {
-      auto [x1, y1, x2, y2] = g.shrunk(d->row_lx[r], d->row_ly[r], d->row_lx[r] + d->row_dx[r],
-                                       d->row_ly[r] + d->row_dy[r]);
-      if (y1 > y2 || x1 > x2) continue;
-      for (int y = y1; y <= y2; y++) spans[y].push_back({x1, x2});
-      int o = d->row_orient[r];
-      g.vdd[y1] = (o == LG_ORIENT_N || o == LG_ORIENT_FN) ? 1 : 0;
-    }
-#pragma omp parallel for num_threads(nt) schedule(static)
-    for (int x = 0; x < g.xs; x++)
-      for (int y = 0; y < g.ys; y++)
-        for (const auto& sp : spans[y])
-          if (sp.first <= x && x <= sp.second) { g.valid[g.at(x, y)] = 1; break; }
-  }
-  auto clip = [&](const std::array<int, 4>& r) {
-    return std::array<int, 4>{std::max(S.core_lx, r[0]), std::max(S.core_ly, r[1]),
-                              std::min(S.core_ux, r[2]), std::min(S.core_uy, r[3])};
-  };
-  for (auto& r : blk_macro) { auto c = clip(r); add_region(S, {c[0], c[1], c[2], c[3], kGroupBlockage, kBlockageRegion}); }
-  for (auto& r : blk_std) { auto c = clip(r); add_region(S, {c[0], c[1], c[2], c[3], kGroupBlockage, kBlockageRegion}); }
-  for (int b = 0; b < d->num_region_boxes; b++) {
-    auto c = clip({d->box_lx[b], d->box_ly[b], d->box_ux[b], d->box_uy[b]});
-    add_region(S, {c[0], c[1], c[2], c[3], d->box_group[b], kFenceRegion});
-  }
-  // default regions: sweep the fence/blockage rectangles bottom-up
-  // (importNonGroupRegions, DatabaseIO.cpp:272-379)
-  {
-    std::vector<int> obst;
-    for (int i = 0; i < (int)S.regions.size(); i++) obst.push_back(i);
-    struct Ev { int y, id; bool enter; };
-    std::vector<Ev> ev;
-    for (int k = 0; k < (int)obst.size(); k++) {
-      ev.push_back({S.regions[obst[k]].ly, k, true});
-      ev.push_back({S.regions[obst[k]].uy, k, false});
-    }
-    std::sort(ev.begin(), ev.end(), [](const Ev& a, const Ev& b) {
-      return a.y != b.y ? a.y < b.y : (!a.enter && b.enter);
-    });
-    std::set<int> live;
-    std::unordered_map<int, std::vector<int>> at_lx;
-    auto emit = [&](int lx, int ly, int ux, int uy) {
-      if (ux - lx < g.sw || uy - ly < g.rh) return;
-      for (int id : at_lx[lx]) {
-        Rgn& r = S.regions[id];
-        if (r.ux - r.lx == ux - lx && r.uy == ly) { r.uy = uy; return; }
-      }
-      at_lx[lx].push_back((int)S.regions.size());
-      add_region(S, {lx, ly, ux, uy, kGroupDefault, kDefaultRegion});
-    };
-    int y0 = S.core_ly;
-    for (const Ev& e : ev) {
-      std::vector<std::pair<int, int>> spans;
-      for (int k : live) spans.push_back({S.regions[obst[k]].lx, S.regions[obst[k]].ux});
-      std::sort(spans.begin(), spans.end());
-      int x = S.core_lx;
-      for (auto& [a, b] : spans) { emit(x, y0, a, e.y); x = b; }
-      emit(x, y0, S.core_ux, e.y);
-      if (e.enter) live.insert(e.id); else live.erase(e.id);
-      y0 = e.y;
-    }
-    emit(S.core_lx, y0, S.core_ux, S.core_uy);
-  }
-  bool any_fence = false;
-  for (auto& r : S.regions) any_fence |= (r.kind == kFenceRegion);
-  // Fences whose pixel boxes do not overlap: a pixel inside one box gets
-  // exactly that fence's overlap counts, full (sw, rh) except on the box's
-  // edge columns / rows, so only the edges need the test of
-  // handleRegionSegments and no per-pixel counts are kept.  Otherwise the
-  // counts are accumulated over the whole grid.
-  bool fences_disjoint = true;
-  {
-    std::vector<std::array<int, 4>> fb;
-    for (auto& r : S.regions)
-      if (r.kind == kFenceRegion) fb.push_back(g.expanded(r.lx, r.ly, r.ux, r.uy));
-    if (fb.size() > 256) fences_disjoint = false;
-    for (size_t i = 0; i < fb.size() && fences_disjoint; i++)
-      for (size_t j = i + 1; j < fb.size() && fences_disjoint; j++)
-        if (fb[i][0] <= fb[j][2] && fb[j][0] <= fb[i][2] && fb[i][1] <= fb[j][3] && fb[j][1] <= fb[i][3])
-          fences_disjoint = false;
-  }
   if (pixels && any_fence && !fences_disjoint) { g.hov.assign(P, 0); g.vov.assign(P, 0); }
   for (int id = 0; id < (int)S.regions.size() && pixels; id++) {
     const Rgn& r = S.regions[id];
@@ -454,6 +317,157 @@ int import_design(Session& S, bool pixels = true) {
       for (int x = ex1; x <= ex2; x++) { test(x, ey1); test(x, ey2); }
     }
   }
+}
+
+// importCells (DatabaseIO.cpp:79-169) + Grid ctor (Grid.cpp:24-67) +
+// region import (DatabaseIO.cpp:210-379) + region marking (Grid.cpp:184-314,
+// DatabaseIO.cpp:661-681) + adjustYHint (Legalizer.cpp:230-247)
+// With pixels == false only the cells, regions and sizes are built (the
+// shape query of lg_import_and_preplace).
+int import_design(Session& S, bool pixels = true) {
+  const LgDesign* d = S.d;
+  if (!d || d->num_rows <= 0) { g_host_err = "design has no rows"; return LG_ERR_ARG; }
+  S.core_lx = d->core_lx; S.core_ly = d->core_ly; S.core_ux = d->core_ux; S.core_uy = d->core_uy;
+  const int row_h = d->row_dy[d->num_rows - 1];
+  std::vector<std::array<int, 4>> blk_macro, blk_std;
+  std::map<int, int>& hdist = S.hdist;
+  hdist.clear();
+  // modified-ISPD2015 cell sizes (setSizeByFile)
+  std::unordered_map<std::string, std::pair<int, int>> sizes;
+  const bool resize = d->size_file && d->size_file[0];
+  if (resize) {
+    if (!d->inst_name) { g_host_err = "size_file needs inst_name"; return LG_ERR_ARG; }
+    int st = read_size_file(d->size_file, sizes);
+    if (st) return st;
+  }
+  const int site_w = d->row_site_width[d->num_rows - 1];
+  S.cells.reserve((size_t)d->num_insts);
+  // height_distribution_ counted in a short list of (height, count) pairs
+  // (a handful of distinct heights), moved into the map afterwards
+  std::vector<std::pair<int, int>> hcount;
+  int last_h = -1, last_k = -1;
+  for (int i = 0; i < d->num_insts; i++) {
+    if (d->inst_dx[i] == 0 || d->inst_dy[i] < row_h) continue;
+    std::array<int, 4> r = {d->inst_lx[i], d->inst_ly[i], d->inst_lx[i] + d->inst_dx[i],
+                            d->inst_ly[i] + d->inst_dy[i]};
+    if (d->inst_macro[i]) {
+      if (!d->inst_fixed[i]) { g_host_err = "movable macros are not supported"; return LG_ERR_DESIGN; }
+      blk_macro.push_back(r);
+    } else if (d->inst_fixed[i]) {
+      blk_std.push_back(r);
+    } else {
+      MovCell c{r[0], r[1], d->inst_dx[i], d->inst_dy[i], r[0], r[1],
+                d->inst_group ? d->inst_group[i] : -1, d->inst_ground_at_bottom[i] ? 1 : 0, i};
+      if (resize && d->inst_name[i]) {
+        auto it = sizes.find(d->inst_name[i]);
+        if (it != sizes.end()) {
+          c.w = it->second.first * site_w;
+          c.h = it->second.second * row_h;
+        }
+      }
+      S.cells.push_back(c);
+      if (c.h != last_h) {
+        last_k = -1;
+        for (int k = 0; k < (int)hcount.size(); k++)
+          if (hcount[k].first == c.h) { last_k = k; break; }
+        if (last_k < 0) { hcount.push_back({c.h, 0}); last_k = (int)hcount.size() - 1; }
+        last_h = c.h;
+      }
+      hcount[last_k].second++;
+    }
+  }
+  for (auto& [hh, k] : hcount) hdist[hh] += k;
+  PixelGrid& g = S.grid;
+  g.lx = d->core_lx; g.ly = d->core_ly;
+  int min_rh = INT_MAX;
+  for (int r = 0; r < d->num_rows; r++) min_rh = std::min(min_rh, d->row_dy[r]);
+  g.rh = min_rh;
+  g.sw = d->row_site_width[d->num_rows - 1];
+  if (g.sw <= 0 || g.rh <= 0) { g_host_err = "bad site width / row height"; return LG_ERR_ARG; }
+  g.xs = (d->core_ux - d->core_lx) / g.sw;
+  g.ys = (d->core_uy - d->core_ly) / g.rh;
+  if (g.xs <= 0 || g.ys <= 0) { g_host_err = "empty core"; return LG_ERR_ARG; }
+  // markDbRow (Grid.cpp:316-342), rows first: the rows' x-spans per grid
+  // row and the rail of each row; the pixels follow in paint_pixels
+  S.spans.assign(g.ys, {});
+  g.vdd.assign(g.ys, 0);
+  for (int r = 0; r < d->num_rows; r++) {
+    auto [x1, y1, x2, y2] = g.shrunk(d->row_lx[r], d->row_ly[r], d->row_lx[r] + d->row_dx[r],
+                                     d->row_ly[r] + d->row_dy[r]);
+    if (y1 > y2 || x1 > x2) continue;
+    for (int y = y1; y <= y2; y++) S.spans[y].push_back({x1, x2});
+    int o = d->row_orient[r];
+    g.vdd[y1] = (o == LG_ORIENT_N || o == LG_ORIENT_FN) ? 1 : 0;
+  }
+  auto clip = [&](const std::array<int, 4>& r) {
+    return std::array<int, 4>{std::max(S.core_lx, r[0]), std::max(S.core_ly, r[1]),
+                              std::min(S.core_ux, r[2]), std::min(S.core_uy, r[3])};
+  };
+  for (auto& r : blk_macro) { auto c = clip(r); add_region(S, {c[0], c[1], c[2], c[3], kGroupBlockage, kBlockageRegion}); }
+  for (auto& r : blk_std) { auto c = clip(r); add_region(S, {c[0], c[1], c[2], c[3], kGroupBlockage, kBlockageRegion}); }
+  for (int b = 0; b < d->num_region_boxes; b++) {
+    auto c = clip({d->box_lx[b], d->box_ly[b], d->box_ux[b], d->box_uy[b]});
+    add_region(S, {c[0], c[1], c[2], c[3], d->box_group[b], kFenceRegion});
+  }
+  // default regions: sweep the fence/blockage rectangles bottom-up
+  // (importNonGroupRegions, DatabaseIO.cpp:272-379)
+  {
+    std::vector<int> obst;
+    for (int i = 0; i < (int)S.regions.size(); i++) obst.push_back(i);
+    struct Ev { int y, id; bool enter; };
+    std::vector<Ev> ev;
+    for (int k = 0; k < (int)obst.size(); k++) {
+      ev.push_back({S.regions[obst[k]].ly, k, true});
+      ev.push_back({S.regions[obst[k]].uy, k, false});
+    }
+    std::sort(ev.begin(), ev.end(), [](const Ev& a, const Ev& b) {
+      return a.y != b.y ? a.y < b.y : (!a.enter && b.enter);
+    });
+    std::set<int> live;
+    std::unordered_map<int, std::vector<int>> at_lx;
+    auto emit = [&](int lx, int ly, int ux, int uy) {
+      if (ux - lx < g.sw || uy - ly < g.rh) return;
+      for (int id : at_lx[lx]) {
+        Rgn& r = S.regions[id];
+        if (r.ux - r.lx == ux - lx && r.uy == ly) { r.uy = uy; return; }
+      }
+      at_lx[lx].push_back((int)S.regions.size());
+      add_region(S, {lx, ly, ux, uy, kGroupDefault, kDefaultRegion});
+    };
+    int y0 = S.core_ly;
+    for (const Ev& e : ev) {
+      std::vector<std::pair<int, int>> spans;
+      for (int k : live) spans.push_back({S.regions[obst[k]].lx, S.regions[obst[k]].ux});
+      std::sort(spans.begin(), spans.end());
+      int x = S.core_lx;
+      for (auto& [a, b] : spans) { emit(x, y0, a, e.y); x = b; }
+      emit(x, y0, S.core_ux, e.y);
+      if (e.enter) live.insert(e.id); else live.erase(e.id);
+      y0 = e.y;
+    }
+    emit(S.core_lx, y0, S.core_ux, S.core_uy);
+  }
+  bool& any_fence = S.any_fence;
+  any_fence = false;
+  for (auto& r : S.regions) any_fence |= (r.kind == kFenceRegion);
+  // Fences whose pixel boxes do not overlap: a pixel inside one box gets
+  // exactly that fence's overlap counts, full (sw, rh) except on the box's
+  // edge columns / rows, so only the edges need the test of
+  // handleRegionSegments and no per-pixel counts are kept.  Otherwise the
+  // counts are accumulated over the whole grid.
+  bool& fences_disjoint = S.fences_disjoint;
+  fences_disjoint = true;
+  {
+    std::vector<std::array<int, 4>> fb;
+    for (auto& r : S.regions)
+      if (r.kind == kFenceRegion) fb.push_back(g.expanded(r.lx, r.ly, r.ux, r.uy));
+    if (fb.size() > 256) fences_disjoint = false;
+    for (size_t i = 0; i < fb.size() && fences_disjoint; i++)
+      for (size_t j = i + 1; j < fb.size() && fences_disjoint; j++)
+        if (fb[i][0] <= fb[j][2] && fb[j][0] <= fb[i][2] && fb[i][1] <= fb[j][3] && fb[j][1] <= fb[i][3])
+          fences_disjoint = false;
+  }
+  if (pixels) paint_pixels(S);
   for (auto& [hh, n] : hdist) S.max_height = std::max(S.max_height, hh);
   int mh = S.max_height / g.rh;
   if (S.p.y_hint < 3 * mh) S.p.y_hint = 3 * mh;
@@ -582,6 +596,72 @@ void preplace(Session& S) {
   S.preplace_fail = fails;
 }
 
+// The device path of the import: pixel codes built and cells preplaced on
+// the GPU (lg_grid.hip) from the rows, the regions and the cells that
+// import_design(S, false) collected.  Available without the adaptive
+// search (which reads host region maps) and when the fence boxes do not
+// overlap (else the overlap counts need the whole grid): *used = false then.
+int device_grid(Session& S, void** d_pcode, bool* used) {
+  *used = false;
+  *d_pcode = nullptr;
+  const PixelGrid& g = S.grid;
+  int num_paint = 0;
+  for (const Rgn& r : S.regions) num_paint += r.kind != kDefaultRegion;
+  int max_group = -1;
+  for (const Rgn& r : S.regions) max_group = std::max(max_group, r.group);
+  if (S.p.use_adaptive_search || (S.any_fence && !S.fences_disjoint) || num_paint > 65535 ||
+      max_group + 2 > 0x7fff || g.xs > 65535 || g.ys > 65535)
+    return LG_OK;
+  LgGridBuild in;
+  in.xs = g.xs; in.ys = g.ys; in.sw = g.sw; in.rh = g.rh; in.lx = g.lx; in.ly = g.ly;
+  in.core_lx = S.core_lx; in.core_ly = S.core_ly; in.core_ux = S.core_ux; in.core_uy = S.core_uy;
+  std::vector<int32_t> ss(g.ys + 1, 0), sp, paint, fence, cell, reg, gs, gl, dirs;
+  for (int y = 0; y < g.ys; y++) {
+    ss[y + 1] = ss[y] + (int)S.spans[y].size();
+    for (auto& [a, b] : S.spans[y]) { sp.push_back(a); sp.push_back(b); }
+  }
+  for (const Rgn& r : S.regions) {
+    if (r.kind == kBlockageRegion) {  // markBlockageRegion: clamped, invalid
+      auto [x1, y1, x2, y2] = g.expanded(r.lx, r.ly, r.ux, r.uy);
+      paint.insert(paint.end(), {std::max(0, x1), std::max(0, y1), std::min(g.xs - 1, x2), std::min(g.ys - 1, y2), 0});
+    } else if (r.kind == kFenceRegion) {  // markGroupRegion: valid, the fence's group
+      auto [x1, y1, x2, y2] = g.expanded(r.lx, r.ly, r.ux, r.uy);
+      paint.insert(paint.end(), {x1, y1, x2, y2, r.group + 2});
+      auto [sx1, sy1, sx2, sy2] = g.shrunk(r.lx, r.ly, r.ux, r.uy);
+      fence.insert(fence.end(), {x1, y1, x2, y2, sx1, sy1, sx2, sy2, r.lx, r.ly, r.ux, r.uy});
+    }
+    reg.insert(reg.end(), {r.lx, r.ly, r.ux, r.uy});
+  }
+  for (const MovCell& c : S.cells) cell.insert(cell.end(), {c.lx, c.ly, c.w, c.h, c.ilx, c.ily, c.group, c.gnd});
+  int gmin = 0, gmax = -1;
+  for (auto& [k, v] : S.by_group) { gmin = std::min(gmin, k); gmax = std::max(gmax, k); }
+  for (const MovCell& c : S.cells) { gmin = std::min(gmin, c.group); gmax = std::max(gmax, c.group); }
+  gs.assign(gmax - gmin + 2, 0);
+  for (int k = gmin; k <= gmax; k++) {
+    auto it = S.by_group.find(k);
+    if (it != S.by_group.end()) gl.insert(gl.end(), it->second.begin(), it->second.end());
+    gs[k - gmin + 1] = (int)gl.size();
+  }
+  for (auto& d : S.dirs) { dirs.push_back(d[0]); dirs.push_back(d[1]); }
+  in.vdd = g.vdd.data();
+  in.span_start = ss.data(); in.span = sp.data();
+  in.num_paint = (int)paint.size() / 5; in.paint = paint.data();
+  in.num_fences = S.any_fence ? (int)fence.size() / 12 : 0; in.fence = fence.data();
+  in.num_cells = (int)S.cells.size(); in.cell = cell.data();
+  in.num_regions = (int)S.regions.size(); in.region = reg.data();
+  in.group_min = gmin; in.num_group_lists = gmax - gmin + 1; in.glist_start = gs.data(); in.glist = gl.data();
+  in.num_dirs = (int)S.dirs.size(); in.dirs = dirs.data();
+  std::vector<int32_t> lx(S.cells.size()), ly(S.cells.size());
+  int fails = 0;
+  std::string err;
+  const int st = lg_grid_preplace_device(in, d_pcode, lx.data(), ly.data(), &fails, nullptr, err);
+  if (st) { g_host_err = err; return st; }
+  for (size_t i = 0; i < S.cells.size(); i++) { S.cells[i].lx = lx[i]; S.cells[i].ly = ly[i]; }
+  S.preplace_fail = fails;
+  *used = true;
+  return LG_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -661,6 +741,40 @@ int lg_import_and_preplace(const LgDesign* d, const LgParams* p, int32_t* shape,
   return LG_OK;
 }
 
+int lg_import_and_preplace_device(const LgDesign* d, const LgParams* p, int32_t* shape, int32_t* valid,
+                                  int32_t* group, int32_t* cell_lx, int32_t* cell_ly) {
+  if (!d || !p || !shape) { g_host_err = "null argument"; return LG_ERR_ARG; }
+  Session S;
+  S.p = *p;
+  S.d = d;
+  int st = import_design(S, false);
+  if (st) return st;
+  const PixelGrid& g = S.grid;
+  shape[0] = g.xs; shape[1] = g.ys;
+  shape[2] = (int)S.cells.size();
+  shape[3] = (int)S.regions.size();
+  if (!valid) return LG_OK;
+  if (!group || !cell_lx || !cell_ly) { g_host_err = "null output array"; return LG_ERR_ARG; }
+  void* d_pcode = nullptr;
+  bool used = false;
+  if ((st = device_grid(S, &d_pcode, &used))) return st;
+  if (!used) { g_host_err = "no device path for this design (adaptive search or overlapping fences)"; return LG_ERR_STATE; }
+  const size_t P = (size_t)g.xs * g.ys;
+  std::vector<int16_t> pc(P);
+  std::string err;
+  st = lg_grid_pcode_to_host(d_pcode, P, pc.data(), err);
+  lg_grid_free(d_pcode);
+  if (st) { g_host_err = err; return st; }
+  for (int y = 0; y < g.ys; y++)
+    for (int x = 0; x < g.xs; x++) {
+      const int v = pc[(size_t)y * g.xs + x];
+      valid[g.at(x, y)] = v != 0;
+      group[g.at(x, y)] = v != 0 ? v - 2 : -1;
+    }
+  for (size_t i = 0; i < S.cells.size(); i++) { cell_lx[i] = S.cells[i].lx; cell_ly[i] = S.cells[i].ly; }
+  return LG_OK;
+}
+
 int lg_run_design(const LgDesign* d, const LgParams* p, LgResult* r) {
   if (!d || !p || !r) { g_host_err = "null argument"; return LG_ERR_ARG; }
   const double t0 = now_s();
@@ -669,7 +783,7 @@ int lg_run_design(const LgDesign* d, const LgParams* p, LgResult* r) {
   S.d = d;
   int st = check_netlist(d);
   if (st) return st;
-  st = import_design(S);
+  st = import_design(S, false);
   if (st) return st;
   const int row_h = d->row_dy[d->num_rows - 1];
   const bool constrained = d->constraint_file && d->constraint_file[0];
@@ -677,7 +791,14 @@ int lg_run_design(const LgDesign* d, const LgParams* p, LgResult* r) {
   // original_hpwl_ = computeHpwlFromDB() at import (DatabaseIO.cpp:68)
   const int64_t hpwl0 = design_hpwl(d, d->inst_lx, d->inst_ly, nullptr);
   const double t1 = now_s();
-  preplace(S);
+  // pixel grid and preplace on the device where possible, else on the host
+  void* d_pcode = nullptr;
+  bool on_device = false;
+  if ((st = device_grid(S, &d_pcode, &on_device))) return st;
+  if (!on_device) {
+    paint_pixels(S);
+    preplace(S);
+  }
   const double t2 = now_s();
   const PixelGrid& g = S.grid;
   const int N = (int)S.cells.size();
@@ -696,7 +817,8 @@ int lg_run_design(const LgDesign* d, const LgParams* p, LgResult* r) {
                 g.vdd.data(), (int)S.regions.size(), rx0.data(), ry0.data(), rx1.data(), ry1.data()};
   LgCellsSoA cs{N, ox.data(), oy.data(), lx.data(), ly.data(), w.data(), h.data(), grp.data(), gnd.data()};
   LgHandle* hd = nullptr;
-  if ((st = lg_create(&gd, &cs, &S.p, &hd))) return st;
+  if ((st = on_device ? lg_create_device_grid(&gd, &cs, &S.p, d_pcode, &hd) : lg_create(&gd, &cs, &S.p, &hd)))
+    return st;
   auto done = [&](int code) { lg_destroy(hd); return code; };
   if ((st = lg_solve(hd, nullptr))) return done(st);
   const double t3 = now_s();

@@ -26,6 +26,7 @@ _i32p = C.POINTER(C.c_int32)
 
 
 EXPORTED = ["lg_default_params", "lg_last_error", "lg_run_design", "lg_import_and_preplace",
+            "lg_import_and_preplace_device",
             "lg_create", "lg_solve", "lg_postprocess", "lg_check", "lg_copy_out", "lg_get_stats",
             "lg_get_pixel_state", "lg_get_ovf_trace", "lg_destroy", "lg_profile_partition",
             "lg_profile_read", "lg_use_graphs", "lg_debug_phase_ticks", "lg_srv_default_config", "lg_srv_create",
@@ -48,6 +49,7 @@ def lib():
         L.lg_default_params.argtypes = [C.POINTER(LgParams)]
         L.lg_run_design.argtypes = [C.c_void_p, C.POINTER(LgParams), C.c_void_p]
         L.lg_import_and_preplace.argtypes = [C.c_void_p, C.POINTER(LgParams)] + [_i32p] * 8
+        L.lg_import_and_preplace_device.argtypes = [C.c_void_p, C.POINTER(LgParams)] + [_i32p] * 5
         L.lg_create.argtypes = [C.POINTER(LgGridDesc), C.POINTER(LgCellsSoA), C.POINTER(LgParams),
                                 C.POINTER(C.c_void_p)]
         L.lg_solve.argtypes = [C.c_void_p, C.c_void_p]
@@ -150,6 +152,23 @@ def import_and_preplace(design: Design, params: LgParams | None = None, size_fil
     rb = rb[:4 * nr].reshape(nr, 4)
     return dict(x_size=xs, y_size=ys, valid=valid, group=group, region=region, vdd_up=vdd,
                 region_bounds=rb, cell_lx=lx[:n], cell_ly=ly[:n])
+
+
+def import_and_preplace_device(design: Design, params: LgParams | None = None) -> dict:
+    """lg_import_and_preplace_device: the pixel grid built and the cells
+    preplaced on the GPU (valid / group x-major, group -1 on invalid pixels)."""
+    params = params or default_params()
+    md = MarshalledDesign(design)
+    shape = np.zeros(4, np.int32)
+    null = C.cast(None, _i32p)
+    _check(lib().lg_import_and_preplace_device(C.byref(md.struct), C.byref(params), _p(shape), null, null, null,
+                                               null))
+    xs, ys, n, _ = (int(v) for v in shape)
+    valid, group = np.zeros(xs * ys, np.int32), np.zeros(xs * ys, np.int32)
+    lx, ly = np.zeros(max(n, 1), np.int32), np.zeros(max(n, 1), np.int32)
+    _check(lib().lg_import_and_preplace_device(C.byref(md.struct), C.byref(params), _p(shape), _p(valid),
+                                               _p(group), _p(lx), _p(ly)))
+    return dict(x_size=xs, y_size=ys, valid=valid, group=group, cell_lx=lx[:n], cell_ly=ly[:n])
 
 
 def device_inputs(design: Design, params: LgParams | None = None):

@@ -265,6 +265,15 @@ int lg_run_design(const LgDesign* d, const LgParams* p, LgResult* r);
 int lg_import_and_preplace(const LgDesign* d, const LgParams* p, int32_t* shape, int32_t* valid,
                            int32_t* group, int32_t* region, int32_t* vdd_up, int32_t* region_bounds,
                            int32_t* cell_lx, int32_t* cell_ly);
+/* The same import with the pixel grid built and the cells preplaced on the
+ * device: the path lg_run_design takes when the adaptive search is off and
+ * no two fence boxes overlap (else LG_ERR_STATE).  shape[4] as above (a
+ * query with valid == NULL works for every design); valid / group are the
+ * x-major maps, group -1 where a pixel is invalid (the device keeps no group
+ * there); cell_lx / cell_ly the preplaced positions.  No counterpart in the
+ * reference (whose grid and preplace are host code). */
+int lg_import_and_preplace_device(const LgDesign* d, const LgParams* p, int32_t* shape, int32_t* valid,
+                                  int32_t* group, int32_t* cell_lx, int32_t* cell_ly);
 
 /* Device layer.  lg_create uploads the grid and the preplaced cells
  * (CudaDatabase ctor + copyFromHostCells, CudaDatabase.cu:24-236). */

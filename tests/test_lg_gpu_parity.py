@@ -282,3 +282,36 @@ def test_partition_above_lds_sort_size(gpu, algo):
     p.sorting_algorithm = SORT_ALGO[algo]
     g, o = _run_both(d, p, do_post=False)
     _assert_same(g, o)
+
+
+@pytest.mark.parametrize("case", ["plain_20k", "mixed_fences", "tiny_fences", "wide", "placed"])
+def test_device_grid_and_preplace_equal_the_host_import(gpu, case):
+    """lg_run_design builds the pixel grid and preplaces the cells on the GPU
+    (lg_import_and_preplace_device); the maps and positions equal the host
+    import's (lg_import_and_preplace), whose parity with the oracle the CPU
+    tests check."""
+    from admm_legalization_amd import gp
+    if case == "plain_20k":
+        d = synth.plain_20k()
+    elif case == "mixed_fences":
+        d = synth.mixed(30000, seed=5, util=0.85)
+    elif case == "tiny_fences":
+        d = synth.tiny(60, seed=3, fences=2)
+    elif case == "wide":
+        d = synth.with_wide_cells(synth.mixed(20000, seed=2, num_fences=0))
+    else:  # positions of a global placement (off the site grid)
+        d = synth.e2e(6000, seed=4)
+        p = gp.default_params()
+        p.num_bin_x = p.num_bin_y = 64
+        p.scaling_factor, p.reference_hpwl_ratio, p.target_density = 100.0, 0.01, 0.95
+        lx, ly, _ = gp.place_design(d, p)
+        d = d.copy_positions(lx, ly)
+    print(f"[case] device grid + preplace {case}", flush=True)
+    h = lg.import_and_preplace(d)
+    g = lg.import_and_preplace_device(d)
+    assert (g["x_size"], g["y_size"]) == (h["x_size"], h["y_size"])
+    np.testing.assert_array_equal(g["valid"], h["valid"])
+    v = h["valid"] != 0
+    np.testing.assert_array_equal(g["group"][v], h["group"][v])
+    np.testing.assert_array_equal(g["cell_lx"], h["cell_lx"])
+    np.testing.assert_array_equal(g["cell_ly"], h["cell_ly"])
