@@ -122,6 +122,8 @@ def main():
     ap.add_argument("--load", type=float, default=0.7,
                     help="served_stream: Poisson arrival rate as a fraction of the measured capacity")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--served-only", action="store_true",
+                    help="served_stream: only the timed served run (for profiling); prints its figures")
     ap.add_argument("--dump", default=None, help="served_stream: write per-request sizes, plain service "
                                                   "times and iterations to this JSON file")
     ap.add_argument("--graphs", action="store_true",
@@ -515,6 +517,14 @@ def bench_served(args):
     if not all(legal):
         print(f"# served results not legal: {R - sum(legal)} of {R}", file=sys.stderr)
         sys.exit(1)
+    if args.served_only:
+        it_us = prof0["ms"] / max(prof0["iterations"], 1) * 1e3
+        print(json.dumps({"workload": f"served_stream: {R} requests, served leg only", "cells": total,
+                          "served_s": t_served, "served_cells_per_s": served_rate,
+                          "partition_us_per_batched_iteration": it_us,
+                          "batched_iterations": prof0["iterations"],
+                          "algorithmic_GBs": prof0["bytes"] / (prof0["ms"] / 1e3) / 1e9}), flush=True)
+        return
 
     # -- plain: every plain_stride-th request alone (lg_create .. lg_copy_out)
     def plain_one(g, c, p):
