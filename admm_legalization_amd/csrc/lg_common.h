@@ -40,6 +40,9 @@ struct Ctl {
   unsigned long long fence_vio;
   unsigned long long core_vio;
   unsigned long long rail_vio;
+  unsigned long long pin_short;   // technology counts of lg_check (cells; edge_vio sums both sides)
+  unsigned long long pin_access;
+  unsigned long long edge_vio;
   unsigned long long phase_ticks[16];  // per-phase wall-clock ticks (LG_PHASE_TIMERS builds)
 };
 

@@ -177,6 +177,13 @@ struct HostPrep {
   // dual segments: maximal runs of columns (rows) inside one partition
   // column (row) under all three fold offsets; segment -> first x (y)
   std::vector<int32_t> xseg, yseg, xseg0, yseg0;
+  // ICCAD 2017 technology (tech): power metal per pixel (row-major), edge
+  // types per cell (left | right << 8), master per cell, pins {bgn, end,
+  // layer, 0} per master, the rule table (indices 0..4)
+  bool tech = false;
+  std::vector<int8_t> pmetal;
+  std::vector<int32_t> cedge, cmacro, mpin, pins;
+  int32_t rules[5] = {0, 0, 0, 0, 0};
 };
 
 // Segments of [0, n) cut wherever the partition column (row) changes under
@@ -300,18 +307,21 @@ size_t sort_bytes_per(int algo) {
 
 // LDS of one partition workgroup: pixel tile {usage|code, dual} + dual/rho +
 // vdd rows, plus the sort buffers when the sorts run in LDS.
-size_t partition_lds_bytes(int x_hint, int y_hint, int algo, int npad_hi, int lds_sort) {
+size_t partition_lds_bytes(int x_hint, int y_hint, int algo, int npad_hi, int lds_sort, bool tech) {
   // three tile arrays (usage|code, penalty term, dual) with the kernel's row
   // stride, plus the vdd rows
   const size_t rs = x_hint <= kRowStride ? kRowStride : x_hint;
   const size_t tsz = (rs * y_hint + 3) & ~(size_t)3;
   size_t tile = 3 * tsz * 4 + (size_t)((y_hint + 3) & ~3) * 4;
+  // with technology: the left / right edge maps (4 bytes each) and the
+  // power metal (1 byte) of every tile pixel
+  if (tech) tile += tsz * 9;
   // + the four-wave launches' exchange words (64) and sort arrays (2 x 256)
   return tile + (lds_sort ? (size_t)npad_hi * sort_bytes_per(algo) : (64 + 512) * 4);
 }
 
 size_t partition_lds_bytes(const LgHandle* h, int npad_hi, int lds_sort) {
-  return partition_lds_bytes(h->a.x_hint, h->a.y_hint, h->p.sorting_algorithm, npad_hi, lds_sort);
+  return partition_lds_bytes(h->a.x_hint, h->a.y_hint, h->p.sorting_algorithm, npad_hi, lds_sort, h->a.tech != 0);
 }
 
 // LDS of a workgroup on gfx950
@@ -322,7 +332,10 @@ constexpr size_t kMaxLds = 160 * 1024;
 // slots the default bitonic order is sorted in registers.
 template <int MODE>
 int launch_partition(LgHandle* h, hipStream_t s) {
-  const int reg_ok = h->p.sorting_algorithm == LG_SORT_M_BITONIC || h->p.sorting_algorithm == LG_SORT_S_BITONIC;
+  // the pin-access and edge-spacing terms are evaluated by the one-wave
+  // launches only (their LDS tile holds the edge maps and power metal)
+  const int reg_ok = !h->a.tech &&
+                     (h->p.sorting_algorithm == LG_SORT_M_BITONIC || h->p.sorting_algorithm == LG_SORT_S_BITONIC);
   const bool prof = h->prof && MODE == MODE_PRIMAL && h->ev_used + 3 <= (int)h->ev.size();
   hipEvent_t* e = prof ? &h->ev[h->ev_used] : nullptr;
   if (prof) HIP_TRY(hipEventRecord(e[0], s));
@@ -372,6 +385,65 @@ int read_ctl(LgHandle* h, hipStream_t s) {
   return h->prof ? collect_profile(h) : LG_OK;
 }
 
+
+// The optional ICCAD 2017 technology of lg_create: validated and laid out
+// for the device (power metal row-major as int8, edge types packed per
+// cell).  Absent parts are zero: no power metal, edge type 0, no pins.
+int prepare_tech(const LgGridDesc* g, const LgCellsSoA* c, HostPrep& hp) {
+  hp.tech = g->pixel_power_layer || g->num_edge_rules > 0 || c->ledge_type || c->redge_type || c->macro_id;
+  if (!hp.tech) return LG_OK;
+  const int XS = g->x_size, YS = g->y_size, N = c->num_cells;
+  if (g->num_edge_rules < 0 || (g->num_edge_rules > 0 && !g->edge_spacing_rules))
+    return fail(LG_ERR_ARG, "edge_spacing_rules missing");
+  // types are 0..2, so the kernels read indices up to 4; entries the table
+  // does not hold are 0
+  for (int i = 0; i < std::min(g->num_edge_rules, 5); i++) {
+    if (g->edge_spacing_rules[i] < 0 || g->edge_spacing_rules[i] > 1024)
+      return fail(LG_ERR_ARG, "edge spacing outside [0, 1024] sites");
+    hp.rules[i] = g->edge_spacing_rules[i];
+  }
+  hp.pmetal.assign((size_t)XS * YS, (int8_t)-1);
+  if (g->pixel_power_layer)
+    for (int x = 0; x < XS; x++)
+      for (int y = 0; y < YS; y++) {
+        const int l = g->pixel_power_layer[(size_t)x * YS + y];
+        if (l < -1 || l > 127) return fail(LG_ERR_ARG, "power-metal layer index outside [-1, 127]");
+        hp.pmetal[(size_t)y * XS + x] = (int8_t)l;
+      }
+  hp.cedge.assign(N, 0);
+  for (int i = 0; i < N; i++) {
+    const int lt = c->ledge_type ? c->ledge_type[i] : 0, rt = c->redge_type ? c->redge_type[i] : 0;
+    if (lt < 0 || lt > 2 || rt < 0 || rt > 2) return fail(LG_ERR_ARG, "edge type outside {0, 1, 2}");
+    hp.cedge[i] = lt | rt << 8;
+  }
+  if (c->macro_id) {
+    const int M = c->num_macros;
+    if (M <= 0 || !c->macro_pin_start) return fail(LG_ERR_ARG, "macro_id given without masters");
+    hp.mpin.assign(c->macro_pin_start, c->macro_pin_start + M + 1);
+    if (hp.mpin[0] != 0) return fail(LG_ERR_ARG, "macro_pin_start[0] must be 0");
+    for (int m = 0; m < M; m++)
+      if (hp.mpin[m + 1] < hp.mpin[m]) return fail(LG_ERR_ARG, "macro_pin_start not non-decreasing");
+    const int np = hp.mpin[M];
+    if (np > 0 && (!c->pin_bgn || !c->pin_end || !c->pin_layer)) return fail(LG_ERR_ARG, "pin arrays missing");
+    hp.pins.resize(4 * (size_t)np);
+    for (int i = 0; i < np; i++) {
+      hp.pins[4 * i] = c->pin_bgn[i];
+      hp.pins[4 * i + 1] = c->pin_end[i];
+      hp.pins[4 * i + 2] = c->pin_layer[i];
+      hp.pins[4 * i + 3] = 0;
+    }
+    hp.cmacro.assign(c->macro_id, c->macro_id + N);
+    for (int i = 0; i < N; i++) {
+      const int m = hp.cmacro[i];
+      if (m < 0 || m >= M) return fail(LG_ERR_ARG, "macro_id outside [0, num_macros)");
+      const int cells = c->w_grid[i] * c->h_grid[i];
+      for (int k = hp.mpin[m]; k < hp.mpin[m + 1]; k++)  // the pin box lies in the cell
+        if (c->pin_bgn[k] < 0 || c->pin_bgn[k] >= cells || c->pin_end[k] < 0 || c->pin_end[k] >= cells)
+          return fail(LG_ERR_ARG, "pin box outside its cell");
+    }
+  }
+  return LG_OK;
+}
 
 // Validation and host-side preparation of lg_create (no device calls).
 int prepare_host(const LgGridDesc* g, const LgCellsSoA* c, const LgParams* p, HostPrep& hp,
@@ -494,7 +566,7 @@ int prepare_host(const LgGridDesc* g, const LgCellsSoA* c, const LgParams* p, Ho
   }
   dual_segments(XS, p->x_hint, hp.xseg, hp.xseg0);
   dual_segments(YS, p->y_hint, hp.yseg, hp.yseg0);
-  return LG_OK;
+  return prepare_tech(g, c, hp);
 }
 
 // Partitions of one step that can sort above kMaxNpadLds slots at once:
@@ -515,6 +587,7 @@ size_t handle_bytes(const HostPrep& hp) {
   // dual segments: tables, stamps, step history
   b += (2 * (size_t)hp.x_size + 2 * (size_t)hp.y_size + hp.xseg0.size() * hp.yseg0.size()) * 4 + 256 * 4;
   b += (size_t)std::max(hp.p.max_admm_iter, 1) * 4;
+  if (hp.tech) b += P * (1 + 4 * 4) + N * 8 + (hp.mpin.size() + hp.pins.size()) * 4;  // power metal, edge maps
   return b + 64 * blk;
 }
 
@@ -535,7 +608,7 @@ int create_from_prep(const HostPrep& hp, DevPool* pool, Ctl* ext_ctl, hipStream_
   {
     // a partition's pixels are staged in LDS (the large size class also
     // holds its sort buffers there)
-    const size_t need = partition_lds_bytes(p->x_hint, p->y_hint, p->sorting_algorithm, kMaxNpadLds, 1);
+    const size_t need = partition_lds_bytes(p->x_hint, p->y_hint, p->sorting_algorithm, kMaxNpadLds, 1, hp.tech);
     if (need > kMaxLds)
       return early(fail(LG_ERR_ARG, "partition of x_hint x y_hint pixels needs " + std::to_string(need) +
                                   " bytes of LDS, more than the " + std::to_string(kMaxLds) + " of a workgroup"));
@@ -655,6 +728,30 @@ int create_from_prep(const HostPrep& hp, DevPool* pool, Ctl* ext_ctl, hipStream_
     // prices anywhere in the grid, so it keeps every price current.
     a.lazy = !p->use_adaptive_search && pool != nullptr ? 1 : 0;
   }
+  if (hp.tech) {
+    a.tech = 1;
+    int8_t* d_pm;
+    int32_t *d_ce, *d_cm = nullptr, *d_mp = nullptr, *d_pins = nullptr;
+    if ((st = upload(h, &d_pm, hp.pmetal.data(), P))) return bail(st);
+    if ((st = upload(h, &d_ce, hp.cedge.data(), N))) return bail(st);
+    if (!hp.cmacro.empty()) {
+      if ((st = upload(h, &d_cm, hp.cmacro.data(), N))) return bail(st);
+      if ((st = upload(h, &d_mp, hp.mpin.data(), hp.mpin.size()))) return bail(st);
+      if ((st = upload(h, &d_pins, hp.pins.data(), hp.pins.size()))) return bail(st);
+    }
+    a.pmetal = d_pm;
+    a.cedge = d_ce;
+    a.cmacro = d_cm;
+    a.mpin = d_mp;
+    a.pins = reinterpret_cast<const int4*>(d_pins);
+    for (int i = 0; i < 5; i++) a.rules[i] = hp.rules[i];
+    a.tech_penalty = p->tech_penalty;
+    a.edge_pen = p->edge_spacing_penalty;
+    a.rhg = hp.row_height / hp.site_width;  // row_height_in_grid (AdmmKernels.hpp:701)
+    if ((st = halloc(h, &a.edgeL, P)) || (st = halloc(h, &a.edgeR, P)) || (st = halloc(h, &a.edgeLs, P)) ||
+        (st = halloc(h, &a.edgeRs, P)))
+      return bail(st);
+  }
   if ((st = halloc(h, &a.ovf_flag, N))) return bail(st);
   if ((st = halloc(h, &a.ovf_list, N))) return bail(st);
   if ((st = halloc(h, &a.trace, std::max(p->max_admm_iter, 1)))) return bail(st);
@@ -719,6 +816,11 @@ int solve_begin(LgHandle* h, hipStream_t s) {
   h->solved = 0;
   h->host_ctl.iter = 0;
   HIP_TRY(hipMemsetAsync(a.usage, 0, h->P * sizeof(int32_t), s));
+  if (a.tech) {  // updatePixelEdgeMapInGrid (CudaAdmmLegalizer.cu:270), live and snapshot
+    hipLaunchKernelGGL(k_edge_clear, dim3(2048), dim3(256), 0, s, a);
+    if (N > 0) hipLaunchKernelGGL(k_edge_init, dim3(nb), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_edge_snap, dim3(2048), dim3(256), 0, s, a);
+  }
   if (N > 0) {
     hipLaunchKernelGGL(k_usage_scatter, dim3(nb), dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_demand, dim3(nb), dim3(256), 0, s, a);
@@ -776,6 +878,8 @@ int launch_iteration(LgHandle* h, hipStream_t s, bool catchup) {
   }
   int st;
   if ((st = launch_partition<MODE_PRIMAL>(h, s))) return st;
+  // the edge maps as the next step (or the final refine) begins
+  if (a.tech) hipLaunchKernelGGL(k_edge_snap, dim3(2048), dim3(256), 0, s, a);
   if (fork) {
     HIP_TRY(hipStreamWaitEvent(s, h->ev_join, 0));
   } else if (pass) {
@@ -959,19 +1063,31 @@ int lg_check(LgHandle* h, LgVioStat* out) {
   const int nb = (a.N + 255) / 256;
   // CudaDatabase::checkPlace (CudaDatabase.cu:326-353)
   HIP_TRY(hipMemsetAsync(a.usage, 0, h->P * sizeof(int32_t), s));
-  // disp_sum, disp_max, check_ovf, row_vio and the four strict counts
-  static_assert(offsetof(Ctl, rail_vio) - offsetof(Ctl, disp_sum) == 7 * sizeof(unsigned long long),
+  // disp_sum, disp_max, check_ovf, row_vio, the four strict counts and the
+  // three technology counts
+  static_assert(offsetof(Ctl, edge_vio) - offsetof(Ctl, disp_sum) == 10 * sizeof(unsigned long long),
                 "check counters must be contiguous");
-  HIP_TRY(hipMemsetAsync(&a.ctl->disp_sum, 0, 8 * sizeof(unsigned long long), s));
+  HIP_TRY(hipMemsetAsync(&a.ctl->disp_sum, 0, 11 * sizeof(unsigned long long), s));
   if (a.N > 0) hipLaunchKernelGGL(k_usage_scatter, dim3(nb), dim3(256), 0, s, a);
   hipLaunchKernelGGL(k_overflow, dim3(2048), dim3(256), 0, s, a);
   if (a.N > 0) hipLaunchKernelGGL(k_cell_stats, dim3(nb), dim3(256), 0, s, a);
+  if (a.tech) {
+    // countPinVio / countEdgeSpacingVio (CudaDatabase.cu:341-350) on edge
+    // maps rebuilt from the final positions (the reference counts on the
+    // maps its solve left, which the QP refine's x moves do not update)
+    hipLaunchKernelGGL(k_edge_clear, dim3(2048), dim3(256), 0, s, a);
+    if (a.N > 0) {
+      hipLaunchKernelGGL(k_edge_init, dim3(nb), dim3(256), 0, s, a);
+      hipLaunchKernelGGL(k_tech_check, dim3(nb), dim3(256), 0, s, a);
+    }
+  }
   HIP_TRY(hipGetLastError());
   int st = read_ctl(h, s);
   if (st) return st;
   const Ctl& k = h->host_ctl;
-  h->vio = LgVioStat{(int32_t)k.check_ovf, (int32_t)k.row_vio, 0, 0, 0, (int32_t)k.site_vio,
-                     (int32_t)k.fence_vio, (int32_t)k.core_vio, (int32_t)k.rail_vio};
+  h->vio = LgVioStat{(int32_t)k.check_ovf, (int32_t)k.row_vio, (int32_t)k.pin_short, (int32_t)k.pin_access,
+                     (int32_t)(k.edge_vio / 2), (int32_t)k.site_vio, (int32_t)k.fence_vio, (int32_t)k.core_vio,
+                     (int32_t)k.rail_vio};
   h->checked = 1;
   if (out) *out = h->vio;
   return LG_OK;

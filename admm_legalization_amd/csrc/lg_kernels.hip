@@ -176,6 +176,25 @@ struct Args {
   int use_adaptive, stagnation_window, min_iter_adaptive, cooltime_adaptive;
   float ovf_ratio_adaptive, coeff_neglect_disp;
   int pack_limit;      // sort keys at or above it are ranked before packing ((1 << 22) - 1)
+  // ICCAD 2017 technology (tech != 0; AdmmKernels.hpp:193-331): pixel maps
+  // row-major.  An edge-map word counts the cells whose left (edgeL) or
+  // right (edgeR) edge of type 1 (low half) / type 2 (high half) lies on the
+  // pixel, each half biased by 0x8000; edgeLs / edgeRs hold the maps as they
+  // stood when the current primal or refine step began, which a partition
+  // reads for pixels outside its own.
+  int tech;
+  const int8_t* pmetal;  // [P] power-metal layer over the pixel, or -1
+  uint32_t* edgeL;
+  uint32_t* edgeR;
+  uint32_t* edgeLs;
+  uint32_t* edgeRs;
+  const int32_t* cedge;  // [N] left | right << 8 edge type (0, 1, 2)
+  const int32_t* cmacro; // [N] master, or null (no pins)
+  const int32_t* mpin;   // [masters + 1] pin range of each master
+  const int4* pins;      // {bgn, end, layer, 0}: corners flattened x * h + y in the cell
+  int rules[5];          // spacing in sites between edge types a and b at a + b
+  float tech_penalty, edge_pen;
+  int rhg;               // row height in sites
   Ctl* ctl;
 };
 
@@ -834,8 +853,116 @@ struct Tile {
   float* t;
   float* lam;
   int* vdd;
+  uint32_t* eL;  // technology only: edge maps and power metal of the tile
+  uint32_t* eR;
+  int8_t* pm;
   int rs;
   __device__ int at(int lx, int ly) const { return ly * rs + lx; }
+};
+
+// -- ICCAD 2017 technology: edge maps, pin-access and edge-spacing costs ----
+constexpr uint32_t kEdgeBias = 0x80008000u;  // both 16-bit counts at zero
+__device__ __forceinline__ int edge_count(uint32_t v, int type) {
+  return type == 1 ? (int)(v & 0xffffu) - 0x8000 : (int)(v >> 16) - 0x8000;
+}
+// the word to add for one edge of `type` (delta +1 / -1; type 0: none)
+__device__ __forceinline__ uint32_t edge_delta(int type, int delta) {
+  return type == 1 ? (uint32_t)delta : (type == 2 ? (uint32_t)delta << 16 : 0u);
+}
+// willBeRotated (AdmmKernels.hpp:532-538): odd heights take the row's rail
+__device__ __forceinline__ bool will_rotate(int h, int vdd, int gnd) { return (h & 1) != 0 && vdd != gnd; }
+
+// changePixelEdgeType (AdmmKernels.hpp:428-465) on the LDS tile: the left
+// column of the footprint at (x0, y0) takes its left edge, the right column
+// its right edge; lanes take the rows
+__device__ __forceinline__ void edge_add_tile(const PartGeo& g, const Tile& T, int x0, int y0, int w, int h,
+                                              int lt, int rt, int delta) {
+  const int la = g.lx(x0);
+  int lb = la + w - 1;
+  if (lb >= g.xh) lb -= g.xh;
+  const int ly0 = g.ly(y0);
+  for (int i = threadIdx.x; i < h; i += blockDim.x) {
+    int ly = ly0 + i;
+    if (ly >= g.yh) ly -= g.yh;
+    if (lt) T.eL[T.at(la, ly)] += edge_delta(lt, delta);
+    if (rt) T.eR[T.at(lb, ly)] += edge_delta(rt, delta);
+  }
+}
+
+// computeNumPinVio (AdmmKernels.hpp:193-264) for a footprint at local
+// corner (lx0, ly0) of the tile: a pin box over power metal of its own layer
+// is a short, of the layer above an access violation; each counts once
+__device__ int pin_vio_tile(const Args& a, const PartGeo& g, const Tile& T, int p0, int p1, int w, int h, bool rot,
+                            int lx0, int ly0) {
+  int shrt = 0, acc = 0;
+  for (int i = p0; i < p1; i++) {
+    const int4 pn = a.pins[i];
+    int bx = pn.x / h, by = pn.x - (pn.x / h) * h, ex = pn.y / h, ey = pn.y - (pn.y / h) * h;
+    if (rot) {
+      const int ox = bx, oy = by;
+      bx = w - 1 - ex; by = h - 1 - ey;
+      ex = w - 1 - ox; ey = h - 1 - oy;
+    }
+    const int x1 = min(bx, ex), x2 = max(bx, ex), y1 = min(by, ey), y2 = max(by, ey);
+    for (int x = x1; x <= x2; x++) {
+      int lx = lx0 + x;
+      if (lx >= g.xh) lx -= g.xh;
+      for (int y = y1; y <= y2; y++) {
+        int ly = ly0 + y;
+        if (ly >= g.yh) ly -= g.yh;
+        const int l = T.pm[T.at(lx, ly)];
+        if (l > 0) {
+          if (l == pn.z) shrt = 1;
+          else if (l == pn.z + 1) acc = 1;
+        }
+      }
+    }
+  }
+  return shrt + acc;
+}
+
+// Edge counts of `type` in columns [x0, x1) of rows [ny, ny + h): pixels of
+// the partition from the live LDS tile, others from the snapshot in HBM
+// (left == true: the left-edge map, else the right-edge map)
+__device__ int edge_window(const Args& a, const PartGeo& g, const Tile& T, bool left, int type, int x0, int x1,
+                           int ny, int h) {
+  int n = 0;
+  const int xe = min(x1, a.x_size);
+  for (int x = max(x0, 0); x < xe; x++) {
+    if (g.in_x(x)) {
+      const int lx = g.lx(x);
+      int ly = g.ly(ny);
+      for (int y = 0; y < h; y++) {
+        n += edge_count((left ? T.eL : T.eR)[T.at(lx, ly)], type);
+        if (++ly == g.yh) ly = 0;
+      }
+    } else {
+      for (int y = ny; y < ny + h; y++) n += edge_count((left ? a.edgeLs : a.edgeRs)[pidx(a, x, y)], type);
+    }
+  }
+  return n;
+}
+
+// computeEdgeSpacingCost (AdmmKernels.hpp:266-331): the right edges of the
+// neighbours within the rule's spacing left of the cell, their left edges
+// right of it, times edge_spacing_penalty
+__device__ float edge_cost_tile(const Args& a, const PartGeo& g, const Tile& T, int lt, int rt, int w, int h,
+                                int nx, int ny) {
+  int l = 0, r = 0;
+  if (lt > 0)
+    l = edge_window(a, g, T, false, 1, nx - a.rules[lt + 1], nx, ny, h) +
+        edge_window(a, g, T, false, 2, nx - a.rules[lt + 2], nx, ny, h);
+  if (rt > 0)
+    r = edge_window(a, g, T, true, 1, nx + w, nx + w + a.rules[rt + 1], ny, h) +
+        edge_window(a, g, T, true, 2, nx + w, nx + w + a.rules[rt + 2], ny, h);
+  const float cl = float(l), cr = float(r);
+  return a.edge_pen * (cl + cr);
+}
+
+// the technology of the cell being moved
+struct TechCell {
+  int lt, rt;  // edge types as stored (unrotated)
+  int p0, p1;  // pin range of its master
 };
 
 __device__ __forceinline__ float pen_term(int ui, float l, float rho, bool standard) {
@@ -976,9 +1103,11 @@ __device__ __forceinline__ void eval_candidates(const Args& a, const PartGeo& g,
                                                 float disp_coeff, float thr, int cx, int cy,
                                                 int w, int h, int code, int gnd, int ox, int oy, int d0,
                                                 const int2 (&dir)[NC], float (&cst)[NC], int ucode,
-                                                unsigned long long vmask) {
+                                                unsigned long long vmask, const TechCell& tc) {
   const bool flat = g.ncol > 1 && g.nrow > 1;  // footprints inside one interval never wrap
-  if (MODE == MODE_PRIMAL && ucode >= 0 && flat && T.rs == kRowStride && h <= 4 && g.yh <= 64) {
+  // the technology terms are evaluated by the one-wave launches (NC == 4)
+  const bool tech = NC == 4 && a.tech != 0;
+  if (MODE == MODE_PRIMAL && !tech && ucode >= 0 && flat && T.rs == kRowStride && h <= 4 && g.yh <= 64) {
     // The common case without branches: a tile of one placement code, a
     // footprint of at most four rows.  The clamp of getNewCellPosition keeps
     // a candidate on the grid (cells fit the grid), and -- the partition's
@@ -1128,6 +1257,21 @@ __device__ __forceinline__ void eval_candidates(const Args& a, const PartGeo& g,
   for (int j = 0; j < NC; j++) {
     float pc = (MODE == MODE_REFINE) ? 0.0f : s[j];
     cst[j] = bad[j] ? kInf : cst[j] + pc;
+  }
+  if (tech) {
+    // cost = disp + penalty + pin_vio + edge_spacing (AdmmKernels.hpp:808-830),
+    // the last two only where the penalty is finite
+    const float pin_unit = a.tech_penalty * float(a.rhg);
+    for (int j = 0; j < NC; j++) {
+      if (bad[j]) continue;
+      int nx, ny;
+      new_pos(a, dir[j], w, h, cx, cy, nx, ny);
+      const bool rot = will_rotate(h, T.vdd[lyi0[j]], gnd);
+      const int npv = a.cmacro ? pin_vio_tile(a, g, T, tc.p0, tc.p1, w, h, rot, g.lx(nx), lyi0[j]) : 0;
+      const float pin = pin_unit * float(npv);
+      const float edge = edge_cost_tile(a, g, T, rot ? tc.rt : tc.lt, rot ? tc.lt : tc.rt, w, h, nx, ny);
+      cst[j] = (cst[j] + pin) + edge;
+    }
   }
 }
 
@@ -1828,10 +1972,19 @@ __device__ __forceinline__ void partition_body(const Args& a, int bx, int npad_l
   T.t = reinterpret_cast<float*>(T.uc + tsz);                  // [rs * yh]
   T.lam = T.t + tsz;                                           // [rs * yh]
   T.vdd = reinterpret_cast<int*>(T.lam + tsz);                 // [yh]
+  int* after_tile = T.vdd + ((g.yh + 3) & ~3);
+  T.eL = T.eR = nullptr;
+  T.pm = nullptr;
+  if (NW == 1 && a.tech) {  // edge maps and power metal of the tile
+    T.eL = reinterpret_cast<uint32_t*>(after_tile);            // [rs * yh]
+    T.eR = T.eL + tsz;                                         // [rs * yh]
+    T.pm = reinterpret_cast<int8_t*>(T.eR + tsz);              // [rs * yh] bytes
+    after_tile = reinterpret_cast<int*>(T.pm + tsz);           // tsz is a multiple of 4
+  }
   // LDS sorts: key / slot / buffer arrays of lds_hi slots in LDS, or, for a
   // partition above kMaxNpadLds slots, in its own global-memory buffer
   const int lds_hi = npad_hi < kMaxNpadLds ? npad_hi : kMaxNpadLds;
-  int* keys = T.vdd + ((g.yh + 3) & ~3);                       // LDS sorts only
+  int* keys = after_tile;                                      // LDS sorts only
   int* slots = keys + lds_hi;
   int* aux = slots + lds_hi;
   int sort_hi = lds_hi;
@@ -1951,6 +2104,20 @@ __device__ __forceinline__ void partition_body(const Args& a, int bx, int npad_l
   for (int b = tid; b < g.yh; b += NT) {
     int yy = g.gy(b);
     T.vdd[b] = yy >= 0 ? a.vdd_up[yy] : 0;
+  }
+  if (NW == 1 && a.tech) {
+    // the tile's edge maps (no other partition writes them this step) and
+    // power metal; pixels outside the grid carry no edges and no metal
+    for (int i = tid; i < npix; i += NT) {
+      const int lb = i / g.xh, la = i - lb * g.xh;
+      const int xx = g.gx(la), yy = g.gy(lb);
+      const int li = T.at(la, lb);
+      const bool in = xx >= 0 && yy >= 0;
+      const unsigned p = in ? pidx(a, xx, yy) : 0u;
+      T.eL[li] = in ? a.edgeL[p] : kEdgeBias;
+      T.eR[li] = in ? a.edgeR[p] : kEdgeBias;
+      T.pm[li] = in ? a.pmetal[p] : (int8_t)-1;
+    }
   }
   __syncthreads();
   if (NW > 1) {
@@ -2084,8 +2251,9 @@ __device__ __forceinline__ void partition_body(const Args& a, int bx, int npad_l
           float disp_coeff = 1.0f;
           if (a.flag_iccad17) disp_coeff = float(a.N) / float(a.k_height_num[h - 1]);
           float cst[NC];
+          const TechCell no_tech{0, 0, 0, 0};
           eval_candidates<MODE, NC>(a, g, T, disp_coeff, thr, cx, cy, w, h, code, gnd, pox, poy, d0, dir, cst,
-                                    ucode, vmask);
+                                    ucode, vmask, no_tech);
           SUB_MARK(12);
           // the tree-min as below: a DPP butterfly per wave, then the four
           // wave minima
@@ -2173,14 +2341,31 @@ __device__ __forceinline__ void partition_body(const Args& a, int bx, int npad_l
   #ifdef LG_PHASE_TIMERS
           sub_acc_[5]++;
   #endif
+          // the moved cell's technology: edge types and its master's pins
+          TechCell tc{0, 0, 0, 0};
+          const bool tech = NW == 1 && a.tech != 0;
+          if (tech) {
+            const int ce = a.cedge[pid];
+            tc.lt = ce & 0xff;
+            tc.rt = ce >> 8;
+            if (a.cmacro) {
+              const int m = a.cmacro[pid];
+              tc.p0 = a.mpin[m];
+              tc.p1 = a.mpin[m + 1];
+            }
+          }
           // every wave has finished its reads of the last commit's pixels
           if (NW > 1) __syncthreads();
           footprint_add<NW>(g, T, cx, cy, w, h, -1, rho, standard);  // rip-up
+          if (tech) {  // doCommitOrRipup (AdmmKernels.hpp:467-506): the edges at the current orientation
+            const bool was = will_rotate(h, T.vdd[g.ly(cy)], gnd);
+            edge_add_tile(g, T, cx, cy, w, h, was ? tc.rt : tc.lt, was ? tc.lt : tc.rt, -1);
+          }
           __syncthreads();
           SUB_MARK(11);
           float cst[NC];
           eval_candidates<MODE, NC>(a, g, T, disp_coeff, thr, cx, cy, w, h, code, gnd, pox, poy, d0, dir, cst, ucode,
-                                    vmask);
+                                    vmask, tc);
           SUB_MARK(12);
           // Tree-min of AdmmKernels.hpp:847-858 over slots s = j*64 + lane: a
           // halving-stride tournament in which a tie keeps the lower slot.  Its
@@ -2232,6 +2417,10 @@ __device__ __forceinline__ void partition_body(const Args& a, int bx, int npad_l
           if (NW == 1) __syncthreads();
           SUB_MARK(13);
           footprint_add<NW>(g, T, nx, ny, w, h, +1, rho, standard);  // commit
+          if (tech) {
+            const bool will = will_rotate(h, T.vdd[g.ly(ny)], gnd);
+            edge_add_tile(g, T, nx, ny, w, h, will ? tc.rt : tc.lt, will ? tc.lt : tc.rt, +1);
+          }
           if (tid == 0) cellrec[pid].x = rec_pos(nx, ny);
           __syncthreads();
           SUB_MARK(14);
@@ -2289,6 +2478,17 @@ __device__ __forceinline__ void partition_body(const Args& a, int bx, int npad_l
           if (ui > 1) flag_next(a, iter, px[q], py[q]);
         }
       }
+    }
+  }
+  if (NW == 1 && a.tech) {  // the tile's edge maps back to the live maps
+    for (int i = tid; i < npix; i += NT) {
+      const int lb = i / g.xh, la = i - lb * g.xh;
+      const int xx = g.gx(la), yy = g.gy(lb);
+      if (xx < 0 || yy < 0) continue;
+      const int li = T.at(la, lb);
+      const unsigned p = pidx(a, xx, yy);
+      a.edgeL[p] = T.eL[li];
+      a.edgeR[p] = T.eR[li];
     }
   }
   if (MODE == MODE_PRIMAL) {
@@ -2578,6 +2778,13 @@ __device__ __forceinline__ void adaptive_body(const Args& a) {
     __syncthreads();
     if (s_ovf == 0) { __syncthreads(); continue; }
     for (int q = t; q < w * h; q += 1024) a.usage[pidx(a, cx + q / h, cy + q % h)] -= 1;
+    if (a.tech && t < h) {  // the edges follow the move (doCommitOrRipup); no pin / edge cost here
+      const int ce = a.cedge[c];
+      const bool r0 = will_rotate(h, a.vdd_up[cy], a.gnd[c]);
+      const int lt = r0 ? ce >> 8 : ce & 0xff, rt = r0 ? ce & 0xff : ce >> 8;
+      if (lt) a.edgeL[pidx(a, cx, cy + t)] += edge_delta(lt, -1);
+      if (rt) a.edgeR[pidx(a, cx + w - 1, cy + t)] += edge_delta(rt, -1);
+    }
     __syncthreads();
     float disp_coeff = 1.0f;
     if (a.flag_iccad17) disp_coeff = float(a.N) / float(a.k_height_num[h - 1]);
@@ -2621,6 +2828,13 @@ __device__ __forceinline__ void adaptive_body(const Args& a) {
     if (s_cost[0] != kInf) proj_pos(a, s_idx[0], w, h, cx, cy, nx, ny);
     __syncthreads();
     for (int q = t; q < w * h; q += 1024) a.usage[pidx(a, nx + q / h, ny + q % h)] += 1;
+    if (a.tech && t < h) {
+      const int ce = a.cedge[c];
+      const bool r1 = will_rotate(h, a.vdd_up[ny], a.gnd[c]);
+      const int lt = r1 ? ce >> 8 : ce & 0xff, rt = r1 ? ce & 0xff : ce >> 8;
+      if (lt) a.edgeL[pidx(a, nx, ny + t)] += edge_delta(lt, 1);
+      if (rt) a.edgeR[pidx(a, nx + w - 1, ny + t)] += edge_delta(rt, 1);
+    }
     if (t == 0) a.cellrec[4 * c] = (nx & 0xffff) | (ny << 16);
     __syncthreads();
   }
@@ -2631,6 +2845,101 @@ __global__ __launch_bounds__(1024) void k_adaptive(Args a) { adaptive_body(a); }
 // ---------------------------------------------------------------------------
 // post-process and checks
 // ---------------------------------------------------------------------------
+
+// -- technology maps and checks ------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_edge_clear(Args a) {
+  const size_t P = (size_t)a.x_size * a.y_size;
+  for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < P; p += (size_t)gridDim.x * blockDim.x) {
+    a.edgeL[p] = kEdgeBias;
+    a.edgeR[p] = kEdgeBias;
+  }
+}
+
+// computePixelEdgeMapKernel (DatabaseKernels.hpp:105-164): one thread per
+// cell; the orientation is "rotated" whenever the row's rail differs from
+// the cell's, for every height, as that kernel takes it
+__global__ __launch_bounds__(256) void k_edge_init(Args a) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.N) return;
+  const int x = a.lx[c], y = a.ly[c], w = a.w[c], h = a.h[c];
+  if (x < 0 || y < 0 || x + w > a.x_size || y + h > a.y_size) return;
+  const int ce = a.cedge[c];
+  const bool rot = a.vdd_up[y] != a.gnd[c];
+  const int lt = rot ? ce >> 8 : ce & 0xff, rt = rot ? ce & 0xff : ce >> 8;
+  for (int yy = y; yy < y + h; yy++) {
+    if (lt) atomicAdd(&a.edgeL[pidx(a, x, yy)], edge_delta(lt, 1));
+    if (rt) atomicAdd(&a.edgeR[pidx(a, x + w - 1, yy)], edge_delta(rt, 1));
+  }
+}
+
+// the edge maps as the next primal or refine step begins
+__global__ __launch_bounds__(256) void k_edge_snap(Args a) {
+  const size_t P = (size_t)a.x_size * a.y_size;
+  for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < P; p += (size_t)gridDim.x * blockDim.x) {
+    a.edgeLs[p] = a.edgeL[p];
+    a.edgeRs[p] = a.edgeR[p];
+  }
+}
+
+// countPinVioKernel + countEdgeSpacingVioKernel (DatabaseKernels.hpp:286-466)
+// on the live maps: a short and an access violation count once per cell;
+// edge_vio sums the neighbours' edges within spacing on both sides (each
+// pair is seen from both cells)
+__global__ __launch_bounds__(256) void k_tech_check(Args a) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long shrt = 0, acc = 0, edges = 0;
+  if (c < a.N) {
+    const int x = a.lx[c], y = a.ly[c], w = a.w[c], h = a.h[c];
+    if (x >= 0 && y >= 0 && x + w <= a.x_size && y + h <= a.y_size) {
+      const bool rot = a.vdd_up[y] != a.gnd[c];
+      if (a.cmacro) {
+        const int m = a.cmacro[c];
+        for (int i = a.mpin[m]; i < a.mpin[m + 1]; i++) {
+          const int4 pn = a.pins[i];
+          int bx = pn.x / h, by = pn.x % h, ex = pn.y / h, ey = pn.y % h;
+          if (rot) {
+            const int ox = bx, oy = by;
+            bx = w - 1 - ex; by = h - 1 - ey;
+            ex = w - 1 - ox; ey = h - 1 - oy;
+          }
+          for (int xx = x + min(bx, ex); xx <= x + max(bx, ex); xx++)
+            for (int yy = y + min(by, ey); yy <= y + max(by, ey); yy++) {
+              const int l = a.pmetal[pidx(a, xx, yy)];
+              if (l > 0 && l == pn.z) shrt = 1;
+              else if (l > 0 && l == pn.z + 1) acc = 1;
+            }
+        }
+      }
+      const int ce = a.cedge[c];
+      const int lt = rot ? ce >> 8 : ce & 0xff, rt = rot ? ce & 0xff : ce >> 8;
+      int n = 0;
+      auto window = [&](const uint32_t* map, int type, int x0, int x1) {
+        for (int xx = max(x0, 0); xx < min(x1, a.x_size); xx++)
+          for (int yy = y; yy < y + h; yy++) n += edge_count(map[pidx(a, xx, yy)], type);
+      };
+      if (lt > 0) {
+        window(a.edgeR, 1, x - a.rules[lt + 1], x);
+        window(a.edgeR, 2, x - a.rules[lt + 2], x);
+      }
+      if (rt > 0) {
+        window(a.edgeL, 1, x + w, x + w + a.rules[rt + 1]);
+        window(a.edgeL, 2, x + w, x + w + a.rules[rt + 2]);
+      }
+      edges = (unsigned long long)(long long)n;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    shrt += __shfl_down(shrt, o);
+    acc += __shfl_down(acc, o);
+    edges += __shfl_down(edges, o);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (shrt) atomicAdd(&a.ctl->pin_short, shrt);
+    if (acc) atomicAdd(&a.ctl->pin_access, acc);
+    if (edges) atomicAdd(&a.ctl->edge_vio, edges);
+  }
+}
 
 // rotateKernel (PostProcessKernel.hpp:6-26)
 __global__ void k_rotate(Args a) {

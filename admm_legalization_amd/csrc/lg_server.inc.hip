@@ -510,6 +510,10 @@ int lg_srv_submit(LgServer* S, const LgGridDesc* g, const LgCellsSoA* c, const L
     delete r;
     return st;
   }
+  if (r->prep.tech) {
+    delete r;
+    return fail(LG_ERR_ARG, "request server: pin / edge-spacing technology is legalized by lg_create .. lg_solve only");
+  }
   r->bytes = handle_bytes(r->prep);
   r->num_cells = r->prep.N;
   if (r->bytes > S->pool.budget) {

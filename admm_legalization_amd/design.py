@@ -267,13 +267,57 @@ class LgGridDesc(C.Structure):
                 ("row_height", C.c_int32), ("core_lx", C.c_int32), ("core_ly", C.c_int32),
                 ("pixel_valid", _i32p), ("pixel_group", _i32p), ("pixel_region", _i32p),
                 ("row_vdd_up", _i32p), ("num_regions", C.c_int32), ("region_x_min", _i32p),
-                ("region_y_min", _i32p), ("region_x_max", _i32p), ("region_y_max", _i32p)]
+                ("region_y_min", _i32p), ("region_x_max", _i32p), ("region_y_max", _i32p),
+                ("pixel_power_layer", _i32p), ("num_edge_rules", C.c_int32), ("edge_spacing_rules", _i32p)]
 
 
 class LgCellsSoA(C.Structure):
     _fields_ = [("num_cells", C.c_int32), ("orig_lx_dbu", _i32p), ("orig_ly_dbu", _i32p),
                 ("lx_dbu", _i32p), ("ly_dbu", _i32p), ("w_grid", _i32p), ("h_grid", _i32p),
-                ("group", _i32p), ("ground_at_bottom", _i32p)]
+                ("group", _i32p), ("ground_at_bottom", _i32p),
+                ("ledge_type", _i32p), ("redge_type", _i32p), ("macro_id", _i32p), ("num_macros", C.c_int32),
+                ("macro_pin_start", _i32p), ("pin_bgn", _i32p), ("pin_end", _i32p), ("pin_layer", _i32p)]
+
+
+def marshal_device_inputs(grid: dict, cells: dict):
+    """LgGridDesc / LgCellsSoA over the arrays of `grid` and `cells` (dicts of
+    numpy arrays with the fields of the two structs; the technology keys
+    power_layer, edge_rules, ledge_type, redge_type, macro_id,
+    macro_pin_start, pin_bgn, pin_end, pin_layer are optional).  The third
+    value keeps the int32 copies alive for as long as the structs are used."""
+    keep = []
+
+    def arr(a):
+        a = np.ascontiguousarray(a, np.int32)
+        keep.append(a)
+        return a.ctypes.data_as(_i32p)
+
+    g = LgGridDesc()
+    for k in ("x_size", "y_size", "site_width", "row_height", "core_lx", "core_ly"):
+        setattr(g, k, int(grid[k]))
+    g.pixel_valid = arr(grid["valid"])
+    g.pixel_group = arr(grid["group"])
+    g.pixel_region = arr(grid.get("region", np.full(int(grid["x_size"]) * int(grid["y_size"]), -1)))
+    g.row_vdd_up = arr(grid["vdd_up"])
+    g.num_regions = len(grid.get("region_x_min", []))
+    for k in ("region_x_min", "region_y_min", "region_x_max", "region_y_max"):
+        setattr(g, k, arr(grid.get(k, np.zeros(0, np.int32))))
+    if grid.get("power_layer") is not None:
+        g.pixel_power_layer = arr(grid["power_layer"])
+    if grid.get("edge_rules") is not None:
+        g.num_edge_rules = len(grid["edge_rules"])
+        g.edge_spacing_rules = arr(grid["edge_rules"])
+    c = LgCellsSoA()
+    c.num_cells = int(len(cells["w_grid"]))
+    for k in ("orig_lx_dbu", "orig_ly_dbu", "lx_dbu", "ly_dbu", "w_grid", "h_grid", "group",
+              "ground_at_bottom"):
+        setattr(c, k, arr(cells[k]))
+    for k in ("ledge_type", "redge_type", "macro_id", "macro_pin_start", "pin_bgn", "pin_end", "pin_layer"):
+        if cells.get(k) is not None:
+            setattr(c, k, arr(cells[k]))
+    if cells.get("macro_pin_start") is not None:
+        c.num_macros = len(cells["macro_pin_start"]) - 1
+    return g, c, keep
 
 
 

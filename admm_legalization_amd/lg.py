@@ -17,7 +17,7 @@ import os
 import numpy as np
 
 from .design import (SORT_ALGO, SORT_POLICY, Design, LgCellsSoA, LgGridDesc, LgParams, LgStats,
-                     LgVioStat, MarshalledDesign, ResultBuffers, default_params)
+                     LgVioStat, MarshalledDesign, ResultBuffers, default_params, marshal_device_inputs)
 
 _LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "libskyline_amd.so")
 _lib = None
@@ -195,36 +195,6 @@ def device_inputs(design: Design, params: LgParams | None = None):
                  group=design.inst_group[mv].astype(np.int32),
                  ground_at_bottom=design.inst_ground_at_bottom[mv].astype(np.int32))
     return grid, cells, p2
-
-
-def marshal_device_inputs(grid: dict, cells: dict):
-    """LgGridDesc / LgCellsSoA over the arrays of `grid` and `cells` (dicts of
-    numpy arrays with the fields of the two structs); the third value keeps
-    the int32 copies alive for as long as the structs are used."""
-    keep = []
-
-    def arr(a):
-        a = np.ascontiguousarray(a, np.int32)
-        keep.append(a)
-        return _p(a)
-
-    g = LgGridDesc()
-    for k in ("x_size", "y_size", "site_width", "row_height", "core_lx", "core_ly"):
-        setattr(g, k, int(grid[k]))
-    g.pixel_valid = arr(grid["valid"])
-    g.pixel_group = arr(grid["group"])
-    g.pixel_region = arr(grid.get("region", np.full(int(grid["x_size"]) * int(grid["y_size"]), -1)))
-    g.row_vdd_up = arr(grid["vdd_up"])
-    nr = len(grid.get("region_x_min", []))
-    g.num_regions = nr
-    for k in ("region_x_min", "region_y_min", "region_x_max", "region_y_max"):
-        setattr(g, k, arr(grid.get(k, np.zeros(0, np.int32))))
-    c = LgCellsSoA()
-    c.num_cells = int(len(cells["w_grid"]))
-    for k in ("orig_lx_dbu", "orig_ly_dbu", "lx_dbu", "ly_dbu", "w_grid", "h_grid", "group",
-              "ground_at_bottom"):
-        setattr(c, k, arr(cells[k]))
-    return g, c, keep
 
 
 class DeviceLegalizer:

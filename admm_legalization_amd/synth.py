@@ -389,6 +389,50 @@ def tiny(num_cells: int = 40, x_size: int = 48, y_size: int = 12, seed: int = 0,
                        name="tiny")
 
 
+def with_tech(grid: dict, cells: dict, seed: int = 0, masters_per_size: int = 3, stripe_period: int = 24,
+              stripe_width: int = 2, rules=(0, 0, 2, 3, 1)):
+    """ICCAD 2017-style technology on device inputs (grid / cells dicts of
+    lg.device_inputs), in place: vertical layer-1 power stripes of
+    `stripe_width` sites every `stripe_period` sites over all rows, short
+    layer-2 pieces on every 7th row; `masters_per_size` masters for each cell
+    size with edge types 0-2 and one to three signal-pin boxes on layers 0
+    and 1 (flattened x * h + y in the cell); the edge spacing rule table
+    `rules` (index a + b: spacing in sites between edge types a and b)."""
+    rng = np.random.default_rng(seed)
+    xs, ys = int(grid["x_size"]), int(grid["y_size"])
+    pm = np.full((xs, ys), -1, np.int32)
+    for x0 in range(int(rng.integers(0, stripe_period)), xs, stripe_period):
+        pm[x0:x0 + stripe_width, :] = 1
+    for y0 in range(3, ys, 7):
+        for _ in range(max(1, xs // 400)):
+            x0 = int(rng.integers(0, max(xs - 12, 1)))
+            pm[x0:x0 + 12, y0] = 2
+    w = np.asarray(cells["w_grid"], np.int64)
+    h = np.asarray(cells["h_grid"], np.int64)
+    sizes, inv = np.unique(np.stack([w, h], 1), axis=0, return_inverse=True)
+    inv = inv.ravel()
+    M = len(sizes) * masters_per_size
+    mstart, pb, pe, pl = [0], [], [], []
+    m_led = rng.integers(0, 3, M).astype(np.int32)
+    m_red = rng.integers(0, 3, M).astype(np.int32)
+    for s, (sw_, sh_) in enumerate(sizes):
+        for _ in range(masters_per_size):
+            for _ in range(int(rng.integers(1, 4))):
+                x1, x2 = sorted(int(v) for v in rng.integers(0, sw_, 2))
+                y1, y2 = sorted(int(v) for v in rng.integers(0, sh_, 2))
+                pb.append(x1 * int(sh_) + y1)
+                pe.append(x2 * int(sh_) + y2)
+                pl.append(int(rng.integers(0, 2)))
+            mstart.append(len(pb))
+    macro = (inv * masters_per_size + rng.integers(0, masters_per_size, len(w))).astype(np.int32)
+    grid["power_layer"] = pm.ravel()
+    grid["edge_rules"] = np.asarray(rules, np.int32)
+    cells.update(ledge_type=m_led[macro], redge_type=m_red[macro], macro_id=macro,
+                 macro_pin_start=np.array(mstart, np.int32), pin_bgn=np.array(pb, np.int32),
+                 pin_end=np.array(pe, np.int32), pin_layer=np.array(pl, np.int32))
+    return grid, cells
+
+
 # ---------------------------------------------------------------------------
 # Electrostatic density workloads (global-placement snapshots)
 # ---------------------------------------------------------------------------

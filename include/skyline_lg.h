@@ -231,6 +231,17 @@ typedef struct LgGridDesc {
   const int32_t* region_y_min;
   const int32_t* region_x_max;
   const int32_t* region_y_max;
+  /* Optional ICCAD 2017 technology (NULL / 0: none).  pixel_power_layer
+   * [x_size*y_size], x-major: routing-layer index of the power metal over
+   * the pixel, or -1 (Grid::markPowerMetalLayer, Grid.cpp:345-362, from the
+   * special nets on layers 1 and 2, Legalizer::importPowerMetals,
+   * DatabaseIO.cpp:528-572).  edge_spacing_rules[num_edge_rules]: the
+   * reference's 1-D rule table, spacing in sites between edge types a and b
+   * at index a + b (Legalizer::importEdgeSpcingRules, DatabaseIO.cpp:416-441);
+   * types are 0 (none), 1 or 2, so indices up to 4 are read. */
+  const int32_t* pixel_power_layer;
+  int32_t num_edge_rules;
+  const int32_t* edge_spacing_rules;
 } LgGridDesc;
 
 typedef struct LgCellsSoA {
@@ -243,6 +254,21 @@ typedef struct LgCellsSoA {
   const int32_t* h_grid;           /* height in rows                               */
   const int32_t* group;            /* -1 or fence group                            */
   const int32_t* ground_at_bottom; /* 0/1                                          */
+  /* Optional cell technology (NULL / 0: none), what CudaDatabase copies from
+   * LGCellTechInfo (CudaDatabase.cu:85-148; DatabaseIO.cpp:450-512): the
+   * left / right edge type of each cell's master (0, 1 or 2), the master
+   * index of each cell and, per master, its signal-pin boxes in grid units,
+   * each corner flattened x * h_grid + y inside the cell, with the pin's
+   * routing-layer index.  With any of these the candidate cost gains the
+   * pin-access and edge-spacing terms (AdmmKernels.hpp:193-331, 806-830). */
+  const int32_t* ledge_type;       /* [num_cells] */
+  const int32_t* redge_type;       /* [num_cells] */
+  const int32_t* macro_id;         /* [num_cells], index < num_macros             */
+  int32_t num_macros;
+  const int32_t* macro_pin_start;  /* [num_macros + 1]                            */
+  const int32_t* pin_bgn;          /* [macro_pin_start[num_macros]]               */
+  const int32_t* pin_end;
+  const int32_t* pin_layer;
 } LgCellsSoA;
 
 typedef struct LgHandle LgHandle;

@@ -12,7 +12,8 @@ import subprocess
 import numpy as np
 
 from admm_legalization_amd.design import (Design, LgCellsSoA, LgGridDesc, LgParams, LgStats,
-                                          MarshalledDesign, ResultBuffers, default_params)
+                                          MarshalledDesign, ResultBuffers, default_params,
+                                          marshal_device_inputs)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "_build", "libref_lg.so")
@@ -108,28 +109,8 @@ def solve(grid: dict, cells: dict, params: LgParams, do_post: bool = True, trace
     """Device-layer restatement on explicit arrays (same dict layout as
     admm_legalization_amd.lg.DeviceLegalizer).  Returns a dict with grid
     positions, rotation flags, pixel usage / dual after the ADMM loop, stats."""
-    keep = []
-
-    def arr(a):
-        a = np.ascontiguousarray(a, np.int32)
-        keep.append(a)
-        return _p(a)
-
-    g = LgGridDesc()
-    for k in ("x_size", "y_size", "site_width", "row_height", "core_lx", "core_ly"):
-        setattr(g, k, int(grid[k]))
-    g.pixel_valid, g.pixel_group = arr(grid["valid"]), arr(grid["group"])
-    g.pixel_region = arr(grid.get("region", np.full(int(grid["x_size"]) * int(grid["y_size"]), -1)))
-    g.row_vdd_up = arr(grid["vdd_up"])
-    g.num_regions = len(grid.get("region_x_min", []))
-    for k in ("region_x_min", "region_y_min", "region_x_max", "region_y_max"):
-        setattr(g, k, arr(grid.get(k, np.zeros(0, np.int32))))
-    c = LgCellsSoA()
-    n = int(len(cells["w_grid"]))
-    c.num_cells = n
-    for k in ("orig_lx_dbu", "orig_ly_dbu", "lx_dbu", "ly_dbu", "w_grid", "h_grid", "group",
-              "ground_at_bottom"):
-        setattr(c, k, arr(cells[k]))
+    g, c, keep = marshal_device_inputs(grid, cells)
+    n = c.num_cells
     P = int(grid["x_size"]) * int(grid["y_size"])
     lx, ly, rot = (np.zeros(max(n, 1), np.int32) for _ in range(3))
     usage = np.zeros(P, np.int32)
@@ -147,25 +128,8 @@ def solve(grid: dict, cells: dict, params: LgParams, do_post: bool = True, trace
 
 
 def _descs(grid: dict, cells: dict, keep: list):
-    def arr(a):
-        a = np.ascontiguousarray(a, np.int32)
-        keep.append(a)
-        return _p(a)
-
-    g = LgGridDesc()
-    for k in ("x_size", "y_size", "site_width", "row_height", "core_lx", "core_ly"):
-        setattr(g, k, int(grid[k]))
-    g.pixel_valid, g.pixel_group = arr(grid["valid"]), arr(grid["group"])
-    g.pixel_region = arr(grid.get("region", np.full(int(grid["x_size"]) * int(grid["y_size"]), -1)))
-    g.row_vdd_up = arr(grid["vdd_up"])
-    g.num_regions = len(grid.get("region_x_min", []))
-    for k in ("region_x_min", "region_y_min", "region_x_max", "region_y_max"):
-        setattr(g, k, arr(grid.get(k, np.zeros(0, np.int32))))
-    c = LgCellsSoA()
-    c.num_cells = int(len(cells["w_grid"]))
-    for k in ("orig_lx_dbu", "orig_ly_dbu", "lx_dbu", "ly_dbu", "w_grid", "h_grid", "group",
-              "ground_at_bottom"):
-        setattr(c, k, arr(cells[k]))
+    g, c, k = marshal_device_inputs(grid, cells)
+    keep.extend(k)
     return g, c
 
 

@@ -73,6 +73,9 @@ enum Mutation {
   kMutNoProjection = 10,      // adaptive candidates never jump past a region
   kMutAdaptiveDispFull = 11,  // adaptive cost without the displacement divisor
   kMutIccadWeightInverted = 12,  // ICCAD17 weight count(h) / N
+  kMutPinNoRotation = 13,     // pin boxes taken unrotated on a rotated row
+  kMutEdgeWindowWide = 14,    // edge-spacing windows one site too wide
+  kMutPinLayerSame = 15,      // access violation counted as a short (no layer + 1)
 };
 static int g_mut = kMutNone;
 
@@ -559,11 +562,135 @@ struct Dev {
   std::vector<int> trace;
   int iterations = 0, converged = 0, adaptive_calls = 0;
   int64_t final_ovf = 0;
+  // ICCAD 2017 technology (CudaDatabase.h:182-213): power metal per pixel,
+  // edge types and pin boxes per cell, the rule table, and the four pixel
+  // edge maps (cells whose left / right edge of type 1 / 2 lies on the
+  // pixel), live and as they stood when the current primal or refine step
+  // began (`snap`, what a partition reads outside its own pixels)
+  bool tech = false;
+  std::vector<int> pmetal, ledge, redge, cmacro, mpin, pbgn, pend, play, rules;
+  std::vector<int> eL1, eL2, eR1, eR2, sL1, sL2, sR1, sR2;
   size_t pid(int x, int y) const { return (size_t)x * y_size + y; }
 };
 
 static bool correct_row(int h, int gnd, int vdd) {  // DeviceUtil.hpp:11-22
   return !((h & 1) == 0 && gnd != vdd);
+}
+
+// willBeRotated (AdmmKernels.hpp:532-538): odd heights take the row's rail
+static bool will_rotate(int h, int vdd, int gnd) { return (h % 2 == 0) ? false : (vdd != gnd); }
+
+// changePixelEdgeType (AdmmKernels.hpp:428-465): the left column of the
+// footprint carries the left edge type, the right column the right one
+// (both, for a one-site cell)
+static void edge_change(Dev& D, int x0, int y0, int w, int h, int lt, int rt, int delta) {
+  for (int y = y0; y < y0 + h; y++) {
+    if (lt == 1) D.eL1[D.pid(x0, y)] += delta;
+    else if (lt == 2) D.eL2[D.pid(x0, y)] += delta;
+    if (rt == 1) D.eR1[D.pid(x0 + w - 1, y)] += delta;
+    else if (rt == 2) D.eR2[D.pid(x0 + w - 1, y)] += delta;
+  }
+}
+
+// rip-up / commit of a cell's edges at (x, y) (doCommitOrRipup,
+// AdmmKernels.hpp:467-506), its orientation from willBeRotated
+static void edge_move(Dev& D, int c, int x, int y, int delta) {
+  if (!D.tech) return;
+  const bool rot = will_rotate(D.h[c], D.vdd_up[y], D.gnd[c]);
+  edge_change(D, x, y, D.w[c], D.h[c], rot ? D.redge[c] : D.ledge[c], rot ? D.ledge[c] : D.redge[c], delta);
+}
+
+// updatePixelEdgeMapInGrid + computePixelEdgeMapKernel (CudaDatabase.cu:465-493,
+// DatabaseKernels.hpp:105-164): the map of the current positions, the
+// orientation taken as rotated whenever the row's rail differs from the
+// cell's (for every height, as that kernel does)
+static void edge_init(Dev& D) {
+  if (!D.tech) return;
+  const size_t P = (size_t)D.x_size * D.y_size;
+  for (auto* v : {&D.eL1, &D.eL2, &D.eR1, &D.eR2}) v->assign(P, 0);
+  for (int c = 0; c < D.N; c++) {
+    const bool rot = D.vdd_up[D.ly[c]] != D.gnd[c];
+    edge_change(D, D.lx[c], D.ly[c], D.w[c], D.h[c], rot ? D.redge[c] : D.ledge[c],
+                rot ? D.ledge[c] : D.redge[c], +1);
+  }
+}
+
+static void edge_snapshot(Dev& D) {
+  if (!D.tech) return;
+  D.sL1 = D.eL1; D.sL2 = D.eL2; D.sR1 = D.eR1; D.sR2 = D.eR2;
+}
+
+// computeNumPinVio (AdmmKernels.hpp:193-264): a pin box over a pixel whose
+// power metal is the pin's layer is a short, one layer up an access
+// violation; each counts once per cell
+static void pin_vio_split(const Dev& D, int c, bool rot, int nx, int ny, int& shrt, int& acc) {
+  shrt = acc = 0;
+  if (D.cmacro.empty()) return;
+  const int w = D.w[c], h = D.h[c], m = D.cmacro[c];
+  for (int i = D.mpin[m]; i < D.mpin[m + 1]; i++) {
+    int bx = D.pbgn[i] / h, by = D.pbgn[i] % h, ex = D.pend[i] / h, ey = D.pend[i] % h;
+    if (rot && g_mut != kMutPinNoRotation) {
+      const int ox = bx, oy = by;
+      bx = w - 1 - ex; by = h - 1 - ey;
+      ex = w - 1 - ox; ey = h - 1 - oy;
+    }
+    const int x1 = std::min(bx, ex) + nx, y1 = std::min(by, ey) + ny;
+    const int x2 = std::max(bx, ex) + nx, y2 = std::max(by, ey) + ny;
+    for (int x = x1; x <= x2; x++)
+      for (int y = y1; y <= y2; y++) {
+        const int l = D.pmetal[D.pid(x, y)];
+        if (l > 0) {
+          if (l == D.play[i]) shrt = 1;
+          else if (l == D.play[i] + (g_mut == kMutPinLayerSame ? 0 : 1)) acc = 1;
+        }
+      }
+  }
+}
+
+static int pin_vio(const Dev& D, int c, bool rot, int nx, int ny) {
+  int s, a;
+  pin_vio_split(D, c, rot, nx, ny, s, a);
+  return s + a;
+}
+
+// the edge counts a cell sees beside a footprint at (nx, ny): reads of a
+// pixel of partition `part` come from the live map, others from the snapshot
+// (part < 0: everything live, as after the solve)
+static int edge_count(const Dev& D, int part, const std::vector<int>& live, const std::vector<int>& snap, int x0,
+                      int x1, int ny, int h) {
+  int n = 0;
+  if (g_mut == kMutEdgeWindowWide) x1++;
+  for (int x = x0; x < x1; x++)
+    for (int y = ny; y < ny + h; y++) {
+      if (x < 0 || x >= D.x_size) continue;
+      const size_t p = D.pid(x, y);
+      n += (part < 0 || D.color[p] == part) ? live[p] : snap[p];
+    }
+  return n;
+}
+
+// computeEdgeSpacingCost (AdmmKernels.hpp:266-331) as counts: neighbours'
+// right edges within the spacing left of the cell, left edges right of it
+static void edge_sides(const Dev& D, int part, int c, bool rot, int nx, int ny, int& left, int& right) {
+  const int lt = rot ? D.redge[c] : D.ledge[c], rt = rot ? D.ledge[c] : D.redge[c];
+  const int w = D.w[c], h = D.h[c];
+  left = right = 0;
+  if (lt > 0) {
+    const int s1 = D.rules[lt + 1], s2 = D.rules[lt + 2];
+    left = edge_count(D, part, D.eR1, D.sR1, nx - s1, nx, ny, h) + edge_count(D, part, D.eR2, D.sR2, nx - s2, nx, ny, h);
+  }
+  if (rt > 0) {
+    const int s1 = D.rules[rt + 1], s2 = D.rules[rt + 2];
+    right = edge_count(D, part, D.eL1, D.sL1, nx + w, nx + w + s1, ny, h) +
+            edge_count(D, part, D.eL2, D.sL2, nx + w, nx + w + s2, ny, h);
+  }
+}
+
+static float edge_cost(const Dev& D, int part, int c, bool rot, int nx, int ny) {
+  int l, r;
+  edge_sides(D, part, c, rot, nx, ny, l, r);
+  const float cl = float(l), cr = float(r);
+  return D.p.edge_spacing_penalty * (cl + cr);
 }
 
 static void init_dev(Dev& D, const LgGridDesc* g, const LgCellsSoA* c, const LgParams* p) {
@@ -614,6 +741,29 @@ static void init_dev(Dev& D, const LgGridDesc* g, const LgCellsSoA* c, const LgP
   D.usage.assign(P, 0);
   D.dual.assign(P, 0.0f);
   D.color.assign(P, 0);
+  // technology: absent parts are zero (no power metal, edge type 0, no pins)
+  D.tech = g->pixel_power_layer || g->num_edge_rules > 0 || c->ledge_type || c->redge_type || c->macro_id;
+  if (D.tech) {
+    if (g->pixel_power_layer) D.pmetal.assign(g->pixel_power_layer, g->pixel_power_layer + P);
+    else D.pmetal.assign(P, -1);
+    // rule table read at indices up to 4 (types 1 and 2); entries beyond the
+    // given table are 0
+    D.rules.assign(5, 0);
+    for (int i = 0; i < std::min(g->num_edge_rules, 5); i++) D.rules[i] = g->edge_spacing_rules[i];
+    D.ledge.assign(D.N, 0);
+    D.redge.assign(D.N, 0);
+    if (c->ledge_type) D.ledge.assign(c->ledge_type, c->ledge_type + D.N);
+    if (c->redge_type) D.redge.assign(c->redge_type, c->redge_type + D.N);
+    if (c->macro_id && c->num_macros > 0) {
+      D.cmacro.assign(c->macro_id, c->macro_id + D.N);
+      D.mpin.assign(c->macro_pin_start, c->macro_pin_start + c->num_macros + 1);
+      const int np = D.mpin[c->num_macros];
+      D.pbgn.assign(c->pin_bgn, c->pin_bgn + np);
+      D.pend.assign(c->pin_end, c->pin_end + np);
+      D.play.assign(c->pin_layer, c->pin_layer + np);
+    }
+    for (auto* v : {&D.eL1, &D.eL2, &D.eR1, &D.eR2}) v->assign(P, 0);
+  }
 }
 
 // computeUsageGridKernel (DatabaseKernels.hpp:69-103)
@@ -922,16 +1072,25 @@ static void move_cell(Dev& D, int part, int c, bool final_refine, std::vector<fl
   const int np = pow2_ceil(nd);
   int cx = D.lx[c], cy = D.ly[c];
   change_usage(D, cx, cy, D.w[c], D.h[c], -1);  // rip-up
+  edge_move(D, c, cx, cy, -1);
   bool standard = final_refine ? false : (D.p.standard_admm != 0);
+  // tech_penalty * row_height_in_grid (AdmmKernels.hpp:814): float * int
+  const float pin_unit = D.p.tech_penalty * float(D.rh / D.sw);
   for (int d = 0; d < np; d++) {
     if (d >= nd) { buf[d] = kInf; continue; }
     int nx, ny;
     new_position(D, D.dir_default, d, c, cx, cy, nx, ny);
     float dc = disp_cost(D, nx, ny, c);
     float pc = penalty_sum(D, standard, true, final_refine, part, nx, ny, c);
-    // pin-access and edge-spacing terms are zero: no pins, no edge rules
+    // pin-access and edge-spacing terms (AdmmKernels.hpp:803-830); zero
+    // without technology
     float pin = 0.0f;
     float edge = (pc == kInf) ? kInf : 0.0f;
+    if (D.tech && pc != kInf) {
+      const bool rot = will_rotate(D.h[c], D.vdd_up[ny], D.gnd[c]);
+      pin = pin_unit * float(pin_vio(D, c, rot, nx, ny));
+      edge = edge_cost(D, part, c, rot, nx, ny);
+    }
     buf[d] = dc + pc + pin + edge;
   }
   if (costs_out) std::copy(buf.begin(), buf.begin() + np, costs_out);
@@ -939,6 +1098,7 @@ static void move_cell(Dev& D, int part, int c, bool final_refine, std::vector<fl
   int nx = cx, ny = cy;
   if (buf[0] != kInf) new_position(D, D.dir_default, best, c, cx, cy, nx, ny);
   change_usage(D, nx, ny, D.w[c], D.h[c], +1);  // commit
+  edge_move(D, c, nx, ny, +1);
   D.lx[c] = nx;
   D.ly[c] = ny;
 }
@@ -995,6 +1155,7 @@ static void adaptive_move(Dev& D, int c, std::vector<float>& buf, float* costs_o
   int cx = D.lx[c], cy = D.ly[c];
   if (cell_overflow(D, cx, cy, D.w[c], D.h[c]) == 0) return;
   change_usage(D, cx, cy, D.w[c], D.h[c], -1);
+  edge_move(D, c, cx, cy, -1);  // edges follow the move; no pin / edge cost here
   for (int d = 0; d < np; d++) {
     if (d >= nd) { buf[d] = kInf; continue; }
     int nx, ny;
@@ -1009,6 +1170,7 @@ static void adaptive_move(Dev& D, int c, std::vector<float>& buf, float* costs_o
   int nx = cx, ny = cy;
   if (buf[0] != kInf) new_position_projected(D, best, c, cx, cy, nx, ny);
   change_usage(D, nx, ny, D.w[c], D.h[c], +1);
+  edge_move(D, c, nx, ny, +1);
   D.lx[c] = nx;
   D.ly[c] = ny;
 }
@@ -1047,6 +1209,7 @@ static void solve(Dev& D) {
   D.last_slow = -1;
   std::fill(D.dual.begin(), D.dual.end(), 0.0f);
   update_usage(D);
+  edge_init(D);  // CudaAdmmLegalizer.cu:270
   initialize_rho(D);
   initialize_dual(D);
   D.trace.clear();
@@ -1059,6 +1222,12 @@ static void solve(Dev& D) {
       D.dual_step += D.p.coeff_to_update_dual_step * D.initial_dual;
     if (!exceptional || !D.p.use_adaptive_search) {
       execute_sorting(D);
+      // The reference's partitions read their neighbours' edge maps while
+      // those move cells (an unsynchronised race between thread blocks);
+      // here a partition reads the pixels outside its own as they stood when
+      // the step began, one of the outcomes that race allows, and the same
+      // for every schedule.
+      edge_snapshot(D);
       bool pert = iter < D.p.iter_perturbation_on;
 #pragma omp parallel num_threads(threads)
       {
@@ -1087,6 +1256,7 @@ static void solve(Dev& D) {
     }
     if (conv) {
       D.converged = 1;
+      edge_snapshot(D);
 #pragma omp parallel num_threads(threads)
       {
         std::vector<float> buf(256);
@@ -1417,6 +1587,26 @@ static LgVioStat check_place(Dev& D) {
     const bool row_in = D.ly[c] >= 0 && D.ly[c] < D.y_size;
     if (D.h[c] % 2 == 0 && (!row_in || D.gnd[c] != D.vdd_up[D.ly[c]])) v.num_rail_vio++;
   }
+  if (D.tech) {
+    // countPinVioKernel / countEdgeSpacingVioKernel (DatabaseKernels.hpp:
+    // 286-504), each cell's orientation from its row's rail for every
+    // height.  The edge maps are rebuilt from the final positions first;
+    // the reference counts on the maps the solve left, which the QP refine
+    // (it moves cells in x) does not update.
+    edge_init(D);
+    int64_t edge_sum = 0;
+    for (int c = 0; c < D.N; c++) {
+      const bool rot = D.vdd_up[D.ly[c]] != D.gnd[c];
+      int s, a;
+      pin_vio_split(D, c, rot, D.lx[c], D.ly[c], s, a);
+      v.num_pin_short_vio += s;
+      v.num_pin_access_vio += a;
+      int l, r;
+      edge_sides(D, -1, c, rot, D.lx[c], D.ly[c], l, r);
+      edge_sum += l + r;
+    }
+    v.num_edge_spacing_vio = (int)(edge_sum / 2);  // pairs, counted from both sides
+  }
   return v;
 }
 
@@ -1691,6 +1881,9 @@ int ref_move_one(const LgGridDesc* g, const LgCellsSoA* c, const LgParams* p, in
   std::copy(color, color + P, D.color.begin());
   D.rho = rho;
   if (cell < 0 || cell >= D.N) return LG_ERR_ARG;
+  // edge maps of the given positions, live and snapshot alike
+  ref::edge_init(D);
+  ref::edge_snapshot(D);
   std::vector<float> buf(mode == 2 ? 4096 : 256);
   if (mode == 2) ref::adaptive_move(D, cell, buf, costs);
   else ref::move_cell(D, part, cell, mode == 1, buf, costs);

@@ -35,6 +35,7 @@ INF = F(np.inf)
 # planted mistakes (oracle/ref_lg.cpp, enum Mutation)
 M_PENALTY_SWAPPED, M_NO_EXCESS, M_THR_IN_SITES, M_DUAL_NO_HALF, M_EVEN_SINGLE_STEP, M_TIE_UPPER = 1, 2, 3, 4, 5, 6
 M_NO_FOLD, M_KEY_W1, M_NO_STRADDLE, M_NO_PROJECTION, M_ADAPTIVE_DISP_FULL, M_ICCAD_INVERTED = 7, 8, 9, 10, 11, 12
+M_PIN_NO_ROTATION, M_EDGE_WINDOW_WIDE, M_PIN_LAYER_SAME = 13, 14, 15
 
 
 # -- direction tables (Legalizer::makeDirectionVectors, Legalizer.cpp:252-351) --
@@ -60,7 +61,7 @@ def test_direction_tables():
 
 # -- random states -----------------------------------------------------------
 def _state(seed, xs=44, ys=18, n=36, x_hint=16, y_hint=6, heights=(1, 1, 1, 2, 2, 3), iccad=False,
-           standard=False):
+           standard=False, tech=False):
     rng = np.random.default_rng(seed)
     valid = np.ones((xs, ys), np.int32)
     group = np.full((xs, ys), -1, np.int32)
@@ -108,9 +109,100 @@ def _state(seed, xs=44, ys=18, n=36, x_hint=16, y_hint=6, heights=(1, 1, 1, 2, 2
     cells = dict(orig_lx_dbu=ox.astype(np.int32), orig_ly_dbu=oy.astype(np.int32),
                  lx_dbu=(clx + lx * sw).astype(np.int32), ly_dbu=(cly + ly * rh).astype(np.int32),
                  w_grid=w, h_grid=h, group=cg, ground_at_bottom=gnd)
-    return dict(grid=grid, cells=cells, p=p, usage=usage, dual=dual, rho=rho, color=color.astype(np.int32),
-                lx=lx, ly=ly, w=w, h=h, cg=cg, gnd=gnd, ox=ox, oy=oy, valid=valid, group=group, region=region,
-                rb=rb, vdd=vdd, xs=xs, ys=ys, sw=sw, rh=rh, clx=clx, cly=cly, n=n)
+    S = dict(grid=grid, cells=cells, p=p, usage=usage, dual=dual, rho=rho, color=color.astype(np.int32),
+             lx=lx, ly=ly, w=w, h=h, cg=cg, gnd=gnd, ox=ox, oy=oy, valid=valid, group=group, region=region,
+             rb=rb, vdd=vdd, xs=xs, ys=ys, sw=sw, rh=rh, clx=clx, cly=cly, n=n, tech=False)
+    if tech:
+        _add_tech(S, rng)
+    return S
+
+
+def _add_tech(S, rng):
+    """Power stripes on layers 1 and 2, edge types 0-2, a master per cell
+    with one to three pin boxes on layers 0-1, a rule table with spacings
+    of 1-3 sites."""
+    xs, ys, n = S["xs"], S["ys"], S["n"]
+    pm = np.full((xs, ys), -1, np.int32)
+    for x0 in range(int(rng.integers(0, 4)), xs, 7):  # vertical layer-1 stripes
+        pm[x0:x0 + int(rng.integers(1, 3)), :] = 1
+    for y0 in range(1, ys, 5):                        # short layer-2 pieces
+        x0 = int(rng.integers(0, xs - 6))
+        pm[x0:x0 + 6, y0] = 2
+    led = rng.integers(0, 3, n).astype(np.int32)
+    red = rng.integers(0, 3, n).astype(np.int32)
+    mstart, pb, pe, pl = [0], [], [], []
+    for c in range(n):  # one master per cell
+        w, h = int(S["w"][c]), int(S["h"][c])
+        for _ in range(int(rng.integers(1, 4))):
+            x1, x2 = sorted(int(v) for v in rng.integers(0, w, 2))
+            y1, y2 = sorted(int(v) for v in rng.integers(0, h, 2))
+            pb.append(x1 * h + y1)
+            pe.append(x2 * h + y2)
+            pl.append(int(rng.integers(0, 2)))
+        mstart.append(len(pb))
+    rules = np.array([0, 0, int(rng.integers(1, 4)), int(rng.integers(1, 4)), int(rng.integers(1, 4))], np.int32)
+    S["grid"]["power_layer"] = pm.ravel()
+    S["grid"]["edge_rules"] = rules
+    S["cells"].update(ledge_type=led, redge_type=red, macro_id=np.arange(n, dtype=np.int32),
+                      macro_pin_start=np.array(mstart, np.int32), pin_bgn=np.array(pb, np.int32),
+                      pin_end=np.array(pe, np.int32), pin_layer=np.array(pl, np.int32))
+    S.update(tech=True, pm=pm, led=led, red=red, mstart=mstart, pb=pb, pe=pe, pl=pl, rules=rules)
+    S["p"].tech_penalty = F(rng.uniform(0.5, 3.0))
+    S["p"].edge_spacing_penalty = F(rng.choice([7.0, 10000.0]))
+
+
+def _edge_maps(S, moved=None):
+    """Pixel edge maps of the positions (computePixelEdgeMapKernel,
+    DatabaseKernels.hpp:105-164: rotated iff the row's rail differs from the
+    cell's, any height), then, for `moved`, the rip-up of its edges as
+    doCommitOrRipup takes them (willBeRotated: odd heights only)."""
+    xs, ys = S["xs"], S["ys"]
+    L = {1: np.zeros((xs, ys), np.int64), 2: np.zeros((xs, ys), np.int64)}
+    R = {1: np.zeros((xs, ys), np.int64), 2: np.zeros((xs, ys), np.int64)}
+
+    def put(c, rot, d):
+        x, y, w, h = int(S["lx"][c]), int(S["ly"][c]), int(S["w"][c]), int(S["h"][c])
+        lt, rt = (int(S["red"][c]), int(S["led"][c])) if rot else (int(S["led"][c]), int(S["red"][c]))
+        if lt:
+            L[lt][x, y:y + h] += d
+        if rt:
+            R[rt][x + w - 1, y:y + h] += d
+
+    for c in range(S["n"]):
+        put(c, int(S["vdd"][S["ly"][c]]) != int(S["gnd"][c]), 1)
+    if moved is not None:
+        c = moved
+        rot = int(S["h"][c]) % 2 == 1 and int(S["vdd"][S["ly"][c]]) != int(S["gnd"][c])
+        put(c, rot, -1)
+    return L, R
+
+
+def _tech_terms(S, L, R, c, nx, ny):
+    """computeNumPinVio + computeEdgeSpacingCost (AdmmKernels.hpp:193-331)."""
+    w, h = int(S["w"][c]), int(S["h"][c])
+    rot = h % 2 == 1 and int(S["vdd"][ny]) != int(S["gnd"][c])
+    short = access = 0
+    for i in range(S["mstart"][c], S["mstart"][c + 1]):
+        bx, by, ex, ey = S["pb"][i] // h, S["pb"][i] % h, S["pe"][i] // h, S["pe"][i] % h
+        if rot:
+            bx, by, ex, ey = w - 1 - ex, h - 1 - ey, w - 1 - bx, h - 1 - by
+        box = S["pm"][nx + min(bx, ex):nx + max(bx, ex) + 1, ny + min(by, ey):ny + max(by, ey) + 1]
+        short |= bool(((box == S["pl"][i]) & (box > 0)).any())
+        access |= bool(((box == S["pl"][i] + 1) & (box > 0)).any())
+    pin = F(F(S["p"].tech_penalty) * F(S["rh"] // S["sw"])) * F(int(short) + int(access))
+    lt, rt = (int(S["red"][c]), int(S["led"][c])) if rot else (int(S["led"][c]), int(S["red"][c]))
+
+    def window(m, x0, x1):
+        x0, x1 = max(x0, 0), min(x1, S["xs"])
+        return int(m[x0:x1, ny:ny + h].sum()) if x1 > x0 else 0
+
+    left = right = 0
+    if lt:
+        left = window(R[1], nx - S["rules"][lt + 1], nx) + window(R[2], nx - S["rules"][lt + 2], nx)
+    if rt:
+        right = window(L[1], nx + w, nx + w + S["rules"][rt + 1]) + window(L[2], nx + w, nx + w + S["rules"][rt + 2])
+    edge = F(F(S["p"].edge_spacing_penalty) * F(F(left) + F(right)))
+    return pin, edge
 
 
 # -- independent numpy evaluation --------------------------------------------
@@ -210,6 +302,7 @@ def np_move(S, c, part, mode):
     usage[cx:cx + w, cy:cy + h] -= 1
     dirs = WIDE if mode == 2 else DIRS
     costs = np.full(4096 if mode == 2 else 256, INF, F)
+    L, R = _edge_maps(S, moved=c) if S["tech"] else (None, None)
     for d, (dx, dy) in enumerate(dirs):
         if mode == 2:
             nx, ny = _projected(S, c, int(dx), int(dy))
@@ -219,8 +312,10 @@ def np_move(S, c, part, mode):
             nx, ny = _candidate(S, c, int(dx), int(dy))
             standard = mode == 0 and bool(S["p"].standard_admm)
             pc = _penalty(S, usage, c, nx, ny, standard, True, mode == 1, part)
-            edge = INF if pc == INF else F(0)
-            costs[d] = F(F(F(_disp_cost(S, c, nx, ny) + pc) + F(0)) + edge)
+            pin, edge = F(0), (INF if pc == INF else F(0))
+            if S["tech"] and pc != INF:
+                pin, edge = _tech_terms(S, L, R, c, nx, ny)
+            costs[d] = F(F(F(_disp_cost(S, c, nx, ny) + pc) + pin) + edge)
     best = _winner(costs)
     if costs[best] == INF:
         return (cx, cy), costs
@@ -282,6 +377,36 @@ def test_final_refine_move_matches_brute_force():
 def test_move_pin_detects_planted_mistake(m):
     with ref.mutation(m):
         bad, checked, _ = _compare_moves(STATES, 0)
+    assert bad > 0, f"mutation {m} went unnoticed in {checked} moves"
+
+
+TECH_STATES = [_state(200 + s, n=40, standard=(s % 3 == 0), iccad=(s % 4 == 1), tech=True) for s in range(12)]
+
+
+def test_move_with_pins_and_edges_matches_brute_force():
+    """The pin-access and edge-spacing terms (AdmmKernels.hpp:193-331,
+    803-830) on random power stripes, edge types, pin boxes and rules: the
+    padded cost vector bit for bit and the winner, primal and final refine."""
+    bad, checked, _ = _compare_moves(TECH_STATES, 0)
+    assert checked == 72 and bad == 0
+    bad, checked, _ = _compare_moves(TECH_STATES[:6], 1)
+    assert checked == 36 and bad == 0
+    # the terms do change costs: some candidate pays a pin or edge penalty
+    S = TECH_STATES[0]
+    L, R = _edge_maps(S)
+    paid = 0
+    for c in range(S["n"]):
+        for dx, dy in DIRS[::7]:
+            nx, ny = _candidate(S, c, int(dx), int(dy))
+            pin, edge = _tech_terms(S, L, R, c, nx, ny)
+            paid += int(pin > 0) + int(edge > 0)
+    assert paid > 0
+
+
+@pytest.mark.parametrize("m", [M_PIN_NO_ROTATION, M_EDGE_WINDOW_WIDE, M_PIN_LAYER_SAME])
+def test_tech_pin_detects_planted_mistake(m):
+    with ref.mutation(m):
+        bad, checked, _ = _compare_moves(TECH_STATES, 0)
     assert bad > 0, f"mutation {m} went unnoticed in {checked} moves"
 
 
