@@ -35,6 +35,7 @@ UNITS = [
     ("wl_device.hip", "hip"),
     ("gp_device.hip", "hip"),
     ("gp_host.cpp", "cxx"),
+    ("lefdef.cpp", "cxx"),
 ]
 
 

@@ -601,6 +601,100 @@ void preplace(Session& S) {
 // import_design(S, false) collected.  Available without the adaptive
 // search (which reads host region maps) and when the fence boxes do not
 // overlap (else the overlap counts need the whole grid): *used = false then.
+// Grid-level ICCAD 2017 technology of lg_create from the design's
+// (Legalizer::importTechInfo, DatabaseIO.cpp:381-572).
+struct Tech {
+  bool on = false;
+  std::vector<int32_t> pmetal;                 // [xs * ys] x-major, -1: none
+  std::vector<int32_t> rules;                  // index a + b
+  std::vector<int32_t> ledge, redge, macro;    // per movable cell
+  std::vector<int32_t> mstart, pbgn, pend, play;
+};
+
+int import_tech(const Session& S, Tech& T) {
+  const LgDesign* d = S.d;
+  if (d->num_masters <= 0) return LG_OK;
+  const PixelGrid& g = S.grid;
+  const int M = d->num_masters;
+  if (!d->inst_master || !d->master_ledge || !d->master_redge || !d->master_is_block || !d->master_pin_start) {
+    g_host_err = "technology: master arrays missing";
+    return LG_ERR_ARG;
+  }
+  T.on = true;
+  // importEdgeSpcingRules (DatabaseIO.cpp:416-441): spacing in dbu with the
+  // dbu as float, truncated, then in whole sites
+  int max_type = 0;
+  for (int k = 0; k < d->num_edge_spacings; k++) max_type = std::max({max_type, d->es_type1[k], d->es_type2[k]});
+  T.rules.assign(max_type * 2 + 1, 0);
+  for (int k = 0; k < d->num_edge_spacings; k++) {
+    const int dbu_sp = static_cast<int>(d->es_spacing_um[k] * static_cast<float>(d->dbu_per_micron));
+    T.rules[d->es_type1[k] + d->es_type2[k]] = dbu_sp / g.sw;
+  }
+  // importCellMacro (DatabaseIO.cpp:450-512): a tech info per master that is
+  // no BLOCK, its signal pins' bounding boxes in whole sites / rows (upper
+  // corner clipped to the master), flattened x * h + y
+  std::vector<int> tech_id(M, -1), first(M, -1);
+  // the master's size in grid units, from its first movable instance
+  for (const MovCell& c : S.cells) {
+    const int m = d->inst_master[c.inst];
+    if (m >= 0 && m < M && first[m] < 0) first[m] = c.inst;
+  }
+  T.mstart.assign(1, 0);
+  for (int m = 0, id = 0; m < M; m++) {
+    if (d->master_is_block[m]) continue;
+    tech_id[m] = id++;
+    const int wg = first[m] >= 0 ? d->inst_dx[first[m]] / g.sw : 0;
+    const int hg = first[m] >= 0 ? d->inst_dy[first[m]] / g.rh : 0;
+    for (int i = d->master_pin_start[m]; i < d->master_pin_start[m + 1]; i++) {
+      if (wg == 0 || hg == 0) break;  // no movable instance: its pins are never read
+      const int x0 = d->mpin_lx[i] / g.sw, y0 = d->mpin_ly[i] / g.rh;
+      const int x1 = std::min(wg - 1, d->mpin_ux[i] / g.sw), y1 = std::min(hg - 1, d->mpin_uy[i] / g.rh);
+      if (x0 < 0 || y0 < 0 || x0 > x1 || y0 > y1) {
+        g_host_err = "technology: a pin box outside its master";
+        return LG_ERR_DESIGN;
+      }
+      T.pbgn.push_back(x0 * hg + y0);
+      T.pend.push_back(x1 * hg + y1);
+      T.play.push_back(d->mpin_layer[i]);
+    }
+    T.mstart.push_back((int32_t)T.pbgn.size());
+  }
+  const int N = (int)S.cells.size();
+  T.ledge.resize(N); T.redge.resize(N); T.macro.resize(N);
+  for (int i = 0; i < N; i++) {
+    const int m = d->inst_master[S.cells[i].inst];
+    if (m < 0 || m >= M || tech_id[m] < 0) {
+      g_host_err = "technology: a movable instance without a cell master";
+      return LG_ERR_DESIGN;
+    }
+    T.ledge[i] = d->master_ledge[m];
+    T.redge[i] = d->master_redge[m];
+    T.macro[i] = tech_id[m];
+  }
+  // importPowerMetals (DatabaseIO.cpp:528-572) + markPowerMetalLayer
+  // (Grid.cpp:345-362): wires on routing layers 1 and 2, widened by half
+  // their width and clipped to the core; a pixel keeps the lowest layer
+  T.pmetal.assign((size_t)g.xs * g.ys, -1);
+  for (int k = 0; k < d->num_pwires; k++) {
+    const int layer = d->pwire_layer[k];
+    if (layer < 0 || layer == 0 || layer > 2) continue;
+    int x1 = d->pwire_x1[k], x2 = d->pwire_x2[k], y1 = d->pwire_y1[k], y2 = d->pwire_y2[k];
+    if (x1 > x2) std::swap(x1, x2);
+    if (y1 > y2) std::swap(y1, y2);
+    if (x1 == x2 && y1 == y2) continue;  // a via
+    const int hw = d->pwire_width[k] / 2;
+    const int lx = std::max(S.core_lx, x1 - hw), ux = std::min(S.core_ux, x2 + hw);
+    const int ly = std::max(S.core_ly, y1 - hw), uy = std::min(S.core_uy, y2 + hw);
+    auto [a, b, c, e] = g.expanded(lx, ly, ux, uy);
+    for (int x = std::max(a, 0); x <= std::min(c, g.xs - 1); x++)
+      for (int y = std::max(b, 0); y <= std::min(e, g.ys - 1); y++) {
+        int32_t& v = T.pmetal[g.at(x, y)];
+        v = v == -1 ? layer : std::min(layer, v);
+      }
+  }
+  return LG_OK;
+}
+
 int device_grid(Session& S, void** d_pcode, bool* used) {
   *used = false;
   *d_pcode = nullptr;
@@ -816,6 +910,26 @@ int lg_run_design(const LgDesign* d, const LgParams* p, LgResult* r) {
   LgGridDesc gd{g.xs, g.ys, g.sw, g.rh, g.lx, g.ly, g.valid.data(), g.group.data(), g.region.data(),
                 g.vdd.data(), (int)S.regions.size(), rx0.data(), ry0.data(), rx1.data(), ry1.data()};
   LgCellsSoA cs{N, ox.data(), oy.data(), lx.data(), ly.data(), w.data(), h.data(), grp.data(), gnd.data()};
+  Tech T;
+  if ((st = import_tech(S, T))) {
+    if (d_pcode) lg_grid_free(d_pcode);
+    return st;
+  }
+  if (T.on) {
+    gd.pixel_power_layer = T.pmetal.data();
+    gd.num_edge_rules = (int32_t)T.rules.size();
+    gd.edge_spacing_rules = T.rules.data();
+    cs.ledge_type = T.ledge.data();
+    cs.redge_type = T.redge.data();
+    if (N > 0 && T.mstart.size() > 1) {
+      cs.macro_id = T.macro.data();
+      cs.num_macros = (int32_t)T.mstart.size() - 1;
+      cs.macro_pin_start = T.mstart.data();
+      cs.pin_bgn = T.pbgn.data();
+      cs.pin_end = T.pend.data();
+      cs.pin_layer = T.play.data();
+    }
+  }
   LgHandle* hd = nullptr;
   if ((st = on_device ? lg_create_device_grid(&gd, &cs, &S.p, d_pcode, &hd) : lg_create(&gd, &cs, &S.p, &hd)))
     return st;

@@ -3,6 +3,10 @@
 // script of commands (one per line or separated by ';', '#' starts a
 // comment) and runs them in order:
 //
+//   read_lef <file.lef>            CmdReadLef: technology / cell LEF (repeatable)
+//   read_def <file.def>            CmdReadDef: the design (after the LEF files)
+//   write_def <file.def>           CmdWriteDef: the DEF read, COMPONENTS at the
+//                                  legalized (else the current) positions
 //   read_design <file.lgd>         design in the LGD1 format (lgd_file.hpp)
 //   global_place [options]         CmdGlobalPlace (tclCmd/CmdGlobalPlace.hpp:11-93):
 //                                  the placed positions replace the design's
@@ -40,6 +44,7 @@
 
 #include "../../include/skyline/Legalizer.hpp"
 #include "../../include/skyline/SkyPlace.hpp"
+#include "../../include/skyline_lefdef.h"
 #include "lgd_file.hpp"
 
 namespace {
@@ -84,11 +89,80 @@ struct Command {
   }
 };
 
+// A working copy of a design read from LEF / DEF: the core arrays in the
+// DesignFile's own vectors (global_place replaces the positions), the
+// names and technology arrays pointing into the LEF / DEF database.
+void copy_design(const LgDesign& s, lgd::DesignFile& f) {
+  auto cp = [](auto* p, int n, auto& v) { v.assign(p, p + n); };
+  const int nr = s.num_rows, n = s.num_insts, nb = s.num_region_boxes;
+  cp(s.row_lx, nr, f.row_lx); cp(s.row_ly, nr, f.row_ly); cp(s.row_dx, nr, f.row_dx);
+  cp(s.row_dy, nr, f.row_dy); cp(s.row_site_width, nr, f.row_sw); cp(s.row_orient, nr, f.row_or);
+  cp(s.inst_lx, n, f.lx); cp(s.inst_ly, n, f.ly); cp(s.inst_dx, n, f.dx); cp(s.inst_dy, n, f.dy);
+  cp(s.inst_group, n, f.group); cp(s.inst_fixed, n, f.fixed); cp(s.inst_macro, n, f.macro);
+  cp(s.inst_ground_at_bottom, n, f.gnd);
+  cp(s.box_group, nb, f.box_group); cp(s.box_lx, nb, f.box_lx); cp(s.box_ly, nb, f.box_ly);
+  cp(s.box_ux, nb, f.box_ux); cp(s.box_uy, nb, f.box_uy);
+  cp(s.net_start, s.num_nets + 1, f.net_start);
+  const int np = s.num_nets > 0 ? s.net_start[s.num_nets] : 0;
+  cp(s.pin_inst, np, f.pin_inst); cp(s.pin_dx, np, f.pin_dx); cp(s.pin_dy, np, f.pin_dy);
+  f.d = s;  // scalars, names, technology
+  f.d.row_lx = f.row_lx.data(); f.d.row_ly = f.row_ly.data(); f.d.row_dx = f.row_dx.data();
+  f.d.row_dy = f.row_dy.data(); f.d.row_site_width = f.row_sw.data(); f.d.row_orient = f.row_or.data();
+  f.d.inst_lx = f.lx.data(); f.d.inst_ly = f.ly.data(); f.d.inst_dx = f.dx.data(); f.d.inst_dy = f.dy.data();
+  f.d.inst_group = f.group.data(); f.d.inst_fixed = f.fixed.data(); f.d.inst_macro = f.macro.data();
+  f.d.inst_ground_at_bottom = f.gnd.data();
+  f.d.box_group = f.box_group.data(); f.d.box_lx = f.box_lx.data(); f.d.box_ly = f.box_ly.data();
+  f.d.box_ux = f.box_ux.data(); f.d.box_uy = f.box_uy.data();
+  f.d.net_start = f.net_start.data(); f.d.pin_inst = f.pin_inst.data();
+  f.d.pin_dx = f.pin_dx.data(); f.d.pin_dy = f.pin_dy.data();
+}
+
 struct Driver {
   std::unique_ptr<lgd::DesignFile> design;
+  LdDb* ld = nullptr;  // LEF / DEF database (read_lef / read_def)
   std::vector<int32_t> lx, ly;
   std::vector<uint8_t> fl;
   bool legalized = false;
+
+  ~Driver() {
+    if (ld) ld_destroy(ld);
+  }
+
+  // CmdReadLef / CmdReadDef / CmdWriteDef (tclCmd/src/CmdCore.cpp:144-150)
+  bool read_lef(const std::vector<std::string>& tok) {
+    if (tok.size() != 2) { std::printf("read_lef <file.lef>\n"); return false; }
+    if (!ld && ld_create(&ld)) { std::printf("read_lef: %s\n", ld_last_error()); return false; }
+    if (ld_read_lef(ld, tok[1].c_str())) { std::printf("read_lef: %s\n", ld_last_error()); return false; }
+    return true;
+  }
+
+  bool read_def(const std::vector<std::string>& tok) {
+    if (tok.size() != 2) { std::printf("read_def <file.def>\n"); return false; }
+    if (!ld) { std::printf("read_def: read_lef first\n"); return false; }
+    if (ld_read_def(ld, tok[1].c_str())) { std::printf("read_def: %s\n", ld_last_error()); return false; }
+    LgDesign s;
+    if (ld_get_design(ld, &s)) { std::printf("read_def: %s\n", ld_last_error()); return false; }
+    auto f = std::make_unique<lgd::DesignFile>();
+    copy_design(s, *f);
+    std::printf("read_def: %d instances, %d rows, %d nets, %d masters\n", s.num_insts, s.num_rows, s.num_nets,
+                s.num_masters);
+    design = std::move(f);
+    legalized = false;
+    return true;
+  }
+
+  bool write_def(const std::vector<std::string>& tok) {
+    if (tok.size() != 2) { std::printf("write_def <file.def>\n"); return false; }
+    if (!ld || !design) { std::printf("write_def: no DEF read\n"); return false; }
+    // the legalized positions, else the design's current (global-placed) ones
+    const int32_t* x = legalized ? lx.data() : design->lx.data();
+    const int32_t* y = legalized ? ly.data() : design->ly.data();
+    if (ld_write_def(ld, tok[1].c_str(), x, y, legalized ? fl.data() : nullptr)) {
+      std::printf("write_def: %s\n", ld_last_error());
+      return false;
+    }
+    return true;
+  }
 
   bool read_design(const std::vector<std::string>& tok) {
     if (tok.size() != 2) { std::printf("read_design <file.lgd>\n"); return false; }
@@ -194,11 +268,12 @@ struct Driver {
     const LgStats& s = r.stats;
     // one machine-readable summary line after the reference-style report
     std::printf("[summary] converged=%d iterations=%d total_disp_dbu=%lld max_disp_dbu=%lld overlap=%d "
-                "row_orient_vio=%d site_vio=%d fence_vio=%d core_vio=%d rail_vio=%d hpwl_before_um=%.6f "
-                "hpwl_after_um=%.6f iccad17_score=%.9f\n",
+                "row_orient_vio=%d site_vio=%d fence_vio=%d core_vio=%d rail_vio=%d pin_short_vio=%d "
+                "pin_access_vio=%d edge_spacing_vio=%d hpwl_before_um=%.6f hpwl_after_um=%.6f iccad17_score=%.9f\n",
                 s.converged, s.iterations, (long long)s.total_disp_dbu, (long long)s.max_disp_dbu, s.vio.total_ovf,
                 s.vio.num_row_orient_vio, s.vio.num_site_vio, s.vio.num_fence_vio, s.vio.num_core_vio,
-                s.vio.num_rail_vio, s.hpwl_before_um, s.hpwl_after_um, s.iccad17_score);
+                s.vio.num_rail_vio, s.vio.num_pin_short_vio, s.vio.num_pin_access_vio, s.vio.num_edge_spacing_vio,
+                s.hpwl_before_um, s.hpwl_after_um, s.iccad17_score);
     legalized = true;
     return true;
   }
@@ -214,6 +289,9 @@ struct Driver {
   }
 
   bool run(const std::vector<std::string>& tok) {
+    if (tok[0] == "read_lef") return read_lef(tok);
+    if (tok[0] == "read_def") return read_def(tok);
+    if (tok[0] == "write_def") return write_def(tok);
     if (tok[0] == "read_design") return read_design(tok);
     if (tok[0] == "legalize") return legalize(tok);
     if (tok[0] == "global_place") return global_place(tok);

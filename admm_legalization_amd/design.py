@@ -57,6 +57,9 @@ class Design:
     inst_name: list = None  # optional instance names (size files)
     dbu_per_micron: int = 1000
     meta: dict = field(default_factory=dict)
+    # optional ICCAD 2017 technology: the LgDesign technology arrays by name
+    # (inst_master, master_ledge, ..., es_spacing_um; include/skyline_lg.h)
+    tech: dict = None
 
     @property
     def num_insts(self) -> int:
@@ -222,7 +225,24 @@ class LgDesign(C.Structure):
         ("num_nets", C.c_int32), ("net_start", _i32p), ("pin_inst", _i32p), ("pin_dx", _i32p),
         ("pin_dy", _i32p), ("inst_name", C.POINTER(C.c_char_p)), ("size_file", C.c_char_p),
         ("constraint_file", C.c_char_p),
+        ("num_masters", C.c_int32), ("inst_master", _i32p), ("master_ledge", _i32p), ("master_redge", _i32p),
+        ("master_is_block", _u8p), ("master_pin_start", _i32p), ("mpin_layer", _i32p), ("mpin_lx", _i32p),
+        ("mpin_ly", _i32p), ("mpin_ux", _i32p), ("mpin_uy", _i32p),
+        ("num_pwires", C.c_int32), ("pwire_layer", _i32p), ("pwire_x1", _i32p), ("pwire_y1", _i32p),
+        ("pwire_x2", _i32p), ("pwire_y2", _i32p), ("pwire_width", _i32p),
+        ("num_edge_spacings", C.c_int32), ("es_type1", _i32p), ("es_type2", _i32p),
+        ("es_spacing_um", C.POINTER(C.c_double)),
     ]
+
+# design-level technology arrays (Design.tech keys) and their C types
+TECH_FIELDS = {
+    "inst_master": C.c_int32, "master_ledge": C.c_int32, "master_redge": C.c_int32,
+    "master_is_block": C.c_uint8, "master_pin_start": C.c_int32, "mpin_layer": C.c_int32,
+    "mpin_lx": C.c_int32, "mpin_ly": C.c_int32, "mpin_ux": C.c_int32, "mpin_uy": C.c_int32,
+    "pwire_layer": C.c_int32, "pwire_x1": C.c_int32, "pwire_y1": C.c_int32, "pwire_x2": C.c_int32,
+    "pwire_y2": C.c_int32, "pwire_width": C.c_int32, "es_type1": C.c_int32, "es_type2": C.c_int32,
+    "es_spacing_um": C.c_double,
+}
 
 
 class LgVioStat(C.Structure):
@@ -361,6 +381,15 @@ class MarshalledDesign:
         self.keep["files"] = (size_file.encode() if size_file else None,
                               constraint_file.encode() if constraint_file else None)
         s.size_file, s.constraint_file = self.keep["files"]
+        if d.tech:
+            t = d.tech
+            for name, ct in TECH_FIELDS.items():
+                a = conv(t[name], np.float64 if ct is C.c_double else (np.uint8 if ct is C.c_uint8 else np.int32))
+                self.keep["tech_" + name] = a
+                setattr(s, name, _ptr(a, ct))
+            s.num_masters = len(t["master_ledge"])
+            s.num_pwires = len(t["pwire_layer"])
+            s.num_edge_spacings = len(t["es_type1"])
         self.struct = s
 
 

@@ -144,6 +144,38 @@ typedef struct LgDesign {
   const char* const* inst_name;
   const char* size_file;
   const char* constraint_file;
+  /* Optional ICCAD 2017 technology (num_masters = 0: none): what
+   * Legalizer::importTechInfo reads from the db (DatabaseIO.cpp:381-572).
+   * inst_master: the LEF master of each instance (-1: none); per master its
+   * left / right edge type (LEF58_EDGETYPE) and signal pins, each pin the
+   * routing-layer index of its ports and the bounding box of their shapes
+   * (dbu from the master's origin; the POWER / GROUND pins are left out);
+   * the special-net wire segments (routing-layer index, end points, width,
+   * dbu); the cell edge spacing table (LEF58_CELLEDGESPACINGTABLE: edge types
+   * and spacing in microns).  The routing-layer index counts ROUTING layers
+   * in LEF order from 0. */
+  int32_t num_masters;
+  const int32_t* inst_master;       /* [num_insts]                     */
+  const int32_t* master_ledge;      /* [num_masters]                   */
+  const int32_t* master_redge;
+  const uint8_t* master_is_block;   /* class BLOCK: no tech info (DatabaseIO.cpp:455) */
+  const int32_t* master_pin_start;  /* [num_masters + 1]               */
+  const int32_t* mpin_layer;        /* [pins]                          */
+  const int32_t* mpin_lx;
+  const int32_t* mpin_ly;
+  const int32_t* mpin_ux;
+  const int32_t* mpin_uy;
+  int32_t num_pwires;
+  const int32_t* pwire_layer;       /* [num_pwires]                    */
+  const int32_t* pwire_x1;
+  const int32_t* pwire_y1;
+  const int32_t* pwire_x2;
+  const int32_t* pwire_y2;
+  const int32_t* pwire_width;
+  int32_t num_edge_spacings;
+  const int32_t* es_type1;          /* [num_edge_spacings]             */
+  const int32_t* es_type2;
+  const double* es_spacing_um;
 } LgDesign;
 
 /* Violation counts of CudaDatabase::checkPlace (objects/LGVioStat.h), then

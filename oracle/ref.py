@@ -81,6 +81,29 @@ def run_design(d: Design, params: LgParams | None = None, trace_cap: int = 30000
     return rb.lx, rb.ly, rb.flipped, rb.stats_dict()
 
 
+def import_tech(d: Design, params: LgParams | None = None):
+    """Grid-level ICCAD 2017 technology of a design (Legalizer::importTechInfo):
+    power-metal map (x-major), rule table, tech-master pins and per-cell
+    edge types / master ids (movable-cell order)."""
+    params = params or default_params()
+    pre = import_and_preplace(d, params)
+    xs, ys, n = pre["x_size"], pre["y_size"], len(pre["cell_lx"])
+    md = MarshalledDesign(d)
+    cap = 1 << 20
+    pm = np.zeros(xs * ys, np.int32)
+    rules, nr = np.zeros(16, np.int32), np.zeros(1, np.int32)
+    mstart = np.zeros(len(d.tech["master_ledge"]) + 1, np.int32)
+    pb, pe, pl = (np.zeros(cap, np.int32) for _ in range(3))
+    npins = np.zeros(1, np.int32)
+    led, red, mac = (np.zeros(max(n, 1), np.int32) for _ in range(3))
+    _check(lib().ref_import_tech(C.byref(md.struct), C.byref(params), _p(pm), _p(rules), _p(nr), _p(mstart),
+                                 _p(pb), _p(pe), _p(pl), cap, _p(npins), _p(led), _p(red), _p(mac)))
+    k = int(npins[0])
+    nm = int((np.asarray(d.tech["master_is_block"]) == 0).sum()) + 1
+    return dict(x_size=xs, y_size=ys, power_layer=pm, rules=rules[:int(nr[0])], macro_pin_start=mstart[:nm],
+                pin_bgn=pb[:k], pin_end=pe[:k], pin_layer=pl[:k], ledge=led[:n], redge=red[:n], macro=mac[:n])
+
+
 def import_and_preplace(d: Design, params: LgParams | None = None, size_file: str | None = None):
     """Pixel maps (x-major), shrunk region bounds and preplaced movable-cell
     positions (dbu)."""

@@ -1652,9 +1652,96 @@ struct Export {
   std::vector<int> valid, group, region, vdd;
   std::vector<int> rx0, ry0, rx1, ry1;
   std::vector<int> ox, oy, lx, ly, w, h, grp, gnd;
+  // technology (importTechInfo)
+  std::vector<int> pmetal, rules, ledge, redge, macro, mstart, pbgn, pend, play;
   LgGridDesc g{};
   LgCellsSoA c{};
 };
+
+// Legalizer::importTechInfo (DatabaseIO.cpp:381-572) into grid units:
+// importEdgeSpcingRules (416-441), importCellMacro (443-526: a tech info per
+// non-BLOCK master, pins as bounding boxes in whole sites / rows, upper
+// corner clipped to the master, flattened x * h + y), importPowerMetals
+// (528-572) with Grid::markPowerMetalLayer (Grid.cpp:345-362).
+static int import_tech(const Host& H, const LgDesign* d, Export& E) {
+  const Grid& g = H.grid;
+  const int M = d->num_masters;
+  int max_type = 0;
+  for (int k = 0; k < d->num_edge_spacings; k++) {
+    max_type = std::max(max_type, d->es_type1[k]);
+    max_type = std::max(max_type, d->es_type2[k]);
+  }
+  E.rules.assign(max_type * 2 + 1, 0);
+  for (int k = 0; k < d->num_edge_spacings; k++) {
+    int sp_dbu = static_cast<int>(d->es_spacing_um[k] * static_cast<float>(d->dbu_per_micron));
+    E.rules[d->es_type1[k] + d->es_type2[k]] = sp_dbu / g.sw;
+  }
+  std::vector<int> id_of(M, -1), size_from(M, -1);
+  for (const Cell& c : H.cells)
+    if (size_from[d->inst_master[c.inst]] < 0) size_from[d->inst_master[c.inst]] = c.inst;
+  E.mstart.assign(1, 0);
+  int next_id = 0;
+  for (int m = 0; m < M; m++) {
+    if (d->master_is_block[m]) continue;  // DatabaseIO.cpp:454-456
+    id_of[m] = next_id++;
+    int wg = 0, hg = 0;
+    if (size_from[m] >= 0) {
+      wg = d->inst_dx[size_from[m]] / g.sw;
+      hg = d->inst_dy[size_from[m]] / g.rh;
+    }
+    for (int i = d->master_pin_start[m]; wg > 0 && i < d->master_pin_start[m + 1]; i++) {
+      int bx0 = d->mpin_lx[i] / g.sw, by0 = d->mpin_ly[i] / g.rh;
+      int bx1 = std::min(wg - 1, d->mpin_ux[i] / g.sw), by1 = std::min(hg - 1, d->mpin_uy[i] / g.rh);
+      if (bx0 < 0 || by0 < 0 || bx0 > bx1 || by0 > by1) return LG_ERR_DESIGN;
+      E.pbgn.push_back(bx0 * hg + by0);
+      E.pend.push_back(bx1 * hg + by1);
+      E.play.push_back(d->mpin_layer[i]);
+    }
+    E.mstart.push_back((int)E.pbgn.size());
+  }
+  for (const Cell& c : H.cells) {
+    int m = d->inst_master[c.inst];
+    if (m < 0 || id_of[m] < 0) return LG_ERR_DESIGN;
+    E.ledge.push_back(d->master_ledge[m]);
+    E.redge.push_back(d->master_redge[m]);
+    E.macro.push_back(id_of[m]);
+  }
+  E.pmetal.assign((size_t)g.x_size * g.y_size, -1);
+  for (int k = 0; k < d->num_pwires; k++) {
+    int layer = d->pwire_layer[k];
+    if (layer < 0) continue;               // not a routing layer
+    if (layer == 0 || layer > 2) continue;  // M1 and above M3 ignored
+    int x1 = d->pwire_x1[k], x2 = d->pwire_x2[k], y1 = d->pwire_y1[k], y2 = d->pwire_y2[k];
+    if (x1 > x2) std::swap(x1, x2);
+    if (y1 > y2) std::swap(y1, y2);
+    if (x1 == x2 && y1 == y2) continue;
+    int width = d->pwire_width[k];
+    int slx = std::max(H.core_lx, x1 - width / 2), sux = std::min(H.core_ux, x2 + width / 2);
+    int sly = std::max(H.core_ly, y1 - width / 2), suy = std::min(H.core_uy, y2 + width / 2);
+    int ax, ay, bx, by;
+    g.expanded(slx, sly, sux, suy, ax, ay, bx, by);
+    for (int x = ax; x <= bx; x++)
+      for (int y = ay; y <= by; y++) {
+        if (x < 0 || y < 0 || x >= g.x_size || y >= g.y_size) continue;
+        int& v = E.pmetal[g.id(x, y)];
+        v = v == -1 ? layer : std::min(layer, v);
+      }
+  }
+  E.g.pixel_power_layer = E.pmetal.data();
+  E.g.num_edge_rules = (int)E.rules.size();
+  E.g.edge_spacing_rules = E.rules.data();
+  E.c.ledge_type = E.ledge.data();
+  E.c.redge_type = E.redge.data();
+  if (!H.cells.empty() && E.mstart.size() > 1) {
+    E.c.macro_id = E.macro.data();
+    E.c.num_macros = (int)E.mstart.size() - 1;
+    E.c.macro_pin_start = E.mstart.data();
+    E.c.pin_bgn = E.pbgn.data();
+    E.c.pin_end = E.pend.data();
+    E.c.pin_layer = E.play.data();
+  }
+  return LG_OK;
+}
 
 static void export_host(const Host& H, Export& E) {
   const Grid& g = H.grid;
@@ -1730,6 +1817,34 @@ int ref_import_and_preplace(const LgDesign* d, const LgParams* p, int32_t* shape
   return LG_OK;
 }
 
+// The grid-level technology of a design (importTechInfo): the power-metal
+// map (x-major, x_size * y_size), the rule table (up to 16 entries; count in
+// *n_rules) and the pins of the tech masters (up to cap; count in *n_pins),
+// with the per-cell edge types and tech master ids (movable-cell order).
+int ref_import_tech(const LgDesign* d, const LgParams* p, int32_t* pmetal, int32_t* rules, int32_t* n_rules,
+                    int32_t* mstart, int32_t* pbgn, int32_t* pend, int32_t* play, int32_t cap, int32_t* n_pins,
+                    int32_t* ledge, int32_t* redge, int32_t* macro) {
+  ref::Host H;
+  int st = ref::import_db(H, d, p);
+  if (st) return st;
+  ref::Export E;
+  ref::export_host(H, E);
+  if ((st = ref::import_tech(H, d, E))) return st;
+  std::copy(E.pmetal.begin(), E.pmetal.end(), pmetal);
+  *n_rules = (int)E.rules.size();
+  for (int i = 0; i < std::min(*n_rules, 16); i++) rules[i] = E.rules[i];
+  *n_pins = (int)E.pbgn.size();
+  if (*n_pins > cap) return LG_ERR_ARG;
+  std::copy(E.mstart.begin(), E.mstart.end(), mstart);
+  std::copy(E.pbgn.begin(), E.pbgn.end(), pbgn);
+  std::copy(E.pend.begin(), E.pend.end(), pend);
+  std::copy(E.play.begin(), E.play.end(), play);
+  std::copy(E.ledge.begin(), E.ledge.end(), ledge);
+  std::copy(E.redge.begin(), E.redge.end(), redge);
+  std::copy(E.macro.begin(), E.macro.end(), macro);
+  return LG_OK;
+}
+
 // Device-layer run from explicit grid/cell descriptors: ADMM (+ optional
 // post-process).  Outputs grid positions, rotation flags, pixel state, stats.
 int ref_solve(const LgGridDesc* g, const LgCellsSoA* c, const LgParams* p, int32_t do_post,
@@ -1778,6 +1893,7 @@ int ref_run_design(const LgDesign* d, const LgParams* p, LgResult* r) {
   double t2 = ref::now_s();
   ref::Export E;
   ref::export_host(H, E);
+  if (d->num_masters > 0 && (st = ref::import_tech(H, d, E))) return st;
   LgParams pp = H.p;  // y_hint possibly adjusted
   ref::Dev D;
   ref::init_dev(D, &E.g, &E.c, &pp);

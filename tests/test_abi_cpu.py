@@ -7,11 +7,14 @@ import subprocess
 
 import pytest
 
-from admm_legalization_amd import build, density, gp, lg, wirelength
+from admm_legalization_amd import build, density, gp, lefdef, lg, wirelength
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADERS = [os.path.join(ROOT, "include", h)
-           for h in ("skyline_lg.h", "skyline_density.h", "skyline_wl.h", "skyline_gp.h")]
+# every C header of include/ (the C++ class headers under include/skyline/
+# are checked by the caller test below)
+HEADERS = sorted(os.path.join(ROOT, "include", h) for h in os.listdir(os.path.join(ROOT, "include"))
+                 if h.endswith(".h"))
+BY_NAME = {os.path.basename(h): h for h in HEADERS}
 
 
 def declared(path):
@@ -28,14 +31,19 @@ def library():
 
 
 def test_headers_declare_the_expected_entry_points():
-    lg_syms, dens_syms, wl_syms, gp_syms = (declared(h) for h in HEADERS)
+    assert set(BY_NAME) == {"skyline_lg.h", "skyline_density.h", "skyline_wl.h", "skyline_gp.h", "skyline_lefdef.h"}
+    lg_syms, dens_syms, wl_syms, gp_syms, ld_syms = (
+        declared(BY_NAME[h]) for h in ("skyline_lg.h", "skyline_density.h", "skyline_wl.h", "skyline_gp.h",
+                                       "skyline_lefdef.h"))
     assert {"lg_create", "lg_solve", "lg_postprocess", "lg_check", "lg_copy_out", "lg_destroy"} <= lg_syms
     assert {"dens_create", "dens_compute_grad", "poisson_create", "poisson_solve"} <= dens_syms
     assert {"wl_create", "wl_compute_grad", "wl_update_pin_coordinates", "tf_precondition"} <= wl_syms
+    assert {"ld_read_lef", "ld_read_def", "ld_write_def", "ld_get_design"} <= ld_syms
     assert set(lg.EXPORTED) == lg_syms
     assert set(density.EXPORTED) == dens_syms
     assert set(wirelength.EXPORTED) == wl_syms
     assert set(gp.EXPORTED) == gp_syms
+    assert set(lefdef.EXPORTED) == ld_syms
 
 
 def test_library_exports_every_declared_symbol(library):
