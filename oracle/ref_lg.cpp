@@ -13,13 +13,16 @@
 // Reductions the reference does with order-dependent FP32 thrust reduces are
 // made order-independent here (documented at each site).
 //
-// Parity pins (tests/test_oracle_*.py): brute-force candidate enumeration,
-// invariants after every iteration, closed forms for rho / lambda0, the
-// bitonic network against a reference sort, Thomas against a dense solve,
-// the cut-row QP run to convergence against the bounded isotonic-regression
-// closed form, the site-alignment DP against exhaustive enumeration.
-// Partially pinned (device == oracle and legality invariants only): the
-// adaptive candidate search and the ICCAD17 displacement weighting.
+// Parity pins (tests/test_oracle_*.py, test_tech_cpu.py, test_lefdef_cpu.py):
+// brute-force enumeration of every candidate of a primal, refine and
+// adaptive move (ICCAD17 weight, pin-access and edge-spacing terms
+// included), closed forms for the dual step and rho / lambda0, the colouring
+// and sort keys recounted, invariants after every iteration, the bitonic
+// network against a reference sort, Thomas against a dense solve, the
+// cut-row QP run to convergence against the bounded isotonic-regression
+// closed form, the site-alignment DP against exhaustive enumeration, the
+// checkPlace technology counts and the technology import against numpy
+// recounts.  Each brute-force pin is shown to catch planted mistakes.
 #include "../include/skyline_lg.h"
 
 #include <algorithm>
