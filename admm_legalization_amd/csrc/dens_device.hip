@@ -38,6 +38,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -243,11 +244,11 @@ constexpr size_t gemm_lds() {
   return PREC == DENS_PREC_F32 ? (size_t)(BM + BN) * (kBK + 4) * 4 : (size_t)(BM + BN) * (kBK + 8) * 2 * 2;
 }
 
-// Tile shapes of the GEMM kernel (BM x BN): 1 = 64x64, 2 = 64x128,
-// 3 = 128x64, 4 = 128x128; 0 = by problem size (128x128 from 512 tiles).
-// Every shape accumulates each output element over k in the same order, so
-// the shape changes the speed only.
-constexpr int kNumTiles = 4;
+// Tile shapes of the GEMM kernels (BM x BN): 1 = 64x64, 2 = 64x128,
+// 3 = 128x64, 4 = 128x128, 5 = 256x128; 0 = by problem size (128x128 from
+// 512 tiles).  Every shape accumulates each output element over k in the
+// same order, so the shape changes the speed only.
+constexpr int kNumTiles = 5;
 
 template <int PREC, int BM, int BN>
 void launch_gemm_tile(const Gemm& g, hipStream_t s) {
@@ -266,10 +267,236 @@ int launch_gemm_p(const Gemm& g, hipStream_t s, int tile) {
     case 1: launch_gemm_tile<PREC, 64, 64>(g, s); break;
     case 2: launch_gemm_tile<PREC, 64, 128>(g, s); break;
     case 3: launch_gemm_tile<PREC, 128, 64>(g, s); break;
-    default: launch_gemm_tile<PREC, 128, 128>(g, s); break;
+    case 4: launch_gemm_tile<PREC, 128, 128>(g, s); break;
+    default: launch_gemm_tile<PREC, 256, 128>(g, s); break;
   }
   DENS_TRY(hipGetLastError());
   return DENS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// GEMM on operands already in bf16 (BF16 and split-BF16 chains): C = alpha *
+// A * B^T with A = Ah (+ Al), B = Bh (+ Bl), k-contiguous, K a multiple of 8.
+// The constant cosine / sine matrices are split once at poisson_create and
+// every intermediate of the chain is split by the epilogue of the GEMM that
+// produces it, so the k-loop only moves bf16: 16-byte global loads into
+// registers for the next k-step while the MFMAs run on the current one, two
+// LDS stages, one barrier per k-step.  Per 16-deep k chunk each 32 x 32
+// block takes al*bh, ah*bl, ah*bh (split) or ah*bh, in that order, as the
+// FP32-operand kernel does, and the epilogue splits the same FP32 values,
+// so both kernels give the same bits.  256 threads = 2 x 2 waves, each
+// (BM/2) x (BN/2) of 32 x 32 MFMA blocks.
+// ---------------------------------------------------------------------------
+struct GemmBf {
+  const __bf16* Ah;
+  const __bf16* Al;
+  const __bf16* Bh;
+  const __bf16* Bl;
+  float* C0;          // FP32 output rows [0, split) ...
+  float* C1;          // ... and [split, R), or
+  __bf16* Ch;         // bf16 hi / lo output (the next GEMM's operand), all rows
+  __bf16* Cl;
+  int lda, ldb, ldc, split;
+  int R, Cn, K;
+  float alpha;
+  const float* wy2;
+  const float* wx2;
+};
+
+// k-step BK: 64, or 32 for the split tiles of 128 x 128 and up, whose two
+// LDS stages at 64 would leave one workgroup per CU
+template <bool SPLIT, int BM, int BN>
+constexpr int bf_bk() {
+  return (SPLIT && BM * BN >= 128 * 128) ? 32 : 64;
+}
+
+template <bool SPLIT, int BM, int BN>
+__global__ __launch_bounds__(256) void k_gemm_bf(GemmBf g) {
+  constexpr int BK = bf_bk<SPLIT, BM, BN>();
+  constexpr int CPR = BK / 8;              // 16-byte chunks per operand row of a k-step
+  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int FM = WM / 32, FN = WN / 32;
+  constexpr int LD = BK + 8;               // bf16 row stride: 16 B of padding, conflict-free 16-B reads
+  constexpr int CA = BM * BK / 8 / 256;    // 16-byte chunks per thread, A (per part)
+  constexpr int CB = BN * BK / 8 / 256;
+  constexpr int NP = SPLIT ? 2 : 1;        // operand parts: hi (+ lo)
+  constexpr int STAGE = (BM + BN) * LD * NP;  // bf16 elements per LDS stage
+  static_assert(CA >= 1 && CB >= 1 && CA * 256 * 8 == BM * BK && CB * 256 * 8 == BN * BK,
+                "tile does not split into 16-byte chunks over 256 threads");
+
+  extern __shared__ __align__(16) unsigned char smem[];
+  __bf16* lds = reinterpret_cast<__bf16*>(smem);
+
+  const int tiles_n = (g.Cn + BN - 1) / BN;
+  const int tm = blockIdx.x / tiles_n, tn = blockIdx.x - tm * tiles_n;
+  const int r0 = tm * BM, c0 = tn * BN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  const int l32 = lane & 31, lhi = lane >> 5;
+
+  typedef int int4v __attribute__((ext_vector_type(4)));
+  int4v ra[NP][CA], rb[NP][CB];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int i = 0; i < CA; i++) {
+      const int f = tid + i * 256, row = f / CPR, kc = (f % CPR) * 8;
+      const int gr = r0 + row, gk = k0 + kc;
+      const bool ok = gr < g.R && gk < g.K;
+      const size_t o = ok ? (size_t)gr * g.lda + gk : 0;
+      ra[0][i] = ok ? *reinterpret_cast<const int4v*>(g.Ah + o) : int4v{0, 0, 0, 0};
+      if (SPLIT) ra[NP - 1][i] = ok ? *reinterpret_cast<const int4v*>(g.Al + o) : int4v{0, 0, 0, 0};
+    }
+#pragma unroll
+    for (int i = 0; i < CB; i++) {
+      const int f = tid + i * 256, row = f / CPR, kc = (f % CPR) * 8;
+      const int gr = c0 + row, gk = k0 + kc;
+      const bool ok = gr < g.Cn && gk < g.K;
+      const size_t o = ok ? (size_t)gr * g.ldb + gk : 0;
+      rb[0][i] = ok ? *reinterpret_cast<const int4v*>(g.Bh + o) : int4v{0, 0, 0, 0};
+      if (SPLIT) rb[NP - 1][i] = ok ? *reinterpret_cast<const int4v*>(g.Bl + o) : int4v{0, 0, 0, 0};
+    }
+  };
+  // stage layout: A hi [BM][LD], B hi [BN][LD], then A lo, B lo
+  auto sstore = [&](int st) {
+    __bf16* base = lds + st * STAGE;
+#pragma unroll
+    for (int p = 0; p < NP; p++) {
+      __bf16* sa = base + p * (BM + BN) * LD;
+      __bf16* sb = sa + BM * LD;
+#pragma unroll
+      for (int i = 0; i < CA; i++) {
+        const int f = tid + i * 256, row = f / CPR, kc = (f % CPR) * 8;
+        *reinterpret_cast<int4v*>(sa + row * LD + kc) = ra[p][i];
+      }
+#pragma unroll
+      for (int i = 0; i < CB; i++) {
+        const int f = tid + i * 256, row = f / CPR, kc = (f % CPR) * 8;
+        *reinterpret_cast<int4v*>(sb + row * LD + kc) = rb[p][i];
+      }
+    }
+  };
+
+  f32x16 acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; i++)
+#pragma unroll
+    for (int j = 0; j < FN; j++)
+#pragma unroll
+      for (int r = 0; r < 16; r++) acc[i][j][r] = 0.f;
+
+  const int nk = (g.K + BK - 1) / BK;
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  for (int ks = 0; ks < nk; ks++) {
+    const bool more = ks + 1 < nk;
+    if (more) gload((ks + 1) * BK);
+    const __bf16* base = lds + (ks & 1) * STAGE;
+    const __bf16* sah = base;
+    const __bf16* sbh = base + BM * LD;
+    const __bf16* sal = base + (BM + BN) * LD;
+    const __bf16* sbl = sal + BM * LD;
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 16) {
+      bf16x8 ah[FM], al[FM], bh[FN], bl[FN];
+#pragma unroll
+      for (int i = 0; i < FM; i++) {
+        const int off = (wr * WM + i * 32 + l32) * LD + kk + 8 * lhi;
+        ah[i] = *reinterpret_cast<const bf16x8*>(sah + off);
+        if (SPLIT) al[i] = *reinterpret_cast<const bf16x8*>(sal + off);
+      }
+#pragma unroll
+      for (int j = 0; j < FN; j++) {
+        const int off = (wc * WN + j * 32 + l32) * LD + kk + 8 * lhi;
+        bh[j] = *reinterpret_cast<const bf16x8*>(sbh + off);
+        if (SPLIT) bl[j] = *reinterpret_cast<const bf16x8*>(sbl + off);
+      }
+#pragma unroll
+      for (int i = 0; i < FM; i++)
+#pragma unroll
+        for (int j = 0; j < FN; j++) {
+          if (SPLIT) {
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+          }
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+        }
+    }
+    // the other stage was last read in step ks - 1, which every wave left
+    // at that step's barrier
+    if (more) sstore((ks + 1) & 1);
+    __syncthreads();
+  }
+
+  // accumulator of a 32 x 32 block: lane l, register r -> row 8*(r/4) +
+  // 4*(l/32) + r%4, column l%32
+#pragma unroll
+  for (int i = 0; i < FM; i++)
+#pragma unroll
+    for (int j = 0; j < FN; j++)
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const int row = r0 + wr * WM + i * 32 + 8 * (r >> 2) + 4 * lhi + (r & 3);
+        const int col = c0 + wc * WN + j * 32 + l32;
+        if (row < g.R && col < g.Cn) {
+          float v = g.alpha * acc[i][j][r];
+          if (g.wy2) v = (row == 0 && col == 0) ? 0.f : v / (g.wy2[row] + g.wx2[col]);
+          if (g.Ch) {
+            __bf16 hi, lo;
+            split_bf16(v, hi, lo);
+            g.Ch[(size_t)row * g.ldc + col] = hi;
+            if (SPLIT) g.Cl[(size_t)row * g.ldc + col] = lo;
+          } else {
+            float* dst = row < g.split ? g.C0 + (size_t)row * g.ldc : g.C1 + (size_t)(row - g.split) * g.ldc;
+            dst[col] = v;
+          }
+        }
+      }
+}
+
+// FP32 -> bf16 hi (+ lo) of n values (the chain's input density)
+template <bool SPLIT>
+__global__ __launch_bounds__(256) void k_split_bf16(const float* __restrict__ x, __bf16* hi, __bf16* lo, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+    __bf16 h, l;
+    split_bf16(x[i], h, l);
+    hi[i] = h;
+    if (SPLIT) lo[i] = l;
+  }
+}
+
+template <bool SPLIT, int BM, int BN>
+constexpr size_t gemm_bf_lds() {
+  return (size_t)2 * (BM + BN) * (bf_bk<SPLIT, BM, BN>() + 8) * (SPLIT ? 2 : 1) * 2;
+}
+
+template <bool SPLIT, int BM, int BN>
+void launch_gemm_bf_tile(const GemmBf& g, hipStream_t s) {
+  const int grid = ((g.R + BM - 1) / BM) * ((g.Cn + BN - 1) / BN);
+  constexpr size_t lds = gemm_bf_lds<SPLIT, BM, BN>();
+  hipLaunchKernelGGL((k_gemm_bf<SPLIT, BM, BN>), dim3(grid), dim3(256), lds, s, g);
+}
+
+template <bool SPLIT>
+int launch_gemm_bf_p(const GemmBf& g, hipStream_t s, int tile) {
+  switch (tile) {
+    case 1: launch_gemm_bf_tile<SPLIT, 64, 64>(g, s); break;
+    case 2: launch_gemm_bf_tile<SPLIT, 64, 128>(g, s); break;
+    case 3: launch_gemm_bf_tile<SPLIT, 128, 64>(g, s); break;
+    case 4: launch_gemm_bf_tile<SPLIT, 128, 128>(g, s); break;
+    default: launch_gemm_bf_tile<SPLIT, 256, 128>(g, s); break;
+  }
+  DENS_TRY(hipGetLastError());
+  return DENS_OK;
+}
+
+int launch_gemm_bf(int prec, const GemmBf& g, hipStream_t s, int tile) {
+  if (g.K % 8 || g.lda % 8 || g.ldb % 8) return fail(DENS_ERR_ARG, "bf16 GEMM operands must be multiples of 8 wide");
+  if (tile == 0) {
+    const long tiles128 = (long)((g.R + 127) / 128) * ((g.Cn + 127) / 128);
+    tile = tiles128 >= 512 ? 4 : 1;
+  }
+  return prec == DENS_PREC_SPLIT_BF16 ? launch_gemm_bf_p<true>(g, s, tile) : launch_gemm_bf_p<false>(g, s, tile);
 }
 
 int launch_gemm(int prec, const Gemm& g, hipStream_t s, int tile) {
@@ -660,6 +887,11 @@ struct PoissonHandle {
   std::vector<void*> bufs;
   float *CN = nullptr, *CM = nullptr, *A3 = nullptr, *A4 = nullptr, *wx2 = nullptr, *wy2 = nullptr;
   float *U = nullptr, *Ap = nullptr, *V = nullptr;
+  // the tables split into bf16 hi / lo once (bin counts multiples of 8;
+  // else the BF16 chains take the FP32-operand kernel)
+  bool bf = false;
+  __bf16 *CNh = nullptr, *CNl = nullptr, *CMh = nullptr, *CMl = nullptr;
+  __bf16 *A3h = nullptr, *A3l = nullptr, *A4h = nullptr, *A4l = nullptr;
 };
 
 struct DensHandle {
@@ -804,11 +1036,52 @@ int scratch_for_current_device(float** out) {
   return DENS_OK;
 }
 
+// The chain on bf16 operands.  The workspace (4 B floats) holds the same
+// intermediates as bf16 hi / lo pairs: U hi, lo in the first B floats' bytes,
+// A' in the next, [V; Vx] in the last 2 B; the input density's hi / lo sit in
+// A''s place until GEMM 2 overwrites them.
+int poisson_chain_bf(PoissonHandle* h, const float* rho, float* psi, float* ex, float* ey, int prec, hipStream_t s,
+                     int tile, PoissonWork w) {
+  const int M = h->M, N = h->N;
+  const size_t B = (size_t)M * N;
+  const bool split = prec == DENS_PREC_SPLIT_BF16;
+  __bf16* Uh = reinterpret_cast<__bf16*>(w.U);
+  __bf16* Ul = Uh + B;
+  __bf16* Ph = reinterpret_cast<__bf16*>(w.Ap);
+  __bf16* Pl = Ph + B;
+  __bf16* Vh = reinterpret_cast<__bf16*>(w.V);
+  __bf16* Vl = Vh + 2 * B;
+  const int sb = (int)std::min<size_t>((B + 255) / 256, 4096);
+  if (split) hipLaunchKernelGGL((k_split_bf16<true>), dim3(sb), dim3(256), 0, s, rho, Ph, Pl, B);
+  else hipLaunchKernelGGL((k_split_bf16<false>), dim3(sb), dim3(256), 0, s, rho, Ph, Pl, B);
+  DENS_TRY(hipGetLastError());
+  int st;
+  GemmBf g{};
+  // U = CN * rho^T
+  g = GemmBf{h->CNh, h->CNl, Ph, Pl, nullptr, nullptr, Uh, Ul, N, N, M, N, N, M, N, 1.f, nullptr, nullptr};
+  if ((st = launch_gemm_bf(prec, g, s, tile))) return st;
+  // A' = 4/(MN) CM * U^T / (wy^2 + wx^2)
+  g = GemmBf{h->CMh, h->CMl, Uh, Ul, nullptr, nullptr, Ph, Pl, M, M, N, M, M, N, M,
+             (float)(4.0 / ((double)M * N)), h->wy2, h->wx2};
+  if ((st = launch_gemm_bf(prec, g, s, tile))) return st;
+  // [V; Vx] = [DNt; SNw] * A'^T
+  g = GemmBf{h->A3h, h->A3l, Ph, Pl, nullptr, nullptr, Vh, Vl, N, N, M, 2 * N, 2 * N, M, N, 1.f, nullptr, nullptr};
+  if ((st = launch_gemm_bf(prec, g, s, tile))) return st;
+  // [psi; Ey] = [DMt; SMw] * V^T
+  g = GemmBf{h->A4h, h->A4l, Vh, Vl, psi, ey, nullptr, nullptr, M, M, N, M, 2 * M, N, M, 1.f, nullptr, nullptr};
+  if ((st = launch_gemm_bf(prec, g, s, tile))) return st;
+  // Ex = DMt * Vx^T
+  g = GemmBf{h->A4h, h->A4l, Vh + B, Vl + B, ex, nullptr, nullptr, nullptr, M, M, N, M, M, N, M, 1.f, nullptr,
+             nullptr};
+  return launch_gemm_bf(prec, g, s, tile);
+}
+
 // Every element of U, A' and [V; Vx] is written before it is read, so the
 // workspace needs no clearing between solves.
 int poisson_chain(PoissonHandle* h, const float* rho, float* psi, float* ex, float* ey, int prec, hipStream_t s,
                   int tile, PoissonWork w) {
   const int M = h->M, N = h->N;
+  if (h->bf && prec != DENS_PREC_F32) return poisson_chain_bf(h, rho, psi, ex, ey, prec, s, tile, w);
   int st;
   Gemm g{};
   // U = CN * rho^T
@@ -858,6 +1131,24 @@ int poisson_create(int32_t nbx, int32_t nby, PoissonHandle** out) {
   if ((st = dalloc(h->bufs, &h->U, B))) return bail(st);
   if ((st = dalloc(h->bufs, &h->Ap, B))) return bail(st);
   if ((st = dalloc(h->bufs, &h->V, 2 * B))) return bail(st);
+  // SKYLINE_POISSON_FP32_OPERANDS=1 keeps the FP32-operand kernel for every
+  // precision (the two give the same bits; tests compare them)
+  const char* fp32_ops = std::getenv("SKYLINE_POISSON_FP32_OPERANDS");
+  if (nbx % 8 == 0 && nby % 8 == 0 && !(fp32_ops && fp32_ops[0] == '1')) {
+    // the tables' bf16 hi / lo, split on the device as the kernels split
+    struct Part { const float* src; size_t n; __bf16** hi; __bf16** lo; };
+    const Part parts[4] = {{h->CN, CN.size(), &h->CNh, &h->CNl}, {h->CM, CM.size(), &h->CMh, &h->CMl},
+                           {h->A3, A3.size(), &h->A3h, &h->A3l}, {h->A4, A4.size(), &h->A4h, &h->A4l}};
+    for (const Part& q : parts) {
+      __bf16* d = nullptr;
+      if ((st = dalloc(h->bufs, &d, 2 * q.n))) return bail(st);
+      *q.hi = d;
+      *q.lo = d + q.n;
+      hipLaunchKernelGGL((k_split_bf16<true>), dim3(1024), dim3(256), 0, 0, q.src, *q.hi, *q.lo, q.n);
+    }
+    if (hipDeviceSynchronize() != hipSuccess) return bail(fail(DENS_ERR_DEVICE, "splitting the tables"));
+    h->bf = true;
+  }
   *out = h;
   return DENS_OK;
 }
@@ -888,7 +1179,7 @@ int poisson_solve_noalloc(PoissonHandle* h, const float* rho, float* psi, float*
 }
 
 int poisson_set_tile(PoissonHandle* h, int32_t tile) {
-  if (!h || tile < 0 || tile > kNumTiles) return fail(DENS_ERR_ARG, "tile must be 0 (tuned) or 1..4");
+  if (!h || tile < 0 || tile > kNumTiles) return fail(DENS_ERR_ARG, "tile must be 0 (tuned) or 1..5");
   h->tile_mode = tile;
   return DENS_OK;
 }

@@ -17,6 +17,9 @@ import numpy as np
 from .lg import lib as _lglib
 
 PREC_BF16, PREC_SPLIT_BF16, PREC_F32 = 0, 1, 2
+# Poisson GEMM tile shapes (poisson_set_tile 1..NUM_TILES)
+TILE_NAMES = ["tuned", "64x64", "64x128", "128x64", "128x128", "256x128"]
+NUM_TILES = len(TILE_NAMES) - 1
 PRECISIONS = {"bf16": PREC_BF16, "split_bf16": PREC_SPLIT_BF16, "f32": PREC_F32}
 
 _f32p = C.POINTER(C.c_float)
@@ -239,13 +242,14 @@ class PoissonSolver:
             _check(lib().poisson_solve(self.h, *args, int(precision), s))
 
     def setTile(self, tile: int):
-        """GEMM tile shape: 0 = tuned per precision, 1 = 64x64, 2 = 64x128, 3 = 128x64, 4 = 128x128."""
+        """GEMM tile shape: 0 = tuned per precision, 1 = 64x64, 2 = 64x128, 3 = 128x64, 4 = 128x128,
+        5 = 256x128."""
         _check(lib().poisson_set_tile(self.h, int(tile)))
 
     def getTile(self, precision=PREC_SPLIT_BF16):
         """(shape in use, tuner's whole-solve ms per shape) for `precision`."""
         t = C.c_int32()
-        ms = (C.c_float * 4)()
+        ms = (C.c_float * NUM_TILES)()
         _check(lib().poisson_get_tile(self.h, int(precision), C.byref(t), ms))
         return t.value, list(ms)
 

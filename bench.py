@@ -804,7 +804,7 @@ def bench_density(args):
             torch.cuda.synchronize()
             solve_ms[name] = a.elapsed_time(b) / reps
             tile, tune_ms = ps.getTile(prec)
-            solve_tiles[name] = {"tile": ["", "64x64", "64x128", "128x64", "128x128"][tile],
+            solve_tiles[name] = {"tile": D.TILE_NAMES[tile],
                                  "tuner_ms": [round(m, 4) for m in tune_ms]}
         ps.close()
     n = g.num_bin_x

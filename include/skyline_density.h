@@ -155,10 +155,11 @@ int poisson_solve(PoissonHandle* h, const float* d_rho, float* d_psi, float* d_e
 int poisson_solve_noalloc(PoissonHandle* h, const float* d_rho, float* d_psi, float* d_ex, float* d_ey,
                           int32_t precision, void* stream);
 /* GEMM tile shape: 0 = tuned per precision (default), 1 = 64x64, 2 = 64x128,
- * 3 = 128x64, 4 = 128x128.  No counterpart in the reference (cuFFT there). */
+ * 3 = 128x64, 4 = 128x128, 5 = 256x128.  No counterpart in the reference
+ * (cuFFT there). */
 int poisson_set_tile(PoissonHandle* h, int32_t tile);
 /* The shape in use for `precision` (0 before tuning) and, if ms != NULL, the
- * tuner's time of the whole solve per shape [4] (zeros if not tuned). */
+ * tuner's time of the whole solve per shape [5] (zeros if not tuned). */
 int poisson_get_tile(PoissonHandle* h, int32_t precision, int32_t* tile, float* ms);
 int poisson_destroy(PoissonHandle* h);
 

@@ -193,10 +193,10 @@ def test_poisson_tile_shapes_agree(gpu, M, N, prec):
     print(f"[case] poisson tiles {M}x{N} prec={prec}", flush=True)
     rho = _density_map(M, N, seed=M + N)
     ps = D.PoissonSolver(N, M)
-    assert ps.getTile(prec) == (0, [0.0] * 4)
+    assert ps.getTile(prec) == (0, [0.0] * D.NUM_TILES)
     t = torch.tensor(rho, dtype=torch.float32, device="cuda")
     outs = []
-    for tile in (0, 1, 2, 3, 4):
+    for tile in range(D.NUM_TILES + 1):
         ps.setTile(tile)
         psi, ex, ey = (torch.empty_like(t) for _ in range(3))
         ps.solvePoisson(t, psi, ex, ey, precision=prec)
@@ -205,12 +205,37 @@ def test_poisson_tile_shapes_agree(gpu, M, N, prec):
         if tile == 0:
             chosen, ms = ps.getTile(prec)
             print(f"  tuned tile {chosen}, ms per shape {ms}")
-            assert 1 <= chosen <= 4 and all(m > 0 for m in ms) and ms[chosen - 1] == min(ms)
+            assert 1 <= chosen <= D.NUM_TILES and all(m > 0 for m in ms) and ms[chosen - 1] == min(ms)
     for o in outs[1:]:
         assert np.array_equal(o.view(np.uint32), outs[0].view(np.uint32))
     with pytest.raises(RuntimeError):
-        ps.setTile(5)
+        ps.setTile(D.NUM_TILES + 1)
     ps.close()
+
+
+@pytest.mark.parametrize("prec", [D.PREC_SPLIT_BF16, D.PREC_BF16])
+@pytest.mark.parametrize("M,N", [(64, 64), (1024, 1024), (96, 160)])
+def test_poisson_bf16_operands_equal_fp32_operands(gpu, M, N, prec, monkeypatch):
+    """The chain on bf16 operands (tables split once, intermediates split by
+    the producing GEMM's epilogue) gives the same bits as the chain that
+    loads FP32 operands and splits them in every k-step."""
+    import torch
+    print(f"[case] poisson operand paths {M}x{N} prec={prec}", flush=True)
+    rho = _density_map(M, N, seed=M * 3 + N)
+    t = torch.tensor(rho, dtype=torch.float32, device="cuda")
+    outs = []
+    for fp32_ops in ("0", "1"):
+        monkeypatch.setenv("SKYLINE_POISSON_FP32_OPERANDS", fp32_ops)
+        ps = D.PoissonSolver(N, M)
+        for tile in (1, 4, 5):
+            ps.setTile(tile)
+            psi, ex, ey = (torch.empty_like(t) for _ in range(3))
+            ps.solvePoisson(t, psi, ex, ey, precision=prec)
+            torch.cuda.synchronize()
+            outs.append(torch.stack([psi, ex, ey]).cpu().numpy())
+        ps.close()
+    for o in outs[1:]:
+        assert np.array_equal(o.view(np.uint32), outs[0].view(np.uint32))
 
 
 def test_density_dense_cluster_spans_many_chunks(gpu):
