@@ -25,7 +25,8 @@ QUEUED, RUNNING, DONE, FAILED = 0, 1, 2, 3
 class LgSrvConfig(C.Structure):
     _fields_ = [("max_slots", C.c_int32), ("iters_per_sync", C.c_int32),
                 ("mem_budget_bytes", C.c_int64), ("pool_chunk_bytes", C.c_int64),
-                ("use_graphs", C.c_int32), ("reserved", C.c_int32)]
+                ("use_graphs", C.c_int32), ("short_cells", C.c_int32), ("max_overtakes", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 class LgSrvResult(C.Structure):
@@ -71,8 +72,13 @@ class LegalizationServer:
 
     def __init__(self, max_slots: int | None = None, iters_per_sync: int | None = None,
                  mem_budget_bytes: int | None = None, pool_chunk_bytes: int | None = None,
-                 use_graphs: bool | None = None):
+                 use_graphs: bool | None = None, short_cells: int | None = None,
+                 max_overtakes: int | None = None):
         cfg = default_config()
+        if short_cells is not None:
+            cfg.short_cells = int(short_cells)
+        if max_overtakes is not None:
+            cfg.max_overtakes = int(max_overtakes)
         if use_graphs is not None:
             cfg.use_graphs = 1 if use_graphs else 0
         if max_slots is not None:

@@ -18,7 +18,9 @@ utilisation, 66.9M cells in all), prepared on the host first.
     bit-identical to its served result.
   * latency: the whole stream again with Poisson arrivals at --load of the
     served capacity (served p50 / p99), and a FIFO replay of the plain
-    service times at the same fraction of plain capacity.
+    service times at the same fraction of plain capacity; then the plain
+    leg's requests through a server with --latency-slots resident slots, at
+    --load of its capacity on them.
   * the 1M-cell mixed-height design end to end through lg_run_design.
   * cpu_baseline: the oracle on every 400th request of the stream.
 Every served result must be strictly legal, else the bench exits with an
@@ -122,8 +124,20 @@ def main():
     ap.add_argument("--load", type=float, default=0.7,
                     help="served_stream: Poisson arrival rate as a fraction of the measured capacity")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--short-cells", type=int, default=None,
+                    help="served_stream: requests with fewer cells iterate in the server's short lane "
+                         "(0: one lane; default: the server's)")
+    ap.add_argument("--max-overtakes", type=int, default=None,
+                    help="served_stream: smallest-first admission with this bound on how many later "
+                         "requests may pass a queued one (0: arrival order; default: the server's)")
+    ap.add_argument("--latency-slots", type=int, default=16,
+                    help="served_stream: resident slots of the latency-oriented server run on the plain "
+                         "leg's requests (0: skip)")
     ap.add_argument("--served-only", action="store_true",
                     help="served_stream: only the timed served run (for profiling); prints its figures")
+    ap.add_argument("--latency-only", action="store_true",
+                    help="served_stream: the timed served run and the Poisson-arrival latency run; "
+                         "prints their figures (no plain leg)")
     ap.add_argument("--dump", default=None, help="served_stream: write per-request sizes, plain service "
                                                   "times and iterations to this JSON file")
     ap.add_argument("--graphs", action="store_true",
@@ -476,9 +490,13 @@ def bench_served(args):
           f"synth + host import/preplace {t_prep:.1f}s; {K} steps of {R / K:.0f} requests",
           file=sys.stderr, flush=True)
 
-    def served_run(idx, arrivals=None, prof=False, progress=True):
-        with LegalizationServer(max_slots=args.slots, iters_per_sync=args.iters_per_sync,
-                                use_graphs=args.graphs) as srv:
+    srv_cfg = {}
+
+    def served_run(idx, arrivals=None, prof=False, progress=True, slots=None):
+        with LegalizationServer(max_slots=slots or args.slots, iters_per_sync=args.iters_per_sync,
+                                use_graphs=args.graphs, short_cells=args.short_cells,
+                                max_overtakes=args.max_overtakes) as srv:
+            srv_cfg.update(short_cells=srv.config.short_cells, max_overtakes=srv.config.max_overtakes)
             if prof:
                 srv.profile(True)
             tk = []
@@ -526,6 +544,34 @@ def bench_served(args):
                           "algorithmic_GBs": prof0["bytes"] / (prof0["ms"] / 1e3) / 1e9}), flush=True)
         return
 
+    def served_latency():
+        """The whole stream again, arriving as a Poisson process at `load` of
+        the measured capacity: (rate, the unit-mean gaps, latencies, latency
+        percentiles per size class, queueing percentiles)."""
+        rng = np.random.default_rng(7)
+        gaps = rng.exponential(1.0, R)
+        lam = args.load * R / t_served
+        arr = np.cumsum(gaps) / lam
+        arr -= arr[0]
+        rp, _, _, _ = served_run(list(range(R)), arrivals=arr)
+        lat = [r["latency_s"] for r in rp]
+        by_size = {}
+        for lo, hi in ((0, 5000), (5000, 20000), (20000, 80000), (80000, 10**9)):
+            xs = [lat[i] for i in range(R) if lo <= cells[i] < hi]
+            if xs:
+                by_size[f"{lo}-{hi if hi < 10**9 else 'max'}"] = percentiles(xs)
+        return lam, gaps, lat, by_size, percentiles([r["queued_s"] for r in rp])
+
+    if args.latency_only:
+        lam_s, _, lat_served, by_size, queued = served_latency()
+        print(json.dumps({"workload": f"served_stream: {R} requests, served capacity and latency",
+                          "short_cells": srv_cfg["short_cells"], "max_overtakes": srv_cfg["max_overtakes"],
+                          "served_s": t_served, "served_cells_per_s": served_rate,
+                          "load": args.load, "arrival_rate_per_s": lam_s,
+                          "latency_ms": percentiles(lat_served), "by_cells": by_size,
+                          "queued_ms": queued}), flush=True)
+        return
+
     # -- plain: every plain_stride-th request alone (lg_create .. lg_copy_out)
     def plain_one(g, c, p):
         dev = lg.DeviceLegalizer(g, c, p)
@@ -564,13 +610,7 @@ def bench_served(args):
         sys.exit(1)
 
     # -- latency under Poisson arrivals at `load` of each system's capacity
-    rng = np.random.default_rng(7)
-    gaps = rng.exponential(1.0, R)
-    lam_s = args.load * R / t_served
-    arr_s = np.cumsum(gaps) / lam_s
-    arr_s -= arr_s[0]
-    rp, el_p, ctr_p, _ = served_run(list(range(R)), arrivals=arr_s)
-    lat_served = [r["latency_s"] for r in rp]
+    lam_s, gaps, lat_served, by_size, queued = served_latency()
     lam_p = args.load * len(pidx) / t_plain
     gp = gaps[:len(pidx)]
     arr_p = np.cumsum(gp) / lam_p
@@ -578,12 +618,28 @@ def bench_served(args):
     lat_plain = fifo_replay(arr_p, service)
     print(f"# poisson: served p99 {np.percentile(lat_served, 99) * 1e3:.1f} ms, plain p99 "
           f"{np.percentile(lat_plain, 99) * 1e3:.1f} ms", file=sys.stderr, flush=True)
-    by_size = {}
-    for lo, hi in ((0, 5000), (5000, 20000), (20000, 80000), (80000, 10**9)):
-        m = [(cells[i] >= lo and cells[i] < hi) for i in range(R)]
-        xs = [lat_served[i] for i in range(R) if m[i]]
-        if xs:
-            by_size[f"{lo}-{hi if hi < 10**9 else 'max'}"] = percentiles(xs)
+
+    # -- the plain leg's requests through a server with fewer resident slots
+    # (latency-oriented): its capacity on those requests, then the same
+    # Poisson arrival sequence as the plain replay at `load` of that capacity
+    lat_cfg = None
+    if args.latency_slots > 0:
+        _, t_sub, _, _ = served_run(pidx, slots=args.latency_slots, progress=False)
+        lam_l = args.load * len(pidx) / t_sub
+        arr_l = np.cumsum(gp) / lam_l
+        arr_l -= arr_l[0]
+        rl, _, _, _ = served_run(pidx, arrivals=arr_l, slots=args.latency_slots, progress=False)
+        lat_l = [r["latency_s"] for r in rl]
+        lat_cfg = {"slots": args.latency_slots, "requests": len(pidx),
+                   "capacity_cells_per_s": plain_cells / t_sub,
+                   "capacity_vs_plain": plain_cells / t_sub / plain_rate,
+                   "arrival_rate_per_s": lam_l, "served": percentiles(lat_l),
+                   "plain": percentiles(lat_plain),
+                   "note": "the plain leg's requests; served with max_slots = slots (the other server "
+                           "settings as above) at `load` of its capacity on them, plain = the FIFO replay "
+                           "at `load` of plain capacity, both under the same unit Poisson sequence"}
+        print(f"# latency config ({args.latency_slots} slots): capacity {plain_cells / t_sub:.0f} cells/s, "
+              f"p99 {np.percentile(lat_l, 99) * 1e3:.1f} ms", file=sys.stderr, flush=True)
 
     e2e = None
     if not args.no_e2e:
@@ -625,14 +681,17 @@ def bench_served(args):
                 "load": args.load,
                 "served": percentiles(lat_served),
                 "served_by_cells": by_size,
+                "served_queued": queued,
                 "served_arrival_rate_per_s": lam_s,
                 "plain": percentiles(lat_plain),
                 "plain_arrival_rate_per_s": lam_p,
                 "plain_note": "FIFO replay of the measured per-request plain service times under Poisson "
                               "arrivals at the same fraction of plain capacity",
                 "served_saturated": percentiles([r["latency_s"] for r in res]),
+                "latency_config": lat_cfg,
             },
             "server": {"slots": args.slots, "iters_per_sync": args.iters_per_sync,
+                       "short_cells": srv_cfg["short_cells"], "max_overtakes": srv_cfg["max_overtakes"],
                        "peak_slots": ctr0["peak_slots"], "batches": ctr0["batches"],
                        "batched_iterations": ctr0["iterations"],
                        "pool_reserved_bytes": ctr0["pool_reserved_bytes"],

@@ -400,7 +400,18 @@ typedef struct LgSrvConfig {
                                  one HIP graph launch, recaptured when the set of
                                  resident requests changes; 0 (default): kernel
                                  launches.  Results are the same either way.     */
-  int32_t reserved;
+  int32_t short_cells;        /* > 0: requests with fewer cells iterate in a second
+                                 lane with its own streams and batch cadence, so
+                                 they are not paced by the long requests' launches;
+                                 max_slots bounds the resident requests of both
+                                 lanes together.  0: one lane.                    */
+  int32_t max_overtakes;      /* admission order.  0: arrival order.  > 0: the
+                                 smallest queued request that fits goes first, but
+                                 no request is passed over by more than this many
+                                 later requests: once the oldest queued request has
+                                 been, requests are admitted in arrival order until
+                                 it is in.  Results do not depend on either field. */
+  int32_t reserved;           /* 0 */
 } LgSrvConfig;
 
 typedef struct LgSrvResult {
@@ -417,8 +428,8 @@ typedef struct LgSrvResult {
 
 typedef struct LgSrvCounters {
   int64_t submitted, completed, failed;
-  int64_t batches;            /* host checks of the batched loop                    */
-  int64_t iterations;         /* batched ADMM iterations launched                   */
+  int64_t batches;            /* batches ended (host checks), summed over lanes     */
+  int64_t iterations;         /* batched ADMM iterations launched, over lanes       */
   int32_t peak_slots;         /* most requests resident at once                     */
   int32_t active, queued;
   int64_t pool_reserved_bytes, pool_peak_bytes;
@@ -427,7 +438,11 @@ typedef struct LgSrvCounters {
 
 typedef struct LgServer LgServer;
 
-/* max_slots 64, iters_per_sync 16, budget 200 GB, chunks of 256 MiB, no graphs. */
+/* max_slots 64, iters_per_sync 16, budget 200 GB, chunks of 256 MiB, no graphs,
+ * two lanes split at 20000 cells, smallest first with max_overtakes 64.
+ * max_slots trades capacity for tail latency: each resident request advances
+ * one iteration per batched iteration of its lane, whose duration grows with
+ * the lane's resident work, while the capacity grows with the residency. */
 void lg_srv_default_config(LgSrvConfig* c);
 /* Starts the server thread on the calling thread's current device. */
 int lg_srv_create(const LgSrvConfig* c, LgServer** out);

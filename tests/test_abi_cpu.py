@@ -69,6 +69,25 @@ def test_default_params_and_error_text(library):
     assert isinstance(L.lg_last_error(), bytes)
 
 
+def test_server_config_mirror_matches_the_library(library):
+    """lg_srv_default_config fills the Python mirror of LgSrvConfig field for
+    field, and lg_srv_create refuses a bad configuration before any device
+    call."""
+    from admm_legalization_amd import server
+    c = server.default_config()
+    assert C.sizeof(server.LgSrvConfig) == 40
+    assert (c.max_slots, c.iters_per_sync, c.mem_budget_bytes, c.pool_chunk_bytes, c.use_graphs) == (
+        64, 16, 200 * 10**9, 256 << 20, 0)
+    assert c.short_cells >= 0 and c.max_overtakes >= 0 and c.reserved == 0
+    L = server._bind()
+    h = C.c_void_p()
+    for field, bad in (("short_cells", -1), ("max_overtakes", -5), ("max_slots", 0)):
+        b = server.default_config()
+        setattr(b, field, bad)
+        assert L.lg_srv_create(C.byref(b), C.byref(h)) == 1  # LG_ERR_ARG
+        assert b"configuration" in L.lg_last_error()
+
+
 def test_null_arguments_are_rejected_without_a_device(library):
     L = lg.lib()
     assert L.lg_create(None, None, None, None) != 0

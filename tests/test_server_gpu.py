@@ -56,13 +56,19 @@ def cases(gpu):
     return out
 
 
-@pytest.mark.parametrize("slots,iters,graphs", [(64, 16, True), (3, 8, True), (1, 5, True), (64, 16, False),
-                                                 (3, 8, False)])
-def test_batched_results_equal_solo(cases, slots, iters, graphs):
+@pytest.mark.parametrize("slots,iters,graphs,short_cells,overtakes",
+                         [(64, 16, True, 0, 0), (3, 8, True, 0, 0), (1, 5, True, 0, 0), (64, 16, False, 0, 0),
+                          (3, 8, False, 0, 0), (64, 16, True, 3000, 0), (3, 8, False, 3000, 2),
+                          (1, 5, False, 0, 2), (8, 4, True, 2000, 3)])
+def test_batched_results_equal_solo(cases, slots, iters, graphs, short_cells, overtakes):
     """Iterations as graph launches (recaptured whenever the resident set
-    changes) or as kernel launches: the same results."""
+    changes) or as kernel launches, in one lane or two, admitted in arrival
+    order or smallest first: the same results."""
+    print(f"[case] slots={slots} iters={iters} graphs={graphs} short_cells={short_cells} "
+          f"overtakes={overtakes}", flush=True)
     with LegalizationServer(max_slots=slots, iters_per_sync=iters, mem_budget_bytes=4 << 30,
-                            pool_chunk_bytes=64 << 20, use_graphs=graphs) as srv:
+                            pool_chunk_bytes=64 << 20, use_graphs=graphs, short_cells=short_cells,
+                            max_overtakes=overtakes) as srv:
         tickets = [srv.submit(g, c, p) for _, g, c, p, _ in cases]
         res = [srv.wait(t) for t in tickets]
         ctr = srv.counters()
@@ -72,8 +78,80 @@ def test_batched_results_equal_solo(cases, slots, iters, graphs):
     assert ctr["completed"] == len(cases) and ctr["failed"] == 0
     assert (ctr["graph_captures"] > 0) == graphs
     assert 1 <= ctr["peak_slots"] <= min(slots, len(cases))
-    if slots >= len(cases):
+    if slots >= len(cases) and short_cells == 0:
         assert ctr["peak_slots"] == len(cases)  # all resident together
+
+
+def _admit_order(res, t_sub):
+    """Tickets in the order the server admitted them (submit time on the
+    caller's clock plus the request's queueing time)."""
+    return sorted(res, key=lambda t: t_sub[t] + res[t]["queued_s"])
+
+
+def test_smallest_first_admission_respects_the_overtake_bound(cases):
+    """One slot, a long request resident, then a 9000-cell request and six
+    small ones queued behind it: with max_overtakes = 3 the three smallest go
+    first, then the 9000-cell request (passed over three times), then the
+    rest by the same rule; no request is passed by more than three later
+    ones, and every result is its solo result."""
+    long_g, long_c, long_p = lg.device_inputs(synth.plain_20k(0))
+    big = [x for x in cases if x[0].startswith("n9000")][0]
+    small = [(n, u, s) for n, u, s in ((1500, 0.8, 61), (600, 0.7, 62), (1200, 0.8, 63), (800, 0.7, 64),
+                                       (1000, 0.8, 65), (700, 0.7, 66))]
+    small_in = [(f"n{n}_s{s}",) + _inputs(n, u, s, 0) for n, u, s in small]
+    with LegalizationServer(max_slots=1, iters_per_sync=8, max_overtakes=3) as srv:
+        t0 = srv.submit(long_g, long_c, long_p)
+        while srv.poll(t0) == 0:  # QUEUED: wait until it holds the slot
+            time.sleep(0.001)
+        order_in, t_sub = [], {}
+        for tag, g, c, p in [big[:4]] + small_in:
+            t_sub_now = time.perf_counter()
+            t = srv.submit(g, c, p)
+            t_sub[t] = t_sub_now
+            order_in.append((t, tag))
+        assert srv.poll(t0) == 1, "the long request finished before the queue was built"
+        res = {t: srv.wait(t) for t, _ in order_in}
+        srv.wait(t0)
+    admitted = _admit_order(res, t_sub)
+    pos_in = {t: i for i, (t, _) in enumerate(order_in)}
+    size = {t: len(res[t]["lx"]) for t in res}
+    # the rule, step by step: smallest first unless the oldest queued request
+    # has been passed over max_overtakes times
+    queue, passed, want = [t for t, _ in order_in], {t: 0 for t, _ in order_in}, []
+    while queue:
+        pick = 0 if passed[queue[0]] >= 3 else min(range(len(queue)), key=lambda i: (size[queue[i]], i))
+        for t in queue[:pick]:
+            passed[t] += 1
+        want.append(queue.pop(pick))
+    assert admitted == want, ([size[t] for t in admitted], [size[t] for t in want])
+    for t in admitted:
+        later_first = sum(1 for u in admitted[:admitted.index(t)] if pos_in[u] > pos_in[t])
+        assert later_first <= 3, (size[t], later_first)
+    # the 600-cell request first, the 9000-cell one fourth
+    assert admitted[0] == order_in[2][0] and admitted.index(order_in[0][0]) == 3
+    _same(res[order_in[0][0]], big[4], big[0])
+    for (t, tag), (_, g, c, p) in zip(order_in[1:], small_in):
+        _same(res[t], _solo(g, c, p), tag)
+
+
+def test_two_lanes_equal_solo(gpu):
+    """Two lanes: small requests submitted while a 20k-cell request iterates
+    in the long lane run beside it in the short lane and finish first, each
+    with its solo result."""
+    long_g, long_c, long_p = lg.device_inputs(synth.plain_20k(0))
+    want_long = _solo(long_g, long_c, long_p)
+    small_in = [_inputs(n, 0.8, 70 + i, 0) for i, n in enumerate((900, 1300, 700, 1100))]
+    with LegalizationServer(max_slots=8, iters_per_sync=8, short_cells=5000) as srv:
+        tl = srv.submit(long_g, long_c, long_p)
+        while srv.poll(tl) == 0:
+            time.sleep(0.001)
+        ts = [srv.submit(g, c, p) for g, c, p in small_in]
+        res_s = [srv.wait(t) for t in ts]
+        assert srv.poll(tl) == 1, "the long request finished before the short ones"
+        res_l = srv.wait(tl)
+    _same(res_l, want_long, "long")
+    for (g, c, p), got in zip(small_in, res_s):
+        _same(got, _solo(g, c, p), "short")
 
 
 def test_staggered_arrivals_equal_solo(cases):
