@@ -123,12 +123,17 @@ struct LgServer {
   LgSrvCounters ctr{};
   std::string fatal;  // a device error of the loop (every later request fails)
   std::thread worker;
-  // lg_srv_profile: summed partition-launch time, bracketed iterations and
-  // the algorithmic bytes (12 per pixel + 24 per cell) of the iterations
-  // each slot executed in them
+  // lg_srv_profile: summed partition-launch time, bracketed iterations, the
+  // algorithmic bytes (12 per pixel + 24 per cell) of the iterations each
+  // slot executed in them, and every launch interval in milliseconds from
+  // prof_ref (recorded on an idle stream when profiling starts), for the
+  // time during which any lane's partition launches ran
   int prof = 0;  // set by lg_srv_profile (under mu)
   double prof_ms = 0, prof_bytes = 0;
   int64_t prof_launches = 0;
+  hipStream_t prof_stream = nullptr;
+  hipEvent_t prof_ref = nullptr;
+  std::vector<std::pair<double, double>> prof_iv;
 };
 
 namespace {
@@ -466,9 +471,16 @@ int srv_lane_finish(LgServer* S, SrvLane& L) {
     S->ctr.batches++;
     S->ctr.iterations += iters;
     if (L.prof_batch) {
+      // intervals from the batch's first event (fine) placed on the common
+      // time line by that event's offset from prof_ref (coarse)
+      float base = 0;
+      const bool on_line = hipEventElapsedTime(&base, S->prof_ref, L.ev[0]) == hipSuccess;
       for (int it = 0; it < iters; it++) {
-        float ms = 0;
+        float ms = 0, t0 = 0, t1 = 0;
         if (hipEventElapsedTime(&ms, L.ev[2 * it], L.ev[2 * it + 1]) == hipSuccess) S->prof_ms += ms;
+        if (on_line && hipEventElapsedTime(&t0, L.ev[0], L.ev[2 * it]) == hipSuccess &&
+            hipEventElapsedTime(&t1, L.ev[0], L.ev[2 * it + 1]) == hipSuccess)
+          S->prof_iv.emplace_back((double)base + t0, (double)base + t1);
       }
       S->prof_launches += iters;
       for (int sl : L.active) {
@@ -586,6 +598,8 @@ void srv_free(LgServer* S) {
   if (S->h_ctl) (void)hipHostFree(S->h_ctl);
   if (S->h_tab) (void)hipHostFree(S->h_tab);
   if (S->cap) (void)hipStreamDestroy(S->cap);
+  if (S->prof_ref) (void)hipEventDestroy(S->prof_ref);
+  if (S->prof_stream) (void)hipStreamDestroy(S->prof_stream);
   delete S;
 }
 
@@ -751,24 +765,47 @@ int lg_srv_counters(LgServer* S, LgSrvCounters* out) {
 int lg_srv_profile(LgServer* S, int enable) {
   if (!S) return fail(LG_ERR_ARG, "null server");
   std::lock_guard<std::mutex> lk(S->mu);
-  if (enable)
+  if (enable) {
     for (SrvLane& L : S->lanes)
       if (L.ev.empty()) {
         L.ev.resize(2 * (size_t)S->cfg.iters_per_sync);
         for (auto& e : L.ev) HIP_TRY(hipEventCreate(&e));
       }
+    if (!S->prof_stream) HIP_TRY(hipStreamCreateWithFlags(&S->prof_stream, hipStreamNonBlocking));
+    if (!S->prof_ref) HIP_TRY(hipEventCreate(&S->prof_ref));
+    HIP_TRY(hipEventRecord(S->prof_ref, S->prof_stream));
+  }
   S->prof = enable ? 1 : 0;
   S->prof_ms = S->prof_bytes = 0;
   S->prof_launches = 0;
+  S->prof_iv.clear();
   return LG_OK;
 }
 
-int lg_srv_profile_read(LgServer* S, double* ms, int64_t* iterations, double* bytes) {
+int lg_srv_profile_read(LgServer* S, double* ms, int64_t* iterations, double* bytes, double* ms_union) {
   if (!S) return fail(LG_ERR_ARG, "null server");
   std::lock_guard<std::mutex> lk(S->mu);
   if (ms) *ms = S->prof_ms;
   if (iterations) *iterations = S->prof_launches;
   if (bytes) *bytes = S->prof_bytes;
+  if (ms_union) {
+    std::vector<std::pair<double, double>> iv = S->prof_iv;
+    std::sort(iv.begin(), iv.end());
+    double u = 0, lo = 0, hi = 0;
+    bool open = false;
+    for (const auto& x : iv) {
+      if (open && x.first <= hi) {
+        hi = std::max(hi, x.second);
+        continue;
+      }
+      if (open) u += hi - lo;
+      lo = x.first;
+      hi = x.second;
+      open = true;
+    }
+    if (open) u += hi - lo;
+    *ms_union = u;
+  }
   return LG_OK;
 }
 

@@ -55,7 +55,7 @@ def _bind():
         L.lg_srv_counters.argtypes = [C.c_void_p, C.POINTER(LgSrvCounters)]
         L.lg_srv_profile.argtypes = [C.c_void_p, C.c_int]
         L.lg_srv_profile_read.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64),
-                                          C.POINTER(C.c_double)]
+                                          C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.lg_srv_destroy.argtypes = [C.c_void_p]
         L._srv_bound = True
     return L
@@ -134,9 +134,11 @@ class LegalizationServer:
         _check(_bind().lg_srv_profile(self.h, int(bool(enable))))
 
     def profile_read(self) -> dict:
-        ms, n, b = C.c_double(), C.c_int64(), C.c_double()
-        _check(_bind().lg_srv_profile_read(self.h, C.byref(ms), C.byref(n), C.byref(b)))
-        return dict(ms=ms.value, iterations=n.value, bytes=b.value)
+        """ms: summed partition-launch time; ms_union: the time during which
+        any lane's partition launches ran."""
+        ms, n, b, u = C.c_double(), C.c_int64(), C.c_double(), C.c_double()
+        _check(_bind().lg_srv_profile_read(self.h, C.byref(ms), C.byref(n), C.byref(b), C.byref(u)))
+        return dict(ms=ms.value, iterations=n.value, bytes=b.value, ms_union=u.value)
 
     def close(self):
         if self.h:

@@ -458,13 +458,16 @@ int lg_srv_poll(LgServer* s, int64_t ticket, int32_t* state);
 int lg_srv_wait(LgServer* s, int64_t ticket, LgSrvResult* r);
 int lg_srv_counters(LgServer* s, LgSrvCounters* out);
 /* Instrumentation: with enable != 0, every later batched iteration brackets
- * its partition-kernel launches (the primal step of all resident requests)
- * with HIP events on the server's stream.  lg_srv_profile_read returns the
- * summed milliseconds, the number of bracketed iterations and the
- * algorithmic bytes of the iterations the requests executed in them
- * (12 per pixel + 24 per cell and iteration) since the last enable. */
+ * its partition-kernel launches (the primal step of the lane's resident
+ * requests) with HIP events on the lane's stream.  lg_srv_profile_read
+ * returns, since the last enable: the summed milliseconds of those launches,
+ * the number of bracketed iterations (over lanes), the algorithmic bytes of
+ * the iterations the requests executed in them (12 per pixel + 24 per cell
+ * and iteration) and, if ms_union is not NULL, the milliseconds during which
+ * at least one lane's partition launches ran (the lanes' launches overlap,
+ * so this is at most the sum). */
 int lg_srv_profile(LgServer* s, int enable);
-int lg_srv_profile_read(LgServer* s, double* ms, int64_t* iterations, double* bytes);
+int lg_srv_profile_read(LgServer* s, double* ms, int64_t* iterations, double* bytes, double* ms_union);
 /* Stops the server: queued and running requests are dropped; threads
  * blocked in lg_srv_wait return LG_ERR_STATE (or their result, for a request
  * that finished first) and lg_srv_destroy returns only after every such
