@@ -609,7 +609,7 @@ extern "C" {
 
 void lg_srv_default_config(LgSrvConfig* c) {
   if (!c) return;
-  c->max_slots = 64;
+  c->max_slots = 128;
   c->iters_per_sync = 16;
   c->mem_budget_bytes = 200ll * 1000 * 1000 * 1000;
   c->pool_chunk_bytes = 256ll << 20;

@@ -54,7 +54,7 @@ WORKLOADS = {
     "served_stream": "stream of legalization requests (ECO windows below 20k cells, sub-designs with "
                      "3 fences from 20k) with cell counts log-uniform 500-200k, heights 86/10/3/1 % of "
                      "1/2/3/4 rows (even heights under rail parity), utilisation U(0.6, 0.9), "
-                     "seed 0; served = continuous batching in one server (64 slots, 200 GB "
+                     "seed 0; served = continuous batching in one server (128 slots, 200 GB "
                      "device-memory admission budget) vs plain = one request at a time",
     "mixed_1m": "1M mixed-height cells (86/10/3/1 % of 1/2/3/4-row cells, even heights under "
                 "rail parity), 1600 rows x 12000 sites, 85 % utilisation, 3 fences, 200 macros, seed 0",
@@ -119,7 +119,7 @@ def main():
                     help="served_stream: plain leg over every Nth request of the stream")
     ap.add_argument("--no-e2e", action="store_true",
                     help="served_stream: skip the 1M-cell end-to-end legalization")
-    ap.add_argument("--slots", type=int, default=64, help="served_stream: resident requests")
+    ap.add_argument("--slots", type=int, default=128, help="served_stream: resident requests")
     ap.add_argument("--iters-per-sync", type=int, default=16, help="served_stream: batch length")
     ap.add_argument("--load", type=float, default=0.7,
                     help="served_stream: Poisson arrival rate as a fraction of the measured capacity")

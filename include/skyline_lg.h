@@ -438,7 +438,7 @@ typedef struct LgSrvCounters {
 
 typedef struct LgServer LgServer;
 
-/* max_slots 64, iters_per_sync 16, budget 200 GB, chunks of 256 MiB, no graphs,
+/* max_slots 128, iters_per_sync 16, budget 200 GB, chunks of 256 MiB, no graphs,
  * two lanes split at 20000 cells, smallest first with max_overtakes 64.
  * max_slots trades capacity for tail latency: each resident request advances
  * one iteration per batched iteration of its lane, whose duration grows with

@@ -77,7 +77,7 @@ def test_server_config_mirror_matches_the_library(library):
     c = server.default_config()
     assert C.sizeof(server.LgSrvConfig) == 40
     assert (c.max_slots, c.iters_per_sync, c.mem_budget_bytes, c.pool_chunk_bytes, c.use_graphs) == (
-        64, 16, 200 * 10**9, 256 << 20, 0)
+        128, 16, 200 * 10**9, 256 << 20, 0)
     assert c.short_cells >= 0 and c.max_overtakes >= 0 and c.reserved == 0
     L = server._bind()
     h = C.c_void_p()
@@ -165,7 +165,7 @@ def test_server_rejects_bad_arguments_without_a_device(library):
     from admm_legalization_amd import server
     L = server._bind()
     c = server.default_config()
-    assert (c.max_slots, c.iters_per_sync) == (64, 16)
+    assert (c.max_slots, c.iters_per_sync) == (128, 16)
     assert c.mem_budget_bytes == 200 * 10 ** 9 and c.pool_chunk_bytes == 256 << 20
     h = C.c_void_p()
     assert L.lg_srv_create(None, C.byref(h)) == 1  # LG_ERR_ARG
