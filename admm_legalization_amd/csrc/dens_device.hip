@@ -321,6 +321,10 @@ __global__ __launch_bounds__(256) void k_gemm_bf(GemmBf g) {
   constexpr int CB = BN * BK / 8 / 256;
   constexpr int NP = SPLIT ? 2 : 1;        // operand parts: hi (+ lo)
   constexpr int STAGE = (BM + BN) * LD * NP;  // bf16 elements per LDS stage
+  // k-steps whose operands are in flight in registers: the small tiles (one
+  // workgroup per CU at 1024^2, little MFMA work per step) load 3 steps
+  // ahead, the large ones 1 or 2
+  constexpr int PD = BM * BN <= 64 * 128 ? 3 : (BM * BN <= 128 * 128 ? 2 : 1);
   static_assert(CA >= 1 && CB >= 1 && CA * 256 * 8 == BM * BK && CB * 256 * 8 == BN * BK,
                 "tile does not split into 16-byte chunks over 256 threads");
 
@@ -335,16 +339,18 @@ __global__ __launch_bounds__(256) void k_gemm_bf(GemmBf g) {
   const int l32 = lane & 31, lhi = lane >> 5;
 
   typedef int int4v __attribute__((ext_vector_type(4)));
-  int4v ra[NP][CA], rb[NP][CB];
-  auto gload = [&](int k0) {
+  // PD register sets; set u holds the k-steps congruent to u mod PD (every
+  // index below is a constant once the step loop is unrolled)
+  int4v ra[PD][NP][CA], rb[PD][NP][CB];
+  auto gload = [&](int4v (&xa)[NP][CA], int4v (&xb)[NP][CB], int k0) {
 #pragma unroll
     for (int i = 0; i < CA; i++) {
       const int f = tid + i * 256, row = f / CPR, kc = (f % CPR) * 8;
       const int gr = r0 + row, gk = k0 + kc;
       const bool ok = gr < g.R && gk < g.K;
       const size_t o = ok ? (size_t)gr * g.lda + gk : 0;
-      ra[0][i] = ok ? *reinterpret_cast<const int4v*>(g.Ah + o) : int4v{0, 0, 0, 0};
-      if (SPLIT) ra[NP - 1][i] = ok ? *reinterpret_cast<const int4v*>(g.Al + o) : int4v{0, 0, 0, 0};
+      xa[0][i] = ok ? *reinterpret_cast<const int4v*>(g.Ah + o) : int4v{0, 0, 0, 0};
+      if (SPLIT) xa[NP - 1][i] = ok ? *reinterpret_cast<const int4v*>(g.Al + o) : int4v{0, 0, 0, 0};
     }
 #pragma unroll
     for (int i = 0; i < CB; i++) {
@@ -352,12 +358,12 @@ __global__ __launch_bounds__(256) void k_gemm_bf(GemmBf g) {
       const int gr = c0 + row, gk = k0 + kc;
       const bool ok = gr < g.Cn && gk < g.K;
       const size_t o = ok ? (size_t)gr * g.ldb + gk : 0;
-      rb[0][i] = ok ? *reinterpret_cast<const int4v*>(g.Bh + o) : int4v{0, 0, 0, 0};
-      if (SPLIT) rb[NP - 1][i] = ok ? *reinterpret_cast<const int4v*>(g.Bl + o) : int4v{0, 0, 0, 0};
+      xb[0][i] = ok ? *reinterpret_cast<const int4v*>(g.Bh + o) : int4v{0, 0, 0, 0};
+      if (SPLIT) xb[NP - 1][i] = ok ? *reinterpret_cast<const int4v*>(g.Bl + o) : int4v{0, 0, 0, 0};
     }
   };
   // stage layout: A hi [BM][LD], B hi [BN][LD], then A lo, B lo
-  auto sstore = [&](int st) {
+  auto sstore = [&](int st, const int4v (&xa)[NP][CA], const int4v (&xb)[NP][CB]) {
     __bf16* base = lds + st * STAGE;
 #pragma unroll
     for (int p = 0; p < NP; p++) {
@@ -366,12 +372,12 @@ __global__ __launch_bounds__(256) void k_gemm_bf(GemmBf g) {
 #pragma unroll
       for (int i = 0; i < CA; i++) {
         const int f = tid + i * 256, row = f / CPR, kc = (f % CPR) * 8;
-        *reinterpret_cast<int4v*>(sa + row * LD + kc) = ra[p][i];
+        *reinterpret_cast<int4v*>(sa + row * LD + kc) = xa[p][i];
       }
 #pragma unroll
       for (int i = 0; i < CB; i++) {
         const int f = tid + i * 256, row = f / CPR, kc = (f % CPR) * 8;
-        *reinterpret_cast<int4v*>(sb + row * LD + kc) = rb[p][i];
+        *reinterpret_cast<int4v*>(sb + row * LD + kc) = xb[p][i];
       }
     }
   };
@@ -385,12 +391,19 @@ __global__ __launch_bounds__(256) void k_gemm_bf(GemmBf g) {
       for (int r = 0; r < 16; r++) acc[i][j][r] = 0.f;
 
   const int nk = (g.K + BK - 1) / BK;
-  gload(0);
-  sstore(0);
+#pragma unroll
+  for (int u = 0; u < PD; u++)
+    if (u < nk) gload(ra[u], rb[u], u * BK);
+  sstore(0, ra[0], rb[0]);
   __syncthreads();
-  for (int ks = 0; ks < nk; ks++) {
+  for (int ks0 = 0; ks0 < nk; ks0 += PD)
+#pragma unroll
+  for (int u = 0; u < PD; u++) {
+    const int ks = ks0 + u;
+    if (ks >= nk) break;
+    // set u's step ks is in LDS: the set takes step ks + PD
+    if (ks + PD < nk) gload(ra[u], rb[u], (ks + PD) * BK);
     const bool more = ks + 1 < nk;
-    if (more) gload((ks + 1) * BK);
     const __bf16* base = lds + (ks & 1) * STAGE;
     const __bf16* sah = base;
     const __bf16* sbh = base + BM * LD;
@@ -423,8 +436,8 @@ __global__ __launch_bounds__(256) void k_gemm_bf(GemmBf g) {
         }
     }
     // the other stage was last read in step ks - 1, which every wave left
-    // at that step's barrier
-    if (more) sstore((ks + 1) & 1);
+    // at that step's barrier; step ks + 1 is in set (u + 1) mod PD
+    if (more) sstore((ks + 1) & 1, ra[(u + 1) % PD], rb[(u + 1) % PD]);
     __syncthreads();
   }
 
