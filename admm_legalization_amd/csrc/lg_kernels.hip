@@ -542,6 +542,11 @@ __device__ __forceinline__ void scan_small(const Args& a, int K, bool skip_ok, b
 // listed partition's workgroup then starts in the first dispatch round
 // instead of behind thousands of workgroups that only find their partition
 // idle.  One workgroup of 1024 threads.
+// LARGE: the path for more than 8192 partitions is compiled in, with its
+// 135 KB of LDS; without it a scan workgroup takes little LDS and shares a
+// CU with the partition kernels' workgroups (callers pick it when every K is
+// at most 8192).
+template <bool LARGE>
 __device__ __forceinline__ void scan_body(const Args& a) {
   Ctl* ctl = a.ctl;
   if (ctl->done) return;
@@ -557,6 +562,10 @@ __device__ __forceinline__ void scan_body(const Args& a) {
   const unsigned long long lt = (1ull << lane) - 1ull;
   if (t < 2) s_n[t] = 0;
   __syncthreads();
+  if constexpr (!LARGE) {
+    scan_small(a, K, skip_ok, adaptive_now, cf, s_n, s_max);
+    return;
+  }
   if (K <= 8 * 1024) {
     scan_small(a, K, skip_ok, adaptive_now, cf, s_n, s_max);
     return;
@@ -617,7 +626,8 @@ __device__ __forceinline__ void scan_body(const Args& a) {
     scan_finish(a, K, m, K > 0 ? a.off[K - 1] + last : 0);
   }
 }
-__global__ __launch_bounds__(1024) void k_scan(Args a) { scan_body(a); }
+template <bool LARGE>
+__global__ __launch_bounds__(1024) void k_scan(Args a) { scan_body<LARGE>(a); }
 
 // Grouping for the final refine, which visits every partition under the
 // fold offset of the converged iteration.  A partition ordered by the last
@@ -3033,9 +3043,10 @@ __global__ __launch_bounds__(256) void kb_bucket(const Args* __restrict__ tab, c
   bucket_body(a, m_.y);
 }
 
+template <bool LARGE>
 __global__ __launch_bounds__(1024) void kb_scan(const Args* __restrict__ tab, const int2* __restrict__ map) {
   LG_BATCH_SLOT();
-  scan_body(a);
+  scan_body<LARGE>(a);
 }
 
 __global__ void kb_extract_congested(const Args* __restrict__ tab, const int2* __restrict__ map) {

@@ -85,6 +85,7 @@ struct SrvLane {
   // geometry of the current tables (top_slot: one past the highest slot)
   MapRange m256, m1, mk, mpix, mbk;
   int any_adaptive = 0, any_reg = 0, any_nonreg = 0, top_slot = 0;
+  int large_k = 0;  // a request with more than 8192 partitions (the scan's large path)
   size_t lds_reg = 0, lds_small = 0, lds_big = 0;
   int64_t iterations = 0;  // batched iterations launched in this lane
   // primal-step timing (lg_srv_profile): two events per batched iteration
@@ -155,7 +156,7 @@ int srv_grow_map(SrvLane& L, size_t need) {
 // Slot table and block maps of the lane's requests (lane between batches).
 int srv_build_tables(LgServer* S, SrvLane& L) {
   std::vector<int2> m[5];
-  L.any_adaptive = L.any_reg = L.any_nonreg = 0;
+  L.any_adaptive = L.any_reg = L.any_nonreg = L.large_k = 0;
   L.lds_reg = L.lds_small = L.lds_big = 0;
   L.top_slot = L.slot_lo;
   for (int sl : L.active) {
@@ -170,6 +171,7 @@ int srv_build_tables(LgServer* S, SrvLane& L) {
     for (int b = 0; b < gx * a.y_size; b++) m[3].push_back(make_int2(sl, b));
     for (int b = 0; b < (a.N + kBucketCells - 1) / kBucketCells; b++) m[4].push_back(make_int2(sl, b));
     L.any_adaptive |= a.use_adaptive ? 1 : 0;
+    L.large_k |= a.K > 8 * 1024 ? 1 : 0;
     const int reg = h->p.sorting_algorithm == LG_SORT_M_BITONIC || h->p.sorting_algorithm == LG_SORT_S_BITONIC;
     if (reg) {
       L.any_reg = 1;
@@ -210,7 +212,8 @@ int srv_launch_iteration(LgServer* S, SrvLane& L, int it, hipStream_t s, bool pa
   const int2* mp = L.d_map;
   auto at = [&](const MapRange& r) { return mp + r.off; };
   if (L.mbk.n > 0) hipLaunchKernelGGL(kb_bucket, dim3(L.mbk.n), dim3(256), 0, s, tab, at(L.mbk));
-  hipLaunchKernelGGL(kb_scan, dim3(L.m1.n), dim3(1024), 0, s, tab, at(L.m1));
+  if (L.large_k) hipLaunchKernelGGL(kb_scan<true>, dim3(L.m1.n), dim3(1024), 0, s, tab, at(L.m1));
+  else hipLaunchKernelGGL(kb_scan<false>, dim3(L.m1.n), dim3(1024), 0, s, tab, at(L.m1));
   // the dual pass on the side stream, beside the partition kernels
   if (pass) {
     HIP_TRY(hipEventRecord(L.ev_fork, s));
