@@ -70,5 +70,6 @@ constexpr int kMaxNpadLds = 4096;
 constexpr int kMaxStagnationWindow = 64;  // Ctl::ovf_window entries
 constexpr int kBucketCells = 1024;  // cells per workgroup of k_bucket
 constexpr int kNpadSmall = 1024;
+constexpr int kLargeGroups = 128;  // workgroups per slot of the batched large-class launch
 
 }  // namespace lgdev

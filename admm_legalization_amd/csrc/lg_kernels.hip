@@ -3072,6 +3072,26 @@ __global__ __launch_bounds__(NW * 64) void kb_partition(const Args* __restrict__
   partition_body<MODE, LDS_SORT, NW>(a, m_.y, npad_lo, npad_hi);
 }
 
+// The large class of the primal step, batched: min(K, kLargeGroups)
+// workgroups per slot (map {slot, g}), workgroup g taking the entries g,
+// g + G, g + 2G, ... of the slot's class-1 list, whose length the scan and
+// the four-wave launch fixed before this launch.  The list is short, so a
+// workgroup per partition of every slot would mostly exit at once; the
+// partitions of one step are independent, so the order changes nothing.
+template <int MODE, int LDS_SORT, int NW>
+__global__ __launch_bounds__(NW * 64) void kb_partition_large(const Args* __restrict__ tab,
+                                                              const int2* __restrict__ map, int npad_lo,
+                                                              int npad_hi) {
+  LG_BATCH_SLOT();
+  if (a.ctl->done) return;
+  const int G = min(a.K, kLargeGroups);
+  const int n = a.ctl->n_work[1];
+  for (int j = m_.y; j < n; j += G) {
+    partition_body<MODE, LDS_SORT, NW>(a, j, npad_lo, npad_hi);
+    __syncthreads();  // the LDS tile is staged anew for the next entry
+  }
+}
+
 // local block l of the solo launch's (ceil(x_size / 1024), y_size) grid
 __global__ __launch_bounds__(256) void kb_dual_pass(const Args* __restrict__ tab, const int2* __restrict__ map) {
   LG_BATCH_SLOT();

@@ -83,7 +83,7 @@ struct SrvLane {
   bool dirty = true;
   bool in_flight = false;
   // geometry of the current tables (top_slot: one past the highest slot)
-  MapRange m256, m1, mk, mpix, mbk;
+  MapRange m256, m1, mk, mpix, mbk, mlg;
   int any_adaptive = 0, any_reg = 0, any_nonreg = 0, top_slot = 0;
   int large_k = 0;  // a request with more than 8192 partitions (the scan's large path)
   size_t lds_reg = 0, lds_small = 0, lds_big = 0;
@@ -155,7 +155,7 @@ int srv_grow_map(SrvLane& L, size_t need) {
 
 // Slot table and block maps of the lane's requests (lane between batches).
 int srv_build_tables(LgServer* S, SrvLane& L) {
-  std::vector<int2> m[5];
+  std::vector<int2> m[6];
   L.any_adaptive = L.any_reg = L.any_nonreg = L.large_k = 0;
   L.lds_reg = L.lds_small = L.lds_big = 0;
   L.top_slot = L.slot_lo;
@@ -170,6 +170,7 @@ int srv_build_tables(LgServer* S, SrvLane& L) {
     const int gx = (a.x_size + 1023) / 1024;
     for (int b = 0; b < gx * a.y_size; b++) m[3].push_back(make_int2(sl, b));
     for (int b = 0; b < (a.N + kBucketCells - 1) / kBucketCells; b++) m[4].push_back(make_int2(sl, b));
+    for (int b = 0; b < std::min(a.K, kLargeGroups); b++) m[5].push_back(make_int2(sl, b));
     L.any_adaptive |= a.use_adaptive ? 1 : 0;
     L.large_k |= a.K > 8 * 1024 ? 1 : 0;
     const int reg = h->p.sorting_algorithm == LG_SORT_M_BITONIC || h->p.sorting_algorithm == LG_SORT_S_BITONIC;
@@ -183,8 +184,8 @@ int srv_build_tables(LgServer* S, SrvLane& L) {
     L.lds_big = std::max(L.lds_big, partition_lds_bytes(h, kMaxNpadLds, 1));
   }
   size_t total = 0;
-  MapRange* r[5] = {&L.m256, &L.m1, &L.mk, &L.mpix, &L.mbk};
-  for (int i = 0; i < 5; i++) {
+  MapRange* r[6] = {&L.m256, &L.m1, &L.mk, &L.mpix, &L.mbk, &L.mlg};
+  for (int i = 0; i < 6; i++) {
     r[i]->off = (int)total;
     r[i]->n = (int)m[i].size();
     total += m[i].size();
@@ -192,7 +193,7 @@ int srv_build_tables(LgServer* S, SrvLane& L) {
   if (total > (size_t)INT_MAX) return fail(LG_ERR_MEMORY, "request server: block maps too large");
   int st = srv_grow_map(L, total);
   if (st) return st;
-  for (int i = 0; i < 5; i++)
+  for (int i = 0; i < 6; i++)
     if (!m[i].empty()) std::memcpy(L.h_map + r[i]->off, m[i].data(), m[i].size() * sizeof(int2));
   // only the lane's own entries: the other lane's kernels read theirs meanwhile
   HIP_TRY(hipMemcpyAsync(S->d_tab + L.slot_lo, S->h_tab + L.slot_lo, (L.top_slot - L.slot_lo) * sizeof(Args),
@@ -238,9 +239,10 @@ int srv_launch_iteration(LgServer* S, SrvLane& L, int it, hipStream_t s, bool pa
   if (L.any_nonreg)
     hipLaunchKernelGGL((kb_partition<MODE_PRIMAL, 1, 1>), dim3(L.mk.n), dim3(kWave), L.lds_small, s, tab, at(L.mk),
                        0, kNpadSmall, 0);
-  // the large class also takes the partitions the four-wave launch deferred
-  hipLaunchKernelGGL((kb_partition<MODE_PRIMAL, 1, 1>), dim3(L.mk.n), dim3(kWave), L.lds_big, s, tab, at(L.mk), 1,
-                     kMaxNpad, -1);
+  // the large class also takes the partitions the four-wave launch deferred;
+  // min(K, kLargeGroups) workgroups per slot step through its list
+  hipLaunchKernelGGL((kb_partition_large<MODE_PRIMAL, 1, 1>), dim3(L.mlg.n), dim3(kWave), L.lds_big, s, tab,
+                     at(L.mlg), 1, kMaxNpad);
   if (e) HIP_TRY(hipEventRecord(e[1], s));
   if (pass) HIP_TRY(hipStreamWaitEvent(s, L.ev_join, 0));
   hipLaunchKernelGGL(kb_finish, dim3(L.m1.n), dim3(1), 0, s, tab, at(L.m1));

@@ -134,6 +134,30 @@ def test_smallest_first_admission_respects_the_overtake_bound(cases):
         _same(res[t], _solo(g, c, p), tag)
 
 
+def test_large_class_with_more_partitions_than_workgroups(gpu):
+    """Single-site cells at 80 % on 600 x 240 sites with 60 x 12 partitions:
+    about 580 cells in each of the 231 partitions, so every partition goes
+    to the large class -- more entries than the batched large-class launch
+    has workgroups per slot (128), each workgroup taking several in turn.
+    Results equal the solo path's, beside a small request."""
+    d = synth.make_design(115000, 600, 240, width_range=(1, 1), num_macros=0, seed=5, name="all_large")
+    p = lg.default_params()
+    p.max_admm_iter = 20
+    p.x_hint, p.y_hint = 60, 12
+    g, c, p = lg.device_inputs(d, p)
+    want = _solo(g, c, p)
+    sg, sc, sp = _inputs(800, 0.7, 21, 0)
+    want_small = _solo(sg, sc, sp)
+    with LegalizationServer(max_slots=4, iters_per_sync=8) as srv:
+        t = srv.submit(g, c, p)
+        ts = srv.submit(sg, sc, sp)
+        got = srv.wait(t)
+        got_small = srv.wait(ts)
+    _same(got, want, "all_large")
+    _same(got_small, want_small, "small")
+    assert want["stats"]["iterations"] == 20
+
+
 def test_two_lanes_equal_solo(gpu):
     """Two lanes: small requests submitted while a 20k-cell request iterates
     in the long lane run beside it in the short lane and finish first, each
