@@ -311,7 +311,7 @@ constexpr int bf_bk() {
 }
 
 template <bool SPLIT, int BM, int BN>
-__global__ __launch_bounds__(256) void k_gemm_bf(GemmBf g) {
+__device__ __forceinline__ void gemm_bf_body(const GemmBf& g, int bid) {
   constexpr int BK = bf_bk<SPLIT, BM, BN>();
   constexpr int CPR = BK / 8;              // 16-byte chunks per operand row of a k-step
   constexpr int WM = BM / 2, WN = BN / 2;
@@ -332,7 +332,7 @@ __global__ __launch_bounds__(256) void k_gemm_bf(GemmBf g) {
   __bf16* lds = reinterpret_cast<__bf16*>(smem);
 
   const int tiles_n = (g.Cn + BN - 1) / BN;
-  const int tm = blockIdx.x / tiles_n, tn = blockIdx.x - tm * tiles_n;
+  const int tm = bid / tiles_n, tn = bid - tm * tiles_n;
   const int r0 = tm * BM, c0 = tn * BN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 1, wc = wave & 1;
@@ -467,6 +467,19 @@ __global__ __launch_bounds__(256) void k_gemm_bf(GemmBf g) {
       }
 }
 
+template <bool SPLIT, int BM, int BN>
+__global__ __launch_bounds__(256) void k_gemm_bf(GemmBf g) {
+  gemm_bf_body<SPLIT, BM, BN>(g, blockIdx.x);
+}
+
+// Two independent GEMMs of one tile shape in one launch: workgroups
+// [0, nb0) compute g0's tiles, the rest g1's.
+template <bool SPLIT, int BM, int BN>
+__global__ __launch_bounds__(256) void k_gemm_bf2(GemmBf g0, GemmBf g1, int nb0) {
+  if ((int)blockIdx.x < nb0) gemm_bf_body<SPLIT, BM, BN>(g0, blockIdx.x);
+  else gemm_bf_body<SPLIT, BM, BN>(g1, blockIdx.x - nb0);
+}
+
 // FP32 -> bf16 hi (+ lo) of n values (the chain's input density)
 template <bool SPLIT>
 __global__ __launch_bounds__(256) void k_split_bf16(const float* __restrict__ x, __bf16* hi, __bf16* lo, size_t n) {
@@ -484,32 +497,40 @@ constexpr size_t gemm_bf_lds() {
 }
 
 template <bool SPLIT, int BM, int BN>
-void launch_gemm_bf_tile(const GemmBf& g, hipStream_t s) {
+void launch_gemm_bf_tile(const GemmBf& g, const GemmBf* g1, hipStream_t s) {
   const int grid = ((g.R + BM - 1) / BM) * ((g.Cn + BN - 1) / BN);
   constexpr size_t lds = gemm_bf_lds<SPLIT, BM, BN>();
-  hipLaunchKernelGGL((k_gemm_bf<SPLIT, BM, BN>), dim3(grid), dim3(256), lds, s, g);
+  if (g1) {
+    const int grid1 = ((g1->R + BM - 1) / BM) * ((g1->Cn + BN - 1) / BN);
+    hipLaunchKernelGGL((k_gemm_bf2<SPLIT, BM, BN>), dim3(grid + grid1), dim3(256), lds, s, g, *g1, grid);
+  } else {
+    hipLaunchKernelGGL((k_gemm_bf<SPLIT, BM, BN>), dim3(grid), dim3(256), lds, s, g);
+  }
 }
 
 template <bool SPLIT>
-int launch_gemm_bf_p(const GemmBf& g, hipStream_t s, int tile) {
+int launch_gemm_bf_p(const GemmBf& g, const GemmBf* g1, hipStream_t s, int tile) {
   switch (tile) {
-    case 1: launch_gemm_bf_tile<SPLIT, 64, 64>(g, s); break;
-    case 2: launch_gemm_bf_tile<SPLIT, 64, 128>(g, s); break;
-    case 3: launch_gemm_bf_tile<SPLIT, 128, 64>(g, s); break;
-    case 4: launch_gemm_bf_tile<SPLIT, 128, 128>(g, s); break;
-    default: launch_gemm_bf_tile<SPLIT, 256, 128>(g, s); break;
+    case 1: launch_gemm_bf_tile<SPLIT, 64, 64>(g, g1, s); break;
+    case 2: launch_gemm_bf_tile<SPLIT, 64, 128>(g, g1, s); break;
+    case 3: launch_gemm_bf_tile<SPLIT, 128, 64>(g, g1, s); break;
+    case 4: launch_gemm_bf_tile<SPLIT, 128, 128>(g, g1, s); break;
+    default: launch_gemm_bf_tile<SPLIT, 256, 128>(g, g1, s); break;
   }
   DENS_TRY(hipGetLastError());
   return DENS_OK;
 }
 
-int launch_gemm_bf(int prec, const GemmBf& g, hipStream_t s, int tile) {
+// g1 (optional): a second GEMM, independent of g, computed in the same launch
+int launch_gemm_bf(int prec, const GemmBf& g, hipStream_t s, int tile, const GemmBf* g1 = nullptr) {
   if (g.K % 8 || g.lda % 8 || g.ldb % 8) return fail(DENS_ERR_ARG, "bf16 GEMM operands must be multiples of 8 wide");
+  if (g1 && (g1->K % 8 || g1->lda % 8 || g1->ldb % 8))
+    return fail(DENS_ERR_ARG, "bf16 GEMM operands must be multiples of 8 wide");
   if (tile == 0) {
     const long tiles128 = (long)((g.R + 127) / 128) * ((g.Cn + 127) / 128);
     tile = tiles128 >= 512 ? 4 : 1;
   }
-  return prec == DENS_PREC_SPLIT_BF16 ? launch_gemm_bf_p<true>(g, s, tile) : launch_gemm_bf_p<false>(g, s, tile);
+  return prec == DENS_PREC_SPLIT_BF16 ? launch_gemm_bf_p<true>(g, g1, s, tile) : launch_gemm_bf_p<false>(g, g1, s, tile);
 }
 
 int launch_gemm(int prec, const Gemm& g, hipStream_t s, int tile) {
@@ -1080,13 +1101,12 @@ int poisson_chain_bf(PoissonHandle* h, const float* rho, float* psi, float* ex, 
   // [V; Vx] = [DNt; SNw] * A'^T
   g = GemmBf{h->A3h, h->A3l, Ph, Pl, nullptr, nullptr, Vh, Vl, N, N, M, 2 * N, 2 * N, M, N, 1.f, nullptr, nullptr};
   if ((st = launch_gemm_bf(prec, g, s, tile))) return st;
-  // [psi; Ey] = [DMt; SMw] * V^T
+  // [psi; Ey] = [DMt; SMw] * V^T and Ex = DMt * Vx^T: both read A4 and
+  // [V; Vx] only, so they share one launch
   g = GemmBf{h->A4h, h->A4l, Vh, Vl, psi, ey, nullptr, nullptr, M, M, N, M, 2 * M, N, M, 1.f, nullptr, nullptr};
-  if ((st = launch_gemm_bf(prec, g, s, tile))) return st;
-  // Ex = DMt * Vx^T
-  g = GemmBf{h->A4h, h->A4l, Vh + B, Vl + B, ex, nullptr, nullptr, nullptr, M, M, N, M, M, N, M, 1.f, nullptr,
-             nullptr};
-  return launch_gemm_bf(prec, g, s, tile);
+  const GemmBf gx{h->A4h, h->A4l, Vh + B, Vl + B, ex, nullptr, nullptr, nullptr, M, M, N, M, M, N, M, 1.f, nullptr,
+                  nullptr};
+  return launch_gemm_bf(prec, g, s, tile, &gx);
 }
 
 // Every element of U, A' and [V; Vx] is written before it is read, so the
