@@ -97,3 +97,49 @@ def test_precondition_matches_oracle(gpu):
     wx, wy = W.precondition(np.float32(0.7), 1.0, *arrs)
     np.testing.assert_allclose(gx.cpu().numpy(), wx, rtol=1e-6, atol=1e-7)
     np.testing.assert_allclose(gy.cpu().numpy(), wy, rtol=1e-6, atol=1e-7)
+
+
+def test_grad_degree_edges_match_oracle(gpu):
+    """Nets of 0 and 1 pins, of 63 / 64 / 65 pins (the split between the
+    chunk kernel and the wave-per-net kernel), runs of small nets long
+    enough to cut chunks at both the pin and the net limit, and a net above
+    1000 pins, in one netlist: the gradient matches the FP64 oracle and
+    repeats bit for bit."""
+    import torch
+    from admm_legalization_amd.wirelength import WireLengthGradient
+    rng = np.random.default_rng(5)
+    n_cells = 3000
+    pl = synth.density_placement(n_cells, seed=5)
+    deg = np.concatenate([[0, 1, 2, 0, 1], np.full(600, 2), [63, 64, 65, 0, 1, 64, 65], rng.integers(0, 9, 700),
+                          [1500], np.full(300, 1), rng.integers(2, 70, 200)]).astype(np.int64)
+    net_start = np.zeros(len(deg) + 1, np.int64)
+    np.cumsum(deg, out=net_start[1:])
+    n_pins = int(net_start[-1])
+    pin2cell = rng.integers(0, n_cells, n_pins).astype(np.int32)
+    pin2cell[rng.random(n_pins) < 0.01] = -1
+    c = np.maximum(pin2cell, 0)
+    off_x = ((rng.random(n_pins) - 0.5) * pl["dx"][c]).astype(np.float32)
+    off_y = ((rng.random(n_pins) - 0.5) * pl["dy"][c]).astype(np.float32)
+    off_x[pin2cell < 0] = 0.0
+    off_y[pin2cell < 0] = 0.0
+    die = pl["die"]
+    pin_x = (die[0] + rng.random(n_pins) * (die[2] - die[0])).astype(np.float32)
+    pin_y = (die[1] + rng.random(n_pins) * (die[3] - die[1])).astype(np.float32)
+    w = rng.uniform(0.5, 1.5, len(deg)).astype(np.float32)
+    ns = net_start.astype(np.int32)
+    print(f"[case] nets={len(deg)} pins={n_pins}", flush=True)
+    wl = WireLengthGradient(n_cells, ns, pin2cell, off_x, off_y, pin_x, pin_y, w)
+    wl.setGammaInv(0.5)
+    cx = torch.tensor(pl["cx"], device="cuda")
+    cy = torch.tensor(pl["cy"], device="cuda")
+    gx, gy = torch.empty_like(cx), torch.empty_like(cx)
+    wl.computeGradAt(cx, cy, gx, gy)
+    got = (gx.cpu().numpy(), gy.cpu().numpy())
+    px, py = W.update_pins(pin_x, pin_y, pin2cell, off_x, off_y, pl["cx"], pl["cy"])
+    want = W.wa_grad(ns, px, py, pin2cell, 0.5, w, n_cells)
+    for g, w_ in zip(got, want):
+        assert _rel(g, w_) <= 2e-5
+        np.testing.assert_allclose(g, w_, atol=1e-4 * np.abs(w_).max())
+    gx2, gy2 = torch.empty_like(cx), torch.empty_like(cx)
+    wl.computeGradAt(cx, cy, gx2, gy2)
+    assert torch.equal(gx, gx2) and torch.equal(gy, gy2)
