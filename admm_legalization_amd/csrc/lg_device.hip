@@ -355,8 +355,14 @@ int launch_partition(LgHandle* h, hipStream_t s) {
   size_t lds1 = partition_lds_bytes(h, kMaxNpadLds, 1);
   // the primal step's large class also takes the partitions the four-wave
   // launch deferred (its list, every size)
-  const int lo1 = MODE == MODE_PRIMAL ? 1 : kNpadSmall + 1;
-  hipLaunchKernelGGL((k_partition<MODE, 1, 1>), dim3(h->a.K), dim3(kWave), lds1, s, h->a, lo1, kMaxNpad);
+  if constexpr (MODE == MODE_PRIMAL) {
+    // min(K, 1024) workgroups step through the list, which is short
+    hipLaunchKernelGGL((k_partition_large<MODE, 1, 1>), dim3(std::min(h->a.K, 1024)), dim3(kWave), lds1, s, h->a, 1,
+                       kMaxNpad);
+  } else {
+    hipLaunchKernelGGL((k_partition<MODE, 1, 1>), dim3(h->a.K), dim3(kWave), lds1, s, h->a, kNpadSmall + 1,
+                       kMaxNpad);
+  }
   if (prof) {
     HIP_TRY(hipEventRecord(e[2], s));
     h->ev_used += 3;

@@ -3072,6 +3072,18 @@ __global__ __launch_bounds__(NW * 64) void kb_partition(const Args* __restrict__
   partition_body<MODE, LDS_SORT, NW>(a, m_.y, npad_lo, npad_hi);
 }
 
+// The large class of the primal step, solo: gridDim.x workgroups, workgroup
+// g taking the entries g, g + gridDim.x, ... of the class-1 list.
+template <int MODE, int LDS_SORT, int NW>
+__global__ __launch_bounds__(NW * 64) void k_partition_large(Args a, int npad_lo, int npad_hi) {
+  if (a.ctl->done) return;
+  const int n = a.ctl->n_work[1];
+  for (int j = blockIdx.x; j < n; j += gridDim.x) {
+    partition_body<MODE, LDS_SORT, NW>(a, j, npad_lo, npad_hi);
+    __syncthreads();  // the LDS tile is staged anew for the next entry
+  }
+}
+
 // The large class of the primal step, batched: min(K, kLargeGroups)
 // workgroups per slot (map {slot, g}), workgroup g taking the entries g,
 // g + G, g + 2G, ... of the slot's class-1 list, whose length the scan and
