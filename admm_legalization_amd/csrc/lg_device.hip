@@ -866,8 +866,9 @@ int launch_iteration(LgHandle* h, hipStream_t s, bool catchup) {
   const int nseg_blocks = (a.nxs * a.nys + 255) / 256;
   // cells of the listed partitions into their buckets
   if (a.N > 0) hipLaunchKernelGGL(k_bucket, dim3((a.N + kBucketCells - 1) / kBucketCells), dim3(256), 0, s, a);
-  if (a.K > 8 * 1024) hipLaunchKernelGGL(k_scan<true>, dim3(1), dim3(1024), 0, s, a);
-  else hipLaunchKernelGGL(k_scan<false>, dim3(1), dim3(1024), 0, s, a);
+  if (a.K > 8 * 1024) hipLaunchKernelGGL((k_scan<true, 1024>), dim3(1), dim3(1024), 0, s, a);
+  else if (a.K > 8 * 256) hipLaunchKernelGGL((k_scan<false, 1024>), dim3(1), dim3(1024), 0, s, a);
+  else hipLaunchKernelGGL((k_scan<false, 256>), dim3(1), dim3(256), 0, s, a);
   // the dual pass needs the step k_scan sets; the pixels it updates belong
   // to partitions the primal step leaves alone, so it runs on the side
   // stream beside the partition kernels, joined before k_finish

@@ -443,13 +443,17 @@ __device__ __forceinline__ void scan_finish(const Args& a, int K, int m, int tot
   }
 }
 
-// scan_body for K <= 8192: eight partitions per thread (i = j * 1024 + t),
-// counts kept in registers and scanned there: the exclusive prefix of i is
-// the sum of the (j', wave') totals before (j, wave) plus the wave's
-// exclusive prefix -- three barriers, no second pass over HBM.
+// scan_body for K <= 8 * NT (NT = 1024 or 256 threads): eight partitions per
+// thread (i = j * NT + t), counts kept in registers and scanned there: the
+// exclusive prefix of i is the sum of the (j', wave') totals before (j, wave)
+// plus the wave's exclusive prefix -- three barriers, no second pass over HBM.
+template <int NT>
 __device__ __forceinline__ void scan_small(const Args& a, int K, bool skip_ok, bool adaptive_now,
                                            const int32_t* cf, int* s_n, int* s_max) {
-  __shared__ int tot[128], pre[128], s_total;
+  constexpr int W = NT / 64;   // waves
+  constexpr int E = 8 * W;     // (row, wave) totals
+  static_assert(E <= 128, "at most 16 waves");
+  __shared__ int tot[E], pre[E], s_total;
   Ctl* ctl = a.ctl;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const unsigned long long lt = (1ull << lane) - 1ull;
@@ -457,14 +461,14 @@ __device__ __forceinline__ void scan_small(const Args& a, int K, bool skip_ok, b
   int cn[8], fl[8];
 #pragma unroll
   for (int j = 0; j < 8; j++) {
-    const int i = j * 1024 + t;
+    const int i = j * NT + t;
     cn[j] = i < K ? a.bcnt[i] : 0;
     fl[j] = (i < K && skip_ok) ? cf[i] : 1;
   }
   int mx = 0;
 #pragma unroll
   for (int j = 0; j < 8; j++) {
-    const int i = j * 1024 + t;
+    const int i = j * NT + t;
     const bool in = i < K;
     const int c = cn[j];
     if (in) {
@@ -489,7 +493,7 @@ __device__ __forceinline__ void scan_small(const Args& a, int K, bool skip_ok, b
   }
   // activity flags of the next iteration, set by the primal step
   int32_t* nf = a.act_flags + (size_t)((iter + 1) & 1) * K;
-  for (int i = t; i < K; i += 1024) nf[i] = 0;
+  for (int i = t; i < K; i += NT) nf[i] = 0;
   // inclusive wave scans of the eight rows
   int incl[8];
 #pragma unroll
@@ -504,33 +508,33 @@ __device__ __forceinline__ void scan_small(const Args& a, int K, bool skip_ok, b
   }
   if (lane == kWave - 1)
 #pragma unroll
-    for (int j = 0; j < 8; j++) tot[j * 16 + wave] = incl[j];
+    for (int j = 0; j < 8; j++) tot[j * W + wave] = incl[j];
   for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o));
   if (lane == 0) s_max[wave] = mx;
   __syncthreads();
   if (t < 2) ctl->n_work[t] = s_n[t];
   if (t < kWave) {
-    // exclusive scan of the 128 (row, wave) totals, two per lane
-    const int v0 = tot[2 * t], v1 = tot[2 * t + 1];
+    // exclusive scan of the E (row, wave) totals, two per lane
+    const int v0 = 2 * t < E ? tot[2 * t] : 0, v1 = 2 * t + 1 < E ? tot[2 * t + 1] : 0;
     int x = v0 + v1;
     for (int o = 1; o < kWave; o <<= 1) {
       const int z = __shfl_up(x, o);
       if (lane >= o) x += z;
     }
     const int ex = x - v0 - v1;
-    pre[2 * t] = ex;
-    pre[2 * t + 1] = ex + v0;
+    if (2 * t < E) pre[2 * t] = ex;
+    if (2 * t + 1 < E) pre[2 * t + 1] = ex + v0;
     if (t == kWave - 1) s_total = x;
   }
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < 8; j++) {
-    const int i = j * 1024 + t;
-    if (i < K) a.off[i] = pre[j * 16 + wave] + incl[j] - cn[j];
+    const int i = j * NT + t;
+    if (i < K) a.off[i] = pre[j * W + wave] + incl[j] - cn[j];
   }
   if (t == 0) {
     int m = 0;
-    for (int q = 0; q < 16; q++) m = max(m, s_max[q]);
+    for (int q = 0; q < W; q++) m = max(m, s_max[q]);
     scan_finish(a, K, m, s_total);
   }
 }
@@ -546,8 +550,9 @@ __device__ __forceinline__ void scan_small(const Args& a, int K, bool skip_ok, b
 // 135 KB of LDS; without it a scan workgroup takes little LDS and shares a
 // CU with the partition kernels' workgroups (callers pick it when every K is
 // at most 8192).
-template <bool LARGE>
+template <bool LARGE, int NT>
 __device__ __forceinline__ void scan_body(const Args& a) {
+  static_assert(!LARGE || NT == 1024, "the large path runs on 1024 threads");
   Ctl* ctl = a.ctl;
   if (ctl->done) return;
   __shared__ int s_max[16];
@@ -563,11 +568,11 @@ __device__ __forceinline__ void scan_body(const Args& a) {
   if (t < 2) s_n[t] = 0;
   __syncthreads();
   if constexpr (!LARGE) {
-    scan_small(a, K, skip_ok, adaptive_now, cf, s_n, s_max);
+    scan_small<NT>(a, K, skip_ok, adaptive_now, cf, s_n, s_max);
     return;
   }
   if (K <= 8 * 1024) {
-    scan_small(a, K, skip_ok, adaptive_now, cf, s_n, s_max);
+    scan_small<1024>(a, K, skip_ok, adaptive_now, cf, s_n, s_max);
     return;
   }
   // cells per partition from the bucket counts (off <- counts, scanned
@@ -626,8 +631,8 @@ __device__ __forceinline__ void scan_body(const Args& a) {
     scan_finish(a, K, m, K > 0 ? a.off[K - 1] + last : 0);
   }
 }
-template <bool LARGE>
-__global__ __launch_bounds__(1024) void k_scan(Args a) { scan_body<LARGE>(a); }
+template <bool LARGE, int NT>
+__global__ __launch_bounds__(NT) void k_scan(Args a) { scan_body<LARGE, NT>(a); }
 
 // Grouping for the final refine, which visits every partition under the
 // fold offset of the converged iteration.  A partition ordered by the last
@@ -3043,10 +3048,10 @@ __global__ __launch_bounds__(256) void kb_bucket(const Args* __restrict__ tab, c
   bucket_body(a, m_.y);
 }
 
-template <bool LARGE>
-__global__ __launch_bounds__(1024) void kb_scan(const Args* __restrict__ tab, const int2* __restrict__ map) {
+template <bool LARGE, int NT>
+__global__ __launch_bounds__(NT) void kb_scan(const Args* __restrict__ tab, const int2* __restrict__ map) {
   LG_BATCH_SLOT();
-  scan_body<LARGE>(a);
+  scan_body<LARGE, NT>(a);
 }
 
 __global__ void kb_extract_congested(const Args* __restrict__ tab, const int2* __restrict__ map) {

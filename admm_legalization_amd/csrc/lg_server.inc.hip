@@ -86,6 +86,7 @@ struct SrvLane {
   MapRange m256, m1, mk, mpix, mbk, mlg;
   int any_adaptive = 0, any_reg = 0, any_nonreg = 0, top_slot = 0;
   int large_k = 0;  // a request with more than 8192 partitions (the scan's large path)
+  int max_k = 0;    // most partitions of a resident request
   size_t lds_reg = 0, lds_small = 0, lds_big = 0;
   int64_t iterations = 0;  // batched iterations launched in this lane
   // primal-step timing (lg_srv_profile): two events per batched iteration
@@ -156,7 +157,7 @@ int srv_grow_map(SrvLane& L, size_t need) {
 // Slot table and block maps of the lane's requests (lane between batches).
 int srv_build_tables(LgServer* S, SrvLane& L) {
   std::vector<int2> m[6];
-  L.any_adaptive = L.any_reg = L.any_nonreg = L.large_k = 0;
+  L.any_adaptive = L.any_reg = L.any_nonreg = L.large_k = L.max_k = 0;
   L.lds_reg = L.lds_small = L.lds_big = 0;
   L.top_slot = L.slot_lo;
   for (int sl : L.active) {
@@ -173,6 +174,7 @@ int srv_build_tables(LgServer* S, SrvLane& L) {
     for (int b = 0; b < std::min(a.K, kLargeGroups); b++) m[5].push_back(make_int2(sl, b));
     L.any_adaptive |= a.use_adaptive ? 1 : 0;
     L.large_k |= a.K > 8 * 1024 ? 1 : 0;
+    L.max_k = std::max(L.max_k, a.K);
     const int reg = h->p.sorting_algorithm == LG_SORT_M_BITONIC || h->p.sorting_algorithm == LG_SORT_S_BITONIC;
     if (reg) {
       L.any_reg = 1;
@@ -213,8 +215,10 @@ int srv_launch_iteration(LgServer* S, SrvLane& L, int it, hipStream_t s, bool pa
   const int2* mp = L.d_map;
   auto at = [&](const MapRange& r) { return mp + r.off; };
   if (L.mbk.n > 0) hipLaunchKernelGGL(kb_bucket, dim3(L.mbk.n), dim3(256), 0, s, tab, at(L.mbk));
-  if (L.large_k) hipLaunchKernelGGL(kb_scan<true>, dim3(L.m1.n), dim3(1024), 0, s, tab, at(L.m1));
-  else hipLaunchKernelGGL(kb_scan<false>, dim3(L.m1.n), dim3(1024), 0, s, tab, at(L.m1));
+  // the scan's workgroup: 256 threads for up to 2048 partitions, else 1024
+  if (L.large_k) hipLaunchKernelGGL((kb_scan<true, 1024>), dim3(L.m1.n), dim3(1024), 0, s, tab, at(L.m1));
+  else if (L.max_k > 8 * 256) hipLaunchKernelGGL((kb_scan<false, 1024>), dim3(L.m1.n), dim3(1024), 0, s, tab, at(L.m1));
+  else hipLaunchKernelGGL((kb_scan<false, 256>), dim3(L.m1.n), dim3(256), 0, s, tab, at(L.m1));
   // the dual pass on the side stream, beside the partition kernels
   if (pass) {
     HIP_TRY(hipEventRecord(L.ev_fork, s));
